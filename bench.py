@@ -1,0 +1,143 @@
+"""Benchmark of the per-slice march on one MI355X: the laser-wakefield deck (256 x 256 Fourier modes, 1024 uniform
+zeta slices, h = 0.25/k0, uniform plasma 1e-3 n_c streamed as 2 x 2 electrons per cell per slice = 262144 per
+slice, two passes per half-cell, Gaussian grid laser a0 = 2, w0 = 30/k0, c tau = 15/k0 at zeta = 200/k0, sponge
+24/k0, dz = 20/k0).  A step is one s-step of the march (currents, the 1023 half-cells x 2 passes of the streaming
+plasma, sponge and real fields).  Metric: streaming-particle pushes per second including deposit and gather.
+
+Prints one JSON line.  --gpus above 1 is refused: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gem-pic-3d_amd"))
+
+HBM_PEAK_GBS = 8000.0
+LAMBDA0 = 2 * np.pi * 1e-6
+
+
+def lwfa_deck(nx=256, ny=256, nz=1024, h=0.25):
+    import g3d_amd
+    d = g3d_amd.Deck(LAMBDA0)
+    d.c.nx = nx; d.c.Lx = 240.0
+    d.y_uniform(ny, 240.0)
+    d.zeta_uniform(nz, (nz - 1) * h)
+    d.set(dz=20.0, z_end=2000.0, sponge=24.0, sponge_y=24.0, move_x=0, move_y=0, plasma_on=1, n0=1e-3,
+          ppc_x=2, ppc_y=2, smooth_length=0.0, iterations=2)
+    d.add_grid_laser(a0=2.0, waist=30.0, tau=15.0, zeta_c=min(200.0, 0.78 * (nz - 1) * h), z_focus=0.0)
+    return d
+
+
+def stream_bytes_per_launch(nstrm, nk):
+    """Algorithmic HBM bytes of one streaming half-cell pass (k_stream_halfcell).
+    Per particle: read x y z zeta px py pz w q qm (80 B) + alive sp (8 B), write x y z zeta px py pz q qm (72 B)
+    + alive (4 B) = 164 B.  Per point: gather of F slices n, n+1 (6 floats each, 48 B) and the deposit into the
+    5 real planes of jsr (40 B) = 88 B."""
+    return nstrm * 164.0 + nk * 88.0
+
+
+def cpu_baseline(threads):
+    """The reference's own CPU build (oracle/_ref) on a bounded sample of the same workload: the full 256 x 256
+    transverse grid and 262144 streaming electrons per slice, 32 slices instead of 1024 (the per-half-cell work does
+    not depend on the window length), 2 steps after init."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import refpy
+    if not refpy.available():
+        return None
+    refpy.set_threads(threads)
+    nz = 32
+    d = lwfa_deck(nz=nz)
+    ref = refpy.RefSim(d)
+    ref.init()
+    t = time.perf_counter()
+    steps = 2
+    for _ in range(steps):
+        ref.step()
+    dt = time.perf_counter() - t
+    nstrm = ref.info()["nstrm"]
+    pushes = nstrm * (nz - 1) * 2 * steps
+    return {"value": pushes / dt, "unit": "pushes/s", "cores": threads, "kind": "reference",
+            "sample": "reference CPU build (g++ -O3 -fopenmp, %d threads), 256x256 modes, %d streaming electrons per "
+                      "slice, %d slices, 2 passes, %d steps: %.2f s" % (threads, nstrm, nz, steps, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nz", type=int, default=1024)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU (no collectives); --gpus %d is not supported" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    assert torch.cuda.is_available(), "no GPU"
+    import g3d_amd
+
+    d = lwfa_deck(nz=args.nz)
+    sim = g3d_amd.Sim(d)
+    t0 = time.perf_counter()
+    sim.init()
+    t_init = time.perf_counter() - t0
+    print("init %.2f s" % t_init, file=sys.stderr, flush=True)
+    for k in range(args.warmup):
+        sim.step()
+        print("warmup step %d done" % k, file=sys.stderr, flush=True)
+    info = sim.info()
+    N, nk = info["N"], info["nk"]
+    torch.cuda.synchronize()
+    sim.timing(True)
+    t0 = time.perf_counter()
+    pushes = 0
+    for k in range(args.steps):
+        sim.step()
+        pushes += sim.info()["nstrm"] * (N - 1) * 2
+        print("step %d done" % k, file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    timers = sim.timers()
+    sim.timing(False)
+    ms_step = 1000 * dt / args.steps
+    value = pushes / dt
+    ms, cnt = timers["stream_halfcell"]
+    avg_s = ms / 1000 / max(cnt, 1)
+    nstrm = sim.info()["nstrm"]
+    bpl = stream_bytes_per_launch(nstrm, nk)
+    achieved = bpl / avg_s / 1e9
+    fam = {k: {"ms_total": round(v[0], 3), "launches": v[1]} for k, v in timers.items() if v[1]}
+    cpu = None
+    if not args.no_cpu:
+        thr = args.cpu_threads or min(16, os.cpu_count() or 1)
+        try:
+            cpu = cpu_baseline(thr)
+        except Exception as e:  # the baseline is reported, never the target
+            print("cpu baseline failed: %s" % e, file=sys.stderr)
+    out = {
+        "metric": "particle pushes/s incl. deposit+gather; FP32-mixed vs FP64 at <=1e-4 field err",
+        "value": value, "unit": "pushes/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "LWFA plasma march 256x256x%d, 262144 streaming e- per slice, 2 passes, FP64 form"
+                               % args.nz, "N": N, "nstrm": nstrm, "pushes_per_step": pushes // args.steps},
+        "roofline": {"kernel": "k_stream_halfcell", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": bpl, "avg_launch_us": avg_s * 1e6},
+        "cpu_baseline": cpu,
+        "kernel_families": fam,
+        "init_s": t_init,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

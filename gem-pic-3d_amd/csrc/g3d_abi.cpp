@@ -1,0 +1,178 @@
+// C ABI of the engine (include/g3d.h): create a Simulation of the march, run init/step or single member
+// functions, and move its arrays between host and device in the reference's layouts.
+#include <cstring>
+#include <map>
+#include "g3d_engine.hpp"
+
+struct g3d_sim { g3d::Sim* s; };
+
+namespace {
+using g3d::DevBuf;
+using g3d::cx;
+
+template <class T> size_t bytes_of(const DevBuf<T>& b) { return b.size() * sizeof(T); }
+
+// name -> (pointer to the device buffer as bytes, size in bytes); particles are handled separately
+bool lookup(g3d::Sim& s, const std::string& n, void*& dev, size_t& bytes) {
+    std::map<std::string, std::pair<void*, size_t>> m = {
+        {"Rx", {s.Rx.data(), bytes_of(s.Rx)}}, {"Ry", {s.Ry.data(), bytes_of(s.Ry)}},
+        {"Ez", {s.Ez.data(), bytes_of(s.Ez)}}, {"Bz", {s.Bz.data(), bytes_of(s.Bz)}},
+        {"Lx", {s.Lx.data(), bytes_of(s.Lx)}}, {"Ly", {s.Ly.data(), bytes_of(s.Ly)}},
+        {"j0x", {s.j0x.data(), bytes_of(s.j0x)}}, {"j0y", {s.j0y.data(), bytes_of(s.j0y)}},
+        {"jkx", {s.jkx.data(), bytes_of(s.jkx)}}, {"jky", {s.jky.data(), bytes_of(s.jky)}},
+        {"jkz", {s.jkz.data(), bytes_of(s.jkz)}}, {"js0x", {s.js0x.data(), bytes_of(s.js0x)}},
+        {"js0y", {s.js0y.data(), bytes_of(s.js0y)}}, {"js0z", {s.js0z.data(), bytes_of(s.js0z)}},
+        {"jw", {s.jw.data(), bytes_of(s.jw)}}, {"F", {s.F.data(), bytes_of(s.F)}},
+        {"jsr", {s.jsr.data(), bytes_of(s.jsr)}}, {"jsn", {s.jsn.data(), bytes_of(s.jsn)}},
+        {"rho_lines", {s.rho_lines.data(), bytes_of(s.rho_lines)}}};
+    auto it = m.find(n);
+    if (it == m.end()) return false;
+    dev = it->second.first; bytes = it->second.second;
+    return true;
+}
+int check_size(size_t want, size_t got) {
+    if (want != got) { g3d::set_error("size mismatch: " + std::to_string(got) + " bytes, array has " + std::to_string(want)); return 2; }
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+int g3d_sim_create(const g3d_config* cfg, g3d_sim** out) {
+    return g3d::guard([&] {
+        *out = nullptr;
+        auto* h = new g3d_sim{nullptr};
+        try { h->s = new g3d::Sim(g3d::Config::from_abi(*cfg)); }
+        catch (...) { delete h; throw; }
+        *out = h;
+    });
+}
+int g3d_sim_destroy(g3d_sim* sim) {
+    return g3d::guard([&] { if (sim) { delete sim->s; delete sim; } });
+}
+int g3d_sim_info_get(g3d_sim* sim, g3d_sim_info* o) {
+    return g3d::guard([&] {
+        g3d::Sim& s = *sim->s;
+        o->N = s.N; o->ns = s.ns; o->nk = s.nk; o->np = s.np; o->nstrm = s.nstrm; o->istep = s.istep;
+        o->born_cap = s.born_cap; o->s = s.s; o->dx = s.dx; o->dy = s.dy; o->X0 = s.X0; o->Y0 = s.Y0;
+        o->beta_b = s.beta_b; o->gm2_b = s.gm2_b; o->trapped_total = s.trapped_total;
+        o->restreamed_total = s.restreamed_total; o->lost_total = s.lost_total;
+    });
+}
+int g3d_sim_init(g3d_sim* sim) { return g3d::guard([&] { sim->s->init(); g3d::hip_check(hipStreamSynchronize(g3d::stream()), "synchronize"); }); }
+int g3d_sim_step(g3d_sim* sim) { return g3d::guard([&] { sim->s->step(); }); }
+
+int g3d_sim_call(g3d_sim* sim, const char* op_, const double* a, int na) {
+    int rc = 0;
+    const int g = g3d::guard([&] {
+        g3d::Sim& s = *sim->s;
+        const std::string op(op_);
+        auto need = [&](int k) {
+            if (na < k) throw std::runtime_error(op + ": needs " + std::to_string(k) + " arguments");
+        };
+        if (op == "init") s.init();
+        else if (op == "step") s.step();
+        else if (op == "march") { need(1); s.march(a[0]); }
+        else if (op == "currents") s.currents();
+        else if (op == "sponge_and_real_fields") s.sponge_and_real_fields();
+        else if (op == "solve_fields") { need(1); s.solve_fields(a[0]); }
+        else if (op == "stream_spectra") s.stream_spectra();
+        else if (op == "march_slice") { need(2); s.march_slice(int(a[0]), a[1]); }
+        else if (op == "real_fields_slice") { need(1); s.real_fields_slice(int(a[0])); }
+        else if (op == "push") { need(2); s.push(a[0], a[1]); }
+        else if (op == "march_plasma") { need(2); s.march_plasma(a[0], a[1]); }
+        else if (op == "comoving_field") s.comoving_field();
+        else if (op == "relax_start_field") s.relax_start_field();
+        else if (op == "add_grid_lasers") s.add_grid_lasers();
+        else if (op == "load_plasma") { need(2); s.load_plasma(a[0], a[1]); }
+        else if (op == "set_s") { need(2); s.s = a[0]; s.istep = long(a[1]); }
+        else if (op == "set_starting") { need(1); s.starting_ = a[0] != 0; }
+        else if (op == "j0_from_jk") s.j0_from_jk();
+        else { g3d::set_error("unknown op " + op); rc = 3; return; }
+        g3d::hip_check(hipStreamSynchronize(g3d::stream()), "synchronize");
+    });
+    return g ? g : rc;
+}
+
+int g3d_sim_array_size(g3d_sim* sim, const char* name, size_t* bytes) {
+    int rc = 0;
+    const int g = g3d::guard([&] {
+        g3d::Sim& s = *sim->s;
+        const std::string n(name);
+        void* d;
+        if (n == "parts") *bytes = size_t(s.np) * sizeof(g3d_particle);
+        else if (n == "strm") *bytes = size_t(s.nstrm) * sizeof(g3d_particle);
+        else if (!lookup(s, n, d, *bytes)) { g3d::set_error("unknown array " + n); rc = 3; }
+    });
+    return g ? g : rc;
+}
+int g3d_sim_get(g3d_sim* sim, const char* name, void* host, size_t bytes) {
+    int rc = 0;
+    const int g = g3d::guard([&] {
+        g3d::Sim& s = *sim->s;
+        const std::string n(name);
+        if (n == "parts" || n == "strm") {
+            const std::vector<g3d_particle> v = n == "parts" ? s.host_particles() : s.host_strm();
+            if ((rc = check_size(v.size() * sizeof(g3d_particle), bytes))) return;
+            std::memcpy(host, v.data(), bytes);
+            return;
+        }
+        void* d; size_t b;
+        if (!lookup(s, n, d, b)) { g3d::set_error("unknown array " + n); rc = 3; return; }
+        if ((rc = check_size(b, bytes))) return;
+        if (!bytes) return;
+        g3d::hip_check(hipMemcpyAsync(host, d, bytes, hipMemcpyDeviceToHost, g3d::stream()), "hipMemcpy D2H");
+        g3d::hip_check(hipStreamSynchronize(g3d::stream()), "synchronize");
+    });
+    return g ? g : rc;
+}
+int g3d_sim_set(g3d_sim* sim, const char* name, const void* host, size_t bytes) {
+    int rc = 0;
+    const int g = g3d::guard([&] {
+        g3d::Sim& s = *sim->s;
+        void* d; size_t b;
+        if (!lookup(s, name, d, b)) { g3d::set_error(std::string("unknown array ") + name); rc = 3; return; }
+        if ((rc = check_size(b, bytes))) return;
+        if (!bytes) return;
+        g3d::hip_check(hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, g3d::stream()), "hipMemcpy H2D");
+        g3d::hip_check(hipStreamSynchronize(g3d::stream()), "synchronize");
+    });
+    return g ? g : rc;
+}
+int g3d_sim_set_particles(g3d_sim* sim, const g3d_particle* p, long n) {
+    return g3d::guard([&] { sim->s->set_particles(std::vector<g3d_particle>(p, p + n)); });
+}
+int g3d_sim_set_joining(g3d_sim* sim, const g3d_particle* p, long n) {
+    return g3d::guard([&] { sim->s->joining_.assign(p, p + n); });
+}
+int g3d_sim_get_joining(g3d_sim* sim, g3d_particle* p, long cap, long* n) {
+    int rc = 0;
+    const int g = g3d::guard([&] {
+        const auto& j = sim->s->joining_;
+        *n = long(j.size());
+        if (*n > cap) { g3d::set_error("joining list larger than the buffer"); rc = 2; return; }
+        std::memcpy(p, j.data(), j.size() * sizeof(g3d_particle));
+    });
+    return g ? g : rc;
+}
+int g3d_sim_timing(g3d_sim* sim, int enable) {
+    return g3d::guard([&] {
+        g3d::Sim& s = *sim->s;
+        s.flush_timers();
+        s.timing = enable != 0;
+        for (int k = 0; k < g3d::Sim::kFam; ++k) { s.fam_ms[k] = 0; s.fam_n[k] = 0; }
+    });
+}
+int g3d_sim_timers(g3d_sim* sim, double* ms, long* counts, int nfam, char* names, int name_len) {
+    return g3d::guard([&] {
+        g3d::Sim& s = *sim->s;
+        s.flush_timers();
+        for (int k = 0; k < nfam && k < g3d::Sim::kFam; ++k) {
+            ms[k] = s.fam_ms[k]; counts[k] = s.fam_n[k];
+            if (names && name_len > 0) {
+                std::strncpy(names + size_t(k) * name_len, g3d::kFamilyNames[k], size_t(name_len) - 1);
+                names[size_t(k) * name_len + name_len - 1] = 0;
+            }
+        }
+    });
+}
+}

@@ -1,0 +1,825 @@
+// Engine of the per-slice march: setup, init, step and the member functions of the reference's drop-in files
+// (src/sim_setup.cpp, sim_particles.cpp, sim_fields.cpp, sim_plasma.cpp, sim_ions.cpp) over the gfx950 kernels.
+#include "g3d_engine.hpp"
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <random>
+
+namespace g3d {
+
+const char* const kFamilyNames[Sim::kFam] = {"stream_halfcell", "spectra_fft", "plasma_modes", "fields_fft",
+                                             "fields_F", "beam_push", "currents", "sponge_real_fields"};
+enum { F_STREAM = 0, F_SFFT, F_MODES, F_IFFT, F_TOF, F_PUSH, F_CUR, F_SPONGE };
+
+// ---------------------------------------------------------------- configuration
+Config Config::from_abi(const g3d_config& g) {
+    Config c;
+    c.lambda0 = g.lambda0;
+    c.Lu = g.lambda0 / (2 * M_PI); c.Tu = c.Lu / kC;
+    const double w0 = kC / c.Lu;
+    c.nc = kEps0 * kMe * w0 * w0 / (kE * kE);
+    c.nx = g.nx; c.ny = g.ny; c.nzeta = g.nzeta; c.Lx = g.Lx; c.Ly = g.Ly; c.y_fourier = g.y_fourier != 0;
+    if (!g.zeta || !g.ynode || c.nzeta < 4 || c.nx < 2 || c.ny < 2) throw std::runtime_error("grids not set");
+    c.zeta.assign(g.zeta, g.zeta + g.nzeta);
+    c.ynode.assign(g.ynode, g.ynode + g.ny + 1);
+    c.Lwin = c.zeta.back();
+    c.dz = g.dz; c.z_end = g.z_end; c.dz_push = g.dz_push;
+    c.sponge = g.sponge; c.sponge_y = g.sponge_y; c.sponge_rate = g.sponge_rate;
+    c.move_x = g.move_x; c.move_y = g.move_y; c.window_lag = g.window_lag;
+    c.self_fields = g.self_fields; c.zeta_filter = g.zeta_filter; c.k_filter = g.k_filter;
+    c.push_phase = g.push_phase;
+    c.laser_on = g.laser_on; c.pol = g.laser_pol; c.a0 = g.a0; c.kL = g.kL; c.theta = g.theta; c.tau = g.tau;
+    c.cep = g.cep; c.start_fwhm = g.start_fwhm; c.stop_fwhm = g.stop_fwhm;
+    c.charge = g.charge; c.gamma = g.gamma; c.sigx = g.sigx; c.sigy = g.sigy; c.sigz = g.sigz; c.emitx = g.emitx;
+    c.emity = g.emity; c.espread = g.espread; c.zeta_center = g.zeta_center; c.nmacro = g.nmacro; c.seed = g.seed;
+    c.glasers.assign(g.glasers, g.glasers + g.n_glasers);
+    c.plasma_on = g.plasma_on; c.n0 = g.n0; c.r_max = g.r_max; c.ppc_x = g.ppc_x; c.ppc_y = g.ppc_y;
+    c.smooth_length = g.smooth_length;
+    c.pz_points.assign(g.pz_points, g.pz_points + g.n_pz);
+    c.pz_values.assign(g.pz_values, g.pz_values + g.n_pz);
+    c.parabolic = g.parabolic; c.channel_depth = g.channel_depth; c.r_channel = g.r_channel;
+    c.iterations = g.iterations; c.vz_min = g.vz_min; c.vz_beam = g.vz_beam; c.trap_advance = g.trap_advance;
+    c.predictor_extrapolate = g.predictor_extrapolate; c.free_electrons = g.free_electrons;
+    c.ions.assign(g.ions, g.ions + g.n_ions);
+    c.freeze_bunch = g.freeze_bunch; c.start_relax = g.start_relax; c.box_mean = g.box_mean;
+    if (c.iterations < 1) throw std::runtime_error("numerics.iterations must be >= 1");
+    return c;
+}
+
+// n = n0 f(z) g(r) (src/sim_config.cpp:49-67; the deck's density_function is not part of the march)
+double Config::plasma_density(double x, double y, double z) const {
+    if (!plasma_on) return 0;
+    const double r2 = x * x + y * y;
+    if (r2 >= r_max * r_max) return 0;
+    double f = 1;
+    if (!pz_points.empty()) {
+        if (z <= pz_points.front()) f = pz_values.front();
+        else if (z >= pz_points.back()) f = pz_values.back();
+        else {
+            size_t k = 1;
+            while (pz_points[k] < z) ++k;
+            const double t = (z - pz_points[k - 1]) / (pz_points[k] - pz_points[k - 1]);
+            f = (1 - t) * pz_values[k - 1] + t * pz_values[k];
+        }
+    }
+    const double g = parabolic ? 1 + channel_depth * r2 / (r_channel * r_channel) : 1.0;
+    return f * g > 0 ? n0 * f * g : 0.0;
+}
+
+// ---------------------------------------------------------------- particle storage
+void PartStore::resize(size_t n, bool own) {
+    cap = n;
+    for (DevBuf<double>* b : {&x, &y, &zeta, &px, &py, &pz, &z, &q, &qm}) b->resize(n);
+    alive.resize(n);
+    if (own) { w.resize(n); rem.resize(n); sp.resize(n); id.resize(n); }
+}
+Parts PartStore::view(const PartStore* sh) {
+    const PartStore& c = sh ? *sh : *this;
+    Parts p;
+    p.x = x.data(); p.y = y.data(); p.zeta = zeta.data(); p.px = px.data(); p.py = py.data(); p.pz = pz.data();
+    p.z = z.data(); p.q = q.data(); p.qm = qm.data(); p.alive = alive.data();
+    p.w = const_cast<double*>(c.w.data()); p.rem = const_cast<double*>(c.rem.data());
+    p.sp = const_cast<int*>(c.sp.data()); p.id = const_cast<unsigned long long*>(c.id.data());
+    return p;
+}
+
+// ---------------------------------------------------------------- ADK tables (src/ionization.cpp:55-99)
+namespace {
+constexpr double kUH = 13.605693122994, kEa = 5.14220674763e11, kWa = 4.1341373335e16;
+void adk_coefficients(double U, int Zafter, double& logpre, double& power, double& expo) {
+    const double x = U / kUH;
+    const double nstar = Zafter / std::sqrt(x);
+    const double logC2 = 2.0 * nstar * std::log(2.0) - std::log(nstar) - std::lgamma(nstar + 1.0) - std::lgamma(nstar);
+    power = 2.0 * nstar - 1.0;
+    logpre = std::log(kWa) + logC2 + std::log(0.5 * x) + power * std::log(2.0 * std::pow(x, 1.5) * kEa);
+    expo = -(2.0 / 3.0) * std::pow(x, 1.5) * kEa;
+}
+double adk_raw(const AdkLevel& L, double E) {
+    E = std::min(E, L.Epeak);
+    return E > 0 ? std::exp(L.logpre - L.power * std::log(E) + L.expo / E) : 0.0;
+}
+std::vector<AdkLevel> adk_levels(const g3d_ion& s, double E0, double omega0, double T) {
+    std::vector<AdkLevel> lev(std::max(s.Zmax, 0), AdkLevel{0, 0, 0, 1e300, 1e300});
+    for (int k = s.Z0; k < s.Zmax; ++k) {
+        if (k >= s.n_ip) throw std::runtime_error("ion: ionization potentials missing");
+        AdkLevel& L = lev[k];
+        double logpre, power, expo;
+        adk_coefficients(s.ip_eV[k], k + 1, logpre, power, expo);
+        L.power = power;
+        L.logpre = logpre - std::log(omega0) - power * std::log(E0);
+        L.expo = expo / E0;
+        L.Epeak = power > 0 ? -L.expo / power : 1e300;
+        double lo = 1e-12, hi = std::min(L.Epeak, 1e6);
+        if (adk_raw(L, hi) * T < 1e-10) L.Emin = hi;
+        else {
+            for (int it = 0; it < 200; ++it) {
+                const double mid = std::sqrt(lo * hi);
+                (adk_raw(L, mid) * T < 1e-10 ? lo : hi) = mid;
+            }
+            L.Emin = lo;
+        }
+    }
+    return lev;
+}
+}  // namespace
+
+// ---------------------------------------------------------------- setup (src/sim_setup.cpp:5-164)
+Sim::Sim(const Config& c) : cfg(c) {
+    N = cfg.nzeta; nk = long(cfg.nx) * cfg.ny;
+    dx = cfg.Lx / cfg.nx;
+    if (int(cfg.zeta.size()) != N || int(cfg.ynode.size()) != cfg.ny + 1) throw std::runtime_error("grids not set");
+    if (!cfg.y_fourier) throw std::runtime_error("g3d engine: the free-hand y grid is not available in this build");
+    a = 0; ns = N; b = N;
+    const int nx = cfg.nx, ny = cfg.ny;
+    std::vector<double> hkx(nx), hsx(nx), hf(nx), hm(nx);
+    auto prof = [&](double dd, double w) {
+        if (w <= 0 || dd >= w) return 1.0;
+        const double r = 1 - dd / w;
+        return 1 - cfg.sponge_rate * r * r;
+    };
+    for (int ixx = 0; ixx < nx; ++ixx) {
+        const int mx = ixx <= nx / 2 ? ixx : ixx - nx;
+        hsx[ixx] = 2 * M_PI * mx / cfg.Lx;
+        hkx[ixx] = (2 * ixx == nx) ? 0 : hsx[ixx];
+        hf[ixx] = cfg.k_filter ? std::pow(std::cos(0.5 * hsx[ixx] * dx), 2) : 1.0;
+        if (2 * ixx == nx) hf[ixx] = 0;
+        hm[ixx] = prof(std::min(ixx * dx, cfg.Lx - ixx * dx), cfg.sponge);
+    }
+    auto up = [](DevBuf<double>& d, const std::vector<double>& h) { d.resize(h.size()); d.from_host(h.data(), 0, h.size()); };
+    up(kxd, hkx); up(kxs, hsx); up(filt, hf); up(maskx, hm);
+    yf = true;
+    {   // Fourier y (src/sim_setup.cpp:63-82)
+        dy = cfg.Ly / ny;
+        std::vector<double> hky(ny), hsy(ny), hfy(ny), hmy(ny), hi(ny, 1 / dy);
+        for (int j = 0; j < ny; ++j) {
+            const int my = j <= ny / 2 ? j : j - ny;
+            hsy[j] = 2 * M_PI * my / cfg.Ly;
+            hky[j] = (2 * j == ny) ? 0 : hsy[j];
+            hfy[j] = cfg.k_filter ? std::pow(std::cos(0.5 * hsy[j] * dy), 2) : 1.0;
+            if (2 * j == ny) hfy[j] = 0;
+            const double dd = std::min(j * dy, cfg.Ly - j * dy), w = cfg.sponge_y;
+            double m = 1;
+            if (w > 0 && dd < w) { const double r = 1 - dd / w; m = 1 - cfg.sponge_rate * r * r; }
+            hmy[j] = m;
+        }
+        up(kyd, hky); up(kys, hsy); up(filty, hfy); up(masky_n, hmy); up(masky_h, hmy); up(iD, hi); up(iA, hi);
+        yax.X = nullptr; yax.n = ny + 1; yax.uniform = 1; yax.x0 = 0; yax.ih = 1 / dy;
+    }
+    {   // zeta (src/sim_setup.cpp:148-164)
+        const std::vector<double>& Z = cfg.zeta;
+        std::vector<double> hh(N - 1);
+        for (int m = 0; m < N - 1; ++m) hh[m] = Z[m + 1] - Z[m];
+        bool uni = true;
+        for (int m = 0; m < N - 1; ++m) uni = uni && hh[m] == hh[0];
+        if (uni) for (double& v : hh) v = cfg.Lwin / (N - 1);
+        std::vector<double> g(N + 3), lo(N), hi(N), ws(N);
+        for (int m = -2; m <= N; ++m) g[m + 2] = hh[std::min(std::max(m, 0), N - 2)];
+        for (int n = 0; n < N; ++n) {
+            const double hm_ = g[n - 1 + 2], hp = g[n + 2];
+            lo[n] = hp / (hm_ + hp); hi[n] = hm_ / (hm_ + hp);
+            ws[n] = 0.5 * ((n > 0 ? hh[n - 1] : 0) + (n < N - 1 ? hh[n] : 0));
+        }
+        up(zs, Z); up(hcg, g); up(wlo, lo); up(whi, hi); up(wsl, ws);
+        wlo_h = lo; hcg_h = g;
+        zax.X = zs.data(); zax.n = N; zax.uniform = uni; zax.x0 = 0; zax.ih = 1 / hh[0];
+    }
+    const size_t pl = size_t(nk);
+    for (DevBuf<cx>* p : {&Rx, &Ry, &Ez, &Bz, &Lx, &Ly}) p->resize(ns * pl);
+    for (DevBuf<cx>* p : {&j0x, &j0y, &jkx, &jky, &jkz, &S1, &S2}) p->resize((ns + 1) * pl);
+    T3.resize(3 * size_t(ns) * pl);
+    jw.resize(3 * (ns + 3) * pl);
+    F.resize(6 * (ns + 2) * pl);
+    fft_plan_init(fft, cfg.nx, cfg.ny, true);
+
+    laser.on = cfg.laser_on; laser.a0 = cfg.a0; laser.kL = cfg.kL; laser.tau = cfg.tau; laser.cep = cfg.cep;
+    laser.kx = std::sin(cfg.theta); laser.kz = std::cos(cfg.theta); laser.pol = cfg.pol;
+    laser.eta_c = cfg.start_fwhm * cfg.tau;
+    if (cfg.plasma_on) {   // src/sim_plasma.cpp:21-38
+        for (DevBuf<cx>* p : {&js0x, &js0y, &js0z}) p->resize((ns + 1) * pl);
+        jsr.resize(5 * pl);
+        rho_lines.resize(size_t(N) * (cfg.nx + cfg.ny));
+        bool mob = false;
+        for (const auto& s_ : cfg.ions) mob = mob || cfg.loaded(s_);
+        if (mob) rho_lines_i.resize(size_t(N) * (cfg.nx + cfg.ny));
+        jsn.resize(3 * pl); jsin.resize(3 * pl); bk.resize(4 * pl); sp3.resize(2 * pl); tp3.resize(3 * pl);
+        jpred.resize(3 * pl); jsi0.resize(3 * pl); dlt.resize(3 * pl);
+    }
+    counters.resize(4);
+    cent_dev.resize(3);
+    setup_ions();
+    if (cfg.plasma_on) setup_tiles();
+}
+
+// Tiles of TX x TY cells (16 x 8 on large grids) with a margin of M nodes around the TSC reach of the tile; a
+// window of WX = TX + 3 + 2M nodes must fit the period, and a window may reach only the neighbouring tiles.
+void Sim::setup_tiles() {
+    auto pick = [](int n, int pref) { for (int t = pref; t >= 8; t >>= 1) if (n % t == 0 && n >= 2 * t) return t; return 0; };
+    const int TX = pick(cfg.nx, 16), TY = pick(cfg.ny, 8);
+    if (!TX || !TY) return;
+    int M = 3;
+    M = std::min({M, TX - 1, TY - 1, (cfg.nx - TX - 3) / 2, (cfg.ny - TY - 3) / 2});
+    if (M < 0) return;
+    tiles.TX = TX; tiles.TY = TY; tiles.M = M; tiles.WX = TX + 3 + 2 * M; tiles.WY = TY + 3 + 2 * M;
+    tiles.ntx = cfg.nx / TX; tiles.nty = cfg.ny / TY; tiles.ntile = long(tiles.ntx) * tiles.nty;
+    tile_off.resize(tiles.ntile + 1);
+    slab.resize(size_t(tiles.ntile) * 5 * tiles.WX * tiles.WY);
+    tiles.off = tile_off.data(); tiles.slab = slab.data(); tiles.n_tiled = 0;
+}
+
+Sim::~Sim() {
+    (void)hipStreamSynchronize(stream());
+    fft_plan_free(fft);
+    for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+    for (auto& p : ev_pending) { (void)hipEventDestroy(p.second.first); (void)hipEventDestroy(p.second.second); }
+}
+
+void Sim::setup_ions() {   // src/sim_ions.cpp:68-87
+    if (cfg.ions.empty()) return;
+    std::vector<IonDev> iv;
+    std::vector<AdkLevel> lev;
+    const double omega0 = kC / cfg.Lu, E0 = kMe * kC * omega0 / kE;
+    for (const auto& s_ : cfg.ions) {
+        IonDev d{};
+        d.mass = s_.mass; d.mobile = s_.mobile; d.ionizing = s_.ionization; d.Zmax = s_.Zmax; d.off = int(lev.size());
+        d.Emin = 1e300;
+        if (s_.ionization) {
+            const std::vector<AdkLevel> L = adk_levels(s_, E0, omega0, 2 * cfg.Lwin);
+            for (int k = s_.Z0; k < s_.Zmax; ++k) d.Emin = std::min(d.Emin, L[k].Emin);
+            lev.insert(lev.end(), L.begin(), L.end());
+        }
+        iv.push_back(d);
+    }
+    ion_dev.resize(iv.size()); ion_dev.from_host(iv.data(), 0, iv.size());
+    if (!lev.empty()) { adk_dev.resize(lev.size()); adk_dev.from_host(lev.data(), 0, lev.size()); }
+    ion_stats.assign(cfg.ions.size(), std::vector<double>());
+}
+
+SpecGeom Sim::geom() const {
+    SpecGeom g;
+    g.nx = cfg.nx; g.ny = cfg.ny; g.ns = ns; g.N = N; g.a = a; g.nk = nk;
+    g.kx = kxd.data(); g.ky = kyd.data(); g.kxs = kxs.data(); g.kys = kys.data(); g.filt = filt.data();
+    g.filty = filty.data();
+    return g;
+}
+MarchView Sim::mview(double D) {
+    MarchView v;
+    v.Rx = Rx.data(); v.Ry = Ry.data(); v.Ez = Ez.data(); v.Bz = Bz.data(); v.Lx = Lx.data(); v.Ly = Ly.data();
+    v.j0x = j0x.data(); v.j0y = j0y.data(); v.jkx = jkx.data(); v.jky = jky.data(); v.jkz = jkz.data();
+    v.kx = kxd.data(); v.ky = kyd.data(); v.hcg = hcg.data(); v.wlo = wlo.data(); v.whi = whi.data();
+    v.D = D; v.a = a; v.N = N; v.nx = cfg.nx; v.nk = nk;
+    return v;
+}
+FieldView Sim::fview() {
+    FieldView f;
+    f.F = F.data(); f.a = a; f.ns = ns; f.nx = cfg.nx; f.ny = cfg.ny; f.yf = yf; f.self = cfg.self_fields; f.nk = nk;
+    f.dx = dx; f.dy = dy; f.Lx = cfg.Lx; f.Ly = cfg.Ly; f.zax = zax; f.yax = yax;
+    return f;
+}
+
+// ---------------------------------------------------------------- timing
+void Sim::tic(int fam) {
+    if (!timing) return;
+    hipEvent_t e;
+    if (!ev_pool.empty()) { e = ev_pool.back(); ev_pool.pop_back(); }
+    else hip_check(hipEventCreate(&e), "hipEventCreate");
+    hip_check(hipEventRecord(e, stream()), "hipEventRecord");
+    ev_open = e; ev_open_fam = fam;
+}
+void Sim::toc(int fam) {
+    if (!timing || ev_open_fam != fam) return;
+    hipEvent_t e;
+    if (!ev_pool.empty()) { e = ev_pool.back(); ev_pool.pop_back(); }
+    else hip_check(hipEventCreate(&e), "hipEventCreate");
+    hip_check(hipEventRecord(e, stream()), "hipEventRecord");
+    ev_pending.push_back({fam, {ev_open, e}});
+    ev_open = nullptr; ev_open_fam = -1;
+    if (ev_pending.size() > 8192) flush_timers();
+}
+void Sim::flush_timers() {
+    if (ev_pending.empty()) return;
+    hip_check(hipStreamSynchronize(stream()), "synchronize");
+    for (auto& p : ev_pending) {
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, p.second.first, p.second.second), "hipEventElapsedTime");
+        fam_ms[p.first] += ms; fam_n[p.first] += 1;
+        ev_pool.push_back(p.second.first); ev_pool.push_back(p.second.second);
+    }
+    ev_pending.clear();
+}
+
+// ---------------------------------------------------------------- particles of the beam list
+std::vector<g3d_particle> Sim::host_particles() {
+    std::vector<g3d_particle> v(static_cast<size_t>(np));
+    if (np) {
+        if (aos.size() < size_t(np)) aos.resize(size_t(np));
+        parts_to_aos(beam.view(), aos.data(), np);
+        aos.to_host(v.data(), 0, size_t(np));
+    }
+    return v;
+}
+void Sim::set_particles(const std::vector<g3d_particle>& v) {
+    np = long(v.size());
+    if (beam.cap < v.size() || beam.cap > 2 * v.size() + 1024) beam.resize(v.size() + v.size() / 4 + 64);
+    if (!np) return;
+    if (aos.size() < v.size()) aos.resize(v.size());
+    aos.from_host(v.data(), 0, v.size());
+    parts_from_aos(aos.data(), beam.view(), np);
+}
+// the streaming list in the reference's order: the fresh plasma by id (load order), then the rest as listed
+std::vector<g3d_particle> Sim::host_strm() {
+    std::vector<g3d_particle> v(static_cast<size_t>(nstrm));
+    if (nstrm) {
+        if (aos.size() < size_t(nstrm)) aos.resize(size_t(nstrm));
+        parts_to_aos(strmA.view(), aos.data(), nstrm);
+        aos.to_host(v.data(), 0, size_t(nstrm));
+        std::stable_sort(v.begin(), v.begin() + std::min(n_tiled, nstrm),
+                         [](const g3d_particle& a, const g3d_particle& b) { return a.id < b.id; });
+    }
+    return v;
+}
+
+// bunch loading with the reference's generator and draw order (src/sim_bunches.cpp:92-217, built-in Gaussian
+// profiles); libstdc++'s mt19937_64 and normal_distribution give the reference's particles bit for bit
+void Sim::load_bunch() {
+    const double ylo = -1e300, yhi = 1e300;
+    const double yoff = 0.5 * cfg.Ly;
+    std::vector<g3d_particle> mine;
+    if (cfg.nmacro > 0) {
+        std::mt19937_64 rng(cfg.seed);
+        std::normal_distribution<double> G(0, 1);
+        const double Ne = cfg.charge / kE;
+        const double W = (Ne / cfg.nmacro) / (cfg.nc * std::pow(cfg.Lu, 3));
+        const double p0 = std::sqrt(cfg.gamma * cfg.gamma - 1);
+        Axis zh = zax; zh.X = cfg.zeta.data();
+        for (long ib = 0; ib < cfg.nmacro; ++ib) {
+            double x, y, z, px, py, pz;
+            x = 0 + cfg.sigx * G(rng); y = 0 + cfg.sigy * G(rng);
+            z = cfg.sigz * G(rng);
+            px = cfg.sigx > 0 ? cfg.emitx / cfg.sigx * G(rng) : 0;
+            py = cfg.sigy > 0 ? cfg.emity / cfg.sigy * G(rng) : 0;
+            pz = p0 * (1 + cfg.espread * G(rng));
+            const double g = std::sqrt(1 + px * px + py * py + pz * pz);
+            const double vx = px / g, vy = py / g, vz = pz / g;
+            const double zz = z + 0.0;
+            g3d_particle p{};
+            p.x = x - vx * zz / vz + 0.5 * cfg.Lx; p.y = y - vy * zz / vz + yoff;
+            p.zeta = cfg.zeta_center + zz / vz;
+            p.px = px; p.py = py; p.pz = pz; p.w = W; p.alive = 1;
+            p.q = -1; p.qm = -1; p.sp = 0; p.z = 0;
+            int hc; double f;
+            zh.locate(p.zeta, hc, f);
+            if (p.zeta < 0 || hc > N - 2 || p.y < ylo || p.y > yhi) { ++lost_total; continue; }
+            mine.push_back(p);
+        }
+    }
+    set_particles(mine);
+    X0 = -0.5 * cfg.Lx; Y0 = -yoff;
+}
+
+// ---------------------------------------------------------------- time step (src/sim_setup.cpp:178-217)
+void Sim::init() {
+    load_bunch();
+    s = 0; istep = 0;
+    if (cfg.self_fields) {
+        starting_ = true;
+        currents();
+        comoving_field();
+        if (cfg.start_relax) relax_start_field();
+        add_grid_lasers();
+        march(0.0);
+        starting_ = false;
+        sponge_and_real_fields();
+        j0_from_jk();
+    }
+    window_shift_post();
+}
+
+void Sim::step() {
+    window_shift_start();
+    push(s, s + cfg.dz);
+    if (cfg.plasma_on) collect_restreaming();
+    s += cfg.dz; ++istep;
+    if (cfg.self_fields && cfg.plasma_on) {
+        currents();
+        march_plasma(s - cfg.dz, s);
+        sponge_and_real_fields();
+        j0_from_jk();
+    } else if (cfg.self_fields) solve_fields(0.5 * cfg.dz);
+    window_shift_post();
+    hip_check(hipStreamSynchronize(stream()), "synchronize");
+}
+
+void Sim::j0_from_jk() {
+    const long n = long(ns + 1) * nk;
+    copy_planes(j0x.data(), jkx.data(), n);
+    copy_planes(j0y.data(), jky.data(), n);
+}
+
+// ---------------------------------------------------------------- beam push (src/sim_particles.cpp:6-24)
+void Sim::push(double s0, double s1) {
+    if (np <= 0) return;
+    PushArgs c;
+    c.fv = fview(); c.laser = laser; c.X0 = X0; c.Y0 = Y0; c.zc = cfg.zeta_center; c.dz_push = cfg.dz_push;
+    c.push_phase = cfg.push_phase; c.zhand = -1e300; c.zmax = cfg.zeta[N - 1]; c.s1 = s1;
+    c.vz_min = cfg.plasma_on ? cfg.vz_min : 0; c.freeze = cfg.freeze_bunch;
+    c.hmin = 1e300;
+    for (int m = 0; m + 1 < N; ++m) c.hmin = std::min(c.hmin, cfg.zeta[m + 1] - cfg.zeta[m]);
+    c.ylo = -1e300; c.yhi = 1e300;
+    tic(F_PUSH);
+    bunch_push(c, beam.view(), np, s0);
+    toc(F_PUSH);
+}
+
+// ---------------------------------------------------------------- currents (src/sim_particles.cpp:56-207)
+void Sim::currents() {
+    tic(F_CUR);
+    jw.zero();
+    DepositArgs d;
+    d.zax = zax; d.yax = yax; d.hcg = hcg.data(); d.ia = iA.data(); d.id = iD.data(); d.J = jw.data(); d.dx = dx;
+    d.dy = dy; d.nh = ns + 3; d.a = a; d.N = N; d.nx = cfg.nx; d.ny = cfg.ny; d.yf = yf; d.nk = nk;
+    bunch_deposit(d, beam.view(), np);
+    const SpecGeom g = geom();
+    currents_zfilter(g, jw.data(), hcg.data(), cfg.zeta_filter, zax.uniform, S1.data(), jkz.data());
+    fft_run(fft, S1.data(), ns + 1, false);
+    fft_run(fft, jkz.data(), ns + 1, false);
+    const bool keep0 = cfg.plasma_on && cfg.box_mean && !starting_;
+    currents_unpack(g, S1.data(), jkx.data(), jky.data(), jkz.data(), keep0);
+    toc(F_CUR);
+}
+
+void Sim::march(double D) { march_all(mview(D), a, b); }
+
+// ---------------------------------------------------------------- sponge and real fields (src/sim_fields.cpp:156-307)
+void Sim::sponge_and_real_fields() {
+    tic(F_SPONGE);
+    const SpecGeom g = geom();
+    const long ts = long(ns) * nk;
+    const bool sp = cfg.sponge > 0 || cfg.sponge_y > 0;
+    const double opb = 1 + beta_b;
+    if (sp) {
+        sponge_pack(g, Rx.data(), Ry.data(), jkz.data(), hcg.data(), wlo.data(), whi.data(), gm2_b, opb, S1.data(),
+                    S2.data());
+        fft_run(fft, S1.data(), ns, true);
+        sponge_mask(g, S1.data(), maskx.data(), masky_n.data(), masky_h.data(), ns);
+        fft_run(fft, S1.data(), ns, false);
+    }
+    fields_pack3(g, sp ? S1.data() : nullptr, S2.data(), opb, Rx.data(), Ry.data(), Lx.data(), Ly.data(), Ez.data(),
+                 Bz.data(), T3.data(), ts, ns);
+    fft_run(fft, T3.data(), 3 * ns, true);
+    fields_to_F(T3.data(), ts, F.data(), long(ns + 2) * nk, nk, ns);
+    toc(F_SPONGE);
+}
+
+void Sim::solve_fields(double D) {
+    currents();
+    march(D);
+    sponge_and_real_fields();
+    j0_from_jk();
+}
+
+// ---------------------------------------------------------------- start field (src/sim_fields.cpp:44-151, 384-410)
+void Sim::comoving_field() {
+    double sums[2] = {0, 0};
+    for (auto& p : host_particles())
+        if (p.alive == 1) { sums[0] += p.w * p.pz / std::sqrt(1 + p.px * p.px + p.py * p.py + p.pz * p.pz); sums[1] += p.w; }
+    const double beta = comoving_c_ ? 1 : sums[1] > 0 ? sums[0] / sums[1] : 1;
+    const double gm2 = 1 - beta * beta;
+    if (!comoving_c_) { beta_b = beta; gm2_b = gm2; }
+    const SpecGeom g = geom();
+    comoving_solve(g, jkz.data(), hcg.data(), wlo.data(), whi.data(), beta, gm2, S1.data(), S2.data());
+    comoving_R(g, S2.data(), beta, Rx.data(), Ry.data(), Ez.data(), Bz.data());
+}
+
+void Sim::relax_start_field() {
+    const long n = long(ns) * nk, nj = long(ns + 1) * nk;
+    DevBuf<cx> dx_(n), dy_(n);
+    copy_planes(dx_.data(), Rx.data(), n);
+    copy_planes(dy_.data(), Ry.data(), n);
+    comoving_c_ = true; comoving_field(); comoving_c_ = false;
+    axpy_planes(dx_.data(), Rx.data(), -1, n);
+    axpy_planes(dy_.data(), Ry.data(), -1, n);
+    const double D = 100 * cfg.dz;
+    for (int it = 0; it < 3; ++it) {
+        zero_planes(Ez.data(), n); zero_planes(Bz.data(), n);
+        zero_planes(j0x.data(), nj); zero_planes(j0y.data(), nj);
+        march(D);
+    }
+    axpy_planes(Rx.data(), dx_.data(), 1, n);
+    axpy_planes(Ry.data(), dy_.data(), 1, n);
+    hip_check(hipStreamSynchronize(stream()), "synchronize");
+}
+
+void Sim::add_grid_lasers() {
+    if (cfg.glasers.empty()) return;
+    const SpecGeom g = geom();
+    for (const auto& Lz : cfg.glasers) {
+        GridLaserArgs la;
+        la.amp = Lz.a0 * Lz.kL; la.waist = Lz.waist; la.zR = 0.5 * Lz.kL * Lz.waist * Lz.waist; la.zfoc = Lz.z_focus;
+        la.zc = Lz.zeta_c; la.tau = Lz.tau; la.kL = Lz.kL; la.cep = Lz.cep;
+        la.rc = std::cos(Lz.pol_angle); la.rs = std::sin(Lz.pol_angle); la.dx = dx; la.dy = dy; la.X0 = X0; la.Y0 = Y0;
+        la.circular = Lz.circular; la.helicity = Lz.helicity; la.nx = cfg.nx; la.ny = cfg.ny; la.nk = nk;
+        la.zeta = zs.data();
+        grid_laser_fill(la, T3.data(), a, ns);
+        fft_run(fft, T3.data(), ns, false);
+        grid_laser_add(g, T3.data(), Rx.data(), Ry.data(), ns);
+    }
+}
+
+// ---------------------------------------------------------------- plasma (src/sim_plasma.cpp)
+void Sim::ensure_strm(size_t need) {
+    if (strmA.cap >= need) return;
+    const size_t cap = need + need / 8 + 64;
+    strmA.resize(cap); strmB.resize(cap, false);
+}
+
+void Sim::load_plasma(double s0, double s1) {
+    // the list in the reference's order (src/sim_plasma.cpp:43-90): lattice j, i, b2, a2 with the ions of every
+    // loaded species after each electron, ids in that order; then regrouped tile by tile (stable) for the deposit
+    std::vector<g3d_particle> v;
+    std::vector<int> tile_of;
+    long cap = 0;
+    const unsigned long long id0 = (unsigned long long)(istep + 1) << 32;
+    id0_ = id0;
+    const double zmid = 0.5 * (s0 + s1), Delta = s1 - s0;
+    const int nx = cfg.nx, ny = cfg.ny, px = cfg.ppc_x, py = cfg.ppc_y;
+    const double zf = cfg.zeta[N - 1];
+    const bool tiled = tiles.ntile > 0;
+    v.reserve(size_t(nx) * ny * px * py * (1 + cfg.ions.size()) + joining_.size());
+    for (int j = 0; j < ny; ++j) {
+        const double y0 = j * dy, hy = dy;
+        for (int i = 0; i < nx; ++i) {
+            const int t = tiled ? (j / tiles.TY) * tiles.ntx + i / tiles.TX : 0;
+            for (int b2 = 0; b2 < py; ++b2)
+                for (int a2 = 0; a2 < px; ++a2) {
+                    g3d_particle p{};
+                    p.x = (i + (a2 + 0.5) / px) * dx;
+                    p.y = y0 + (b2 + 0.5) / py * hy;
+                    const double n = cfg.plasma_density(p.x + X0, p.y + Y0, zmid);
+                    if (n <= 0) continue;
+                    p.w = n * dx * hy * Delta / (px * py);
+                    p.q = -1; p.qm = -1; p.sp = SP_PLASMA; p.alive = 1;
+                    p.zeta = zf; p.z = s0;
+                    if (cfg.free_electrons) { p.id = id0 + v.size(); v.push_back(p); tile_of.push_back(t); }
+                    for (size_t k = 0; k < cfg.ions.size(); ++k) {
+                        const g3d_ion& is = cfg.ions[k];
+                        if (!cfg.loaded(is)) continue;
+                        g3d_particle q = p;
+                        q.q = is.Z0; q.qm = is.Z0 / is.mass; q.w = is.atoms * p.w; q.sp = SP_ION + int(k);
+                        q.id = id0 + v.size();
+                        v.push_back(q); tile_of.push_back(t);
+                    }
+                }
+        }
+    }
+    fresh_last = long(v.size());
+    if (tiled) {
+        std::vector<long> off(tiles.ntile + 1, 0);
+        for (int t : tile_of) ++off[t + 1];
+        for (long t = 0; t < tiles.ntile; ++t) off[t + 1] += off[t];
+        std::vector<long> pos(off.begin(), off.end() - 1);
+        std::vector<g3d_particle> w(v.size());
+        for (size_t k = 0; k < v.size(); ++k) w[pos[tile_of[k]]++] = v[k];
+        v.swap(w);
+        tile_off.from_host(off.data(), 0, off.size());
+        n_tiled = fresh_last;
+        tiles.n_tiled = n_tiled;
+    } else n_tiled = 0;
+    for (g3d_particle p : joining_) { p.alive = 6; v.push_back(p); }
+    for (const auto& p : v)
+        if (is_ion(p.sp) && cfg.ions[p.sp - SP_ION].ionization) cap += cfg.ions[p.sp - SP_ION].Zmax - std::lround(p.q);
+    nstrm = long(v.size());
+    born_cap = cap;
+    ensure_strm(v.size() + size_t(cap));
+    if (cap > 0 && born.cap < size_t(cap)) born.resize(size_t(cap));
+    if (!nstrm) return;
+    if (aos.size() < v.size()) aos.resize(v.size());
+    aos.from_host(v.data(), 0, v.size());
+    parts_from_aos(aos.data(), strmA.view(), nstrm);
+    parts_copy(strmA.view(), strmB.view(&strmA), nstrm);
+}
+
+void Sim::stream_spectra() {
+    // stand-alone form: jsr -> jsn through the pack, transform and unpack of the march (jsr is consumed)
+    sspec_pack(jsr.data(), nk, cfg.nx, cfg.ny, sp3.data(), nullptr, nullptr, 0, 0, TileGeom{});
+    fft_run(fft, sp3.data(), 2, false);
+    sspec_unpack(geom(), sp3.data(), jsn.data(), cfg.smooth_length * cfg.smooth_length, cfg.box_mean);
+}
+
+void Sim::march_slice(int n, double D) { march_one(mview(D), n, nullptr, nullptr, nullptr, nullptr, 0); }
+
+void Sim::real_fields_slice(int n) {
+    const SpecGeom g = geom();
+    const long l = n - a;
+    fields_pack3(g, nullptr, nullptr, 0, Rx.data() + l * nk, Ry.data() + l * nk, Lx.data() + l * nk, Ly.data() + l * nk,
+                 Ez.data() + l * nk, Bz.data() + l * nk, tp3.data(), nk, 1);
+    fft_run(fft, tp3.data(), 3, true);
+    fields_to_F(tp3.data(), nk, F.data() + l * nk, long(ns + 2) * nk, nk, 1);
+}
+
+void Sim::march_plasma(double s0, double s1) {
+    const double Delta = s1 - s0, D = 0.5 * Delta;
+    const size_t pl = size_t(nk);
+    load_plasma(s0, s1);
+    joining_.clear();
+    const bool extrap = cfg.predictor_extrapolate;
+    std::vector<double> cen(N - 1);
+    for (int m = 0; m < N - 1; ++m) cen[m] = 0.5 * (cfg.zeta[m] + cfg.zeta[m + 1]);
+    const SpecGeom g = geom();
+    jpred.zero();
+    jsr.zero();
+    ModeArgs m{};
+    m.mv = mview(D);
+    m.jsn = jsn.data(); m.sin_ = jsin.data(); m.si0 = jsi0.data(); m.pin = jpred.data(); m.dl = dlt.data();
+    m.js0x = js0x.data(); m.js0y = js0y.data(); m.js0z = js0z.data();
+    m.jkx = jkx.data(); m.jky = jky.data(); m.jkz = jkz.data();
+    m.T = tp3.data(); m.kxs = kxs.data(); m.kys = kys.data(); m.filt = filt.data(); m.filty = filty.data();
+    m.a2 = cfg.smooth_length * cfg.smooth_length; m.extrap = extrap; m.keep0 = cfg.box_mean; m.ny = cfg.ny;
+    const long fpl = long(ns + 2) * nk;
+    auto real_fields = [&](int n) {
+        tic(F_IFFT);
+        fft_run(fft, tp3.data(), 3, true);
+        toc(F_IFFT);
+        tic(F_TOF);
+        fields_to_F(tp3.data(), nk, F.data() + long(n - a) * nk, fpl, nk, 1);
+        toc(F_TOF);
+    };
+    // front: half-cell N-2 predicted with its level-s0 value; slice N-1
+    m.pk = nullptr; m.n = N - 1; m.first = 1; m.last = 1; m.top = 1; m.wex = 0; m.qcorr = 0;
+    m.bRx = m.bRy = m.bEz = m.bBz = nullptr;
+    tic(F_MODES);
+    plasma_modes(m);
+    toc(F_MODES);
+    real_fields(N - 1);
+
+    StreamArgs sc;
+    sc.fv = fview(); sc.laser = laser; sc.X0 = X0; sc.Y0 = Y0; sc.zc = cfg.zeta_center; sc.ylo = -1e300; sc.yhi = 1e300;
+    sc.zs = zs.data(); sc.J = jsr.data(); sc.ia = iA.data(); sc.id = iD.data();
+    sc.Delta = Delta; sc.s1 = s1; sc.sTrap = s1 + (cfg.trap_advance - 1) * Delta; sc.vz_beam = cfg.vz_beam;
+    sc.ions = ion_dev.data(); sc.adk = adk_dev.data(); sc.born = born.cap ? born.view() : Parts{}; sc.bcnt = counters.data();
+    sc.bcap = 0; sc.key = 0;
+    const bool ionz = born_cap > 0;
+    // tiles only when this step loaded tiled plasma (otherwise the slabs would hold an earlier step's current)
+    const TileGeom tg = n_tiled > 0 ? tiles : TileGeom{};
+    const uint64_t kst = key_step(cfg.seed, istep);
+    const int passes = cfg.iterations;
+    const int nx = cfg.nx, ny = cfg.ny;
+    const int i0 = ((int(std::lround(-X0 / dx)) % nx) + nx) % nx;
+    const int j0 = ((int(std::lround(-Y0 / dy)) % ny) + ny) % ny;
+    bool have_dl = false;
+    long born_base = nstrm;
+    long nlist = nstrm;          // particles in the list (freed electrons are appended on the device)
+    for (int n = N - 2; n >= a; --n) {
+        const double wex = (n >= 1 && n + 1 <= N - 2 && have_dl) ? (cen[n] - cen[n - 1]) / (cen[n + 1] - cen[n]) : 0.0;
+        for (int pass = 0; pass < passes; ++pass) {
+            sc.n = n;
+            if (ionz) {
+                hip_check(hipMemsetAsync(counters.data(), 0, sizeof(int), stream()), "hipMemset");
+                sc.key = key_slice(kst, n, pass);
+                sc.bcap = int(std::min<long>(long(born.cap), long(strmA.cap) - nlist));
+            }
+            tic(F_STREAM);
+            stream_halfcell(sc, strmA.view(), strmB.view(&strmA), nlist, tg);
+            toc(F_STREAM);
+            const bool last = pass == passes - 1;
+            sspec_pack(jsr.data(), nk, nx, ny, sp3.data(), last ? rho_lines.data() + long(n) * (nx + ny) : nullptr,
+                       last && rho_lines_i.size() ? rho_lines_i.data() + long(n) * (nx + ny) : nullptr, i0, j0, tg);
+            tic(F_SFFT);
+            fft_run(fft, sp3.data(), 2, false);
+            toc(F_SFFT);
+            m.pk = sp3.data(); m.n = n; m.first = pass == 0; m.last = last; m.top = 0; m.wex = wex;
+            m.qcorr = D * wlo_h[n + 1];
+            if (passes > 1) { m.bRx = bk.data(); m.bRy = bk.data() + pl; m.bEz = bk.data() + 2 * pl; m.bBz = bk.data() + 3 * pl; }
+            tic(F_MODES);
+            plasma_modes(m);
+            toc(F_MODES);
+            real_fields(n);
+        }
+        // the passes wrote the new state into strmB: it becomes the list
+        std::swap(strmA, strmB);
+        // strmB borrowed the constants of strmA; after the swap the owner is strmA again only if the constant arrays
+        // move with it, so swap the constants back
+        strmA.w.swap(strmB.w); strmA.rem.swap(strmB.rem); strmA.sp.swap(strmB.sp); strmA.id.swap(strmB.id);
+        if (ionz) {
+            append_born(born.view(), counters.data(), strmA.view(), strmB.view(&strmA), nlist, nlist + born_cap);
+            int cnt = 0;
+            hip_check(hipMemcpyAsync(&cnt, counters.data(), sizeof(int), hipMemcpyDeviceToHost, stream()), "hipMemcpy D2H");
+            hip_check(hipStreamSynchronize(stream()), "synchronize");
+            if (cnt > sc.bcap) throw std::runtime_error("ionization: list of freed electrons overflowed");
+            nlist += cnt;
+        }
+        have_dl = true;
+    }
+    (void)born_base;
+    nstrm = nlist;
+    // trapped particles (alive 5): fast ones join the beam list, slow ones stream again in the next step
+    std::vector<g3d_particle> sv;
+    bool need_all = false;
+    for (const auto& is : cfg.ions) need_all = need_all || is.ionization;
+    if (need_all) sv = host_strm();
+    else {
+        counters.zero();
+        if (aos.size() < size_t(std::max<long>(nstrm, 1))) aos.resize(size_t(std::max<long>(nstrm, 1)));
+        if (aos_idx.size() < aos.size()) aos_idx.resize(aos.size());
+        collect_alive(strmA.view(), nstrm, 5, aos.data(), aos_idx.data(), counters.data() + 1, long(aos.size()));
+        int cnt = 0;
+        hip_check(hipMemcpyAsync(&cnt, counters.data() + 1, sizeof(int), hipMemcpyDeviceToHost, stream()), "hipMemcpy D2H");
+        hip_check(hipStreamSynchronize(stream()), "synchronize");
+        if (cnt > 0) {
+            std::vector<g3d_particle> t(cnt);
+            std::vector<long> idx(cnt);
+            aos.to_host(t.data(), 0, cnt);
+            aos_idx.to_host(idx.data(), 0, cnt);
+            std::vector<int> ord(cnt);
+            std::iota(ord.begin(), ord.end(), 0);
+            // the reference's list order: the fresh plasma by id, then the untiled tail in place
+            auto key = [&](int k) { return idx[k] < n_tiled ? long(t[k].id - id0_) : idx[k]; };
+            std::sort(ord.begin(), ord.end(), [&](int x, int y) { return key(x) < key(y); });
+            for (int k : ord) sv.push_back(t[k]);
+        }
+    }
+    std::vector<g3d_particle> add;
+    for (auto& h : ion_stats) h.clear();
+    for (auto& p : sv) {
+        if (is_ion(p.sp) && cfg.ions[p.sp - SP_ION].ionization) {
+            const auto& is = cfg.ions[p.sp - SP_ION];
+            std::vector<double>& h = ion_stats[p.sp - SP_ION];
+            if (h.empty()) h.assign(is.Zmax + 2, 0.0);
+            const int Z = int(std::lround(p.q));
+            if (p.alive == 1) h[Z] += p.w;
+            h[is.Zmax + 1] += (Z - is.Z0) * p.w;
+        }
+        if (p.alive != 5) continue;
+        const double gg = std::sqrt(1 + p.px * p.px + p.py * p.py + p.pz * p.pz);
+        p.z = s1;
+        if (p.pz / gg >= cfg.vz_beam) {
+            p.alive = 1; p.sp = trapped_species(p.sp); ++trapped_total;
+            add.push_back(p);
+        } else { p.alive = 6; joining_.push_back(p); }
+    }
+    if (!add.empty()) {
+        std::vector<g3d_particle> all = host_particles(), keep;
+        keep.reserve(all.size() + add.size());
+        for (auto& p : all) if (p.alive == 1) keep.push_back(p);
+        keep.insert(keep.end(), add.begin(), add.end());
+        set_particles(keep);
+    }
+    sub_planes(jkz.data(), js0z.data(), long(ns + 1) * nk);
+    (void)g;
+}
+
+void Sim::collect_restreaming() {
+    if (np <= 0) return;
+    counters.zero();
+    count_alive(beam.view(), np, 4, counters.data() + 2);
+    int cnt = 0;
+    hip_check(hipMemcpyAsync(&cnt, counters.data() + 2, sizeof(int), hipMemcpyDeviceToHost, stream()), "hipMemcpy D2H");
+    hip_check(hipStreamSynchronize(stream()), "synchronize");
+    if (!cnt) return;
+    std::vector<g3d_particle> all = host_particles(), keep;
+    for (auto& p : all) {
+        if (p.alive == 4) { g3d_particle q = p; q.alive = 6; ++restreamed_total; joining_.push_back(q); }
+        else if (p.alive == 1) keep.push_back(p);
+    }
+    set_particles(keep);
+}
+
+// ---------------------------------------------------------------- window (src/sim_setup.cpp:222-281)
+void Sim::window_shift_post() {
+    if (!cfg.move_x && !(yf && cfg.move_y)) return;
+    std::array<double, 3> c{0, 0, 0};
+    if (np > 0) {
+        bunch_centroid(beam.view(), np, X0, Y0, cent_dev.data());
+        cent_dev.to_host(c.data(), 0, 3);
+    }
+    cent_.push_back(c);
+}
+void Sim::window_shift_start() {
+    if (!cfg.move_x && !(yf && cfg.move_y)) return;
+    if (long(cent_.size()) < cfg.window_lag) return;
+    const std::array<double, 3> c = cent_.front();
+    cent_.pop_front();
+    if (c[2] <= 0) return;
+    const int nxs = cfg.move_x ? int(std::lround((c[0] / c[2] - (X0 + 0.5 * cfg.Lx)) / dx)) : 0;
+    const int nys = yf && cfg.move_y ? int(std::lround((c[1] / c[2] - (Y0 + 0.5 * cfg.Ly)) / dy)) : 0;
+    if (std::abs(nxs) >= 2 || std::abs(nys) >= 2) shift_window(nxs, nys);
+}
+void Sim::shift_window(int nxs, int nys) {
+    X0 += nxs * dx; Y0 += nys * dy;
+    const double sx = nxs * dx, sy = nys * dy;
+    parts_shift(beam.view(), np, sx, sy);
+    for (auto& p : joining_) { p.x -= sx; p.y -= sy; }
+    if (!cfg.self_fields) return;
+    auto rot = [&](DevBuf<cx>& A) {
+        rotate_planes(A.data(), long(A.size()), nk, cfg.nx, kxs.data(), kys.data(), sx, sy, nys != 0);
+    };
+    rot(Rx); rot(Ry); rot(Ez); rot(Bz); rot(j0x); rot(j0y);
+    if (cfg.plasma_on) { rot(js0x); rot(js0y); rot(js0z); }
+    DevBuf<float> tmp(F.size());
+    roll_F(F.data(), tmp.data(), long(F.size()), nk, cfg.nx, cfg.ny, nxs, nys);
+    F.swap(tmp);
+    hip_check(hipStreamSynchronize(stream()), "synchronize");
+}
+
+}  // namespace g3d

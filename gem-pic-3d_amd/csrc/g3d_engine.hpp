@@ -1,0 +1,154 @@
+// The engine: a Simulation of the per-slice march on one MI355X.  The member functions keep the names and the
+// contracts of the reference's Simulation (src/sim.hpp:222-360) so that each one is the drop-in of the
+// reference's function of the same name; their bodies are sequences of the gfx950 kernels of this library on one
+// stream, with no host synchronisation inside the half-cell loop.
+#pragma once
+#include <array>
+#include <deque>
+#include <string>
+#include <vector>
+#include "g3d_kernels.hpp"
+#include "g3d_runtime.hpp"
+
+namespace g3d {
+
+// physical constants (SI) of the reference's unit system
+constexpr double kC = 299792458.0, kE = 1.602176634e-19, kMe = 9.1093837015e-31, kEps0 = 8.8541878128e-12;
+
+struct Config {
+    double lambda0 = 0, Lu = 0, Tu = 0, nc = 0;
+    int nx = 0, ny = 0, nzeta = 0;
+    double Lx = 0, Ly = 0, Lwin = 0;
+    bool y_fourier = true;
+    std::vector<double> zeta, ynode;
+    double dz = 0, z_end = 0, dz_push = 0, sponge = 0, sponge_y = 0, sponge_rate = 0.1;
+    bool move_x = true, move_y = true;
+    int window_lag = 4;
+    bool self_fields = true, zeta_filter = true, k_filter = true;
+    double push_phase = 0.05;
+    bool laser_on = false;
+    int pol = 0;
+    double a0 = 0, kL = 1, theta = 0, tau = 1, cep = 0, start_fwhm = 3.5, stop_fwhm = 3.5;
+    double charge = 0, gamma = 1, sigx = 0, sigy = 0, sigz = 0, emitx = 0, emity = 0, espread = 0, zeta_center = 0;
+    long nmacro = 0;
+    unsigned long seed = 1;
+    std::vector<g3d_glaser> glasers;
+    bool plasma_on = false;
+    double n0 = 0, r_max = 1e300, channel_depth = 0, r_channel = 1, smooth_length = 0;
+    std::vector<double> pz_points, pz_values;
+    bool parabolic = false;
+    int ppc_x = 2, ppc_y = 2, iterations = 2;
+    double vz_min = 0.8, vz_beam = 0.9, trap_advance = 1.0;
+    bool predictor_extrapolate = true, free_electrons = true;
+    std::vector<g3d_ion> ions;
+    bool freeze_bunch = false, start_relax = true, box_mean = true;
+
+    static Config from_abi(const g3d_config& g);
+    double plasma_density(double x, double y, double z) const;
+    bool loaded(const g3d_ion& s) const { return s.mobile || s.ionization; }
+};
+
+// structure-of-arrays particle storage; `shared` borrows w, sp, id, rem of another store (the second buffer of
+// the streaming list)
+struct PartStore {
+    DevBuf<double> x, y, zeta, px, py, pz, w, z, q, qm, rem;
+    DevBuf<int> alive, sp;
+    DevBuf<unsigned long long> id;
+    size_t cap = 0;
+    void resize(size_t n, bool own_constants = true);
+    Parts view(const PartStore* shared = nullptr);
+};
+
+class Sim {
+public:
+    explicit Sim(const Config& c);
+    ~Sim();
+    void init();
+    void step();
+    bool finished() const { return s >= cfg.z_end - 1e-9; }
+
+    // the reference's member functions of the five drop-in files
+    void push(double s0, double s1);
+    void currents();
+    void march(double D);
+    void sponge_and_real_fields();
+    void solve_fields(double D);
+    void comoving_field();
+    void relax_start_field();
+    void add_grid_lasers();
+    void load_plasma(double s0, double s1);
+    void stream_spectra();
+    void march_slice(int n, double D);
+    void real_fields_slice(int n);
+    void march_plasma(double s0, double s1);
+    void collect_restreaming();
+    void load_bunch();
+    void window_shift_start();
+    void window_shift_post();
+    void shift_window(int nxs, int nys);
+    void j0_from_jk();
+
+    std::vector<g3d_particle> host_particles();
+    void set_particles(const std::vector<g3d_particle>& v);
+    std::vector<g3d_particle> host_strm();
+
+    Config cfg;
+    int N = 0, a = 0, b = 0, ns = 0;
+    long nk = 0;
+    double dx = 0, dy = 0, s = 0, X0 = 0, Y0 = 0, beta_b = 1, gm2_b = 0;
+    bool yf = true, starting_ = false, comoving_c_ = false;
+    long istep = 0, np = 0, nstrm = 0, born_cap = 0, fresh_last = 0;
+    long trapped_total = 0, restreamed_total = 0, lost_total = 0;
+    PlaneWave laser{};
+    std::vector<g3d_particle> joining_;
+    std::vector<std::vector<double>> ion_stats;
+
+    // device arrays in the reference's layouts
+    DevBuf<cx> Rx, Ry, Ez, Bz, Lx, Ly, j0x, j0y, jkx, jky, jkz, S1, S2, T3;
+    DevBuf<cx> js0x, js0y, js0z, jsn, jsin, jsi0, jpred, dlt, bk, sp3, tp3;
+    DevBuf<double> jw, jsr, rho_lines, rho_lines_i;
+    DevBuf<float> F;
+    DevBuf<double> kxd, kxs, filt, maskx, kyd, kys, filty, masky_n, masky_h, iD, iA, zs, hcg, wlo, whi, wsl;
+    std::vector<double> wlo_h, hcg_h;
+    Axis zax{}, yax{};
+    FftPlan fft;
+    PartStore beam, strmA, strmB, born;
+    DevBuf<IonDev> ion_dev;
+    DevBuf<AdkLevel> adk_dev;
+    DevBuf<int> counters;             // [0] freed electrons of the pass, [1] collected, [2] counted
+    DevBuf<double> cent_dev;
+    DevBuf<g3d_particle> aos;         // staging of AoS lists
+    DevBuf<long> aos_idx;
+    std::deque<std::array<double, 3>> cent_;
+    // transverse tiles of the streaming list (tile-ordered fresh plasma, LDS deposit windows)
+    TileGeom tiles{};
+    DevBuf<long> tile_off;
+    DevBuf<double> slab;
+    long n_tiled = 0;
+    unsigned long long id0_ = 0;
+
+    // timing of kernel families with HIP events on the library stream
+    bool timing = false;
+    static constexpr int kFam = 8;
+    double fam_ms[kFam] = {};
+    long fam_n[kFam] = {};
+    void tic(int fam);
+    void toc(int fam);
+    void flush_timers();
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+    std::vector<hipEvent_t> ev_pool;
+    hipEvent_t ev_open = nullptr;
+    int ev_open_fam = -1;
+
+private:
+    SpecGeom geom() const;
+    MarchView mview(double D);
+    FieldView fview();
+    void setup_ions();
+    void setup_tiles();
+    void ensure_strm(size_t need);
+};
+
+extern const char* const kFamilyNames[Sim::kFam];
+
+}  // namespace g3d

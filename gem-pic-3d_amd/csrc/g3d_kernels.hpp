@@ -1,0 +1,141 @@
+// Launch wrappers of the gfx950 kernels used by the engine (g3d_engine.cpp).  All launches go to the library
+// stream; none synchronises.
+#pragma once
+#include "g3d_common.hpp"
+#include "g3d_fft.hpp"
+#include "g3d_march.hpp"
+
+namespace g3d {
+
+using Parts = g3d_parts;
+
+// ---------------------------------------------------------------- particles
+// the push context of the reference (src/sim_kernels.hpp:10-17) without the external elements
+struct PushArgs {
+    FieldView fv;
+    PlaneWave laser;
+    double X0, Y0, zc, dz_push, push_phase, zhand, zmax, ylo, yhi, s1, vz_min, hmin;
+    int freeze;
+};
+// the streaming context (src/sim_stream.hpp:20-29)
+struct StreamArgs {
+    FieldView fv;
+    PlaneWave laser;
+    double X0, Y0, zc, ylo, yhi;
+    const double* zs;
+    double* J;                       // jsr: jx jy jz rho- rho+ (5 nk)
+    const double *ia, *id;
+    double Delta, s1, sTrap, vz_beam;
+    int n;
+    const IonDev* ions;
+    const AdkLevel* adk;
+    Parts born;                      // electrons freed in this pass
+    int* bcnt;
+    int bcap;
+    uint64_t key;
+};
+
+// Transverse tiles of the streaming list: the fresh plasma is loaded tile by tile (TX x TY cells), particles
+// [off[t], off[t+1]) belong to tile t.  A tile's workgroup deposits into an LDS window of WX x WY nodes starting
+// at node (tx TX - 1 - M, ty TY - 1 - M) and writes it to its slab (5 planes of WX x WY); deposits that leave the
+// window go to jsr with global atomics.  Particles from off[ntile] on (joining, freed) are not tiled.
+struct TileGeom {
+    int TX, TY, M, WX, WY, ntx, nty;
+    long ntile, n_tiled;              // tiles, particles in tiles (= off[ntile])
+    const long* off;
+    double* slab;
+};
+
+void bunch_push(const PushArgs& a, Parts p, long np, double s0);
+// beam deposit into jw [3][ns+3][nk] (src/sim_particles.cpp:66-106)
+struct DepositArgs {
+    Axis zax, yax;
+    const double *hcg, *ia, *id;
+    double* J;
+    double dx, dy;
+    int nh, a, N, nx, ny, yf;
+    long nk;
+};
+void bunch_deposit(const DepositArgs& a, Parts p, long np);
+// one half-cell pass of the streaming particles: src -> dst (the state of every particle is written); tiles
+// with tg.ntile > 0, otherwise every deposit is a global atomic
+void stream_halfcell(const StreamArgs& a, Parts src, Parts dst, long nstrm, const TileGeom& tg);
+// append the freed electrons of the last pass (count on the device) to both buffers; nstrm_dev += count
+void append_born(Parts born, const int* bcnt, Parts p0, Parts p1, long base, long cap);
+// window centroid sums of the bunch particles (weight, w x, w y), deterministic tree order
+void bunch_centroid(Parts p, long np, double X0, double Y0, double* out3);
+// copy the particles with alive == code into a compact AoS list (slot order is arbitrary; idx holds the source
+// index for a stable host-side ordering); *count on the device
+void collect_alive(Parts p, long n, int code, g3d_particle* out, long* idx, int* count, long cap);
+void count_alive(Parts p, long n, int code, int* count);
+// AoS <-> SoA
+void parts_from_aos(const g3d_particle* in, Parts p, long n);
+void parts_to_aos(Parts p, g3d_particle* out, long n);
+void parts_copy(Parts src, Parts dst, long n);
+void parts_shift(Parts p, long n, double sx, double sy);
+
+// ---------------------------------------------------------------- spectra and fields (Fourier y)
+struct SpecGeom {
+    int nx, ny, ns, N, a;
+    long nk;
+    const double *kx, *ky, *kxs, *kys, *filt, *filty;
+};
+// currents: zeta filter of jw into S1 (jx + i jy) and jkz (jz + 0 i), ns+1 half-cell planes
+void currents_zfilter(const SpecGeom& g, const double* jw, const double* hcg, int zfilter, int zuniform, cx* S1, cx* jkz);
+// unpack the pair, transverse filter, k = 0 rule
+void currents_unpack(const SpecGeom& g, const cx* S1, cx* jkx, cx* jky, cx* jkz, int keep0);
+// sponge: bound field psi (S2) and R - R_b packed (S1) over ns slices
+void sponge_pack(const SpecGeom& g, const cx* Rx, const cx* Ry, const cx* jkz, const double* hcg, const double* wlo,
+                 const double* whi, double g2, double opb, cx* S1, cx* S2);
+void sponge_mask(const SpecGeom& g, cx* S1, const double* mx, const double* mn, const double* mh, int count);
+// unpack the sponged R and add R_b (when S1 != null), then pack R, L, E_z/B_z for the inverse transforms
+void fields_pack3(const SpecGeom& g, const cx* S1, const cx* S2, double opb, cx* Rx, cx* Ry, const cx* Lx,
+                  const cx* Ly, const cx* Ez, const cx* Bz, cx* T, long tstride, int count);
+// real fields from the three inverse transforms into F planes fl0 ... fl0+count-1
+void fields_to_F(const cx* T, long tstride, float* F, long fpl, long nk, int count);
+// streaming current of one half-cell: the sum of jsr and of the tile slabs covering each point (tg.ntile > 0),
+// packed as (jx + i jy, jz); the density lines (when rl != null); jsr is zeroed for the next pass
+void sspec_pack(double* J, long nk, int nx, int ny, cx* pk, double* rl, double* rli, int i0, int j0,
+                const TileGeom& tg);
+// unpack + transverse filter + smoothing of the streaming current spectra (src/sim_plasma.cpp:105-115)
+void sspec_unpack(const SpecGeom& g, const cx* pk, cx* jsn, double a2, int keep0);
+
+// the per-mode work of one plasma half-cell pass (see g3d_plasma.hip)
+struct ModeArgs {
+    MarchView mv;
+    const cx* pk;                     // spectra of the streaming current (2 planes), null at the front slice
+    cx *jsn, *sin_, *si0, *pin, *dl;  // 3 planes each
+    cx *js0x, *js0y, *js0z;
+    cx *jkx, *jky, *jkz;              // writable views of the march's jk
+    cx *bRx, *bRy, *bEz, *bBz;        // backup of slice n for the passes
+    cx* T;                            // 3 packed planes for the inverse transforms
+    const double *kxs, *kys, *filt, *filty;
+    double wex, qcorr, a2;
+    int n, first, last, top, extrap, keep0, ny;
+};
+void plasma_modes(const ModeArgs& m);
+// jz -= js0z over count planes
+void sub_planes(cx* a, const cx* b, long n);
+void copy_planes(cx* dst, const cx* src, long n);
+void zero_planes(cx* a, long n);
+
+// grid laser (src/sim_fields.cpp:321-374, Fourier y): R_x + i R_y = 2 E_perp of the paraxial Gaussian of every
+// slice into T (count planes), then the engine transforms and adds
+struct GridLaserArgs {
+    double amp, waist, zR, zfoc, zc, tau, kL, cep, rc, rs, dx, dy, X0, Y0;
+    int circular, helicity, nx, ny;
+    long nk;
+    const double* zeta;
+};
+void grid_laser_fill(const GridLaserArgs& a, cx* T, int n0, int count);
+void grid_laser_add(const SpecGeom& g, const cx* T, cx* Rx, cx* Ry, int count);
+// phase rotation e^{i(k_x sx + k_y sy)} of count planes (window shift, src/sim_setup.cpp:259-267)
+void rotate_planes(cx* A, long n, long nk, int nx, const double* kxs, const double* kys, double sx, double sy, int ys);
+void roll_F(const float* F, float* out, long n, long nk, int nx, int ny, int nxs, int nys);
+// comoving field: Thomas solve along zeta per mode (src/sim_fields.cpp:81-114, Fourier y)
+void comoving_solve(const SpecGeom& g, const cx* jkz, const double* hcg, const double* wlo, const double* whi,
+                    double beta, double gm2, cx* cp, cx* dp);
+void comoving_R(const SpecGeom& g, const cx* psi, double beta, cx* Rx, cx* Ry, cx* Ez, cx* Bz);
+void axpy_planes(cx* y, const cx* x, double s, long n);    // y = y + s x (s = +-1)
+
+}  // namespace g3d

@@ -1,0 +1,498 @@
+// Particle kernels: the RK4 push in z of the beam list with its gathers and the analytic laser, the beam deposit,
+// the streaming half-cell pass (two Boris rotations, the deposit, trapping, ADK ionization), and the list tools.
+// One lane per particle over structure-of-arrays streams.
+#include "g3d_kernels.hpp"
+#include "g3d_runtime.hpp"
+
+namespace g3d {
+namespace {
+constexpr int kT = 256;
+
+// ---------------------------------------------------------------- RK4 in z (src/sim_kernels.hpp:73-149)
+struct St { double x, y, zeta, px, py, pz; };
+
+__device__ __forceinline__ bool rhs(const PushArgs& c, const St& Y, double z, double qm, St& d) {
+    const double g = sqrt(1 + Y.px * Y.px + Y.py * Y.py + Y.pz * Y.pz);
+    if (Y.pz < 0.3 * g) return false;
+    const double gm = (1 + Y.px * Y.px + Y.py * Y.py) / (g + Y.pz);
+    double E[3], B[3], Ee[3], Be[3];
+    gather(c.fv, wrap_mod(Y.x, c.fv.Lx), c.fv.yf ? wrap_mod(Y.y, c.fv.Ly) : Y.y, Y.zeta, E, B);
+    c.laser.field(Y.x + c.X0, z, z - (Y.zeta - c.zc), Ee, Be);
+    for (int k = 0; k < 3; ++k) { E[k] += Ee[k]; B[k] += Be[k]; }
+    const double ip = 1 / Y.pz;
+    d.x = Y.px * ip; d.y = Y.py * ip; d.zeta = -gm * ip;
+    d.px = qm * (g * E[0] + Y.py * B[2] - Y.pz * B[1]) * ip;
+    d.py = qm * (g * E[1] + Y.pz * B[0] - Y.px * B[2]) * ip;
+    d.pz = qm * (g * E[2] + Y.px * B[1] - Y.py * B[0]) * ip;
+    return true;
+}
+__device__ __forceinline__ St axpy(const St& a, double h, const St& b) {
+    return St{a.x + h * b.x, a.y + h * b.y, a.zeta + h * b.zeta, a.px + h * b.px, a.py + h * b.py, a.pz + h * b.pz};
+}
+
+__global__ void __launch_bounds__(kT) k_bunch_push(PushArgs c, Parts P, long np, double z0) {
+    const long ip = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (ip >= np) return;
+    const int alive = P.alive[ip];
+    if (alive != 1 && alive != 3) return;
+    double z = alive == 3 ? c.s1 - P.rem[ip] : z0;
+    if (c.freeze && is_bunch(P.sp[ip])) { P.z[ip] = c.s1; return; }
+    St Y{P.x[ip], P.y[ip], P.zeta[ip], P.px[ip], P.py[ip], P.pz[ip]};
+    const double qm = P.qm[ip];
+    auto store = [&](const St& s) {
+        P.x[ip] = s.x; P.y[ip] = s.y; P.zeta[ip] = s.zeta; P.px[ip] = s.px; P.py[ip] = s.py; P.pz[ip] = s.pz;
+    };
+    while (z < c.s1 - 1e-12 * fabs(c.s1)) {
+        if (Y.zeta < c.zhand) { store(Y); P.alive[ip] = 2; P.rem[ip] = c.s1 - z; return; }
+        if (c.vz_min > 0 && Y.pz < c.vz_min * sqrt(1 + Y.px * Y.px + Y.py * Y.py + Y.pz * Y.pz)) {
+            store(Y); P.z[ip] = z; P.alive[ip] = 4;
+            return;
+        }
+        double ds = c.dz_push < c.s1 - z ? c.dz_push : c.s1 - z;
+        {
+            const double g = sqrt(1 + Y.px * Y.px + Y.py * Y.py + Y.pz * Y.pz);
+            const double lz = 0.5 * c.hmin * Y.pz * (g + Y.pz) / (1 + Y.px * Y.px + Y.py * Y.py);
+            if (lz < ds) ds = (c.s1 - z) / ceil((c.s1 - z) / lz);
+        }
+        if (c.laser.on) {
+            const double g = sqrt(1 + Y.px * Y.px + Y.py * Y.py + Y.pz * Y.pz);
+            const double hl = g - Y.px * c.laser.kx - Y.pz * c.laser.kz;
+            const double lim = c.push_phase * Y.pz / (c.laser.kL * (hl > 1e-12 ? hl : 1e-12));
+            if (lim < ds) ds = (c.s1 - z) / ceil((c.s1 - z) / lim);
+        }
+        St k1, k2, k3, k4;
+        const double hd = 0.5 * ds;
+        if (!(rhs(c, Y, z, qm, k1) && rhs(c, axpy(Y, hd, k1), z + hd, qm, k2) && rhs(c, axpy(Y, hd, k2), z + hd, qm, k3) &&
+              rhs(c, axpy(Y, ds, k3), z + ds, qm, k4))) {
+            P.alive[ip] = 0;
+            return;
+        }
+        const double w = ds / 6;
+        Y.x += w * (k1.x + 2 * k2.x + 2 * k3.x + k4.x);
+        Y.y += w * (k1.y + 2 * k2.y + 2 * k3.y + k4.y);
+        Y.zeta += w * (k1.zeta + 2 * k2.zeta + 2 * k3.zeta + k4.zeta);
+        Y.px += w * (k1.px + 2 * k2.px + 2 * k3.px + k4.px);
+        Y.py += w * (k1.py + 2 * k2.py + 2 * k3.py + k4.py);
+        Y.pz += w * (k1.pz + 2 * k2.pz + 2 * k3.pz + k4.pz);
+        z += ds;
+    }
+    store(Y);
+    P.z[ip] = c.s1;
+    int al = (Y.zeta < 0 || Y.zeta >= c.zmax || Y.y < c.ylo || Y.y > c.yhi) ? 0 : 1;
+    if (al == 1 && Y.zeta < c.zhand) { al = 2; P.rem[ip] = 0; }
+    P.alive[ip] = al;
+}
+
+// ---------------------------------------------------------------- beam deposit (src/sim_particles.cpp:66-106)
+__global__ void __launch_bounds__(kT) k_bunch_deposit(DepositArgs c, Parts P, long np) {
+    const long ip = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (ip >= np || P.alive[ip] != 1) return;
+    int m; double fz;
+    c.zax.locate(P.zeta[ip], m, fz);
+    const int hw = m - (c.a - 2);
+    if (hw < 0 || hw >= c.nh || m < 0 || m > c.N - 2) return;
+    const double px = P.px[ip], py = P.py[ip], pz = P.pz[ip];
+    const double g = sqrt(1 + px * px + py * py + pz * pz);
+    const double vx = px / g, vy = py / g, vz = pz / g;
+    const double f = P.q[ip] * P.w[ip] / (c.hcg[m + 2] * c.dx);
+    int ix[3], jn[3], jh[3]; double wx[3], wn[3], wh[3];
+    tsc_periodic(P.x[ip], c.dx, c.nx, ix, wx);
+    double* J0 = c.J + long(0L * c.nh + hw) * c.nk;
+    double* J1 = c.J + long(1L * c.nh + hw) * c.nk;
+    double* J2 = c.J + long(2L * c.nh + hw) * c.nk;
+    if (c.yf) {
+        tsc_periodic(P.y[ip], c.dy, c.ny, jn, wn);
+        for (int v = 0; v < 3; ++v)
+            for (int u = 0; u < 3; ++u) {
+                const long k = long(jn[v]) * c.nx + ix[u];
+                const double w = f * wx[u] * wn[v] * c.ia[0];
+                atomicAdd(J0 + k, w * vx / vz);
+                atomicAdd(J1 + k, w * vy / vz);
+                atomicAdd(J2 + k, w);
+            }
+        return;
+    }
+    tsc_freehand(c.yax, P.y[ip], jn, wn, jh, wh);
+    for (int v = 0; v < 3; ++v) {
+        const bool nod = jn[v] >= 1 && jn[v] <= c.ny - 1, hal = jh[v] >= 0 && jh[v] <= c.ny - 1;
+        for (int u = 0; u < 3; ++u) {
+            if (nod) {
+                const long k = long(jn[v]) * c.nx + ix[u];
+                const double w = f * wx[u] * wn[v] * c.ia[jn[v]];
+                atomicAdd(J0 + k, w * vx / vz);
+                atomicAdd(J2 + k, w);
+            }
+            if (hal) {
+                const long k = long(jh[v]) * c.nx + ix[u];
+                atomicAdd(J1 + k, f * wx[u] * wh[v] * c.id[jh[v]] * vy / vz);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- streaming particles (src/sim_stream.hpp)
+// LDS window of a tile: 5 planes of WX x WY nodes from node (x0, y0) (periodic); null for untiled particles
+struct Win {
+    double* lds;
+    int x0, y0, WX, WY;
+};
+
+// TSC deposit of f (dt/Delta) v/(h dx dy) (src/sim_stream.hpp:33-64) into plane comp of the half-cell current:
+// the tile's LDS window when the whole 3 x 3 stencil lies inside it, else jsr with global atomics.  Zero
+// contributions are skipped (adding +-0 to a sum that starts at +0 never changes it).
+__device__ __forceinline__ void sdeposit(const StreamArgs& c, const Win* win, double x, double y, double f, double vx,
+                                         double vy, double vz) {
+    const FieldView& P = c.fv;
+    int ix[3], jn[3], jh[3]; double wx[3], wn[3], wh[3];
+    tsc_periodic(x, P.dx, P.nx, ix, wx);
+    const long nk = P.nk;
+    const int cq = f < 0 ? 3 : 4;
+    const bool mv = vx != 0 || vy != 0 || vz != 0;
+    if (P.yf) {
+        tsc_periodic(y, P.dy, P.ny, jn, wn);
+        double w[9];
+#pragma unroll
+        for (int v = 0; v < 3; ++v)
+#pragma unroll
+            for (int u = 0; u < 3; ++u) w[3 * v + u] = f * wx[u] * wn[v] * c.ia[0];
+        int u0 = 0, v0 = 0;
+        bool in = false;
+        if (win) {
+            u0 = ix[0] - win->x0; u0 = ((u0 % P.nx) + P.nx) % P.nx;
+            v0 = jn[0] - win->y0; v0 = ((v0 % P.ny) + P.ny) % P.ny;
+            in = u0 + 2 < win->WX && v0 + 2 < win->WY;
+        }
+        if (in) {
+            const long wpl = long(win->WX) * win->WY;
+#pragma unroll
+            for (int v = 0; v < 3; ++v)
+#pragma unroll
+                for (int u = 0; u < 3; ++u) {
+                    double* L = win->lds + long(v0 + v) * win->WX + (u0 + u);
+                    const double ww = w[3 * v + u];
+                    if (mv) {
+                        atomicAdd(L, ww * vx);
+                        atomicAdd(L + wpl, ww * vy);
+                        atomicAdd(L + 2 * wpl, ww * vz);
+                    }
+                    atomicAdd(L + cq * wpl, ww);
+                }
+            return;
+        }
+#pragma unroll
+        for (int v = 0; v < 3; ++v)
+#pragma unroll
+            for (int u = 0; u < 3; ++u) {
+                const long k = long(jn[v]) * P.nx + ix[u];
+                const double ww = w[3 * v + u];
+                if (mv) {
+                    atomicAdd(&c.J[k], ww * vx);
+                    atomicAdd(&c.J[nk + k], ww * vy);
+                    atomicAdd(&c.J[2 * nk + k], ww * vz);
+                }
+                atomicAdd(&c.J[cq * nk + k], ww);
+            }
+        return;
+    }
+    tsc_freehand(P.yax, y, jn, wn, jh, wh);
+    for (int v = 0; v < 3; ++v) {
+        const bool nod = jn[v] >= 1 && jn[v] <= P.ny - 1, hal = jh[v] >= 0 && jh[v] <= P.ny - 1;
+        for (int u = 0; u < 3; ++u) {
+            if (nod) {
+                const long k = long(jn[v]) * P.nx + ix[u];
+                const double w = f * wx[u] * wn[v] * c.ia[jn[v]];
+                if (mv) { atomicAdd(&c.J[k], w * vx); atomicAdd(&c.J[2 * nk + k], w * vz); }
+                atomicAdd(&c.J[cq * nk + k], w);
+            }
+            if (hal && mv) {
+                const long k = long(jh[v]) * P.nx + ix[u];
+                atomicAdd(&c.J[nk + k], f * wx[u] * wh[v] * c.id[jh[v]] * vy);
+            }
+        }
+    }
+}
+
+// |E| of the grid and the analytic laser (src/sim_stream.hpp:67-74)
+__device__ __forceinline__ double field_abs(const StreamArgs& c, double x, double y, double z, double zeta) {
+    double E[3], B[3], Ee[3], Be[3];
+    gather(c.fv, wrap_mod(x, c.fv.Lx), c.fv.yf ? wrap_mod(y, c.fv.Ly) : y, zeta, E, B);
+    c.laser.field(x + c.X0, z, z - (zeta - c.zc), Ee, Be);
+    for (int k = 0; k < 3; ++k) E[k] += Ee[k];
+    return sqrt(E[0] * E[0] + E[1] * E[1] + E[2] * E[2]);
+}
+
+struct PState {
+    double x, y, zeta, px, py, pz, w, z, q, qm;
+    int alive, sp;
+    unsigned long long id;
+};
+
+// Monte Carlo ADK (src/sim_stream.hpp:80-101): sequential levels with exponential waiting times; every event
+// clones the ion as an electron into the born list
+__device__ __forceinline__ void ionize(const StreamArgs& c, PState& p, double E1, double E0, double dt) {
+    const IonDev I = c.ions[p.sp - SP_ION];
+    if (!I.ionizing || (E1 < I.Emin && E0 < I.Emin)) return;
+    const uint64_t base = key_particle(c.key, p.id);
+    int Z = int(floor(p.q + 0.5));
+    const int Z0 = Z;
+    unsigned draw = 0;
+    double tleft = dt;
+    while (Z < I.Zmax) {
+        const AdkLevel L = c.adk[I.off + Z];
+        const double w = 0.5 * (adk_rate(L, E1) + adk_rate(L, E0));
+        if (!(w > 0)) break;
+        const double tw = -log(1.0 - uniform01(base + draw++)) / w;
+        if (tw > tleft) break;
+        tleft -= tw; ++Z;
+        const int j = atomicAdd(c.bcnt, 1);
+        if (j < c.bcap) {
+            const Parts& b = c.born;
+            b.x[j] = p.x; b.y[j] = p.y; b.zeta[j] = p.zeta; b.px[j] = p.px; b.py[j] = p.py; b.pz[j] = p.pz;
+            b.w[j] = p.w; b.z[j] = p.z; b.q[j] = -1; b.qm[j] = -1; b.rem[j] = 0;
+            b.alive[j] = 1; b.sp[j] = SP_FREED + (p.sp - SP_ION); b.id[j] = p.id;
+        }
+    }
+    if (Z != Z0) { p.q = Z; p.qm = Z / I.mass; }
+}
+
+// one particle through half-cell n (src/sim_stream.hpp:104-171)
+__device__ __forceinline__ void stream_one(const StreamArgs& c, const Win* win, PState& p) {
+    const int n = c.n;
+    const double zn = c.zs[n], zn1 = c.zs[n + 1], h = zn1 - zn;
+    if (p.alive == 6) {
+        if (p.zeta > zn) p.alive = 1;
+        else return;
+    }
+    if (p.alive != 1) return;
+    const double hl = p.zeta >= zn1 - 1e-9 * h ? h : (p.zeta - zn > 0 ? p.zeta - zn : 0.0);
+    const FieldView& P = c.fv;
+    const bool ion = is_ion(p.sp);
+    if (ion && !c.ions[p.sp - SP_ION].mobile) {
+        const double f = p.q * p.w * hl / (c.Delta * h * P.dx);
+        if (f != 0) sdeposit(c, win, p.x, p.y, f, 0, 0, 0);
+        const double E1 = field_abs(c, p.x, p.y, p.z, p.zeta), E0 = field_abs(c, p.x, p.y, p.z, zn);
+        p.zeta = zn;
+        ionize(c, p, E1, E0, hl);
+        return;
+    }
+    const double zeta0 = p.zeta;
+    const double g0 = sqrt(1 + p.px * p.px + p.py * p.py + p.pz * p.pz);
+    const double omv0 = (1 + p.px * p.px + p.py * p.py) / (g0 * (g0 + p.pz));
+    const double dt0 = hl / omv0;
+    const double xm = p.x + 0.5 * dt0 * p.px / g0, ym = p.y + 0.5 * dt0 * p.py / g0;
+    const double zm = p.z + 0.5 * dt0 * p.pz / g0, zetam = p.zeta - 0.5 * hl;
+    double E[3], B[3], Ee[3], Be[3];
+    gather(P, wrap_mod(xm, P.Lx), P.yf ? wrap_mod(ym, P.Ly) : ym, zetam, E, B);
+    c.laser.field(xm + c.X0, zm, zm - (zetam - c.zc), Ee, Be);
+    for (int k = 0; k < 3; ++k) { E[k] += Ee[k]; B[k] += Be[k]; }
+    double ux = p.px, uy = p.py, uz = p.pz;
+    boris(ux, uy, uz, E, B, p.qm, dt0);
+    double g1 = sqrt(1 + ux * ux + uy * uy + uz * uz);
+    double dt = 2 * hl / (omv0 + (1 + ux * ux + uy * uy) / (g1 * (g1 + uz)));
+    ux = p.px; uy = p.py; uz = p.pz;
+    boris(ux, uy, uz, E, B, p.qm, dt);
+    g1 = sqrt(1 + ux * ux + uy * uy + uz * uz);
+    double vx = 0.5 * (p.px / g0 + ux / g1), vy = 0.5 * (p.py / g0 + uy / g1), vz = 0.5 * (p.pz / g0 + uz / g1);
+    double zeta_end = zn;
+    bool trapped = false;
+    if (p.z + vz * dt >= c.sTrap) {
+        const double vzc = vz > 1e-3 ? vz : 1e-3;
+        const double dtp = (c.sTrap - p.z) / vzc > 0 ? (c.sTrap - p.z) / vzc : 0.0;
+        ux = p.px; uy = p.py; uz = p.pz;
+        boris(ux, uy, uz, E, B, p.qm, dtp);
+        g1 = sqrt(1 + ux * ux + uy * uy + uz * uz);
+        vx = 0.5 * (p.px / g0 + ux / g1); vy = 0.5 * (p.py / g0 + uy / g1); vz = 0.5 * (p.pz / g0 + uz / g1);
+        dt = dtp;
+        zeta_end = p.zeta - (1 - vz) * dtp;
+        trapped = true;
+    }
+    const double xo = p.x, yo = p.y;
+    p.x += vx * dt; p.y += vy * dt;
+    p.z = trapped ? c.s1 : p.z + vz * dt;
+    p.px = ux; p.py = uy; p.pz = uz;
+    p.zeta = zeta_end;
+    const double f = p.q * p.w * dt / (c.Delta * h * P.dx);
+    if (f != 0) sdeposit(c, win, 0.5 * (xo + p.x), 0.5 * (yo + p.y), f, vx, vy, vz);
+    if (!P.yf && (p.y < c.ylo || p.y > c.yhi)) { p.alive = 0; return; }
+    if (ion && !trapped && c.ions[p.sp - SP_ION].ionizing)
+        ionize(c, p, field_abs(c, xm, ym, zm, zeta0), field_abs(c, xm, ym, zm, zn), dt);
+    if (trapped) {
+        p.alive = 5;
+        const double vze = uz / g1;
+        if (vze >= c.vz_beam) sdeposit(c, win, p.x, p.y, p.q * p.w / (h * P.dx * vze), ux / g1, uy / g1, vze);
+    }
+}
+
+__device__ __forceinline__ void stream_index(const StreamArgs& c, const Win* win, Parts S, Parts D, long i) {
+    PState p{S.x[i], S.y[i], S.zeta[i], S.px[i], S.py[i], S.pz[i], S.w[i], S.z[i], S.q[i], S.qm[i], S.alive[i],
+             S.sp[i], S.id[i]};
+    stream_one(c, win, p);
+    D.x[i] = p.x; D.y[i] = p.y; D.zeta[i] = p.zeta; D.px[i] = p.px; D.py[i] = p.py; D.pz[i] = p.pz;
+    D.z[i] = p.z; D.q[i] = p.q; D.qm[i] = p.qm; D.alive[i] = p.alive;
+}
+
+// workgroups 0 .. ntile-1: the particles of one tile with the LDS window, written to the tile's slab at the end;
+// the following workgroups: the untiled particles from off[ntile] on, one per lane, global atomics
+__global__ void __launch_bounds__(kT) k_stream_halfcell(StreamArgs c, Parts S, Parts D, long n, TileGeom tg) {
+    extern __shared__ double win_lds[];
+    const long b = blockIdx.x;
+    if (b < tg.ntile) {
+        const long wn = 5L * tg.WX * tg.WY;
+        for (long k = threadIdx.x; k < wn; k += blockDim.x) win_lds[k] = 0;
+        __syncthreads();
+        const int tx = int(b % tg.ntx), ty = int(b / tg.ntx);
+        Win w{win_lds, tx * tg.TX - 1 - tg.M, ty * tg.TY - 1 - tg.M, tg.WX, tg.WY};
+        const long hi = tg.off[b + 1];
+        for (long i = tg.off[b] + threadIdx.x; i < hi; i += blockDim.x) stream_index(c, &w, S, D, i);
+        __syncthreads();
+        double* sl = tg.slab + b * wn;
+        for (long k = threadIdx.x; k < wn; k += blockDim.x) sl[k] = win_lds[k];
+        return;
+    }
+    const long i = tg.n_tiled + (b - tg.ntile) * blockDim.x + threadIdx.x;
+    if (i < n) stream_index(c, nullptr, S, D, i);
+}
+
+__global__ void k_append_born(Parts b, const int* bcnt, Parts p0, Parts p1, long base, long cap) {
+    const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long cnt = *bcnt;
+    if (i >= cnt || base + i >= cap) return;
+    const long d = base + i;
+    for (Parts* t : {&p0, &p1}) {
+        t->x[d] = b.x[i]; t->y[d] = b.y[i]; t->zeta[d] = b.zeta[i]; t->px[d] = b.px[i]; t->py[d] = b.py[i];
+        t->pz[d] = b.pz[i]; t->z[d] = b.z[i]; t->q[d] = b.q[i]; t->qm[d] = b.qm[i]; t->alive[d] = b.alive[i];
+    }
+    p0.w[d] = b.w[i]; p0.sp[d] = b.sp[i]; p0.id[d] = b.id[i]; p0.rem[d] = 0;
+}
+
+// ---------------------------------------------------------------- list tools
+__global__ void k_centroid_partial(Parts P, long np, double X0, double Y0, double* part) {
+    __shared__ double s[3][kT];
+    double a = 0, b = 0, w = 0;
+    for (long i = long(blockIdx.x) * kT + threadIdx.x; i < np; i += long(gridDim.x) * kT) {
+        if (P.alive[i] != 1 || !is_bunch(P.sp[i])) continue;
+        a += P.w[i] * (P.x[i] + X0); b += P.w[i] * (P.y[i] + Y0); w += P.w[i];
+    }
+    s[0][threadIdx.x] = a; s[1][threadIdx.x] = b; s[2][threadIdx.x] = w;
+    __syncthreads();
+    for (int o = kT / 2; o > 0; o >>= 1) {
+        if (int(threadIdx.x) < o)
+            for (int k = 0; k < 3; ++k) s[k][threadIdx.x] += s[k][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) for (int k = 0; k < 3; ++k) part[3 * blockIdx.x + k] = s[k][0];
+}
+__global__ void k_centroid_final(const double* part, int nb, double* out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double a = 0, b = 0, w = 0;
+    for (int k = 0; k < nb; ++k) { a += part[3 * k]; b += part[3 * k + 1]; w += part[3 * k + 2]; }
+    out[0] = a; out[1] = b; out[2] = w;
+}
+
+__device__ void to_aos(const Parts& p, long i, g3d_particle& o) {
+    o.x = p.x[i]; o.y = p.y[i]; o.zeta = p.zeta[i]; o.px = p.px[i]; o.py = p.py[i]; o.pz = p.pz[i]; o.w = p.w[i];
+    o.rem = p.rem[i]; o.z = p.z[i]; o.q = p.q[i]; o.qm = p.qm[i]; o.alive = p.alive[i]; o.sp = p.sp[i]; o.id = p.id[i];
+}
+__global__ void k_collect(Parts p, long n, int code, g3d_particle* out, long* idx, int* count, long cap) {
+    const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n || p.alive[i] != code) return;
+    const int j = atomicAdd(count, 1);
+    if (j >= cap) return;
+    to_aos(p, i, out[j]);
+    idx[j] = i;
+}
+__global__ void k_count(Parts p, long n, int code, int* count) {
+    const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool hit = i < n && p.alive[i] == code;
+    const unsigned long long m = __ballot(hit);
+    if (m && (threadIdx.x & 63) == 0) atomicAdd(count, int(__popcll(m)));
+}
+__global__ void k_from_aos(const g3d_particle* in, Parts p, long n) {
+    const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const g3d_particle o = in[i];
+    p.x[i] = o.x; p.y[i] = o.y; p.zeta[i] = o.zeta; p.px[i] = o.px; p.py[i] = o.py; p.pz[i] = o.pz; p.w[i] = o.w;
+    p.rem[i] = o.rem; p.z[i] = o.z; p.q[i] = o.q; p.qm[i] = o.qm; p.alive[i] = o.alive; p.sp[i] = o.sp; p.id[i] = o.id;
+}
+__global__ void k_to_aos(Parts p, g3d_particle* out, long n) {
+    const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    to_aos(p, i, out[i]);
+}
+__global__ void k_copy(Parts s, Parts d, long n) {
+    const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    d.x[i] = s.x[i]; d.y[i] = s.y[i]; d.zeta[i] = s.zeta[i]; d.px[i] = s.px[i]; d.py[i] = s.py[i]; d.pz[i] = s.pz[i];
+    d.z[i] = s.z[i]; d.q[i] = s.q[i]; d.qm[i] = s.qm[i]; d.alive[i] = s.alive[i];
+}
+__global__ void k_shift(Parts p, long n, double sx, double sy) {
+    const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    p.x[i] -= sx; p.y[i] -= sy;
+}
+}  // namespace
+
+void bunch_push(const PushArgs& a, Parts p, long np, double s0) {
+    if (np <= 0) return;
+    hipLaunchKernelGGL(k_bunch_push, dim3(blocks_for(np, kT)), dim3(kT), 0, stream(), a, p, np, s0);
+    launch_check("bunch push");
+}
+void bunch_deposit(const DepositArgs& a, Parts p, long np) {
+    if (np <= 0) return;
+    hipLaunchKernelGGL(k_bunch_deposit, dim3(blocks_for(np, kT)), dim3(kT), 0, stream(), a, p, np);
+    launch_check("bunch deposit");
+}
+void stream_halfcell(const StreamArgs& a, Parts src, Parts dst, long nstrm, const TileGeom& tg) {
+    if (nstrm <= 0) return;
+    const long tail_blocks = nstrm > tg.n_tiled ? long(blocks_for(nstrm - tg.n_tiled, kT)) : 0;
+    const size_t lds = tg.ntile ? size_t(5) * tg.WX * tg.WY * sizeof(double) : 0;
+    hipLaunchKernelGGL(k_stream_halfcell, dim3(unsigned(tg.ntile + tail_blocks)), dim3(kT), lds, stream(), a, src, dst,
+                       nstrm, tg);
+    launch_check("stream half-cell");
+}
+void append_born(Parts born, const int* bcnt, Parts p0, Parts p1, long base, long cap) {
+    if (cap <= base) return;
+    hipLaunchKernelGGL(k_append_born, dim3(blocks_for(cap - base, kT)), dim3(kT), 0, stream(), born, bcnt, p0, p1, base,
+                       cap);
+    launch_check("append born");
+}
+void bunch_centroid(Parts p, long np, double X0, double Y0, double* out3) {
+    static double* part = nullptr;
+    const int nb = 256;
+    if (!part) hip_check(hipMalloc(reinterpret_cast<void**>(&part), 3 * nb * sizeof(double)), "hipMalloc");
+    hipLaunchKernelGGL(k_centroid_partial, dim3(nb), dim3(kT), 0, stream(), p, np, X0, Y0, part);
+    hipLaunchKernelGGL(k_centroid_final, dim3(1), dim3(64), 0, stream(), part, nb, out3);
+    launch_check("centroid");
+}
+void collect_alive(Parts p, long n, int code, g3d_particle* out, long* idx, int* count, long cap) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_collect, dim3(blocks_for(n, kT)), dim3(kT), 0, stream(), p, n, code, out, idx, count, cap);
+    launch_check("collect");
+}
+void count_alive(Parts p, long n, int code, int* count) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_count, dim3(blocks_for(n, kT)), dim3(kT), 0, stream(), p, n, code, count);
+    launch_check("count");
+}
+void parts_from_aos(const g3d_particle* in, Parts p, long n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_from_aos, dim3(blocks_for(n, kT)), dim3(kT), 0, stream(), in, p, n);
+    launch_check("particles from AoS");
+}
+void parts_to_aos(Parts p, g3d_particle* out, long n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_to_aos, dim3(blocks_for(n, kT)), dim3(kT), 0, stream(), p, out, n);
+    launch_check("particles to AoS");
+}
+void parts_copy(Parts s, Parts d, long n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_copy, dim3(blocks_for(n, kT)), dim3(kT), 0, stream(), s, d, n);
+    launch_check("particles copy");
+}
+void parts_shift(Parts p, long n, double sx, double sy) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_shift, dim3(blocks_for(n, kT)), dim3(kT), 0, stream(), p, n, sx, sy);
+    launch_check("particles shift");
+}
+
+}  // namespace g3d

@@ -1,0 +1,289 @@
+"""Python view of the MI355X march library (libg3d.so) over its C ABI (include/g3d.h).
+
+The configuration helpers mirror the reference's Config grid helpers (src/sim_config.cpp:5-47:
+base_units, zeta_uniform, y_uniform) and its test defaults (tests/test_util.hpp:36-43), so that a run is set up
+here as the reference's tests set it up.  `Sim` mirrors the reference's Simulation: init(), step(), the member
+functions by name (call), and host copies of the device arrays in the reference's layouts.
+
+The library is loaded from this directory (lib/libg3d.so); if it is missing the import fails with an error -
+there is no CPU fallback.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libg3d.so")
+
+# SI constants of the reference's units (src/sim_internal.hpp:10)
+C_SI, E_SI, ME_SI, EPS0 = 299792458.0, 1.602176634e-19, 9.1093837015e-31, 8.8541878128e-12
+
+
+class Glaser(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("a0", "kL", "waist", "tau", "zeta_c", "z_focus", "cep", "pol_angle")] + \
+               [("circular", C.c_int), ("helicity", C.c_int)]
+
+
+class Ion(C.Structure):
+    _fields_ = [("mass", C.c_double), ("atoms", C.c_double), ("Z0", C.c_int), ("Zmax", C.c_int), ("mobile", C.c_int),
+                ("ionization", C.c_int), ("n_ip", C.c_int), ("ip_eV", C.c_double * 16)]
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("lambda0", C.c_double), ("nx", C.c_int), ("ny", C.c_int), ("nzeta", C.c_int), ("Lx", C.c_double),
+        ("Ly", C.c_double), ("y_fourier", C.c_int), ("zeta", C.POINTER(C.c_double)), ("ynode", C.POINTER(C.c_double)),
+        ("dz", C.c_double), ("z_end", C.c_double), ("dz_push", C.c_double), ("sponge", C.c_double),
+        ("sponge_y", C.c_double), ("sponge_rate", C.c_double), ("move_x", C.c_int), ("move_y", C.c_int),
+        ("window_lag", C.c_int), ("self_fields", C.c_int), ("zeta_filter", C.c_int), ("k_filter", C.c_int),
+        ("push_phase", C.c_double), ("laser_on", C.c_int), ("laser_pol", C.c_int), ("a0", C.c_double),
+        ("kL", C.c_double), ("theta", C.c_double), ("tau", C.c_double), ("cep", C.c_double),
+        ("start_fwhm", C.c_double), ("stop_fwhm", C.c_double), ("charge", C.c_double), ("gamma", C.c_double),
+        ("sigx", C.c_double), ("sigy", C.c_double), ("sigz", C.c_double), ("emitx", C.c_double),
+        ("emity", C.c_double), ("espread", C.c_double), ("zeta_center", C.c_double), ("nmacro", C.c_long),
+        ("seed", C.c_ulong), ("n_glasers", C.c_int), ("glasers", Glaser * 4), ("plasma_on", C.c_int),
+        ("n0", C.c_double), ("r_max", C.c_double), ("ppc_x", C.c_int), ("ppc_y", C.c_int),
+        ("smooth_length", C.c_double), ("n_pz", C.c_int), ("pz_points", C.c_double * 16),
+        ("pz_values", C.c_double * 16), ("parabolic", C.c_int), ("channel_depth", C.c_double),
+        ("r_channel", C.c_double), ("iterations", C.c_int), ("vz_min", C.c_double), ("vz_beam", C.c_double),
+        ("trap_advance", C.c_double), ("predictor_extrapolate", C.c_int), ("free_electrons", C.c_int),
+        ("n_ions", C.c_int), ("ions", Ion * 4), ("freeze_bunch", C.c_int), ("start_relax", C.c_int),
+        ("box_mean", C.c_int)]
+
+
+class SimInfo(C.Structure):
+    _fields_ = [(n, C.c_long) for n in ("N", "ns", "nk", "np", "nstrm", "istep", "born_cap")] + \
+               [(n, C.c_double) for n in ("s", "dx", "dy", "X0", "Y0", "beta_b", "gm2_b")] + \
+               [(n, C.c_long) for n in ("trapped_total", "restreamed_total", "lost_total")]
+
+
+# the 104-byte particle record (src/sim.hpp:217)
+PARTICLE = np.dtype([("x", "f8"), ("y", "f8"), ("zeta", "f8"), ("px", "f8"), ("py", "f8"), ("pz", "f8"), ("w", "f8"),
+                     ("rem", "f8"), ("z", "f8"), ("q", "f8"), ("qm", "f8"), ("alive", "i4"), ("sp", "i4"),
+                     ("id", "u8")])
+assert PARTICLE.itemsize == 104
+
+
+class Deck:
+    """A run configuration in code units with the reference's defaults (src/sim.hpp:36-187).
+
+    The grid tables live here (numpy arrays) so the C struct can borrow them."""
+
+    def __init__(self, lambda0=10e-6):
+        self.c = Config()
+        c = self.c
+        c.lambda0 = lambda0
+        c.dz_push = 0.0
+        c.sponge_rate = 0.1
+        c.move_x = 1; c.move_y = 1; c.window_lag = 4
+        c.self_fields = 1; c.zeta_filter = 1; c.k_filter = 1
+        c.push_phase = 0.05
+        c.kL = 1.0; c.tau = 1.0; c.start_fwhm = 3.5; c.stop_fwhm = 3.5
+        c.gamma = 1.0; c.seed = 1
+        c.r_max = 1e300; c.r_channel = 1.0
+        c.ppc_x = 2; c.ppc_y = 2; c.iterations = 2
+        c.vz_min = 0.8; c.vz_beam = 0.9; c.trap_advance = 1.0
+        c.predictor_extrapolate = 1; c.free_electrons = 1
+        c.start_relax = 1; c.box_mean = 1
+        self.zeta = None
+        self.ynode = None
+
+    # units (src/sim_config.cpp:5-11)
+    @property
+    def Lu(self):
+        return self.c.lambda0 / (2 * math.pi)
+
+    @property
+    def nc(self):
+        w0 = C_SI / self.Lu
+        return EPS0 * ME_SI * w0 * w0 / (E_SI * E_SI)
+
+    def zeta_uniform(self, n, Lw):                     # src/sim_config.cpp:14-20
+        h = Lw / (n - 1)
+        self.zeta = np.array([i * h for i in range(n)], dtype=np.float64)
+        self.c.nzeta = n
+        self.c.zeta = self.zeta.ctypes.data_as(C.POINTER(C.c_double))
+
+    def y_uniform(self, n, Ly):                        # src/sim_config.cpp:32-38
+        h = Ly / n
+        self.ynode = np.array([(j - n // 2) * h for j in range(n + 1)], dtype=np.float64)
+        self.c.ny = n; self.c.Ly = Ly; self.c.y_fourier = 1
+        self.c.ynode = self.ynode.ctypes.data_as(C.POINTER(C.c_double))
+
+    def grid(self, nx, ny, nz, Lx, Ly, Lw):
+        """Fourier x and y, nz uniform slices; no sponge, fixed window (tests/test_util.hpp:36-43)."""
+        self.c.nx = nx; self.c.Lx = Lx
+        self.y_uniform(ny, Ly)
+        self.zeta_uniform(nz, Lw)
+        self.c.sponge = self.c.sponge_y = 0.0
+        self.c.move_x = 0
+        return self
+
+    def add_grid_laser(self, a0, waist, tau, zeta_c, z_focus=0.0, kL=1.0, cep=0.0, pol_angle=0.0):
+        g = self.c.glasers[self.c.n_glasers]
+        g.a0 = a0; g.kL = kL; g.waist = waist; g.tau = tau; g.zeta_c = zeta_c; g.z_focus = z_focus; g.cep = cep
+        g.pol_angle = pol_angle; g.circular = 0; g.helicity = 1
+        self.c.n_glasers += 1
+        return self
+
+    def set(self, **kw):
+        for k, v in kw.items():
+            if not hasattr(self.c, k):
+                raise AttributeError("config has no field " + k)
+            setattr(self.c, k, v)
+        if "dz" in kw and "dz_push" not in kw and self.c.dz_push == 0:
+            self.c.dz_push = kw["dz"]
+        return self
+
+    def ptr(self):
+        if self.c.dz_push == 0:
+            self.c.dz_push = self.c.dz
+        return C.byref(self.c)
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libg3d.so is not built (%s): run __graft_entry__.build()" % LIB_PATH)
+    lib = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    sig = {
+        "g3d_init": ([C.c_int], C.c_int), "g3d_last_error": ([], C.c_char_p), "g3d_sync": ([], C.c_int),
+        "g3d_status_string": ([C.c_int], C.c_char_p),
+        "g3d_malloc": ([C.POINTER(P), C.c_size_t], C.c_int), "g3d_free": ([P], C.c_int),
+        "g3d_memset0": ([P, C.c_size_t], C.c_int),
+        "g3d_memcpy_h2d": ([P, P, C.c_size_t], C.c_int), "g3d_memcpy_d2h": ([P, P, C.c_size_t], C.c_int),
+        "g3d_memcpy_d2d": ([P, P, C.c_size_t], C.c_int),
+        "g3d_fft_plan": ([C.c_int, C.c_int, C.c_int, C.POINTER(P)], C.c_int),
+        "g3d_fft_exec": ([P, P, C.c_int, C.c_int], C.c_int), "g3d_fft_destroy": ([P], C.c_int),
+        "g3d_march": ([P, C.c_int, C.c_int], C.c_int), "g3d_march_slice": ([P, C.c_int, C.c_int], C.c_int),
+        "g3d_sim_create": ([P, C.POINTER(P)], C.c_int), "g3d_sim_destroy": ([P], C.c_int),
+        "g3d_sim_info_get": ([P, P], C.c_int), "g3d_sim_init": ([P], C.c_int), "g3d_sim_step": ([P], C.c_int),
+        "g3d_sim_call": ([P, C.c_char_p, C.POINTER(C.c_double), C.c_int], C.c_int),
+        "g3d_sim_array_size": ([P, C.c_char_p, C.POINTER(C.c_size_t)], C.c_int),
+        "g3d_sim_get": ([P, C.c_char_p, P, C.c_size_t], C.c_int),
+        "g3d_sim_set": ([P, C.c_char_p, P, C.c_size_t], C.c_int),
+        "g3d_sim_set_particles": ([P, P, C.c_long], C.c_int), "g3d_sim_set_joining": ([P, P, C.c_long], C.c_int),
+        "g3d_sim_get_joining": ([P, P, C.c_long, C.POINTER(C.c_long)], C.c_int),
+        "g3d_sim_timing": ([P, C.c_int], C.c_int),
+        "g3d_sim_timers": ([P, C.POINTER(C.c_double), C.POINTER(C.c_long), C.c_int, C.c_char_p, C.c_int], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = res
+    return lib
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = _load()
+        if _lib.g3d_init(0) != 0:
+            raise RuntimeError("g3d_init: " + _lib.g3d_last_error().decode())
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise RuntimeError(lib().g3d_last_error().decode() or lib().g3d_status_string(rc).decode())
+
+
+ARRAY_DTYPES = {"F": np.float32, "jw": np.float64, "jsr": np.float64, "rho_lines": np.float64,
+                "parts": PARTICLE, "strm": PARTICLE}
+
+
+class SimBase:
+    """Common front of the engine and of the reference harness (same method names)."""
+
+    def array(self, name):
+        n = self._size(name)
+        dt = ARRAY_DTYPES.get(name, np.complex128)
+        out = np.empty(n // np.dtype(dt).itemsize, dtype=dt)
+        self._get(name, out)
+        return out
+
+    def info(self):
+        i = SimInfo()
+        self._info(i)
+        return {f: getattr(i, f) for f, _ in SimInfo._fields_}
+
+
+class Sim(SimBase):
+    """The MI355X engine (g3d_sim)."""
+
+    def __init__(self, deck):
+        L = lib()
+        self._deck = deck
+        h = C.c_void_p()
+        check(L.g3d_sim_create(deck.ptr(), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            check(lib().g3d_sim_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init(self):
+        check(lib().g3d_sim_init(self.h))
+
+    def step(self):
+        check(lib().g3d_sim_step(self.h))
+
+    def call(self, op, *args):
+        a = (C.c_double * max(1, len(args)))(*args)
+        check(lib().g3d_sim_call(self.h, op.encode(), a, len(args)))
+
+    def _size(self, name):
+        b = C.c_size_t()
+        check(lib().g3d_sim_array_size(self.h, name.encode(), C.byref(b)))
+        return b.value
+
+    def _get(self, name, out):
+        check(lib().g3d_sim_get(self.h, name.encode(), out.ctypes.data_as(C.c_void_p), out.nbytes))
+
+    def set(self, name, arr):
+        arr = np.ascontiguousarray(arr)
+        check(lib().g3d_sim_set(self.h, name.encode(), arr.ctypes.data_as(C.c_void_p), arr.nbytes))
+
+    def set_particles(self, p):
+        p = np.ascontiguousarray(p, dtype=PARTICLE)
+        check(lib().g3d_sim_set_particles(self.h, p.ctypes.data_as(C.c_void_p), len(p)))
+
+    def set_joining(self, p):
+        p = np.ascontiguousarray(p, dtype=PARTICLE)
+        check(lib().g3d_sim_set_joining(self.h, p.ctypes.data_as(C.c_void_p), len(p)))
+
+    def joining(self, cap=1 << 22):
+        out = np.empty(cap, dtype=PARTICLE)
+        n = C.c_long()
+        check(lib().g3d_sim_get_joining(self.h, out.ctypes.data_as(C.c_void_p), cap, C.byref(n)))
+        return out[:n.value].copy()
+
+    def _info(self, i):
+        check(lib().g3d_sim_info_get(self.h, C.byref(i)))
+
+    def timing(self, on):
+        check(lib().g3d_sim_timing(self.h, int(on)))
+
+    def timers(self):
+        nf = 8
+        ms = (C.c_double * nf)()
+        cnt = (C.c_long * nf)()
+        names = C.create_string_buffer(nf * 32)
+        check(lib().g3d_sim_timers(self.h, ms, cnt, nf, names, 32))
+        out = {}
+        for k in range(nf):
+            nm = names.raw[k * 32:(k + 1) * 32].split(b"\0")[0].decode()
+            out[nm] = (ms[k], cnt[k])
+        return out

@@ -1,0 +1,173 @@
+/* g3d.h - C ABI of the MI355X (gfx950) per-slice march of GEM-PIC-3D.
+ *
+ * Every entry point returns 0 on success or a nonzero g3d status; g3d_last_error() holds the message
+ * ("HIP error in <what>: <msg>", or the logic error), worded as the reference's std::runtime_error texts
+ * (src/portable.hpp:43-49, src/sim_plasma.cpp:358).  Pointers passed to kernel entry points are device
+ * pointers (g3d_malloc) in the reference's layouts:
+ *   - spectra: planes of nk = nx*ny complex values, index j*nx + ix, cpx = {re, im}   (src/sim.hpp:255,268-270)
+ *   - real fields F: float [6][ns+2][nk], components Ex Ey Ez Bx By Bz              (src/sim.hpp:273)
+ *   - beam deposit jw: double [3][ns+3][nk]; streaming deposit jsr: double [5][nk]   (src/sim.hpp:272,297)
+ *   - particles on the device: structure of arrays (g3d_parts); the 104-byte AoS record g3d_particle is the
+ *     host-side view (src/sim.hpp:217).
+ * All work is enqueued on one HIP stream per process; g3d_sync() waits for it.  Calls that read device data
+ * back to the host synchronise first, as the reference does before every host read.
+ */
+#ifndef G3D_H
+#define G3D_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ types */
+typedef struct { double re, im; } g3d_cpx;                     /* src/portable.hpp:25 (cufftDoubleComplex layout) */
+
+/* one particle (host view), field for field the reference's Particle (src/sim.hpp:217), 104 bytes */
+typedef struct {
+    double x, y, zeta, px, py, pz, w, rem, z, q, qm;
+    int alive, sp;
+    unsigned long long id;
+} g3d_particle;
+
+/* device particle list, structure of arrays (one entry per particle; n used, cap allocated) */
+typedef struct {
+    double *x, *y, *zeta, *px, *py, *pz, *w, *z, *q, *qm, *rem;
+    int *alive, *sp;
+    unsigned long long* id;
+} g3d_parts;
+
+/* grid laser (laser block with type = gaussian), src/sim.hpp:95-101 */
+typedef struct {
+    double a0, kL, waist, tau, zeta_c, z_focus, cep, pol_angle;
+    int circular, helicity;
+} g3d_glaser;
+
+/* ion species (ion blocks), src/sim.hpp:118-127 */
+typedef struct {
+    double mass, atoms;
+    int Z0, Zmax, mobile, ionization, n_ip;
+    double ip_eV[16];
+} g3d_ion;
+
+/* run configuration: the fields of the reference's Config (src/sim.hpp:36-187) that the march uses, in code
+ * units (lengths 1/k0, times 1/omega0, densities n_c).  Tables are borrowed for the duration of g3d_sim_create. */
+typedef struct {
+    double lambda0;                               /* reference wavelength [m] (units only) */
+    int nx, ny, nzeta;                            /* ny: Fourier modes, or cells between the walls (free-hand) */
+    double Lx, Ly;
+    int y_fourier;
+    const double* zeta;                           /* nzeta slice positions, zeta[0] = 0 */
+    const double* ynode;                          /* ny + 1 y nodes */
+    double dz, z_end, dz_push;
+    double sponge, sponge_y, sponge_rate;
+    int move_x, move_y, window_lag;
+    int self_fields, zeta_filter, k_filter;
+    double push_phase;
+    /* analytic plane-wave laser (src/sim.hpp:191-210) */
+    int laser_on, laser_pol;                      /* pol: 0 s, 1 p, 2 circular */
+    double a0, kL, theta, tau, cep, start_fwhm, stop_fwhm;
+    /* the bunch (first bunch block) */
+    double charge, gamma, sigx, sigy, sigz, emitx, emity, espread, zeta_center;
+    long nmacro;
+    unsigned long seed;
+    /* grid lasers */
+    int n_glasers;
+    g3d_glaser glasers[4];
+    /* plasma */
+    int plasma_on;
+    double n0, r_max;
+    int ppc_x, ppc_y;
+    double smooth_length;
+    int n_pz;
+    double pz_points[16], pz_values[16];
+    int parabolic;
+    double channel_depth, r_channel;
+    int iterations;
+    double vz_min, vz_beam, trap_advance;
+    int predictor_extrapolate, free_electrons;
+    int n_ions;
+    g3d_ion ions[4];
+    int freeze_bunch, start_relax, box_mean;
+} g3d_config;
+
+/* context of the per-mode march (src/sim_kernels.hpp:161-167 MarchCtx), device pointers */
+typedef struct {
+    g3d_cpx *Rx, *Ry, *Ez, *Bz, *Lx, *Ly;         /* ns planes each */
+    const g3d_cpx *bnd;                           /* 6 planes: slice b from above (unused on one GPU: b = N) */
+    const g3d_cpx *j0x, *j0y, *jkx, *jky, *jkz;   /* ns + 1 half-cell planes each */
+    g3d_cpx *bkRx, *bkRy, *bkEz, *bkBz;           /* backup of slice n (march_slice with passes), may be null */
+    const double *kx, *ky, *iD, *iA, *hcg, *wlo, *whi;
+    double D;
+    int a, b, N, nx, ny;
+    long nk;
+} g3d_march_ctx;
+
+/* ------------------------------------------------------------------ runtime and memory (src/portable.hpp) */
+int g3d_init(int device);
+const char* g3d_last_error(void);
+const char* g3d_status_string(int code);
+int g3d_sync(void);
+int g3d_malloc(void** p, size_t bytes);           /* zero-filled, as Buf<T>::resize (portable.hpp:113-122) */
+int g3d_free(void* p);
+int g3d_memset0(void* p, size_t bytes);
+int g3d_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int g3d_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int g3d_memcpy_d2d(void* dst, const void* src, size_t bytes);
+
+/* ------------------------------------------------------------------ batched FFT (src/fftmany.hpp:26-40) */
+typedef struct g3d_fft g3d_fft;
+/* FftMany(nx, ny, two_d); nx, ny powers of two in 2 ... 1024 */
+int g3d_fft_plan(int nx, int ny, int two_d, g3d_fft** plan);
+/* FftMany::run: in place over count planes; forward sign -1, inverse scaled by 1/(nx ny) (2D) or 1/nx (rows) */
+int g3d_fft_exec(g3d_fft* plan, g3d_cpx* data, int count, int inverse);
+int g3d_fft_destroy(g3d_fft* plan);
+
+/* ------------------------------------------------------------------ march (src/sim_kernels.hpp:173-313) */
+/* Simulation::march (src/sim_fields.cpp:7-39): every mode (Fourier y) or k_x line (free-hand y) from slice
+ * hi-1 down to lo */
+int g3d_march(const g3d_march_ctx* c, int lo, int hi);
+/* Simulation::march_slice (src/sim_plasma.cpp:189-197): one slice n.  With the bk* pointers set, restore=0
+ * copies slice n of R_x, R_y, E_z, B_z into them before the update and restore=1 reads slice n from them
+ * (the two-pass backup of src/sim_plasma.cpp:306-326 without the extra copy kernels). */
+int g3d_march_slice(const g3d_march_ctx* c, int n, int restore);
+
+/* ------------------------------------------------------------------ engine: the Simulation of the march
+ * g3d_sim mirrors the reference's Simulation (src/sim.hpp:222-360) on one GPU: init() and step() and the
+ * member functions of the five drop-in files, each over the kernels of this library. */
+typedef struct g3d_sim g3d_sim;
+typedef struct {
+    long N, ns, nk, np, nstrm, istep, born_cap;
+    double s, dx, dy, X0, Y0, beta_b, gm2_b;
+    long trapped_total, restreamed_total, lost_total;
+} g3d_sim_info;
+
+int g3d_sim_create(const g3d_config* cfg, g3d_sim** out);
+int g3d_sim_destroy(g3d_sim* sim);
+int g3d_sim_info_get(g3d_sim* sim, g3d_sim_info* info);
+/* Simulation::init (src/sim_setup.cpp:178-197) and Simulation::step (199-217) */
+int g3d_sim_init(g3d_sim* sim);
+int g3d_sim_step(g3d_sim* sim);
+/* a member function by name with numeric arguments:
+ *   "march" D | "currents" | "sponge_and_real_fields" | "solve_fields" D | "stream_spectra" |
+ *   "march_slice" n D | "real_fields_slice" n | "push" s0 s1 | "march_plasma" s0 s1 | "comoving_field" |
+ *   "relax_start_field" | "add_grid_lasers" | "load_plasma" s0 s1 | "j0_from_jk" */
+int g3d_sim_call(g3d_sim* sim, const char* op, const double* args, int nargs);
+/* host copies of the device arrays by name (reference layouts): Rx Ry Ez Bz Lx Ly j0x j0y jkx jky jkz js0x
+ * js0y js0z jw F jsr jsn rho_lines; "parts" and "strm" as g3d_particle records.  bytes must match the size. */
+int g3d_sim_array_size(g3d_sim* sim, const char* name, size_t* bytes);
+int g3d_sim_get(g3d_sim* sim, const char* name, void* host, size_t bytes);
+int g3d_sim_set(g3d_sim* sim, const char* name, const void* host, size_t bytes);
+/* set the beam list (Simulation::set_particles, src/sim_setup.cpp:171-175) / the joining list */
+int g3d_sim_set_particles(g3d_sim* sim, const g3d_particle* p, long n);
+int g3d_sim_set_joining(g3d_sim* sim, const g3d_particle* p, long n);
+int g3d_sim_get_joining(g3d_sim* sim, g3d_particle* p, long cap, long* n);
+/* enable (1) or disable (0) HIP event timing of the kernel families; read the accumulated milliseconds */
+int g3d_sim_timing(g3d_sim* sim, int enable);
+int g3d_sim_timers(g3d_sim* sim, double* ms, long* counts, int nfam, char* names, int name_len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

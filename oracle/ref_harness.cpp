@@ -1,0 +1,266 @@
+// Test-only harness around the reference's own CPU build (oracle/_ref): builds a reference Config from the
+// g3d_config POD, owns a reference Simulation and exposes its member functions and arrays over a C ABI, so that
+// the tests can run the reference and the MI355X library on the same inputs.  Compiled against the sources under
+// /root/reference (never copied); only tests/, smoke() and bench.py's CPU baseline load the result.
+#include "sim.hpp"
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+#include <omp.h>
+#include "../include/g3d.h"
+
+static_assert(sizeof(Particle) == sizeof(g3d_particle), "particle record");
+static_assert(sizeof(cpx) == sizeof(g3d_cpx), "cpx");
+
+namespace {
+thread_local std::string err_;
+Comm comm_;
+
+struct Ref {
+    Config cfg;
+    std::unique_ptr<Simulation> sim;
+};
+
+int fail(const std::exception& e) { err_ = e.what(); return 1; }
+
+Config make_config(const g3d_config& g) {
+    Config c = Config::base_units(g.lambda0);
+    c.nx = g.nx; c.Lx = g.Lx;
+    c.ny = g.ny; c.Ly = g.Ly; c.y_fourier = g.y_fourier != 0;
+    c.nzeta = g.nzeta;
+    c.zeta.assign(g.zeta, g.zeta + g.nzeta);
+    c.ynode.assign(g.ynode, g.ynode + g.ny + 1);
+    c.Lwin = c.zeta.back();
+    c.dz = g.dz; c.z_end = g.z_end; c.dz_push = g.dz_push;
+    c.sponge = g.sponge; c.sponge_y = g.sponge_y; c.sponge_rate = g.sponge_rate;
+    c.move_x = g.move_x; c.move_y = g.move_y; c.window_lag = g.window_lag;
+    c.self_fields = g.self_fields; c.zeta_filter = g.zeta_filter; c.k_filter = g.k_filter;
+    c.push_phase = g.push_phase;
+    c.laser_on = g.laser_on; c.a0 = g.a0; c.kL = g.kL; c.theta = g.theta; c.tau = g.tau; c.cep = g.cep;
+    c.pol = g.laser_pol == 0 ? "s" : g.laser_pol == 1 ? "p" : "circular";
+    c.start_fwhm = g.start_fwhm; c.stop_fwhm = g.stop_fwhm;
+    c.charge = g.charge; c.gamma = g.gamma; c.sigx = g.sigx; c.sigy = g.sigy; c.sigz = g.sigz;
+    c.emitx = g.emitx; c.emity = g.emity; c.espread = g.espread; c.zeta_center = g.zeta_center;
+    c.nmacro = g.nmacro; c.seed = g.seed;
+    for (int k = 0; k < g.n_glasers; ++k) {
+        Config::GridLaser L;
+        const g3d_glaser& s = g.glasers[k];
+        L.a0 = s.a0; L.kL = s.kL; L.waist = s.waist; L.tau = s.tau; L.zeta_c = s.zeta_c; L.z_focus = s.z_focus;
+        L.cep = s.cep; L.pol_angle = s.pol_angle; L.circular = s.circular; L.helicity = s.helicity;
+        c.glasers.push_back(L);
+    }
+    c.plasma_on = g.plasma_on; c.n0 = g.n0; c.r_max = g.r_max; c.ppc_x = g.ppc_x; c.ppc_y = g.ppc_y;
+    c.smooth_length = g.smooth_length;
+    c.pz_points.assign(g.pz_points, g.pz_points + g.n_pz);
+    c.pz_values.assign(g.pz_values, g.pz_values + g.n_pz);
+    c.parabolic = g.parabolic; c.channel_depth = g.channel_depth; c.r_channel = g.r_channel;
+    c.iterations = g.iterations; c.vz_min = g.vz_min; c.vz_beam = g.vz_beam; c.trap_advance = g.trap_advance;
+    c.predictor_extrapolate = g.predictor_extrapolate; c.free_electrons = g.free_electrons;
+    for (int k = 0; k < g.n_ions; ++k) {
+        Config::IonSpecies s;
+        const g3d_ion& i = g.ions[k];
+        s.name = "ion" + std::to_string(k + 1);
+        s.mass = i.mass; s.atoms = i.atoms; s.Z0 = i.Z0; s.Zmax = i.Zmax; s.mobile = i.mobile; s.ionization = i.ionization;
+        s.ip_eV.assign(i.ip_eV, i.ip_eV + i.n_ip);
+        c.ions.push_back(s);
+    }
+    c.freeze_bunch = g.freeze_bunch; c.start_relax = g.start_relax; c.box_mean = g.box_mean;
+    return c;
+}
+
+template <class T> int copy_out(const Buf<T>& b, void* out, size_t bytes) {
+    if (bytes != b.size() * sizeof(T)) { err_ = "size mismatch"; return 2; }
+    b.to_host(static_cast<T*>(out), 0, b.size());
+    return 0;
+}
+template <class T> int copy_in(Buf<T>& b, const void* in, size_t bytes) {
+    if (bytes != b.size() * sizeof(T)) { err_ = "size mismatch"; return 2; }
+    b.from_host(static_cast<const T*>(in), 0, b.size());
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+const char* ref_last_error() { return err_.c_str(); }
+void ref_set_threads(int n) { omp_set_num_threads(n); }
+
+void* ref_sim_create(const g3d_config* g) {
+    try {
+        auto* r = new Ref;
+        r->cfg = make_config(*g);
+        r->sim.reset(new Simulation(r->cfg, comm_));
+        return r;
+    } catch (const std::exception& e) { err_ = e.what(); return nullptr; }
+}
+void ref_sim_destroy(void* h) { delete static_cast<Ref*>(h); }
+
+int ref_sim_info(void* h, g3d_sim_info* o) {
+    Simulation& s = *static_cast<Ref*>(h)->sim;
+    o->N = s.N; o->ns = s.ns; o->nk = s.nk; o->np = s.np; o->nstrm = s.nstrm; o->istep = s.istep;
+    o->born_cap = s.born_cap; o->s = s.s; o->dx = s.dx; o->dy = s.dy; o->X0 = s.X0; o->Y0 = s.Y0;
+    o->beta_b = s.beta_b; o->gm2_b = s.gm2_b; o->trapped_total = s.trapped_total;
+    o->restreamed_total = s.restreamed_total; o->lost_total = s.lost_total;
+    return 0;
+}
+
+int ref_sim_call(void* h, const char* op_, const double* a, int na) {
+    try {
+        Simulation& s = *static_cast<Ref*>(h)->sim;
+        const std::string op(op_);
+        auto need = [&](int k) { if (na < k) throw std::runtime_error(op + ": needs " + std::to_string(k) + " arguments"); };
+        if (op == "init") s.init();
+        else if (op == "step") s.step();
+        else if (op == "march") { need(1); s.march(a[0]); }
+        else if (op == "currents") s.currents();
+        else if (op == "sponge_and_real_fields") s.sponge_and_real_fields();
+        else if (op == "solve_fields") { need(1); s.solve_fields(a[0]); }
+        else if (op == "stream_spectra") s.stream_spectra();
+        else if (op == "march_slice") { need(2); s.march_slice(int(a[0]), a[1]); }
+        else if (op == "real_fields_slice") { need(1); s.real_fields_slice(int(a[0])); }
+        else if (op == "push") { need(2); s.push(a[0], a[1], 0, s.np); }
+        else if (op == "march_plasma") { need(2); s.march_plasma(a[0], a[1]); }
+        else if (op == "comoving_field") s.comoving_field();
+        else if (op == "relax_start_field") s.relax_start_field();
+        else if (op == "add_grid_lasers") s.add_grid_lasers();
+        else if (op == "load_plasma") { need(2); s.load_plasma(a[0], a[1], s.joining_, nullptr); }
+        else if (op == "set_s") { need(2); s.s = a[0]; s.istep = long(a[1]); }
+        else if (op == "set_starting") { need(1); s.starting_ = a[0] != 0; }
+        else if (op == "j0_from_jk") {
+            const long n = long(s.ns + 1) * s.nk;
+            std::vector<cpx> t(n);
+            s.jkx.to_host(t.data(), 0, n); s.j0x.from_host(t.data(), 0, n);
+            s.jky.to_host(t.data(), 0, n); s.j0y.from_host(t.data(), 0, n);
+        } else { err_ = "unknown op " + op; return 3; }
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
+}
+
+int ref_sim_array_size(void* h, const char* name_, size_t* bytes) {
+    Simulation& s = *static_cast<Ref*>(h)->sim;
+    const std::string n(name_);
+    const std::map<std::string, size_t> m = {
+        {"Rx", s.Rx.size() * 16}, {"Ry", s.Ry.size() * 16}, {"Ez", s.Ez.size() * 16}, {"Bz", s.Bz.size() * 16},
+        {"Lx", s.Lx.size() * 16}, {"Ly", s.Ly.size() * 16}, {"j0x", s.j0x.size() * 16}, {"j0y", s.j0y.size() * 16},
+        {"jkx", s.jkx.size() * 16}, {"jky", s.jky.size() * 16}, {"jkz", s.jkz.size() * 16},
+        {"js0x", s.js0x.size() * 16}, {"js0y", s.js0y.size() * 16}, {"js0z", s.js0z.size() * 16},
+        {"jw", s.jw.size() * 8}, {"F", s.F.size() * 4}, {"jsr", s.jsr.size() * 8}, {"jsn", s.jsn.size() * 16},
+        {"rho_lines", s.rho_lines.size() * 8},
+        {"parts", size_t(s.np) * sizeof(Particle)}, {"strm", size_t(s.nstrm) * sizeof(Particle)}};
+    auto it = m.find(n);
+    if (it == m.end()) { err_ = "unknown array " + n; return 3; }
+    *bytes = it->second;
+    return 0;
+}
+
+int ref_sim_get(void* h, const char* name_, void* out, size_t bytes) {
+    try {
+        Simulation& s = *static_cast<Ref*>(h)->sim;
+        const std::string n(name_);
+        if (n == "Rx") return copy_out(s.Rx, out, bytes);
+        if (n == "Ry") return copy_out(s.Ry, out, bytes);
+        if (n == "Ez") return copy_out(s.Ez, out, bytes);
+        if (n == "Bz") return copy_out(s.Bz, out, bytes);
+        if (n == "Lx") return copy_out(s.Lx, out, bytes);
+        if (n == "Ly") return copy_out(s.Ly, out, bytes);
+        if (n == "j0x") return copy_out(s.j0x, out, bytes);
+        if (n == "j0y") return copy_out(s.j0y, out, bytes);
+        if (n == "jkx") return copy_out(s.jkx, out, bytes);
+        if (n == "jky") return copy_out(s.jky, out, bytes);
+        if (n == "jkz") return copy_out(s.jkz, out, bytes);
+        if (n == "js0x") return copy_out(s.js0x, out, bytes);
+        if (n == "js0y") return copy_out(s.js0y, out, bytes);
+        if (n == "js0z") return copy_out(s.js0z, out, bytes);
+        if (n == "jw") return copy_out(s.jw, out, bytes);
+        if (n == "F") return copy_out(s.F, out, bytes);
+        if (n == "jsr") return copy_out(s.jsr, out, bytes);
+        if (n == "jsn") return copy_out(s.jsn, out, bytes);
+        if (n == "rho_lines") return copy_out(s.rho_lines, out, bytes);
+        if (n == "parts") {
+            if (bytes != size_t(s.np) * sizeof(Particle)) { err_ = "size mismatch"; return 2; }
+            s.parts.to_host(static_cast<Particle*>(out), 0, size_t(s.np));
+            return 0;
+        }
+        if (n == "strm") {
+            if (bytes != size_t(s.nstrm) * sizeof(Particle)) { err_ = "size mismatch"; return 2; }
+            s.strm.to_host(static_cast<Particle*>(out), 0, size_t(s.nstrm));
+            return 0;
+        }
+        err_ = "unknown array " + n;
+        return 3;
+    } catch (const std::exception& e) { return fail(e); }
+}
+
+int ref_sim_set(void* h, const char* name_, const void* in, size_t bytes) {
+    try {
+        Simulation& s = *static_cast<Ref*>(h)->sim;
+        const std::string n(name_);
+        if (n == "Rx") return copy_in(s.Rx, in, bytes);
+        if (n == "Ry") return copy_in(s.Ry, in, bytes);
+        if (n == "Ez") return copy_in(s.Ez, in, bytes);
+        if (n == "Bz") return copy_in(s.Bz, in, bytes);
+        if (n == "Lx") return copy_in(s.Lx, in, bytes);
+        if (n == "Ly") return copy_in(s.Ly, in, bytes);
+        if (n == "j0x") return copy_in(s.j0x, in, bytes);
+        if (n == "j0y") return copy_in(s.j0y, in, bytes);
+        if (n == "jkx") return copy_in(s.jkx, in, bytes);
+        if (n == "jky") return copy_in(s.jky, in, bytes);
+        if (n == "jkz") return copy_in(s.jkz, in, bytes);
+        if (n == "js0x") return copy_in(s.js0x, in, bytes);
+        if (n == "js0y") return copy_in(s.js0y, in, bytes);
+        if (n == "js0z") return copy_in(s.js0z, in, bytes);
+        if (n == "jw") return copy_in(s.jw, in, bytes);
+        if (n == "F") return copy_in(s.F, in, bytes);
+        if (n == "jsr") return copy_in(s.jsr, in, bytes);
+        err_ = "unknown array " + n;
+        return 3;
+    } catch (const std::exception& e) { return fail(e); }
+}
+
+int ref_sim_set_particles(void* h, const g3d_particle* p, long n) {
+    try {
+        Simulation& s = *static_cast<Ref*>(h)->sim;
+        std::vector<Particle> v(reinterpret_cast<const Particle*>(p), reinterpret_cast<const Particle*>(p) + n);
+        s.set_particles(v);
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
+}
+int ref_sim_set_joining(void* h, const g3d_particle* p, long n) {
+    Simulation& s = *static_cast<Ref*>(h)->sim;
+    s.joining_.assign(reinterpret_cast<const Particle*>(p), reinterpret_cast<const Particle*>(p) + n);
+    return 0;
+}
+int ref_sim_get_joining(void* h, g3d_particle* p, long cap, long* n) {
+    Simulation& s = *static_cast<Ref*>(h)->sim;
+    *n = long(s.joining_.size());
+    if (*n > cap) { err_ = "joining list larger than the buffer"; return 2; }
+    std::memcpy(p, s.joining_.data(), s.joining_.size() * sizeof(Particle));
+    return 0;
+}
+
+// the reference's own batched FFT (src/fftmany.hpp) on host data
+int ref_fft_many(int nx, int ny, int two_d, g3d_cpx* data, int count, int inverse) {
+    try {
+        FftMany f(nx, ny, two_d != 0);
+        f.run(reinterpret_cast<cpx*>(data), count, inverse != 0);
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
+}
+
+// ADK table of one species (src/ionization.cpp:76-99): 5 doubles per level Z0 ... Zmax-1
+int ref_adk_levels(const double* ip_eV, int n_ip, int Z0, int Zmax, double E0, double omega0, double T, double* out) {
+    try {
+        std::vector<double> ip(ip_eV, ip_eV + n_ip);
+        const std::vector<AdkLevel> L = adk_levels(ip, Z0, Zmax, E0, omega0, T);
+        for (int k = 0; k < Zmax; ++k) {
+            out[5 * k + 0] = L[k].logpre; out[5 * k + 1] = L[k].power; out[5 * k + 2] = L[k].expo;
+            out[5 * k + 3] = L[k].Epeak; out[5 * k + 4] = L[k].Emin;
+        }
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
+}
+double ref_adk_rate_SI(double U_eV, int Zafter, double E) { return adk_rate_SI(U_eV, Zafter, E); }
+unsigned long long ref_mix64(unsigned long long z) { return mix64(z); }
+double ref_uniform01(unsigned long long k) { return uniform01(k); }
+}

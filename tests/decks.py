@@ -1,0 +1,54 @@
+"""Small decks of the parity tests (code units, lambda0 = 2 pi um so that 1 um = 1/k0) and the comparison metrics."""
+import numpy as np
+
+import g3d_amd
+
+LAMBDA0 = 2 * np.pi * 1e-6
+
+
+def plasma_deck(nx=32, ny=32, nz=48, ppc=2, sponge=6.0, passes=2, smooth=0.0, a0=1.0, n0=1e-2, extrap=1):
+    """A reduced LWFA deck: uniform plasma, Gaussian grid laser (config 1 of the benchmark, shrunk)."""
+    h = 0.5
+    d = g3d_amd.Deck(LAMBDA0).grid(nx, ny, nz, 60.0, 60.0, (nz - 1) * h)
+    d.set(dz=5.0, z_end=1000.0, plasma_on=1, n0=n0, ppc_x=ppc, ppc_y=ppc, move_y=0, sponge=sponge, sponge_y=sponge,
+          iterations=passes, smooth_length=smooth, predictor_extrapolate=extrap)
+    d.add_grid_laser(a0=a0, waist=8.0, tau=3.0, zeta_c=(nz - 1) * h * 0.6)
+    return d
+
+
+def bunch_deck(nx=32, ny=32, nz=64, nmacro=4096, laser=True, sponge=4.0):
+    """A reduced config-0 deck: electron bunch at gamma 1000 plus the oblique plane-wave laser, vacuum."""
+    h = 0.5
+    d = g3d_amd.Deck(LAMBDA0).grid(nx, ny, nz, 40.0, 40.0, (nz - 1) * h)
+    Lw = (nz - 1) * h
+    d.set(dz=20.0, z_end=1000.0, sponge=sponge, sponge_y=sponge, move_x=1, move_y=1,
+          charge=11.4014089e-12 * nmacro / 262144, gamma=1000.0, sigx=2.0, sigy=2.0, sigz=4.0, emitx=2e-4, emity=2e-4,
+          espread=1e-3, zeta_center=0.5 * Lw, nmacro=nmacro, seed=1)
+    if laser:
+        d.set(laser_on=1, laser_pol=1, a0=1.0, theta=2e-3, tau=30.0, start_fwhm=0.0, stop_fwhm=0.0)
+    return d
+
+
+def slice_rel_l2(a, b, nplanes, floor=1e-6):
+    """max over planes of ||a - b|| / ||b||; planes whose norm is below floor * (largest plane norm) use that floor."""
+    a = np.asarray(a).reshape(nplanes, -1)
+    b = np.asarray(b).reshape(nplanes, -1)
+    nb = np.linalg.norm(b, axis=1)
+    den = np.maximum(nb, floor * max(nb.max(), 1e-300))
+    return float((np.linalg.norm(a - b, axis=1) / den).max())
+
+
+def group_rel_l2(A, B, nplanes, floor=1e-6):
+    """Relative L2 per plane over a group of field components taken together (the field vector of a slice):
+    max_n ||A_n - B_n|| / ||B_n||, with the planes below floor * (largest plane norm) measured against that floor.
+    A, B: lists of arrays of nplanes planes each."""
+    a = np.concatenate([np.asarray(x).reshape(nplanes, -1) for x in A], axis=1)
+    b = np.concatenate([np.asarray(x).reshape(nplanes, -1) for x in B], axis=1)
+    return slice_rel_l2(a, b, nplanes, floor)
+
+
+def F_rel_l2(Fa, Fb, ns, nk):
+    """relative L2 per slice of the real fields F [6][ns+2][nk] (E and B of a slice together), own slices"""
+    A = np.asarray(Fa, dtype=np.float64).reshape(6, ns + 2, nk)[:, :ns]
+    B = np.asarray(Fb, dtype=np.float64).reshape(6, ns + 2, nk)[:, :ns]
+    return group_rel_l2([A[c] for c in range(6)], [B[c] for c in range(6)], ns)
