@@ -84,6 +84,7 @@ int g3d_sim_call(g3d_sim* sim, const char* op_, const double* a, int na) {
         else if (op == "relax_start_field") s.relax_start_field();
         else if (op == "add_grid_lasers") s.add_grid_lasers();
         else if (op == "load_plasma") { need(2); s.load_plasma(a[0], a[1]); }
+        else if (op == "stream_pass") { need(4); s.stream_pass(int(a[0]), int(a[1]), a[2], a[3]); }
         else if (op == "set_s") { need(2); s.s = a[0]; s.istep = long(a[1]); }
         else if (op == "set_starting") { need(1); s.starting_ = a[0] != 0; }
         else if (op == "j0_from_jk") s.j0_from_jk();
@@ -129,6 +130,11 @@ int g3d_sim_set(g3d_sim* sim, const char* name, const void* host, size_t bytes) 
     int rc = 0;
     const int g = g3d::guard([&] {
         g3d::Sim& s = *sim->s;
+        if (std::string(name) == "strm") {
+            const g3d_particle* p = static_cast<const g3d_particle*>(host);
+            s.set_strm(std::vector<g3d_particle>(p, p + bytes / sizeof(g3d_particle)));
+            return;
+        }
         void* d; size_t b;
         if (!lookup(s, name, d, b)) { g3d::set_error(std::string("unknown array ") + name); rc = 3; return; }
         if ((rc = check_size(b, bytes))) return;

@@ -618,6 +618,44 @@ void Sim::real_fields_slice(int n) {
     fields_to_F(tp3.data(), nk, F.data() + l * nk, long(ns + 2) * nk, nk, 1);
 }
 
+// the streaming context of the step s0 -> s1 (src/sim_plasma.cpp:280-291)
+StreamArgs Sim::stream_args(double s0, double s1) {
+    const double Delta = s1 - s0;
+    StreamArgs sc;
+    sc.fv = fview(); sc.laser = laser; sc.X0 = X0; sc.Y0 = Y0; sc.zc = cfg.zeta_center; sc.ylo = -1e300; sc.yhi = 1e300;
+    sc.zs = zs.data(); sc.J = jsr.data(); sc.ia = iA.data(); sc.id = iD.data();
+    sc.Delta = Delta; sc.s1 = s1; sc.sTrap = s1 + (cfg.trap_advance - 1) * Delta; sc.vz_beam = cfg.vz_beam;
+    sc.ions = ion_dev.data(); sc.adk = adk_dev.data(); sc.born = born.cap ? born.view() : Parts{};
+    sc.bcnt = counters.data(); sc.bcap = 0; sc.key = 0; sc.n = 0;
+    return sc;
+}
+
+// one pass of the (untiled) streaming list through half-cell n into a zeroed jsr; the list keeps the result
+void Sim::stream_pass(int n, int pass, double s0, double s1) {
+    StreamArgs sc = stream_args(s0, s1);
+    sc.n = n;
+    jsr.zero();
+    if (born_cap > 0) {
+        counters.zero();
+        sc.key = key_slice(key_step(cfg.seed, istep), n, pass);
+        sc.bcap = int(std::min<long>(long(born.cap), long(strmA.cap) - nstrm));
+    }
+    stream_halfcell(sc, strmA.view(), strmB.view(&strmA), nstrm, TileGeom{});
+    parts_copy(strmB.view(&strmA), strmA.view(), nstrm);
+}
+
+// a streaming list from the host (untiled)
+void Sim::set_strm(const std::vector<g3d_particle>& v) {
+    nstrm = long(v.size());
+    n_tiled = 0;
+    ensure_strm(v.size() + 1);
+    if (!nstrm) return;
+    if (aos.size() < v.size()) aos.resize(v.size());
+    aos.from_host(v.data(), 0, v.size());
+    parts_from_aos(aos.data(), strmA.view(), nstrm);
+    parts_copy(strmA.view(), strmB.view(&strmA), nstrm);
+}
+
 void Sim::march_plasma(double s0, double s1) {
     const double Delta = s1 - s0, D = 0.5 * Delta;
     const size_t pl = size_t(nk);
@@ -653,12 +691,7 @@ void Sim::march_plasma(double s0, double s1) {
     toc(F_MODES);
     real_fields(N - 1);
 
-    StreamArgs sc;
-    sc.fv = fview(); sc.laser = laser; sc.X0 = X0; sc.Y0 = Y0; sc.zc = cfg.zeta_center; sc.ylo = -1e300; sc.yhi = 1e300;
-    sc.zs = zs.data(); sc.J = jsr.data(); sc.ia = iA.data(); sc.id = iD.data();
-    sc.Delta = Delta; sc.s1 = s1; sc.sTrap = s1 + (cfg.trap_advance - 1) * Delta; sc.vz_beam = cfg.vz_beam;
-    sc.ions = ion_dev.data(); sc.adk = adk_dev.data(); sc.born = born.cap ? born.view() : Parts{}; sc.bcnt = counters.data();
-    sc.bcap = 0; sc.key = 0;
+    StreamArgs sc = stream_args(s0, s1);
     const bool ionz = born_cap > 0;
     // tiles only when this step loaded tiled plasma (otherwise the slabs would hold an earlier step's current)
     const TileGeom tg = n_tiled > 0 ? tiles : TileGeom{};

@@ -81,6 +81,9 @@ public:
     void march_slice(int n, double D);
     void real_fields_slice(int n);
     void march_plasma(double s0, double s1);
+    void stream_pass(int n, int pass, double s0, double s1);
+    void set_strm(const std::vector<g3d_particle>& v);
+    StreamArgs stream_args(double s0, double s1);
     void collect_restreaming();
     void load_bunch();
     void window_shift_start();

@@ -3,6 +3,7 @@
 // the tests can run the reference and the MI355X library on the same inputs.  Compiled against the sources under
 // /root/reference (never copied); only tests/, smoke() and bench.py's CPU baseline load the result.
 #include "sim.hpp"
+#include "sim_stream.hpp"
 #include <cstring>
 #include <map>
 #include <stdexcept>
@@ -24,6 +25,36 @@ struct Ref {
 };
 
 int fail(const std::exception& e) { err_ = e.what(); return 1; }
+
+// one pass of the streaming particles through half-cell n, with the context march_plasma builds
+// (src/sim_plasma.cpp:280-291, 327-337); jsr is zeroed first
+void stream_pass(Simulation& s, int n, int pass, double s0, double s1) {
+    const Config& cfg = s.cfg;
+    const double Delta = s1 - s0;
+    StreamCtx sc;
+    {
+        PushCtx& c = sc.pc;
+        c.F = s.F.data(); c.a = s.a; c.ns = s.ns; c.N = s.N; c.nx = cfg.nx; c.ny = cfg.ny; c.nk = s.nk; c.self = 1;
+        c.yf = s.yf; c.dx = s.dx; c.dy = s.dy; c.Lx = cfg.Lx; c.Ly = cfg.Ly; c.zax = s.zax; c.yax = s.yax;
+        c.laser = s.laser; c.X0 = s.X0; c.zc = cfg.zeta_center; c.dz_push = cfg.dz_push; c.push_phase = cfg.push_phase;
+        c.zhand = -1e300; c.zmax = cfg.zeta[s.N - 1]; c.s1 = s1; c.vz_min = 0; c.freeze = 0; c.hmin = 1e300; c.Y0 = s.Y0;
+        c.ext = s.external();
+        c.ylo = s.yf ? -1e300 : cfg.ynode[1]; c.yhi = s.yf ? 1e300 : cfg.ynode[cfg.ny - 1];
+    }
+    sc.zs = s.zs.data(); sc.J = s.jsr.data(); sc.ia = s.iA.data(); sc.id = s.iD.data();
+    sc.Delta = Delta; sc.s1 = s1; sc.sTrap = s1 + (cfg.trap_advance - 1) * Delta; sc.vz_beam = cfg.vz_beam;
+    sc.ions = s.ion_dev.data(); sc.adk = s.adk_dev.data(); sc.born = s.born.data(); sc.bcnt = s.bcnt.data();
+    sc.bcap = 0; sc.key = 0; sc.n = n;
+    s.jsr.zero();
+    if (s.born_cap > 0) {
+        s.bcnt.zero();
+        sc.key = ion_key_slice(ion_key_step(cfg.seed, s.istep), n, pass);
+        sc.bcap = int(std::min<long>(long(s.born.size()), long(s.strm.size()) - s.nstrm));
+    }
+    Particle* P = s.strm.data();
+    const StreamCtx scc = sc;
+    par_for(s.nstrm, [=](long i) { stream_one(scc, P[i]); });
+}
 
 Config make_config(const g3d_config& g) {
     Config c = Config::base_units(g.lambda0);
@@ -127,6 +158,7 @@ int ref_sim_call(void* h, const char* op_, const double* a, int na) {
         else if (op == "load_plasma") { need(2); s.load_plasma(a[0], a[1], s.joining_, nullptr); }
         else if (op == "set_s") { need(2); s.s = a[0]; s.istep = long(a[1]); }
         else if (op == "set_starting") { need(1); s.starting_ = a[0] != 0; }
+        else if (op == "stream_pass") { need(4); stream_pass(s, int(a[0]), int(a[1]), a[2], a[3]); }
         else if (op == "j0_from_jk") {
             const long n = long(s.ns + 1) * s.nk;
             std::vector<cpx> t(n);
@@ -213,6 +245,14 @@ int ref_sim_set(void* h, const char* name_, const void* in, size_t bytes) {
         if (n == "jw") return copy_in(s.jw, in, bytes);
         if (n == "F") return copy_in(s.F, in, bytes);
         if (n == "jsr") return copy_in(s.jsr, in, bytes);
+        if (n == "strm") {
+            const size_t cnt = bytes / sizeof(Particle);
+            std::vector<Particle> v(static_cast<const Particle*>(in), static_cast<const Particle*>(in) + cnt);
+            if (s.strm.size() < cnt) { s.strm.resize(cnt); s.strm_bk.resize(cnt); }
+            s.strm.from_host(v.data(), 0, cnt);
+            s.nstrm = long(cnt);
+            return 0;
+        }
         err_ = "unknown array " + n;
         return 3;
     } catch (const std::exception& e) { return fail(e); }

@@ -120,6 +120,6 @@ class RefSim(g3d_amd.SimBase):
 
 def fft_many(nx, ny, two_d, data, inverse):
     """The reference's FftMany::run on a host array of shape (count, ny, nx), in place."""
-    a = np.ascontiguousarray(data, dtype=np.complex128)
+    a = np.array(data, dtype=np.complex128, copy=True, order="C")
     check(lib().ref_fft_many(nx, ny, int(two_d), a.ctypes.data_as(C.c_void_p), a.shape[0], int(inverse)))
     return a

@@ -1,0 +1,75 @@
+"""Writes the golden vectors of the per-slice march from the reference's own CPU build (oracle/_ref, one thread):
+small seeded inputs and the reference's outputs, one .npz per function (inputs and expected outputs only).
+
+Run in the build container (the reference tree must be present to build oracle/_ref):
+    python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+sys.path.insert(0, os.path.join(ROOT, "gem-pic-3d_amd"))
+import refpy  # noqa: E402
+import golden_inputs as gi  # noqa: E402
+
+
+def main():
+    refpy.set_threads(1)
+    out = {}
+    # batched FFT (src/fftmany.hpp): 2D planes and rows, both directions
+    x = gi.fft_input()
+    out["fft"] = dict(x=x, fwd2d=refpy.fft_many(8, 16, True, x, 0), inv2d=refpy.fft_many(8, 16, True, x, 1),
+                      fwd1d=refpy.fft_many(8, 16, False, x, 0), inv1d=refpy.fft_many(8, 16, False, x, 1))
+    # march of random spectra (src/sim_fields.cpp:7-39)
+    deck = gi.small_deck()
+    ref = refpy.RefSim(deck)
+    arr = gi.march_input(ref)
+    for k, v in arr.items():
+        ref.set(k, v)
+    ref.call("march", gi.MARCH_D)
+    out["march"] = dict(**{"in_" + k: v for k, v in arr.items()},
+                        **{"out_" + k: ref.array(k) for k in ("Rx", "Ry", "Ez", "Bz", "Lx", "Ly")})
+    # streaming current spectra (src/sim_plasma.cpp:94-117)
+    deck = gi.small_deck(smooth=1.5)
+    ref = refpy.RefSim(deck)
+    J = gi.jsr_input()
+    ref.set("jsr", J)
+    ref.call("stream_spectra")
+    out["stream_spectra"] = dict(J=J, jsn=ref.array("jsn"), a2=1.5 ** 2)
+    # one streaming half-cell pass (src/sim_stream.hpp:104-171) in random fields
+    deck = gi.small_deck()
+    ref = refpy.RefSim(deck)
+    ref.call("init")
+    F, P = gi.stream_input(ref)
+    ref.set("F", F)
+    ref.set("strm", P)
+    ref.call("stream_pass", gi.STREAM_N, 0, 0.0, 5.0)
+    out["stream_pass"] = dict(F=F, parts_in=P, parts_out=ref.array("strm"), jsr=ref.array("jsr"),
+                              X0=ref.info()["X0"], Y0=ref.info()["Y0"])
+    # RK4 push of bunch particles in random fields and the oblique plane wave (src/sim_kernels.hpp:98-149)
+    deck = gi.bunch_small_deck()
+    ref = refpy.RefSim(deck)
+    ref.call("init")
+    F, P = gi.push_input(ref)
+    ref.set("F", F)
+    ref.set_particles(P)
+    ref.call("push", 0.0, 20.0)
+    out["push"] = dict(F=F, parts_in=P, parts_out=ref.array("parts"), X0=ref.info()["X0"], Y0=ref.info()["Y0"])
+    # ADK levels of nitrogen (src/ionization.cpp:76-99), code units of lambda0 = 0.8 um
+    out["adk"] = dict(ip=gi.N_IP, table=gi.ref_adk_table())
+    # SplitMix64 finaliser and the draw keys (src/ionization.hpp:37-46)
+    keys = np.array([0, 1, 1234567, 0x9e3779b97f4a7c15, 2 ** 63 + 12345, 2 ** 64 - 1], dtype=np.uint64)
+    out["mix64"] = dict(keys=keys, mix=np.array([refpy.lib().ref_mix64(int(k)) for k in keys], dtype=np.uint64),
+                        u01=np.array([refpy.lib().ref_uniform01(int(k)) for k in keys]))
+    for name, d in out.items():
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **d)
+        print("wrote", name, os.path.getsize(os.path.join(HERE, name + ".npz")), "bytes")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,107 @@
+"""Seeded inputs of the golden vectors (shared by tests/golden/make_golden.py and the tests that replay them)."""
+import numpy as np
+
+import g3d_amd
+from decks import LAMBDA0
+
+MARCH_D = 2.5
+STREAM_N = 9
+# NIST ionization energies of nitrogen [eV] (the reference's atomic-data row N, src/atomic_data.cpp:36)
+N_IP = np.array([14.53413, 29.60125, 47.4453, 77.4735, 97.8901, 552.06733, 667.04612])
+
+
+def small_deck(smooth=0.0):
+    d = g3d_amd.Deck(LAMBDA0).grid(8, 8, 12, 16.0, 16.0, 11 * 0.5)
+    d.set(dz=5.0, z_end=100.0, plasma_on=1, n0=1e-2, ppc_x=1, ppc_y=1, move_y=0, smooth_length=smooth)
+    return d
+
+
+def bunch_small_deck():
+    d = g3d_amd.Deck(LAMBDA0).grid(8, 8, 16, 16.0, 16.0, 15 * 0.5)
+    d.set(dz=20.0, z_end=100.0, laser_on=1, laser_pol=1, a0=1.0, theta=2e-3, tau=30.0, start_fwhm=0.0,
+          stop_fwhm=0.0, zeta_center=3.75)
+    return d
+
+
+def fft_input():
+    r = np.random.default_rng(11)
+    return r.standard_normal((3, 16, 8)) + 1j * r.standard_normal((3, 16, 8))
+
+
+def march_input(sim):
+    r = np.random.default_rng(12)
+    out = {}
+    for n in ("Rx", "Ry", "Ez", "Bz", "Lx", "Ly", "j0x", "j0y", "jkx", "jky", "jkz"):
+        m = sim._size(n) // 16
+        out[n] = r.standard_normal(m) + 1j * r.standard_normal(m)
+    return out
+
+
+def jsr_input():
+    return np.random.default_rng(13).uniform(-1, 1, 5 * 64)
+
+
+def stream_input(sim):
+    """random fields of moderate amplitude and 64 electrons in half-cell STREAM_N (some waiting, some off the
+    slice top, some in a strong transverse kick)"""
+    r = np.random.default_rng(14)
+    F = (0.05 * r.standard_normal(sim._size("F") // 4)).astype(np.float32)
+    P = np.zeros(64, dtype=g3d_amd.PARTICLE)
+    z = np.linspace(0, 5.5, 12)
+    P["x"] = r.uniform(0, 16, 64)
+    P["y"] = r.uniform(0, 16, 64)
+    P["zeta"] = z[STREAM_N + 1]
+    P["zeta"][::7] = z[STREAM_N] + 0.3 * (z[STREAM_N + 1] - z[STREAM_N])
+    P["px"] = 0.3 * r.standard_normal(64)
+    P["py"] = 0.3 * r.standard_normal(64)
+    P["pz"] = 0.2 * r.standard_normal(64)
+    P["pz"][:3] = 30.0                       # fast: trapped in this pass
+    P["w"] = 1e-4
+    P["q"] = -1
+    P["qm"] = -1
+    P["sp"] = 2
+    P["alive"] = 1
+    P["alive"][5] = 6
+    P["zeta"][5] = z[STREAM_N] + 0.5 * (z[STREAM_N + 1] - z[STREAM_N])
+    P["alive"][6] = 6
+    P["zeta"][6] = z[STREAM_N] - 0.1         # waiting below the half-cell: stays
+    P["alive"][7] = 0
+    P["z"] = 0.0
+    P["id"] = np.arange(64, dtype=np.uint64) + (1 << 32)
+    return F, P
+
+
+def push_input(sim):
+    r = np.random.default_rng(15)
+    F = (0.01 * r.standard_normal(sim._size("F") // 4)).astype(np.float32)
+    P = np.zeros(32, dtype=g3d_amd.PARTICLE)
+    P["x"] = 8 + r.standard_normal(32)
+    P["y"] = 8 + r.standard_normal(32)
+    P["zeta"] = 3.75 + 0.5 * r.standard_normal(32)
+    P["px"] = 1e-2 * r.standard_normal(32)
+    P["py"] = 1e-2 * r.standard_normal(32)
+    P["pz"] = 1000 * (1 + 1e-3 * r.standard_normal(32))
+    P["pz"][0] = 3.0                          # slow: small sub-steps
+    P["w"] = 1e-3
+    P["q"] = -1
+    P["qm"] = -1
+    P["alive"] = 1
+    return F, P
+
+
+def units(lambda0=0.8e-6):
+    Lu = lambda0 / (2 * np.pi)
+    omega0 = g3d_amd.C_SI / Lu
+    E0 = g3d_amd.ME_SI * g3d_amd.C_SI * omega0 / g3d_amd.E_SI
+    return E0, omega0
+
+
+def ref_adk_table():
+    import ctypes as C
+    import refpy
+    E0, omega0 = units()
+    out = np.zeros((7, 5))
+    ip = np.ascontiguousarray(N_IP)
+    refpy.check(refpy.lib().ref_adk_levels(ip.ctypes.data_as(C.c_void_p), 7, 0, 7, E0, omega0, 2 * 500.0,
+                                           out.ctypes.data_as(C.c_void_p)))
+    return out

@@ -112,26 +112,27 @@ def test_plasma_smoothing():
 
 
 def ref_spread(deck_fn, steps, **kw):
-    """The reference's own reproducibility on a deck: 1 thread against 8 threads (its FP64 atomic deposits sum in
-    thread order, so the last bits of the fields depend on the schedule)."""
-    runs = []
-    for thr in (1, 8):
+    """The reference's own reproducibility on a deck: the largest difference between its 1-thread run and its runs
+    on 2, 4 and 8 threads (its FP64 atomic deposits sum in thread order, so the last bits of the fields depend on
+    the schedule)."""
+    runs = {}
+    for thr in (1, 2, 4, 8):
         refpy.set_threads(thr)
         r = refpy.RefSim(deck_fn(**kw))
         r.init()
         for _ in range(steps):
             r.step()
-        runs.append(r)
+        runs[thr] = r
     refpy.set_threads(1)
-    a, b = runs
-    ns = a.info()["ns"]
-    return group_rel_l2([a.array(n) for n in SPEC], [b.array(n) for n in SPEC], ns)
+    ns = runs[1].info()["ns"]
+    base = [runs[1].array(n) for n in SPEC]
+    return max(group_rel_l2([runs[t].array(n) for n in SPEC], base, ns) for t in (2, 4, 8))
 
 
 def test_bunch_init_and_steps():
     """config-0 style: bunch + oblique plane wave in vacuum; init (co-moving start field, relaxation), 3 steps.
     The beam deposit sums 27 FP64 atomics per particle in an order that depends on the schedule, in the reference
-    as on the GPU, so the bar is max(1e-12, twice the reference's own 1-thread/8-thread spread)."""
+    as on the GPU, so the bar is max(1e-12, 3 x the reference's own spread over thread counts)."""
     gpu, ref = pair(bunch_deck)
     gpu.init(); ref.init()
     pa, pb = gpu.array("parts"), ref.array("parts")
@@ -141,7 +142,7 @@ def test_bunch_init_and_steps():
     compare_fields(gpu, ref)
     for _ in range(3):
         gpu.step(); ref.step()
-    tol = max(TOL, 2 * ref_spread(bunch_deck, 3))
+    tol = max(TOL, 3 * ref_spread(bunch_deck, 3))
     compare_fields(gpu, ref, tol=tol)
     pa, pb = gpu.array("parts"), ref.array("parts")
     np.testing.assert_array_equal(pa["alive"], pb["alive"])
@@ -160,3 +161,17 @@ def test_stream_spectra_random():
         sim.call("stream_spectra")
     a, b = gpu.array("jsn"), ref.array("jsn")
     assert slice_rel_l2(a, b, 3) < 1e-13
+
+
+def test_plasma_strong_laser_tiles():
+    """a0 = 4 in a denser plasma: the electrons are blown out by many cells, so deposits leave their tile's LDS
+    window and take the global-atomic path beside the slabs; 2 steps against the reference."""
+    gpu, ref = pair(plasma_deck, a0=4.0, n0=3e-2)
+    gpu.init(); ref.init()
+    for _ in range(2):
+        gpu.step(); ref.step()
+    compare_fields(gpu, ref, tol=1e-11, groups=("fields", "js0"))
+    a, b = gpu.array("strm"), ref.array("strm")
+    np.testing.assert_array_equal(a["alive"], b["alive"])
+    dx = np.abs(((a["x"] - b["x"]) + 30) % 60 - 30).max()      # periodic box of 60/k0
+    assert dx < 1e-9
