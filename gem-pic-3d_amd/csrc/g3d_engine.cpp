@@ -9,8 +9,8 @@
 
 namespace g3d {
 
-const char* const kFamilyNames[Sim::kFam] = {"stream_halfcell", "spectra_fft", "plasma_modes", "fields_fft",
-                                             "fields_F", "beam_push", "currents", "sponge_real_fields"};
+const char* const kFamilyNames[Sim::kFam] = {"stream_halfcell", "pass_rows_fwd", "pass_cols", "pass_rows_inv",
+                                             "unused", "beam_push", "currents", "sponge_real_fields"};
 enum { F_STREAM = 0, F_SFFT, F_MODES, F_IFFT, F_TOF, F_PUSH, F_CUR, F_SPONGE };
 
 // ---------------------------------------------------------------- configuration
@@ -203,7 +203,7 @@ Sim::Sim(const Config& c) : cfg(c) {
         bool mob = false;
         for (const auto& s_ : cfg.ions) mob = mob || cfg.loaded(s_);
         if (mob) rho_lines_i.resize(size_t(N) * (cfg.nx + cfg.ny));
-        jsn.resize(3 * pl); jsin.resize(3 * pl); bk.resize(4 * pl); sp3.resize(2 * pl); tp3.resize(3 * pl);
+        jsn.resize(3 * pl); jsin.resize(3 * pl); bk.resize(4 * pl); sp3.resize(3 * pl); tp3.resize(3 * pl);
         jpred.resize(3 * pl); jsi0.resize(3 * pl); dlt.resize(3 * pl);
     }
     counters.resize(4);
@@ -601,10 +601,13 @@ void Sim::load_plasma(double s0, double s1) {
 }
 
 void Sim::stream_spectra() {
-    // stand-alone form: jsr -> jsn through the pack, transform and unpack of the march (jsr is consumed)
-    sspec_pack(jsr.data(), nk, cfg.nx, cfg.ny, sp3.data(), nullptr, nullptr, 0, 0, TileGeom{});
-    fft_run(fft, sp3.data(), 2, false);
-    sspec_unpack(geom(), sp3.data(), jsn.data(), cfg.smooth_length * cfg.smooth_length, cfg.box_mean);
+    // stand-alone form (jsr is consumed): the same row and column kernels as the march, spectra only
+    pass_rows_fwd(jsr.data(), TileGeom{}, fft, sp3.data(), nullptr, nullptr, 0, 0);
+    ModeArgs m{};
+    m.mv = mview(0.0);
+    m.kxs = kxs.data(); m.kys = kys.data(); m.filt = filt.data(); m.filty = filty.data();
+    m.a2 = cfg.smooth_length * cfg.smooth_length; m.keep0 = cfg.box_mean; m.ny = cfg.ny;
+    pass_cols(m, fft, sp3.data(), tp3.data(), jsn.data(), 2);
 }
 
 void Sim::march_slice(int n, double D) { march_one(mview(D), n, nullptr, nullptr, nullptr, nullptr, 0); }
@@ -677,17 +680,14 @@ void Sim::march_plasma(double s0, double s1) {
     const long fpl = long(ns + 2) * nk;
     auto real_fields = [&](int n) {
         tic(F_IFFT);
-        fft_run(fft, tp3.data(), 3, true);
+        pass_rows_inv(tp3.data(), F.data() + long(n - a) * nk, fpl, fft);
         toc(F_IFFT);
-        tic(F_TOF);
-        fields_to_F(tp3.data(), nk, F.data() + long(n - a) * nk, fpl, nk, 1);
-        toc(F_TOF);
     };
     // front: half-cell N-2 predicted with its level-s0 value; slice N-1
     m.pk = nullptr; m.n = N - 1; m.first = 1; m.last = 1; m.top = 1; m.wex = 0; m.qcorr = 0;
     m.bRx = m.bRy = m.bEz = m.bBz = nullptr;
     tic(F_MODES);
-    plasma_modes(m);
+    pass_cols(m, fft, nullptr, tp3.data(), nullptr, 1);
     toc(F_MODES);
     real_fields(N - 1);
 
@@ -716,16 +716,15 @@ void Sim::march_plasma(double s0, double s1) {
             stream_halfcell(sc, strmA.view(), strmB.view(&strmA), nlist, tg);
             toc(F_STREAM);
             const bool last = pass == passes - 1;
-            sspec_pack(jsr.data(), nk, nx, ny, sp3.data(), last ? rho_lines.data() + long(n) * (nx + ny) : nullptr,
-                       last && rho_lines_i.size() ? rho_lines_i.data() + long(n) * (nx + ny) : nullptr, i0, j0, tg);
             tic(F_SFFT);
-            fft_run(fft, sp3.data(), 2, false);
+            pass_rows_fwd(jsr.data(), tg, fft, sp3.data(), last ? rho_lines.data() + long(n) * (nx + ny) : nullptr,
+                          last && rho_lines_i.size() ? rho_lines_i.data() + long(n) * (nx + ny) : nullptr, i0, j0);
             toc(F_SFFT);
-            m.pk = sp3.data(); m.n = n; m.first = pass == 0; m.last = last; m.top = 0; m.wex = wex;
+            m.n = n; m.first = pass == 0; m.last = last; m.top = 0; m.wex = wex;
             m.qcorr = D * wlo_h[n + 1];
             if (passes > 1) { m.bRx = bk.data(); m.bRy = bk.data() + pl; m.bEz = bk.data() + 2 * pl; m.bBz = bk.data() + 3 * pl; }
             tic(F_MODES);
-            plasma_modes(m);
+            pass_cols(m, fft, sp3.data(), tp3.data(), nullptr, 0);
             toc(F_MODES);
             real_fields(n);
         }

@@ -1,5 +1,6 @@
 // Batched transverse FFT (rows, columns) and the runtime/memory entry points of the C ABI.
 #include "g3d_fft.hpp"
+#include "g3d_fft_core.hpp"
 #include "g3d_runtime.hpp"
 #include <cmath>
 #include <complex>
@@ -30,41 +31,16 @@ hipStream_t stream() {
 
 // ------------------------------------------------------------------ FFT kernels
 namespace {
+using namespace fftcore;
 constexpr int kThreads = 256;
 constexpr int kTile = 2048;          // complex values per workgroup tile (32 KiB of LDS)
-
-__device__ __forceinline__ int bitrev(int i, int logn) { return logn ? int(__brev(unsigned(i)) >> (32 - logn)) : 0; }
-
-// in-place radix-2 DIT stages over nseq sequences of length n = 2^logn held in LDS (bit-reversed order)
-__device__ void lds_fft(cx* buf, int n, int logn, int nseq, const cx* tws) {
-    const int hn = n >> 1, nb = nseq * hn;
-    for (int s = 1; s <= logn; ++s) {
-        const int half = 1 << (s - 1), step = n >> s;
-        for (int t = threadIdx.x; t < nb; t += blockDim.x) {
-            const int q = t >> (logn - 1), b = t & (hn - 1), j = b & (half - 1);
-            const int base = q * n + ((b >> (s - 1)) << s) + j;
-            const cx w = tws[j * step];
-            const cx u = buf[base], v = buf[base + half] * w;
-            buf[base] = u + v;
-            buf[base + half] = u - v;
-        }
-        __syncthreads();
-    }
-}
-
-__device__ void load_twiddles(cx* tws, const cx* tw, int hn, bool inv) {
-    for (int k = threadIdx.x; k < hn; k += blockDim.x) {
-        cx w = tw[k];
-        if (inv) w.im = -w.im;
-        tws[k] = w;
-    }
-}
 
 __global__ void __launch_bounds__(kThreads) k_fft_rows(cx* data, long nrows, int n, int logn, int rpw, const cx* tw,
                                                        int inv, double scale) {
     extern __shared__ cx sm[];
+    const int sp = padded(n);
     cx* buf = sm;
-    cx* tws = sm + size_t(rpw) * n;
+    cx* tws = sm + size_t(rpw) * sp;
     const long row0 = long(blockIdx.x) * rpw;
     const long left = nrows - row0;
     const int nr = left < rpw ? int(left) : rpw;
@@ -73,19 +49,24 @@ __global__ void __launch_bounds__(kThreads) k_fft_rows(cx* data, long nrows, int
     const int tot = nr * n;
     for (int e = threadIdx.x; e < tot; e += blockDim.x) {
         const int r = e >> logn, i = e & (n - 1);
-        buf[r * n + bitrev(i, logn)] = g[e];
+        buf[r * sp + lpos(bitrev(i, logn))] = g[e];
     }
     __syncthreads();
-    lds_fft(buf, n, logn, nr, tws);
-    for (int e = threadIdx.x; e < tot; e += blockDim.x) g[e] = scale == 1.0 ? buf[e] : scale * buf[e];
+    lds_fft(buf, n, logn, nr, sp, tws);
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+        const int r = e >> logn, i = e & (n - 1);
+        const cx v = buf[r * sp + lpos(i)];
+        g[e] = scale == 1.0 ? v : scale * v;
+    }
 }
 
-// columns c0 .. c0+nc-1 of plane p; LDS holds column c at [c*ny]
+// columns c0 .. c0+nc-1 of plane p; LDS holds column c at [c * padded(ny)]
 __global__ void __launch_bounds__(kThreads) k_fft_cols(cx* data, int nx, int ny, int logny, int nc, const cx* tw,
                                                        int inv, double scale) {
     extern __shared__ cx sm[];
+    const int sp = padded(ny);
     cx* buf = sm;
-    cx* tws = sm + size_t(nc) * ny;
+    cx* tws = sm + size_t(nc) * sp;
     const int tiles = nx / nc;
     const long p = blockIdx.x / tiles;
     const int c0 = (blockIdx.x % tiles) * nc;
@@ -94,18 +75,20 @@ __global__ void __launch_bounds__(kThreads) k_fft_cols(cx* data, int nx, int ny,
     const int tot = nc * ny;
     for (int e = threadIdx.x; e < tot; e += blockDim.x) {
         const int j = e / nc, c = e - j * nc;
-        buf[c * ny + bitrev(j, logny)] = g[long(j) * nx + c];
+        buf[c * sp + lpos(bitrev(j, logny))] = g[long(j) * nx + c];
     }
     __syncthreads();
-    lds_fft(buf, ny, logny, nc, tws);
+    lds_fft(buf, ny, logny, nc, sp, tws);
     for (int e = threadIdx.x; e < tot; e += blockDim.x) {
         const int j = e / nc, c = e - j * nc;
-        const cx v = buf[c * ny + j];
+        const cx v = buf[c * sp + lpos(j)];
         g[long(j) * nx + c] = scale == 1.0 ? v : scale * v;
     }
 }
 
 int ilog2(int n) { int l = 0; while ((1 << l) < n) ++l; return l; }
+// a workgroup of 64 ... 256 lanes: one lane per group of eight elements of the tile
+int threads_for(int groups) { int t = 64; while (t < groups && t < kThreads) t <<= 1; return t; }
 bool pow2(int n) { return n >= 2 && (n & (n - 1)) == 0; }
 
 cx* twiddles(int n) {
@@ -139,9 +122,11 @@ void fft_rows(const FftPlan& p, cx* data, long nrows, bool inverse, double scale
     if (nrows <= 0) return;
     const int n = p.nx;
     int rpw = kTile / n;
+    if (rpw < 1) rpw = 1;
     while (rpw > 1 && (nrows + rpw - 1) / rpw < 512) rpw >>= 1;
-    const size_t lds = (size_t(rpw) * n + n / 2) * sizeof(cx);
-    hipLaunchKernelGGL(k_fft_rows, dim3(blocks_for(nrows, rpw)), dim3(kThreads), lds, stream(), data, nrows, n, p.lx,
+    const size_t lds = (size_t(rpw) * (n + n / 8) + n / 2) * sizeof(cx);
+    const int thr = threads_for(rpw * (n / 8 > 0 ? n / 8 : 1));
+    hipLaunchKernelGGL(k_fft_rows, dim3(blocks_for(nrows, rpw)), dim3(thr), lds, stream(), data, nrows, n, p.lx,
                        rpw, p.twx, int(inverse), scale);
     launch_check("fft rows");
 }
@@ -151,9 +136,10 @@ void fft_cols(const FftPlan& p, cx* data, int count, bool inverse, double scale)
     if (nc > p.nx) nc = p.nx;
     while (nc > 2 && long(count) * (p.nx / nc) < 512) nc >>= 1;
     if (nc < 1) nc = 1;
-    const size_t lds = (size_t(nc) * p.ny + p.ny / 2) * sizeof(cx);
+    const size_t lds = (size_t(nc) * (p.ny + p.ny / 8) + p.ny / 2) * sizeof(cx);
     const long blocks = long(count) * (p.nx / nc);
-    hipLaunchKernelGGL(k_fft_cols, dim3(unsigned(blocks)), dim3(kThreads), lds, stream(), data, p.nx, p.ny, p.ly, nc,
+    const int thr = threads_for(nc * (p.ny / 8 > 0 ? p.ny / 8 : 1));
+    hipLaunchKernelGGL(k_fft_cols, dim3(unsigned(blocks)), dim3(thr), lds, stream(), data, p.nx, p.ny, p.ly, nc,
                        p.twy, int(inverse), scale);
     launch_check("fft columns");
 }

@@ -113,7 +113,11 @@ struct ModeArgs {
     double wex, qcorr, a2;
     int n, first, last, top, extrap, keep0, ny;
 };
-void plasma_modes(const ModeArgs& m);
+// the fused kernels of a plasma pass (g3d_plasma.hip): rows forward (jsr + slabs -> X, 3 planes), columns
+// (kind 0 pass, 1 front slice, 2 spectra only into jsn; X -> T, 3 planes), rows inverse (T -> F planes of a slice)
+void pass_rows_fwd(double* J, const TileGeom& tg, const FftPlan& p, cx* X, double* rl, double* rli, int i0, int j0);
+void pass_cols(const ModeArgs& m, const FftPlan& p, const cx* X, cx* T, cx* jsn, int kind);
+void pass_rows_inv(const cx* T, float* Fn, long fpl, const FftPlan& p);
 // jz -= js0z over count planes
 void sub_planes(cx* a, const cx* b, long n);
 void copy_planes(cx* dst, const cx* src, long n);

@@ -1,30 +1,40 @@
-// The per-mode work of one pass of a plasma half-cell, fused into one kernel (one lane per mode of the plane):
-//   unpack + filter the spectra of the streaming current (src/sim_plasma.cpp:105-115)
-//   replace the streaming part of jk(n) and the extrapolated prediction of jk(n-1) (src/sim_plasma.cpp:255-351)
-//   march slice n with the pass backup of R, E_z, B_z (src/sim_plasma.cpp:306-326, 352)
-//   on the last pass the correction of R at slice n+1 and the level-s0 current (src/sim_plasma.cpp:381-396)
-//   pack R_x + i R_y, L_x + i L_y, E_z + i B_z of slice n for the inverse transforms (src/sim_plasma.cpp:205-209)
-// In the reference these are about a dozen kernels with a host synchronisation after most of them.
+// One pass of a plasma half-cell after the particles, fused into three kernels around the transverse FFT:
+//
+//  k_pass_rows_fwd   the streaming current of the half-cell (jsr plus the tile slabs), the density lines,
+//                    (j_x + i j_y, j_z) of a row -> x transform -> the pair unpacked per row (a real row has a
+//                    Hermitian x spectrum, so the unpack needs only the row itself); jsr is zeroed
+//  k_pass_cols       y transforms of the three planes of a block of columns in LDS, then per mode: the filters
+//                    and smoothing of the streaming current (src/sim_plasma.cpp:105-115), the replacement of the
+//                    streaming part of jk(n) and the extrapolated prediction of jk(n-1) (src/sim_plasma.cpp:
+//                    255-351), the march of slice n with the pass backup (src/sim_kernels.hpp:173-210), on the last
+//                    pass the correction of R at slice n+1 (src/sim_plasma.cpp:381-396), and the packed planes
+//                    R_x + i R_y, L_x + i L_y, E_z + i B_z, which are y-transformed back in LDS
+//  k_pass_rows_inv   x inverse transforms, scaled, and E, B of slice n into the float F (src/sim_plasma.cpp:205-218)
+//
+// In the reference these are about a dozen kernels, two batched FFTs and a host synchronisation after most of
+// them; here they are three launches with no host synchronisation.
+#include "g3d_fft_core.hpp"
 #include "g3d_kernels.hpp"
 #include "g3d_runtime.hpp"
 
 namespace g3d {
 namespace {
+using namespace fftcore;
+constexpr int kT = 256;
+constexpr int kMaxModes = 4;          // modes per lane in k_pass_cols (nc * ny <= kMaxModes * kT)
 
-__global__ void __launch_bounds__(256) k_plasma_modes(ModeArgs m) {
+// the per-mode work of the pass (see the file head); u: the filtered streaming current of this pass
+__device__ __forceinline__ void mode_work(const ModeArgs& m, long i, const cx u[3], cx T[3]) {
     const MarchView& c = m.mv;
-    const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= c.nk) return;
     const long nk = c.nk;
     const int n = m.n;
     const long hp = long(n + 1) * nk + i;           // half-cell n   (plane n + 1 of jk, js0)
     const long hq = long(n) * nk + i;               // half-cell n-1 (plane n)
     cx* jk[3] = {m.jkx, m.jky, m.jkz};
     cx* js0[3] = {m.js0x, m.js0y, m.js0z};
-    // ------------------------------------------------ start of the half-cell (first pass)
     if (m.first) {
         if (m.top) {
-            // front: half-cell n-1 = N-2 predicted with its level-s0 value (the prediction buffer starts at zero)
+            // front: half-cell N-2 predicted with its level-s0 value (the prediction buffer starts at zero)
             for (int k = 0; k < 3; ++k) {
                 const cx t = js0[k][hq];
                 jk[k][hq] = jk[k][hq] + t - m.pin[k * nk + i];
@@ -44,20 +54,8 @@ __global__ void __launch_bounds__(256) k_plasma_modes(ModeArgs m) {
                 }
         }
     }
-    // ------------------------------------------------ the streaming current of this pass
-    cx u[3];
     if (!m.top) {
-        const int ixx = int(i % c.nx), j = int(i / c.nx);
-        const long mi = long((m.ny - j) % m.ny) * c.nx + (c.nx - ixx) % c.nx;
-        const cx z = m.pk[i], zm = conjg(m.pk[mi]);
-        const double k2 = m.kxs[ixx] * m.kxs[ixx] + m.kys[j] * m.kys[j];
-        const double f = (ixx == 0 && j == 0 && !m.keep0) ? 0.0 : m.filt[ixx] * m.filty[j] / (1 + m.a2 * k2);
-        const cx d = z - zm;
-        u[0] = f * (0.5 * (z + zm));
-        u[1] = f * (0.5 * mk(d.im, -d.re));
-        u[2] = f * m.pk[nk + i];
         for (int k = 0; k < 3; ++k) {
-            m.jsn[k * nk + i] = u[k];
             const cx s = m.sin_[k * nk + i];
             jk[k][hp] = jk[k][hp] + u[k] - s;
             m.sin_[k * nk + i] = u[k];
@@ -70,7 +68,6 @@ __global__ void __launch_bounds__(256) k_plasma_modes(ModeArgs m) {
                 m.pin[k * nk + i] = t;
             }
     }
-    // ------------------------------------------------ march slice n
     const long o = long(n - c.a) * nk + i;
     Mode6 old;
     if (!m.first && m.bRx) { old.Rx = m.bRx[i]; old.Ry = m.bRy[i]; old.Ez = m.bEz[i]; old.Bz = m.bBz[i]; }
@@ -83,7 +80,6 @@ __global__ void __launch_bounds__(256) k_plasma_modes(ModeArgs m) {
     if (n < c.N - 1) up = Mode6{c.Rx[on], c.Ry[on], c.Ez[on], c.Bz[on], c.Lx[on], c.Ly[on]};
     const Mode6 r = march_mode(c, i, n, old, up);
     c.Rx[o] = r.Rx; c.Ry[o] = r.Ry; c.Ez[o] = r.Ez; c.Bz[o] = r.Bz; c.Lx[o] = r.Lx; c.Ly[o] = r.Ly;
-    // ------------------------------------------------ half-cell n final: correction of R at slice n+1
     if (m.last && !m.top) {
         const cx s0x = m.si0[i], s0y = m.si0[nk + i];
         c.Rx[on] = c.Rx[on] - m.qcorr * (u[0] - s0x);
@@ -93,16 +89,202 @@ __global__ void __launch_bounds__(256) k_plasma_modes(ModeArgs m) {
             js0[k][hp] = u[k];
         }
     }
-    // ------------------------------------------------ packed planes for the real fields of slice n
-    m.T[i] = mk(r.Rx.re - r.Ry.im, r.Rx.im + r.Ry.re);
-    m.T[nk + i] = mk(r.Lx.re - r.Ly.im, r.Lx.im + r.Ly.re);
-    m.T[2 * nk + i] = mk(r.Ez.re - r.Bz.im, r.Ez.im + r.Bz.re);
+    T[0] = mk(r.Rx.re - r.Ry.im, r.Rx.im + r.Ry.re);
+    T[1] = mk(r.Lx.re - r.Ly.im, r.Lx.im + r.Ly.re);
+    T[2] = mk(r.Ez.re - r.Bz.im, r.Ez.im + r.Bz.re);
 }
+
+// ---------------------------------------------------------------- rows forward
+__global__ void __launch_bounds__(kT) k_pass_rows_fwd(double* J, TileGeom tg, int nx, int ny, int lx, int rpw,
+                                                     const cx* twx, cx* X, double* rl, double* rli, int i0, int j0) {
+    extern __shared__ cx sm[];
+    const int sp = padded(nx);
+    cx* buf = sm;                                    // 2 sequences per row: j_x + i j_y, j_z
+    cx* tws = sm + size_t(2 * rpw) * sp;
+    load_twiddles(tws, twx, nx >> 1, false);
+    const long nk = long(nx) * ny;
+    const int row0 = blockIdx.x * rpw;
+    const int tot = rpw * nx;
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+        const int r = e / nx, ixx = e - r * nx, j = row0 + r;
+        const long i = long(j) * nx + ixx;
+        double v[5];
+        for (int cc = 0; cc < 5; ++cc) v[cc] = J[cc * nk + i];
+        if (tg.ntile) {
+            const int tpx = ixx / tg.TX, tpy = j / tg.TY;
+            const long wpl = long(tg.WX) * tg.WY;
+            for (int dy = -1; dy <= 1; ++dy) {
+                if ((dy == 1 && tg.nty <= 2) || (dy != 0 && tg.nty == 1)) continue;
+                const int ty = ((tpy + dy) % tg.nty + tg.nty) % tg.nty;
+                int vv = j - (ty * tg.TY - 1 - tg.M);
+                vv = ((vv % ny) + ny) % ny;
+                if (vv >= tg.WY) continue;
+                for (int dx = -1; dx <= 1; ++dx) {
+                    if ((dx == 1 && tg.ntx <= 2) || (dx != 0 && tg.ntx == 1)) continue;
+                    const int tx = ((tpx + dx) % tg.ntx + tg.ntx) % tg.ntx;
+                    int uu = ixx - (tx * tg.TX - 1 - tg.M);
+                    uu = ((uu % nx) + nx) % nx;
+                    if (uu >= tg.WX) continue;
+                    const double* sl = tg.slab + (long(ty) * tg.ntx + tx) * 5 * wpl + long(vv) * tg.WX + uu;
+                    for (int cc = 0; cc < 5; ++cc) v[cc] += sl[cc * wpl];
+                }
+            }
+        }
+        if (rl) {                                    // density lines y = 0 and x = 0 (src/sim_plasma.cpp:363-374)
+            if (j == j0) rl[ixx] = v[3];
+            if (ixx == i0) rl[nx + j] = v[3];
+            if (rli) {
+                if (j == j0) rli[ixx] = v[4];
+                if (ixx == i0) rli[nx + j] = v[4];
+            }
+        }
+        for (int cc = 0; cc < 5; ++cc) J[cc * nk + i] = 0;
+        const int b = lpos(bitrev(ixx, lx));
+        buf[(2 * r) * sp + b] = mk(v[0], v[1]);
+        buf[(2 * r + 1) * sp + b] = mk(v[2], 0);
+    }
+    __syncthreads();
+    lds_fft(buf, nx, lx, 2 * rpw, sp, tws);
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+        const int r = e / nx, kx = e - r * nx, j = row0 + r;
+        const long i = long(j) * nx + kx;
+        const cx z = buf[(2 * r) * sp + lpos(kx)], zm = conjg(buf[(2 * r) * sp + lpos((nx - kx) & (nx - 1))]);
+        const cx d = z - zm;
+        X[i] = 0.5 * (z + zm);
+        X[nk + i] = 0.5 * mk(d.im, -d.re);
+        X[2 * nk + i] = buf[(2 * r + 1) * sp + lpos(kx)];
+    }
+}
+
+// ---------------------------------------------------------------- columns: transforms and the per-mode work
+// kind 0: a pass of half-cell n; 1: the front slice (no current); 2: spectra only (jsn, for the stand-alone
+// stream_spectra)
+__global__ void __launch_bounds__(kT) k_pass_cols(ModeArgs m, const cx* X, cx* T, cx* jsn, const cx* twy, int nc,
+                                                 int ly, int kind) {
+    extern __shared__ cx sm[];
+    const MarchView& c = m.mv;
+    const int nx = c.nx, ny = m.ny, sp = padded(ny);
+    const long nk = c.nk;
+    cx* buf = sm;                                    // 3 nc sequences: component k of column c at (k nc + c) sp
+    cx* twf = sm + size_t(3 * nc) * sp;
+    cx* twi = twf + (ny >> 1);
+    load_twiddles(twf, twy, ny >> 1, false);
+    load_twiddles(twi, twy, ny >> 1, true);
+    const int c0 = blockIdx.x * nc;
+    const int nm = nc * ny;
+    if (kind != 1) {
+        for (int e = threadIdx.x; e < 3 * nm; e += blockDim.x) {
+            const int k = e / nm, rem = e - k * nm, j = rem / nc, cc = rem - j * nc;
+            buf[(k * nc + cc) * sp + lpos(bitrev(j, ly))] = X[k * nk + long(j) * nx + c0 + cc];
+        }
+        __syncthreads();
+        lds_fft(buf, ny, ly, 3 * nc, sp, twf);
+    }
+    cx u[kMaxModes][3];
+#pragma unroll
+    for (int q = 0; q < kMaxModes; ++q) {
+        const int e = threadIdx.x + q * blockDim.x;
+        if (e >= nm || kind == 1) continue;
+        const int j = e / nc, cc = e - j * nc, kx = c0 + cc;
+        const double k2 = m.kxs[kx] * m.kxs[kx] + m.kys[j] * m.kys[j];
+        const double f = (kx == 0 && j == 0 && !m.keep0) ? 0.0 : m.filt[kx] * m.filty[j] / (1 + m.a2 * k2);
+        for (int k = 0; k < 3; ++k) u[q][k] = f * buf[(k * nc + cc) * sp + lpos(j)];
+    }
+    __syncthreads();
+    if (kind == 2) {
+#pragma unroll
+        for (int q = 0; q < kMaxModes; ++q) {
+            const int e = threadIdx.x + q * blockDim.x;
+            if (e >= nm) continue;
+            const int j = e / nc, cc = e - j * nc;
+            const long i = long(j) * nx + c0 + cc;
+            for (int k = 0; k < 3; ++k) jsn[k * nk + i] = u[q][k];
+        }
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < kMaxModes; ++q) {
+        const int e = threadIdx.x + q * blockDim.x;
+        if (e >= nm) continue;
+        const int j = e / nc, cc = e - j * nc;
+        const long i = long(j) * nx + c0 + cc;
+        cx t[3];
+        mode_work(m, i, u[q], t);
+        const int b = lpos(bitrev(j, ly));
+        for (int k = 0; k < 3; ++k) buf[(k * nc + cc) * sp + b] = t[k];
+    }
+    __syncthreads();
+    lds_fft(buf, ny, ly, 3 * nc, sp, twi);
+    for (int e = threadIdx.x; e < 3 * nm; e += blockDim.x) {
+        const int k = e / nm, rem = e - k * nm, j = rem / nc, cc = rem - j * nc;
+        T[k * nk + long(j) * nx + c0 + cc] = buf[(k * nc + cc) * sp + lpos(j)];
+    }
+}
+
+// ---------------------------------------------------------------- rows inverse and the real fields of slice n
+__global__ void __launch_bounds__(kT) k_pass_rows_inv(const cx* T, float* Fn, long fpl, int nx, int ny, int lx, int rpw,
+                                                     const cx* twx, double scale) {
+    extern __shared__ cx sm[];
+    const int sp = padded(nx);
+    cx* buf = sm;                                    // 3 sequences per row: R, L, E_z/B_z packed
+    cx* tws = sm + size_t(3 * rpw) * sp;
+    load_twiddles(tws, twx, nx >> 1, true);
+    const long nk = long(nx) * ny;
+    const int row0 = blockIdx.x * rpw;
+    const int tot = rpw * nx;
+    for (int e = threadIdx.x; e < 3 * tot; e += blockDim.x) {
+        const int k = e / tot, rem = e - k * tot, r = rem / nx, ixx = rem - r * nx;
+        buf[(3 * r + k) * sp + lpos(bitrev(ixx, lx))] = T[k * nk + long(row0 + r) * nx + ixx];
+    }
+    __syncthreads();
+    lds_fft(buf, nx, lx, 3 * rpw, sp, tws);
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+        const int r = e / nx, ixx = e - r * nx;
+        const long i = long(row0 + r) * nx + ixx;
+        const cx R = scale * buf[(3 * r) * sp + lpos(ixx)], L = scale * buf[(3 * r + 1) * sp + lpos(ixx)];
+        const cx Z = scale * buf[(3 * r + 2) * sp + lpos(ixx)];
+        const double r1 = R.re, r2 = R.im, l1 = L.re, l2 = L.im;
+        Fn[0 * fpl + i] = float(0.5 * (r1 + l1));
+        Fn[1 * fpl + i] = float(0.5 * (r2 + l2));
+        Fn[3 * fpl + i] = float(0.5 * (l2 - r2));
+        Fn[4 * fpl + i] = float(0.5 * (r1 - l1));
+        Fn[2 * fpl + i] = float(Z.re);
+        Fn[5 * fpl + i] = float(Z.im);
+    }
+}
+
+int rows_per_wg(int ny) { int r = 1; while (r < 8 && ny / (2 * r) >= 128) r <<= 1; return r; }
+int cols_per_wg(int nx, int ny) {
+    int nc = 1;
+    while (nc < 8 && 2 * nc * ny <= kMaxModes * kT && nx / (2 * nc) >= 128) nc <<= 1;
+    while (nc * ny > kMaxModes * kT && nc > 1) nc >>= 1;
+    return nc;
+}
+int lanes(int work) { int t = 64; while (t < work && t < kT) t <<= 1; return t; }
 }  // namespace
 
-void plasma_modes(const ModeArgs& m) {
-    hipLaunchKernelGGL(k_plasma_modes, dim3(blocks_for(m.mv.nk, 256)), dim3(256), 0, stream(), m);
-    launch_check("plasma modes");
+void pass_rows_fwd(double* J, const TileGeom& tg, const FftPlan& p, cx* X, double* rl, double* rli, int i0, int j0) {
+    const int rpw = rows_per_wg(p.ny);
+    if (p.ny % rpw) throw std::runtime_error("pass rows: ny not a multiple of the row tile");
+    const size_t lds = (size_t(2 * rpw) * (p.nx + p.nx / 8) + p.nx / 2) * sizeof(cx);
+    hipLaunchKernelGGL(k_pass_rows_fwd, dim3(p.ny / rpw), dim3(lanes(rpw * p.nx)), lds, stream(), J, tg, p.nx, p.ny, p.lx,
+                       rpw, p.twx, X, rl, rli, i0, j0);
+    launch_check("pass rows forward");
+}
+void pass_cols(const ModeArgs& m, const FftPlan& p, const cx* X, cx* T, cx* jsn, int kind) {
+    const int nc = cols_per_wg(p.nx, p.ny);
+    if (nc * p.ny > kMaxModes * kT) throw std::runtime_error("pass columns: ny above " + std::to_string(kMaxModes * kT));
+    const size_t lds = (size_t(3 * nc) * (p.ny + p.ny / 8) + p.ny) * sizeof(cx);
+    hipLaunchKernelGGL(k_pass_cols, dim3(p.nx / nc), dim3(lanes(nc * p.ny)), lds, stream(), m, X, T, jsn, p.twy, nc, p.ly,
+                       kind);
+    launch_check("pass columns");
+}
+void pass_rows_inv(const cx* T, float* Fn, long fpl, const FftPlan& p) {
+    const int rpw = rows_per_wg(p.ny);
+    const size_t lds = (size_t(3 * rpw) * (p.nx + p.nx / 8) + p.nx / 2) * sizeof(cx);
+    hipLaunchKernelGGL(k_pass_rows_inv, dim3(p.ny / rpw), dim3(lanes(rpw * p.nx)), lds, stream(), T, Fn, fpl, p.nx, p.ny,
+                       p.lx, rpw, p.twx, 1.0 / (double(p.nx) * p.ny));
+    launch_check("pass rows inverse");
 }
 
 }  // namespace g3d
