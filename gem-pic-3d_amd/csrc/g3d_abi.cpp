@@ -164,7 +164,7 @@ int g3d_sim_timing(g3d_sim* sim, int enable) {
     return g3d::guard([&] {
         g3d::Sim& s = *sim->s;
         s.flush_timers();
-        s.timing = enable != 0;
+        s.timing = unsigned(enable);
         for (int k = 0; k < g3d::Sim::kFam; ++k) { s.fam_ms[k] = 0; s.fam_n[k] = 0; }
     });
 }

@@ -1,0 +1,123 @@
+// Beam deposit in the reference's summation order (src/sim_particles.cpp:56-107).
+//
+// The reference adds every particle's TSC contributions into jw one particle after the other; with FP64 addition
+// the order fixes the last bits, and the start field amplifies them (the relaxation's long step D = 100 dz and the
+// near-singular co-moving modes turn 1e-16 into ~1e-12).  Global atomics would sum in schedule order.  Here the
+// deposit is a sort instead:
+//   1. one lane per particle writes its contribution records: record r = ip R + slot (R = 9 on the Fourier y grid,
+//      18 on the free-hand grid: 9 node cells with j_x, j_z and 9 half-node cells with j_y), the cell key
+//      (group, half-cell, point) and the values; cells a particle does not touch get the key ~0;
+//   2. a stable radix sort of (key, r) groups each cell's records in particle order;
+//   3. one lane per cell (the head of its run of keys) sums the run in that order, from zero, as the reference
+//      does, and stores the sums into jw.
+// The result is the reference's single-thread jw, bit for bit.
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include "g3d_kernels.hpp"
+#include "g3d_runtime.hpp"
+
+namespace g3d {
+namespace {
+constexpr int kT = 256;
+constexpr unsigned long long kNone = ~0ull;
+constexpr int kCellBits = 40;                       // cell = hw nk + point < 2^40; bit 40: the half-node group
+
+__global__ void __launch_bounds__(kT) k_dep_emit(DepositArgs c, Parts P, long np, int R, unsigned long long* key,
+                                                 unsigned* rec, double* val) {
+    const long ip = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (ip >= np) return;
+    const long r0 = ip * R;
+    for (int s = 0; s < R; ++s) { key[r0 + s] = kNone; rec[r0 + s] = unsigned(r0 + s); }
+    if (P.alive[ip] != 1) return;
+    int m; double fz;
+    c.zax.locate(P.zeta[ip], m, fz);
+    const int hw = m - (c.a - 2);
+    if (hw < 0 || hw >= c.nh || m < 0 || m > c.N - 2) return;
+    const double px = P.px[ip], py = P.py[ip], pz = P.pz[ip];
+    const double g = sqrt(1 + px * px + py * py + pz * pz);
+    const double vx = px / g, vy = py / g, vz = pz / g;
+    const double f = P.q[ip] * P.w[ip] / (c.hcg[m + 2] * c.dx);
+    int ix[3], jn[3], jh[3]; double wx[3], wn[3], wh[3];
+    tsc_periodic(P.x[ip], c.dx, c.nx, ix, wx);
+    const unsigned long long base = (unsigned long long)hw * (unsigned long long)c.nk;
+    double* V = val + 3 * r0;
+    if (c.yf) {
+        tsc_periodic(P.y[ip], c.dy, c.ny, jn, wn);
+        for (int v = 0; v < 3; ++v)
+            for (int u = 0; u < 3; ++u) {
+                const int s = 3 * v + u;
+                const double w = f * wx[u] * wn[v] * c.ia[0];
+                key[r0 + s] = base + (unsigned long long)(long(jn[v]) * c.nx + ix[u]);
+                V[3 * s] = w * vx / vz; V[3 * s + 1] = w * vy / vz; V[3 * s + 2] = w;
+            }
+        return;
+    }
+    tsc_freehand(c.yax, P.y[ip], jn, wn, jh, wh);
+    for (int v = 0; v < 3; ++v) {
+        const bool nod = jn[v] >= 1 && jn[v] <= c.ny - 1, hal = jh[v] >= 0 && jh[v] <= c.ny - 1;
+        for (int u = 0; u < 3; ++u) {
+            const int s = 3 * v + u;
+            if (nod) {
+                const double w = f * wx[u] * wn[v] * c.ia[jn[v]];
+                key[r0 + s] = base + (unsigned long long)(long(jn[v]) * c.nx + ix[u]);
+                V[3 * s] = w * vx / vz; V[3 * s + 1] = 0; V[3 * s + 2] = w;
+            }
+            if (hal) {
+                key[r0 + 9 + s] = (1ull << kCellBits) | (base + (unsigned long long)(long(jh[v]) * c.nx + ix[u]));
+                V[3 * (9 + s)] = 0; V[3 * (9 + s) + 2] = 0;
+                V[3 * (9 + s) + 1] = f * wx[u] * wh[v] * c.id[jh[v]] * vy / vz;
+            }
+        }
+    }
+}
+
+// one lane per sorted record; the head of each run of equal keys sums the run in record (= particle) order
+__global__ void __launch_bounds__(kT) k_dep_sum(const unsigned long long* key, const unsigned* rec, const double* val,
+                                                long n, double* J, long nh, long nk, int yf) {
+    const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned long long k = key[i];
+    if (k == kNone || (i > 0 && key[i - 1] == k)) return;
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (long t = i; t < n && key[t] == k; ++t) {
+        const double* v = val + 3 * long(rec[t]);
+        s0 += v[0]; s1 += v[1]; s2 += v[2];
+    }
+    const bool half = (k >> kCellBits) & 1;
+    const long cell = long(k & ((1ull << kCellBits) - 1));
+    const long hw = cell / nk, p = cell - hw * nk;
+    if (half) { J[(1 * nh + hw) * nk + p] = s1; return; }
+    J[(0 * nh + hw) * nk + p] = s0;
+    if (yf) J[(1 * nh + hw) * nk + p] = s1;
+    J[(2 * nh + hw) * nk + p] = s2;
+}
+}  // namespace
+
+void bunch_deposit(const DepositArgs& a, Parts p, long np, DepositScratch& s) {
+    if (np <= 0) return;
+    const int R = a.yf ? 9 : 18;
+    const long n = np * R;
+    if (n > long(0xffffffffu)) throw std::runtime_error("beam deposit: too many contribution records");
+    if (long((1ull << kCellBits) / (unsigned long long)a.nk) < a.nh) throw std::runtime_error("beam deposit: grid too large");
+    if (s.key0.size() < size_t(n)) {
+        const size_t cap = size_t(n) + size_t(n) / 4;
+        s.key0.resize(cap); s.key1.resize(cap); s.rec0.resize(cap); s.rec1.resize(cap); s.val.resize(3 * cap);
+    }
+    hipLaunchKernelGGL(k_dep_emit, dim3(blocks_for(np, kT)), dim3(kT), 0, stream(), a, p, np, R, s.key0.data(),
+                       s.rec0.data(), s.val.data());
+    launch_check("beam deposit records");
+    // bits 0 .. 41: the cell, the group bit, and above every valid key the low bits of the empty key ~0
+    const unsigned end_bit = kCellBits + 2;
+    size_t bytes = 0;
+    hip_check(rocprim::radix_sort_pairs(nullptr, bytes, s.key0.data(), s.key1.data(), s.rec0.data(), s.rec1.data(),
+                                        n, 0, end_bit, stream()), "beam deposit sort size");
+    if (s.tmp.size() < bytes) s.tmp.resize(bytes + bytes / 4 + 256);
+    bytes = s.tmp.size();
+    hip_check(rocprim::radix_sort_pairs(s.tmp.data(), bytes, s.key0.data(), s.key1.data(), s.rec0.data(),
+                                        s.rec1.data(), n, 0, end_bit, stream()), "beam deposit sort");
+    hipLaunchKernelGGL(k_dep_sum, dim3(blocks_for(n, kT)), dim3(kT), 0, stream(), s.key1.data(), s.rec1.data(),
+                       s.val.data(), n, a.J, long(a.nh), a.nk, a.yf);
+    launch_check("beam deposit sums");
+}
+
+}  // namespace g3d

@@ -3,6 +3,7 @@
 #include "g3d_engine.hpp"
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <random>
@@ -130,7 +131,6 @@ Sim::Sim(const Config& c) : cfg(c) {
     N = cfg.nzeta; nk = long(cfg.nx) * cfg.ny;
     dx = cfg.Lx / cfg.nx;
     if (int(cfg.zeta.size()) != N || int(cfg.ynode.size()) != cfg.ny + 1) throw std::runtime_error("grids not set");
-    if (!cfg.y_fourier) throw std::runtime_error("g3d engine: the free-hand y grid is not available in this build");
     a = 0; ns = N; b = N;
     const int nx = cfg.nx, ny = cfg.ny;
     std::vector<double> hkx(nx), hsx(nx), hf(nx), hm(nx);
@@ -149,8 +149,9 @@ Sim::Sim(const Config& c) : cfg(c) {
     }
     auto up = [](DevBuf<double>& d, const std::vector<double>& h) { d.resize(h.size()); d.from_host(h.data(), 0, h.size()); };
     up(kxd, hkx); up(kxs, hsx); up(filt, hf); up(maskx, hm);
-    yf = true;
-    {   // Fourier y (src/sim_setup.cpp:63-82)
+    yf = cfg.y_fourier;
+    if (!yf) setup_y();
+    else {   // Fourier y (src/sim_setup.cpp:63-82)
         dy = cfg.Ly / ny;
         std::vector<double> hky(ny), hsy(ny), hfy(ny), hmy(ny), hi(ny, 1 / dy);
         for (int j = 0; j < ny; ++j) {
@@ -191,7 +192,7 @@ Sim::Sim(const Config& c) : cfg(c) {
     T3.resize(3 * size_t(ns) * pl);
     jw.resize(3 * (ns + 3) * pl);
     F.resize(6 * (ns + 2) * pl);
-    fft_plan_init(fft, cfg.nx, cfg.ny, true);
+    fft_plan_init(fft, cfg.nx, cfg.ny, yf);
 
     laser.on = cfg.laser_on; laser.a0 = cfg.a0; laser.kL = cfg.kL; laser.tau = cfg.tau; laser.cep = cfg.cep;
     laser.kx = std::sin(cfg.theta); laser.kz = std::cos(cfg.theta); laser.pol = cfg.pol;
@@ -209,7 +210,7 @@ Sim::Sim(const Config& c) : cfg(c) {
     counters.resize(4);
     cent_dev.resize(3);
     setup_ions();
-    if (cfg.plasma_on) setup_tiles();
+    if (cfg.plasma_on && yf) setup_tiles();
 }
 
 // Tiles of TX x TY cells (16 x 8 on large grids) with a margin of M nodes around the TSC reach of the tile; a
@@ -233,6 +234,131 @@ Sim::~Sim() {
     fft_plan_free(fft);
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
     for (auto& p : ev_pending) { (void)hipEventDestroy(p.second.first); (void)hipEventDestroy(p.second.second); }
+}
+
+// Symmetric tridiagonal eigenproblem by the implicit QL method with Wilkinson shifts (EISPACK tql2: Bowdler,
+// Martin, Reinsch and Wilkinson, Numer. Math. 11 (1968) 293).  d: the diagonal, e[k]: the coupling of k and k+1;
+// on return d holds the eigenvalues (unordered) and column m of z (z[k n + m]) the unit eigenvector of d[m].
+// The floating-point operations run in the order of src/grid.cpp:56-100: the smallest eigenvalues of -Lap_y on a
+// graded grid are fixed by any method only to about 1e-12 relative, so the co-moving field of those modes stays
+// within the FP64 parity bound only with the reference's rounding.
+static bool ql_implicit(std::vector<double>& d, std::vector<double> e, std::vector<double>& z) {
+    const int n = int(d.size());
+    z.assign(size_t(n) * n, 0.0);
+    for (int k = 0; k < n; ++k) z[size_t(k) * n + k] = 1;
+    if (n == 0) return true;
+    e.resize(n);
+    e[n - 1] = 0;
+    constexpr double kEps = 2.2e-16;
+    // end of the unreduced block that starts at l: the first m >= l whose coupling is negligible
+    auto block_end = [&](int l) {
+        int m = l;
+        while (m < n - 1 && !(std::fabs(e[m]) <= kEps * (std::fabs(d[m]) + std::fabs(d[m + 1])))) ++m;
+        return m;
+    };
+    // plane rotation of the columns i, i+1 of z
+    auto rotate = [&](int i, double sn, double cs) {
+        for (int k = 0; k < n; ++k) {
+            double* row = &z[size_t(k) * n];
+            const double t = row[i + 1];
+            row[i + 1] = sn * row[i] + cs * t;
+            row[i] = cs * row[i] - sn * t;
+        }
+    };
+    for (int l = 0; l < n; ++l) {
+        for (int sweep = 0;; ++sweep) {
+            const int m = block_end(l);
+            if (m == l) break;
+            if (sweep == 60) return false;
+            // shifted start of the chase: the eigenvalue of the leading 2 x 2 block nearer to d[l]
+            double g = (d[l + 1] - d[l]) / (2 * e[l]);
+            double r = std::hypot(g, 1.0);
+            g = d[m] - d[l] + e[l] / (g + std::copysign(r, g));
+            double sn = 1, cs = 1, p = 0;
+            bool split = false;
+            for (int i = m - 1; i >= l; --i) {       // chase the bulge from m up to l
+                const double f = sn * e[i], bb = cs * e[i];
+                r = std::hypot(f, g);
+                e[i + 1] = r;
+                if (r == 0) { d[i + 1] -= p; e[m] = 0; split = true; break; }
+                sn = f / r; cs = g / r;
+                g = d[i + 1] - p;
+                r = (d[i] - g) * sn + 2 * cs * bb;
+                p = sn * r;
+                d[i + 1] = g + p;
+                g = cs * r - bb;
+                rotate(i, sn, cs);
+            }
+            if (split) continue;
+            d[l] -= p; e[l] = g; e[m] = 0;
+        }
+    }
+    return true;
+}
+
+// Free-hand y (src/sim_setup.cpp:86-143): nodes 0..ny (walls at 0 and ny), half-nodes between; 1/Delta_j,
+// 1/A_j, the sponge on nodes and half-nodes, the LU factors of k_x^2 - Lap_y (bound field of the sponge) and the
+// eigenvectors of -Lap_y (co-moving field).
+void Sim::setup_y() {
+    const int ny = cfg.ny, nx = cfg.nx;
+    if (ny < 4) throw std::runtime_error("free-hand y grid: at least 4 cells between the walls");
+    const std::vector<double>& Y = cfg.ynode;
+    dy = 0;
+    hDel.assign(ny, 0); hA.assign(ny, 0);
+    for (int j = 0; j < ny; ++j) hDel[j] = Y[j + 1] - Y[j];
+    hA[0] = 0.5 * hDel[0];
+    for (int j = 1; j < ny; ++j) hA[j] = 0.5 * (hDel[j - 1] + hDel[j]);
+    std::vector<double> hiD(ny), hiA(ny), hmn(ny), hmh(ny);
+    auto prof = [&](double y) {
+        const double dd = 0.5 * cfg.Ly - std::fabs(y), w = cfg.sponge_y;
+        if (w <= 0 || dd >= w) return 1.0;
+        const double r = 1 - dd / w;
+        return 1 - cfg.sponge_rate * r * r;
+    };
+    for (int j = 0; j < ny; ++j) {
+        hiD[j] = 1 / hDel[j]; hiA[j] = 1 / hA[j];
+        hmn[j] = prof(Y[j]); hmh[j] = prof(0.5 * (Y[j] + Y[j + 1]));
+    }
+    auto up = [](DevBuf<double>& d, const std::vector<double>& h) { d.resize(h.size()); d.from_host(h.data(), 0, h.size()); };
+    up(yn, Y); up(iD, hiD); up(iA, hiA); up(masky_n, hmn); up(masky_h, hmh);
+    yax.X = yn.data(); yax.n = ny + 1;
+    bool uni = true;
+    for (int j = 0; j < ny; ++j) uni = uni && hDel[j] == hDel[0];
+    yax.uniform = uni; yax.x0 = Y[0]; yax.ih = 1 / hDel[0];
+    std::vector<double> kx(nx);
+    kxd.to_host(kx.data(), 0, nx);
+    std::vector<double> hcp(size_t(nk), 0.0), hid(size_t(nk), 0.0);
+    for (int i = 0; i < nx; ++i) {
+        double cprev = 0;
+        for (int j = 1; j < ny; ++j) {
+            const double lo = j > 1 ? -hiA[j] * hiD[j - 1] : 0, upv = j < ny - 1 ? -hiA[j] * hiD[j] : 0;
+            const double den = kx[i] * kx[i] + hiA[j] * (hiD[j - 1] + hiD[j]) - lo * cprev;
+            hid[size_t(j) * nx + i] = 1 / den;
+            hcp[size_t(j) * nx + i] = cprev = upv / den;
+        }
+    }
+    up(pcp, hcp); up(pid, hid);
+    if (cfg.self_fields) {
+        const int M = ny - 1;
+        std::vector<double> d(M), e(M, 0.0), z;
+        for (int j = 1; j < ny; ++j) d[j - 1] = hiA[j] * (hiD[j - 1] + hiD[j]);
+        for (int j = 1; j < ny - 1; ++j) e[j - 1] = -hiD[j] / std::sqrt(hA[j] * hA[j + 1]);
+        if (!ql_implicit(d, e, z)) throw std::runtime_error("eigenvectors of the y Laplacian: no convergence");
+        std::vector<double> hf(size_t(ny) * ny, 0.0), hb(size_t(ny) * ny, 0.0), hl(ny, 0.0);
+        for (int m = 0; m < M; ++m) {
+            hl[m + 1] = d[m];
+            for (int j = 1; j < ny; ++j) {
+                const double u = z[size_t(j - 1) * M + m], sq = std::sqrt(hA[j]);
+                hf[size_t(m + 1) * ny + j] = u * sq;
+                hb[size_t(j) * ny + m + 1] = u / sq;
+            }
+        }
+        up(evf, hf); up(evb, hb); up(evl, hl);
+    }
+    scp.resize(2 * size_t(nk));
+    // the y wave numbers of the Fourier form are not used; filty = 1 keeps the shared kernels valid
+    std::vector<double> zero(ny, 0.0), one(ny, 1.0);
+    up(kyd, zero); up(kys, zero); up(filty, one);
 }
 
 void Sim::setup_ions() {   // src/sim_ions.cpp:68-87
@@ -271,6 +397,11 @@ MarchView Sim::mview(double D) {
     v.D = D; v.a = a; v.N = N; v.nx = cfg.nx; v.nk = nk;
     return v;
 }
+LineView Sim::lview(double D) {
+    LineView L;
+    L.mv = mview(D); L.iD = iD.data(); L.iA = iA.data(); L.scp = scp.data(); L.ny = cfg.ny;
+    return L;
+}
 FieldView Sim::fview() {
     FieldView f;
     f.F = F.data(); f.a = a; f.ns = ns; f.nx = cfg.nx; f.ny = cfg.ny; f.yf = yf; f.self = cfg.self_fields; f.nk = nk;
@@ -280,7 +411,7 @@ FieldView Sim::fview() {
 
 // ---------------------------------------------------------------- timing
 void Sim::tic(int fam) {
-    if (!timing) return;
+    if (!(timing >> fam & 1u)) return;
     hipEvent_t e;
     if (!ev_pool.empty()) { e = ev_pool.back(); ev_pool.pop_back(); }
     else hip_check(hipEventCreate(&e), "hipEventCreate");
@@ -288,7 +419,7 @@ void Sim::tic(int fam) {
     ev_open = e; ev_open_fam = fam;
 }
 void Sim::toc(int fam) {
-    if (!timing || ev_open_fam != fam) return;
+    if (!(timing >> fam & 1u) || ev_open_fam != fam) return;
     hipEvent_t e;
     if (!ev_pool.empty()) { e = ev_pool.back(); ev_pool.pop_back(); }
     else hip_check(hipEventCreate(&e), "hipEventCreate");
@@ -343,8 +474,8 @@ std::vector<g3d_particle> Sim::host_strm() {
 // bunch loading with the reference's generator and draw order (src/sim_bunches.cpp:92-217, built-in Gaussian
 // profiles); libstdc++'s mt19937_64 and normal_distribution give the reference's particles bit for bit
 void Sim::load_bunch() {
-    const double ylo = -1e300, yhi = 1e300;
-    const double yoff = 0.5 * cfg.Ly;
+    const double ylo = yf ? -1e300 : cfg.ynode[1], yhi = yf ? 1e300 : cfg.ynode[cfg.ny - 1];
+    const double yoff = yf ? 0.5 * cfg.Ly : 0;   // Fourier y: box coordinate, physical y = p.y + Y0
     std::vector<g3d_particle> mine;
     if (cfg.nmacro > 0) {
         std::mt19937_64 rng(cfg.seed);
@@ -426,7 +557,7 @@ void Sim::push(double s0, double s1) {
     c.vz_min = cfg.plasma_on ? cfg.vz_min : 0; c.freeze = cfg.freeze_bunch;
     c.hmin = 1e300;
     for (int m = 0; m + 1 < N; ++m) c.hmin = std::min(c.hmin, cfg.zeta[m + 1] - cfg.zeta[m]);
-    c.ylo = -1e300; c.yhi = 1e300;
+    c.ylo = yf ? -1e300 : cfg.ynode[1]; c.yhi = yf ? 1e300 : cfg.ynode[cfg.ny - 1];
     tic(F_PUSH);
     bunch_push(c, beam.view(), np, s0);
     toc(F_PUSH);
@@ -439,17 +570,22 @@ void Sim::currents() {
     DepositArgs d;
     d.zax = zax; d.yax = yax; d.hcg = hcg.data(); d.ia = iA.data(); d.id = iD.data(); d.J = jw.data(); d.dx = dx;
     d.dy = dy; d.nh = ns + 3; d.a = a; d.N = N; d.nx = cfg.nx; d.ny = cfg.ny; d.yf = yf; d.nk = nk;
-    bunch_deposit(d, beam.view(), np);
+    bunch_deposit(d, beam.view(), np, dep_scratch);
     const SpecGeom g = geom();
     currents_zfilter(g, jw.data(), hcg.data(), cfg.zeta_filter, zax.uniform, S1.data(), jkz.data());
     fft_run(fft, S1.data(), ns + 1, false);
     fft_run(fft, jkz.data(), ns + 1, false);
     const bool keep0 = cfg.plasma_on && cfg.box_mean && !starting_;
-    currents_unpack(g, S1.data(), jkx.data(), jky.data(), jkz.data(), keep0);
+    if (yf) currents_unpack(g, S1.data(), jkx.data(), jky.data(), jkz.data(), keep0);
+    else fh_currents(S1.data(), jkx.data(), jky.data(), jkz.data(), ns + 1, lview(0.0), filt.data(), cfg.k_filter,
+                     keep0, cfg.Ly);
     toc(F_CUR);
 }
 
-void Sim::march(double D) { march_all(mview(D), a, b); }
+void Sim::march(double D) {
+    if (yf) march_all(mview(D), a, b);
+    else march_lines(lview(D), a, b);
+}
 
 // ---------------------------------------------------------------- sponge and real fields (src/sim_fields.cpp:156-307)
 void Sim::sponge_and_real_fields() {
@@ -458,15 +594,24 @@ void Sim::sponge_and_real_fields() {
     const long ts = long(ns) * nk;
     const bool sp = cfg.sponge > 0 || cfg.sponge_y > 0;
     const double opb = 1 + beta_b;
+    FhSpongeArgs fs{};
+    const LineView L = lview(0.0);
+    if (!yf) {
+        fs.jz = jkz.data(); fs.kx = kxd.data(); fs.pc = pcp.data(); fs.pd = pid.data(); fs.hc = hcg.data();
+        fs.wl = wlo.data(); fs.wu = whi.data(); fs.g2 = gm2_b; fs.opb = opb; fs.ns = ns; fs.N = N; fs.a = a;
+    }
     if (sp) {
-        sponge_pack(g, Rx.data(), Ry.data(), jkz.data(), hcg.data(), wlo.data(), whi.data(), gm2_b, opb, S1.data(),
-                    S2.data());
+        if (yf)
+            sponge_pack(g, Rx.data(), Ry.data(), jkz.data(), hcg.data(), wlo.data(), whi.data(), gm2_b, opb, S1.data(),
+                        S2.data());
+        else fh_sponge_psi(fs, L, Rx.data(), Ry.data(), S1.data(), S2.data());
         fft_run(fft, S1.data(), ns, true);
         sponge_mask(g, S1.data(), maskx.data(), masky_n.data(), masky_h.data(), ns);
         fft_run(fft, S1.data(), ns, false);
+        if (!yf) fh_sponge_unpack(fs, L, S1.data(), S2.data(), Rx.data(), Ry.data());
     }
-    fields_pack3(g, sp ? S1.data() : nullptr, S2.data(), opb, Rx.data(), Ry.data(), Lx.data(), Ly.data(), Ez.data(),
-                 Bz.data(), T3.data(), ts, ns);
+    fields_pack3(g, sp && yf ? S1.data() : nullptr, S2.data(), opb, Rx.data(), Ry.data(), Lx.data(), Ly.data(),
+                 Ez.data(), Bz.data(), T3.data(), ts, ns);
     fft_run(fft, T3.data(), 3 * ns, true);
     fields_to_F(T3.data(), ts, F.data(), long(ns + 2) * nk, nk, ns);
     toc(F_SPONGE);
@@ -488,8 +633,18 @@ void Sim::comoving_field() {
     const double gm2 = 1 - beta * beta;
     if (!comoving_c_) { beta_b = beta; gm2_b = gm2; }
     const SpecGeom g = geom();
-    comoving_solve(g, jkz.data(), hcg.data(), wlo.data(), whi.data(), beta, gm2, S1.data(), S2.data());
-    comoving_R(g, S2.data(), beta, Rx.data(), Ry.data(), Ez.data(), Bz.data());
+    if (yf) {
+        comoving_solve(g, jkz.data(), hcg.data(), wlo.data(), whi.data(), beta, gm2, S1.data(), S2.data());
+        comoving_R(g, S2.data(), beta, Rx.data(), Ry.data(), Ez.data(), Bz.data());
+        return;
+    }
+    // free-hand y: the half-cell charges in the eigenvectors of -Lap_y (T3 as scratch), one solve along zeta per
+    // (k_x, eigenvalue), back to the nodes
+    const int nx = cfg.nx, ny = cfg.ny;
+    fh_eigen_forward(jkz.data(), evf.data(), T3.data(), ns + 1, nx, ny, nk);
+    comoving_solve_ev(g, T3.data(), evl.data(), hcg.data(), wlo.data(), whi.data(), beta, gm2, S1.data(), S2.data());
+    fh_eigen_back(S2.data(), evb.data(), S1.data(), ns, nx, ny, nk);
+    fh_comoving_R(S1.data(), kxd.data(), iD.data(), beta, Rx.data(), Ry.data(), Ez.data(), Bz.data(), ns, nx, ny, nk);
 }
 
 void Sim::relax_start_field() {
@@ -520,10 +675,10 @@ void Sim::add_grid_lasers() {
         la.zc = Lz.zeta_c; la.tau = Lz.tau; la.kL = Lz.kL; la.cep = Lz.cep;
         la.rc = std::cos(Lz.pol_angle); la.rs = std::sin(Lz.pol_angle); la.dx = dx; la.dy = dy; la.X0 = X0; la.Y0 = Y0;
         la.circular = Lz.circular; la.helicity = Lz.helicity; la.nx = cfg.nx; la.ny = cfg.ny; la.nk = nk;
-        la.zeta = zs.data();
+        la.zeta = zs.data(); la.yn = yf ? nullptr : yn.data();
         grid_laser_fill(la, T3.data(), a, ns);
         fft_run(fft, T3.data(), ns, false);
-        grid_laser_add(g, T3.data(), Rx.data(), Ry.data(), ns);
+        grid_laser_add(g, T3.data(), Rx.data(), Ry.data(), ns, !yf);
     }
 }
 
@@ -548,7 +703,11 @@ void Sim::load_plasma(double s0, double s1) {
     const bool tiled = tiles.ntile > 0;
     v.reserve(size_t(nx) * ny * px * py * (1 + cfg.ions.size()) + joining_.size());
     for (int j = 0; j < ny; ++j) {
-        const double y0 = j * dy, hy = dy;
+        double y0 = j * dy, hy = dy;
+        if (!yf) {
+            if (j == 0 || j == ny - 1) continue;      // no plasma in the cells at the walls
+            y0 = cfg.ynode[j]; hy = cfg.ynode[j + 1] - cfg.ynode[j];
+        }
         for (int i = 0; i < nx; ++i) {
             const int t = tiled ? (j / tiles.TY) * tiles.ntx + i / tiles.TX : 0;
             for (int b2 = 0; b2 < py; ++b2)
@@ -606,11 +765,15 @@ void Sim::stream_spectra() {
     ModeArgs m{};
     m.mv = mview(0.0);
     m.kxs = kxs.data(); m.kys = kys.data(); m.filt = filt.data(); m.filty = filty.data();
-    m.a2 = cfg.smooth_length * cfg.smooth_length; m.keep0 = cfg.box_mean; m.ny = cfg.ny;
-    pass_cols(m, fft, sp3.data(), tp3.data(), jsn.data(), 2);
+    m.a2 = cfg.smooth_length * cfg.smooth_length; m.keep0 = cfg.box_mean; m.ny = cfg.ny; m.kf = cfg.k_filter;
+    if (yf) pass_cols(m, fft, sp3.data(), tp3.data(), jsn.data(), 2);
+    else pass_lines(m, lview(0.0), sp3.data(), tp3.data(), jsn.data(), 2);
 }
 
-void Sim::march_slice(int n, double D) { march_one(mview(D), n, nullptr, nullptr, nullptr, nullptr, 0); }
+void Sim::march_slice(int n, double D) {
+    if (yf) march_one(mview(D), n, nullptr, nullptr, nullptr, nullptr, 0);
+    else march_lines(lview(D), n, n + 1);
+}
 
 void Sim::real_fields_slice(int n) {
     const SpecGeom g = geom();
@@ -625,7 +788,8 @@ void Sim::real_fields_slice(int n) {
 StreamArgs Sim::stream_args(double s0, double s1) {
     const double Delta = s1 - s0;
     StreamArgs sc;
-    sc.fv = fview(); sc.laser = laser; sc.X0 = X0; sc.Y0 = Y0; sc.zc = cfg.zeta_center; sc.ylo = -1e300; sc.yhi = 1e300;
+    sc.fv = fview(); sc.laser = laser; sc.X0 = X0; sc.Y0 = Y0; sc.zc = cfg.zeta_center;
+    sc.ylo = yf ? -1e300 : cfg.ynode[1]; sc.yhi = yf ? 1e300 : cfg.ynode[cfg.ny - 1];
     sc.zs = zs.data(); sc.J = jsr.data(); sc.ia = iA.data(); sc.id = iD.data();
     sc.Delta = Delta; sc.s1 = s1; sc.sTrap = s1 + (cfg.trap_advance - 1) * Delta; sc.vz_beam = cfg.vz_beam;
     sc.ions = ion_dev.data(); sc.adk = adk_dev.data(); sc.born = born.cap ? born.view() : Parts{};
@@ -677,6 +841,15 @@ void Sim::march_plasma(double s0, double s1) {
     m.jkx = jkx.data(); m.jky = jky.data(); m.jkz = jkz.data();
     m.T = tp3.data(); m.kxs = kxs.data(); m.kys = kys.data(); m.filt = filt.data(); m.filty = filty.data();
     m.a2 = cfg.smooth_length * cfg.smooth_length; m.extrap = extrap; m.keep0 = cfg.box_mean; m.ny = cfg.ny;
+    m.kf = cfg.k_filter;
+    const LineView L = lview(D);
+    // the columns (Fourier y) or lines (free-hand y) of a pass
+    static const bool split = [] { const char* e = getenv("G3D_PASS_SPLIT"); return e && atoi(e) > 0; }();
+    auto pass_work = [&](cx* X, int kind) {
+        if (!yf) pass_lines(m, L, X, tp3.data(), nullptr, kind);
+        else if (split) pass_modes(m, fft, X, tp3.data(), nullptr, kind);
+        else pass_cols(m, fft, X, tp3.data(), nullptr, kind);
+    };
     const long fpl = long(ns + 2) * nk;
     auto real_fields = [&](int n) {
         tic(F_IFFT);
@@ -687,7 +860,7 @@ void Sim::march_plasma(double s0, double s1) {
     m.pk = nullptr; m.n = N - 1; m.first = 1; m.last = 1; m.top = 1; m.wex = 0; m.qcorr = 0;
     m.bRx = m.bRy = m.bEz = m.bBz = nullptr;
     tic(F_MODES);
-    pass_cols(m, fft, nullptr, tp3.data(), nullptr, 1);
+    pass_work(sp3.data(), 1);
     toc(F_MODES);
     real_fields(N - 1);
 
@@ -699,7 +872,7 @@ void Sim::march_plasma(double s0, double s1) {
     const int passes = cfg.iterations;
     const int nx = cfg.nx, ny = cfg.ny;
     const int i0 = ((int(std::lround(-X0 / dx)) % nx) + nx) % nx;
-    const int j0 = ((int(std::lround(-Y0 / dy)) % ny) + ny) % ny;
+    const int j0 = yf ? ((int(std::lround(-Y0 / dy)) % ny) + ny) % ny : ny / 2;
     bool have_dl = false;
     long born_base = nstrm;
     long nlist = nstrm;          // particles in the list (freed electrons are appended on the device)
@@ -724,7 +897,7 @@ void Sim::march_plasma(double s0, double s1) {
             m.qcorr = D * wlo_h[n + 1];
             if (passes > 1) { m.bRx = bk.data(); m.bRy = bk.data() + pl; m.bEz = bk.data() + 2 * pl; m.bBz = bk.data() + 3 * pl; }
             tic(F_MODES);
-            pass_cols(m, fft, sp3.data(), tp3.data(), nullptr, 0);
+            pass_work(sp3.data(), 0);
             toc(F_MODES);
             real_fields(n);
         }

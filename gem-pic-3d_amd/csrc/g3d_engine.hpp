@@ -108,11 +108,14 @@ public:
 
     // device arrays in the reference's layouts
     DevBuf<cx> Rx, Ry, Ez, Bz, Lx, Ly, j0x, j0y, jkx, jky, jkz, S1, S2, T3;
+    DepositScratch dep_scratch;      // ordered beam deposit
     DevBuf<cx> js0x, js0y, js0z, jsn, jsin, jsi0, jpred, dlt, bk, sp3, tp3;
     DevBuf<double> jw, jsr, rho_lines, rho_lines_i;
     DevBuf<float> F;
     DevBuf<double> kxd, kxs, filt, maskx, kyd, kys, filty, masky_n, masky_h, iD, iA, zs, hcg, wlo, whi, wsl;
-    std::vector<double> wlo_h, hcg_h;
+    std::vector<double> wlo_h, hcg_h, hDel, hA;
+    // free-hand y: node table, LU factors of k_x^2 - Lap_y, eigenvectors of -Lap_y, Thomas factors of the lines
+    DevBuf<double> yn, pcp, pid, evf, evb, evl, scp;
     Axis zax{}, yax{};
     FftPlan fft;
     PartStore beam, strmA, strmB, born;
@@ -131,7 +134,7 @@ public:
     unsigned long long id0_ = 0;
 
     // timing of kernel families with HIP events on the library stream
-    bool timing = false;
+    unsigned timing = 0;              // bit f: time family f
     static constexpr int kFam = 8;
     double fam_ms[kFam] = {};
     long fam_n[kFam] = {};
@@ -146,8 +149,10 @@ public:
 private:
     SpecGeom geom() const;
     MarchView mview(double D);
+    LineView lview(double D);
     FieldView fview();
     void setup_ions();
+    void setup_y();
     void setup_tiles();
     void ensure_strm(size_t need);
 };

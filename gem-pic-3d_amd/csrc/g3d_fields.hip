@@ -180,16 +180,10 @@ __global__ void k_sspec_unpack(SpecGeom g, const cx* pk, cx* ox, double a2, int 
 
 // ---------------------------------------------------------------- grid laser (src/sim_fields.cpp:321-374)
 // complex Gaussian beam E = amp env exp(-(x^2 + y^2)/(w^2 q)) / q car at z = 0, q = 1 - i z_focus/z_R
-__global__ void k_glaser_fill(GridLaserArgs a, cx* T, int n0, long tot) {
-    const long idx = long(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (idx >= tot) return;
-    const int l = int(idx / a.nk);
-    const long i = idx % a.nk;
-    const int ixx = int(i % a.nx), j = int(i / a.nx), n = n0 + l;
+// the paraxial Gaussian at transverse point (xp, yp) of slice n: 2 E_x, 2 E_y
+__device__ void glaser_field(const GridLaserArgs& a, int n, double xp, double yp, double& ex, double& ey) {
     const double d = a.zeta[n] - a.zc;
     const double env = exp(-2 * 0.69314718055994531 * d * d / (a.tau * a.tau));
-    if (fabs(env) < 1e-12) { T[idx] = mk(0); return; }
-    const double xp = ixx * a.dx + a.X0, yp = j * a.dy + a.Y0;
     // 1/q with q = 1 + i qi
     const double qi = (0.0 - a.zfoc) / a.zR;
     const double qq = 1 + qi * qi;
@@ -201,17 +195,40 @@ __global__ void k_glaser_fill(GridLaserArgs a, cx* T, int n0, long tot) {
     const double ph = a.kL * d + a.cep;
     const cx car = mk(cos(ph), sin(ph));
     const cx E = (a.amp * env) * (e1 * iq) * car;
-    double ex, ey;
     if (a.circular) { ex = E.re; ey = a.helicity * E.im; }
     else { ex = E.re * a.rc; ey = E.re * a.rs; }
-    T[idx] = mk(2 * ex, 2 * ey);
 }
-__global__ void k_glaser_add(SpecGeom g, const cx* T, cx* rx, cx* ry, long tot) {
+// R_x + i R_y = 2 E_perp on the grid; free-hand y (yn != null): R_x on the nodes (zero on the wall), R_y on the
+// half-nodes
+__global__ void k_glaser_fill(GridLaserArgs a, cx* T, int n0, long tot) {
+    const long idx = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= tot) return;
+    const int l = int(idx / a.nk);
+    const long i = idx % a.nk;
+    const int ixx = int(i % a.nx), j = int(i / a.nx), n = n0 + l;
+    const double d = a.zeta[n] - a.zc;
+    const double env = exp(-2 * 0.69314718055994531 * d * d / (a.tau * a.tau));
+    if (fabs(env) < 1e-12) { T[idx] = mk(0); return; }
+    const double xp = ixx * a.dx + a.X0;
+    double ex, ey;
+    if (!a.yn) {
+        glaser_field(a, n, xp, j * a.dy + a.Y0, ex, ey);
+        T[idx] = mk(2 * ex, 2 * ey);
+        return;
+    }
+    double ex2, ey2;
+    glaser_field(a, n, xp, a.yn[j], ex, ey);
+    glaser_field(a, n, xp, 0.5 * (a.yn[j] + a.yn[j + 1]), ex2, ey2);
+    T[idx] = mk(j == 0 ? 0.0 : 2 * ex, 2 * ey2);
+}
+__global__ void k_glaser_add(SpecGeom g, const cx* T, cx* rx, cx* ry, long tot, int fh) {
     const long idx = long(blockIdx.x) * blockDim.x + threadIdx.x;
     if (idx >= tot) return;
     const long i = idx % g.nk, base = idx - i;
+    const int ixx = int(i % g.nx);
+    const long m = fh ? idx - ixx + (g.nx - ixx) % g.nx : base + mirror(i, g.nx, g.ny);
     cx X, Y;
-    unpack_pair(T[idx], conjg(T[base + mirror(i, g.nx, g.ny)]), X, Y);
+    unpack_pair(T[idx], conjg(T[m]), X, Y);
     rx[idx] = rx[idx] + X; ry[idx] = ry[idx] + Y;
 }
 
@@ -233,12 +250,13 @@ __global__ void k_roll(const float* f, float* t, long n, long nk, int nx, int ny
 }
 
 // ---------------------------------------------------------------- co-moving field (src/sim_fields.cpp:76-150)
-__global__ void k_comoving(SpecGeom g, const cx* e1, const double* hc, const double* wl, const double* wu, double beta,
-                           double gm2, cx* cp, cx* dp) {
+// evt: eigenvalue per slot s of the free-hand y grid (slot 0 unused); null: Fourier y, k_y^2
+__global__ void k_comoving(SpecGeom g, const cx* e1, const double* evt, const double* hc, const double* wl,
+                           const double* wu, double beta, double gm2, cx* cp, cx* dp) {
     const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= g.nk) return;
     const int s = int(i / g.nx), ixx = int(i % g.nx);
-    const double ev = g.ky[s] * g.ky[s];
+    const double ev = evt ? evt[s] : g.ky[s] * g.ky[s];
     cx cprev = mk(0), dprev = mk(0);
     for (int l = 0; l < g.ns; ++l) {
         const long o = long(l) * g.nk + i;
@@ -246,7 +264,7 @@ __global__ void k_comoving(SpecGeom g, const cx* e1, const double* hc, const dou
         const double hm = hc[n + 1], hp = hc[n + 2];
         const double lo = -gm2 * 2 / ((hm + hp) * hm), up = -gm2 * 2 / ((hm + hp) * hp);
         const double dg = g.kx[ixx] * g.kx[ixx] + ev + gm2 * 2 / (hm * hp);
-        if (dg == 0) { cp[o] = mk(0); dp[o] = mk(0); continue; }
+        if ((evt && s == 0) || dg == 0) { cp[o] = mk(0); dp[o] = mk(0); continue; }
         const cx rhs = (1 / beta) * (wl[n] * e1[o] + wu[n] * e1[o + g.nk]);
         if (n == 0) { cprev = mk(0); dprev = mk(0); }
         const cx den = mk(dg) - lo * cprev;
@@ -333,8 +351,8 @@ void sspec_unpack(const SpecGeom& g, const cx* pk, cx* jsn, double a2, int keep0
 void grid_laser_fill(const GridLaserArgs& a, cx* T, int n0, int count) {
     G3D_LAUNCH(k_glaser_fill, long(count) * a.nk, a, T, n0, long(count) * a.nk);
 }
-void grid_laser_add(const SpecGeom& g, const cx* T, cx* Rx, cx* Ry, int count) {
-    G3D_LAUNCH(k_glaser_add, long(count) * g.nk, g, T, Rx, Ry, long(count) * g.nk);
+void grid_laser_add(const SpecGeom& g, const cx* T, cx* Rx, cx* Ry, int count, int fh) {
+    G3D_LAUNCH(k_glaser_add, long(count) * g.nk, g, T, Rx, Ry, long(count) * g.nk, fh);
 }
 void rotate_planes(cx* A, long n, long nk, int nx, const double* kxs, const double* kys, double sx, double sy, int ys) {
     G3D_LAUNCH(k_rotate, n, A, n, nk, nx, kxs, kys, sx, sy, ys);
@@ -344,7 +362,11 @@ void roll_F(const float* F, float* out, long n, long nk, int nx, int ny, int nxs
 }
 void comoving_solve(const SpecGeom& g, const cx* jkz, const double* hcg, const double* wlo, const double* whi,
                     double beta, double gm2, cx* cp, cx* dp) {
-    G3D_LAUNCH(k_comoving, g.nk, g, jkz, hcg, wlo, whi, beta, gm2, cp, dp);
+    G3D_LAUNCH(k_comoving, g.nk, g, jkz, (const double*)nullptr, hcg, wlo, whi, beta, gm2, cp, dp);
+}
+void comoving_solve_ev(const SpecGeom& g, const cx* e1, const double* ev, const double* hcg, const double* wlo,
+                       const double* whi, double beta, double gm2, cx* cp, cx* dp) {
+    G3D_LAUNCH(k_comoving, g.nk, g, e1, ev, hcg, wlo, whi, beta, gm2, cp, dp);
 }
 void comoving_R(const SpecGeom& g, const cx* psi, double beta, cx* Rx, cx* Ry, cx* Ez, cx* Bz) {
     G3D_LAUNCH(k_comoving_R, long(g.ns) * g.nk, g, psi, beta, Rx, Ry, Ez, Bz, long(g.ns) * g.nk);

@@ -1,0 +1,461 @@
+// The free-hand y grid: x is Fourier (one x transform per row), y is a staggered finite-difference grid between
+// conducting walls, so every k_x is a line along y and each slice couples the points of a line through two
+// tridiagonal systems.  One lane runs one line (the eliminations along y are sequential); lanes are adjacent k_x,
+// so every load and store of a step along the line is one coalesced row segment.
+//
+//   march of a slice          src/sim_kernels.hpp:213-313
+//   current line filters      src/sim_particles.cpp:160-204
+//   sponge bound potential    src/sim_fields.cpp:202-268
+//   co-moving field           src/sim_fields.cpp:55-69, 79-147
+//   plasma pass per line      src/sim_plasma.cpp:119-184 (filters, smoothing), 300-396 (bookkeeping, march)
+#include "g3d_pass.hpp"
+#include "g3d_runtime.hpp"
+
+namespace g3d {
+namespace {
+constexpr int kT = 64;
+
+// ---------------------------------------------------------------- march of one line
+// Slice n of line ik at level s1, with the reference's operations and operand order (bit-identical results).
+// Fused form: one sweep up the line forms the explicit parts of R (from slice n at s0) and L (from slice n+1 at
+// s1) and runs the forward eliminations of both systems, E_z on the nodes 1 .. ny-1 and B_z on the half-nodes
+// 0 .. ny-1, the latter one node behind; the explicit parts wait in the R and L slots of slice n, the eliminated
+// right-hand sides in the E_z and B_z slots, the factors in scp.  One sweep down back-substitutes both systems
+// and finishes R and L one node behind.  Every value of slice n at s0 is read before its slot is rewritten.
+__device__ void march_line(const LineView& L, int ik, int n) {
+    const MarchView& c = L.mv;
+    const int nx = c.nx, ny = L.ny;
+    const long nk = c.nk;
+    const long P0 = long(n - c.a) * nk + ik;        // slice n = plane of half-cell n-1 in the currents
+    const long P1 = P0 + nk;                        // slice n+1 = half-cell n
+    const cx a = mk(0, c.kx[ik]);
+    const double kx2 = c.kx[ik] * c.kx[ik], D = c.D, wl = c.wlo[n], wu = c.whi[n];
+    const double* iD = L.iD;
+    const double* iA = L.iA;
+    double* cpE = L.scp;
+    double* cpB = L.scp + nk;
+    // explicit part of R at node / half-node j from slice n at level s0 and the currents of both levels
+    auto known_R = [&](int j, cx ez0, cx ezu, cx bz0, cx bzm, cx& sx, cx& sy) {
+        const long o = P0 + long(j) * nx;
+        if (j == 0) sx = mk(0);
+        else {
+            const cx Jx = wl * (c.j0x[o] + c.jkx[o]) + wu * (c.j0x[o + nk] + c.jkx[o + nk]);
+            const cx bm = (bz0 - bzm) * iA[j];
+            sx = c.Rx[o] + D * (a * ez0 + bm) - D * Jx;
+        }
+        const cx Jy = wl * (c.j0y[o] + c.jky[o]) + wu * (c.j0y[o + nk] + c.jky[o + nk]);
+        sy = c.Ry[o] + D * ((ezu - ez0) * iD[j] - a * bz0) - D * Jy;
+    };
+    cx ez0 = c.Ez[P0], bzm = mk(0);
+    if (n == c.N - 1) {                              // front of the window: L = E_z = B_z = 0
+        for (int j = 0; j < ny; ++j) {
+            const long o = P0 + long(j) * nx;
+            const cx ezu = j + 1 < ny ? c.Ez[o + nx] : mk(0), bz0 = c.Bz[o];
+            cx sx, sy;
+            known_R(j, ez0, ezu, bz0, bzm, sx, sy);
+            c.Rx[o] = sx; c.Ry[o] = sy;
+            c.Ez[o] = c.Bz[o] = c.Lx[o] = c.Ly[o] = mk(0);
+            ez0 = ezu; bzm = bz0;
+        }
+        return;
+    }
+    const double h = c.hcg[n + 2], qh = 0.25 * h, hh = 0.5 * h, cc = qh * (D + qh);
+    auto P = [&](const cx* A, int j) { return A[P1 + long(j) * nx]; };
+    // slice n+1 at node j (and j-1: ...m), explicit parts of node j-1, elimination state of both systems
+    cx pRx = P(c.Rx, 0), pRy = P(c.Ry, 0), pLx = P(c.Lx, 0), pLy = P(c.Ly, 0), pBz = P(c.Bz, 0), pEz = mk(0);
+    cx pRxm = mk(0), pRym = mk(0), pLxm = mk(0), pLym = mk(0), pBzm = mk(0);
+    cx sRxm = mk(0), sRym = mk(0), sLxm = mk(0), sLym = mk(0);
+    double cE = 0, cB = 0;
+    cx dE = mk(0), dB = mk(0);
+    // forward elimination of B_z at half-node jb (fu, su: the R + L sums of node jb+1, zero above the last)
+    auto elim_B = [&](int jb, cx fu, cx f0, cx ry, cx ly, cx pbz, cx su, cx s0, cx sry, cx sly) {
+        const cx Cp = 0.5 * (a * (ry + ly) - (fu - f0) * iD[jb]);
+        const cx SB = a * (sry + sly) - (su - s0) * iD[jb];
+        const cx r = pbz - hh * Cp - qh * SB;
+        const double al = jb > 0 ? iA[jb] : 0, au = jb < ny - 1 ? iA[jb + 1] : 0;
+        const double lo = -cc * iD[jb] * al, up = -cc * iD[jb] * au;
+        const double den = 1 + cc * kx2 + cc * iD[jb] * (al + au) - lo * cB;
+        dB = (r - lo * dB) / den;
+        cB = up / den;
+        c.Bz[P0 + long(jb) * nx] = dB;
+        cpB[long(jb) * nx + ik] = cB;
+    };
+    for (int j = 0; j < ny; ++j) {
+        const long o = P0 + long(j) * nx, u = P1 + long(j) * nx;
+        const cx ezu = j + 1 < ny ? c.Ez[o + nx] : mk(0), bz0 = c.Bz[o];
+        cx sRx, sRy;
+        known_R(j, ez0, ezu, bz0, bzm, sRx, sRy);
+        const cx pEzu = j + 1 < ny ? c.Ez[u + nx] : mk(0);
+        cx sLx;
+        if (j == 0) sLx = mk(0);
+        else {
+            const cx bm = (pBz - pBzm) * iA[j];
+            sLx = pLx - qh * (a * pEz - bm) - hh * c.jkx[u];
+        }
+        const cx bp = (pEzu - pEz) * iD[j];
+        const cx sLy = pLy - qh * (a * pBz + bp) - hh * c.jky[u];
+        c.Rx[o] = sRx; c.Ry[o] = sRy; c.Lx[o] = sLx; c.Ly[o] = sLy;
+        if (j >= 1) {
+            // E_z at node j
+            const cx Qp = 0.5 * (-(a * (pRx - pLx)) + ((pLy - pRy) - (pLym - pRym)) * iA[j]);
+            const cx SE = -(a * (sRx - sLx)) + ((sLy - sRy) - (sLym - sRym)) * iA[j];
+            const cx r = pEz - hh * Qp - qh * SE - h * c.jkz[u];
+            const double lo = j > 1 ? -cc * iA[j] * iD[j - 1] : 0, up = j < ny - 1 ? -cc * iA[j] * iD[j] : 0;
+            const double den = 1 + cc * kx2 + cc * iA[j] * (iD[j - 1] + iD[j]) - lo * cE;
+            dE = (r - lo * dE) / den;
+            cE = up / den;
+            c.Ez[o] = dE;
+            cpE[long(j) * nx + ik] = cE;
+            // B_z at half-node j-1
+            elim_B(j - 1, pRx + pLx, pRxm + pLxm, pRym, pLym, pBzm, sRx + sLx, sRxm + sLxm, sRym, sLym);
+        }
+        ez0 = ezu; bzm = bz0;
+        pRxm = pRx; pRym = pRy; pLxm = pLx; pLym = pLy; pBzm = pBz;
+        sRxm = sRx; sRym = sRy; sLxm = sLx; sLym = sLy;
+        pEz = pEzu;
+        if (j + 1 < ny) {
+            pRx = P(c.Rx, j + 1); pRy = P(c.Ry, j + 1); pLx = P(c.Lx, j + 1); pLy = P(c.Ly, j + 1);
+            pBz = P(c.Bz, j + 1);
+        }
+    }
+    elim_B(ny - 1, mk(0), pRxm + pLxm, pRym, pLym, pBzm, mk(0), sRxm + sLxm, sRym, sLym);
+    // back substitution of both systems; R and L of node jj once E_z(jj), E_z(jj+1), B_z(jj), B_z(jj-1) are known
+    auto finish = [&](int jj, cx ez, cx ezu, cx bz, cx bzl) {
+        const long o = P0 + long(jj) * nx;
+        c.Ez[o] = ez; c.Bz[o] = bz;
+        if (jj == 0) { c.Rx[o] = mk(0); c.Lx[o] = mk(0); }
+        else {
+            const cx bm = (bz - bzl) * iA[jj];
+            c.Rx[o] = c.Rx[o] + D * (a * ez + bm);
+            c.Lx[o] = c.Lx[o] - qh * (a * ez - bm);
+        }
+        const cx bp = (ezu - ez) * iD[jj];
+        c.Ry[o] = c.Ry[o] + D * (bp - a * bz);
+        c.Ly[o] = c.Ly[o] - qh * (a * bz + bp);
+    };
+    cx nE = mk(0), nB = mk(0);
+    cx e1 = mk(0), e2 = mk(0), b1 = mk(0);          // E_z(j+1), E_z(j+2), B_z(j+1)
+    for (int j = ny - 1; j >= 0; --j) {
+        const long o = P0 + long(j) * nx;
+        cx e0 = mk(0);
+        if (j >= 1) { nE = c.Ez[o] - cpE[long(j) * nx + ik] * nE; e0 = nE; }
+        nB = c.Bz[o] - cpB[long(j) * nx + ik] * nB;
+        if (j + 1 < ny) finish(j + 1, e1, e2, b1, nB);
+        e2 = e1; e1 = e0; b1 = nB;
+    }
+    finish(0, e1, e2, b1, mk(0));
+}
+
+__global__ void __launch_bounds__(kT) k_march_lines(LineView L, int lo, int hi) {
+    const int ik = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ik >= L.mv.nx) return;
+    for (int n = hi - 1; n >= lo; --n) march_line(L, ik, n);
+}
+
+// ---------------------------------------------------------------- filters of a line of currents
+// (1,2,1)/4 along y of the charges q_j = A_j / w_j, out_j = (q_{j-1}/4 + q_j/2 + q_{j+1}/4) w_j, streamed one
+// node per step.  Nodes (w = 1/A): q = 0 beyond the walls, node 0 untouched; half-nodes (w = 1/Delta): mirrored.
+struct Filter121 {
+    cx qm, q0;
+    __device__ cx step(cx qp, double w) {
+        const cx v = (0.25 * qm + 0.5 * q0 + 0.25 * qp) * w;
+        qm = q0; q0 = qp;
+        return v;
+    }
+};
+
+// one line of the half-cell currents (src/sim_particles.cpp:160-204): unpack of the pair per row, the filters,
+// the x filter, node 0 and the k_x = 0 mean of j_y
+__global__ void __launch_bounds__(kT) k_fh_currents(const cx* S1, cx* jx, cx* jy, cx* jz, int planes, LineView L,
+                                                   const double* fl, int kf, int keep0, double Ly) {
+    const long line = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int nx = L.mv.nx, ny = L.ny;
+    const long nk = L.mv.nk;
+    if (line >= long(planes) * nx) return;
+    const int ixx = int(line % nx), mx = (nx - ixx) % nx;
+    const long base = (line / nx) * nk, o = base + ixx;
+    const double* ia = L.iA;
+    const double* id = L.iD;
+    auto unpack = [&](int j, cx& ux, cx& uy) {
+        const long k = base + long(j) * nx;
+        const cx z = S1[k + ixx], zm = conjg(S1[k + mx]);
+        const cx d = z - zm;
+        ux = 0.5 * (z + zm);
+        uy = 0.5 * mk(d.im, -d.re);
+    };
+    const double fx = fl[ixx];
+    const bool with_mean = ixx == 0 && !keep0;
+    cx ax, ay, az = jz[o];
+    unpack(0, ax, ay);
+    Filter121 fxn{mk(0), mk(0)}, fzn{mk(0), mk(0)}, fyh{mk(0), mk(0)};
+    if (kf) { fyh.qm = ay / id[0]; fyh.q0 = fyh.qm; }
+    cx mean = mk(0);
+    for (int j = 0; j < ny; ++j) {
+        const bool more = j + 1 < ny;
+        cx bx = mk(0), by = mk(0), bz = mk(0);
+        if (more) { unpack(j + 1, bx, by); bz = jz[o + long(j + 1) * nx]; }
+        cx vx = ax, vy = ay, vz = az;
+        if (kf) {
+            vy = fyh.step(more ? by / id[j + 1] : fyh.q0, id[j]);
+            if (j == 0) { fxn.q0 = bx / ia[1]; fzn.q0 = bz / ia[1]; }
+            else {
+                vx = fxn.step(more ? bx / ia[j + 1] : mk(0), ia[j]);
+                vz = fzn.step(more ? bz / ia[j + 1] : mk(0), ia[j]);
+            }
+        }
+        const long k = o + long(j) * nx;
+        jx[k] = fx * (j == 0 ? mk(0) : vx);
+        jz[k] = fx * (j == 0 ? mk(0) : vz);
+        if (with_mean) { jy[k] = vy; mean = mean + vy / id[j]; }
+        else jy[k] = fx * (vy - mk(0));
+        ax = bx; ay = by; az = bz;
+    }
+    if (with_mean) {
+        mean = mean / Ly;
+        for (int j = 0; j < ny; ++j) {
+            const long k = o + long(j) * nx;
+            jy[k] = fx * (jy[k] - mean);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- sponge (src/sim_fields.cpp:202-242, 259-268)
+// psi of line (slice l, k_x) = T^-1 (rho + gamma^-2 T^-1 d_zeta^2 rho), T = k_x^2 - Lap_y with its LU factors
+// (pc, pd), Dirichlet at the walls; S1 = (R - R_b) packed, R_b = (1 + beta) E_perp of psi
+__global__ void __launch_bounds__(kT) k_fh_sponge_psi(FhSpongeArgs s, LineView L, const cx* rx, const cx* ry, cx* s1,
+                                                     cx* s2) {
+    const int nx = L.mv.nx, ny = L.ny;
+    const long nk = L.mv.nk;
+    const long line = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (line >= long(s.ns) * nx) return;
+    const int ixx = int(line % nx), l = int(line / nx), n = s.a + l;
+    const long o = long(l) * nk + ixx;
+    const double* ia = L.iA;
+    const double* id = L.iD;
+    // half-cell planes; beyond the window (no neighbouring blocks on one GPU) the charge is zero
+    auto half = [&](int hk, long jo) -> cx { return (hk < 0 || hk > s.ns) ? mk(0) : s.jz[long(hk) * nk + ixx + jo]; };
+    auto rho_at = [&](int dl, long jo) -> cx {
+        const int nn = n + dl;
+        if (nn < 0 || nn > s.N - 1) return mk(0);
+        return s.wl[nn] * half(l + dl, jo) + s.wu[nn] * half(l + dl + 1, jo);
+    };
+    const double hm = s.hc[n + 1], hp = s.hc[n + 2];
+    const double cm = 2 / ((hm + hp) * hm), cp_ = 2 / ((hm + hp) * hp);
+    for (int second = 0; second < 2; ++second) {
+        cx dpv = mk(0);
+        for (int j = 1; j < ny; ++j) {
+            const long jo = long(j) * nx, k = o + jo;
+            cx r;
+            if (!second) r = s.g2 * (cp_ * rho_at(1, jo) - (cm + cp_) * rho_at(0, jo) + cm * rho_at(-1, jo));
+            else r = rho_at(0, jo) + s2[k];
+            const double lo = j > 1 ? -ia[j] * id[j - 1] : 0;
+            dpv = (r - lo * dpv) * s.pd[long(j) * nx + ixx];
+            s2[k] = dpv;
+        }
+        cx next = mk(0);
+        for (int j = ny - 1; j >= 1; --j) {
+            next = s2[o + long(j) * nx] - s.pc[long(j) * nx + ixx] * next;
+            s2[o + long(j) * nx] = next;
+        }
+    }
+    s2[o] = mk(0);
+    const cx a = mk(0, s.kx[ixx]);
+    for (int j = 0; j < ny; ++j) {
+        const long k = o + long(j) * nx;
+        const cx pu = j + 1 < ny ? s2[k + nx] : mk(0);
+        const cx bx = -s.opb * (a * s2[k]), by = -s.opb * ((pu - s2[k]) * id[j]);
+        const cx u = rx[k] - bx, v = ry[k] - by;
+        s1[k] = mk(u.re - v.im, u.im + v.re);
+    }
+}
+// the sponged R - R_b unpacked per row, R_b added back (node 0: R_x = 0)
+__global__ void k_fh_sponge_unpack(FhSpongeArgs s, LineView L, const cx* s1, const cx* s2, cx* rx, cx* ry, long tot) {
+    const long idx = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= tot) return;
+    const int nx = L.mv.nx, ny = L.ny;
+    const long i = idx % L.mv.nk;
+    const int j = int(i / nx), ixx = int(i % nx);
+    const long m = idx - ixx + (nx - ixx) % nx;
+    const cx z = s1[idx], zm = conjg(s1[m]);
+    const cx pu = j + 1 < ny ? s2[idx + nx] : mk(0);
+    const cx bx = -s.opb * (mk(0, s.kx[ixx]) * s2[idx]), by = -s.opb * ((pu - s2[idx]) * L.iD[j]);
+    const cx d = z - zm;
+    rx[idx] = j == 0 ? mk(0) : 0.5 * (z + zm) + bx;
+    ry[idx] = 0.5 * mk(d.im, -d.re) + by;
+}
+
+// ---------------------------------------------------------------- co-moving field
+// coefficient s of the eigenvector expansion of the node values (slot 0 unused: zero)
+__global__ void k_fh_eig_fwd(const cx* jz, const double* ef, cx* e1, long tot, int nx, int ny, long nk) {
+    const long idx = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= tot) return;
+    const long i = idx % nk, base = idx - i;
+    const int s = int(i / nx), ixx = int(i % nx);
+    cx v = mk(0);
+    if (s > 0)
+        for (int j = 1; j < ny; ++j) v = v + ef[long(s) * ny + j] * jz[base + long(j) * nx + ixx];
+    e1[idx] = v;
+}
+// node value j from the coefficients (node 0 on the wall: zero)
+__global__ void k_fh_eig_back(const cx* c, const double* eb, cx* out, long tot, int nx, int ny, long nk) {
+    const long idx = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= tot) return;
+    const long i = idx % nk, base = idx - i;
+    const int j = int(i / nx), ixx = int(i % nx);
+    cx v = mk(0);
+    if (j > 0)
+        for (int s = 1; s < ny; ++s) v = v + eb[long(j) * ny + s] * c[base + long(s) * nx + ixx];
+    out[idx] = v;
+}
+__global__ void k_fh_comoving_R(const cx* psi, const double* kx, const double* id, double beta, cx* rx, cx* ry, cx* ez,
+                                cx* bz, long tot, int nx, int ny, long nk) {
+    const long idx = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= tot) return;
+    const long i = idx % nk;
+    const int j = int(i / nx), ixx = int(i % nx);
+    rx[idx] = -(1 + beta) * (mk(0, kx[ixx]) * psi[idx]);
+    const cx up = j + 1 < ny ? psi[idx + nx] : mk(0);
+    ry[idx] = -(1 + beta) * ((up - psi[idx]) * id[j]);
+    ez[idx] = mk(0); bz[idx] = mk(0);
+}
+
+// ---------------------------------------------------------------- one pass of a plasma half-cell, per line
+// Per line ik: the y filters, the x filter, node 0 and the smoothing (1 + a^2 (k_x^2 - Lap_y))^-1 of the streaming
+// current of this pass (X: the per-row unpacked x transforms, modified in place); kind 2 stops there (jsn).
+// Otherwise per point the bookkeeping of the pass and the pass backup of slice n, the march of the line, on the
+// last pass the correction of R at slice n+1, and the three packed planes of slice n for the inverse transforms.
+__global__ void __launch_bounds__(kT) k_pass_lines(ModeArgs m, LineView L, cx* X, cx* T, cx* jsn, int kind) {
+    const int ik = blockIdx.x * blockDim.x + threadIdx.x;
+    const MarchView& c = m.mv;
+    const int nx = c.nx, ny = L.ny;
+    if (ik >= nx) return;
+    const long nk = c.nk;
+    const int n = m.n;
+    const double* ia = L.iA;
+    const double* id = L.iD;
+    cx* ox = X;
+    cx* oy = X + nk;
+    cx* oz = X + 2 * nk;
+    const long o = ik;
+    auto at = [&](cx* A, int j) -> cx& { return A[o + long(j) * nx]; };
+    if (kind != 1) {
+        const double fx = m.filt[ik];
+        if (m.kf) {
+            Filter121 fxn{mk(0), at(ox, 1) / ia[1]}, fzn{mk(0), at(oz, 1) / ia[1]};
+            Filter121 fyh{at(oy, 0) / id[0], at(oy, 0) / id[0]};
+            for (int j = 0; j < ny; ++j) {
+                const bool more = j + 1 < ny;
+                const cx y1 = more ? at(oy, j + 1) / id[j + 1] : fyh.q0;
+                cx x1 = mk(0), z1 = mk(0);
+                if (more && j >= 1) { x1 = at(ox, j + 1) / ia[j + 1]; z1 = at(oz, j + 1) / ia[j + 1]; }
+                at(oy, j) = fx * fyh.step(y1, id[j]);
+                if (j >= 1) { at(ox, j) = fx * fxn.step(x1, ia[j]); at(oz, j) = fx * fzn.step(z1, ia[j]); }
+            }
+        } else
+            for (int j = 0; j < ny; ++j) {
+                if (j >= 1) { at(ox, j) = fx * at(ox, j); at(oz, j) = fx * at(oz, j); }
+                at(oy, j) = fx * at(oy, j);
+            }
+        at(ox, 0) = fx * mk(0); at(oz, 0) = fx * mk(0);
+        const double a2 = m.a2;
+        if (a2 > 0) {
+            const double kx2 = m.kxs[ik] * m.kxs[ik];
+            double* cpv = L.scp;
+            // nodes (j_x, j_z): zero at the walls, as E_z in the march
+            for (int t = 0; t < 2; ++t) {
+                cx* A = t ? oz : ox;
+                double cp = 0;
+                cx dp = mk(0);
+                for (int j = 1; j < ny; ++j) {
+                    const double lo = j > 1 ? -a2 * ia[j] * id[j - 1] : 0, up = j < ny - 1 ? -a2 * ia[j] * id[j] : 0;
+                    const double den = 1 + a2 * kx2 + a2 * ia[j] * (id[j - 1] + id[j]) - lo * cp;
+                    dp = (at(A, j) - lo * dp) / den;
+                    cp = up / den;
+                    at(A, j) = dp; cpv[o + long(j) * nx] = cp;
+                }
+                cx next = mk(0);
+                for (int j = ny - 1; j >= 1; --j) { next = at(A, j) - cpv[o + long(j) * nx] * next; at(A, j) = next; }
+            }
+            // half-nodes (j_y): as B_z in the march
+            double cp = 0;
+            cx dp = mk(0);
+            for (int j = 0; j < ny; ++j) {
+                const double al = j > 0 ? ia[j] : 0, au = j < ny - 1 ? ia[j + 1] : 0;
+                const double lo = -a2 * id[j] * al, up = -a2 * id[j] * au;
+                const double den = 1 + a2 * kx2 + a2 * id[j] * (al + au) - lo * cp;
+                dp = (at(oy, j) - lo * dp) / den;
+                cp = up / den;
+                at(oy, j) = dp; cpv[o + long(j) * nx] = cp;
+            }
+            cx next = mk(0);
+            for (int j = ny - 1; j >= 0; --j) { next = at(oy, j) - cpv[o + long(j) * nx] * next; at(oy, j) = next; }
+        }
+        if (kind == 2) {
+            for (int j = 0; j < ny; ++j) {
+                const long i = o + long(j) * nx;
+                jsn[i] = ox[i]; jsn[nk + i] = oy[i]; jsn[2 * nk + i] = oz[i];
+            }
+            return;
+        }
+    }
+    for (int j = 0; j < ny; ++j) {
+        const long i = o + long(j) * nx, so = long(n - c.a) * nk + i;
+        cx u[3] = {mk(0), mk(0), mk(0)};
+        if (kind != 1) { u[0] = ox[i]; u[1] = oy[i]; u[2] = oz[i]; }
+        pass_currents(m, i, u);
+        if (m.bRx) {                                 // slice n at s0: saved on the first pass, restored after
+            if (m.first) { m.bRx[i] = c.Rx[so]; m.bRy[i] = c.Ry[so]; m.bEz[i] = c.Ez[so]; m.bBz[i] = c.Bz[so]; }
+            else { c.Rx[so] = m.bRx[i]; c.Ry[so] = m.bRy[i]; c.Ez[so] = m.bEz[i]; c.Bz[so] = m.bBz[i]; }
+        }
+    }
+    march_line(L, ik, n);
+    for (int j = 0; j < ny; ++j) {
+        const long i = o + long(j) * nx, so = long(n - c.a) * nk + i;
+        if (m.last && !m.top) {
+            const cx u[3] = {ox[i], oy[i], oz[i]};
+            pass_correct(m, i, u);
+        }
+        cx t[3];
+        pack3(t, c.Rx[so], c.Ry[so], c.Lx[so], c.Ly[so], c.Ez[so], c.Bz[so]);
+        T[i] = t[0]; T[nk + i] = t[1]; T[2 * nk + i] = t[2];
+    }
+}
+
+template <class K, class... A> void launch(K k, long n, int t, const char* what, A... a) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k, dim3(blocks_for(n, t)), dim3(t), 0, stream(), a...);
+    launch_check(what);
+}
+}  // namespace
+
+void march_lines(const LineView& L, int lo, int hi) { launch(k_march_lines, L.mv.nx, kT, "march lines", L, lo, hi); }
+void fh_currents(const cx* S1, cx* jkx, cx* jky, cx* jkz, int planes, const LineView& L, const double* filt, int kf,
+                 int keep0, double Ly) {
+    launch(k_fh_currents, long(planes) * L.mv.nx, kT, "free-hand currents", S1, jkx, jky, jkz, planes, L, filt, kf,
+           keep0, Ly);
+}
+void fh_sponge_psi(const FhSpongeArgs& s, const LineView& L, const cx* Rx, const cx* Ry, cx* S1, cx* S2) {
+    launch(k_fh_sponge_psi, long(s.ns) * L.mv.nx, kT, "free-hand sponge", s, L, Rx, Ry, S1, S2);
+}
+void fh_sponge_unpack(const FhSpongeArgs& s, const LineView& L, const cx* S1, const cx* S2, cx* Rx, cx* Ry) {
+    const long tot = long(s.ns) * L.mv.nk;
+    launch(k_fh_sponge_unpack, tot, 256, "free-hand sponge unpack", s, L, S1, S2, Rx, Ry, tot);
+}
+void fh_eigen_forward(const cx* jz, const double* evf, cx* e1, int planes, int nx, int ny, long nk) {
+    const long tot = long(planes) * nk;
+    launch(k_fh_eig_fwd, tot, 256, "eigen forward", jz, evf, e1, tot, nx, ny, nk);
+}
+void fh_eigen_back(const cx* cf, const double* evb, cx* out, int planes, int nx, int ny, long nk) {
+    const long tot = long(planes) * nk;
+    launch(k_fh_eig_back, tot, 256, "eigen back", cf, evb, out, tot, nx, ny, nk);
+}
+void fh_comoving_R(const cx* psi, const double* kx, const double* iD, double beta, cx* Rx, cx* Ry, cx* Ez, cx* Bz,
+                   int planes, int nx, int ny, long nk) {
+    const long tot = long(planes) * nk;
+    launch(k_fh_comoving_R, tot, 256, "free-hand comoving R", psi, kx, iD, beta, Rx, Ry, Ez, Bz, tot, nx, ny, nk);
+}
+void pass_lines(const ModeArgs& m, const LineView& L, cx* X, cx* T, cx* jsn, int kind) {
+    launch(k_pass_lines, m.mv.nx, kT, "pass lines", m, L, X, T, jsn, kind);
+}
+
+}  // namespace g3d

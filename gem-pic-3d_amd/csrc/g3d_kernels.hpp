@@ -4,6 +4,7 @@
 #include "g3d_common.hpp"
 #include "g3d_fft.hpp"
 #include "g3d_march.hpp"
+#include "g3d_runtime.hpp"
 
 namespace g3d {
 
@@ -56,7 +57,15 @@ struct DepositArgs {
     int nh, a, N, nx, ny, yf;
     long nk;
 };
-void bunch_deposit(const DepositArgs& a, Parts p, long np);
+// scratch of the ordered deposit (records, their sorted copies, the sort's temporary storage)
+struct DepositScratch {
+    DevBuf<unsigned long long> key0, key1;
+    DevBuf<unsigned> rec0, rec1;
+    DevBuf<double> val;
+    DevBuf<unsigned char> tmp;
+};
+// the reference's single-thread summation order, bit for bit (g3d_deposit.hip)
+void bunch_deposit(const DepositArgs& a, Parts p, long np, DepositScratch& s);
 // one half-cell pass of the streaming particles: src -> dst (the state of every particle is written); tiles
 // with tg.ntile > 0, otherwise every deposit is a global atomic
 void stream_halfcell(const StreamArgs& a, Parts src, Parts dst, long nstrm, const TileGeom& tg);
@@ -111,13 +120,16 @@ struct ModeArgs {
     cx* T;                            // 3 packed planes for the inverse transforms
     const double *kxs, *kys, *filt, *filty;
     double wex, qcorr, a2;
-    int n, first, last, top, extrap, keep0, ny;
+    int n, first, last, top, extrap, keep0, ny, kf;
 };
 // the fused kernels of a plasma pass (g3d_plasma.hip): rows forward (jsr + slabs -> X, 3 planes), columns
 // (kind 0 pass, 1 front slice, 2 spectra only into jsn; X -> T, 3 planes), rows inverse (T -> F planes of a slice)
 void pass_rows_fwd(double* J, const TileGeom& tg, const FftPlan& p, cx* X, double* rl, double* rli, int i0, int j0);
 void pass_cols(const ModeArgs& m, const FftPlan& p, const cx* X, cx* T, cx* jsn, int kind);
 void pass_rows_inv(const cx* T, float* Fn, long fpl, const FftPlan& p);
+// the same columns step as three launches: y transforms of X (in place), the per-mode work (one lane per mode),
+// inverse y transforms of T
+void pass_modes(const ModeArgs& m, const FftPlan& p, cx* X, cx* T, cx* jsn, int kind);
 // jz -= js0z over count planes
 void sub_planes(cx* a, const cx* b, long n);
 void copy_planes(cx* dst, const cx* src, long n);
@@ -130,9 +142,10 @@ struct GridLaserArgs {
     int circular, helicity, nx, ny;
     long nk;
     const double* zeta;
+    const double* yn;                 // free-hand y: the node table (null: Fourier y)
 };
 void grid_laser_fill(const GridLaserArgs& a, cx* T, int n0, int count);
-void grid_laser_add(const SpecGeom& g, const cx* T, cx* Rx, cx* Ry, int count);
+void grid_laser_add(const SpecGeom& g, const cx* T, cx* Rx, cx* Ry, int count, int fh);
 // phase rotation e^{i(k_x sx + k_y sy)} of count planes (window shift, src/sim_setup.cpp:259-267)
 void rotate_planes(cx* A, long n, long nk, int nx, const double* kxs, const double* kys, double sx, double sy, int ys);
 void roll_F(const float* F, float* out, long n, long nk, int nx, int ny, int nxs, int nys);
@@ -141,5 +154,45 @@ void comoving_solve(const SpecGeom& g, const cx* jkz, const double* hcg, const d
                     double beta, double gm2, cx* cp, cx* dp);
 void comoving_R(const SpecGeom& g, const cx* psi, double beta, cx* Rx, cx* Ry, cx* Ez, cx* Bz);
 void axpy_planes(cx* y, const cx* x, double s, long n);    // y = y + s x (s = +-1)
+
+// ---------------------------------------------------------------- free-hand y (g3d_freehand.hip)
+// A k_x line of the free-hand y grid: element j of line ik sits at j nx + ik of a plane.  R_x, L_x, E_z, j_x, j_z
+// live on the nodes (j = 0 is the wall), R_y, L_y, B_z, j_y on the half-nodes.
+struct LineView {
+    MarchView mv;
+    const double *iD, *iA;            // 1/Delta_j (half-nodes), 1/A_j (nodes)
+    double* scp;                      // 2 nk: Thomas factors of the E_z and B_z systems (march, smoothing)
+    int ny;
+};
+// slices hi-1 ... lo of every k_x line (src/sim_fields.cpp:7-31, src/sim_kernels.hpp:213-313)
+void march_lines(const LineView& L, int lo, int hi);
+// the pair of x transforms (S1: j_x + i j_y per half-cell plane) unpacked per row, the (1,2,1)/4 y filters of the
+// charges, the x filter, the node-0 rule and the mean of j_y at k_x = 0 (src/sim_particles.cpp:160-204);
+// jkz holds the x transforms of j_z and is filtered in place
+void fh_currents(const cx* S1, cx* jkx, cx* jky, cx* jkz, int planes, const LineView& L, const double* filt, int kf,
+                 int keep0, double Ly);
+// sponge of the free-hand grid (src/sim_fields.cpp:202-268): bound potential psi per line into S2 (LU factors of
+// k_x^2 - Lap_y: pcp, pid), R - R_b packed into S1; then (after the masks) the sponged R with R_b added back
+struct FhSpongeArgs {
+    const cx* jz;
+    const double *kx, *pc, *pd, *hc, *wl, *wu;
+    double g2, opb;
+    int ns, N, a;
+};
+void fh_sponge_psi(const FhSpongeArgs& s, const LineView& L, const cx* Rx, const cx* Ry, cx* S1, cx* S2);
+void fh_sponge_unpack(const FhSpongeArgs& s, const LineView& L, const cx* S1, const cx* S2, cx* Rx, cx* Ry);
+// co-moving field (src/sim_fields.cpp:55-69, 122-147): node values of the half-cell charge to the coefficients of
+// the eigenvectors of -Lap_y (evf), back to the nodes (evb), and R of -grad psi on the free-hand grid
+void fh_eigen_forward(const cx* jz, const double* evf, cx* e1, int planes, int nx, int ny, long nk);
+void fh_eigen_back(const cx* c, const double* evb, cx* out, int planes, int nx, int ny, long nk);
+void fh_comoving_R(const cx* psi, const double* kx, const double* iD, double beta, cx* Rx, cx* Ry, cx* Ez, cx* Bz,
+                   int planes, int nx, int ny, long nk);
+// the co-moving solve along zeta of every (k_x, eigenvalue) pair; ev: the eigenvalue per slot, slot 0 unused
+void comoving_solve_ev(const SpecGeom& g, const cx* e1, const double* ev, const double* hcg, const double* wlo,
+                       const double* whi, double beta, double gm2, cx* cp, cx* dp);
+// one pass of a plasma half-cell per k_x line (kind 0), the front slice (kind 1) or the spectra only into jsn
+// (kind 2); X: the per-row unpacked x transforms of the streaming current (from pass_rows_fwd), T: 3 packed planes
+// for pass_rows_inv
+void pass_lines(const ModeArgs& m, const LineView& L, cx* X, cx* T, cx* jsn, int kind);
 
 }  // namespace g3d

@@ -1,6 +1,6 @@
 // March kernels: every mode from the front slice to the rear (one lane per mode, slice n+1 carried in registers,
 // the explicit inputs of the next slice prefetched while the current one is computed), and one slice.
-#include "g3d_march.hpp"
+#include "g3d_kernels.hpp"
 #include "g3d_runtime.hpp"
 
 namespace g3d {
@@ -59,8 +59,14 @@ MarchView view_of(const g3d_march_ctx& m) {
     return v;
 }
 void require_one_gpu(const g3d_march_ctx& m) {
-    if (!m.ky) throw std::runtime_error("g3d march: the free-hand y march needs g3d_march_lines");
     if (m.a != 0 || m.b != m.N) throw std::runtime_error("g3d march: one GPU owns all slices (a = 0, b = N)");
+    if (!m.ky && (!m.scp || !m.iD || !m.iA || m.ny < 4))
+        throw std::runtime_error("g3d march: the free-hand y march needs iD, iA, scp (2 nk doubles) and ny >= 4");
+}
+LineView lines_of(const g3d_march_ctx& m) {
+    LineView L;
+    L.mv = view_of(m); L.iD = m.iD; L.iA = m.iA; L.scp = m.scp; L.ny = m.ny;
+    return L;
 }
 }  // namespace
 
@@ -79,12 +85,29 @@ void march_one(const MarchView& v, int n, cx* bRx, cx* bRy, cx* bEz, cx* bBz, in
 
 extern "C" {
 int g3d_march(const g3d_march_ctx* c, int lo, int hi) {
-    return g3d::guard([&] { g3d::require_one_gpu(*c); g3d::march_all(g3d::view_of(*c), lo, hi); });
+    return g3d::guard([&] {
+        g3d::require_one_gpu(*c);
+        if (c->ky) g3d::march_all(g3d::view_of(*c), lo, hi);
+        else g3d::march_lines(g3d::lines_of(*c), lo, hi);
+    });
 }
 int g3d_march_slice(const g3d_march_ctx* c, int n, int restore) {
     return g3d::guard([&] {
         g3d::require_one_gpu(*c);
-        g3d::march_one(g3d::view_of(*c), n, c->bkRx, c->bkRy, c->bkEz, c->bkBz, restore);
+        if (c->ky) {
+            g3d::march_one(g3d::view_of(*c), n, c->bkRx, c->bkRy, c->bkEz, c->bkBz, restore);
+            return;
+        }
+        if (c->bkRx) {                     // the backup of slice n, then the lines of slice n
+            const long o = long(n - c->a) * c->nk;
+            g3d::cx* sl[4] = {c->Rx + o, c->Ry + o, c->Ez + o, c->Bz + o};
+            g3d::cx* bk[4] = {c->bkRx, c->bkRy, c->bkEz, c->bkBz};
+            for (int k = 0; k < 4; ++k) {
+                if (restore) g3d::copy_planes(sl[k], bk[k], c->nk);
+                else g3d::copy_planes(bk[k], sl[k], c->nk);
+            }
+        }
+        g3d::march_lines(g3d::lines_of(*c), n, n + 1);
     });
 }
 }

@@ -83,53 +83,6 @@ __global__ void __launch_bounds__(kT) k_bunch_push(PushArgs c, Parts P, long np,
     P.alive[ip] = al;
 }
 
-// ---------------------------------------------------------------- beam deposit (src/sim_particles.cpp:66-106)
-__global__ void __launch_bounds__(kT) k_bunch_deposit(DepositArgs c, Parts P, long np) {
-    const long ip = long(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (ip >= np || P.alive[ip] != 1) return;
-    int m; double fz;
-    c.zax.locate(P.zeta[ip], m, fz);
-    const int hw = m - (c.a - 2);
-    if (hw < 0 || hw >= c.nh || m < 0 || m > c.N - 2) return;
-    const double px = P.px[ip], py = P.py[ip], pz = P.pz[ip];
-    const double g = sqrt(1 + px * px + py * py + pz * pz);
-    const double vx = px / g, vy = py / g, vz = pz / g;
-    const double f = P.q[ip] * P.w[ip] / (c.hcg[m + 2] * c.dx);
-    int ix[3], jn[3], jh[3]; double wx[3], wn[3], wh[3];
-    tsc_periodic(P.x[ip], c.dx, c.nx, ix, wx);
-    double* J0 = c.J + long(0L * c.nh + hw) * c.nk;
-    double* J1 = c.J + long(1L * c.nh + hw) * c.nk;
-    double* J2 = c.J + long(2L * c.nh + hw) * c.nk;
-    if (c.yf) {
-        tsc_periodic(P.y[ip], c.dy, c.ny, jn, wn);
-        for (int v = 0; v < 3; ++v)
-            for (int u = 0; u < 3; ++u) {
-                const long k = long(jn[v]) * c.nx + ix[u];
-                const double w = f * wx[u] * wn[v] * c.ia[0];
-                atomicAdd(J0 + k, w * vx / vz);
-                atomicAdd(J1 + k, w * vy / vz);
-                atomicAdd(J2 + k, w);
-            }
-        return;
-    }
-    tsc_freehand(c.yax, P.y[ip], jn, wn, jh, wh);
-    for (int v = 0; v < 3; ++v) {
-        const bool nod = jn[v] >= 1 && jn[v] <= c.ny - 1, hal = jh[v] >= 0 && jh[v] <= c.ny - 1;
-        for (int u = 0; u < 3; ++u) {
-            if (nod) {
-                const long k = long(jn[v]) * c.nx + ix[u];
-                const double w = f * wx[u] * wn[v] * c.ia[jn[v]];
-                atomicAdd(J0 + k, w * vx / vz);
-                atomicAdd(J2 + k, w);
-            }
-            if (hal) {
-                const long k = long(jh[v]) * c.nx + ix[u];
-                atomicAdd(J1 + k, f * wx[u] * wh[v] * c.id[jh[v]] * vy / vz);
-            }
-        }
-    }
-}
-
 // ---------------------------------------------------------------- streaming particles (src/sim_stream.hpp)
 // LDS window of a tile: 5 planes of WX x WY nodes from node (x0, y0) (periodic); null for untiled particles
 struct Win {
@@ -436,11 +389,6 @@ void bunch_push(const PushArgs& a, Parts p, long np, double s0) {
     if (np <= 0) return;
     hipLaunchKernelGGL(k_bunch_push, dim3(blocks_for(np, kT)), dim3(kT), 0, stream(), a, p, np, s0);
     launch_check("bunch push");
-}
-void bunch_deposit(const DepositArgs& a, Parts p, long np) {
-    if (np <= 0) return;
-    hipLaunchKernelGGL(k_bunch_deposit, dim3(blocks_for(np, kT)), dim3(kT), 0, stream(), a, p, np);
-    launch_check("bunch deposit");
 }
 void stream_halfcell(const StreamArgs& a, Parts src, Parts dst, long nstrm, const TileGeom& tg) {
     if (nstrm <= 0) return;

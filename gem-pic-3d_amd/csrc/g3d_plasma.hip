@@ -13,8 +13,10 @@
 //
 // In the reference these are about a dozen kernels, two batched FFTs and a host synchronisation after most of
 // them; here they are three launches with no host synchronisation.
+#include <cstdlib>
+
 #include "g3d_fft_core.hpp"
-#include "g3d_kernels.hpp"
+#include "g3d_pass.hpp"
 #include "g3d_runtime.hpp"
 
 namespace g3d {
@@ -28,46 +30,7 @@ __device__ __forceinline__ void mode_work(const ModeArgs& m, long i, const cx u[
     const MarchView& c = m.mv;
     const long nk = c.nk;
     const int n = m.n;
-    const long hp = long(n + 1) * nk + i;           // half-cell n   (plane n + 1 of jk, js0)
-    const long hq = long(n) * nk + i;               // half-cell n-1 (plane n)
-    cx* jk[3] = {m.jkx, m.jky, m.jkz};
-    cx* js0[3] = {m.js0x, m.js0y, m.js0z};
-    if (m.first) {
-        if (m.top) {
-            // front: half-cell N-2 predicted with its level-s0 value (the prediction buffer starts at zero)
-            for (int k = 0; k < 3; ++k) {
-                const cx t = js0[k][hq];
-                jk[k][hq] = jk[k][hq] + t - m.pin[k * nk + i];
-                m.pin[k * nk + i] = t;
-            }
-        } else {
-            for (int k = 0; k < 3; ++k) {
-                const cx p = m.pin[k * nk + i];
-                m.sin_[k * nk + i] = p;
-                m.si0[k * nk + i] = p;
-            }
-            if (n >= 1)
-                for (int k = 0; k < 3; ++k) {
-                    const cx t = js0[k][hq];
-                    jk[k][hq] = jk[k][hq] + t - mk(0);
-                    m.pin[k * nk + i] = t;
-                }
-        }
-    }
-    if (!m.top) {
-        for (int k = 0; k < 3; ++k) {
-            const cx s = m.sin_[k * nk + i];
-            jk[k][hp] = jk[k][hp] + u[k] - s;
-            m.sin_[k * nk + i] = u[k];
-        }
-        if (n >= 1 && m.extrap)
-            for (int k = 0; k < 3; ++k) {
-                const cx d0 = u[k] - js0[k][hp];
-                const cx t = js0[k][hq] + d0 + m.wex * (d0 - m.dl[k * nk + i]);
-                jk[k][hq] = jk[k][hq] + t - m.pin[k * nk + i];
-                m.pin[k * nk + i] = t;
-            }
-    }
+    pass_currents(m, i, u);
     const long o = long(n - c.a) * nk + i;
     Mode6 old;
     if (!m.first && m.bRx) { old.Rx = m.bRx[i]; old.Ry = m.bRy[i]; old.Ez = m.bEz[i]; old.Bz = m.bBz[i]; }
@@ -80,18 +43,8 @@ __device__ __forceinline__ void mode_work(const ModeArgs& m, long i, const cx u[
     if (n < c.N - 1) up = Mode6{c.Rx[on], c.Ry[on], c.Ez[on], c.Bz[on], c.Lx[on], c.Ly[on]};
     const Mode6 r = march_mode(c, i, n, old, up);
     c.Rx[o] = r.Rx; c.Ry[o] = r.Ry; c.Ez[o] = r.Ez; c.Bz[o] = r.Bz; c.Lx[o] = r.Lx; c.Ly[o] = r.Ly;
-    if (m.last && !m.top) {
-        const cx s0x = m.si0[i], s0y = m.si0[nk + i];
-        c.Rx[on] = c.Rx[on] - m.qcorr * (u[0] - s0x);
-        c.Ry[on] = c.Ry[on] - m.qcorr * (u[1] - s0y);
-        for (int k = 0; k < 3; ++k) {
-            m.dl[k * nk + i] = u[k] - js0[k][hp];
-            js0[k][hp] = u[k];
-        }
-    }
-    T[0] = mk(r.Rx.re - r.Ry.im, r.Rx.im + r.Ry.re);
-    T[1] = mk(r.Lx.re - r.Ly.im, r.Lx.im + r.Ly.re);
-    T[2] = mk(r.Ez.re - r.Bz.im, r.Ez.im + r.Bz.re);
+    if (m.last && !m.top) pass_correct(m, i, u);
+    pack3(T, r.Rx, r.Ry, r.Lx, r.Ly, r.Ez, r.Bz);
 }
 
 // ---------------------------------------------------------------- rows forward
@@ -221,6 +174,29 @@ __global__ void __launch_bounds__(kT) k_pass_cols(ModeArgs m, const cx* X, cx* T
     }
 }
 
+// ---------------------------------------------------------------- the per-mode work on its own
+// X: the three planes after both forward transforms; T: the packed planes before the inverse y transforms.  One
+// lane per mode (kind as k_pass_cols).
+__global__ void __launch_bounds__(kT) k_pass_modes(ModeArgs m, const cx* X, cx* T, cx* jsn, int kind) {
+    const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long nk = m.mv.nk;
+    if (i >= nk) return;
+    const int kx = int(i % m.mv.nx), j = int(i / m.mv.nx);
+    cx u[3] = {mk(0), mk(0), mk(0)};
+    if (kind != 1) {
+        const double k2 = m.kxs[kx] * m.kxs[kx] + m.kys[j] * m.kys[j];
+        const double f = (kx == 0 && j == 0 && !m.keep0) ? 0.0 : m.filt[kx] * m.filty[j] / (1 + m.a2 * k2);
+        for (int k = 0; k < 3; ++k) u[k] = f * X[k * nk + i];
+        if (kind == 2) {
+            for (int k = 0; k < 3; ++k) jsn[k * nk + i] = u[k];
+            return;
+        }
+    }
+    cx t[3];
+    mode_work(m, i, u, t);
+    for (int k = 0; k < 3; ++k) T[k * nk + i] = t[k];
+}
+
 // ---------------------------------------------------------------- rows inverse and the real fields of slice n
 __global__ void __launch_bounds__(kT) k_pass_rows_inv(const cx* T, float* Fn, long fpl, int nx, int ny, int lx, int rpw,
                                                      const cx* twx, double scale) {
@@ -255,6 +231,8 @@ __global__ void __launch_bounds__(kT) k_pass_rows_inv(const cx* T, float* Fn, lo
 
 int rows_per_wg(int ny) { int r = 1; while (r < 8 && ny / (2 * r) >= 128) r <<= 1; return r; }
 int cols_per_wg(int nx, int ny) {
+    static const int forced = [] { const char* e = getenv("G3D_COLS_PER_WG"); return e ? atoi(e) : 0; }();
+    if (forced > 0 && nx % forced == 0 && forced * ny <= kMaxModes * kT) return forced;
     int nc = 1;
     while (nc < 8 && 2 * nc * ny <= kMaxModes * kT && nx / (2 * nc) >= 128) nc <<= 1;
     while (nc * ny > kMaxModes * kT && nc > 1) nc >>= 1;
@@ -279,11 +257,17 @@ void pass_cols(const ModeArgs& m, const FftPlan& p, const cx* X, cx* T, cx* jsn,
                        kind);
     launch_check("pass columns");
 }
+void pass_modes(const ModeArgs& m, const FftPlan& p, cx* X, cx* T, cx* jsn, int kind) {
+    if (kind != 1) fft_cols(p, X, 3, false, 1.0);
+    hipLaunchKernelGGL(k_pass_modes, dim3(blocks_for(m.mv.nk, kT)), dim3(kT), 0, stream(), m, X, T, jsn, kind);
+    launch_check("pass modes");
+    if (kind != 2) fft_cols(p, T, 3, true, 1.0);
+}
 void pass_rows_inv(const cx* T, float* Fn, long fpl, const FftPlan& p) {
     const int rpw = rows_per_wg(p.ny);
     const size_t lds = (size_t(3 * rpw) * (p.nx + p.nx / 8) + p.nx / 2) * sizeof(cx);
     hipLaunchKernelGGL(k_pass_rows_inv, dim3(p.ny / rpw), dim3(lanes(rpw * p.nx)), lds, stream(), T, Fn, fpl, p.nx, p.ny,
-                       p.lx, rpw, p.twx, 1.0 / (double(p.nx) * p.ny));
+                       p.lx, rpw, p.twx, p.two_d ? 1.0 / (double(p.nx) * p.ny) : 1.0 / p.nx);
     launch_check("pass rows inverse");
 }
 

@@ -8,6 +8,7 @@ functions by name (call), and host copies of the device arrays in the reference'
 The library is loaded from this directory (lib/libg3d.so); if it is missing the import fails with an error -
 there is no CPU fallback.
 """
+import bisect
 import ctypes as C
 import math
 import os
@@ -112,6 +113,37 @@ class Deck:
         self.c.ny = n; self.c.Ly = Ly; self.c.y_fourier = 1
         self.c.ynode = self.ynode.ctypes.data_as(C.POINTER(C.c_double))
 
+    def zeta_table(self, pts, steps, Lw):               # src/sim_config.cpp:21-30
+        m = StepMapping(pts, steps)
+        cells = int(math.ceil(m.index(Lw) - 1e-9))
+        if cells < 3:
+            raise ValueError("grid: the zeta window needs at least 3 cells")
+        self.zeta = np.array([m.coord(i) for i in range(cells + 1)], dtype=np.float64)
+        self.c.nzeta = cells + 1
+        self.c.zeta = self.zeta.ctypes.data_as(C.POINTER(C.c_double))
+
+    def y_table(self, pts, steps, ymax):               # src/sim_config.cpp:39-48
+        """free-hand y: nodes symmetric about y = 0 from the step table, conducting walls at the ends"""
+        m = StepMapping(pts, steps)
+        half = int(math.ceil(m.index(ymax) - 1e-9))
+        if half < 2:
+            raise ValueError("grid: the y grid needs at least 2 cells on each side")
+        ny = 2 * half
+        y = np.zeros(ny + 1)
+        for k in range(half + 1):
+            y[half + k] = m.coord(k)
+            y[half - k] = -m.coord(k)
+        self.y_nodes(y)
+
+    def y_nodes(self, y):
+        """free-hand y from an explicit node table (strictly increasing; the ends are the walls)"""
+        y = np.array(y, dtype=np.float64)
+        if y.ndim != 1 or len(y) < 5 or not np.all(np.diff(y) > 0):
+            raise ValueError("free-hand y: at least 5 strictly increasing nodes")
+        self.ynode = y
+        self.c.ny = len(y) - 1; self.c.Ly = float(y[-1] - y[0]); self.c.y_fourier = 0; self.c.move_y = 0
+        self.c.ynode = self.ynode.ctypes.data_as(C.POINTER(C.c_double))
+
     def grid(self, nx, ny, nz, Lx, Ly, Lw):
         """Fourier x and y, nz uniform slices; no sponge, fixed window (tests/test_util.hpp:36-43)."""
         self.c.nx = nx; self.c.Lx = Lx
@@ -128,6 +160,19 @@ class Deck:
         self.c.n_glasers += 1
         return self
 
+    def add_ion(self, mass, Z0, Zmax, atoms=1.0, mobile=True, ionization=False, ip_eV=()):
+        """an ion species (src/sim.hpp IonSpecies): mass in electron masses, atoms per plasma electron, charge
+        states Z0 .. Zmax, ionization potentials [eV] of the levels Z0 ... Zmax-1 (indexed by charge state)"""
+        if self.c.n_ions >= 4:
+            raise ValueError("at most 4 ion species")
+        s = self.c.ions[self.c.n_ions]
+        s.mass = mass; s.atoms = atoms; s.Z0 = Z0; s.Zmax = Zmax; s.mobile = int(mobile); s.ionization = int(ionization)
+        s.n_ip = len(ip_eV)
+        for k, v in enumerate(ip_eV):
+            s.ip_eV[k] = v
+        self.c.n_ions += 1
+        return self
+
     def set(self, **kw):
         for k, v in kw.items():
             if not hasattr(self.c, k):
@@ -141,6 +186,50 @@ class Deck:
         if self.c.dz_push == 0:
             self.c.dz_push = self.c.dz
         return C.byref(self.c)
+
+
+class StepMapping:
+    """A grid from a table of steps h_k at reference positions x_k >= 0 (src/grid.cpp:11-52): h(x) is linear
+    between the points and constant beyond them, the index is xi(x) = int_0^x dx'/h(x') and the nodes are
+    X(i) = xi^-1(i).  On a segment with slope b, xi = xi_k + ln(1 + b d/h_k)/b and X = x_k + h_k (e^{b t} - 1)/b."""
+
+    def __init__(self, x, h):
+        x, h = [float(v) for v in x], [float(v) for v in h]
+        if not x or len(x) != len(h):
+            raise ValueError("needs as many steps as reference points (at least one)")
+        for k in range(len(x)):
+            if not x[k] >= 0 or (k and not x[k] > x[k - 1]) or not h[k] > 0:
+                raise ValueError("reference points >= 0 and increasing, steps > 0")
+        if x[0] > 0:
+            x, h = [0.0] + x, [h[0]] + h
+        K = len(x)
+        self.x, self.h, self.b, self.xi = x, h, [0.0] * K, [0.0] * K
+        for k in range(K - 1):
+            L = x[k + 1] - x[k]
+            self.b[k] = (h[k + 1] - h[k]) / L
+            self.xi[k + 1] = self.xi[k] + (L / h[k]) * _log1p_over(self.b[k] * L / h[k])
+
+    def index(self, x):
+        if x <= 0:
+            return x / self.h[0]
+        k = min(bisect.bisect_right(self.x, x) - 1, len(self.x) - 1)
+        d = x - self.x[k]
+        return self.xi[k] + (d / self.h[k]) * _log1p_over(self.b[k] * d / self.h[k])
+
+    def coord(self, xi):
+        if xi <= 0:
+            return xi * self.h[0]
+        k = min(bisect.bisect_right(self.xi, xi) - 1, len(self.xi) - 1)
+        t = xi - self.xi[k]
+        return self.x[k] + self.h[k] * t * _expm1_over(self.b[k] * t)
+
+
+def _log1p_over(y):      # ln(1 + y)/y, its limit 1 - y/2 near 0
+    return 1.0 - 0.5 * y if abs(y) < 1e-8 else math.log1p(y) / y
+
+
+def _expm1_over(y):      # (e^y - 1)/y
+    return 1.0 + 0.5 * y if abs(y) < 1e-8 else math.expm1(y) / y
 
 
 def _load():
@@ -273,8 +362,18 @@ class Sim(SimBase):
     def _info(self, i):
         check(lib().g3d_sim_info_get(self.h, C.byref(i)))
 
+    FAMILIES = ("stream_halfcell", "pass_rows_fwd", "pass_cols", "pass_rows_inv", "unused", "beam_push", "currents",
+                "sponge_real_fields")
+
     def timing(self, on):
-        check(lib().g3d_sim_timing(self.h, int(on)))
+        """HIP event timing: True = every kernel family, False = off, or a list of family names"""
+        if on is True:
+            mask = -1
+        elif not on:
+            mask = 0
+        else:
+            mask = sum(1 << self.FAMILIES.index(n) for n in on)
+        check(lib().g3d_sim_timing(self.h, mask))
 
     def timers(self):
         nf = 8

@@ -98,10 +98,12 @@ typedef struct {
     const g3d_cpx *bnd;                           /* 6 planes: slice b from above (unused on one GPU: b = N) */
     const g3d_cpx *j0x, *j0y, *jkx, *jky, *jkz;   /* ns + 1 half-cell planes each */
     g3d_cpx *bkRx, *bkRy, *bkEz, *bkBz;           /* backup of slice n (march_slice with passes), may be null */
-    const double *kx, *ky, *iD, *iA, *hcg, *wlo, *whi;
+    const double *kx, *ky, *iD, *iA, *hcg, *wlo, *whi;   /* ky = NULL: free-hand y (k_x lines between walls) */
     double D;
     int a, b, N, nx, ny;
     long nk;
+    double *scp;            /* free-hand y: 2 nk doubles of device scratch (the Thomas factors of a line; the
+                               reference's sRx ... scp lines); unused for Fourier y */
 } g3d_march_ctx;
 
 /* ------------------------------------------------------------------ runtime and memory (src/portable.hpp) */
@@ -163,8 +165,9 @@ int g3d_sim_set(g3d_sim* sim, const char* name, const void* host, size_t bytes);
 int g3d_sim_set_particles(g3d_sim* sim, const g3d_particle* p, long n);
 int g3d_sim_set_joining(g3d_sim* sim, const g3d_particle* p, long n);
 int g3d_sim_get_joining(g3d_sim* sim, g3d_particle* p, long cap, long* n);
-/* enable (1) or disable (0) HIP event timing of the kernel families; read the accumulated milliseconds */
-int g3d_sim_timing(g3d_sim* sim, int enable);
+/* HIP event timing of the kernel families: bit f of mask times family f (-1: all, 0: off); clears the sums.
+ * g3d_sim_timers reads the accumulated milliseconds and launch counts with the family names. */
+int g3d_sim_timing(g3d_sim* sim, int mask);
 int g3d_sim_timers(g3d_sim* sim, double* ms, long* counts, int nfam, char* names, int name_len);
 
 #ifdef __cplusplus

@@ -470,9 +470,121 @@ static void march_point(const or_march_ctx* c, long i, int n) {
     c->Lx[o] = sub(Lsx, scl(0.25 * h, sub(mul(a, Ezn), mul(b, Bzn))));
     c->Ly[o] = sub(Lsy, scl(0.25 * h, add(mul(a, Bzn), mul(b, Ezn))));
 }
-void or_march(const or_march_ctx* c, int lo, int hi) {               /* src/sim_fields.cpp:19-24 (CPU order) */
+/* ---------------------------------------------------------------- march of a k_x line, free-hand y
+ * (src/sim_kernels.hpp:213-313): explicit parts of R and L in scratch lines, the E_z system on the nodes 1..ny-1
+ * and the B_z system on the half-nodes 0..ny-1 by Thomas elimination, then R and L.  Scratch: 5 lines of cx and
+ * one of double per k_x, element j of line ik at j nx + ik. */
+typedef struct { cx *sRx, *sRy, *sLx, *sLy, *rhs; double* cp; } line_scratch;
+static void march_line(const or_march_ctx* c, const line_scratch* w, int kx, int n) {
+    const int nx = c->nx, ny = c->ny;
+    const long nk = c->nk;
+    const long P0 = (long)(n - c->a) * nk + kx, P1 = P0 + nk;
+    const cx a = ik(c->kx[kx]);
+    const double kx2 = c->kx[kx] * c->kx[kx], D = c->D, wl = c->wlo[n], wu = c->whi[n];
+    const double *iD = c->iD, *iA = c->iA;
+#define SL(A, j) (A)[(long)(j) * nx + kx]
+#define NX(A, j) (((j) < 0 || (j) >= ny) ? mk(0, 0) : (A)[P1 + (long)(j) * nx])
+    for (int j = 0; j < ny; ++j) {
+        const long o = P0 + (long)j * nx;
+        if (j == 0) SL(w->sRx, 0) = mk(0, 0);
+        else {
+            const cx Jx = add(scl(wl, add(c->j0x[o], c->jkx[o])), scl(wu, add(c->j0x[o + nk], c->jkx[o + nk])));
+            const cx bm = scl(iA[j], sub(c->Bz[o], c->Bz[o - nx]));
+            SL(w->sRx, j) = sub(add(c->Rx[o], scl(D, add(mul(a, c->Ez[o]), bm))), scl(D, Jx));
+        }
+        const cx Jy = add(scl(wl, add(c->j0y[o], c->jky[o])), scl(wu, add(c->j0y[o + nk], c->jky[o + nk])));
+        const cx ezu = j + 1 < ny ? c->Ez[o + nx] : mk(0, 0);
+        SL(w->sRy, j) = sub(add(c->Ry[o], scl(D, sub(scl(iD[j], sub(ezu, c->Ez[o])), mul(a, c->Bz[o])))), scl(D, Jy));
+    }
+    if (n == c->N - 1) {
+        for (int j = 0; j < ny; ++j) {
+            const long o = P0 + (long)j * nx;
+            c->Rx[o] = SL(w->sRx, j); c->Ry[o] = SL(w->sRy, j);
+            c->Ez[o] = c->Bz[o] = c->Lx[o] = c->Ly[o] = mk(0, 0);
+        }
+        return;
+    }
+    const double h = c->hcg[n + 2], cc = 0.25 * h * (D + 0.25 * h);
+#define PEZ(j) ((j) == 0 ? mk(0, 0) : NX(c->Ez, j))
+    for (int j = 0; j < ny; ++j) {
+        const long u = P1 + (long)j * nx;
+        if (j == 0) SL(w->sLx, 0) = mk(0, 0);
+        else {
+            const cx bm = scl(iA[j], sub(NX(c->Bz, j), NX(c->Bz, j - 1)));
+            SL(w->sLx, j) = sub(sub(NX(c->Lx, j), scl(0.25 * h, sub(mul(a, PEZ(j)), bm))), scl(0.5 * h, c->jkx[u]));
+        }
+        const cx bp = scl(iD[j], sub(PEZ(j + 1), PEZ(j)));
+        SL(w->sLy, j) = sub(sub(NX(c->Ly, j), scl(0.25 * h, add(mul(a, NX(c->Bz, j)), bp))), scl(0.5 * h, c->jky[u]));
+    }
+    double cp = 0;
+    cx dp = mk(0, 0);
+    for (int j = 1; j < ny; ++j) {                       /* E_z, nodes */
+        const cx Qp = scl(0.5, add(neg(mul(a, sub(NX(c->Rx, j), NX(c->Lx, j)))),
+                                   scl(iA[j], sub(sub(NX(c->Ly, j), NX(c->Ry, j)), sub(NX(c->Ly, j - 1), NX(c->Ry, j - 1))))));
+        const cx SE = add(neg(mul(a, sub(SL(w->sRx, j), SL(w->sLx, j)))),
+                          scl(iA[j], sub(sub(SL(w->sLy, j), SL(w->sRy, j)), sub(SL(w->sLy, j - 1), SL(w->sRy, j - 1)))));
+        const cx r = sub(sub(sub(PEZ(j), scl(0.5 * h, Qp)), scl(0.25 * h, SE)), scl(h, c->jkz[P1 + (long)j * nx]));
+        const double lo = j > 1 ? -cc * iA[j] * iD[j - 1] : 0, up = j < ny - 1 ? -cc * iA[j] * iD[j] : 0;
+        const double den = 1 + cc * kx2 + cc * iA[j] * (iD[j - 1] + iD[j]) - lo * cp;
+        dp = dvs(sub(r, scl(lo, dp)), den);
+        cp = up / den;
+        SL(w->rhs, j) = dp; SL(w->cp, j) = cp;
+    }
+    {
+        cx next = mk(0, 0);
+        for (int j = ny - 1; j >= 1; --j) { next = sub(SL(w->rhs, j), scl(SL(w->cp, j), next)); c->Ez[P0 + (long)j * nx] = next; }
+        c->Ez[P0] = mk(0, 0);
+    }
+    cp = 0; dp = mk(0, 0);
+    for (int j = 0; j < ny; ++j) {                       /* B_z, half-nodes */
+        const cx fu = j + 1 < ny ? add(NX(c->Rx, j + 1), NX(c->Lx, j + 1)) : mk(0, 0), f0 = add(NX(c->Rx, j), NX(c->Lx, j));
+        const cx Cp = scl(0.5, sub(mul(a, add(NX(c->Ry, j), NX(c->Ly, j))), scl(iD[j], sub(fu, f0))));
+        const cx su = j + 1 < ny ? add(SL(w->sRx, j + 1), SL(w->sLx, j + 1)) : mk(0, 0), s0 = add(SL(w->sRx, j), SL(w->sLx, j));
+        const cx SB = sub(mul(a, add(SL(w->sRy, j), SL(w->sLy, j))), scl(iD[j], sub(su, s0)));
+        const cx r = sub(sub(NX(c->Bz, j), scl(0.5 * h, Cp)), scl(0.25 * h, SB));
+        const double al = j > 0 ? iA[j] : 0, au = j < ny - 1 ? iA[j + 1] : 0;
+        const double lo = -cc * iD[j] * al, up = -cc * iD[j] * au;
+        const double den = 1 + cc * kx2 + cc * iD[j] * (al + au) - lo * cp;
+        dp = dvs(sub(r, scl(lo, dp)), den);
+        cp = up / den;
+        SL(w->rhs, j) = dp; SL(w->cp, j) = cp;
+    }
+    {
+        cx next = mk(0, 0);
+        for (int j = ny - 1; j >= 0; --j) { next = sub(SL(w->rhs, j), scl(SL(w->cp, j), next)); c->Bz[P0 + (long)j * nx] = next; }
+    }
+    for (int j = 0; j < ny; ++j) {                       /* R and L */
+        const long o = P0 + (long)j * nx;
+        if (j == 0) { c->Rx[o] = mk(0, 0); c->Lx[o] = mk(0, 0); }
+        else {
+            const cx bm = scl(iA[j], sub(c->Bz[o], c->Bz[o - nx]));
+            c->Rx[o] = add(SL(w->sRx, j), scl(D, add(mul(a, c->Ez[o]), bm)));
+            c->Lx[o] = sub(SL(w->sLx, j), scl(0.25 * h, sub(mul(a, c->Ez[o]), bm)));
+        }
+        const cx ezu = j + 1 < ny ? c->Ez[o + nx] : mk(0, 0);
+        const cx bp = scl(iD[j], sub(ezu, c->Ez[o]));
+        c->Ry[o] = add(SL(w->sRy, j), scl(D, sub(bp, mul(a, c->Bz[o]))));
+        c->Ly[o] = sub(SL(w->sLy, j), scl(0.25 * h, add(mul(a, c->Bz[o]), bp)));
+    }
+#undef PEZ
+#undef NX
+#undef SL
+}
+
+void or_march(const or_march_ctx* c, int lo, int hi) {               /* src/sim_fields.cpp:19-29 (CPU order) */
+    if (c->ky) {
+        for (int n = hi - 1; n >= lo; --n)
+            for (long i = 0; i < c->nk; ++i) march_point(c, i, n);
+        return;
+    }
+    line_scratch w;
+    w.sRx = (cx*)malloc(sizeof(cx) * 5 * (size_t)c->nk);
+    w.sRy = w.sRx + c->nk; w.sLx = w.sRy + c->nk; w.sLy = w.sLx + c->nk; w.rhs = w.sLy + c->nk;
+    w.cp = (double*)malloc(sizeof(double) * (size_t)c->nk);
     for (int n = hi - 1; n >= lo; --n)
-        for (long i = 0; i < c->nk; ++i) march_point(c, i, n);
+        for (int i = 0; i < c->nx; ++i) march_line(c, &w, i, n);
+    free(w.cp);
+    free(w.sRx);
 }
 
 /* ---------------------------------------------------------------- spectra and real fields */

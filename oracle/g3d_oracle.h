@@ -77,6 +77,7 @@ uint64_t or_key_step(uint64_t seed, long step);
 uint64_t or_key_slice(uint64_t ks, int n, int pass);
 uint64_t or_key_particle(uint64_t kl, uint64_t id);
 /* march (src/sim_kernels.hpp:173-210), slices hi-1 ... lo, Fourier y */
+/* the march of slices hi-1 .. lo: per mode (Fourier y) or, with ky = NULL, per k_x line of the free-hand y grid */
 void or_march(const or_march_ctx* c, int lo, int hi);
 /* spectra of a streaming half-cell current (src/sim_plasma.cpp:94-117, Fourier y): J (5 nk) -> jsn (3 nk) */
 void or_stream_spectra(int nx, int ny, const double* J, const double* kxs, const double* kys, const double* filt,

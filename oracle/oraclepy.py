@@ -135,6 +135,14 @@ class Grid:
         self.masky = np.array([prof(min(j * self.dy, c.Ly - j * self.dy), c.sponge_y) for j in range(ny)])
         self.iD = np.full(ny, 1 / self.dy)
         self.iA = np.full(ny, 1 / self.dy)
+        self.freehand = not c.y_fourier
+        if self.freehand:                    # src/sim_setup.cpp:86-102: walls at the end nodes
+            Y = np.array(deck.ynode, dtype=np.float64)
+            Dl = np.diff(Y)
+            A = np.empty(ny)
+            A[0] = 0.5 * Dl[0]
+            A[1:] = 0.5 * (Dl[:-1] + Dl[1:])
+            self.iD, self.iA = 1 / Dl, 1 / A
         Z = np.array(deck.zeta, dtype=np.float64)
         self.zeta = Z
         N = self.N
@@ -180,7 +188,8 @@ def march(g, arrays, D, lo=0, hi=None):
     hi = g.N if hi is None else hi
     a = {k: np.ascontiguousarray(v, dtype=np.complex128).copy() for k, v in arrays.items()}
     c = MarchCtx(*[ptr(a[k]) for k in ("Rx", "Ry", "Ez", "Bz", "Lx", "Ly", "j0x", "j0y", "jkx", "jky", "jkz")],
-                 dptr(g.kx), dptr(g.ky), dptr(g.iD), dptr(g.iA), dptr(g.hcg), dptr(g.wlo), dptr(g.whi),
+                 dptr(g.kx), None if g.freehand else dptr(g.ky), dptr(g.iD), dptr(g.iA), dptr(g.hcg), dptr(g.wlo),
+                 dptr(g.whi),
                  D, 0, g.N, g.nx, g.ny, g.nk)
     lib().or_march(C.byref(c), lo, hi)
     return a

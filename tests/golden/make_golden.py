@@ -34,6 +34,15 @@ def main():
     ref.call("march", gi.MARCH_D)
     out["march"] = dict(**{"in_" + k: v for k, v in arr.items()},
                         **{"out_" + k: ref.array(k) for k in ("Rx", "Ry", "Ez", "Bz", "Lx", "Ly")})
+    # march of the k_x lines of a free-hand y grid (src/sim_kernels.hpp:213-313)
+    deck = gi.small_fh_deck()
+    ref = refpy.RefSim(deck)
+    arr = gi.march_input(ref)
+    for k, v in arr.items():
+        ref.set(k, v)
+    ref.call("march", gi.MARCH_D)
+    out["march_fh"] = dict(ynode=deck.ynode, **{"in_" + k: v for k, v in arr.items()},
+                           **{"out_" + k: ref.array(k) for k in ("Rx", "Ry", "Ez", "Bz", "Lx", "Ly")})
     # streaming current spectra (src/sim_plasma.cpp:94-117)
     deck = gi.small_deck(smooth=1.5)
     ref = refpy.RefSim(deck)

@@ -16,6 +16,16 @@ def small_deck(smooth=0.0):
     return d
 
 
+def small_fh_deck():
+    """free-hand y: 14 cells between walls at +-6.6/k0, graded 0.4 -> 0.8/k0; 8 Fourier x modes, 12 slices"""
+    d = g3d_amd.Deck(LAMBDA0)
+    d.c.nx = 8; d.c.Lx = 16.0
+    d.y_table([0.0, 2.0, 5.0], [0.4, 0.4, 0.8], 6.5)
+    d.zeta_uniform(12, 11 * 0.5)
+    d.set(dz=5.0, z_end=100.0, plasma_on=1, n0=1e-2, ppc_x=1, ppc_y=1, move_x=0, sponge=0.0, sponge_y=0.0)
+    return d
+
+
 def bunch_small_deck():
     d = g3d_amd.Deck(LAMBDA0).grid(8, 8, 16, 16.0, 16.0, 15 * 0.5)
     d.set(dz=20.0, z_end=100.0, laser_on=1, laser_pol=1, a0=1.0, theta=2e-3, tau=30.0, start_fwhm=0.0,

@@ -36,6 +36,21 @@ def test_march_bit_exact():
         assert np.array_equal(sim.array(k), want[k]), k
 
 
+def test_march_freehand_bit_exact():
+    """the line march of the free-hand y grid (k_march_lines: one lane per k_x, both eliminations in one sweep up
+    the line) on the golden free-hand inputs, bit for bit against the oracle"""
+    deck = gi.small_fh_deck()
+    sim = g3d_amd.Sim(deck)
+    g = orc.Grid(deck)
+    arr = gi.march_input(sim)
+    for k, v in arr.items():
+        sim.set(k, v)
+    sim.call("march", gi.MARCH_D)
+    want = orc.march(g, arr, gi.MARCH_D)
+    for k in ("Rx", "Ry", "Ez", "Bz", "Lx", "Ly"):
+        assert np.array_equal(sim.array(k), want[k]), k
+
+
 def test_stream_spectra():
     """the fused row/column kernels of the march: x transform, the pair unpacked per row, y transforms, filters and
     smoothing.  The reference unpacks after the 2D transform; the two orders agree to rounding."""

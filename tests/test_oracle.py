@@ -111,6 +111,38 @@ def test_march_golden():
         assert np.array_equal(out[k], g["out_" + k]), k
 
 
+def test_march_freehand_golden():
+    """the march of the k_x lines between conducting walls (free-hand y) against the reference's own output"""
+    g = gold("march_fh")
+    deck = gi.small_fh_deck()
+    assert np.array_equal(deck.ynode, g["ynode"])
+    grid = orc.Grid(deck)
+    arr = {k[3:]: g[k] for k in g.files if k.startswith("in_")}
+    out = orc.march(grid, arr, gi.MARCH_D)
+    for k in ("Rx", "Ry", "Ez", "Bz", "Lx", "Ly"):
+        assert np.array_equal(out[k], g["out_" + k]), k
+
+
+def test_march_freehand_walls_and_front():
+    """the wall rules of the line march: E_z, R_x, L_x vanish on the wall node 0 of every slice below the front,
+    and the front slice keeps only R (L = E_z = B_z = 0)"""
+    deck = gi.small_fh_deck()
+    grid = orc.Grid(deck)
+    r = np.random.default_rng(5)
+    N, nk, nx = grid.N, grid.nk, grid.nx
+    arr = {k: r.standard_normal(N * nk) + 1j * r.standard_normal(N * nk) for k in ("Rx", "Ry", "Ez", "Bz", "Lx", "Ly")}
+    arr.update({k: r.standard_normal((N + 1) * nk) + 1j * r.standard_normal((N + 1) * nk)
+                for k in ("j0x", "j0y", "jkx", "jky", "jkz")})
+    out = orc.march(grid, arr, 1.0)
+    node0 = np.zeros((N, deck.c.ny, nx), bool)
+    node0[:, 0, :] = True
+    for k in ("Ez", "Rx", "Lx"):
+        assert np.all(out[k].reshape(N, deck.c.ny, nx)[:-1, 0, :] == 0), k
+    front = slice((N - 1) * nk, N * nk)
+    for k in ("Ez", "Bz", "Lx", "Ly"):
+        assert np.all(out[k][front] == 0), k
+
+
 def test_march_vacuum_free_wave():
     """a mode with no current and D = 0 (the level-s0 solve) keeps R and gives L = O(h k) (src/sim_kernels.hpp:184)"""
     deck = gi.small_deck()
