@@ -27,6 +27,24 @@ G3D_HD cx conjg(cx a) { return cx{a.re, -a.im}; }
 G3D_HD cx ik(double k) { return cx{0, k}; }                       // i k
 G3D_HD cx times_minus_i(cx z) { return cx{z.im, -z.re}; }         // -i z
 
+// single-precision complex of the FP32-mixed form, with the same operand order
+struct cxf { float re, im; };
+G3D_HD cxf mkf(float r, float i = 0) { return cxf{r, i}; }
+G3D_HD cxf operator+(cxf a, cxf b) { return cxf{a.re + b.re, a.im + b.im}; }
+G3D_HD cxf operator-(cxf a, cxf b) { return cxf{a.re - b.re, a.im - b.im}; }
+G3D_HD cxf operator-(cxf a) { return cxf{-a.re, -a.im}; }
+G3D_HD cxf operator*(cxf a, cxf b) { return cxf{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+G3D_HD cxf operator*(float s, cxf a) { return cxf{s * a.re, s * a.im}; }
+G3D_HD cxf operator*(cxf a, float s) { return cxf{s * a.re, s * a.im}; }
+G3D_HD cxf operator/(cxf a, float s) { return cxf{a.re / s, a.im / s}; }
+G3D_HD cxf conjg(cxf a) { return cxf{a.re, -a.im}; }
+G3D_HD cxf to_f(cx a) { return cxf{float(a.re), float(a.im)}; }
+G3D_HD cx to_d(cxf a) { return cx{a.re, a.im}; }
+// a twiddle of the double table in the precision of the transform
+template <class C> G3D_HD C twiddle_as(cx w);
+template <> G3D_HD cx twiddle_as<cx>(cx w) { return w; }
+template <> G3D_HD cxf twiddle_as<cxf>(cx w) { return to_f(w); }
+
 // ------------------------------------------------------------------ grids
 // Quadratic spline (TSC) weights about the nearest integer point of xi (src/sim_kernels.hpp:21-26).
 G3D_HD void tsc_w(double xi, int idx[3], double w[3]) {
@@ -129,7 +147,9 @@ struct FieldView {
 
 // E, B at (x, y, zeta): TSC in x and y, linear in zeta between slices (src/sim_kernels.hpp:44-72).  The sums run
 // over y then x in the reference's order.
-G3D_HD void gather(const FieldView& c, double x, double y, double zeta, double E[3], double B[3]) {
+// YF: 1 Fourier y, 0 free-hand y, -1 decided at run time from c.yf
+template <int YF>
+G3D_HD void gather_t(const FieldView& c, double x, double y, double zeta, double E[3], double B[3]) {
     E[0] = E[1] = E[2] = B[0] = B[1] = B[2] = 0;
     if (!c.self) return;
     int n; double fz;
@@ -140,7 +160,7 @@ G3D_HD void gather(const FieldView& c, double x, double y, double zeta, double E
     int ix[3], jn[3], jh[3]; double wx[3], wn[3], wh[3];
     tsc_periodic(x, c.dx, c.nx, ix, wx);
     const double gz = 1 - fz;
-    if (c.yf) {
+    if (YF == 1 || (YF < 0 && c.yf)) {
         // Fourier y: all components on the periodic nodes; the nine offsets and weights are shared by the six
         // components and every array stays in registers
         tsc_periodic(y, c.dy, c.ny, jn, wn);
@@ -164,23 +184,30 @@ G3D_HD void gather(const FieldView& c, double x, double y, double zeta, double E
         E[0] = v[0]; E[1] = v[1]; E[2] = v[2]; B[0] = v[3]; B[1] = v[4]; B[2] = v[5];
         return;
     }
+    if (YF == 1) return;
     tsc_freehand(c.yax, y, jn, wn, jh, wh);
+#pragma unroll
     for (int comp = 0; comp < 6; ++comp) {
         const float* p0 = c.F + (size_t(comp) * (c.ns + 2) + fl) * c.nk;
         const float* p1 = p0 + c.nk;
-        const bool hf = !c.yf && half_comp(comp);
-        const int* jj = hf ? jh : jn;
-        const double* wy = hf ? wh : wn;
+        const bool hf = half_comp(comp);
         double v = 0;
+#pragma unroll
         for (int j = 0; j < 3; ++j) {
-            if (jj[j] < 0 || jj[j] >= c.ny) continue;
+            const int jj = hf ? jh[j] : jn[j];
+            const double wy = hf ? wh[j] : wn[j];
+            if (jj < 0 || jj >= c.ny) continue;
+#pragma unroll
             for (int i = 0; i < 3; ++i) {
-                const long k = long(jj[j]) * c.nx + ix[i];
-                v += wx[i] * wy[j] * (gz * double(p0[k]) + fz * double(p1[k]));
+                const long k = long(jj) * c.nx + ix[i];
+                v += wx[i] * wy * (gz * double(p0[k]) + fz * double(p1[k]));
             }
         }
         if (comp < 3) E[comp] = v; else B[comp - 3] = v;
     }
+}
+G3D_HD void gather(const FieldView& c, double x, double y, double zeta, double E[3], double B[3]) {
+    gather_t<-1>(c, x, y, zeta, E, B);
 }
 
 // relativistic Boris step of the momentum u over dt (src/sim_stream.hpp:8-18)

@@ -227,6 +227,8 @@ void Sim::setup_tiles() {
     tile_off.resize(tiles.ntile + 1);
     slab.resize(size_t(tiles.ntile) * 5 * tiles.WX * tiles.WY);
     tiles.off = tile_off.data(); tiles.slab = slab.data(); tiles.n_tiled = 0;
+    const char* e = getenv("G3D_STREAM_FWIN");
+    tiles.fwin = e ? atoi(e) : 1;
 }
 
 Sim::~Sim() {
@@ -844,11 +846,11 @@ void Sim::march_plasma(double s0, double s1) {
     m.kf = cfg.k_filter;
     const LineView L = lview(D);
     // the columns (Fourier y) or lines (free-hand y) of a pass
-    static const bool split = [] { const char* e = getenv("G3D_PASS_SPLIT"); return e && atoi(e) > 0; }();
+    static const bool fused = [] { const char* e = getenv("G3D_PASS_FUSED"); return e && atoi(e) > 0; }();
     auto pass_work = [&](cx* X, int kind) {
         if (!yf) pass_lines(m, L, X, tp3.data(), nullptr, kind);
-        else if (split) pass_modes(m, fft, X, tp3.data(), nullptr, kind);
-        else pass_cols(m, fft, X, tp3.data(), nullptr, kind);
+        else if (fused) pass_cols(m, fft, X, tp3.data(), nullptr, kind);
+        else pass_modes(m, fft, X, tp3.data(), nullptr, kind);
     };
     const long fpl = long(ns + 2) * nk;
     auto real_fields = [&](int n) {

@@ -1,4 +1,6 @@
 // Batched transverse FFT (rows, columns) and the runtime/memory entry points of the C ABI.
+#include <cstdlib>
+
 #include "g3d_fft.hpp"
 #include "g3d_fft_core.hpp"
 #include "g3d_runtime.hpp"
@@ -132,10 +134,12 @@ void fft_rows(const FftPlan& p, cx* data, long nrows, bool inverse, double scale
 }
 void fft_cols(const FftPlan& p, cx* data, int count, bool inverse, double scale) {
     if (count <= 0) return;
+    static const int forced = [] { const char* e = getenv("G3D_FFT_COLS_NC"); return e ? atoi(e) : 0; }();
     int nc = kTile / p.ny;
     if (nc > p.nx) nc = p.nx;
     while (nc > 2 && long(count) * (p.nx / nc) < 512) nc >>= 1;
     if (nc < 1) nc = 1;
+    if (forced > 0 && forced <= p.nx && p.nx % forced == 0) nc = forced;
     const size_t lds = (size_t(nc) * (p.ny + p.ny / 8) + p.ny / 2) * sizeof(cx);
     const long blocks = long(count) * (p.nx / nc);
     const int thr = threads_for(nc * (p.ny / 8 > 0 ? p.ny / 8 : 1));

@@ -16,12 +16,12 @@ __device__ __forceinline__ int lpos(int i) { return i + (i >> 3); }
 // K radix-2 DIT stages s0 .. s0+K-1 for one closed group of 2^K elements held in registers: the members
 // h 2^(s0-1+K) + m 2^(s0-1) + r, m = 0 .. 2^K-1.  Same butterflies, same twiddles, same order of operations as
 // the reference's iterative transform (src/fft.hpp:203-214), so the result is bit for bit the same.
-template <int K>
-__device__ __forceinline__ void group_stages(cx* seq, int n, int s0, int h, int r, const cx* tws) {
+template <int K, class C>
+__device__ __forceinline__ void group_stages(C* seq, int n, int s0, int h, int r, const C* tws) {
     constexpr int G = 1 << K;
     const int low = s0 - 1;
     const int base = (h << (low + K)) + r;
-    cx v[G];
+    C v[G];
 #pragma unroll
     for (int m = 0; m < G; ++m) v[m] = seq[lpos(base + (m << low))];
 #pragma unroll
@@ -31,8 +31,8 @@ __device__ __forceinline__ void group_stages(cx* seq, int n, int s0, int h, int 
         for (int m = 0; m < G; ++m) {
             if (m & half) continue;
             const int j = ((m & (half - 1)) << low) + r;
-            const cx w = tws[j * step];
-            const cx u = v[m], x = v[m + half] * w;
+            const C w = tws[j * step];
+            const C u = v[m], x = v[m + half] * w;
             v[m] = u + x;
             v[m + half] = u - x;
         }
@@ -43,7 +43,8 @@ __device__ __forceinline__ void group_stages(cx* seq, int n, int s0, int h, int 
 
 // the transform of nseq sequences of length n = 2^L in LDS (bit-reversed order, padded layout, sequence q at
 // q * spad): the stages in groups of three, one LDS round trip and one barrier per group
-__device__ __forceinline__ void lds_fft(cx* buf, int n, int L, int nseq, int spad, const cx* tws) {
+template <class C>
+__device__ __forceinline__ void lds_fft(C* buf, int n, int L, int nseq, int spad, const C* tws) {
     for (int s0 = 1; s0 <= L; s0 += 3) {
         const int K = L - s0 + 1 < 3 ? L - s0 + 1 : 3;
         const int lgn = L - K;                       // log2 of the groups per sequence
@@ -52,20 +53,22 @@ __device__ __forceinline__ void lds_fft(cx* buf, int n, int L, int nseq, int spa
         for (int t = threadIdx.x; t < tot; t += blockDim.x) {
             const int q = t >> lgn, g = t & ((1 << lgn) - 1);
             const int r = g & ((1 << low) - 1), h = g >> low;
-            cx* seq = buf + q * spad;
-            if (K == 3) group_stages<3>(seq, n, s0, h, r, tws);
-            else if (K == 2) group_stages<2>(seq, n, s0, h, r, tws);
-            else group_stages<1>(seq, n, s0, h, r, tws);
+            C* seq = buf + q * spad;
+            if (K == 3) group_stages<3, C>(seq, n, s0, h, r, tws);
+            else if (K == 2) group_stages<2, C>(seq, n, s0, h, r, tws);
+            else group_stages<1, C>(seq, n, s0, h, r, tws);
         }
         __syncthreads();
     }
 }
 
-__device__ __forceinline__ void load_twiddles(cx* tws, const cx* tw, int hn, bool inv) {
+// the double twiddle table in the transform's precision (conjugated for the inverse)
+template <class C>
+__device__ __forceinline__ void load_twiddles(C* tws, const cx* tw, int hn, bool inv) {
     for (int k = threadIdx.x; k < hn; k += blockDim.x) {
         cx w = tw[k];
         if (inv) w.im = -w.im;
-        tws[k] = w;
+        tws[k] = twiddle_as<C>(w);
     }
 }
 

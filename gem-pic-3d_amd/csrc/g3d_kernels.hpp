@@ -42,6 +42,7 @@ struct StreamArgs {
 // window go to jsr with global atomics.  Particles from off[ntile] on (joining, freed) are not tiled.
 struct TileGeom {
     int TX, TY, M, WX, WY, ntx, nty;
+    int fwin;                         // stage the tile's field window in LDS for the gather
     long ntile, n_tiled;              // tiles, particles in tiles (= off[ntile])
     const long* off;
     double* slab;

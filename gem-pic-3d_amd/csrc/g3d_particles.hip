@@ -84,15 +84,57 @@ __global__ void __launch_bounds__(kT) k_bunch_push(PushArgs c, Parts P, long np,
 }
 
 // ---------------------------------------------------------------- streaming particles (src/sim_stream.hpp)
-// LDS window of a tile: 5 planes of WX x WY nodes from node (x0, y0) (periodic); null for untiled particles
+// LDS windows of a tile, WX x WY nodes from node (x0, y0) (periodic); null for untiled particles:
+//   lds   the 5 current planes of the deposit (double)
+//   fld   the fields of slices fl and fl+1 for the gather (float, [slice][component][WY][WX]), loaded once per
+//         workgroup instead of 108 scattered loads per particle
 struct Win {
     double* lds;
-    int x0, y0, WX, WY;
+    const float* fld;
+    int x0, y0, WX, WY, fl;
 };
+
+// E, B from the tile's field window (Fourier y): the same weights and the same order of the sums as gather(), so
+// the values are identical; false when the stencil or the slice pair is not in the window (then the caller
+// gathers from F)
+__device__ __forceinline__ bool gather_window(const FieldView& c, const Win& w, double x, double y, double zeta,
+                                              double E[3], double B[3]) {
+    if (!c.self) return false;
+    int n; double fz;
+    c.zax.locate(zeta, n, fz);
+    if (n == c.a - 1 && fz > 1 - 1e-9) { n = c.a; fz = 0; }
+    if (n - c.a != w.fl) return false;
+    int ix[3], jn[3]; double wx[3], wn[3];
+    tsc_periodic(x, c.dx, c.nx, ix, wx);
+    tsc_periodic(y, c.dy, c.ny, jn, wn);
+    int u0 = ix[0] - w.x0; u0 = ((u0 % c.nx) + c.nx) % c.nx;
+    int v0 = jn[0] - w.y0; v0 = ((v0 % c.ny) + c.ny) % c.ny;
+    if (u0 + 2 >= w.WX || v0 + 2 >= w.WY) return false;
+    const double gz = 1 - fz;
+    int k[9]; double wt[9];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) { k[3 * j + i] = (v0 + j) * w.WX + u0 + i; wt[3 * j + i] = wx[i] * wn[j]; }
+    const int pl = w.WX * w.WY;
+    double v[6];
+#pragma unroll
+    for (int comp = 0; comp < 6; ++comp) {
+        const float* q0 = w.fld + comp * pl;
+        const float* q1 = w.fld + (6 + comp) * pl;
+        double s = 0;
+#pragma unroll
+        for (int t = 0; t < 9; ++t) s += wt[t] * (gz * double(q0[k[t]]) + fz * double(q1[k[t]]));
+        v[comp] = s;
+    }
+    E[0] = v[0]; E[1] = v[1]; E[2] = v[2]; B[0] = v[3]; B[1] = v[4]; B[2] = v[5];
+    return true;
+}
 
 // TSC deposit of f (dt/Delta) v/(h dx dy) (src/sim_stream.hpp:33-64) into plane comp of the half-cell current:
 // the tile's LDS window when the whole 3 x 3 stencil lies inside it, else jsr with global atomics.  Zero
 // contributions are skipped (adding +-0 to a sum that starts at +0 never changes it).
+template <int YF>
 __device__ __forceinline__ void sdeposit(const StreamArgs& c, const Win* win, double x, double y, double f, double vx,
                                          double vy, double vz) {
     const FieldView& P = c.fv;
@@ -101,7 +143,7 @@ __device__ __forceinline__ void sdeposit(const StreamArgs& c, const Win* win, do
     const long nk = P.nk;
     const int cq = f < 0 ? 3 : 4;
     const bool mv = vx != 0 || vy != 0 || vz != 0;
-    if (P.yf) {
+    if (YF == 1) {
         tsc_periodic(y, P.dy, P.ny, jn, wn);
         double w[9];
 #pragma unroll
@@ -209,6 +251,8 @@ __device__ __forceinline__ void ionize(const StreamArgs& c, PState& p, double E1
 }
 
 // one particle through half-cell n (src/sim_stream.hpp:104-171)
+// YF: the grid (1 Fourier y, 0 free-hand y); ION: the list may hold ions
+template <int YF, bool ION>
 __device__ __forceinline__ void stream_one(const StreamArgs& c, const Win* win, PState& p) {
     const int n = c.n;
     const double zn = c.zs[n], zn1 = c.zs[n + 1], h = zn1 - zn;
@@ -219,10 +263,10 @@ __device__ __forceinline__ void stream_one(const StreamArgs& c, const Win* win, 
     if (p.alive != 1) return;
     const double hl = p.zeta >= zn1 - 1e-9 * h ? h : (p.zeta - zn > 0 ? p.zeta - zn : 0.0);
     const FieldView& P = c.fv;
-    const bool ion = is_ion(p.sp);
+    const bool ion = ION && is_ion(p.sp);
     if (ion && !c.ions[p.sp - SP_ION].mobile) {
         const double f = p.q * p.w * hl / (c.Delta * h * P.dx);
-        if (f != 0) sdeposit(c, win, p.x, p.y, f, 0, 0, 0);
+        if (f != 0) sdeposit<YF>(c, win, p.x, p.y, f, 0, 0, 0);
         const double E1 = field_abs(c, p.x, p.y, p.z, p.zeta), E0 = field_abs(c, p.x, p.y, p.z, zn);
         p.zeta = zn;
         ionize(c, p, E1, E0, hl);
@@ -235,7 +279,8 @@ __device__ __forceinline__ void stream_one(const StreamArgs& c, const Win* win, 
     const double xm = p.x + 0.5 * dt0 * p.px / g0, ym = p.y + 0.5 * dt0 * p.py / g0;
     const double zm = p.z + 0.5 * dt0 * p.pz / g0, zetam = p.zeta - 0.5 * hl;
     double E[3], B[3], Ee[3], Be[3];
-    gather(P, wrap_mod(xm, P.Lx), P.yf ? wrap_mod(ym, P.Ly) : ym, zetam, E, B);
+    const double xg = wrap_mod(xm, P.Lx), yg = YF ? wrap_mod(ym, P.Ly) : ym;
+    if (!(YF && win && win->fld && gather_window(P, *win, xg, yg, zetam, E, B))) gather_t<YF>(P, xg, yg, zetam, E, B);
     c.laser.field(xm + c.X0, zm, zm - (zetam - c.zc), Ee, Be);
     for (int k = 0; k < 3; ++k) { E[k] += Ee[k]; B[k] += Be[k]; }
     double ux = p.px, uy = p.py, uz = p.pz;
@@ -265,45 +310,60 @@ __device__ __forceinline__ void stream_one(const StreamArgs& c, const Win* win, 
     p.px = ux; p.py = uy; p.pz = uz;
     p.zeta = zeta_end;
     const double f = p.q * p.w * dt / (c.Delta * h * P.dx);
-    if (f != 0) sdeposit(c, win, 0.5 * (xo + p.x), 0.5 * (yo + p.y), f, vx, vy, vz);
-    if (!P.yf && (p.y < c.ylo || p.y > c.yhi)) { p.alive = 0; return; }
+    if (f != 0) sdeposit<YF>(c, win, 0.5 * (xo + p.x), 0.5 * (yo + p.y), f, vx, vy, vz);
+    if (!YF && (p.y < c.ylo || p.y > c.yhi)) { p.alive = 0; return; }
     if (ion && !trapped && c.ions[p.sp - SP_ION].ionizing)
         ionize(c, p, field_abs(c, xm, ym, zm, zeta0), field_abs(c, xm, ym, zm, zn), dt);
     if (trapped) {
         p.alive = 5;
         const double vze = uz / g1;
-        if (vze >= c.vz_beam) sdeposit(c, win, p.x, p.y, p.q * p.w / (h * P.dx * vze), ux / g1, uy / g1, vze);
+        if (vze >= c.vz_beam) sdeposit<YF>(c, win, p.x, p.y, p.q * p.w / (h * P.dx * vze), ux / g1, uy / g1, vze);
     }
 }
 
+template <int YF, bool ION>
 __device__ __forceinline__ void stream_index(const StreamArgs& c, const Win* win, Parts S, Parts D, long i) {
     PState p{S.x[i], S.y[i], S.zeta[i], S.px[i], S.py[i], S.pz[i], S.w[i], S.z[i], S.q[i], S.qm[i], S.alive[i],
              S.sp[i], S.id[i]};
-    stream_one(c, win, p);
+    stream_one<YF, ION>(c, win, p);
     D.x[i] = p.x; D.y[i] = p.y; D.zeta[i] = p.zeta; D.px[i] = p.px; D.py[i] = p.py; D.pz[i] = p.pz;
     D.z[i] = p.z; D.q[i] = p.q; D.qm[i] = p.qm; D.alive[i] = p.alive;
 }
 
 // workgroups 0 .. ntile-1: the particles of one tile with the LDS window, written to the tile's slab at the end;
 // the following workgroups: the untiled particles from off[ntile] on, one per lane, global atomics
+template <int YF, bool ION>
 __global__ void __launch_bounds__(kT) k_stream_halfcell(StreamArgs c, Parts S, Parts D, long n, TileGeom tg) {
     extern __shared__ double win_lds[];
     const long b = blockIdx.x;
     if (b < tg.ntile) {
-        const long wn = 5L * tg.WX * tg.WY;
-        for (long k = threadIdx.x; k < wn; k += blockDim.x) win_lds[k] = 0;
-        __syncthreads();
+        const int pl = tg.WX * tg.WY;
+        const long wn = 5L * pl;
         const int tx = int(b % tg.ntx), ty = int(b / tg.ntx);
-        Win w{win_lds, tx * tg.TX - 1 - tg.M, ty * tg.TY - 1 - tg.M, tg.WX, tg.WY};
+        const int x0 = tx * tg.TX - 1 - tg.M, y0 = ty * tg.TY - 1 - tg.M;
+        const FieldView& F = c.fv;
+        const int fl = c.n - F.a;
+        float* fw = reinterpret_cast<float*>(win_lds + wn);
+        for (long k = threadIdx.x; k < wn; k += blockDim.x) win_lds[k] = 0;
+        const bool fwin = tg.fwin && F.self;
+        if (fwin)        // slices fl, fl + 1 of the 6 components: [slice][comp][v][u]
+            for (int k = threadIdx.x; k < 12 * pl; k += blockDim.x) {
+                const int sc = k / pl, r = k - sc * pl, v = r / tg.WX, u = r - v * tg.WX;
+                const int sl = sc / 6, comp = sc - 6 * sl;
+                const int gx = ((x0 + u) % F.nx + F.nx) % F.nx, gy = ((y0 + v) % F.ny + F.ny) % F.ny;
+                fw[k] = F.F[(long(comp) * (F.ns + 2) + fl + sl) * F.nk + long(gy) * F.nx + gx];
+            }
+        __syncthreads();
+        Win w{win_lds, fwin ? fw : nullptr, x0, y0, tg.WX, tg.WY, fl};
         const long hi = tg.off[b + 1];
-        for (long i = tg.off[b] + threadIdx.x; i < hi; i += blockDim.x) stream_index(c, &w, S, D, i);
+        for (long i = tg.off[b] + threadIdx.x; i < hi; i += blockDim.x) stream_index<YF, ION>(c, &w, S, D, i);
         __syncthreads();
         double* sl = tg.slab + b * wn;
         for (long k = threadIdx.x; k < wn; k += blockDim.x) sl[k] = win_lds[k];
         return;
     }
     const long i = tg.n_tiled + (b - tg.ntile) * blockDim.x + threadIdx.x;
-    if (i < n) stream_index(c, nullptr, S, D, i);
+    if (i < n) stream_index<YF, ION>(c, nullptr, S, D, i);
 }
 
 __global__ void k_append_born(Parts b, const int* bcnt, Parts p0, Parts p1, long base, long cap) {
@@ -393,9 +453,17 @@ void bunch_push(const PushArgs& a, Parts p, long np, double s0) {
 void stream_halfcell(const StreamArgs& a, Parts src, Parts dst, long nstrm, const TileGeom& tg) {
     if (nstrm <= 0) return;
     const long tail_blocks = nstrm > tg.n_tiled ? long(blocks_for(nstrm - tg.n_tiled, kT)) : 0;
-    const size_t lds = tg.ntile ? size_t(5) * tg.WX * tg.WY * sizeof(double) : 0;
-    hipLaunchKernelGGL(k_stream_halfcell, dim3(unsigned(tg.ntile + tail_blocks)), dim3(kT), lds, stream(), a, src, dst,
-                       nstrm, tg);
+    // the deposit window (5 double planes) and the field window (12 float planes)
+    const size_t lds = tg.ntile ? size_t(tg.WX) * tg.WY * (5 * sizeof(double) + (tg.fwin ? 12 * sizeof(float) : 0)) : 0;
+    const dim3 grid(unsigned(tg.ntile + tail_blocks)), blk(kT);
+    const bool ion = a.ions != nullptr;
+    if (a.fv.yf) {
+        if (ion) hipLaunchKernelGGL((k_stream_halfcell<1, true>), grid, blk, lds, stream(), a, src, dst, nstrm, tg);
+        else hipLaunchKernelGGL((k_stream_halfcell<1, false>), grid, blk, lds, stream(), a, src, dst, nstrm, tg);
+    } else {
+        if (ion) hipLaunchKernelGGL((k_stream_halfcell<0, true>), grid, blk, lds, stream(), a, src, dst, nstrm, tg);
+        else hipLaunchKernelGGL((k_stream_halfcell<0, false>), grid, blk, lds, stream(), a, src, dst, nstrm, tg);
+    }
     launch_check("stream half-cell");
 }
 void append_born(Parts born, const int* bcnt, Parts p0, Parts p1, long base, long cap) {

@@ -52,3 +52,22 @@ def F_rel_l2(Fa, Fb, ns, nk):
     A = np.asarray(Fa, dtype=np.float64).reshape(6, ns + 2, nk)[:, :ns]
     B = np.asarray(Fb, dtype=np.float64).reshape(6, ns + 2, nk)[:, :ns]
     return group_rel_l2([A[c] for c in range(6)], [B[c] for c in range(6)], ns)
+
+
+def particle_parity(a, b, deck, tol=1e-12):
+    """particle lists a (GPU) and b (reference) of the same length and order: cells bit-exact (x cell, y cell of
+    the Fourier nodes or of the free-hand node table, zeta cell), positions and the momentum vectors within tol
+    relative L2 over the list (the momentum bar of the FP64 form)"""
+    dx = deck.c.Lx / deck.c.nx
+    np.testing.assert_array_equal(np.floor(a["x"] / dx), np.floor(b["x"] / dx))
+    if deck.c.y_fourier:
+        dy = deck.c.Ly / deck.c.ny
+        np.testing.assert_array_equal(np.floor(a["y"] / dy), np.floor(b["y"] / dy))
+    else:
+        np.testing.assert_array_equal(np.searchsorted(deck.ynode, a["y"]), np.searchsorted(deck.ynode, b["y"]))
+    np.testing.assert_array_equal(np.searchsorted(deck.zeta, a["zeta"]), np.searchsorted(deck.zeta, b["zeta"]))
+    for f in ("x", "y", "zeta"):
+        assert np.linalg.norm(a[f] - b[f]) <= tol * np.linalg.norm(b[f]), f
+    p = np.sqrt(b["px"] ** 2 + b["py"] ** 2 + b["pz"] ** 2)
+    dp = np.sqrt((a["px"] - b["px"]) ** 2 + (a["py"] - b["py"]) ** 2 + (a["pz"] - b["pz"]) ** 2)
+    assert np.linalg.norm(dp) <= tol * np.linalg.norm(p), np.linalg.norm(dp) / np.linalg.norm(p)

@@ -7,7 +7,7 @@ import pytest
 
 import g3d_amd
 import refpy
-from decks import F_rel_l2, bunch_deck, group_rel_l2, plasma_deck, slice_rel_l2
+from decks import F_rel_l2, bunch_deck, group_rel_l2, particle_parity, plasma_deck, slice_rel_l2
 
 pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not refpy.available(), reason="oracle/_ref not built")]
 TOL = 1e-12
@@ -111,28 +111,10 @@ def test_plasma_smoothing():
     compare_fields(gpu, ref)
 
 
-def ref_spread(deck_fn, steps, **kw):
-    """The reference's own reproducibility on a deck: the largest difference between its 1-thread run and its runs
-    on 2, 4 and 8 threads (its FP64 atomic deposits sum in thread order, so the last bits of the fields depend on
-    the schedule)."""
-    runs = {}
-    for thr in (1, 2, 4, 8):
-        refpy.set_threads(thr)
-        r = refpy.RefSim(deck_fn(**kw))
-        r.init()
-        for _ in range(steps):
-            r.step()
-        runs[thr] = r
-    refpy.set_threads(1)
-    ns = runs[1].info()["ns"]
-    base = [runs[1].array(n) for n in SPEC]
-    return max(group_rel_l2([runs[t].array(n) for n in SPEC], base, ns) for t in (2, 4, 8))
-
-
 def test_bunch_init_and_steps():
     """config-0 style: bunch + oblique plane wave in vacuum; init (co-moving start field, relaxation), 3 steps.
-    The beam deposit sums 27 FP64 atomics per particle in an order that depends on the schedule, in the reference
-    as on the GPU, so the bar is max(1e-12, 3 x the reference's own spread over thread counts)."""
+    The beam deposit sums in the reference's single-thread order, so the start field, which amplifies the last
+    bits of the currents by ~1e4, stays within the FP64 bar."""
     gpu, ref = pair(bunch_deck)
     gpu.init(); ref.init()
     pa, pb = gpu.array("parts"), ref.array("parts")
@@ -142,13 +124,10 @@ def test_bunch_init_and_steps():
     compare_fields(gpu, ref)
     for _ in range(3):
         gpu.step(); ref.step()
-    tol = max(TOL, 3 * ref_spread(bunch_deck, 3))
-    compare_fields(gpu, ref, tol=tol)
+    print("after 3 steps:", compare_fields(gpu, ref))
     pa, pb = gpu.array("parts"), ref.array("parts")
     np.testing.assert_array_equal(pa["alive"], pb["alive"])
-    for f in ("x", "y", "zeta", "px", "py", "pz"):
-        d = np.abs(pa[f] - pb[f]).max() / max(np.abs(pb[f]).max(), 1e-300)
-        assert d < 1e-12, (f, d)
+    particle_parity(pa, pb, bunch_deck())
 
 
 def test_stream_spectra_random():
