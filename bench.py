@@ -67,6 +67,69 @@ def cpu_baseline(threads):
                       "slice, %d slices, 2 passes, %d steps: %.2f s" % (threads, nstrm, nz, steps, dt)}
 
 
+def stream_bytes_mixed(nstrm, nk):
+    """Algorithmic HBM bytes of one launch of k_stream_mixed: per particle 160 B (read x y zeta z px py pz w q qm +
+    alive, write x y zeta z px py pz q qm + alive; the state stays FP64 in HBM), per point the F gather of slices
+    n, n+1 (48 B) and the 5 float current planes (20 B)."""
+    return nstrm * 160.0 + nk * 68.0
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stream_traffic.json")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of the kernel from the committed byte-counter profile (FETCH_SIZE and WRITE_SIZE runs of
+    rocprofv3, scaled by a plain copy of known size), or None"""
+    try:
+        t = json.load(open(TRAFFIC_FILE))[kernel]
+        return t["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def run_form(args, mixed, torch):
+    import g3d_amd
+    d = lwfa_deck(nz=args.nz)
+    d.set(mixed=int(mixed))
+    sim = g3d_amd.Sim(d)
+    t0 = time.perf_counter()
+    sim.init()
+    t_init = time.perf_counter() - t0
+    tag = "mixed" if mixed else "fp64"
+    print("%s: init %.2f s" % (tag, t_init), file=sys.stderr, flush=True)
+    for k in range(args.warmup):
+        sim.step()
+        print("%s: warmup step %d done" % (tag, k), file=sys.stderr, flush=True)
+    info = sim.info()
+    N, nk = info["N"], info["nk"]
+    torch.cuda.synchronize()
+    # timed region: HIP events on the library stream around every launch of the dominant kernel only
+    sim.timing(["stream_halfcell"])
+    t0 = time.perf_counter()
+    pushes = 0
+    for k in range(args.steps):
+        sim.step()
+        pushes += sim.info()["nstrm"] * (N - 1) * 2
+        print("%s: step %d done" % (tag, k), file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms, cnt = sim.timers()["stream_halfcell"]
+    avg_s = ms / 1000 / max(cnt, 1)
+    nstrm = sim.info()["nstrm"]
+    bpl = stream_bytes_mixed(nstrm, nk) if mixed else stream_bytes_per_launch(nstrm, nk)
+    kernel = "k_stream_mixed" if mixed else "k_stream_halfcell"
+    # one more step, outside the timed region, with every kernel family timed (the per-kernel breakdown)
+    sim.timing(True)
+    sim.step()
+    torch.cuda.synchronize()
+    fam = {k: {"ms_total": round(v[0], 3), "launches": v[1], "avg_us": round(1000 * v[0] / v[1], 2)}
+           for k, v in sim.timers().items() if v[1]}
+    sim.timing(False)
+    return {"sim": sim, "value": pushes / dt, "ms_step": 1000 * dt / args.steps, "N": N, "nstrm": nstrm,
+            "pushes_per_step": pushes // args.steps, "kernel": kernel, "bpl": bpl, "avg_us": avg_s * 1e6,
+            "achieved": bpl / avg_s / 1e9, "traffic": measured_traffic(kernel), "fam": fam, "t_init": t_init}
+
+
 def copy_calibration(mib):
     """dst <- src over mib MiB, 5 times, with torch's copy kernel: each dispatch reads and writes mib MiB"""
     import torch
@@ -87,6 +150,9 @@ def main():
     ap.add_argument("--nz", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--precision", choices=("both", "mixed", "fp64"), default="both",
+                    help="both (default): the FP64 and the FP32-mixed march on the same deck; the line reports the "
+                         "mixed one and its field error against FP64")
     ap.add_argument("--copy-calib", type=float, default=0.0,
                     help="only copy a buffer of this many MiB a few times (the reference kernel that scales the HBM "
                          "byte counters of a profile) and exit")
@@ -100,45 +166,26 @@ def main():
         sys.exit(2)
     import torch
     assert torch.cuda.is_available(), "no GPU"
-    import g3d_amd
 
-    d = lwfa_deck(nz=args.nz)
-    sim = g3d_amd.Sim(d)
-    t0 = time.perf_counter()
-    sim.init()
-    t_init = time.perf_counter() - t0
-    print("init %.2f s" % t_init, file=sys.stderr, flush=True)
-    for k in range(args.warmup):
-        sim.step()
-        print("warmup step %d done" % k, file=sys.stderr, flush=True)
-    info = sim.info()
-    N, nk = info["N"], info["nk"]
-    torch.cuda.synchronize()
-    # timed region: HIP events on the library stream around every launch of the dominant kernel only
-    sim.timing(["stream_halfcell"])
-    t0 = time.perf_counter()
-    pushes = 0
-    for k in range(args.steps):
-        sim.step()
-        pushes += sim.info()["nstrm"] * (N - 1) * 2
-        print("step %d done" % k, file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    timers = sim.timers()
-    ms_step = 1000 * dt / args.steps
-    value = pushes / dt
-    ms, cnt = timers["stream_halfcell"]
-    avg_s = ms / 1000 / max(cnt, 1)
-    nstrm = sim.info()["nstrm"]
-    bpl = stream_bytes_per_launch(nstrm, nk)
-    achieved = bpl / avg_s / 1e9
-    # one more step, outside the timed region, with every kernel family timed (the per-kernel breakdown)
-    sim.timing(True)
-    sim.step()
-    torch.cuda.synchronize()
-    fam = {k: {"ms_total": round(v[0], 3), "launches": v[1], "avg_us": round(1000 * v[0] / v[1], 2)}
-           for k, v in sim.timers().items() if v[1]}
-    sim.timing(False)
+    forms = {"fp64": [False], "mixed": [True], "both": [False, True]}[args.precision]
+    runs = {}
+    for mixed in forms:
+        runs["mixed" if mixed else "fp64"] = run_form(args, mixed, torch)
+    head = runs.get("mixed") or runs["fp64"]
+    compare = None
+    if len(runs) == 2:
+        # the two runs took the same steps from the same start: the real fields of every slice compared
+        import numpy as np
+        info = runs["fp64"]["sim"].info()
+        ns, nk = info["ns"], info["nk"]
+        A = runs["mixed"]["sim"].array("F").astype(np.float64).reshape(6, ns + 2, nk)[:, :ns]
+        B = runs["fp64"]["sim"].array("F").astype(np.float64).reshape(6, ns + 2, nk)[:, :ns]
+        nb = np.sqrt((B ** 2).sum(axis=(0, 2)))
+        err = np.sqrt(((A - B) ** 2).sum(axis=(0, 2))) / np.maximum(nb, 1e-6 * nb.max())
+        compare = {"F_rel_l2_max_over_slices": float(err.max()), "F_rel_l2_mean": float(err.mean()),
+                   "speedup_mixed_over_fp64": runs["mixed"]["value"] / runs["fp64"]["value"],
+                   "fp64_value": runs["fp64"]["value"], "fp64_ms_per_step": runs["fp64"]["ms_step"],
+                   "fp64_stream_us": runs["fp64"]["avg_us"]}
     cpu = None
     if not args.no_cpu:
         thr = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -146,19 +193,22 @@ def main():
             cpu = cpu_baseline(thr)
         except Exception as e:  # the baseline is reported, never the target
             print("cpu baseline failed: %s" % e, file=sys.stderr)
+    mixed = "mixed" in runs
     out = {
         "metric": "particle pushes/s incl. deposit+gather; FP32-mixed vs FP64 at <=1e-4 field err",
-        "value": value, "unit": "pushes/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "LWFA plasma march 256x256x%d, 262144 streaming e- per slice, 2 passes, FP64 form"
-                               % args.nz, "N": N, "nstrm": nstrm, "pushes_per_step": pushes // args.steps},
-        "roofline": {"kernel": "k_stream_halfcell", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_launch": bpl, "avg_launch_us": avg_s * 1e6},
+        "value": head["value"], "unit": "pushes/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": head["ms_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32-mixed" if mixed else "f64", "data": "synthetic",
+        "config": {"workload": "LWFA plasma march 256x256x%d, 262144 streaming e- per slice, 2 passes, %s form"
+                               % (args.nz, "FP32-mixed" if mixed else "FP64"),
+                   "N": head["N"], "nstrm": head["nstrm"], "pushes_per_step": head["pushes_per_step"]},
+        "roofline": {"kernel": head["kernel"], "bound": "hbm", "achieved": head["achieved"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": head["achieved"] / HBM_PEAK_GBS, "traffic": head["traffic"],
+                     "bytes_per_launch": head["bpl"], "avg_launch_us": head["avg_us"]},
         "cpu_baseline": cpu,
-        "kernel_families_one_extra_step": fam,
-        "init_s": t_init,
+        "mixed_vs_fp64": compare,
+        "kernel_families_one_extra_step": head["fam"],
+        "init_s": head["t_init"],
     }
     print(json.dumps(out))
 

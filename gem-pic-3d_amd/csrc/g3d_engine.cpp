@@ -45,6 +45,7 @@ Config Config::from_abi(const g3d_config& g) {
     c.predictor_extrapolate = g.predictor_extrapolate; c.free_electrons = g.free_electrons;
     c.ions.assign(g.ions, g.ions + g.n_ions);
     c.freeze_bunch = g.freeze_bunch; c.start_relax = g.start_relax; c.box_mean = g.box_mean;
+    c.mixed = g.mixed != 0;
     if (c.iterations < 1) throw std::runtime_error("numerics.iterations must be >= 1");
     return c;
 }
@@ -858,18 +859,34 @@ void Sim::march_plasma(double s0, double s1) {
         pass_rows_inv(tp3.data(), F.data() + long(n - a) * nk, fpl, fft);
         toc(F_IFFT);
     };
+    // the FP32-mixed pass: Fourier y, and no ion species in the list (ions and ionization stay FP64)
+    bool ions = false;
+    for (const auto& is : cfg.ions) ions = ions || cfg.loaded(is);
+    const bool mx = cfg.mixed && yf && !ions;
+    StreamArgs sc = stream_args(s0, s1);
+    const bool ionz = born_cap > 0;
+    // tiles only when this step loaded tiled plasma (otherwise the slabs would hold an earlier step's current)
+    TileGeom tg = n_tiled > 0 ? tiles : TileGeom{};
+    tg.idx = nullptr;
+    if (mx) {
+        if (jsr_f.size() < 5 * pl) jsr_f.resize(5 * pl);
+        if (tiles.ntile && slab_f.size() < size_t(tiles.ntile) * 5 * tiles.WX * tiles.WY)
+            slab_f.resize(size_t(tiles.ntile) * 5 * tiles.WX * tiles.WY);
+    }
+    cxf* Xf = reinterpret_cast<cxf*>(sp3.data());
+    cxf* Tf = reinterpret_cast<cxf*>(tp3.data());
+    const int rpw = rows_per_wg(cfg.ny);
     // front: half-cell N-2 predicted with its level-s0 value; slice N-1
     m.pk = nullptr; m.n = N - 1; m.first = 1; m.last = 1; m.top = 1; m.wex = 0; m.qcorr = 0;
     m.bRx = m.bRy = m.bEz = m.bBz = nullptr;
     tic(F_MODES);
-    pass_work(sp3.data(), 1);
+    if (mx) pass_mixed(m, fft, jsr_f.data(), slab_f.data(), tg, Xf, Tf, F.data() + long(N - 1 - a) * nk, fpl, nullptr,
+                       nullptr, 0, 0, 1, rpw);
+    else pass_work(sp3.data(), 1);
     toc(F_MODES);
-    real_fields(N - 1);
-
-    StreamArgs sc = stream_args(s0, s1);
-    const bool ionz = born_cap > 0;
-    // tiles only when this step loaded tiled plasma (otherwise the slabs would hold an earlier step's current)
-    const TileGeom tg = n_tiled > 0 ? tiles : TileGeom{};
+    if (!mx) real_fields(N - 1);
+    // every rebin half-cells the live particles are binned again by their current tile (0: never)
+    static const int rebin = [] { const char* e = getenv("G3D_REBIN"); return e ? atoi(e) : 8; }();
     const uint64_t kst = key_step(cfg.seed, istep);
     const int passes = cfg.iterations;
     const int nx = cfg.nx, ny = cfg.ny;
@@ -880,6 +897,14 @@ void Sim::march_plasma(double s0, double s1) {
     long nlist = nstrm;          // particles in the list (freed electrons are appended on the device)
     for (int n = N - 2; n >= a; --n) {
         const double wex = (n >= 1 && n + 1 <= N - 2 && have_dl) ? (cen[n] - cen[n - 1]) / (cen[n + 1] - cen[n]) : 0.0;
+        if (tg.ntile && rebin > 0 && n < N - 2 && (N - 2 - n) % rebin == 0) {
+            if (bin_idx.size() < strmA.cap) bin_idx.resize(strmA.cap);
+            if (bin_cnt.size() < size_t(tg.ntile)) { bin_cnt.resize(size_t(tg.ntile)); bin_cur.resize(size_t(tg.ntile)); }
+            bin_tiles(strmA.view(), strmB.view(&strmA), nlist, tg, dx, dy, nx, ny, bin_cnt.data(), bin_cur.data(),
+                      tile_off.data(), bin_idx.data());
+            tg.idx = bin_idx.data();
+            tg.n_tiled = nlist;
+        }
         for (int pass = 0; pass < passes; ++pass) {
             sc.n = n;
             if (ionz) {
@@ -887,17 +912,27 @@ void Sim::march_plasma(double s0, double s1) {
                 sc.key = key_slice(kst, n, pass);
                 sc.bcap = int(std::min<long>(long(born.cap), long(strmA.cap) - nlist));
             }
+            const bool last = pass == passes - 1;
+            m.n = n; m.first = pass == 0; m.last = last; m.top = 0; m.wex = wex;
+            m.qcorr = D * wlo_h[n + 1];
+            if (passes > 1) { m.bRx = bk.data(); m.bRy = bk.data() + pl; m.bEz = bk.data() + 2 * pl; m.bBz = bk.data() + 3 * pl; }
+            if (mx) {
+                tic(F_STREAM);
+                stream_mixed(sc, jsr_f.data(), slab_f.data(), strmA.view(), strmB.view(&strmA), nlist, tg);
+                toc(F_STREAM);
+                tic(F_MODES);
+                pass_mixed(m, fft, jsr_f.data(), slab_f.data(), tg, Xf, Tf, F.data() + long(n - a) * nk, fpl,
+                           last ? rho_lines.data() + long(n) * (nx + ny) : nullptr, nullptr, i0, j0, 0, rpw);
+                toc(F_MODES);
+                continue;
+            }
             tic(F_STREAM);
             stream_halfcell(sc, strmA.view(), strmB.view(&strmA), nlist, tg);
             toc(F_STREAM);
-            const bool last = pass == passes - 1;
             tic(F_SFFT);
             pass_rows_fwd(jsr.data(), tg, fft, sp3.data(), last ? rho_lines.data() + long(n) * (nx + ny) : nullptr,
                           last && rho_lines_i.size() ? rho_lines_i.data() + long(n) * (nx + ny) : nullptr, i0, j0);
             toc(F_SFFT);
-            m.n = n; m.first = pass == 0; m.last = last; m.top = 0; m.wex = wex;
-            m.qcorr = D * wlo_h[n + 1];
-            if (passes > 1) { m.bRx = bk.data(); m.bRy = bk.data() + pl; m.bEz = bk.data() + 2 * pl; m.bBz = bk.data() + 3 * pl; }
             tic(F_MODES);
             pass_work(sp3.data(), 0);
             toc(F_MODES);

@@ -42,6 +42,7 @@ struct Config {
     bool predictor_extrapolate = true, free_electrons = true;
     std::vector<g3d_ion> ions;
     bool freeze_bunch = false, start_relax = true, box_mean = true;
+    bool mixed = false;               // FP32-mixed plasma pass
 
     static Config from_abi(const g3d_config& g);
     double plasma_density(double x, double y, double z) const;
@@ -111,6 +112,7 @@ public:
     DepositScratch dep_scratch;      // ordered beam deposit
     DevBuf<cx> js0x, js0y, js0z, jsn, jsin, jsi0, jpred, dlt, bk, sp3, tp3;
     DevBuf<double> jw, jsr, rho_lines, rho_lines_i;
+    DevBuf<float> jsr_f, slab_f;      // FP32-mixed pass: the half-cell current and the tile slabs
     DevBuf<float> F;
     DevBuf<double> kxd, kxs, filt, maskx, kyd, kys, filty, masky_n, masky_h, iD, iA, zs, hcg, wlo, whi, wsl;
     std::vector<double> wlo_h, hcg_h, hDel, hA;
@@ -129,6 +131,7 @@ public:
     // transverse tiles of the streaming list (tile-ordered fresh plasma, LDS deposit windows)
     TileGeom tiles{};
     DevBuf<long> tile_off;
+    DevBuf<int> bin_idx, bin_cnt, bin_cur;   // binning of the live list by current tile
     DevBuf<double> slab;
     long n_tiled = 0;
     unsigned long long id0_ = 0;

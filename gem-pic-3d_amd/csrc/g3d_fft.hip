@@ -137,7 +137,7 @@ void fft_cols(const FftPlan& p, cx* data, int count, bool inverse, double scale)
     static const int forced = [] { const char* e = getenv("G3D_FFT_COLS_NC"); return e ? atoi(e) : 0; }();
     int nc = kTile / p.ny;
     if (nc > p.nx) nc = p.nx;
-    while (nc > 2 && long(count) * (p.nx / nc) < 512) nc >>= 1;
+    while (nc > 4 && long(count) * (p.nx / nc) < 512) nc >>= 1;   // >= 4 columns: 64-byte row segments
     if (nc < 1) nc = 1;
     if (forced > 0 && forced <= p.nx && p.nx % forced == 0) nc = forced;
     const size_t lds = (size_t(nc) * (p.ny + p.ny / 8) + p.ny / 2) * sizeof(cx);

@@ -43,10 +43,15 @@ struct StreamArgs {
 struct TileGeom {
     int TX, TY, M, WX, WY, ntx, nty;
     int fwin;                         // stage the tile's field window in LDS for the gather
-    long ntile, n_tiled;              // tiles, particles in tiles (= off[ntile])
+    long ntile, n_tiled;              // tiles; list entries binned (= off[ntile]); from n_tiled on: untiled tail
     const long* off;
+    const int* idx;                   // list indices grouped by tile (null: the list itself is tile-ordered)
     double* slab;
 };
+// bin the live particles of the list by their current tile: off[0 .. ntile], idx[0 .. off[ntile]); the state of
+// the other particles (final for the rest of the step) is copied from S to D.  cnt, cur: ntile ints of scratch
+void bin_tiles(Parts S, Parts D, long n, const TileGeom& tg, double dx, double dy, int nx, int ny, int* cnt,
+               int* cur, long* off, int* idx);
 
 void bunch_push(const PushArgs& a, Parts p, long np, double s0);
 // beam deposit into jw [3][ns+3][nk] (src/sim_particles.cpp:66-106)
@@ -128,6 +133,13 @@ struct ModeArgs {
 void pass_rows_fwd(double* J, const TileGeom& tg, const FftPlan& p, cx* X, double* rl, double* rli, int i0, int j0);
 void pass_cols(const ModeArgs& m, const FftPlan& p, const cx* X, cx* T, cx* jsn, int kind);
 void pass_rows_inv(const cx* T, float* Fn, long fpl, const FftPlan& p);
+int rows_per_wg(int ny);            // rows per workgroup of the row kernels
+// the FP32-mixed plasma pass (g3d_mixed.hip): the streaming half-cell with FP32 deposit (jsr and slabs float), and
+// the field pass of slice m.n (rows forward, y transforms, per-mode work, inverse transforms, F); X, T: 3 planes
+// of cxf scratch; kind as pass_cols
+void stream_mixed(const StreamArgs& a, float* J, float* slab, Parts src, Parts dst, long nstrm, const TileGeom& tg);
+void pass_mixed(const ModeArgs& m, const FftPlan& p, float* J, const float* slab, const TileGeom& tg, cxf* X, cxf* T,
+                float* Fn, long fpl, double* rl, double* rli, int i0, int j0, int kind, int rpw);
 // the same columns step as three launches: y transforms of X (in place), the per-mode work (one lane per mode),
 // inverse y transforms of T
 void pass_modes(const ModeArgs& m, const FftPlan& p, cx* X, cx* T, cx* jsn, int kind);

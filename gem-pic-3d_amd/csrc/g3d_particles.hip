@@ -1,6 +1,8 @@
 // Particle kernels: the RK4 push in z of the beam list with its gathers and the analytic laser, the beam deposit,
 // the streaming half-cell pass (two Boris rotations, the deposit, trapping, ADK ionization), and the list tools.
 // One lane per particle over structure-of-arrays streams.
+#include <algorithm>
+
 #include "g3d_kernels.hpp"
 #include "g3d_runtime.hpp"
 
@@ -346,17 +348,24 @@ __global__ void __launch_bounds__(kT) k_stream_halfcell(StreamArgs c, Parts S, P
         float* fw = reinterpret_cast<float*>(win_lds + wn);
         for (long k = threadIdx.x; k < wn; k += blockDim.x) win_lds[k] = 0;
         const bool fwin = tg.fwin && F.self;
-        if (fwin)        // slices fl, fl + 1 of the 6 components: [slice][comp][v][u]
-            for (int k = threadIdx.x; k < 12 * pl; k += blockDim.x) {
-                const int sc = k / pl, r = k - sc * pl, v = r / tg.WX, u = r - v * tg.WX;
-                const int sl = sc / 6, comp = sc - 6 * sl;
+        if (fwin)        // slices fl, fl + 1 of the 6 components: [slice][comp][v][u]; one node per lane, 12 loads
+            for (int r = threadIdx.x; r < pl; r += blockDim.x) {
+                const int v = r / tg.WX, u = r - v * tg.WX;
                 const int gx = ((x0 + u) % F.nx + F.nx) % F.nx, gy = ((y0 + v) % F.ny + F.ny) % F.ny;
-                fw[k] = F.F[(long(comp) * (F.ns + 2) + fl + sl) * F.nk + long(gy) * F.nx + gx];
+                const float* src = F.F + long(fl) * F.nk + long(gy) * F.nx + gx;
+                const long cs = long(F.ns + 2) * F.nk;
+#pragma unroll
+                for (int comp = 0; comp < 6; ++comp) {
+                    fw[comp * pl + r] = src[comp * cs];
+                    fw[(6 + comp) * pl + r] = src[comp * cs + F.nk];
+                }
             }
         __syncthreads();
         Win w{win_lds, fwin ? fw : nullptr, x0, y0, tg.WX, tg.WY, fl};
+        // the particles binned into this tile (tg.idx: indices into the list; null: the contiguous load ranges)
         const long hi = tg.off[b + 1];
-        for (long i = tg.off[b] + threadIdx.x; i < hi; i += blockDim.x) stream_index<YF, ION>(c, &w, S, D, i);
+        for (long k = tg.off[b] + threadIdx.x; k < hi; k += blockDim.x)
+            stream_index<YF, ION>(c, &w, S, D, tg.idx ? long(tg.idx[k]) : k);
         __syncthreads();
         double* sl = tg.slab + b * wn;
         for (long k = threadIdx.x; k < wn; k += blockDim.x) sl[k] = win_lds[k];
@@ -366,6 +375,78 @@ __global__ void __launch_bounds__(kT) k_stream_halfcell(StreamArgs c, Parts S, P
     if (i < n) stream_index<YF, ION>(c, nullptr, S, D, i);
 }
 
+// ---------------------------------------------------------------- binning of the streaming list by tile
+// The list keeps its order in memory; a tile's workgroup visits the particles whose current cell lies in the tile
+// (idx[off[t] .. off[t+1])), so that particles that have moved away from their loading tile still deposit into
+// and gather from LDS windows.  Only live particles (alive 1, waiting 6) are binned: the others are final for the
+// rest of the step, and their state is copied into the second buffer here so that both buffers agree.
+__device__ __forceinline__ int tile_of(const TileGeom& tg, double x, double y, double dx, double dy, int nx, int ny) {
+    int ix = int(floor(x / dx)) % nx, iy = int(floor(y / dy)) % ny;
+    if (ix < 0) ix += nx;
+    if (iy < 0) iy += ny;
+    return (iy / tg.TY) * tg.ntx + ix / tg.TX;
+}
+__global__ void __launch_bounds__(kT) k_bin_count(Parts S, Parts D, long n, TileGeom tg, double dx, double dy,
+                                                  int nx, int ny, int* cnt) {
+    extern __shared__ int hist[];
+    for (int t = threadIdx.x; t < tg.ntile; t += blockDim.x) hist[t] = 0;
+    __syncthreads();
+    for (long i = long(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += long(gridDim.x) * blockDim.x) {
+        const int al = S.alive[i];
+        if (al == 1 || al == 6) atomicAdd(&hist[tile_of(tg, S.x[i], S.y[i], dx, dy, nx, ny)], 1);
+        else {
+            D.x[i] = S.x[i]; D.y[i] = S.y[i]; D.zeta[i] = S.zeta[i]; D.px[i] = S.px[i]; D.py[i] = S.py[i];
+            D.pz[i] = S.pz[i]; D.z[i] = S.z[i]; D.q[i] = S.q[i]; D.qm[i] = S.qm[i]; D.alive[i] = al;
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < tg.ntile; t += blockDim.x)
+        if (hist[t]) atomicAdd(&cnt[t], hist[t]);
+}
+// exclusive scan of the tile counts into off[0 .. ntile] and the scatter cursors (one workgroup: a chunk of tiles
+// per lane, then a Hillis-Steele scan of the 1024 chunk sums)
+__global__ void __launch_bounds__(1024) k_bin_scan(const int* cnt, long* off, int* cur, long ntile) {
+    __shared__ long part[2][1024];
+    const long per = (ntile + blockDim.x - 1) / blockDim.x;
+    const long t0 = threadIdx.x * per, t1 = t0 + per < ntile ? t0 + per : ntile;
+    long s = 0;
+    for (long t = t0; t < t1; ++t) s += cnt[t];
+    int src = 0;
+    part[0][threadIdx.x] = s;
+    __syncthreads();
+    for (unsigned d = 1; d < blockDim.x; d <<= 1) {
+        const long v = part[src][threadIdx.x] + (threadIdx.x >= d ? part[src][threadIdx.x - d] : 0);
+        part[1 - src][threadIdx.x] = v;
+        src = 1 - src;
+        __syncthreads();
+    }
+    long a = part[src][threadIdx.x] - s;             // exclusive prefix of this lane's chunk
+    for (long t = t0; t < t1; ++t) { off[t] = a; cur[t] = int(a); a += cnt[t]; }
+    if (threadIdx.x == blockDim.x - 1) off[ntile] = part[src][threadIdx.x];
+}
+// one lane per particle: its rank among the workgroup's particles of the same tile (LDS atomics), one global
+// reservation per tile the workgroup touches, then the index into its slot
+__global__ void __launch_bounds__(kT) k_bin_scatter(Parts S, long n, TileGeom tg, double dx, double dy, int nx,
+                                                    int ny, int* cur, int* idx) {
+    extern __shared__ int lcnt[];                    // [ntile] counts, then [ntile] reserved bases
+    int* lbase = lcnt + tg.ntile;
+    for (int t = threadIdx.x; t < tg.ntile; t += blockDim.x) lcnt[t] = 0;
+    __syncthreads();
+    const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    int t = -1, r = 0;
+    if (i < n) {
+        const int al = S.alive[i];
+        if (al == 1 || al == 6) {
+            t = tile_of(tg, S.x[i], S.y[i], dx, dy, nx, ny);
+            r = atomicAdd(&lcnt[t], 1);
+        }
+    }
+    __syncthreads();
+    for (int u = threadIdx.x; u < tg.ntile; u += blockDim.x)
+        if (lcnt[u]) lbase[u] = atomicAdd(&cur[u], lcnt[u]);
+    __syncthreads();
+    if (t >= 0) idx[lbase[t] + r] = int(i);
+}
 __global__ void k_append_born(Parts b, const int* bcnt, Parts p0, Parts p1, long base, long cap) {
     const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
     const long cnt = *bcnt;
@@ -465,6 +546,19 @@ void stream_halfcell(const StreamArgs& a, Parts src, Parts dst, long nstrm, cons
         else hipLaunchKernelGGL((k_stream_halfcell<0, false>), grid, blk, lds, stream(), a, src, dst, nstrm, tg);
     }
     launch_check("stream half-cell");
+}
+void bin_tiles(Parts S, Parts D, long n, const TileGeom& tg, double dx, double dy, int nx, int ny, int* cnt,
+               int* cur, long* off, int* idx) {
+    if (n <= 0 || tg.ntile <= 0) return;
+    if (tg.ntile > 16384) throw std::runtime_error("bin tiles: more than 16384 tiles");
+    hip_check(hipMemsetAsync(cnt, 0, sizeof(int) * size_t(tg.ntile), stream()), "hipMemsetAsync");
+    const unsigned blocks = unsigned(std::min<long>(1024, long(blocks_for(n, kT))));
+    hipLaunchKernelGGL(k_bin_count, dim3(blocks), dim3(kT), sizeof(int) * size_t(tg.ntile), stream(), S, D, n, tg, dx,
+                       dy, nx, ny, cnt);
+    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, stream(), cnt, off, cur, tg.ntile);
+    hipLaunchKernelGGL(k_bin_scatter, dim3(blocks_for(n, kT)), dim3(kT), 2 * sizeof(int) * size_t(tg.ntile), stream(),
+                       S, n, tg, dx, dy, nx, ny, cur, idx);
+    launch_check("bin tiles");
 }
 void append_born(Parts born, const int* bcnt, Parts p0, Parts p1, long base, long cap) {
     if (cap <= base) return;

@@ -75,4 +75,93 @@ __device__ __forceinline__ void pack3(cx T[3], cx Rx, cx Ry, cx Lx, cx Ly, cx Ez
     T[2] = mk(Ez.re - Bz.im, Ez.im + Bz.re);
 }
 
+// The per-mode work of a pass: u is the filtered streaming current of this pass.  The same
+// operations as pass_currents + march_mode + pass_correct in the same order, with every value read once and kept
+// in registers (the bookkeeping planes alias as far as the compiler knows, so it would reload them).
+// MIXED: the implicit part of the march in FP32 (march_mode_mixed)
+template <bool MIXED>
+__device__ __forceinline__ void mode_work(const ModeArgs& m, long i, const cx u[3], cx T[3]) {
+    const MarchView& c = m.mv;
+    const long nk = c.nk;
+    const int n = m.n;
+    const long hp = long(n + 1) * nk + i;           // half-cell n   (plane n + 1 of jk, js0)
+    const long hq = long(n) * nk + i;               // half-cell n-1 (plane n)
+    cx* jk[3] = {m.jkx, m.jky, m.jkz};
+    cx* js0[3] = {m.js0x, m.js0y, m.js0z};
+    cx q[3], pp[3], si0[3], js0p[3];
+    bool have_si0 = false, have_js0p = false;
+    if (m.top) {                                    // front: half-cell N-2 predicted with its level-s0 value
+        for (int k = 0; k < 3; ++k) {
+            const cx t = js0[k][hq];
+            q[k] = jk[k][hq] + t - m.pin[k * nk + i];
+            m.pin[k * nk + i] = t;
+            jk[k][hq] = q[k];
+            pp[k] = jk[k][hp];
+        }
+    } else {
+        cx pin[3], sp[3];
+        for (int k = 0; k < 3; ++k) pin[k] = m.pin[k * nk + i];
+        if (m.first) {
+            for (int k = 0; k < 3; ++k) { sp[k] = pin[k]; si0[k] = pin[k]; m.si0[k * nk + i] = pin[k]; }
+            have_si0 = true;
+        } else
+            for (int k = 0; k < 3; ++k) sp[k] = m.sin_[k * nk + i];
+        cx t0[3];
+        bool qset = false, pin_set = false;
+        if (m.first && n >= 1) {
+            for (int k = 0; k < 3; ++k) {
+                t0[k] = js0[k][hq];
+                q[k] = jk[k][hq] + t0[k] - mk(0);
+                pin[k] = t0[k];
+            }
+            qset = pin_set = true;
+        }
+        for (int k = 0; k < 3; ++k) {
+            pp[k] = jk[k][hp] + u[k] - sp[k];
+            jk[k][hp] = pp[k];
+            m.sin_[k * nk + i] = u[k];
+        }
+        if (n >= 1 && m.extrap) {
+            for (int k = 0; k < 3; ++k) {
+                js0p[k] = js0[k][hp];
+                const cx d0 = u[k] - js0p[k];
+                const cx jq = (m.first && n >= 1) ? t0[k] : js0[k][hq];
+                const cx t = jq + d0 + m.wex * (d0 - m.dl[k * nk + i]);
+                if (!qset) q[k] = jk[k][hq];
+                q[k] = q[k] + t - pin[k];
+                pin[k] = t;
+            }
+            have_js0p = true;
+            qset = pin_set = true;
+        }
+        if (qset) for (int k = 0; k < 3; ++k) jk[k][hq] = q[k];
+        else for (int k = 0; k < 3; ++k) q[k] = jk[k][hq];
+        if (pin_set) for (int k = 0; k < 3; ++k) m.pin[k * nk + i] = pin[k];
+    }
+    const long o = long(n - c.a) * nk + i;
+    Mode6 old;
+    if (!m.first && m.bRx) { old.Rx = m.bRx[i]; old.Ry = m.bRy[i]; old.Ez = m.bEz[i]; old.Bz = m.bBz[i]; }
+    else {
+        old.Rx = c.Rx[o]; old.Ry = c.Ry[o]; old.Ez = c.Ez[o]; old.Bz = c.Bz[o];
+        if (m.bRx) { m.bRx[i] = old.Rx; m.bRy[i] = old.Ry; m.bEz[i] = old.Ez; m.bBz[i] = old.Bz; }
+    }
+    Mode6 up{};
+    const long on = o + nk;
+    if (n < c.N - 1) up = Mode6{c.Rx[on], c.Ry[on], c.Ez[on], c.Bz[on], c.Lx[on], c.Ly[on]};
+    const Cur9 J{c.j0x[o], c.j0x[on], c.j0y[o], c.j0y[on], q[0], pp[0], q[1], pp[1], pp[2]};
+    const Mode6 r = MIXED ? march_mode_mixed(c, i, n, old, up, J) : march_mode(c, i, n, old, up, J);
+    c.Rx[o] = r.Rx; c.Ry[o] = r.Ry; c.Ez[o] = r.Ez; c.Bz[o] = r.Bz; c.Lx[o] = r.Lx; c.Ly[o] = r.Ly;
+    if (m.last && !m.top) {                         // src/sim_plasma.cpp:381-396
+        if (!have_si0) { si0[0] = m.si0[i]; si0[1] = m.si0[nk + i]; }
+        c.Rx[on] = up.Rx - m.qcorr * (u[0] - si0[0]);
+        c.Ry[on] = up.Ry - m.qcorr * (u[1] - si0[1]);
+        for (int k = 0; k < 3; ++k) {
+            if (!have_js0p) js0p[k] = js0[k][hp];
+            m.dl[k * nk + i] = u[k] - js0p[k];
+            js0[k][hp] = u[k];
+        }
+    }
+    pack3(T, r.Rx, r.Ry, r.Lx, r.Ly, r.Ez, r.Bz);
+}
+
 }  // namespace g3d

@@ -25,28 +25,6 @@ using namespace fftcore;
 constexpr int kT = 256;
 constexpr int kMaxModes = 4;          // modes per lane in k_pass_cols (nc * ny <= kMaxModes * kT)
 
-// the per-mode work of the pass (see the file head); u: the filtered streaming current of this pass
-__device__ __forceinline__ void mode_work(const ModeArgs& m, long i, const cx u[3], cx T[3]) {
-    const MarchView& c = m.mv;
-    const long nk = c.nk;
-    const int n = m.n;
-    pass_currents(m, i, u);
-    const long o = long(n - c.a) * nk + i;
-    Mode6 old;
-    if (!m.first && m.bRx) { old.Rx = m.bRx[i]; old.Ry = m.bRy[i]; old.Ez = m.bEz[i]; old.Bz = m.bBz[i]; }
-    else {
-        old.Rx = c.Rx[o]; old.Ry = c.Ry[o]; old.Ez = c.Ez[o]; old.Bz = c.Bz[o];
-        if (m.bRx) { m.bRx[i] = old.Rx; m.bRy[i] = old.Ry; m.bEz[i] = old.Ez; m.bBz[i] = old.Bz; }
-    }
-    Mode6 up{};
-    const long on = o + nk;
-    if (n < c.N - 1) up = Mode6{c.Rx[on], c.Ry[on], c.Ez[on], c.Bz[on], c.Lx[on], c.Ly[on]};
-    const Mode6 r = march_mode(c, i, n, old, up);
-    c.Rx[o] = r.Rx; c.Ry[o] = r.Ry; c.Ez[o] = r.Ez; c.Bz[o] = r.Bz; c.Lx[o] = r.Lx; c.Ly[o] = r.Ly;
-    if (m.last && !m.top) pass_correct(m, i, u);
-    pack3(T, r.Rx, r.Ry, r.Lx, r.Ly, r.Ez, r.Bz);
-}
-
 // ---------------------------------------------------------------- rows forward
 __global__ void __launch_bounds__(kT) k_pass_rows_fwd(double* J, TileGeom tg, int nx, int ny, int lx, int rpw,
                                                      const cx* twx, cx* X, double* rl, double* rli, int i0, int j0) {
@@ -162,7 +140,7 @@ __global__ void __launch_bounds__(kT) k_pass_cols(ModeArgs m, const cx* X, cx* T
         const int j = e / nc, cc = e - j * nc;
         const long i = long(j) * nx + c0 + cc;
         cx t[3];
-        mode_work(m, i, u[q], t);
+        mode_work<false>(m, i, u[q], t);
         const int b = lpos(bitrev(j, ly));
         for (int k = 0; k < 3; ++k) buf[(k * nc + cc) * sp + b] = t[k];
     }
@@ -193,7 +171,7 @@ __global__ void __launch_bounds__(kT) k_pass_modes(ModeArgs m, const cx* X, cx* 
         }
     }
     cx t[3];
-    mode_work(m, i, u, t);
+    mode_work<false>(m, i, u, t);
     for (int k = 0; k < 3; ++k) T[k * nk + i] = t[k];
 }
 
@@ -229,7 +207,6 @@ __global__ void __launch_bounds__(kT) k_pass_rows_inv(const cx* T, float* Fn, lo
     }
 }
 
-int rows_per_wg(int ny) { int r = 1; while (r < 8 && ny / (2 * r) >= 128) r <<= 1; return r; }
 int cols_per_wg(int nx, int ny) {
     static const int forced = [] { const char* e = getenv("G3D_COLS_PER_WG"); return e ? atoi(e) : 0; }();
     if (forced > 0 && nx % forced == 0 && forced * ny <= kMaxModes * kT) return forced;
@@ -241,6 +218,14 @@ int cols_per_wg(int nx, int ny) {
 int lanes(int work) { int t = 64; while (t < work && t < kT) t <<= 1; return t; }
 }  // namespace
 
+// rows per workgroup of the row kernels: at least 256 workgroups (one per CU) before rows are grouped
+int rows_per_wg(int ny) {
+    static const int forced = [] { const char* e = getenv("G3D_ROWS_PER_WG"); return e ? atoi(e) : 0; }();
+    if (forced > 0 && ny % forced == 0) return forced;
+    int r = 1;
+    while (r < 8 && ny / (2 * r) >= 256) r <<= 1;
+    return r;
+}
 void pass_rows_fwd(double* J, const TileGeom& tg, const FftPlan& p, cx* X, double* rl, double* rli, int i0, int j0) {
     const int rpw = rows_per_wg(p.ny);
     if (p.ny % rpw) throw std::runtime_error("pass rows: ny not a multiple of the row tile");

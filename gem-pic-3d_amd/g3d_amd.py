@@ -51,7 +51,7 @@ class Config(C.Structure):
         ("r_channel", C.c_double), ("iterations", C.c_int), ("vz_min", C.c_double), ("vz_beam", C.c_double),
         ("trap_advance", C.c_double), ("predictor_extrapolate", C.c_int), ("free_electrons", C.c_int),
         ("n_ions", C.c_int), ("ions", Ion * 4), ("freeze_bunch", C.c_int), ("start_relax", C.c_int),
-        ("box_mean", C.c_int)]
+        ("box_mean", C.c_int), ("mixed", C.c_int)]
 
 
 class SimInfo(C.Structure):

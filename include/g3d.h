@@ -90,6 +90,8 @@ typedef struct {
     int n_ions;
     g3d_ion ions[4];
     int freeze_bunch, start_relax, box_mean;
+    int mixed;          /* plasma pass in the FP32-mixed form (no counterpart in the reference: 0 = FP64 as the
+                           reference) */
 } g3d_config;
 
 /* context of the per-mode march (src/sim_kernels.hpp:161-167 MarchCtx), device pointers */
