@@ -182,7 +182,9 @@ def main():
         B = runs["fp64"]["sim"].array("F").astype(np.float64).reshape(6, ns + 2, nk)[:, :ns]
         nb = np.sqrt((B ** 2).sum(axis=(0, 2)))
         err = np.sqrt(((A - B) ** 2).sum(axis=(0, 2))) / np.maximum(nb, 1e-6 * nb.max())
+        worst = int(err.argmax())
         compare = {"F_rel_l2_max_over_slices": float(err.max()), "F_rel_l2_mean": float(err.mean()),
+                   "worst_slice": worst, "worst_slice_norm_over_max": float(nb[worst] / nb.max()),
                    "speedup_mixed_over_fp64": runs["mixed"]["value"] / runs["fp64"]["value"],
                    "fp64_value": runs["fp64"]["value"], "fp64_ms_per_step": runs["fp64"]["ms_step"],
                    "fp64_stream_us": runs["fp64"]["avg_us"]}

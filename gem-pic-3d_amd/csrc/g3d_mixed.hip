@@ -47,9 +47,11 @@ __device__ __forceinline__ void boris_f(float& ux, float& uy, float& uz, const f
     ux = bx + qh * E[0]; uy = by + qh * E[1]; uz = bz + qh * E[2];
 }
 
-// LDS windows of a tile in FP32: the 5 current planes and the 12 field planes (slices fl, fl+1)
+// LDS windows of a tile: the 5 current planes (FP64 sums of the FP32 contributions: the FP32 LDS atomic add
+// serialises on the same-address conflicts of a tile's particles, the FP64 one does not) and the 12 field planes
+// (slices fl, fl+1, float)
 struct WinF {
-    float* lds;
+    double* lds;
     const float* fld;
     int x0, y0, WX, WY, fl;
 };
@@ -137,14 +139,14 @@ __device__ __forceinline__ void sdeposit_f(const StreamArgs& c, float* J, const 
             for (int v = 0; v < 3; ++v)
 #pragma unroll
                 for (int u = 0; u < 3; ++u) {
-                    float* L = win->lds + (v0 + v) * win->WX + (u0 + u);
+                    double* L = win->lds + (v0 + v) * win->WX + (u0 + u);
                     const float ww = fi * wx[u] * wn[v];
                     if (mv) {
-                        atomicAdd(L, ww * vx);
-                        atomicAdd(L + wpl, ww * vy);
-                        atomicAdd(L + 2 * wpl, ww * vz);
+                        atomicAdd(L, double(ww * vx));
+                        atomicAdd(L + wpl, double(ww * vy));
+                        atomicAdd(L + 2 * wpl, double(ww * vz));
                     }
-                    atomicAdd(L + cq * wpl, ww);
+                    atomicAdd(L + cq * wpl, double(ww));
                 }
             return;
         }
@@ -169,12 +171,14 @@ __device__ __forceinline__ void stream_one_mixed(const StreamArgs& c, float* J, 
                                                  long i) {
     const int n = c.n;
     const double zn = c.zs[n], zn1 = c.zs[n + 1], h = zn1 - zn;
+    // the whole state up front, so that its loads overlap
     double x = S.x[i], y = S.y[i], zeta = S.zeta[i], z = S.z[i];
     const double pxd = S.px[i], pyd = S.py[i], pzd = S.pz[i];
+    const double q = S.q[i], qmd = S.qm[i], w = S.w[i];
     int alive = S.alive[i];
     auto store = [&](double ux, double uy, double uz) {
         D.x[i] = x; D.y[i] = y; D.zeta[i] = zeta; D.px[i] = ux; D.py[i] = uy; D.pz[i] = uz;
-        D.z[i] = z; D.q[i] = S.q[i]; D.qm[i] = S.qm[i]; D.alive[i] = alive;
+        D.z[i] = z; D.q[i] = q; D.qm[i] = qmd; D.alive[i] = alive;
     };
     if (alive == 6) {
         if (zeta > zn) alive = 1;
@@ -183,7 +187,7 @@ __device__ __forceinline__ void stream_one_mixed(const StreamArgs& c, float* J, 
     if (alive != 1) { store(pxd, pyd, pzd); return; }
     const FieldView& P = c.fv;
     const double hl = zeta >= zn1 - 1e-9 * h ? h : (zeta - zn > 0 ? zeta - zn : 0.0);
-    const float hf = float(hl), qm = float(S.qm[i]);
+    const float hf = float(hl), qm = float(qmd);
     const float px = float(pxd), py = float(pyd), pz = float(pzd);
     const float g0 = sqrtf(1 + px * px + py * py + pz * pz);
     const float omv0 = (1 + px * px + py * py) / (g0 * (g0 + pz));
@@ -222,7 +226,6 @@ __device__ __forceinline__ void stream_one_mixed(const StreamArgs& c, float* J, 
     x += double(vx * dt); y += double(vy * dt);
     z = trapped ? c.s1 : z + double(vz * dt);
     zeta = zeta_end;
-    const double q = S.q[i], w = S.w[i];
     const float f = float(q * w * double(dt) / (c.Delta * h * P.dx));
     if (f != 0) sdeposit_f(c, J, win, 0.5 * (xo + x), 0.5 * (yo + y), f, vx, vy, vz);
     if (trapped) {
@@ -235,7 +238,7 @@ __device__ __forceinline__ void stream_one_mixed(const StreamArgs& c, float* J, 
 
 __global__ void __launch_bounds__(kT) k_stream_mixed(StreamArgs c, float* J, float* slab, Parts S, Parts D, long n,
                                                      TileGeom tg) {
-    extern __shared__ float winf[];
+    extern __shared__ double winf[];
     const long b = blockIdx.x;
     if (b < tg.ntile) {
         const int pl = tg.WX * tg.WY;
@@ -244,7 +247,7 @@ __global__ void __launch_bounds__(kT) k_stream_mixed(StreamArgs c, float* J, flo
         const int x0 = tx * tg.TX - 1 - tg.M, y0 = ty * tg.TY - 1 - tg.M;
         const FieldView& F = c.fv;
         const int fl = c.n - F.a;
-        float* fw = winf + wn;
+        float* fw = reinterpret_cast<float*>(winf + wn);
         for (int k = threadIdx.x; k < wn; k += blockDim.x) winf[k] = 0;
         if (F.self)
             for (int r = threadIdx.x; r < pl; r += blockDim.x) {
@@ -265,7 +268,7 @@ __global__ void __launch_bounds__(kT) k_stream_mixed(StreamArgs c, float* J, flo
             stream_one_mixed(c, J, &w, S, D, tg.idx ? long(tg.idx[k]) : k);
         __syncthreads();
         float* sl = slab + b * long(wn);
-        for (int k = threadIdx.x; k < wn; k += blockDim.x) sl[k] = winf[k];
+        for (int k = threadIdx.x; k < wn; k += blockDim.x) sl[k] = float(winf[k]);
         return;
     }
     const long i = tg.n_tiled + (b - tg.ntile) * blockDim.x + threadIdx.x;
@@ -413,7 +416,7 @@ int lanes(int work) { int t = 64; while (t < work && t < kT) t <<= 1; return t; 
 void stream_mixed(const StreamArgs& a, float* J, float* slab, Parts src, Parts dst, long nstrm, const TileGeom& tg) {
     if (nstrm <= 0) return;
     const long tail_blocks = nstrm > tg.n_tiled ? long(blocks_for(nstrm - tg.n_tiled, kT)) : 0;
-    const size_t lds = tg.ntile ? size_t(tg.WX) * tg.WY * 17 * sizeof(float) : 0;
+    const size_t lds = tg.ntile ? size_t(tg.WX) * tg.WY * (5 * sizeof(double) + 12 * sizeof(float)) : 0;
     hipLaunchKernelGGL(k_stream_mixed, dim3(unsigned(tg.ntile + tail_blocks)), dim3(kT), lds, stream(), a, J, slab, src,
                        dst, nstrm, tg);
     launch_check("stream half-cell (mixed)");
