@@ -19,6 +19,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "gem-pic-3d_amd"))
 
 HBM_PEAK_GBS = 8000.0
+TIME_EVERY = 16
 LAMBDA0 = 2 * np.pi * 1e-6
 
 
@@ -44,14 +45,14 @@ def stream_bytes_per_launch(nstrm, nk):
 
 def cpu_baseline(threads):
     """The reference's own CPU build (oracle/_ref) on a bounded sample of the same workload: the full 256 x 256
-    transverse grid and 262144 streaming electrons per slice, 32 slices instead of 1024 (the per-half-cell work does
-    not depend on the window length), 2 steps after init."""
+    transverse grid and 262144 streaming electrons per slice, 256 slices instead of 1024 (the per-half-cell work does
+    not depend on the window length), 2 steps after init: about 10 s on 16 cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import refpy
     if not refpy.available():
         return None
     refpy.set_threads(threads)
-    nz = 32
+    nz = 256
     d = lwfa_deck(nz=nz)
     ref = refpy.RefSim(d)
     ref.init()
@@ -103,8 +104,10 @@ def run_form(args, mixed, torch):
     info = sim.info()
     N, nk = info["N"], info["nk"]
     torch.cuda.synchronize()
-    # timed region: HIP events on the library stream around every launch of the dominant kernel only
+    # timed region: HIP events carried by one launch in TIME_EVERY of the dominant kernel (the events of a launch
+    # cost a few microseconds of launch spacing; a uniform sample over the slices keeps that out of the step time)
     sim.timing(["stream_halfcell"])
+    sim.timing_every(TIME_EVERY)
     t0 = time.perf_counter()
     pushes = 0
     for k in range(args.steps):
@@ -115,6 +118,7 @@ def run_form(args, mixed, torch):
     dt = time.perf_counter() - t0
     ms, cnt = sim.timers()["stream_halfcell"]
     avg_s = ms / 1000 / max(cnt, 1)
+    sim.timing_every(1)
     nstrm = sim.info()["nstrm"]
     bpl = stream_bytes_mixed(nstrm, nk) if mixed else stream_bytes_per_launch(nstrm, nk)
     kernel = "k_stream_mixed" if mixed else "k_stream_halfcell"
@@ -127,7 +131,8 @@ def run_form(args, mixed, torch):
     sim.timing(False)
     return {"sim": sim, "value": pushes / dt, "ms_step": 1000 * dt / args.steps, "N": N, "nstrm": nstrm,
             "pushes_per_step": pushes // args.steps, "kernel": kernel, "bpl": bpl, "avg_us": avg_s * 1e6,
-            "achieved": bpl / avg_s / 1e9, "traffic": measured_traffic(kernel), "fam": fam, "t_init": t_init}
+            "achieved": bpl / avg_s / 1e9, "traffic": measured_traffic(kernel), "fam": fam, "t_init": t_init,
+            "timed_launches": cnt}
 
 
 def copy_calibration(mib):
@@ -206,7 +211,8 @@ def main():
                    "N": head["N"], "nstrm": head["nstrm"], "pushes_per_step": head["pushes_per_step"]},
         "roofline": {"kernel": head["kernel"], "bound": "hbm", "achieved": head["achieved"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": head["achieved"] / HBM_PEAK_GBS, "traffic": head["traffic"],
-                     "bytes_per_launch": head["bpl"], "avg_launch_us": head["avg_us"]},
+                     "bytes_per_launch": head["bpl"], "avg_launch_us": head["avg_us"],
+                     "timed_launches": head["timed_launches"], "timed_one_in": TIME_EVERY},
         "cpu_baseline": cpu,
         "mixed_vs_fp64": compare,
         "kernel_families_one_extra_step": head["fam"],

@@ -2,6 +2,7 @@
 // functions, and move its arrays between host and device in the reference's layouts.
 #include <cstring>
 #include <map>
+#include <stdexcept>
 #include "g3d_engine.hpp"
 
 struct g3d_sim { g3d::Sim* s; };
@@ -165,7 +166,13 @@ int g3d_sim_timing(g3d_sim* sim, int enable) {
         g3d::Sim& s = *sim->s;
         s.flush_timers();
         s.timing = unsigned(enable);
-        for (int k = 0; k < g3d::Sim::kFam; ++k) { s.fam_ms[k] = 0; s.fam_n[k] = 0; }
+        for (int k = 0; k < g3d::Sim::kFam; ++k) { s.fam_ms[k] = 0; s.fam_n[k] = 0; s.fam_seen[k] = 0; }
+    });
+}
+int g3d_sim_timing_every(g3d_sim* sim, int every) {
+    return g3d::guard([&] {
+        if (every < 1) throw std::invalid_argument("timing: every must be at least 1");
+        sim->s->timing_every = every;
     });
 }
 int g3d_sim_timers(g3d_sim* sim, double* ms, long* counts, int nfam, char* names, int name_len) {

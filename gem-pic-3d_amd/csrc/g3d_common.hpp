@@ -56,10 +56,19 @@ G3D_HD void tsc_w(double xi, int idx[3], double w[3]) {
     w[2] = 0.5 * b * b;
     idx[0] = i0 - 1; idx[1] = i0; idx[2] = i0 + 1;
 }
+// i mod n in [0, n) for n > 0, as ((i % n) + n) % n: a compare and an add when i lies in [-n, 2n) (every stencil
+// node of a wrapped coordinate), the integer division only outside
+G3D_HD int wrap_index(int i, int n) {
+    if (i >= 0 && i < n) return i;
+    if (i < 0 && i >= -n) return i + n;
+    if (i >= n && i < 2 * n) return i - n;
+    const int m = i % n;
+    return m < 0 ? m + n : m;
+}
 // periodic uniform axis with nodes at i*d (src/sim_kernels.hpp:28-31)
 G3D_HD void tsc_periodic(double x, double d, int n, int idx[3], double w[3]) {
     tsc_w(x / d, idx, w);
-    for (int k = 0; k < 3; ++k) { int m = idx[k] % n; idx[k] = m < 0 ? m + n : m; }
+    for (int k = 0; k < 3; ++k) idx[k] = wrap_index(idx[k], n);
 }
 
 // Node table X[0..n-1] with the direct path for uniform grids (src/grid.hpp:130-146): X[i] <= x < X[i+1],

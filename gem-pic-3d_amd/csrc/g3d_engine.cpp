@@ -3,6 +3,7 @@
 #include "g3d_engine.hpp"
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
@@ -218,7 +219,12 @@ Sim::Sim(const Config& c) : cfg(c) {
 // window of WX = TX + 3 + 2M nodes must fit the period, and a window may reach only the neighbouring tiles.
 void Sim::setup_tiles() {
     auto pick = [](int n, int pref) { for (int t = pref; t >= 8; t >>= 1) if (n % t == 0 && n >= 2 * t) return t; return 0; };
-    const int TX = pick(cfg.nx, 16), TY = pick(cfg.ny, 8);
+    int px = 16, py = 8;                      // G3D_TILE=<TX>x<TY> (powers of two from 8) for launch-shape sweeps
+    if (const char* t = getenv("G3D_TILE")) {
+        if (sscanf(t, "%dx%d", &px, &py) != 2 || px < 8 || py < 8)
+            throw std::runtime_error("G3D_TILE: expected <TX>x<TY>, both at least 8");
+    }
+    const int TX = pick(cfg.nx, px), TY = pick(cfg.ny, py);
     if (!TX || !TY) return;
     int M = 3;
     M = std::min({M, TX - 1, TY - 1, (cfg.nx - TX - 3) / 2, (cfg.ny - TY - 3) / 2});
@@ -230,6 +236,12 @@ void Sim::setup_tiles() {
     tiles.off = tile_off.data(); tiles.slab = slab.data(); tiles.n_tiled = 0;
     const char* e = getenv("G3D_STREAM_FWIN");
     tiles.fwin = e ? atoi(e) : 1;
+    const char* b = getenv("G3D_STREAM_BLOCK");  // lanes per tile workgroup (about the particles of a tile)
+    tiles.block = b ? atoi(b) : 512;
+    const char* w = getenv("G3D_STREAM_WPE");     // register cap of the FP32-mixed stream kernel (0 or 4)
+    tiles.wpe = w ? atoi(w) : 4;
+    if (tiles.block < 64 || tiles.block > kStreamBlockMax || tiles.block % 64)
+        throw std::runtime_error("G3D_STREAM_BLOCK: a multiple of 64 up to 512");
 }
 
 Sim::~Sim() {
@@ -429,6 +441,23 @@ void Sim::toc(int fam) {
     hip_check(hipEventRecord(e, stream()), "hipEventRecord");
     ev_pending.push_back({fam, {ev_open, e}});
     ev_open = nullptr; ev_open_fam = -1;
+    if (ev_pending.size() > 8192) flush_timers();
+}
+// the events of one launch (hipExtLaunchKernel records them with the dispatch itself: no marker packets, so the
+// timed run keeps the launch spacing of the untimed one)
+LaunchTimer Sim::launch_timer(int fam) {
+    if (!(timing >> fam & 1u)) return {};
+    if (fam_seen[fam]++ % timing_every) return {};
+    LaunchTimer lt;
+    for (hipEvent_t* e : {&lt.start, &lt.stop}) {
+        if (!ev_pool.empty()) { *e = ev_pool.back(); ev_pool.pop_back(); }
+        else hip_check(hipEventCreate(e), "hipEventCreate");
+    }
+    return lt;
+}
+void Sim::launch_timed(int fam, const LaunchTimer& lt) {
+    if (!lt.start) return;
+    ev_pending.push_back({fam, {lt.start, lt.stop}});
     if (ev_pending.size() > 8192) flush_timers();
 }
 void Sim::flush_timers() {
@@ -917,18 +946,18 @@ void Sim::march_plasma(double s0, double s1) {
             m.qcorr = D * wlo_h[n + 1];
             if (passes > 1) { m.bRx = bk.data(); m.bRy = bk.data() + pl; m.bEz = bk.data() + 2 * pl; m.bBz = bk.data() + 3 * pl; }
             if (mx) {
-                tic(F_STREAM);
-                stream_mixed(sc, jsr_f.data(), slab_f.data(), strmA.view(), strmB.view(&strmA), nlist, tg);
-                toc(F_STREAM);
+                const LaunchTimer lt = launch_timer(F_STREAM);
+                stream_mixed(sc, jsr_f.data(), slab_f.data(), strmA.view(), strmB.view(&strmA), nlist, tg, lt);
+                launch_timed(F_STREAM, lt);
                 tic(F_MODES);
                 pass_mixed(m, fft, jsr_f.data(), slab_f.data(), tg, Xf, Tf, F.data() + long(n - a) * nk, fpl,
                            last ? rho_lines.data() + long(n) * (nx + ny) : nullptr, nullptr, i0, j0, 0, rpw);
                 toc(F_MODES);
                 continue;
             }
-            tic(F_STREAM);
-            stream_halfcell(sc, strmA.view(), strmB.view(&strmA), nlist, tg);
-            toc(F_STREAM);
+            const LaunchTimer lt = launch_timer(F_STREAM);
+            stream_halfcell(sc, strmA.view(), strmB.view(&strmA), nlist, tg, lt);
+            launch_timed(F_STREAM, lt);
             tic(F_SFFT);
             pass_rows_fwd(jsr.data(), tg, fft, sp3.data(), last ? rho_lines.data() + long(n) * (nx + ny) : nullptr,
                           last && rho_lines_i.size() ? rho_lines_i.data() + long(n) * (nx + ny) : nullptr, i0, j0);

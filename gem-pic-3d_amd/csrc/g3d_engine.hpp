@@ -138,12 +138,16 @@ public:
 
     // timing of kernel families with HIP events on the library stream
     unsigned timing = 0;              // bit f: time family f
+    int timing_every = 1;             // launch-event families: time one launch in timing_every
     static constexpr int kFam = 8;
+    long fam_seen[kFam] = {};         // launches seen per family since timing was set
     double fam_ms[kFam] = {};
     long fam_n[kFam] = {};
     void tic(int fam);
     void toc(int fam);
     void flush_timers();
+    LaunchTimer launch_timer(int fam);
+    void launch_timed(int fam, const LaunchTimer& lt);
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
     std::vector<hipEvent_t> ev_pool;
     hipEvent_t ev_open = nullptr;

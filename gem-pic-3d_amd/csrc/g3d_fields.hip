@@ -137,16 +137,16 @@ __global__ void k_sspec_pack(double* J, long nk, int nx, int ny, cx* pk, double*
         const int tpx = ixx / tg.TX, tpy = j / tg.TY;
         const long wpl = long(tg.WX) * tg.WY;
         for (int dy = -1; dy <= 1; ++dy) {
-            const int ty = ((tpy + dy) % tg.nty + tg.nty) % tg.nty;
+            const int ty = wrap_index(tpy + dy, tg.nty);
             if ((dy == 1 && tg.nty <= 2) || (dy != 0 && tg.nty == 1)) continue;
             int vv = j - (ty * tg.TY - 1 - tg.M);
-            vv = ((vv % ny) + ny) % ny;
+            vv = wrap_index(vv, ny);
             if (vv >= tg.WY) continue;
             for (int dx = -1; dx <= 1; ++dx) {
-                const int tx = ((tpx + dx) % tg.ntx + tg.ntx) % tg.ntx;
+                const int tx = wrap_index(tpx + dx, tg.ntx);
                 if ((dx == 1 && tg.ntx <= 2) || (dx != 0 && tg.ntx == 1)) continue;
                 int uu = ixx - (tx * tg.TX - 1 - tg.M);
-                uu = ((uu % nx) + nx) % nx;
+                uu = wrap_index(uu, nx);
                 if (uu >= tg.WX) continue;
                 const double* sl = tg.slab + (long(ty) * tg.ntx + tx) * 5 * wpl + long(vv) * tg.WX + uu;
                 for (int c = 0; c < 5; ++c) v[c] += sl[c * wpl];

@@ -1,6 +1,7 @@
 // Launch wrappers of the gfx950 kernels used by the engine (g3d_engine.cpp).  All launches go to the library
 // stream; none synchronises.
 #pragma once
+#include <hip/hip_ext.h>
 #include "g3d_common.hpp"
 #include "g3d_fft.hpp"
 #include "g3d_march.hpp"
@@ -43,6 +44,8 @@ struct StreamArgs {
 struct TileGeom {
     int TX, TY, M, WX, WY, ntx, nty;
     int fwin;                         // stage the tile's field window in LDS for the gather
+    int block;                        // lanes of a tile's workgroup (a multiple of 64, at most kStreamBlockMax)
+    int wpe;                          // FP32-mixed stream kernel: 4 = registers capped for 4 waves per SIMD, else 0
     long ntile, n_tiled;              // tiles; list entries binned (= off[ntile]); from n_tiled on: untiled tail
     const long* off;
     const int* idx;                   // list indices grouped by tile (null: the list itself is tile-ordered)
@@ -74,7 +77,20 @@ struct DepositScratch {
 void bunch_deposit(const DepositArgs& a, Parts p, long np, DepositScratch& s);
 // one half-cell pass of the streaming particles: src -> dst (the state of every particle is written); tiles
 // with tg.ntile > 0, otherwise every deposit is a global atomic
-void stream_halfcell(const StreamArgs& a, Parts src, Parts dst, long nstrm, const TileGeom& tg);
+// start and stop events of one launch, recorded by the launch itself (no marker packets around it); null: untimed
+struct LaunchTimer {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+// a kernel launch on the library stream, through hipExtLaunchKernel with the timer's events when it has them
+template <typename... KArgs, typename... Args>
+void launch_timed_kernel(const LaunchTimer& lt, void (*k)(KArgs...), dim3 grid, dim3 block, unsigned lds,
+                         Args... args) {
+    if (lt.start) hipExtLaunchKernelGGL(k, grid, block, lds, stream(), lt.start, lt.stop, 0, args...);
+    else hipLaunchKernelGGL(k, grid, block, lds, stream(), args...);
+}
+constexpr int kStreamBlockMax = 512;
+void stream_halfcell(const StreamArgs& a, Parts src, Parts dst, long nstrm, const TileGeom& tg,
+                     const LaunchTimer& lt = {});
 // append the freed electrons of the last pass (count on the device) to both buffers; nstrm_dev += count
 void append_born(Parts born, const int* bcnt, Parts p0, Parts p1, long base, long cap);
 // window centroid sums of the bunch particles (weight, w x, w y), deterministic tree order
@@ -137,7 +153,8 @@ int rows_per_wg(int ny);            // rows per workgroup of the row kernels
 // the FP32-mixed plasma pass (g3d_mixed.hip): the streaming half-cell with FP32 deposit (jsr and slabs float), and
 // the field pass of slice m.n (rows forward, y transforms, per-mode work, inverse transforms, F); X, T: 3 planes
 // of cxf scratch; kind as pass_cols
-void stream_mixed(const StreamArgs& a, float* J, float* slab, Parts src, Parts dst, long nstrm, const TileGeom& tg);
+void stream_mixed(const StreamArgs& a, float* J, float* slab, Parts src, Parts dst, long nstrm, const TileGeom& tg,
+                  const LaunchTimer& lt = {});
 void pass_mixed(const ModeArgs& m, const FftPlan& p, float* J, const float* slab, const TileGeom& tg, cxf* X, cxf* T,
                 float* Fn, long fpl, double* rl, double* rli, int i0, int j0, int kind, int rpw);
 // the same columns step as three launches: y transforms of X (in place), the per-mode work (one lane per mode),

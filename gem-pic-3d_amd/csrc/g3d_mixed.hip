@@ -11,6 +11,7 @@
 //   k_cols_mixed        y transforms of three planes in FP32 (forward or inverse)
 //   k_modes_mixed       filters, bookkeeping (FP64), march with the FP32 implicit solve, packed planes
 //   k_rows_inv_mixed    x inverse transforms in FP32 and E, B of the slice into F
+#include <cstdlib>
 #include "g3d_fft_core.hpp"
 #include "g3d_pass.hpp"
 #include "g3d_runtime.hpp"
@@ -19,46 +20,60 @@ namespace g3d {
 namespace {
 using namespace fftcore;
 constexpr int kT = 256;
+// the FP32 arithmetic of this file may contract a * b + c into one fused operation (the FP64 parity build keeps
+// contraction off everywhere else)
+#pragma clang fp contract(fast)
 
-// TSC weights in FP32 on a periodic uniform axis: the cell from the FP64 index coordinate, the fraction in FP32
-__device__ __forceinline__ void tsc_f(double x, double d, int n, int idx[3], float w[3]) {
-    const double xi = x / d;
+// FP32 reciprocal and square root of the hardware (v_rcp_f32, v_sqrt_f32: 1 ulp), for the FP32 physics
+__device__ __forceinline__ float rcp_f(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float sqrt_f(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// 1/dx, 1/dy and the first inverse area factor, once per lane
+struct InvF {
+    double ix, iy;
+    float ia0;
+};
+
+// TSC weights in FP32 on a periodic uniform axis: the cell from the FP64 index coordinate x / d (x * (1/d): the
+// cell may differ from the FP64 pass where x / d lies within an ulp of a half-integer), the fraction in FP32.
+__device__ __forceinline__ void tsc_f(double x, double inv, int n, int idx[3], float w[3]) {
+    const double xi = x * inv;
     const int i0 = int(floor(xi + 0.5));
     const float f = float(xi - i0);
     const float a = 0.5f - f, b = 0.5f + f;
     w[0] = 0.5f * a * a;
     w[1] = 0.75f - f * f;
     w[2] = 0.5f * b * b;
-    idx[0] = i0 - 1; idx[1] = i0; idx[2] = i0 + 1;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) { const int m = idx[k] % n; idx[k] = m < 0 ? m + n : m; }
+    idx[0] = wrap_index(i0 - 1, n); idx[1] = wrap_index(i0, n); idx[2] = wrap_index(i0 + 1, n);
 }
 
 __device__ __forceinline__ void boris_f(float& ux, float& uy, float& uz, const float E[3], const float B[3], float qm,
                                         float dt) {
     const float qh = 0.5f * qm * dt;
     const float ax = ux + qh * E[0], ay = uy + qh * E[1], az = uz + qh * E[2];
-    const float gm = sqrtf(1.0f + ax * ax + ay * ay + az * az);
-    const float tx = qh * B[0] / gm, ty = qh * B[1] / gm, tz = qh * B[2] / gm;
+    const float qg = qh * rcp_f(sqrt_f(1.0f + ax * ax + ay * ay + az * az));
+    const float tx = qg * B[0], ty = qg * B[1], tz = qg * B[2];
     const float t2 = tx * tx + ty * ty + tz * tz;
     const float px = ax + (ay * tz - az * ty), py = ay + (az * tx - ax * tz), pz = az + (ax * ty - ay * tx);
-    const float sf = 2.0f / (1.0f + t2);
+    const float sf = 2.0f * rcp_f(1.0f + t2);
     const float bx = ax + sf * (py * tz - pz * ty), by = ay + sf * (pz * tx - px * tz), bz = az + sf * (px * ty - py * tx);
     ux = bx + qh * E[0]; uy = by + qh * E[1]; uz = bz + qh * E[2];
 }
 
-// LDS windows of a tile: the 5 current planes (FP64 sums of the FP32 contributions: the FP32 LDS atomic add
-// serialises on the same-address conflicts of a tile's particles, the FP64 one does not) and the 12 field planes
-// (slices fl, fl+1, float)
+// LDS windows of a tile, node-major so that one node is one address: the field window first (12 floats per node:
+// E, B at slice fl then fl + 1, three 16-byte reads) and the current window after it (5 doubles per node: j_x, j_y,
+// j_z, rho-, rho+; FP64 sums of the FP32 contributions, since the FP32 LDS atomic add serialises on the
+// same-address conflicts of a tile's particles and the FP64 one does not)
+constexpr int kFwNode = 12, kDepNode = 5;
 struct WinF {
-    double* lds;
-    const float* fld;
+    double* lds;              // kDepNode doubles per node
+    const float4* fld;        // 3 float4 per node, or null
     int x0, y0, WX, WY, fl;
 };
 
 // E, B at a point: the tile's field window when the stencil and the slice pair are in it, else F
-__device__ __forceinline__ void gather_f(const FieldView& c, const WinF* w, double x, double y, double zeta, float E[3],
-                                         float B[3]) {
+__device__ __forceinline__ void gather_f(const FieldView& c, const InvF& iv, const WinF* w, double x, double y,
+                                         double zeta, float E[3], float B[3]) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) { E[k] = 0; B[k] = 0; }
     if (!c.self) return;
@@ -68,8 +83,8 @@ __device__ __forceinline__ void gather_f(const FieldView& c, const WinF* w, doub
     const int fl = n - c.a;
     if (fl < 0 || fl > c.ns) return;
     int ix[3], jn[3]; float wx[3], wn[3];
-    tsc_f(x, c.dx, c.nx, ix, wx);
-    tsc_f(y, c.dy, c.ny, jn, wn);
+    tsc_f(x, iv.ix, c.nx, ix, wx);
+    tsc_f(y, iv.iy, c.ny, jn, wn);
     const float f1 = float(fz), f0 = 1.0f - f1;
     float wt[9];
 #pragma unroll
@@ -78,25 +93,23 @@ __device__ __forceinline__ void gather_f(const FieldView& c, const WinF* w, doub
         for (int i = 0; i < 3; ++i) wt[3 * j + i] = wx[i] * wn[j];
     float v[6];
     if (w && w->fld && fl == w->fl) {
-        int u0 = ix[0] - w->x0; u0 = ((u0 % c.nx) + c.nx) % c.nx;
-        int v0 = jn[0] - w->y0; v0 = ((v0 % c.ny) + c.ny) % c.ny;
+        const int u0 = wrap_index(ix[0] - w->x0, c.nx), v0 = wrap_index(jn[0] - w->y0, c.ny);
         if (u0 + 2 < w->WX && v0 + 2 < w->WY) {
-            const int pl = w->WX * w->WY;
+            float s0[6] = {0, 0, 0, 0, 0, 0}, s1[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
-            for (int comp = 0; comp < 6; ++comp) {
-                const float* q0 = w->fld + comp * pl;
-                const float* q1 = w->fld + (6 + comp) * pl;
-                float s = 0;
+            for (int j = 0; j < 3; ++j)
 #pragma unroll
-                for (int j = 0; j < 3; ++j)
-#pragma unroll
-                    for (int i = 0; i < 3; ++i) {
-                        const int k = (v0 + j) * w->WX + u0 + i;
-                        s += wt[3 * j + i] * (f0 * q0[k] + f1 * q1[k]);
-                    }
-                v[comp] = s;
-            }
-            E[0] = v[0]; E[1] = v[1]; E[2] = v[2]; B[0] = v[3]; B[1] = v[4]; B[2] = v[5];
+                for (int i = 0; i < 3; ++i) {
+                    const float4* q = w->fld + 3 * ((v0 + j) * w->WX + u0 + i);
+                    const float4 a = q[0], b = q[1], d = q[2];
+                    const float t = wt[3 * j + i];
+                    s0[0] += t * a.x; s0[1] += t * a.y; s0[2] += t * a.z; s0[3] += t * a.w;
+                    s0[4] += t * b.x; s0[5] += t * b.y;
+                    s1[0] += t * b.z; s1[1] += t * b.w;
+                    s1[2] += t * d.x; s1[3] += t * d.y; s1[4] += t * d.z; s1[5] += t * d.w;
+                }
+            E[0] = f0 * s0[0] + f1 * s1[0]; E[1] = f0 * s0[1] + f1 * s1[1]; E[2] = f0 * s0[2] + f1 * s1[2];
+            B[0] = f0 * s0[3] + f1 * s1[3]; B[1] = f0 * s0[4] + f1 * s1[4]; B[2] = f0 * s0[5] + f1 * s1[5];
             return;
         }
     }
@@ -120,33 +133,31 @@ __device__ __forceinline__ void gather_f(const FieldView& c, const WinF* w, doub
 }
 
 // TSC deposit of f v into the FP32 current planes (window or jsr), as the FP64 deposit
-__device__ __forceinline__ void sdeposit_f(const StreamArgs& c, float* J, const WinF* win, double x, double y, float f,
-                                           float vx, float vy, float vz) {
+__device__ __forceinline__ void sdeposit_f(const StreamArgs& c, const InvF& iv, float* J, const WinF* win, double x,
+                                           double y, float f, float vx, float vy, float vz) {
     const FieldView& P = c.fv;
     int ix[3], jn[3]; float wx[3], wn[3];
-    tsc_f(x, P.dx, P.nx, ix, wx);
-    tsc_f(y, P.dy, P.ny, jn, wn);
+    tsc_f(x, iv.ix, P.nx, ix, wx);
+    tsc_f(y, iv.iy, P.ny, jn, wn);
     const long nk = P.nk;
     const int cq = f < 0 ? 3 : 4;
     const bool mv = vx != 0 || vy != 0 || vz != 0;
-    const float fi = f * float(c.ia[0]);
+    const float fi = f * iv.ia0;
     if (win) {
-        int u0 = ix[0] - win->x0; u0 = ((u0 % P.nx) + P.nx) % P.nx;
-        int v0 = jn[0] - win->y0; v0 = ((v0 % P.ny) + P.ny) % P.ny;
+        const int u0 = wrap_index(ix[0] - win->x0, P.nx), v0 = wrap_index(jn[0] - win->y0, P.ny);
         if (u0 + 2 < win->WX && v0 + 2 < win->WY) {
-            const int wpl = win->WX * win->WY;
 #pragma unroll
             for (int v = 0; v < 3; ++v)
 #pragma unroll
                 for (int u = 0; u < 3; ++u) {
-                    double* L = win->lds + (v0 + v) * win->WX + (u0 + u);
+                    double* L = win->lds + kDepNode * ((v0 + v) * win->WX + (u0 + u));
                     const float ww = fi * wx[u] * wn[v];
                     if (mv) {
                         atomicAdd(L, double(ww * vx));
-                        atomicAdd(L + wpl, double(ww * vy));
-                        atomicAdd(L + 2 * wpl, double(ww * vz));
+                        atomicAdd(L + 1, double(ww * vy));
+                        atomicAdd(L + 2, double(ww * vz));
                     }
-                    atomicAdd(L + cq * wpl, double(ww));
+                    atomicAdd(L + cq, double(ww));
                 }
             return;
         }
@@ -166,16 +177,25 @@ __device__ __forceinline__ void sdeposit_f(const StreamArgs& c, float* J, const 
         }
 }
 
-// one electron through half-cell n (src/sim_stream.hpp:104-171), the physics of the FP64 pass in FP32
-__device__ __forceinline__ void stream_one_mixed(const StreamArgs& c, float* J, const WinF* win, Parts S, Parts D,
-                                                 long i) {
+// the state of a streaming electron, loaded ahead of its push (FP64 in HBM)
+struct MState {
+    double x, y, zeta, z, px, py, pz, q, qm, w;
+    int alive;
+};
+__device__ __forceinline__ MState load_mstate(const Parts& S, long i) {
+    return MState{S.x[i], S.y[i], S.zeta[i], S.z[i], S.px[i], S.py[i], S.pz[i], S.q[i], S.qm[i], S.w[i], S.alive[i]};
+}
+
+// one electron through half-cell n (src/sim_stream.hpp:104-171), the physics of the FP64 pass in FP32; s: its state
+// at the start (loaded by the caller), written to D[i]
+__device__ __forceinline__ void stream_one_mixed(const StreamArgs& c, const InvF& iv, float* J, const WinF* win,
+                                                 const MState& s, Parts D, long i) {
     const int n = c.n;
     const double zn = c.zs[n], zn1 = c.zs[n + 1], h = zn1 - zn;
-    // the whole state up front, so that its loads overlap
-    double x = S.x[i], y = S.y[i], zeta = S.zeta[i], z = S.z[i];
-    const double pxd = S.px[i], pyd = S.py[i], pzd = S.pz[i];
-    const double q = S.q[i], qmd = S.qm[i], w = S.w[i];
-    int alive = S.alive[i];
+    double x = s.x, y = s.y, zeta = s.zeta, z = s.z;
+    const double pxd = s.px, pyd = s.py, pzd = s.pz;
+    const double q = s.q, qmd = s.qm, w = s.w;
+    int alive = s.alive;
     auto store = [&](double ux, double uy, double uz) {
         D.x[i] = x; D.y[i] = y; D.zeta[i] = zeta; D.px[i] = ux; D.py[i] = uy; D.pz[i] = uz;
         D.z[i] = z; D.q[i] = q; D.qm[i] = qmd; D.alive[i] = alive;
@@ -189,13 +209,13 @@ __device__ __forceinline__ void stream_one_mixed(const StreamArgs& c, float* J, 
     const double hl = zeta >= zn1 - 1e-9 * h ? h : (zeta - zn > 0 ? zeta - zn : 0.0);
     const float hf = float(hl), qm = float(qmd);
     const float px = float(pxd), py = float(pyd), pz = float(pzd);
-    const float g0 = sqrtf(1 + px * px + py * py + pz * pz);
-    const float omv0 = (1 + px * px + py * py) / (g0 * (g0 + pz));
-    const float dt0 = hf / omv0;
-    const double xm = x + double(0.5f * dt0 * px / g0), ym = y + double(0.5f * dt0 * py / g0);
-    const double zm = z + double(0.5f * dt0 * pz / g0), zetam = zeta - 0.5 * hl;
+    const float g0 = sqrt_f(1 + px * px + py * py + pz * pz), ig0 = rcp_f(g0);
+    const float omv0 = (1 + px * px + py * py) * rcp_f(g0 * (g0 + pz));
+    const float dt0 = hf * rcp_f(omv0), hs = 0.5f * dt0 * ig0;
+    const double xm = x + double(hs * px), ym = y + double(hs * py);
+    const double zm = z + double(hs * pz), zetam = zeta - 0.5 * hl;
     float E[3], B[3];
-    gather_f(P, win, wrap_mod(xm, P.Lx), wrap_mod(ym, P.Ly), zetam, E, B);
+    gather_f(P, iv, win, wrap_mod(xm, P.Lx), wrap_mod(ym, P.Ly), zetam, E, B);
     if (c.laser.on) {
         double Ee[3], Be[3];
         c.laser.field(xm + c.X0, zm, zm - (zetam - c.zc), Ee, Be);
@@ -203,12 +223,13 @@ __device__ __forceinline__ void stream_one_mixed(const StreamArgs& c, float* J, 
     }
     float ux = px, uy = py, uz = pz;
     boris_f(ux, uy, uz, E, B, qm, dt0);
-    float g1 = sqrtf(1 + ux * ux + uy * uy + uz * uz);
-    float dt = 2 * hf / (omv0 + (1 + ux * ux + uy * uy) / (g1 * (g1 + uz)));
+    float g1 = sqrt_f(1 + ux * ux + uy * uy + uz * uz);
+    float dt = 2 * hf * rcp_f(omv0 + (1 + ux * ux + uy * uy) * rcp_f(g1 * (g1 + uz)));
     ux = px; uy = py; uz = pz;
     boris_f(ux, uy, uz, E, B, qm, dt);
-    g1 = sqrtf(1 + ux * ux + uy * uy + uz * uz);
-    float vx = 0.5f * (px / g0 + ux / g1), vy = 0.5f * (py / g0 + uy / g1), vz = 0.5f * (pz / g0 + uz / g1);
+    g1 = sqrt_f(1 + ux * ux + uy * uy + uz * uz);
+    float ig1 = rcp_f(g1);
+    float vx = 0.5f * (px * ig0 + ux * ig1), vy = 0.5f * (py * ig0 + uy * ig1), vz = 0.5f * (pz * ig0 + uz * ig1);
     double zeta_end = zn;
     bool trapped = false;
     if (z + double(vz * dt) >= c.sTrap) {
@@ -216,8 +237,9 @@ __device__ __forceinline__ void stream_one_mixed(const StreamArgs& c, float* J, 
         const float dtp = float((c.sTrap - z) / vzc) > 0 ? float((c.sTrap - z) / vzc) : 0.0f;
         ux = px; uy = py; uz = pz;
         boris_f(ux, uy, uz, E, B, qm, dtp);
-        g1 = sqrtf(1 + ux * ux + uy * uy + uz * uz);
-        vx = 0.5f * (px / g0 + ux / g1); vy = 0.5f * (py / g0 + uy / g1); vz = 0.5f * (pz / g0 + uz / g1);
+        g1 = sqrt_f(1 + ux * ux + uy * uy + uz * uz);
+        ig1 = rcp_f(g1);
+        vx = 0.5f * (px * ig0 + ux * ig1); vy = 0.5f * (py * ig0 + uy * ig1); vz = 0.5f * (pz * ig0 + uz * ig1);
         dt = dtp;
         zeta_end = zeta - double((1 - vz) * dtp);
         trapped = true;
@@ -226,53 +248,77 @@ __device__ __forceinline__ void stream_one_mixed(const StreamArgs& c, float* J, 
     x += double(vx * dt); y += double(vy * dt);
     z = trapped ? c.s1 : z + double(vz * dt);
     zeta = zeta_end;
-    const float f = float(q * w * double(dt) / (c.Delta * h * P.dx));
-    if (f != 0) sdeposit_f(c, J, win, 0.5 * (xo + x), 0.5 * (yo + y), f, vx, vy, vz);
+    // q w dt / (Delta h dx), the cell factors in FP32
+    const float qw = float(q * w) * rcp_f(float(c.Delta * h * P.dx));
+    const float f = qw * dt;
+    if (f != 0)
+        sdeposit_f(c, iv, J, win, wrap_mod(0.5 * (xo + x), P.Lx), wrap_mod(0.5 * (yo + y), P.Ly), f, vx, vy, vz);
     if (trapped) {
         alive = 5;
-        const float vze = uz / g1;
-        if (vze >= float(c.vz_beam)) sdeposit_f(c, J, win, x, y, float(q * w / (h * P.dx * vze)), ux / g1, uy / g1, vze);
+        const float vze = uz * ig1;
+        if (vze >= float(c.vz_beam))
+            sdeposit_f(c, iv, J, win, wrap_mod(x, P.Lx), wrap_mod(y, P.Ly), qw * rcp_f(vze), ux * ig1, uy * ig1, vze);
     }
     store(ux, uy, uz);
 }
 
-__global__ void __launch_bounds__(kT) k_stream_mixed(StreamArgs c, float* J, float* slab, Parts S, Parts D, long n,
-                                                     TileGeom tg) {
-    extern __shared__ double winf[];
+// WPE: waves per SIMD the register allocation must allow (0: the compiler's choice)
+template <int WPE>
+__global__ void __launch_bounds__(kStreamBlockMax) __attribute__((amdgpu_waves_per_eu(WPE ? WPE : 1)))
+k_stream_mixed(StreamArgs c, float* J, float* slab, Parts S, Parts D, long n, TileGeom tg) {
+    extern __shared__ float4 winf4[];
     const long b = blockIdx.x;
     if (b < tg.ntile) {
+        // the tile's range, this lane's first particle and its state first: their dependent loads overlap the
+        // loads of the field window
+        const long lo = tg.off[b], hi = tg.off[b + 1];
+        long k = lo + threadIdx.x;
+        long i = k < hi ? (tg.idx ? long(tg.idx[k]) : k) : 0;
+        MState st{};
+        if (k < hi) st = load_mstate(S, i);
         const int pl = tg.WX * tg.WY;
-        const int wn = 5 * pl;
+        const int wn = kDepNode * pl;
         const int tx = int(b % tg.ntx), ty = int(b / tg.ntx);
         const int x0 = tx * tg.TX - 1 - tg.M, y0 = ty * tg.TY - 1 - tg.M;
         const FieldView& F = c.fv;
         const int fl = c.n - F.a;
-        float* fw = reinterpret_cast<float*>(winf + wn);
-        for (int k = threadIdx.x; k < wn; k += blockDim.x) winf[k] = 0;
-        if (F.self)
+        double* dep = reinterpret_cast<double*>(winf4 + 3 * pl);
+        for (int r = threadIdx.x; r < wn; r += blockDim.x) dep[r] = 0;
+        if (F.self)      // one node per lane: its 12 values from the 6 component planes of slices fl, fl + 1
             for (int r = threadIdx.x; r < pl; r += blockDim.x) {
                 const int v = r / tg.WX, u = r - v * tg.WX;
-                const int gx = ((x0 + u) % F.nx + F.nx) % F.nx, gy = ((y0 + v) % F.ny + F.ny) % F.ny;
+                const int gx = wrap_index(x0 + u, F.nx), gy = wrap_index(y0 + v, F.ny);
                 const float* src = F.F + long(fl) * F.nk + long(gy) * F.nx + gx;
                 const long cs = long(F.ns + 2) * F.nk;
+                float q[kFwNode];
 #pragma unroll
                 for (int comp = 0; comp < 6; ++comp) {
-                    fw[comp * pl + r] = src[comp * cs];
-                    fw[(6 + comp) * pl + r] = src[comp * cs + F.nk];
+                    q[comp] = src[comp * cs];
+                    q[6 + comp] = src[comp * cs + F.nk];
                 }
+                winf4[3 * r] = make_float4(q[0], q[1], q[2], q[3]);
+                winf4[3 * r + 1] = make_float4(q[4], q[5], q[6], q[7]);
+                winf4[3 * r + 2] = make_float4(q[8], q[9], q[10], q[11]);
             }
         __syncthreads();
-        WinF w{winf, F.self ? fw : nullptr, x0, y0, tg.WX, tg.WY, fl};
-        const long hi = tg.off[b + 1];
-        for (long k = tg.off[b] + threadIdx.x; k < hi; k += blockDim.x)
-            stream_one_mixed(c, J, &w, S, D, tg.idx ? long(tg.idx[k]) : k);
+        WinF w{dep, F.self ? winf4 : nullptr, x0, y0, tg.WX, tg.WY, fl};
+        const InvF iv{1.0 / F.dx, 1.0 / F.dy, float(c.ia[0])};
+        while (k < hi) {
+            stream_one_mixed(c, iv, J, &w, st, D, i);
+            k += blockDim.x;
+            if (k < hi) {
+                i = tg.idx ? long(tg.idx[k]) : k;
+                st = load_mstate(S, i);
+            }
+        }
         __syncthreads();
         float* sl = slab + b * long(wn);
-        for (int k = threadIdx.x; k < wn; k += blockDim.x) sl[k] = float(winf[k]);
+        for (int r = threadIdx.x; r < wn; r += blockDim.x) sl[r] = float(dep[r]);
         return;
     }
     const long i = tg.n_tiled + (b - tg.ntile) * blockDim.x + threadIdx.x;
-    if (i < n) stream_one_mixed(c, J, nullptr, S, D, i);
+    if (i < n)
+        stream_one_mixed(c, InvF{1.0 / c.fv.dx, 1.0 / c.fv.dy, float(c.ia[0])}, J, nullptr, load_mstate(S, i), D, i);
 }
 
 // ---------------------------------------------------------------- rows forward (FP32)
@@ -297,18 +343,18 @@ __global__ void __launch_bounds__(kT) k_rows_fwd_mixed(float* J, const float* sl
             const long wpl = long(tg.WX) * tg.WY;
             for (int dy = -1; dy <= 1; ++dy) {
                 if ((dy == 1 && tg.nty <= 2) || (dy != 0 && tg.nty == 1)) continue;
-                const int ty = ((tpy + dy) % tg.nty + tg.nty) % tg.nty;
+                const int ty = wrap_index(tpy + dy, tg.nty);
                 int vv = j - (ty * tg.TY - 1 - tg.M);
-                vv = ((vv % ny) + ny) % ny;
+                vv = wrap_index(vv, ny);
                 if (vv >= tg.WY) continue;
                 for (int dx = -1; dx <= 1; ++dx) {
                     if ((dx == 1 && tg.ntx <= 2) || (dx != 0 && tg.ntx == 1)) continue;
-                    const int tx = ((tpx + dx) % tg.ntx + tg.ntx) % tg.ntx;
+                    const int tx = wrap_index(tpx + dx, tg.ntx);
                     int uu = ixx - (tx * tg.TX - 1 - tg.M);
-                    uu = ((uu % nx) + nx) % nx;
+                    uu = wrap_index(uu, nx);
                     if (uu >= tg.WX) continue;
-                    const float* sl = slab + (long(ty) * tg.ntx + tx) * 5 * wpl + long(vv) * tg.WX + uu;
-                    for (int cc = 0; cc < 5; ++cc) v[cc] += sl[cc * wpl];
+                    const float* sl = slab + ((long(ty) * tg.ntx + tx) * wpl + long(vv) * tg.WX + uu) * kDepNode;
+                    for (int cc = 0; cc < 5; ++cc) v[cc] += sl[cc];
                 }
             }
         }
@@ -413,12 +459,16 @@ __global__ void __launch_bounds__(kT) k_rows_inv_mixed(const cxf* T, float* Fn, 
 int lanes(int work) { int t = 64; while (t < work && t < kT) t <<= 1; return t; }
 }  // namespace
 
-void stream_mixed(const StreamArgs& a, float* J, float* slab, Parts src, Parts dst, long nstrm, const TileGeom& tg) {
+void stream_mixed(const StreamArgs& a, float* J, float* slab, Parts src, Parts dst, long nstrm, const TileGeom& tg,
+                  const LaunchTimer& lt) {
     if (nstrm <= 0) return;
-    const long tail_blocks = nstrm > tg.n_tiled ? long(blocks_for(nstrm - tg.n_tiled, kT)) : 0;
-    const size_t lds = tg.ntile ? size_t(tg.WX) * tg.WY * (5 * sizeof(double) + 12 * sizeof(float)) : 0;
-    hipLaunchKernelGGL(k_stream_mixed, dim3(unsigned(tg.ntile + tail_blocks)), dim3(kT), lds, stream(), a, J, slab, src,
-                       dst, nstrm, tg);
+    const int bs = tg.ntile ? tg.block : kT;
+    if (bs < 64 || bs > kStreamBlockMax || bs % 64) throw std::runtime_error("stream half-cell (mixed): bad block size");
+    const long tail_blocks = nstrm > tg.n_tiled ? long(blocks_for(nstrm - tg.n_tiled, bs)) : 0;
+    const size_t lds = tg.ntile ? size_t(tg.WX) * tg.WY * (kDepNode * sizeof(double) + kFwNode * sizeof(float)) : 0;
+    const dim3 grid(unsigned(tg.ntile + tail_blocks)), blk(bs);
+    launch_timed_kernel(lt, tg.wpe == 4 ? k_stream_mixed<4> : k_stream_mixed<0>, grid, blk, unsigned(lds), a, J, slab,
+                        src, dst, nstrm, tg);
     launch_check("stream half-cell (mixed)");
 }
 void pass_mixed(const ModeArgs& m, const FftPlan& p, float* J, const float* slab, const TileGeom& tg, cxf* X, cxf* T,
@@ -429,9 +479,10 @@ void pass_mixed(const ModeArgs& m, const FftPlan& p, float* J, const float* slab
         hipLaunchKernelGGL(k_rows_fwd_mixed, dim3(p.ny / rpw), dim3(lanes(rpw * p.nx)), lds, stream(), J, slab, tg, p.nx,
                            p.ny, p.lx, rpw, p.twx, X, rl, rli, i0, j0);
     }
-    int nc = 8;
-    while (nc > 4 && 3L * (p.nx / nc) < 512) nc >>= 1;
+    int nc = 8;                        // 64-byte row segments (measured best at 256 x 256 against 4 and 16)
     if (nc > p.nx) nc = p.nx;
+    static const int forced = [] { const char* e = getenv("G3D_FFT_COLS_NC"); return e ? atoi(e) : 0; }();
+    if (forced > 0 && forced <= p.nx && p.nx % forced == 0) nc = forced;
     const size_t lc = (size_t(nc) * (p.ny + p.ny / 8) + p.ny / 2) * sizeof(cxf);
     const int tc = lanes(nc * (p.ny / 8 > 0 ? p.ny / 8 : 1));
     if (kind != 1)

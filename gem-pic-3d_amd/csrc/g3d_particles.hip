@@ -109,8 +109,7 @@ __device__ __forceinline__ bool gather_window(const FieldView& c, const Win& w, 
     int ix[3], jn[3]; double wx[3], wn[3];
     tsc_periodic(x, c.dx, c.nx, ix, wx);
     tsc_periodic(y, c.dy, c.ny, jn, wn);
-    int u0 = ix[0] - w.x0; u0 = ((u0 % c.nx) + c.nx) % c.nx;
-    int v0 = jn[0] - w.y0; v0 = ((v0 % c.ny) + c.ny) % c.ny;
+    const int u0 = wrap_index(ix[0] - w.x0, c.nx), v0 = wrap_index(jn[0] - w.y0, c.ny);
     if (u0 + 2 >= w.WX || v0 + 2 >= w.WY) return false;
     const double gz = 1 - fz;
     int k[9]; double wt[9];
@@ -155,8 +154,8 @@ __device__ __forceinline__ void sdeposit(const StreamArgs& c, const Win* win, do
         int u0 = 0, v0 = 0;
         bool in = false;
         if (win) {
-            u0 = ix[0] - win->x0; u0 = ((u0 % P.nx) + P.nx) % P.nx;
-            v0 = jn[0] - win->y0; v0 = ((v0 % P.ny) + P.ny) % P.ny;
+            u0 = wrap_index(ix[0] - win->x0, P.nx);
+            v0 = wrap_index(jn[0] - win->y0, P.ny);
             in = u0 + 2 < win->WX && v0 + 2 < win->WY;
         }
         if (in) {
@@ -323,10 +322,14 @@ __device__ __forceinline__ void stream_one(const StreamArgs& c, const Win* win, 
     }
 }
 
+__device__ __forceinline__ PState load_pstate(const Parts& S, long i) {
+    return PState{S.x[i], S.y[i], S.zeta[i], S.px[i], S.py[i], S.pz[i], S.w[i], S.z[i], S.q[i], S.qm[i], S.alive[i],
+                  S.sp[i], S.id[i]};
+}
+
+// p: the state of list entry i (loaded by the caller); the pushed state goes to D[i]
 template <int YF, bool ION>
-__device__ __forceinline__ void stream_index(const StreamArgs& c, const Win* win, Parts S, Parts D, long i) {
-    PState p{S.x[i], S.y[i], S.zeta[i], S.px[i], S.py[i], S.pz[i], S.w[i], S.z[i], S.q[i], S.qm[i], S.alive[i],
-             S.sp[i], S.id[i]};
+__device__ __forceinline__ void stream_index(const StreamArgs& c, const Win* win, PState p, Parts D, long i) {
     stream_one<YF, ION>(c, win, p);
     D.x[i] = p.x; D.y[i] = p.y; D.zeta[i] = p.zeta; D.px[i] = p.px; D.py[i] = p.py; D.pz[i] = p.pz;
     D.z[i] = p.z; D.q[i] = p.q; D.qm[i] = p.qm; D.alive[i] = p.alive;
@@ -335,10 +338,19 @@ __device__ __forceinline__ void stream_index(const StreamArgs& c, const Win* win
 // workgroups 0 .. ntile-1: the particles of one tile with the LDS window, written to the tile's slab at the end;
 // the following workgroups: the untiled particles from off[ntile] on, one per lane, global atomics
 template <int YF, bool ION>
-__global__ void __launch_bounds__(kT) k_stream_halfcell(StreamArgs c, Parts S, Parts D, long n, TileGeom tg) {
+__global__ void __launch_bounds__(kStreamBlockMax) k_stream_halfcell(StreamArgs c, Parts S, Parts D, long n,
+                                                                     TileGeom tg) {
     extern __shared__ double win_lds[];
     const long b = blockIdx.x;
     if (b < tg.ntile) {
+        // the particles binned into this tile (tg.idx: indices into the list; null: the contiguous load ranges).
+        // The range, this lane's first particle and its state first: their dependent loads overlap those of the
+        // field window.
+        const long lo = tg.off[b], hi = tg.off[b + 1];
+        long k = lo + threadIdx.x;
+        long i = k < hi ? (tg.idx ? long(tg.idx[k]) : k) : 0;
+        PState p{};
+        if (k < hi) p = load_pstate(S, i);
         const int pl = tg.WX * tg.WY;
         const long wn = 5L * pl;
         const int tx = int(b % tg.ntx), ty = int(b / tg.ntx);
@@ -346,12 +358,12 @@ __global__ void __launch_bounds__(kT) k_stream_halfcell(StreamArgs c, Parts S, P
         const FieldView& F = c.fv;
         const int fl = c.n - F.a;
         float* fw = reinterpret_cast<float*>(win_lds + wn);
-        for (long k = threadIdx.x; k < wn; k += blockDim.x) win_lds[k] = 0;
+        for (long r = threadIdx.x; r < wn; r += blockDim.x) win_lds[r] = 0;
         const bool fwin = tg.fwin && F.self;
         if (fwin)        // slices fl, fl + 1 of the 6 components: [slice][comp][v][u]; one node per lane, 12 loads
             for (int r = threadIdx.x; r < pl; r += blockDim.x) {
                 const int v = r / tg.WX, u = r - v * tg.WX;
-                const int gx = ((x0 + u) % F.nx + F.nx) % F.nx, gy = ((y0 + v) % F.ny + F.ny) % F.ny;
+                const int gx = wrap_index(x0 + u, F.nx), gy = wrap_index(y0 + v, F.ny);
                 const float* src = F.F + long(fl) * F.nk + long(gy) * F.nx + gx;
                 const long cs = long(F.ns + 2) * F.nk;
 #pragma unroll
@@ -362,17 +374,21 @@ __global__ void __launch_bounds__(kT) k_stream_halfcell(StreamArgs c, Parts S, P
             }
         __syncthreads();
         Win w{win_lds, fwin ? fw : nullptr, x0, y0, tg.WX, tg.WY, fl};
-        // the particles binned into this tile (tg.idx: indices into the list; null: the contiguous load ranges)
-        const long hi = tg.off[b + 1];
-        for (long k = tg.off[b] + threadIdx.x; k < hi; k += blockDim.x)
-            stream_index<YF, ION>(c, &w, S, D, tg.idx ? long(tg.idx[k]) : k);
+        while (k < hi) {
+            stream_index<YF, ION>(c, &w, p, D, i);
+            k += blockDim.x;
+            if (k < hi) {
+                i = tg.idx ? long(tg.idx[k]) : k;
+                p = load_pstate(S, i);
+            }
+        }
         __syncthreads();
         double* sl = tg.slab + b * wn;
-        for (long k = threadIdx.x; k < wn; k += blockDim.x) sl[k] = win_lds[k];
+        for (long r = threadIdx.x; r < wn; r += blockDim.x) sl[r] = win_lds[r];
         return;
     }
     const long i = tg.n_tiled + (b - tg.ntile) * blockDim.x + threadIdx.x;
-    if (i < n) stream_index<YF, ION>(c, nullptr, S, D, i);
+    if (i < n) stream_index<YF, ION>(c, nullptr, load_pstate(S, i), D, i);
 }
 
 // ---------------------------------------------------------------- binning of the streaming list by tile
@@ -531,20 +547,21 @@ void bunch_push(const PushArgs& a, Parts p, long np, double s0) {
     hipLaunchKernelGGL(k_bunch_push, dim3(blocks_for(np, kT)), dim3(kT), 0, stream(), a, p, np, s0);
     launch_check("bunch push");
 }
-void stream_halfcell(const StreamArgs& a, Parts src, Parts dst, long nstrm, const TileGeom& tg) {
+void stream_halfcell(const StreamArgs& a, Parts src, Parts dst, long nstrm, const TileGeom& tg,
+                     const LaunchTimer& lt) {
     if (nstrm <= 0) return;
-    const long tail_blocks = nstrm > tg.n_tiled ? long(blocks_for(nstrm - tg.n_tiled, kT)) : 0;
+    // tile workgroups of tg.block lanes, then the untiled tail in workgroups of the same size
+    const int bs = tg.ntile ? tg.block : kT;
+    if (bs < 64 || bs > kStreamBlockMax || bs % 64) throw std::runtime_error("stream half-cell: bad block size");
+    const long tail_blocks = nstrm > tg.n_tiled ? long(blocks_for(nstrm - tg.n_tiled, bs)) : 0;
     // the deposit window (5 double planes) and the field window (12 float planes)
     const size_t lds = tg.ntile ? size_t(tg.WX) * tg.WY * (5 * sizeof(double) + (tg.fwin ? 12 * sizeof(float) : 0)) : 0;
-    const dim3 grid(unsigned(tg.ntile + tail_blocks)), blk(kT);
+    const dim3 grid(unsigned(tg.ntile + tail_blocks)), blk(bs);
     const bool ion = a.ions != nullptr;
-    if (a.fv.yf) {
-        if (ion) hipLaunchKernelGGL((k_stream_halfcell<1, true>), grid, blk, lds, stream(), a, src, dst, nstrm, tg);
-        else hipLaunchKernelGGL((k_stream_halfcell<1, false>), grid, blk, lds, stream(), a, src, dst, nstrm, tg);
-    } else {
-        if (ion) hipLaunchKernelGGL((k_stream_halfcell<0, true>), grid, blk, lds, stream(), a, src, dst, nstrm, tg);
-        else hipLaunchKernelGGL((k_stream_halfcell<0, false>), grid, blk, lds, stream(), a, src, dst, nstrm, tg);
-    }
+    const unsigned sh = unsigned(lds);
+    auto* k = a.fv.yf ? (ion ? k_stream_halfcell<1, true> : k_stream_halfcell<1, false>)
+                      : (ion ? k_stream_halfcell<0, true> : k_stream_halfcell<0, false>);
+    launch_timed_kernel(lt, k, grid, blk, sh, a, src, dst, nstrm, tg);
     launch_check("stream half-cell");
 }
 void bin_tiles(Parts S, Parts D, long n, const TileGeom& tg, double dx, double dy, int nx, int ny, int* cnt,

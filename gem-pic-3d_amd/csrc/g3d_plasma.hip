@@ -46,15 +46,15 @@ __global__ void __launch_bounds__(kT) k_pass_rows_fwd(double* J, TileGeom tg, in
             const long wpl = long(tg.WX) * tg.WY;
             for (int dy = -1; dy <= 1; ++dy) {
                 if ((dy == 1 && tg.nty <= 2) || (dy != 0 && tg.nty == 1)) continue;
-                const int ty = ((tpy + dy) % tg.nty + tg.nty) % tg.nty;
+                const int ty = wrap_index(tpy + dy, tg.nty);
                 int vv = j - (ty * tg.TY - 1 - tg.M);
-                vv = ((vv % ny) + ny) % ny;
+                vv = wrap_index(vv, ny);
                 if (vv >= tg.WY) continue;
                 for (int dx = -1; dx <= 1; ++dx) {
                     if ((dx == 1 && tg.ntx <= 2) || (dx != 0 && tg.ntx == 1)) continue;
-                    const int tx = ((tpx + dx) % tg.ntx + tg.ntx) % tg.ntx;
+                    const int tx = wrap_index(tpx + dx, tg.ntx);
                     int uu = ixx - (tx * tg.TX - 1 - tg.M);
-                    uu = ((uu % nx) + nx) % nx;
+                    uu = wrap_index(uu, nx);
                     if (uu >= tg.WX) continue;
                     const double* sl = tg.slab + (long(ty) * tg.ntx + tx) * 5 * wpl + long(vv) * tg.WX + uu;
                     for (int cc = 0; cc < 5; ++cc) v[cc] += sl[cc * wpl];

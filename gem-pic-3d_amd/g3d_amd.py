@@ -255,7 +255,7 @@ def _load():
         "g3d_sim_set": ([P, C.c_char_p, P, C.c_size_t], C.c_int),
         "g3d_sim_set_particles": ([P, P, C.c_long], C.c_int), "g3d_sim_set_joining": ([P, P, C.c_long], C.c_int),
         "g3d_sim_get_joining": ([P, P, C.c_long, C.POINTER(C.c_long)], C.c_int),
-        "g3d_sim_timing": ([P, C.c_int], C.c_int),
+        "g3d_sim_timing": ([P, C.c_int], C.c_int), "g3d_sim_timing_every": ([P, C.c_int], C.c_int),
         "g3d_sim_timers": ([P, C.POINTER(C.c_double), C.POINTER(C.c_long), C.c_int, C.c_char_p, C.c_int], C.c_int),
     }
     for name, (args, res) in sig.items():
@@ -374,6 +374,10 @@ class Sim(SimBase):
         else:
             mask = sum(1 << self.FAMILIES.index(n) for n in on)
         check(lib().g3d_sim_timing(self.h, mask))
+
+    def timing_every(self, every):
+        """time one launch in `every` of the streaming kernel (its launches carry their own events)"""
+        check(lib().g3d_sim_timing_every(self.h, int(every)))
 
     def timers(self):
         nf = 8

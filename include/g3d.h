@@ -171,6 +171,10 @@ int g3d_sim_get_joining(g3d_sim* sim, g3d_particle* p, long cap, long* n);
  * g3d_sim_timers reads the accumulated milliseconds and launch counts with the family names. */
 int g3d_sim_timing(g3d_sim* sim, int mask);
 int g3d_sim_timers(g3d_sim* sim, double* ms, long* counts, int nfam, char* names, int name_len);
+/* Time one launch in `every` (>= 1) of the streaming half-cell kernel, whose launches carry their own events, so
+ * that timing perturbs the march little; the counts of g3d_sim_timers are the timed launches.  No counterpart in
+ * the reference (a measurement hook). */
+int g3d_sim_timing_every(g3d_sim* sim, int every);
 
 #ifdef __cplusplus
 }
