@@ -25,7 +25,12 @@ bool lookup(g3d::Sim& s, const std::string& n, void*& dev, size_t& bytes) {
         {"js0y", {s.js0y.data(), bytes_of(s.js0y)}}, {"js0z", {s.js0z.data(), bytes_of(s.js0z)}},
         {"jw", {s.jw.data(), bytes_of(s.jw)}}, {"F", {s.F.data(), bytes_of(s.F)}},
         {"jsr", {s.jsr.data(), bytes_of(s.jsr)}}, {"jsn", {s.jsn.data(), bytes_of(s.jsn)}},
-        {"rho_lines", {s.rho_lines.data(), bytes_of(s.rho_lines)}}};
+        {"rho_lines", {s.rho_lines.data(), bytes_of(s.rho_lines)}},
+        // the march geometry (src/sim_setup.cpp:5-60, 148-164): k vectors with the Nyquist mode zeroed, the
+        // half-cell lengths and the two-point current weights of each slice
+        {"kx", {s.kxd.data(), bytes_of(s.kxd)}}, {"ky", {s.kyd.data(), bytes_of(s.kyd)}},
+        {"hcg", {s.hcg.data(), bytes_of(s.hcg)}}, {"wlo", {s.wlo.data(), bytes_of(s.wlo)}},
+        {"whi", {s.whi.data(), bytes_of(s.whi)}}};
     auto it = m.find(n);
     if (it == m.end()) return false;
     dev = it->second.first; bytes = it->second.second;

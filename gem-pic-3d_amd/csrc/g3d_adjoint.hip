@@ -1,0 +1,143 @@
+// Adjoint (reverse-mode) kernels of the march, for gradients through it.  No counterpart in the reference.
+//
+// The per-mode march of slice n (march_mode) is complex-linear in its inputs: the level-s0 values of slice n
+// (R_x, R_y, E_z, B_z), the level-s1 values of slice n+1 (six), and the currents of half-cells n-1 and n.  Its
+// coefficients are real (D, h/4, h/2, h, 1/den, the ζ weights) or the pure imaginary a = i k_x, b = i k_y.  For the
+// real inner product <u, v> = Re Σ conj(u) v, the adjoint of v = A u is u' = A^H v', so every statement of the
+// forward update is transposed in reverse order with conj(a) = -a, conj(b) = -b.  The march over slices runs front
+// to rear (n = hi-1 ... lo, slice n reading slice n+1); its adjoint runs rear to front, carrying the adjoint of the
+// slice n+1 values from the update of slice n to that of slice n+1.
+#include "g3d_kernels.hpp"
+#include "g3d_runtime.hpp"
+
+namespace g3d {
+namespace {
+
+struct AdjView {
+    cx *Rx, *Ry, *Ez, *Bz, *Lx, *Ly;          // in: adjoints of the outputs; out: adjoints of the level-s0 inputs
+    cx* up;                                   // out: 6 planes, adjoint of slice hi (hi < N), or null
+    cx *j0x, *j0y, *jkx, *jky, *jkz;          // out (+=): adjoints of the currents, planes n - a
+    const double *kx, *ky, *hcg, *wlo, *whi;
+    double D;
+    int a, N, nx;
+    long nk;
+};
+
+// adjoints of one slice's inputs given the adjoint r of its outputs; o: old (R_x, R_y, E_z, B_z), u: slice n+1,
+// jd, ju: currents of half-cells n-1 (d) and n (u) as (j0x, j0y, jkx, jky[, jkz])
+struct CurAdj { cx j0x_d, j0x_u, j0y_d, j0y_u, jkx_d, jkx_u, jky_d, jky_u, jkz_u; };
+
+__device__ __forceinline__ void march_mode_adj(const AdjView& c, long i, int n, const Mode6& r, Mode6& o, Mode6& u,
+                                               CurAdj& J) {
+    const int ixx = int(i % c.nx), j = int(i / c.nx);
+    const double kxv = c.kx[ixx], kyv = c.ky[j];
+    const cx ac = ik(-kxv), bc = ik(-kyv);                 // conj(a), conj(b)
+    const double k2 = kxv * kxv + kyv * kyv, D = c.D, wl = c.wlo[n], wu = c.whi[n];
+    cx Rsx = r.Rx, Rsy = r.Ry;
+    u = Mode6{};
+    J = CurAdj{};
+    if (n != c.N - 1) {
+        const double h = c.hcg[n + 2];
+        const double qh = 0.25 * h, hh = 0.5 * h;
+        const double den = 1 + qh * (D + qh) * k2;
+        cx Lsx = r.Lx, Lsy = r.Ly;
+        // r.Rx = Rsx + D (a Ezn + b Bzn), r.Ry = Rsy + D (b Ezn - a Bzn),
+        // r.Lx = Lsx - qh (a Ezn - b Bzn), r.Ly = Lsy - qh (a Bzn + b Ezn), r.Ez = Ezn, r.Bz = Bzn
+        const cx Ezb = r.Ez + D * (ac * r.Rx) + D * (bc * r.Ry) - qh * (ac * r.Lx) - qh * (bc * r.Ly);
+        const cx Bzb = r.Bz + D * (bc * r.Rx) - D * (ac * r.Ry) + qh * (bc * r.Lx) - qh * (ac * r.Ly);
+        // Ezn = (up.Ez - hh Qp - qh SE - h upz) / den,  Bzn = (up.Bz - hh Cp - qh SB) / den
+        const cx te = Ezb / den, tb = Bzb / den;
+        u.Ez = te; u.Bz = tb;
+        const cx Qpb = -hh * te, SEb = -qh * te, Cpb = -hh * tb, SBb = -qh * tb;
+        J.jkz_u = -h * te;
+        // SE = -a Rsx + a Lsx + b Lsy - b Rsy,  SB = a Rsy + a Lsy - b Rsx - b Lsx
+        Rsx = Rsx - ac * SEb - bc * SBb;
+        Rsy = Rsy - bc * SEb + ac * SBb;
+        Lsx = Lsx + ac * SEb - bc * SBb;
+        Lsy = Lsy + bc * SEb + ac * SBb;
+        // Qp = -0.5 a (up.Rx - up.Lx) + 0.5 b (up.Ly - up.Ry),  Cp = 0.5 (a (up.Ry + up.Ly) - b (up.Rx + up.Lx))
+        const cx qa = 0.5 * (ac * Qpb), qb = 0.5 * (bc * Qpb), ca = 0.5 * (ac * Cpb), cb = 0.5 * (bc * Cpb);
+        u.Rx = -qa - cb;
+        u.Lx = qa - cb;
+        u.Ly = qb + ca;
+        u.Ry = -qb + ca;
+        // Lsx = up.Lx - qh (a up.Ez - b up.Bz) - hh jkx_u,  Lsy = up.Ly - qh (a up.Bz + b up.Ez) - hh jky_u
+        u.Lx = u.Lx + Lsx;
+        u.Ly = u.Ly + Lsy;
+        u.Ez = u.Ez - qh * (ac * Lsx) - qh * (bc * Lsy);
+        u.Bz = u.Bz + qh * (bc * Lsx) - qh * (ac * Lsy);
+        J.jkx_u = -hh * Lsx;
+        J.jky_u = -hh * Lsy;
+    }
+    // Rsx = old.Rx + D (a old.Ez + b old.Bz) - D Jx,  Rsy = old.Ry + D (b old.Ez - a old.Bz) - D Jy
+    o.Rx = Rsx;
+    o.Ry = Rsy;
+    o.Ez = D * (ac * Rsx) + D * (bc * Rsy);
+    o.Bz = D * (bc * Rsx) - D * (ac * Rsy);
+    o.Lx = o.Ly = mk(0);
+    const cx Jxb = -D * Rsx, Jyb = -D * Rsy;
+    // Jx = wl (j0x_d + jkx_d) + wu (j0x_u + jkx_u), Jy likewise
+    J.j0x_d = wl * Jxb; J.jkx_d = wl * Jxb; J.j0x_u = wu * Jxb; J.jkx_u = J.jkx_u + wu * Jxb;
+    J.j0y_d = wl * Jyb; J.jky_d = wl * Jyb; J.j0y_u = wu * Jyb; J.jky_u = J.jky_u + wu * Jyb;
+}
+
+// slices lo ... hi-1 of every mode, rear to front: the transpose of k_march_all
+__global__ void __launch_bounds__(256) k_march_adj(AdjView c, int lo, int hi) {
+    const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= c.nk) return;
+    Mode6 carry{};                                  // adjoint of slice n's outputs from slice n-1's use of them
+    cx d0x = mk(0), d0y = mk(0), dkx = mk(0), dky = mk(0), dkz = mk(0);   // half-cell n-1 (plane n - a) so far
+    for (int n = lo; n < hi; ++n) {
+        const long o = long(n - c.a) * c.nk + i;
+        const Mode6 r{c.Rx[o] + carry.Rx, c.Ry[o] + carry.Ry, c.Ez[o] + carry.Ez, c.Bz[o] + carry.Bz,
+                      c.Lx[o] + carry.Lx, c.Ly[o] + carry.Ly};
+        Mode6 old, up;
+        CurAdj J;
+        march_mode_adj(c, i, n, r, old, up, J);
+        c.Rx[o] = old.Rx; c.Ry[o] = old.Ry; c.Ez[o] = old.Ez; c.Bz[o] = old.Bz; c.Lx[o] = old.Lx; c.Ly[o] = old.Ly;
+        // plane n - a is half-cell n-1: the "u" currents of slice n-1 and the "d" currents of slice n
+        c.j0x[o] = c.j0x[o] + d0x + J.j0x_d;
+        c.j0y[o] = c.j0y[o] + d0y + J.j0y_d;
+        c.jkx[o] = c.jkx[o] + dkx + J.jkx_d;
+        c.jky[o] = c.jky[o] + dky + J.jky_d;
+        c.jkz[o] = c.jkz[o] + dkz;
+        d0x = J.j0x_u; d0y = J.j0y_u; dkx = J.jkx_u; dky = J.jky_u; dkz = J.jkz_u;
+        carry = up;
+    }
+    const long o = long(hi - c.a) * c.nk + i;        // plane hi - a: half-cell hi-1
+    c.j0x[o] = c.j0x[o] + d0x;
+    c.j0y[o] = c.j0y[o] + d0y;
+    c.jkx[o] = c.jkx[o] + dkx;
+    c.jky[o] = c.jky[o] + dky;
+    c.jkz[o] = c.jkz[o] + dkz;
+    if (c.up && hi < c.N) {
+        const long nk = c.nk;
+        c.up[i] = carry.Rx; c.up[nk + i] = carry.Ry; c.up[2 * nk + i] = carry.Ez;
+        c.up[3 * nk + i] = carry.Bz; c.up[4 * nk + i] = carry.Lx; c.up[5 * nk + i] = carry.Ly;
+    }
+}
+
+}  // namespace
+}  // namespace g3d
+
+extern "C" {
+int g3d_march_adjoint(const g3d_march_adj_ctx* m, int lo, int hi) {
+    return g3d::guard([&] {
+        if (!m || !m->Rx || !m->Ry || !m->Ez || !m->Bz || !m->Lx || !m->Ly || !m->j0x || !m->j0y || !m->jkx ||
+            !m->jky || !m->jkz || !m->kx || !m->ky || !m->hcg || !m->wlo || !m->whi)
+            throw std::invalid_argument("march adjoint: null pointer in the context");
+        if (m->nx < 1 || m->ny < 1 || m->nk != long(m->nx) * m->ny)
+            throw std::invalid_argument("march adjoint: nk must be nx * ny");
+        if (lo < m->a || hi > m->N || lo >= hi)
+            throw std::invalid_argument("march adjoint: need a <= lo < hi <= N");
+        g3d::AdjView v;
+        v.Rx = m->Rx; v.Ry = m->Ry; v.Ez = m->Ez; v.Bz = m->Bz; v.Lx = m->Lx; v.Ly = m->Ly; v.up = m->up;
+        v.j0x = m->j0x; v.j0y = m->j0y; v.jkx = m->jkx; v.jky = m->jky; v.jkz = m->jkz;
+        v.kx = m->kx; v.ky = m->ky; v.hcg = m->hcg; v.wlo = m->wlo; v.whi = m->whi;
+        v.D = m->D; v.a = m->a; v.N = m->N; v.nx = m->nx; v.nk = m->nk;
+        hipLaunchKernelGGL(g3d::k_march_adj, dim3(g3d::blocks_for(v.nk, 256)), dim3(256), 0, g3d::stream(), v, lo,
+                           hi);
+        g3d::launch_check("march adjoint");
+    });
+}
+}

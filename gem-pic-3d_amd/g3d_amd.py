@@ -247,6 +247,7 @@ def _load():
         "g3d_fft_plan": ([C.c_int, C.c_int, C.c_int, C.POINTER(P)], C.c_int),
         "g3d_fft_exec": ([P, P, C.c_int, C.c_int], C.c_int), "g3d_fft_destroy": ([P], C.c_int),
         "g3d_march": ([P, C.c_int, C.c_int], C.c_int), "g3d_march_slice": ([P, C.c_int, C.c_int], C.c_int),
+        "g3d_march_adjoint": ([P, C.c_int, C.c_int], C.c_int),
         "g3d_sim_create": ([P, C.POINTER(P)], C.c_int), "g3d_sim_destroy": ([P], C.c_int),
         "g3d_sim_info_get": ([P, P], C.c_int), "g3d_sim_init": ([P], C.c_int), "g3d_sim_step": ([P], C.c_int),
         "g3d_sim_call": ([P, C.c_char_p, C.POINTER(C.c_double), C.c_int], C.c_int),
@@ -282,7 +283,8 @@ def check(rc):
         raise RuntimeError(lib().g3d_last_error().decode() or lib().g3d_status_string(rc).decode())
 
 
-ARRAY_DTYPES = {"F": np.float32, "jw": np.float64, "jsr": np.float64, "rho_lines": np.float64,
+ARRAY_DTYPES = {"F": np.float32, "jw": np.float64, "jsr": np.float64, "rho_lines": np.float64, "kx": np.float64,
+                "ky": np.float64, "hcg": np.float64, "wlo": np.float64, "whi": np.float64,
                 "parts": PARTICLE, "strm": PARTICLE}
 
 

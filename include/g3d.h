@@ -137,6 +137,24 @@ int g3d_march(const g3d_march_ctx* c, int lo, int hi);
  * (the two-pass backup of src/sim_plasma.cpp:306-326 without the extra copy kernels). */
 int g3d_march_slice(const g3d_march_ctx* c, int n, int restore);
 
+/* ------------------------------------------------------------------ adjoint of the march (no counterpart in the
+ * reference: the reverse mode of g3d_march for gradients through the march, Fourier y).  The march of slices
+ * lo ... hi-1 is linear in the level-s0 slices (R_x, R_y, E_z, B_z), slice hi at level s1 and the currents; this
+ * applies its transpose (for the real inner product Re sum conj(u) v) to the adjoints of its outputs. */
+typedef struct {
+    g3d_cpx *Rx, *Ry, *Ez, *Bz, *Lx, *Ly;         /* in: adjoints of the outputs of slices lo..hi-1 (ns planes,
+                                                     plane n - a); out: adjoints of the level-s0 inputs R_x, R_y,
+                                                     E_z, B_z of those slices (L_x, L_y: zero, not inputs) */
+    g3d_cpx *up;                                  /* out: 6 planes R_x R_y E_z B_z L_x L_y, the adjoint of slice hi
+                                                     (hi < N); may be null */
+    g3d_cpx *j0x, *j0y, *jkx, *jky, *jkz;         /* out, added to: adjoints of the currents (ns + 1 planes) */
+    const double *kx, *ky, *hcg, *wlo, *whi;      /* as g3d_march_ctx */
+    double D;
+    int a, N, nx, ny;
+    long nk;
+} g3d_march_adj_ctx;
+int g3d_march_adjoint(const g3d_march_adj_ctx* c, int lo, int hi);
+
 /* ------------------------------------------------------------------ engine: the Simulation of the march
  * g3d_sim mirrors the reference's Simulation (src/sim.hpp:222-360) on one GPU: init() and step() and the
  * member functions of the five drop-in files, each over the kernels of this library. */

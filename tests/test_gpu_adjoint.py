@@ -1,0 +1,110 @@
+"""The adjoint of the march (g3d_march_adjoint) behind torch.autograd (g3d_autograd.MarchFunction).
+
+- the dot-product identity <M x, y> = <x, M^T y> for random x and y (FP64, to rounding);
+- the gradient of a scalar loss against central finite differences of the CPU oracle's march (the restatement
+  pinned bit for bit to the reference's build), within 1e-6 relative, in random directions;
+- the tangent-linear run: the march is linear in its inputs, so M(x + dx) - M(x) = M(dx) to rounding."""
+import numpy as np
+import pytest
+import torch
+
+import g3d_amd
+import g3d_autograd as ga
+import golden_inputs as gi
+import oraclepy as orc
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not orc.available(), reason="oracle not built")]
+FIELDS = ("Rx", "Ry", "Ez", "Bz")
+CURRENTS = ("j0x", "j0y", "jkx", "jky", "jkz")
+
+
+def setup(seed=5):
+    deck = gi.small_deck()
+    sim = g3d_amd.Sim(deck)
+    geom = ga.MarchGeometry(sim, gi.MARCH_D)
+    r = np.random.default_rng(seed)
+    ns, nk = geom.ns, geom.nk
+
+    def rnd(rows):
+        return r.standard_normal((rows, nk)) + 1j * r.standard_normal((rows, nk))
+    x = {n: rnd(ns) for n in FIELDS}
+    x.update({n: rnd(ns + 1) for n in CURRENTS})
+    return deck, geom, x, rnd
+
+
+def gpu(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+def run(geom, x):
+    out = ga.march(geom, [gpu(x[n]) for n in FIELDS], [gpu(x[n]) for n in CURRENTS])
+    return [o.cpu().numpy() for o in out]
+
+
+def inner(u, v):
+    return sum(float(np.real(np.vdot(a, b))) for a, b in zip(u, v))
+
+
+def test_march_adjoint_dot_product():
+    deck, geom, x, rnd = setup(5)
+    y = [rnd(geom.ns) for _ in range(6)]
+    mx = run(geom, x)
+    xs = [gpu(x[n]).requires_grad_() for n in FIELDS + CURRENTS]
+    out = ga.march(geom, xs[:4], xs[4:])
+    loss = sum(torch.sum(torch.real(torch.conj(gpu(w)) * o)) for w, o in zip(y, out))
+    loss.backward()
+    lhs = inner(mx, y)
+    rhs = inner([x[n] for n in FIELDS + CURRENTS], [t.grad.cpu().numpy() for t in xs])
+    print("dot product", lhs, rhs)
+    assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), 1.0)
+    assert abs(lhs - float(loss.item())) <= 1e-12 * max(abs(lhs), 1.0)
+
+
+def test_march_gradient_against_oracle_fd():
+    """loss = Re sum conj(w) M(x) over the six output planes; the adjoint gradient against central differences
+    (step 1e-6 relative to the input scale) of the oracle's march, in four random directions"""
+    deck, geom, x, rnd = setup(7)
+    w = [rnd(geom.ns) for _ in range(6)]
+    g = orc.Grid(deck)
+    names = ("Rx", "Ry", "Ez", "Bz", "Lx", "Ly")
+
+    def loss_oracle(z):
+        arr = dict(z)
+        arr["Lx"] = np.zeros_like(z["Rx"])
+        arr["Ly"] = np.zeros_like(z["Rx"])
+        out = orc.march(g, {k: v.ravel() for k, v in arr.items()}, gi.MARCH_D)
+        return sum(float(np.real(np.vdot(wi, out[n].reshape(wi.shape)))) for wi, n in zip(w, names))
+
+    xs = [gpu(x[n]).requires_grad_() for n in FIELDS + CURRENTS]
+    out = ga.march(geom, xs[:4], xs[4:])
+    loss = sum(torch.sum(torch.real(torch.conj(gpu(wi)) * o)) for wi, o in zip(w, out))
+    loss.backward()
+    grad = {n: t.grad.cpu().numpy() for n, t in zip(FIELDS + CURRENTS, xs)}
+    r = np.random.default_rng(11)
+    eps = 1e-6
+    for _ in range(4):
+        v = {n: r.standard_normal(a.shape) + 1j * r.standard_normal(a.shape) for n, a in x.items()}
+        fd = (loss_oracle({n: x[n] + eps * v[n] for n in x}) - loss_oracle({n: x[n] - eps * v[n] for n in x})) / (2 * eps)
+        ad = sum(float(np.real(np.vdot(grad[n], v[n]))) for n in x)
+        print("directional derivative: adjoint %.12e  finite difference %.12e" % (ad, fd))
+        assert abs(ad - fd) <= 1e-6 * abs(fd)
+
+
+def test_march_tangent_linear():
+    deck, geom, x, rnd = setup(9)
+    dx = {n: 1e-3 * (rnd(a.shape[0])) for n, a in x.items()}
+    a = run(geom, x)
+    b = run(geom, {n: x[n] + dx[n] for n in x})
+    t = run(geom, dx)
+    for p, q, s in zip(a, b, t):
+        assert np.linalg.norm((q - p) - s) <= 1e-9 * np.linalg.norm(s)
+
+
+def test_march_adjoint_front_slice():
+    """E_z of the front slice is zero whatever the inputs (L = E_z = B_z = 0 there): its gradient is zero"""
+    deck, geom, x, rnd = setup(3)
+    ns, nk = geom.ns, geom.nk
+    xs = [gpu(x[n]).requires_grad_() for n in FIELDS + CURRENTS]
+    out = ga.march(geom, xs[:4], xs[4:])
+    torch.sum(torch.real(out[2][ns - 1])).backward()
+    assert all(float(torch.abs(t.grad).max()) == 0.0 for t in xs)
