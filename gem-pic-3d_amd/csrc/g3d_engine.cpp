@@ -52,10 +52,7 @@ Config Config::from_abi(const g3d_config& g) {
 }
 
 // n = n0 f(z) g(r) (src/sim_config.cpp:49-67; the deck's density_function is not part of the march)
-double Config::plasma_density(double x, double y, double z) const {
-    if (!plasma_on) return 0;
-    const double r2 = x * x + y * y;
-    if (r2 >= r_max * r_max) return 0;
+double Config::plasma_zfactor(double z) const {
     double f = 1;
     if (!pz_points.empty()) {
         if (z <= pz_points.front()) f = pz_values.front();
@@ -67,6 +64,13 @@ double Config::plasma_density(double x, double y, double z) const {
             f = (1 - t) * pz_values[k - 1] + t * pz_values[k];
         }
     }
+    return f;
+}
+double Config::plasma_density(double x, double y, double z) const {
+    if (!plasma_on) return 0;
+    const double r2 = x * x + y * y;
+    if (r2 >= r_max * r_max) return 0;
+    const double f = plasma_zfactor(z);
     const double g = parabolic ? 1 + channel_depth * r2 / (r_channel * r_channel) : 1.0;
     return f * g > 0 ? n0 * f * g : 0.0;
 }
@@ -722,6 +726,7 @@ void Sim::ensure_strm(size_t need) {
 }
 
 void Sim::load_plasma(double s0, double s1) {
+    if (load_plasma_device(s0, s1)) return;
     // the list in the reference's order (src/sim_plasma.cpp:43-90): lattice j, i, b2, a2 with the ions of every
     // loaded species after each electron, ids in that order; then regrouped tile by tile (stable) for the deposit
     std::vector<g3d_particle> v;
@@ -789,6 +794,57 @@ void Sim::load_plasma(double s0, double s1) {
     aos.from_host(v.data(), 0, v.size());
     parts_from_aos(aos.data(), strmA.view(), nstrm);
     parts_copy(strmA.view(), strmB.view(&strmA), nstrm);
+}
+
+// The common case on the device: Fourier y, tiled, electrons only and every lattice point loaded (the density is
+// positive on the whole box: no radial cut inside it, no negative channel, a positive longitudinal factor).  The
+// list is then the host loader's, regrouped by tile, with the same values and ids.
+bool Sim::load_plasma_device(double s0, double s1) {
+    const char* e = getenv("G3D_HOST_LATTICE");                 // 1: always the host loader (for comparisons)
+    if ((e && atoi(e) > 0) || !yf || tiles.ntile <= 0 || !cfg.free_electrons || !cfg.plasma_on || cfg.n0 <= 0) return false;
+    for (const auto& is : cfg.ions) if (cfg.loaded(is)) return false;
+    const double zmid = 0.5 * (s0 + s1);
+    const double f = cfg.plasma_zfactor(zmid);
+    if (!(f > 0) || (cfg.parabolic && !(cfg.channel_depth >= 0))) return false;
+    const double xm = std::max(std::fabs(X0), std::fabs(X0 + cfg.Lx)), ym = std::max(std::fabs(Y0), std::fabs(Y0 + cfg.Ly));
+    if (!(xm * xm + ym * ym < 0.99 * cfg.r_max * cfg.r_max)) return false;      // no point beyond the radial cut
+    const int nx = cfg.nx, ny = cfg.ny, px = cfg.ppc_x, py = cfg.ppc_y;
+    if (px < 1 || py < 1) return false;
+    LatticeArgs L;
+    L.nx = nx; L.ny = ny; L.px = px; L.py = py; L.TX = tiles.TX; L.TY = tiles.TY; L.ntx = tiles.ntx;
+    L.parabolic = cfg.parabolic;
+    L.per_tile = long(tiles.TX) * tiles.TY * px * py;
+    L.dx = dx; L.dy = dy; L.X0 = X0; L.Y0 = Y0; L.zf = cfg.zeta[N - 1]; L.s0 = s0; L.Delta = s1 - s0;
+    L.n0 = cfg.n0; L.f = f; L.channel_depth = cfg.channel_depth; L.r_channel = cfg.r_channel;
+    const unsigned long long id0 = (unsigned long long)(istep + 1) << 32;
+    id0_ = id0;
+    L.id0 = id0;
+    const long fresh = L.per_tile * tiles.ntile;
+    const long total = fresh + long(joining_.size());
+    fresh_last = fresh;
+    std::vector<long> off(tiles.ntile + 1);
+    for (long t = 0; t <= tiles.ntile; ++t) off[t] = t * L.per_tile;
+    tile_off.from_host(off.data(), 0, off.size());
+    n_tiled = fresh;
+    tiles.n_tiled = n_tiled;
+    nstrm = total;
+    born_cap = 0;
+    ensure_strm(size_t(total));
+    plasma_lattice(L, strmA.view(), strmB.view(&strmA), fresh);
+    if (!joining_.empty()) {      // trapped particles too slow to join the beam stream again, waiting (alive 6)
+        std::vector<g3d_particle> v(joining_);
+        for (auto& p : v) p.alive = 6;
+        if (aos.size() < v.size()) aos.resize(v.size());
+        aos.from_host(v.data(), 0, v.size());
+        auto at = [&](Parts p) {
+            p.x += fresh; p.y += fresh; p.zeta += fresh; p.px += fresh; p.py += fresh; p.pz += fresh; p.w += fresh;
+            p.z += fresh; p.q += fresh; p.qm += fresh; p.rem += fresh; p.alive += fresh; p.sp += fresh; p.id += fresh;
+            return p;
+        };
+        parts_from_aos(aos.data(), at(strmA.view()), long(v.size()));
+        parts_copy(at(strmA.view()), at(strmB.view(&strmA)), long(v.size()));
+    }
+    return true;
 }
 
 void Sim::stream_spectra() {

@@ -46,6 +46,7 @@ struct Config {
 
     static Config from_abi(const g3d_config& g);
     double plasma_density(double x, double y, double z) const;
+    double plasma_zfactor(double z) const;    // the longitudinal profile factor of plasma_density
     bool loaded(const g3d_ion& s) const { return s.mobile || s.ionization; }
 };
 
@@ -162,6 +163,7 @@ private:
     void setup_y();
     void setup_tiles();
     void ensure_strm(size_t need);
+    bool load_plasma_device(double s0, double s1);
 };
 
 extern const char* const kFamilyNames[Sim::kFam];

@@ -103,6 +103,18 @@ void count_alive(Parts p, long n, int code, int* count);
 void parts_from_aos(const g3d_particle* in, Parts p, long n);
 void parts_to_aos(Parts p, g3d_particle* out, long n);
 void parts_copy(Parts src, Parts dst, long n);
+// The fresh plasma of a step on the device (src/sim_plasma.cpp:43-90 for electrons only, Fourier y, every lattice
+// point loaded): lattice points in tile order (tiles of TX x TY cells, per_tile = TX TY px py points each, cells
+// j-major inside a tile, then b2, a2), each with the host loader's values and its id in the reference's order,
+// id0 + ((j nx + i) py + b2) px + a2.  The density is n0 f g with the longitudinal factor f of the step and the
+// channel factor g.
+struct LatticeArgs {
+    int nx, ny, px, py, TX, TY, ntx, parabolic;
+    long per_tile;
+    double dx, dy, X0, Y0, zf, s0, Delta, n0, f, channel_depth, r_channel;
+    unsigned long long id0;
+};
+void plasma_lattice(const LatticeArgs& L, Parts A, Parts B, long n);
 void parts_shift(Parts p, long n, double sx, double sy);
 
 // ---------------------------------------------------------------- spectra and fields (Fourier y)

@@ -611,6 +611,37 @@ void parts_to_aos(Parts p, g3d_particle* out, long n) {
     hipLaunchKernelGGL(k_to_aos, dim3(blocks_for(n, kT)), dim3(kT), 0, stream(), p, out, n);
     launch_check("particles to AoS");
 }
+// the fresh plasma lattice of a step, generated in tile order (see LatticeArgs); both list buffers
+__global__ void k_lattice(LatticeArgs L, Parts A, Parts B, long n) {
+    const long g = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const int pp = L.px * L.py;
+    const long t = g / L.per_tile, r = g - t * L.per_tile;
+    const long c = r / pp;
+    const int sub = int(r - c * pp);
+    const int jl = int(c / L.TX), il = int(c - long(jl) * L.TX);
+    const int b2 = sub / L.px, a2 = sub - b2 * L.px;
+    const int tx = int(t % L.ntx), ty = int(t / L.ntx);
+    const int j = ty * L.TY + jl, i = tx * L.TX + il;
+    // the host loader's formulas and operation order (src/sim_plasma.cpp:62-64)
+    const double y0 = j * L.dy, hy = L.dy;
+    const double x = (i + (a2 + 0.5) / L.px) * L.dx;
+    const double y = y0 + (b2 + 0.5) / L.py * hy;
+    const double X = x + L.X0, Y = y + L.Y0, r2 = X * X + Y * Y;
+    const double gch = L.parabolic ? 1 + L.channel_depth * r2 / (L.r_channel * L.r_channel) : 1.0;
+    const double dens = L.f * gch > 0 ? L.n0 * L.f * gch : 0.0;
+    const double w = dens * L.dx * hy * L.Delta / (L.px * L.py);
+    A.x[g] = x; A.y[g] = y; A.zeta[g] = L.zf; A.px[g] = 0; A.py[g] = 0; A.pz[g] = 0; A.w[g] = w; A.rem[g] = 0;
+    A.z[g] = L.s0; A.q[g] = -1; A.qm[g] = -1; A.alive[g] = 1; A.sp[g] = SP_PLASMA;
+    A.id[g] = L.id0 + (((unsigned long long)j * L.nx + i) * L.py + b2) * L.px + a2;
+    B.x[g] = x; B.y[g] = y; B.zeta[g] = L.zf; B.px[g] = 0; B.py[g] = 0; B.pz[g] = 0; B.z[g] = L.s0; B.q[g] = -1;
+    B.qm[g] = -1; B.alive[g] = 1;
+}
+void plasma_lattice(const LatticeArgs& L, Parts A, Parts B, long n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_lattice, dim3(blocks_for(n, kT)), dim3(kT), 0, stream(), L, A, B, n);
+    launch_check("plasma lattice");
+}
 void parts_copy(Parts s, Parts d, long n) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_copy, dim3(blocks_for(n, kT)), dim3(kT), 0, stream(), s, d, n);

@@ -154,3 +154,17 @@ def test_plasma_strong_laser_tiles():
     np.testing.assert_array_equal(a["alive"], b["alive"])
     dx = np.abs(((a["x"] - b["x"]) + 30) % 60 - 30).max()      # periodic box of 60/k0
     assert dx < 1e-9
+
+
+def test_device_lattice_matches_host(monkeypatch):
+    """the fresh plasma generated on the device (tiled Fourier-y electron plasma) is the host loader's list, value
+    for value and in the same order, ids included"""
+    a, b = g3d_amd.Sim(plasma_deck()), g3d_amd.Sim(plasma_deck())
+    monkeypatch.setenv("G3D_HOST_LATTICE", "1")
+    b.call("load_plasma", 0.0, 5.0)
+    monkeypatch.delenv("G3D_HOST_LATTICE")
+    a.call("load_plasma", 0.0, 5.0)
+    pa, pb = a.array("strm"), b.array("strm")
+    assert len(pa) == len(pb) == 32 * 32 * 4
+    for f in pa.dtype.names:
+        np.testing.assert_array_equal(pa[f], pb[f], err_msg=f)
