@@ -413,7 +413,7 @@ MarchView Sim::mview(double D) {
     v.Rx = Rx.data(); v.Ry = Ry.data(); v.Ez = Ez.data(); v.Bz = Bz.data(); v.Lx = Lx.data(); v.Ly = Ly.data();
     v.j0x = j0x.data(); v.j0y = j0y.data(); v.jkx = jkx.data(); v.jky = jky.data(); v.jkz = jkz.data();
     v.kx = kxd.data(); v.ky = kyd.data(); v.hcg = hcg.data(); v.wlo = wlo.data(); v.whi = whi.data();
-    v.D = D; v.a = a; v.N = N; v.nx = cfg.nx; v.nk = nk;
+    v.D = D; v.a = a; v.N = N; v.nx = cfg.nx; v.nk = nk; v.ny = cfg.ny;
     return v;
 }
 LineView Sim::lview(double D) {
@@ -847,6 +847,25 @@ bool Sim::load_plasma_device(double s0, double s1) {
     return true;
 }
 
+void Sim::spectra_cm(bool to_cm) {
+    const int rows = to_cm ? cfg.ny : cfg.nx, cols = to_cm ? cfg.nx : cfg.ny;
+    auto flip = [&](DevBuf<cx>& A, DevBuf<cx>& scratch) {
+        if (scratch.size() != A.size()) scratch.resize(A.size());
+        transpose_planes(A.data(), scratch.data(), long(A.size()) / nk, rows, cols);
+        A.swap(scratch);
+    };
+    if (to_cm) {
+        for (DevBuf<cx>* A : {&Rx, &Ry, &Ez, &Bz}) flip(*A, trF);     // L: every slice is written before it is read
+        for (DevBuf<cx>* A : {&jkx, &jky, &jkz, &js0x, &js0y, &js0z}) flip(*A, trC);
+        if (j0xT.size() != j0x.size()) { j0xT.resize(j0x.size()); j0yT.resize(j0y.size()); }
+        transpose_planes(j0x.data(), j0xT.data(), long(j0x.size()) / nk, rows, cols);
+        transpose_planes(j0y.data(), j0yT.data(), long(j0y.size()) / nk, rows, cols);
+    } else {
+        for (DevBuf<cx>* A : {&Rx, &Ry, &Ez, &Bz, &Lx, &Ly}) flip(*A, trF);
+        for (DevBuf<cx>* A : {&jkx, &jky, &jkz, &js0x, &js0y, &js0z}) flip(*A, trC);
+    }
+}
+
 void Sim::stream_spectra() {
     // stand-alone form (jsr is consumed): the same row and column kernels as the march, spectra only
     pass_rows_fwd(jsr.data(), TileGeom{}, fft, sp3.data(), nullptr, nullptr, 0, 0);
@@ -922,8 +941,17 @@ void Sim::march_plasma(double s0, double s1) {
     const SpecGeom g = geom();
     jpred.zero();
     jsr.zero();
+    // the FP32-mixed pass: Fourier y, and no ion species in the list (ions and ionization stay FP64).  Its modes
+    // run column-major (one fused column launch per pass) unless G3D_PASS_CM=0
+    bool ions = false;
+    for (const auto& is : cfg.ions) ions = ions || cfg.loaded(is);
+    const bool mx = cfg.mixed && yf && !ions;
+    const char* ecm = getenv("G3D_PASS_CM");
+    const bool cm = mx && !(ecm && atoi(ecm) == 0) && cfg.ny >= 2 && cfg.ny <= 1024;
+    if (cm) spectra_cm(true);
     ModeArgs m{};
     m.mv = mview(D);
+    if (cm) { m.mv.cm = 1; m.mv.j0x = j0xT.data(); m.mv.j0y = j0yT.data(); }
     m.jsn = jsn.data(); m.sin_ = jsin.data(); m.si0 = jsi0.data(); m.pin = jpred.data(); m.dl = dlt.data();
     m.js0x = js0x.data(); m.js0y = js0y.data(); m.js0z = js0z.data();
     m.jkx = jkx.data(); m.jky = jky.data(); m.jkz = jkz.data();
@@ -944,10 +972,6 @@ void Sim::march_plasma(double s0, double s1) {
         pass_rows_inv(tp3.data(), F.data() + long(n - a) * nk, fpl, fft);
         toc(F_IFFT);
     };
-    // the FP32-mixed pass: Fourier y, and no ion species in the list (ions and ionization stay FP64)
-    bool ions = false;
-    for (const auto& is : cfg.ions) ions = ions || cfg.loaded(is);
-    const bool mx = cfg.mixed && yf && !ions;
     StreamArgs sc = stream_args(s0, s1);
     const bool ionz = born_cap > 0;
     // tiles only when this step loaded tiled plasma (otherwise the slabs would hold an earlier step's current)
@@ -1038,6 +1062,7 @@ void Sim::march_plasma(double s0, double s1) {
         }
         have_dl = true;
     }
+    if (cm) spectra_cm(false);
     (void)born_base;
     nstrm = nlist;
     // trapped particles (alive 5): fast ones join the beam list, slow ones stream again in the next step

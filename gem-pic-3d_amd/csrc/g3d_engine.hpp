@@ -164,6 +164,10 @@ private:
     void setup_tiles();
     void ensure_strm(size_t need);
     bool load_plasma_device(double s0, double s1);
+    // the spectra the plasma pass reads and writes, between the reference's layout (mode j nx + ix) and the
+    // column-major layout of the fused column kernel (ix ny + j); j0 goes to j0xT, j0yT (read only in the march)
+    void spectra_cm(bool to_cm);
+    DevBuf<cx> trF, trC, j0xT, j0yT;
 };
 
 extern const char* const kFamilyNames[Sim::kFam];

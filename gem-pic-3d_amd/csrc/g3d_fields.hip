@@ -297,6 +297,27 @@ __global__ void k_sub(cx* a, const cx* b, long n) {
     const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) a[i] = a[i] - b[i];
 }
+// dst = src transposed, plane by plane: dst[p][c][r] = src[p][r][c] (rows x cols complex per plane), through LDS
+// tiles of 32 x 32 so that both the reads and the writes run along rows of 32 elements
+__global__ void __launch_bounds__(256) k_transpose(const cx* src, cx* dst, int rows, int cols, int ntr, int ntc) {
+    __shared__ cx tile[32][33];
+    const long per = long(ntr) * ntc;
+    const long p = blockIdx.x / per;
+    const int t = int(blockIdx.x - p * per), tr = t / ntc, tc = t - tr * ntc;
+    const long off = p * long(rows) * cols;
+    const int x = threadIdx.x & 31, y = threadIdx.x >> 5;       // 32 x 8 lanes
+#pragma unroll
+    for (int k = 0; k < 32; k += 8) {
+        const int r = tr * 32 + y + k, c = tc * 32 + x;
+        if (r < rows && c < cols) tile[y + k][x] = src[off + long(r) * cols + c];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 32; k += 8) {
+        const int c = tc * 32 + y + k, r = tr * 32 + x;
+        if (r < rows && c < cols) dst[off + long(c) * rows + r] = tile[x][y + k];
+    }
+}
 __global__ void k_copyc(cx* a, const cx* b, long n) {
     const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) a[i] = b[i];
@@ -373,6 +394,14 @@ void comoving_R(const SpecGeom& g, const cx* psi, double beta, cx* Rx, cx* Ry, c
 }
 void sub_planes(cx* a, const cx* b, long n) { G3D_LAUNCH(k_sub, n, a, b, n); }
 void copy_planes(cx* dst, const cx* src, long n) { G3D_LAUNCH(k_copyc, n, dst, src, n); }
+void transpose_planes(const cx* src, cx* dst, long planes, int rows, int cols) {
+    if (planes <= 0) return;
+    const int ntr = (rows + 31) / 32, ntc = (cols + 31) / 32;
+    const long nb = planes * ntr * ntc;
+    if (nb > 0x7fffffffL) throw std::runtime_error("transpose: too many tiles");
+    hipLaunchKernelGGL(k_transpose, dim3(unsigned(nb)), dim3(256), 0, stream(), src, dst, rows, cols, ntr, ntc);
+    launch_check("transpose");
+}
 void zero_planes(cx* a, long n) { G3D_LAUNCH(k_zeroc, n, a, n); }
 void axpy_planes(cx* y, const cx* x, double s, long n) { G3D_LAUNCH(k_axpy, n, y, x, s, n); }
 

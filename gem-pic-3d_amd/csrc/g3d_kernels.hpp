@@ -175,6 +175,8 @@ void pass_modes(const ModeArgs& m, const FftPlan& p, cx* X, cx* T, cx* jsn, int 
 // jz -= js0z over count planes
 void sub_planes(cx* a, const cx* b, long n);
 void copy_planes(cx* dst, const cx* src, long n);
+// dst[p][c][r] = src[p][r][c] for `planes` planes of rows x cols (out of place)
+void transpose_planes(const cx* src, cx* dst, long planes, int rows, int cols);
 void zero_planes(cx* a, long n);
 
 // grid laser (src/sim_fields.cpp:321-374, Fourier y): R_x + i R_y = 2 E_perp of the paraxial Gaussian of every

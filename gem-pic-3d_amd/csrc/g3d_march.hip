@@ -55,7 +55,7 @@ MarchView view_of(const g3d_march_ctx& m) {
     v.Rx = m.Rx; v.Ry = m.Ry; v.Ez = m.Ez; v.Bz = m.Bz; v.Lx = m.Lx; v.Ly = m.Ly;
     v.j0x = m.j0x; v.j0y = m.j0y; v.jkx = m.jkx; v.jky = m.jky; v.jkz = m.jkz;
     v.kx = m.kx; v.ky = m.ky; v.hcg = m.hcg; v.wlo = m.wlo; v.whi = m.whi;
-    v.D = m.D; v.a = m.a; v.N = m.N; v.nx = m.nx; v.nk = m.nk;
+    v.D = m.D; v.a = m.a; v.N = m.N; v.nx = m.nx; v.nk = m.nk; v.ny = m.ny;
     return v;
 }
 void require_one_gpu(const g3d_march_ctx& m) {

@@ -12,7 +12,15 @@ struct MarchView {
     double D;
     int a, N, nx;
     long nk;
+    int ny = 0;
+    int cm = 0;          // 1: modes column-major (i = ix ny + j, the plasma pass's layout), else i = j nx + ix
 };
+
+// (ix, j) of mode i
+G3D_HD void mode_xy(const MarchView& c, long i, int& ixx, int& j) {
+    if (c.cm) { ixx = int(i / c.ny); j = int(i - long(ixx) * c.ny); }
+    else { ixx = int(i % c.nx); j = int(i / c.nx); }
+}
 
 // values of one mode at one slice: R_x R_y E_z B_z L_x L_y
 struct Mode6 { cx Rx, Ry, Ez, Bz, Lx, Ly; };
@@ -28,7 +36,8 @@ G3D_HD Cur9 load_cur(const MarchView& c, long i, int n) {
 // `up` the six values of slice n+1 at level s1 (unused at the front slice n = N-1), J the currents.  Returns
 // slice n at s1.  The operand order is that of the reference so that the FP64 results round identically.
 G3D_HD Mode6 march_mode(const MarchView& c, long i, int n, const Mode6& old, const Mode6& up, const Cur9& J) {
-    const int ixx = int(i % c.nx), j = int(i / c.nx);
+    int ixx, j;
+    mode_xy(c, i, ixx, j);
     const double kxv = c.kx[ixx], kyv = c.ky[j];
     const cx a = ik(kxv), b = ik(kyv);
     const double k2 = kxv * kxv + kyv * kyv, D = c.D, wl = c.wlo[n], wu = c.whi[n];
@@ -68,7 +77,8 @@ G3D_HD Mode6 march_mode(const MarchView& c, long i, int n, const Mode6& old, con
 // The same update in the FP32-mixed form: the explicit part R_s = R + D (F - J) and the trapezoidal update of R
 // stay FP64 (the s-integration of R); the implicit solve for E_z, B_z and the L update run in FP32.
 G3D_HD Mode6 march_mode_mixed(const MarchView& c, long i, int n, const Mode6& old, const Mode6& up, const Cur9& J) {
-    const int ixx = int(i % c.nx), j = int(i / c.nx);
+    int ixx, j;
+    mode_xy(c, i, ixx, j);
     const double kxv = c.kx[ixx], kyv = c.ky[j];
     const cx a = ik(kxv), b = ik(kyv);
     const double D = c.D, wl = c.wlo[n], wu = c.whi[n];

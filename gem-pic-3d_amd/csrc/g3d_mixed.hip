@@ -188,6 +188,8 @@ __device__ __forceinline__ MState load_mstate(const Parts& S, long i) {
 
 // one electron through half-cell n (src/sim_stream.hpp:104-171), the physics of the FP64 pass in FP32; s: its state
 // at the start (loaded by the caller), written to D[i]
+// LASER: the deck has the analytic laser (its evaluation would otherwise set the register budget of every launch)
+template <bool LASER>
 __device__ __forceinline__ void stream_one_mixed(const StreamArgs& c, const InvF& iv, float* J, const WinF* win,
                                                  const MState& s, Parts D, long i) {
     const int n = c.n;
@@ -216,7 +218,7 @@ __device__ __forceinline__ void stream_one_mixed(const StreamArgs& c, const InvF
     const double zm = z + double(hs * pz), zetam = zeta - 0.5 * hl;
     float E[3], B[3];
     gather_f(P, iv, win, wrap_mod(xm, P.Lx), wrap_mod(ym, P.Ly), zetam, E, B);
-    if (c.laser.on) {
+    if (LASER && c.laser.on) {
         double Ee[3], Be[3];
         c.laser.field(xm + c.X0, zm, zm - (zetam - c.zc), Ee, Be);
         for (int k = 0; k < 3; ++k) { E[k] += float(Ee[k]); B[k] += float(Be[k]); }
@@ -263,7 +265,7 @@ __device__ __forceinline__ void stream_one_mixed(const StreamArgs& c, const InvF
 }
 
 // WPE: waves per SIMD the register allocation must allow (0: the compiler's choice)
-template <int WPE>
+template <int WPE, bool LASER>
 __global__ void __launch_bounds__(kStreamBlockMax) __attribute__((amdgpu_waves_per_eu(WPE ? WPE : 1)))
 k_stream_mixed(StreamArgs c, float* J, float* slab, Parts S, Parts D, long n, TileGeom tg) {
     extern __shared__ float4 winf4[];
@@ -304,7 +306,7 @@ k_stream_mixed(StreamArgs c, float* J, float* slab, Parts S, Parts D, long n, Ti
         WinF w{dep, F.self ? winf4 : nullptr, x0, y0, tg.WX, tg.WY, fl};
         const InvF iv{1.0 / F.dx, 1.0 / F.dy, float(c.ia[0])};
         while (k < hi) {
-            stream_one_mixed(c, iv, J, &w, st, D, i);
+            stream_one_mixed<LASER>(c, iv, J, &w, st, D, i);
             k += blockDim.x;
             if (k < hi) {
                 i = tg.idx ? long(tg.idx[k]) : k;
@@ -318,13 +320,15 @@ k_stream_mixed(StreamArgs c, float* J, float* slab, Parts S, Parts D, long n, Ti
     }
     const long i = tg.n_tiled + (b - tg.ntile) * blockDim.x + threadIdx.x;
     if (i < n)
-        stream_one_mixed(c, InvF{1.0 / c.fv.dx, 1.0 / c.fv.dy, float(c.ia[0])}, J, nullptr, load_mstate(S, i), D, i);
+        stream_one_mixed<LASER>(c, InvF{1.0 / c.fv.dx, 1.0 / c.fv.dy, float(c.ia[0])}, J, nullptr, load_mstate(S, i), D,
+                                i);
 }
 
 // ---------------------------------------------------------------- rows forward (FP32)
+// tr: X column-major (element (kx, j) at kx ny + j), for the fused column kernel
 __global__ void __launch_bounds__(kT) k_rows_fwd_mixed(float* J, const float* slab, TileGeom tg, int nx, int ny, int lx,
                                                        int rpw, const cx* twx, cxf* X, double* rl, double* rli, int i0,
-                                                       int j0) {
+                                                       int j0, int tr) {
     extern __shared__ cxf smf[];
     const int sp = padded(nx);
     cxf* buf = smf;
@@ -375,7 +379,7 @@ __global__ void __launch_bounds__(kT) k_rows_fwd_mixed(float* J, const float* sl
     lds_fft(buf, nx, lx, 2 * rpw, sp, tws);
     for (int e = threadIdx.x; e < tot; e += blockDim.x) {
         const int r = e / nx, kx = e - r * nx, j = row0 + r;
-        const long i = long(j) * nx + kx;
+        const long i = tr ? long(kx) * ny + j : long(j) * nx + kx;
         const cxf z = buf[(2 * r) * sp + lpos(kx)], zm = conjg(buf[(2 * r) * sp + lpos((nx - kx) & (nx - 1))]);
         const cxf d = z - zm;
         X[i] = 0.5f * (z + zm);
@@ -413,7 +417,8 @@ __global__ void __launch_bounds__(kT) k_modes_mixed(ModeArgs m, const cxf* X, cx
     const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
     const long nk = m.mv.nk;
     if (i >= nk) return;
-    const int kx = int(i % m.mv.nx), j = int(i / m.mv.nx);
+    int kx, j;
+    mode_xy(m.mv, i, kx, j);
     cx u[3] = {mk(0), mk(0), mk(0)};
     if (kind != 1) {
         const double k2 = m.kxs[kx] * m.kxs[kx] + m.kys[j] * m.kys[j];
@@ -427,7 +432,7 @@ __global__ void __launch_bounds__(kT) k_modes_mixed(ModeArgs m, const cxf* X, cx
 
 // ---------------------------------------------------------------- rows inverse (FP32) and F of the slice
 __global__ void __launch_bounds__(kT) k_rows_inv_mixed(const cxf* T, float* Fn, long fpl, int nx, int ny, int lx,
-                                                       int rpw, const cx* twx, float scale) {
+                                                       int rpw, const cx* twx, float scale, int tr) {
     extern __shared__ cxf smf[];
     const int sp = padded(nx);
     cxf* buf = smf;
@@ -438,7 +443,8 @@ __global__ void __launch_bounds__(kT) k_rows_inv_mixed(const cxf* T, float* Fn, 
     const int tot = rpw * nx;
     for (int e = threadIdx.x; e < 3 * tot; e += blockDim.x) {
         const int k = e / tot, rem = e - k * tot, r = rem / nx, ixx = rem - r * nx;
-        buf[(3 * r + k) * sp + lpos(bitrev(ixx, lx))] = T[k * nk + long(row0 + r) * nx + ixx];
+        const long i = tr ? long(ixx) * ny + row0 + r : long(row0 + r) * nx + ixx;
+        buf[(3 * r + k) * sp + lpos(bitrev(ixx, lx))] = T[k * nk + i];
     }
     __syncthreads();
     lds_fft(buf, nx, lx, 3 * rpw, sp, tws);
@@ -456,6 +462,48 @@ __global__ void __launch_bounds__(kT) k_rows_inv_mixed(const cxf* T, float* Fn, 
     }
 }
 
+// ---------------------------------------------------------------- columns + modes + inverse columns, fused
+// One workgroup per k_x column (ny lanes, one mode each), the modes column-major (i = kx ny + j): the column's three
+// planes of X in LDS, y-transforms, the per-mode work of every mode of the column, the inverse y-transforms of the
+// three packed planes, T written back.  The per-mode arrays of a column are contiguous in this layout.
+constexpr int kColMax = 1024;
+__global__ void __launch_bounds__(kColMax) k_colmodes_mixed(ModeArgs m, const cxf* X, cxf* T, int ny, int ly,
+                                                            const cx* twy, int kind) {
+    extern __shared__ cxf smf[];
+    const int sp = padded(ny);
+    cxf* buf = smf;
+    cxf* twf = smf + 3 * sp;
+    cxf* twi = twf + (ny >> 1);
+    load_twiddles(twf, twy, ny >> 1, false);
+    load_twiddles(twi, twy, ny >> 1, true);
+    const long nk = m.mv.nk;
+    const int kx = blockIdx.x;
+    const long c0 = long(kx) * ny;
+    const int jj = threadIdx.x;                      // this lane's mode (kx, jj)
+    if (kind != 1)
+        for (int e = jj; e < 3 * ny; e += blockDim.x) {
+            const int p = e / ny, j = e - p * ny;
+            buf[p * sp + lpos(bitrev(j, ly))] = X[p * nk + c0 + j];
+        }
+    __syncthreads();
+    if (kind != 1) lds_fft(buf, ny, ly, 3, sp, twf);
+    cx u[3] = {mk(0), mk(0), mk(0)};
+    const long i = c0 + jj;
+    if (kind != 1) {
+        const double k2 = m.kxs[kx] * m.kxs[kx] + m.kys[jj] * m.kys[jj];
+        const float f = float((kx == 0 && jj == 0 && !m.keep0) ? 0.0 : m.filt[kx] * m.filty[jj] / (1 + m.a2 * k2));
+        for (int k = 0; k < 3; ++k) u[k] = to_d(f * buf[k * sp + lpos(jj)]);
+    }
+    cx t[3];
+    mode_work<true>(m, i, u, t);
+    __syncthreads();                                 // every lane has read its X before T overwrites the buffer
+    const int b = lpos(bitrev(jj, ly));
+    for (int k = 0; k < 3; ++k) buf[k * sp + b] = to_f(t[k]);
+    __syncthreads();
+    lds_fft(buf, ny, ly, 3, sp, twi);
+    for (int k = 0; k < 3; ++k) T[k * nk + i] = buf[k * sp + lpos(jj)];
+}
+
 int lanes(int work) { int t = 64; while (t < work && t < kT) t <<= 1; return t; }
 }  // namespace
 
@@ -467,17 +515,31 @@ void stream_mixed(const StreamArgs& a, float* J, float* slab, Parts src, Parts d
     const long tail_blocks = nstrm > tg.n_tiled ? long(blocks_for(nstrm - tg.n_tiled, bs)) : 0;
     const size_t lds = tg.ntile ? size_t(tg.WX) * tg.WY * (kDepNode * sizeof(double) + kFwNode * sizeof(float)) : 0;
     const dim3 grid(unsigned(tg.ntile + tail_blocks)), blk(bs);
-    launch_timed_kernel(lt, tg.wpe == 4 ? k_stream_mixed<4> : k_stream_mixed<0>, grid, blk, unsigned(lds), a, J, slab,
+    auto* k = a.laser.on ? (tg.wpe == 4 ? k_stream_mixed<4, true> : k_stream_mixed<0, true>)
+                         : (tg.wpe == 4 ? k_stream_mixed<4, false> : k_stream_mixed<0, false>);
+    launch_timed_kernel(lt, k, grid, blk, unsigned(lds), a, J, slab,
                         src, dst, nstrm, tg);
     launch_check("stream half-cell (mixed)");
 }
 void pass_mixed(const ModeArgs& m, const FftPlan& p, float* J, const float* slab, const TileGeom& tg, cxf* X, cxf* T,
                 float* Fn, long fpl, double* rl, double* rli, int i0, int j0, int kind, int rpw) {
     const long nk = long(p.nx) * p.ny;
+    const int cm = m.mv.cm;
+    if (cm && (p.ny > kColMax || p.ny < 2)) throw std::runtime_error("plasma pass: column kernel needs 2 <= ny <= 1024");
     if (kind != 1) {
         const size_t lds = (size_t(2 * rpw) * (p.nx + p.nx / 8) + p.nx / 2) * sizeof(cxf);
         hipLaunchKernelGGL(k_rows_fwd_mixed, dim3(p.ny / rpw), dim3(lanes(rpw * p.nx)), lds, stream(), J, slab, tg, p.nx,
-                           p.ny, p.lx, rpw, p.twx, X, rl, rli, i0, j0);
+                           p.ny, p.lx, rpw, p.twx, X, rl, rli, i0, j0, cm);
+    }
+    const size_t li = (size_t(3 * rpw) * (p.nx + p.nx / 8) + p.nx / 2) * sizeof(cxf);
+    if (cm) {       // modes column-major: one fused launch between the row transforms
+        const size_t lm = (size_t(3) * (p.ny + p.ny / 8) + p.ny) * sizeof(cxf);
+        hipLaunchKernelGGL(k_colmodes_mixed, dim3(unsigned(p.nx)), dim3(unsigned(p.ny)), lm, stream(), m, X, T, p.ny,
+                           p.ly, p.twy, kind);
+        hipLaunchKernelGGL(k_rows_inv_mixed, dim3(p.ny / rpw), dim3(lanes(rpw * p.nx)), li, stream(), T, Fn, fpl, p.nx,
+                           p.ny, p.lx, rpw, p.twx, float(1.0 / (double(p.nx) * p.ny)), 1);
+        launch_check("plasma pass (mixed, column-major modes)");
+        return;
     }
     int nc = 8;                        // 64-byte row segments (measured best at 256 x 256 against 4 and 16)
     if (nc > p.nx) nc = p.nx;
@@ -491,9 +553,8 @@ void pass_mixed(const ModeArgs& m, const FftPlan& p, float* J, const float* slab
     hipLaunchKernelGGL(k_modes_mixed, dim3(blocks_for(nk, kT)), dim3(kT), 0, stream(), m, X, T, kind);
     hipLaunchKernelGGL(k_cols_mixed, dim3(unsigned(3 * (p.nx / nc))), dim3(tc), lc, stream(), T, p.nx, p.ny, p.ly, nc,
                        p.twy, 1);
-    const size_t li = (size_t(3 * rpw) * (p.nx + p.nx / 8) + p.nx / 2) * sizeof(cxf);
     hipLaunchKernelGGL(k_rows_inv_mixed, dim3(p.ny / rpw), dim3(lanes(rpw * p.nx)), li, stream(), T, Fn, fpl, p.nx, p.ny,
-                       p.lx, rpw, p.twx, float(1.0 / (double(p.nx) * p.ny)));
+                       p.lx, rpw, p.twx, float(1.0 / (double(p.nx) * p.ny)), 0);
     launch_check("plasma pass (mixed)");
 }
 

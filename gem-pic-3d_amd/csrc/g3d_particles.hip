@@ -252,8 +252,9 @@ __device__ __forceinline__ void ionize(const StreamArgs& c, PState& p, double E1
 }
 
 // one particle through half-cell n (src/sim_stream.hpp:104-171)
-// YF: the grid (1 Fourier y, 0 free-hand y); ION: the list may hold ions
-template <int YF, bool ION>
+// YF: the grid (1 Fourier y, 0 free-hand y); ION: the list may hold ions; LASER: the deck has the analytic laser
+// (without it the laser's field is zero and its evaluation would only cost registers)
+template <int YF, bool ION, bool LASER>
 __device__ __forceinline__ void stream_one(const StreamArgs& c, const Win* win, PState& p) {
     const int n = c.n;
     const double zn = c.zs[n], zn1 = c.zs[n + 1], h = zn1 - zn;
@@ -282,8 +283,10 @@ __device__ __forceinline__ void stream_one(const StreamArgs& c, const Win* win, 
     double E[3], B[3], Ee[3], Be[3];
     const double xg = wrap_mod(xm, P.Lx), yg = YF ? wrap_mod(ym, P.Ly) : ym;
     if (!(YF && win && win->fld && gather_window(P, *win, xg, yg, zetam, E, B))) gather_t<YF>(P, xg, yg, zetam, E, B);
-    c.laser.field(xm + c.X0, zm, zm - (zetam - c.zc), Ee, Be);
-    for (int k = 0; k < 3; ++k) { E[k] += Ee[k]; B[k] += Be[k]; }
+    if (LASER) {
+        c.laser.field(xm + c.X0, zm, zm - (zetam - c.zc), Ee, Be);
+        for (int k = 0; k < 3; ++k) { E[k] += Ee[k]; B[k] += Be[k]; }
+    }
     double ux = p.px, uy = p.py, uz = p.pz;
     boris(ux, uy, uz, E, B, p.qm, dt0);
     double g1 = sqrt(1 + ux * ux + uy * uy + uz * uz);
@@ -328,16 +331,16 @@ __device__ __forceinline__ PState load_pstate(const Parts& S, long i) {
 }
 
 // p: the state of list entry i (loaded by the caller); the pushed state goes to D[i]
-template <int YF, bool ION>
+template <int YF, bool ION, bool LASER>
 __device__ __forceinline__ void stream_index(const StreamArgs& c, const Win* win, PState p, Parts D, long i) {
-    stream_one<YF, ION>(c, win, p);
+    stream_one<YF, ION, LASER>(c, win, p);
     D.x[i] = p.x; D.y[i] = p.y; D.zeta[i] = p.zeta; D.px[i] = p.px; D.py[i] = p.py; D.pz[i] = p.pz;
     D.z[i] = p.z; D.q[i] = p.q; D.qm[i] = p.qm; D.alive[i] = p.alive;
 }
 
 // workgroups 0 .. ntile-1: the particles of one tile with the LDS window, written to the tile's slab at the end;
 // the following workgroups: the untiled particles from off[ntile] on, one per lane, global atomics
-template <int YF, bool ION>
+template <int YF, bool ION, bool LASER>
 __global__ void __launch_bounds__(kStreamBlockMax) k_stream_halfcell(StreamArgs c, Parts S, Parts D, long n,
                                                                      TileGeom tg) {
     extern __shared__ double win_lds[];
@@ -375,7 +378,7 @@ __global__ void __launch_bounds__(kStreamBlockMax) k_stream_halfcell(StreamArgs 
         __syncthreads();
         Win w{win_lds, fwin ? fw : nullptr, x0, y0, tg.WX, tg.WY, fl};
         while (k < hi) {
-            stream_index<YF, ION>(c, &w, p, D, i);
+            stream_index<YF, ION, LASER>(c, &w, p, D, i);
             k += blockDim.x;
             if (k < hi) {
                 i = tg.idx ? long(tg.idx[k]) : k;
@@ -388,7 +391,7 @@ __global__ void __launch_bounds__(kStreamBlockMax) k_stream_halfcell(StreamArgs 
         return;
     }
     const long i = tg.n_tiled + (b - tg.ntile) * blockDim.x + threadIdx.x;
-    if (i < n) stream_index<YF, ION>(c, nullptr, load_pstate(S, i), D, i);
+    if (i < n) stream_index<YF, ION, LASER>(c, nullptr, load_pstate(S, i), D, i);
 }
 
 // ---------------------------------------------------------------- binning of the streaming list by tile
@@ -559,8 +562,11 @@ void stream_halfcell(const StreamArgs& a, Parts src, Parts dst, long nstrm, cons
     const dim3 grid(unsigned(tg.ntile + tail_blocks)), blk(bs);
     const bool ion = a.ions != nullptr;
     const unsigned sh = unsigned(lds);
-    auto* k = a.fv.yf ? (ion ? k_stream_halfcell<1, true> : k_stream_halfcell<1, false>)
-                      : (ion ? k_stream_halfcell<0, true> : k_stream_halfcell<0, false>);
+    auto* k = a.laser.on
+        ? (a.fv.yf ? (ion ? k_stream_halfcell<1, true, true> : k_stream_halfcell<1, false, true>)
+                   : (ion ? k_stream_halfcell<0, true, true> : k_stream_halfcell<0, false, true>))
+        : (a.fv.yf ? (ion ? k_stream_halfcell<1, true, false> : k_stream_halfcell<1, false, false>)
+                   : (ion ? k_stream_halfcell<0, true, false> : k_stream_halfcell<0, false, false>));
     launch_timed_kernel(lt, k, grid, blk, sh, a, src, dst, nstrm, tg);
     launch_check("stream half-cell");
 }

@@ -159,7 +159,8 @@ __global__ void __launch_bounds__(kT) k_pass_modes(ModeArgs m, const cx* X, cx* 
     const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
     const long nk = m.mv.nk;
     if (i >= nk) return;
-    const int kx = int(i % m.mv.nx), j = int(i / m.mv.nx);
+    int kx, j;
+    mode_xy(m.mv, i, kx, j);
     cx u[3] = {mk(0), mk(0), mk(0)};
     if (kind != 1) {
         const double k2 = m.kxs[kx] * m.kxs[kx] + m.kys[j] * m.kys[j];
