@@ -116,6 +116,7 @@ int g3d_sim_get(g3d_sim* sim, const char* name, void* host, size_t bytes) {
     int rc = 0;
     const int g = g3d::guard([&] {
         g3d::Sim& s = *sim->s;
+        s.ensure_layout(false, false);      // arrays leave and enter in the reference's layout
         const std::string n(name);
         if (n == "parts" || n == "strm") {
             const std::vector<g3d_particle> v = n == "parts" ? s.host_particles() : s.host_strm();
@@ -136,6 +137,7 @@ int g3d_sim_set(g3d_sim* sim, const char* name, const void* host, size_t bytes) 
     int rc = 0;
     const int g = g3d::guard([&] {
         g3d::Sim& s = *sim->s;
+        s.ensure_layout(false, false);
         if (std::string(name) == "strm") {
             const g3d_particle* p = static_cast<const g3d_particle*>(host);
             s.set_strm(std::vector<g3d_particle>(p, p + bytes / sizeof(g3d_particle)));

@@ -547,6 +547,7 @@ void Sim::load_bunch() {
 
 // ---------------------------------------------------------------- time step (src/sim_setup.cpp:178-217)
 void Sim::init() {
+    ensure_layout(false, false);
     load_bunch();
     s = 0; istep = 0;
     if (cfg.self_fields) {
@@ -579,6 +580,7 @@ void Sim::step() {
 }
 
 void Sim::j0_from_jk() {
+    if (cmR != cmJ) ensure_layout(cmR, cmR);          // j0 takes jk's values in its own layout
     const long n = long(ns + 1) * nk;
     copy_planes(j0x.data(), jkx.data(), n);
     copy_planes(j0y.data(), jky.data(), n);
@@ -602,6 +604,7 @@ void Sim::push(double s0, double s1) {
 // ---------------------------------------------------------------- currents (src/sim_particles.cpp:56-207)
 void Sim::currents() {
     tic(F_CUR);
+    cmJ = false;                 // jk is written whole below, in the reference's layout
     jw.zero();
     DepositArgs d;
     d.zax = zax; d.yax = yax; d.hcg = hcg.data(); d.ia = iA.data(); d.id = iD.data(); d.J = jw.data(); d.dx = dx;
@@ -619,6 +622,7 @@ void Sim::currents() {
 }
 
 void Sim::march(double D) {
+    ensure_layout(false, false);
     if (yf) march_all(mview(D), a, b);
     else march_lines(lview(D), a, b);
 }
@@ -626,7 +630,12 @@ void Sim::march(double D) {
 // ---------------------------------------------------------------- sponge and real fields (src/sim_fields.cpp:156-307)
 void Sim::sponge_and_real_fields() {
     tic(F_SPONGE);
-    const SpecGeom g = geom();
+    // Fourier y reads R, L, E_z, B_z and jk in whichever layout they are (both the same); free-hand y in the
+    // reference's
+    if (!yf) ensure_layout(false, false);
+    else if (cmR != cmJ) ensure_layout(cmR, cmR);
+    SpecGeom g = geom();
+    g.cm = cmR;
     const long ts = long(ns) * nk;
     const bool sp = cfg.sponge > 0 || cfg.sponge_y > 0;
     const double opb = 1 + beta_b;
@@ -662,6 +671,7 @@ void Sim::solve_fields(double D) {
 
 // ---------------------------------------------------------------- start field (src/sim_fields.cpp:44-151, 384-410)
 void Sim::comoving_field() {
+    ensure_layout(false, false);
     double sums[2] = {0, 0};
     for (auto& p : host_particles())
         if (p.alive == 1) { sums[0] += p.w * p.pz / std::sqrt(1 + p.px * p.px + p.py * p.py + p.pz * p.pz); sums[1] += p.w; }
@@ -684,6 +694,7 @@ void Sim::comoving_field() {
 }
 
 void Sim::relax_start_field() {
+    ensure_layout(false, false);
     const long n = long(ns) * nk, nj = long(ns + 1) * nk;
     DevBuf<cx> dx_(n), dy_(n);
     copy_planes(dx_.data(), Rx.data(), n);
@@ -703,6 +714,7 @@ void Sim::relax_start_field() {
 }
 
 void Sim::add_grid_lasers() {
+    ensure_layout(false, false);
     if (cfg.glasers.empty()) return;
     const SpecGeom g = geom();
     for (const auto& Lz : cfg.glasers) {
@@ -847,26 +859,27 @@ bool Sim::load_plasma_device(double s0, double s1) {
     return true;
 }
 
-void Sim::spectra_cm(bool to_cm) {
-    const int rows = to_cm ? cfg.ny : cfg.nx, cols = to_cm ? cfg.nx : cfg.ny;
-    auto flip = [&](DevBuf<cx>& A, DevBuf<cx>& scratch) {
+void Sim::ensure_layout(bool r_cm, bool j_cm) {
+    auto flip = [&](DevBuf<cx>& A, DevBuf<cx>& scratch, bool to_cm) {
+        if (!A.size()) return;
         if (scratch.size() != A.size()) scratch.resize(A.size());
-        transpose_planes(A.data(), scratch.data(), long(A.size()) / nk, rows, cols);
+        transpose_planes(A.data(), scratch.data(), long(A.size()) / nk, to_cm ? cfg.ny : cfg.nx,
+                         to_cm ? cfg.nx : cfg.ny);
         A.swap(scratch);
     };
-    if (to_cm) {
-        for (DevBuf<cx>* A : {&Rx, &Ry, &Ez, &Bz}) flip(*A, trF);     // L: every slice is written before it is read
-        for (DevBuf<cx>* A : {&jkx, &jky, &jkz, &js0x, &js0y, &js0z}) flip(*A, trC);
-        if (j0xT.size() != j0x.size()) { j0xT.resize(j0x.size()); j0yT.resize(j0y.size()); }
-        transpose_planes(j0x.data(), j0xT.data(), long(j0x.size()) / nk, rows, cols);
-        transpose_planes(j0y.data(), j0yT.data(), long(j0y.size()) / nk, rows, cols);
-    } else {
-        for (DevBuf<cx>* A : {&Rx, &Ry, &Ez, &Bz, &Lx, &Ly}) flip(*A, trF);
-        for (DevBuf<cx>* A : {&jkx, &jky, &jkz, &js0x, &js0y, &js0z}) flip(*A, trC);
+    if (cmR != r_cm) {
+        for (DevBuf<cx>* A : {&Rx, &Ry, &Ez, &Bz, &Lx, &Ly}) flip(*A, trF, r_cm);
+        for (DevBuf<cx>* A : {&js0x, &js0y, &js0z, &j0x, &j0y}) flip(*A, trC, r_cm);
+        cmR = r_cm;
+    }
+    if (cmJ != j_cm) {
+        for (DevBuf<cx>* A : {&jkx, &jky, &jkz}) flip(*A, trC, j_cm);
+        cmJ = j_cm;
     }
 }
 
 void Sim::stream_spectra() {
+    ensure_layout(false, false);
     // stand-alone form (jsr is consumed): the same row and column kernels as the march, spectra only
     pass_rows_fwd(jsr.data(), TileGeom{}, fft, sp3.data(), nullptr, nullptr, 0, 0);
     ModeArgs m{};
@@ -878,11 +891,13 @@ void Sim::stream_spectra() {
 }
 
 void Sim::march_slice(int n, double D) {
+    ensure_layout(false, false);
     if (yf) march_one(mview(D), n, nullptr, nullptr, nullptr, nullptr, 0);
     else march_lines(lview(D), n, n + 1);
 }
 
 void Sim::real_fields_slice(int n) {
+    ensure_layout(false, false);
     const SpecGeom g = geom();
     const long l = n - a;
     fields_pack3(g, nullptr, nullptr, 0, Rx.data() + l * nk, Ry.data() + l * nk, Lx.data() + l * nk, Ly.data() + l * nk,
@@ -906,6 +921,7 @@ StreamArgs Sim::stream_args(double s0, double s1) {
 
 // one pass of the (untiled) streaming list through half-cell n into a zeroed jsr; the list keeps the result
 void Sim::stream_pass(int n, int pass, double s0, double s1) {
+    ensure_layout(false, false);
     StreamArgs sc = stream_args(s0, s1);
     sc.n = n;
     jsr.zero();
@@ -947,11 +963,11 @@ void Sim::march_plasma(double s0, double s1) {
     for (const auto& is : cfg.ions) ions = ions || cfg.loaded(is);
     const bool mx = cfg.mixed && yf && !ions;
     const char* ecm = getenv("G3D_PASS_CM");
-    const bool cm = mx && !(ecm && atoi(ecm) == 0) && cfg.ny >= 2 && cfg.ny <= 1024;
-    if (cm) spectra_cm(true);
+    const bool cm = mx && !(ecm && atoi(ecm) == 0) && cfg.ny <= 1024 && cfg.nx % 16 == 0 && cfg.ny % 16 == 0;
+    ensure_layout(cm, cm);
     ModeArgs m{};
     m.mv = mview(D);
-    if (cm) { m.mv.cm = 1; m.mv.j0x = j0xT.data(); m.mv.j0y = j0yT.data(); }
+    m.mv.cm = cm;
     m.jsn = jsn.data(); m.sin_ = jsin.data(); m.si0 = jsi0.data(); m.pin = jpred.data(); m.dl = dlt.data();
     m.js0x = js0x.data(); m.js0y = js0y.data(); m.js0z = js0z.data();
     m.jkx = jkx.data(); m.jky = jky.data(); m.jkz = jkz.data();
@@ -1062,7 +1078,6 @@ void Sim::march_plasma(double s0, double s1) {
         }
         have_dl = true;
     }
-    if (cm) spectra_cm(false);
     (void)born_base;
     nstrm = nlist;
     // trapped particles (alive 5): fast ones join the beam list, slow ones stream again in the next step
@@ -1158,6 +1173,7 @@ void Sim::window_shift_start() {
     if (std::abs(nxs) >= 2 || std::abs(nys) >= 2) shift_window(nxs, nys);
 }
 void Sim::shift_window(int nxs, int nys) {
+    ensure_layout(false, false);
     X0 += nxs * dx; Y0 += nys * dy;
     const double sx = nxs * dx, sy = nys * dy;
     parts_shift(beam.view(), np, sx, sy);

@@ -154,6 +154,13 @@ public:
     hipEvent_t ev_open = nullptr;
     int ev_open_fam = -1;
 
+    // The spectra of the plasma pass in the column-major layout of the fused column kernel (mode ix ny + j) or in
+    // the reference's (j nx + ix): R, E_z, B_z, L, js0, j0 (cmR) and jk (cmJ).  The FP32-mixed march sets them
+    // column-major and they stay so through the sponge and into the next step; every other member function and
+    // array access asks for the reference's layout first.
+    void ensure_layout(bool r_cm, bool j_cm);
+    bool cmR = false, cmJ = false;
+
 private:
     SpecGeom geom() const;
     MarchView mview(double D);
@@ -164,10 +171,7 @@ private:
     void setup_tiles();
     void ensure_strm(size_t need);
     bool load_plasma_device(double s0, double s1);
-    // the spectra the plasma pass reads and writes, between the reference's layout (mode j nx + ix) and the
-    // column-major layout of the fused column kernel (ix ny + j); j0 goes to j0xT, j0yT (read only in the march)
-    void spectra_cm(bool to_cm);
-    DevBuf<cx> trF, trC, j0xT, j0yT;
+    DevBuf<cx> trF, trC;
 };
 
 extern const char* const kFamilyNames[Sim::kFam];

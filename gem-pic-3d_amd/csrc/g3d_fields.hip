@@ -58,13 +58,14 @@ __global__ void k_sponge_pack(SpecGeom g, const cx* rx, const cx* ry, const cx* 
     if (idx >= long(g.ns) * g.nk) return;
     const long i = idx % g.nk;
     const int ixx = int(i % g.nx), j = int(i / g.nx), l = int(idx / g.nk), n = g.a + l;
+    const long ic = g.cm ? long(ixx) * g.ny + j : i;       // the mode in the layout of R and jk
     // charge of slice n + dl from the filtered half-cell currents (the ghost half-cells of one GPU are zero)
     auto rho_at = [&](int dl) -> cx {
         const int nn = n + dl;
         if (nn < 0 || nn > g.N - 1) return mk(0);
         const int h0 = l + dl, h1 = l + dl + 1;
-        const cx a = (h0 < 0 || h0 > g.ns) ? mk(0) : jz[long(h0) * g.nk + i];
-        const cx b = (h1 < 0 || h1 > g.ns) ? mk(0) : jz[long(h1) * g.nk + i];
+        const cx a = (h0 < 0 || h0 > g.ns) ? mk(0) : jz[long(h0) * g.nk + ic];
+        const cx b = (h1 < 0 || h1 > g.ns) ? mk(0) : jz[long(h1) * g.nk + ic];
         return wl[nn] * a + wu[nn] * b;
     };
     const double hm = hc[n + 1], hp = hc[n + 2];
@@ -79,7 +80,8 @@ __global__ void k_sponge_pack(SpecGeom g, const cx* rx, const cx* ry, const cx* 
     }
     s2[idx] = psi;
     const cx bx = -opb * (ik(kxv) * psi), by = -opb * (ik(kyv) * psi);
-    const cx u = rx[idx] - bx, v = ry[idx] - by;
+    const long f = long(l) * g.nk + ic;
+    const cx u = rx[f] - bx, v = ry[f] - by;
     s1[idx] = mk(u.re - v.im, u.im + v.re);
 }
 __global__ void k_sponge_mask(SpecGeom g, cx* s1, const double* mx, const double* mn, const double* mh, long tot) {
@@ -94,20 +96,98 @@ __global__ void k_pack3(SpecGeom g, const cx* s1, const cx* s2, double opb, cx* 
                         const cx* ez, const cx* bz, cx* T, long ts, long tot) {
     const long idx = long(blockIdx.x) * blockDim.x + threadIdx.x;
     if (idx >= tot) return;
-    cx r = rx[idx], q = ry[idx];
+    const long i = idx % g.nk, base = idx - i;
+    const int ixx = int(i % g.nx), j = int(i / g.nx);
+    const long f = g.cm ? base + long(ixx) * g.ny + j : idx;     // the mode in the layout of R, L, E_z, B_z
+    cx r = rx[f], q = ry[f];
     if (s1) {
-        const long i = idx % g.nk, base = idx - i;
-        const int ixx = int(i % g.nx), j = int(i / g.nx);
         cx X, Y;
         unpack_pair(s1[idx], conjg(s1[base + mirror(i, g.nx, g.ny)]), X, Y);
         r = X - opb * (ik(g.kx[ixx]) * s2[idx]);
         q = Y - opb * (ik(g.ky[j]) * s2[idx]);
-        rx[idx] = r; ry[idx] = q;
+        rx[f] = r; ry[f] = q;
     }
-    const cx l = lx[idx], m = ly[idx], e = ez[idx], b = bz[idx];
+    const cx l = lx[f], m = ly[f], e = ez[f], b = bz[f];
     T[idx] = mk(r.re - q.im, r.im + q.re);
     T[ts + idx] = mk(l.re - m.im, l.im + m.re);
     T[2 * ts + idx] = mk(e.re - b.im, e.im + b.re);
+}
+// The same two kernels with R, L, E_z, B_z and jk column-major (mode ix ny + j) and the FFT scratch row-major:
+// one workgroup per tile of 16 x 16 modes of a plane, the column-major arrays staged through LDS so that both
+// layouts are read and written along their rows.  Lane t: (ix0 + t / 16, j0 + t % 16) for the column-major
+// accesses, (ix0 + t % 16, j0 + t / 16) for the row-major ones; LDS tiles are [ix - ix0][j - j0].
+constexpr int kTT = 16;
+__global__ void __launch_bounds__(256) k_sponge_pack_cm(SpecGeom g, const cx* rx, const cx* ry, const cx* jz,
+                                                        const double* hc, const double* wl, const double* wu,
+                                                        double g2, double opb, cx* s1, cx* s2) {
+    __shared__ cx trx[kTT][kTT + 1], tr_y[kTT][kTT + 1], tjz[4][kTT][kTT + 1];
+    const int ntx = g.nx / kTT, nty = g.ny / kTT;
+    const long per = long(ntx) * nty;
+    const int l = int(blockIdx.x / per), t = int(blockIdx.x - l * per), ix0 = (t % ntx) * kTT, j0 = (t / ntx) * kTT;
+    const int a = threadIdx.x / kTT, b = threadIdx.x % kTT;
+    {
+        const long ic = long(ix0 + a) * g.ny + j0 + b;
+        trx[a][b] = rx[long(l) * g.nk + ic];
+        tr_y[a][b] = ry[long(l) * g.nk + ic];
+        for (int q = 0; q < 4; ++q) {                 // half-cell planes l - 1 ... l + 2
+            const int h = l - 1 + q;
+            tjz[q][a][b] = (h < 0 || h > g.ns) ? mk(0) : jz[long(h) * g.nk + ic];
+        }
+    }
+    __syncthreads();
+    const int ixx = ix0 + b, j = j0 + a, n = g.a + l;
+    const long idx = long(l) * g.nk + long(j) * g.nx + ixx;
+    auto rho_at = [&](int dl) -> cx {
+        const int nn = n + dl;
+        if (nn < 0 || nn > g.N - 1) return mk(0);
+        return wl[nn] * tjz[dl + 1][b][a] + wu[nn] * tjz[dl + 2][b][a];
+    };
+    const double hm = hc[n + 1], hp = hc[n + 2];
+    const double cm = 2 / ((hm + hp) * hm), cp = 2 / ((hm + hp) * hp);
+    const double kxv = g.kx[ixx], kyv = g.ky[j];
+    const double k2 = kxv * kxv + kyv * kyv;
+    cx psi = mk(0);
+    if (k2 > 0) {
+        const cx r0 = rho_at(0);
+        const cx d2 = g2 * (cp * rho_at(1) - (cm + cp) * r0 + cm * rho_at(-1));
+        psi = (r0 + d2 / k2) / k2;
+    }
+    s2[idx] = psi;
+    const cx bx = -opb * (ik(kxv) * psi), by = -opb * (ik(kyv) * psi);
+    const cx u = trx[b][a] - bx, v = tr_y[b][a] - by;
+    s1[idx] = mk(u.re - v.im, u.im + v.re);
+}
+__global__ void __launch_bounds__(256) k_pack3_cm(SpecGeom g, const cx* s1, const cx* s2, double opb, cx* rx, cx* ry,
+                                                  const cx* lx, const cx* ly, const cx* ez, const cx* bz, cx* T,
+                                                  long ts) {
+    __shared__ cx tile[6][kTT][kTT + 1];
+    const int ntx = g.nx / kTT, nty = g.ny / kTT;
+    const long per = long(ntx) * nty;
+    const int l = int(blockIdx.x / per), t = int(blockIdx.x - l * per), ix0 = (t % ntx) * kTT, j0 = (t / ntx) * kTT;
+    const int a = threadIdx.x / kTT, b = threadIdx.x % kTT;
+    const long base = long(l) * g.nk;
+    const long ic = base + long(ix0 + a) * g.ny + j0 + b;
+    tile[0][a][b] = rx[ic]; tile[1][a][b] = ry[ic]; tile[2][a][b] = lx[ic];
+    tile[3][a][b] = ly[ic]; tile[4][a][b] = ez[ic]; tile[5][a][b] = bz[ic];
+    __syncthreads();
+    const int ixx = ix0 + b, j = j0 + a;
+    const long i = long(j) * g.nx + ixx, idx = base + i;
+    cx r = tile[0][b][a], q = tile[1][b][a];
+    if (s1) {
+        cx X, Y;
+        unpack_pair(s1[idx], conjg(s1[base + mirror(i, g.nx, g.ny)]), X, Y);
+        r = X - opb * (ik(g.kx[ixx]) * s2[idx]);
+        q = Y - opb * (ik(g.ky[j]) * s2[idx]);
+    }
+    const cx lv = tile[2][b][a], m = tile[3][b][a], e = tile[4][b][a], bv = tile[5][b][a];
+    T[idx] = mk(r.re - q.im, r.im + q.re);
+    T[ts + idx] = mk(lv.re - m.im, lv.im + m.re);
+    T[2 * ts + idx] = mk(e.re - bv.im, e.im + bv.re);
+    if (!s1) return;
+    __syncthreads();
+    tile[0][b][a] = r; tile[1][b][a] = q;
+    __syncthreads();
+    rx[ic] = tile[0][a][b]; ry[ic] = tile[1][a][b];
 }
 // E_x = (R_x + L_x)/2, E_y = (R_y + L_y)/2, B_x = (L_y - R_y)/2, B_y = (R_x - L_x)/2, E_z, B_z
 // (src/sim_fields.cpp:280-293) into float F
@@ -350,6 +430,14 @@ void currents_unpack(const SpecGeom& g, const cx* S1, cx* jkx, cx* jky, cx* jkz,
 }
 void sponge_pack(const SpecGeom& g, const cx* Rx, const cx* Ry, const cx* jkz, const double* hcg, const double* wlo,
                  const double* whi, double g2, double opb, cx* S1, cx* S2) {
+    if (g.cm) {
+        if (g.nx % kTT || g.ny % kTT) throw std::runtime_error("sponge: column-major modes need nx, ny multiples of 16");
+        const long nb = long(g.ns) * (g.nx / kTT) * (g.ny / kTT);
+        hipLaunchKernelGGL(k_sponge_pack_cm, dim3(unsigned(nb)), dim3(256), 0, stream(), g, Rx, Ry, jkz, hcg, wlo, whi,
+                           g2, opb, S1, S2);
+        launch_check("sponge pack (column-major)");
+        return;
+    }
     G3D_LAUNCH(k_sponge_pack, long(g.ns) * g.nk, g, Rx, Ry, jkz, hcg, wlo, whi, g2, opb, S1, S2);
 }
 void sponge_mask(const SpecGeom& g, cx* S1, const double* mx, const double* mn, const double* mh, int count) {
@@ -357,6 +445,15 @@ void sponge_mask(const SpecGeom& g, cx* S1, const double* mx, const double* mn, 
 }
 void fields_pack3(const SpecGeom& g, const cx* S1, const cx* S2, double opb, cx* Rx, cx* Ry, const cx* Lx,
                   const cx* Ly, const cx* Ez, const cx* Bz, cx* T, long ts, int count) {
+    if (g.cm) {
+        if (g.nx % kTT || g.ny % kTT) throw std::runtime_error("pack: column-major modes need nx, ny multiples of 16");
+        const long nb = long(count) * (g.nx / kTT) * (g.ny / kTT);
+        if (nb <= 0) return;
+        hipLaunchKernelGGL(k_pack3_cm, dim3(unsigned(nb)), dim3(256), 0, stream(), g, S1, S2, opb, Rx, Ry, Lx, Ly, Ez, Bz,
+                           T, ts);
+        launch_check("fields pack (column-major)");
+        return;
+    }
     G3D_LAUNCH(k_pack3, long(count) * g.nk, g, S1, S2, opb, Rx, Ry, Lx, Ly, Ez, Bz, T, ts, long(count) * g.nk);
 }
 void fields_to_F(const cx* T, long ts, float* F, long fpl, long nk, int count) {

@@ -122,6 +122,7 @@ struct SpecGeom {
     int nx, ny, ns, N, a;
     long nk;
     const double *kx, *ky, *kxs, *kys, *filt, *filty;
+    int cm = 0;      // sponge and real fields: R, L, E_z, B_z and jk column-major (mode ix ny + j); scratch row-major
 };
 // currents: zeta filter of jw into S1 (jx + i jy) and jkz (jz + 0 i), ns+1 half-cell planes
 void currents_zfilter(const SpecGeom& g, const double* jw, const double* hcg, int zfilter, int zuniform, cx* S1, cx* jkz);
