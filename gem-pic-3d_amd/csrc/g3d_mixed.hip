@@ -466,9 +466,12 @@ __global__ void __launch_bounds__(kT) k_rows_inv_mixed(const cxf* T, float* Fn, 
 // One workgroup per k_x column (ny lanes, one mode each), the modes column-major (i = kx ny + j): the column's three
 // planes of X in LDS, y-transforms, the per-mode work of every mode of the column, the inverse y-transforms of the
 // three packed planes, T written back.  The per-mode arrays of a column are contiguous in this layout.
+// BS: the largest ny of the instance (one lane per mode); BS = 256 leaves each lane the registers to keep its
+// mode's loads in flight together (there is one wave per SIMD at 256 x 256 modes)
 constexpr int kColMax = 1024;
-__global__ void __launch_bounds__(kColMax) k_colmodes_mixed(ModeArgs m, const cxf* X, cxf* T, int ny, int ly,
-                                                            const cx* twy, int kind) {
+template <int BS>
+__global__ void __launch_bounds__(BS) k_colmodes_mixed(ModeArgs m, const cxf* X, cxf* T, int ny, int ly,
+                                                       const cx* twy, int kind) {
     extern __shared__ cxf smf[];
     const int sp = padded(ny);
     cxf* buf = smf;
@@ -480,6 +483,9 @@ __global__ void __launch_bounds__(kColMax) k_colmodes_mixed(ModeArgs m, const cx
     const int kx = blockIdx.x;
     const long c0 = long(kx) * ny;
     const int jj = threadIdx.x;                      // this lane's mode (kx, jj)
+    const long i = c0 + jj;
+    ModeLoad L;
+    if constexpr (BS <= 256) L = mode_load<true>(m, i);     // the mode's state, in flight during the transforms
     if (kind != 1)
         for (int e = jj; e < 3 * ny; e += blockDim.x) {
             const int p = e / ny, j = e - p * ny;
@@ -488,14 +494,14 @@ __global__ void __launch_bounds__(kColMax) k_colmodes_mixed(ModeArgs m, const cx
     __syncthreads();
     if (kind != 1) lds_fft(buf, ny, ly, 3, sp, twf);
     cx u[3] = {mk(0), mk(0), mk(0)};
-    const long i = c0 + jj;
     if (kind != 1) {
         const double k2 = m.kxs[kx] * m.kxs[kx] + m.kys[jj] * m.kys[jj];
         const float f = float((kx == 0 && jj == 0 && !m.keep0) ? 0.0 : m.filt[kx] * m.filty[jj] / (1 + m.a2 * k2));
         for (int k = 0; k < 3; ++k) u[k] = to_d(f * buf[k * sp + lpos(jj)]);
     }
     cx t[3];
-    mode_work<true>(m, i, u, t);
+    if constexpr (BS <= 256) mode_compute<true>(m, i, L, u, t);
+    else mode_work<true>(m, i, u, t);          // (more lanes per workgroup: no registers to hold the state ahead)
     __syncthreads();                                 // every lane has read its X before T overwrites the buffer
     const int b = lpos(bitrev(jj, ly));
     for (int k = 0; k < 3; ++k) buf[k * sp + b] = to_f(t[k]);
@@ -534,8 +540,9 @@ void pass_mixed(const ModeArgs& m, const FftPlan& p, float* J, const float* slab
     const size_t li = (size_t(3 * rpw) * (p.nx + p.nx / 8) + p.nx / 2) * sizeof(cxf);
     if (cm) {       // modes column-major: one fused launch between the row transforms
         const size_t lm = (size_t(3) * (p.ny + p.ny / 8) + p.ny) * sizeof(cxf);
-        hipLaunchKernelGGL(k_colmodes_mixed, dim3(unsigned(p.nx)), dim3(unsigned(p.ny)), lm, stream(), m, X, T, p.ny,
-                           p.ly, p.twy, kind);
+        auto* kc = p.ny <= 256 ? k_colmodes_mixed<256> : k_colmodes_mixed<kColMax>;
+        hipLaunchKernelGGL(kc, dim3(unsigned(p.nx)), dim3(unsigned(p.ny)), lm, stream(), m, X, T, p.ny, p.ly, p.twy,
+                           kind);
         hipLaunchKernelGGL(k_rows_inv_mixed, dim3(p.ny / rpw), dim3(lanes(rpw * p.nx)), li, stream(), T, Fn, fpl, p.nx,
                            p.ny, p.lx, rpw, p.twx, float(1.0 / (double(p.nx) * p.ny)), 1);
         launch_check("plasma pass (mixed, column-major modes)");
