@@ -138,6 +138,7 @@ int g3d_sim_set(g3d_sim* sim, const char* name, const void* host, size_t bytes) 
     const int g = g3d::guard([&] {
         g3d::Sim& s = *sim->s;
         s.ensure_layout(false, false);
+        s.jw_zero = false;
         if (std::string(name) == "strm") {
             const g3d_particle* p = static_cast<const g3d_particle*>(host);
             s.set_strm(std::vector<g3d_particle>(p, p + bytes / sizeof(g3d_particle)));

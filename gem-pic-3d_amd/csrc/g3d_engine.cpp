@@ -605,7 +605,15 @@ void Sim::push(double s0, double s1) {
 void Sim::currents() {
     tic(F_CUR);
     cmJ = false;                 // jk is written whole below, in the reference's layout
+    if (np == 0) {               // no beam particles: the beam current and its spectra are zero
+        if (!jw_zero) { jw.zero(); jw_zero = true; }
+        const long n = long(ns + 1) * nk;
+        zero_planes(jkx.data(), n); zero_planes(jky.data(), n); zero_planes(jkz.data(), n);
+        toc(F_CUR);
+        return;
+    }
     jw.zero();
+    jw_zero = false;
     DepositArgs d;
     d.zax = zax; d.yax = yax; d.hcg = hcg.data(); d.ia = iA.data(); d.id = iD.data(); d.J = jw.data(); d.dx = dx;
     d.dy = dy; d.nh = ns + 3; d.a = a; d.N = N; d.nx = cfg.nx; d.ny = cfg.ny; d.yf = yf; d.nk = nk;

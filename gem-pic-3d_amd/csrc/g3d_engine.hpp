@@ -160,6 +160,7 @@ public:
     // array access asks for the reference's layout first.
     void ensure_layout(bool r_cm, bool j_cm);
     bool cmR = false, cmJ = false;
+    bool jw_zero = false;             // jw known to be zero (no beam deposit since it was cleared)
 
 private:
     SpecGeom geom() const;
