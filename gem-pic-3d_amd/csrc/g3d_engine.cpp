@@ -230,7 +230,8 @@ void Sim::setup_tiles() {
     }
     const int TX = pick(cfg.nx, px), TY = pick(cfg.ny, py);
     if (!TX || !TY) return;
-    int M = 3;
+    int M = 3;                                // G3D_TILE_MARGIN: nodes of drift a window allows (sweeps)
+    if (const char* e = getenv("G3D_TILE_MARGIN")) M = std::max(0, atoi(e));
     M = std::min({M, TX - 1, TY - 1, (cfg.nx - TX - 3) / 2, (cfg.ny - TY - 3) / 2});
     if (M < 0) return;
     tiles.TX = TX; tiles.TY = TY; tiles.M = M; tiles.WX = TX + 3 + 2 * M; tiles.WY = TY + 3 + 2 * M;
@@ -1001,6 +1002,17 @@ void Sim::march_plasma(double s0, double s1) {
     // tiles only when this step loaded tiled plasma (otherwise the slabs would hold an earlier step's current)
     TileGeom tg = n_tiled > 0 ? tiles : TileGeom{};
     tg.idx = nullptr;
+    tg.chunk = 0; tg.max_extra = 0; tg.xtile = nullptr; tg.xstart = nullptr; tg.xcount = nullptr;
+    // FP32-mixed: tiles denser than one workgroup's lanes are streamed in chunks (G3D_STREAM_CHUNK=0: off)
+    const char* ech = getenv("G3D_STREAM_CHUNK");
+    if (mx && tg.ntile && !(ech && atoi(ech) == 0)) {
+        tg.chunk = tg.block;
+        tg.max_extra = int((nstrm + tg.chunk - 1) / tg.chunk);
+        if (chunk_tile.size() < size_t(tg.max_extra)) { chunk_tile.resize(size_t(tg.max_extra)); chunk_start.resize(size_t(tg.max_extra)); }
+        if (chunk_cnt.size() < 1) chunk_cnt.resize(1);
+        tg.xtile = chunk_tile.data(); tg.xstart = chunk_start.data(); tg.xcount = chunk_cnt.data();
+        tile_chunks(tg, chunk_tile.data(), chunk_start.data(), chunk_cnt.data());
+    }
     if (mx) {
         if (jsr_f.size() < 5 * pl) jsr_f.resize(5 * pl);
         if (tiles.ntile && slab_f.size() < size_t(tiles.ntile) * 5 * tiles.WX * tiles.WY)
@@ -1037,6 +1049,7 @@ void Sim::march_plasma(double s0, double s1) {
                       tile_off.data(), bin_idx.data());
             tg.idx = bin_idx.data();
             tg.n_tiled = nlist;
+            if (tg.chunk) tile_chunks(tg, chunk_tile.data(), chunk_start.data(), chunk_cnt.data());
         }
         for (int pass = 0; pass < passes; ++pass) {
             sc.n = n;

@@ -173,6 +173,8 @@ private:
     void ensure_strm(size_t need);
     bool load_plasma_device(double s0, double s1);
     DevBuf<cx> trF, trC;
+    DevBuf<int> chunk_tile, chunk_cnt;        // further chunks of dense tiles (FP32-mixed stream kernel)
+    DevBuf<long> chunk_start;
 };
 
 extern const char* const kFamilyNames[Sim::kFam];

@@ -46,6 +46,13 @@ struct TileGeom {
     int fwin;                         // stage the tile's field window in LDS for the gather
     int block;                        // lanes of a tile's workgroup (a multiple of 64, at most kStreamBlockMax)
     int wpe;                          // FP32-mixed stream kernel: 4 = registers capped for 4 waves per SIMD, else 0
+    // FP32-mixed stream kernel, dense tiles: a workgroup streams at most `chunk` particles (0: a whole tile); the
+    // further chunks of a tile, xtile/xstart[0 .. *xcount), run in max_extra more workgroups after the tiles and
+    // add their window to the current planes with atomics
+    int chunk, max_extra;
+    const int* xtile;
+    const long* xstart;
+    const int* xcount;
     long ntile, n_tiled;              // tiles; list entries binned (= off[ntile]); from n_tiled on: untiled tail
     const long* off;
     const int* idx;                   // list indices grouped by tile (null: the list itself is tile-ordered)
@@ -55,6 +62,9 @@ struct TileGeom {
 // the other particles (final for the rest of the step) is copied from S to D.  cnt, cur: ntile ints of scratch
 void bin_tiles(Parts S, Parts D, long n, const TileGeom& tg, double dx, double dy, int nx, int ny, int* cnt,
                int* cur, long* off, int* idx);
+// the chunks beyond the first of every tile holding more than tg.chunk particles (see TileGeom); xtile, xstart hold
+// tg.max_extra entries
+void tile_chunks(const TileGeom& tg, int* xtile, long* xstart, int* xcount);
 
 void bunch_push(const PushArgs& a, Parts p, long np, double s0);
 // beam deposit into jw [3][ns+3][nk] (src/sim_particles.cpp:66-106)

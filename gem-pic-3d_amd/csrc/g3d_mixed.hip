@@ -270,17 +270,27 @@ __global__ void __launch_bounds__(kStreamBlockMax) __attribute__((amdgpu_waves_p
 k_stream_mixed(StreamArgs c, float* J, float* slab, Parts S, Parts D, long n, TileGeom tg) {
     extern __shared__ float4 winf4[];
     const long b = blockIdx.x;
-    if (b < tg.ntile) {
-        // the tile's range, this lane's first particle and its state first: their dependent loads overlap the
-        // loads of the field window
-        const long lo = tg.off[b], hi = tg.off[b + 1];
+    const long nt = tg.ntile + tg.max_extra;
+    if (b < nt) {
+        // workgroups b < max_extra: the further chunks of dense tiles (first, so that they run beside the first
+        // chunks), added to J with atomics; then tile b - max_extra's first chunk, written to its slab
+        long t, lo, hi;
+        const bool extra = b < tg.max_extra;
+        if (!extra) { t = b - tg.max_extra; lo = tg.off[t]; hi = tg.off[t + 1]; }
+        else {
+            if (b >= tg.xcount[0]) return;
+            t = tg.xtile[b]; lo = tg.xstart[b]; hi = tg.off[t + 1];
+        }
+        if (tg.chunk > 0 && hi - lo > tg.chunk) hi = lo + tg.chunk;
+        // the range, this lane's first particle and its state first: their dependent loads overlap the loads of
+        // the field window
         long k = lo + threadIdx.x;
         long i = k < hi ? (tg.idx ? long(tg.idx[k]) : k) : 0;
         MState st{};
         if (k < hi) st = load_mstate(S, i);
         const int pl = tg.WX * tg.WY;
         const int wn = kDepNode * pl;
-        const int tx = int(b % tg.ntx), ty = int(b / tg.ntx);
+        const int tx = int(t % tg.ntx), ty = int(t / tg.ntx);
         const int x0 = tx * tg.TX - 1 - tg.M, y0 = ty * tg.TY - 1 - tg.M;
         const FieldView& F = c.fv;
         const int fl = c.n - F.a;
@@ -314,11 +324,20 @@ k_stream_mixed(StreamArgs c, float* J, float* slab, Parts S, Parts D, long n, Ti
             }
         }
         __syncthreads();
-        float* sl = slab + b * long(wn);
-        for (int r = threadIdx.x; r < wn; r += blockDim.x) sl[r] = float(dep[r]);
+        if (!extra) {
+            float* sl = slab + t * long(wn);
+            for (int r = threadIdx.x; r < wn; r += blockDim.x) sl[r] = float(dep[r]);
+        } else
+            for (int r = threadIdx.x; r < wn; r += blockDim.x) {
+                const double v = dep[r];
+                if (v == 0) continue;
+                const int node = r / kDepNode, cc = r - node * kDepNode, vv = node / tg.WX, u = node - vv * tg.WX;
+                const long g = long(wrap_index(y0 + vv, F.ny)) * F.nx + wrap_index(x0 + u, F.nx);
+                atomicAdd(&J[cc * F.nk + g], float(v));
+            }
         return;
     }
-    const long i = tg.n_tiled + (b - tg.ntile) * blockDim.x + threadIdx.x;
+    const long i = tg.n_tiled + (b - nt) * blockDim.x + threadIdx.x;
     if (i < n)
         stream_one_mixed<LASER>(c, InvF{1.0 / c.fv.dx, 1.0 / c.fv.dy, float(c.ia[0])}, J, nullptr, load_mstate(S, i), D,
                                 i);
@@ -520,7 +539,7 @@ void stream_mixed(const StreamArgs& a, float* J, float* slab, Parts src, Parts d
     if (bs < 64 || bs > kStreamBlockMax || bs % 64) throw std::runtime_error("stream half-cell (mixed): bad block size");
     const long tail_blocks = nstrm > tg.n_tiled ? long(blocks_for(nstrm - tg.n_tiled, bs)) : 0;
     const size_t lds = tg.ntile ? size_t(tg.WX) * tg.WY * (kDepNode * sizeof(double) + kFwNode * sizeof(float)) : 0;
-    const dim3 grid(unsigned(tg.ntile + tail_blocks)), blk(bs);
+    const dim3 grid(unsigned(tg.ntile + tg.max_extra + tail_blocks)), blk(bs);
     auto* k = a.laser.on ? (tg.wpe == 4 ? k_stream_mixed<4, true> : k_stream_mixed<0, true>)
                          : (tg.wpe == 4 ? k_stream_mixed<4, false> : k_stream_mixed<0, false>);
     launch_timed_kernel(lt, k, grid, blk, unsigned(lds), a, J, slab,

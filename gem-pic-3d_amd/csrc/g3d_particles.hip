@@ -570,6 +570,26 @@ void stream_halfcell(const StreamArgs& a, Parts src, Parts dst, long nstrm, cons
     launch_timed_kernel(lt, k, grid, blk, sh, a, src, dst, nstrm, tg);
     launch_check("stream half-cell");
 }
+// one workgroup: the further chunks of every tile, listed through an LDS counter
+__global__ void __launch_bounds__(1024) k_tile_chunks(const long* off, long ntile, int chunk, int cap, int* xtile,
+                                                      long* xstart, int* xcount) {
+    __shared__ int cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    for (long t = threadIdx.x; t < ntile; t += blockDim.x)
+        for (long s = off[t] + chunk; s < off[t + 1]; s += chunk) {
+            const int e = atomicAdd(&cnt, 1);
+            if (e < cap) { xtile[e] = int(t); xstart[e] = s; }
+        }
+    __syncthreads();
+    if (threadIdx.x == 0) xcount[0] = cnt < cap ? cnt : cap;
+}
+void tile_chunks(const TileGeom& tg, int* xtile, long* xstart, int* xcount) {
+    if (tg.ntile <= 0 || tg.chunk <= 0) return;
+    hipLaunchKernelGGL(k_tile_chunks, dim3(1), dim3(1024), 0, stream(), tg.off, tg.ntile, tg.chunk, tg.max_extra,
+                       xtile, xstart, xcount);
+    launch_check("tile chunks");
+}
 void bin_tiles(Parts S, Parts D, long n, const TileGeom& tg, double dx, double dy, int nx, int ny, int* cnt,
                int* cur, long* off, int* idx) {
     if (n <= 0 || tg.ntile <= 0) return;
