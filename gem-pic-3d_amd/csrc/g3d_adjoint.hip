@@ -1,4 +1,6 @@
 // Adjoint (reverse-mode) kernels of the march, for gradients through it.  No counterpart in the reference.
+// Here: the transposed per-mode update (below), the TSC deposit with its transpose (the gather) and the gather's
+// position derivatives, and the reverse of the Boris rotation.
 //
 // The per-mode march of slice n (march_mode) is complex-linear in its inputs: the level-s0 values of slice n
 // (R_x, R_y, E_z, B_z), the level-s1 values of slice n+1 (six), and the currents of half-cells n-1 and n.  Its
@@ -117,10 +119,153 @@ __global__ void __launch_bounds__(256) k_march_adj(AdjView c, int lo, int hi) {
     }
 }
 
+// ---------------------------------------------------------------- TSC deposit and its transpose
+__global__ void __launch_bounds__(256) k_tsc_deposit(g3d_tsc_ctx c, const double* q, double* plane) {
+    const long p = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= c.np) return;
+    int ix[3], jn[3]; double wx[3], wn[3];
+    tsc_periodic(c.x[p], c.dx, c.nx, ix, wx);
+    tsc_periodic(c.y[p], c.dy, c.ny, jn, wn);
+    for (int v = 0; v < 3; ++v)
+        for (int u = 0; u < 3; ++u) atomicAdd(&plane[long(jn[v]) * c.nx + ix[u]], wx[u] * wn[v] * q[p]);
+}
+// d/dx of the TSC weights of x (index coordinate xi = x/d, f = xi - round(xi)): the derivative of
+// 1/2 (1/2 - f)^2, 3/4 - f^2, 1/2 (1/2 + f)^2, times 1/d
+__device__ __forceinline__ void tsc_dweights(double x, double d, double dw[3]) {
+    const double xi = x / d;
+    const double f = xi - floor(xi + 0.5);
+    dw[0] = -(0.5 - f) / d;
+    dw[1] = -2 * f / d;
+    dw[2] = (0.5 + f) / d;
+}
+__global__ void __launch_bounds__(256) k_tsc_gather(g3d_tsc_ctx c, const double* plane, double* out, double* gx,
+                                                    double* gy) {
+    const long p = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= c.np) return;
+    int ix[3], jn[3]; double wx[3], wn[3], dx[3], dy[3];
+    tsc_periodic(c.x[p], c.dx, c.nx, ix, wx);
+    tsc_periodic(c.y[p], c.dy, c.ny, jn, wn);
+    double s = 0, sx = 0, sy = 0;
+    const bool grad = gx && gy;
+    if (grad) { tsc_dweights(c.x[p], c.dx, dx); tsc_dweights(c.y[p], c.dy, dy); }
+    for (int v = 0; v < 3; ++v)
+        for (int u = 0; u < 3; ++u) {
+            const double P = plane[long(jn[v]) * c.nx + ix[u]];
+            s += wx[u] * wn[v] * P;
+            if (grad) { sx += dx[u] * wn[v] * P; sy += wx[u] * dy[v] * P; }
+        }
+    out[p] = s;
+    if (grad) { gx[p] = sx; gy[p] = sy; }
+}
+
+// ---------------------------------------------------------------- Boris rotation and its reverse
+__global__ void __launch_bounds__(256) k_boris(g3d_boris_args a) {
+    const long p = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= a.np) return;
+    const double E[3] = {a.Ex[p], a.Ey[p], a.Ez[p]}, B[3] = {a.Bx[p], a.By[p], a.Bz[p]};
+    double ux = a.ux[p], uy = a.uy[p], uz = a.uz[p];
+    boris(ux, uy, uz, E, B, a.qm[p], a.dt[p]);
+    a.ux[p] = ux; a.uy[p] = uy; a.uz[p] = uz;
+}
+__device__ __forceinline__ void cross3(const double a[3], const double b[3], double c[3]) {
+    c[0] = a[1] * b[2] - a[2] * b[1]; c[1] = a[2] * b[0] - a[0] * b[2]; c[2] = a[0] * b[1] - a[1] * b[0];
+}
+// reverse mode of boris() (g3d_common.hpp), statement by statement
+__global__ void __launch_bounds__(256) k_boris_vjp(g3d_boris_args a, g3d_boris_adj g) {
+    const long p = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= a.np) return;
+    const double E[3] = {a.Ex[p], a.Ey[p], a.Ez[p]}, B[3] = {a.Bx[p], a.By[p], a.Bz[p]};
+    const double u[3] = {a.ux[p], a.uy[p], a.uz[p]};
+    const double qm = a.qm[p], dt = a.dt[p];
+    // forward values
+    const double qh = 0.5 * qm * dt;
+    double av[3], t[3], pv[3], c[3];
+    for (int k = 0; k < 3; ++k) av[k] = u[k] + qh * E[k];
+    const double gm = sqrt(1.0 + av[0] * av[0] + av[1] * av[1] + av[2] * av[2]);
+    for (int k = 0; k < 3; ++k) t[k] = qh * B[k] / gm;
+    const double t2 = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
+    cross3(av, t, c);
+    for (int k = 0; k < 3; ++k) pv[k] = av[k] + c[k];
+    const double sf = 2.0 / (1.0 + t2);
+    // reverse: u' = b + qh E, b = a + sf (p x t)
+    const double ub[3] = {g.ubx[p], g.uby[p], g.ubz[p]};
+    double ab[3], Eb[3], Bb[3], tb[3], pb[3];
+    double qhb = ub[0] * E[0] + ub[1] * E[1] + ub[2] * E[2];
+    for (int k = 0; k < 3; ++k) { Eb[k] = qh * ub[k]; ab[k] = ub[k]; }
+    double pxt[3];
+    cross3(pv, t, pxt);
+    const double sfb = ub[0] * pxt[0] + ub[1] * pxt[1] + ub[2] * pxt[2];
+    double txb[3], bxp[3];
+    cross3(t, ub, txb);                  // d(p x t) . ub / dp = t x ub
+    cross3(ub, pv, bxp);                 // d(p x t) . ub / dt = ub x p
+    for (int k = 0; k < 3; ++k) { pb[k] = sf * txb[k]; tb[k] = sf * bxp[k]; }
+    // sf = 2 / (1 + t2)
+    const double t2b = -sfb * sf * sf * 0.5;
+    // p = a + a x t
+    double txp[3], pxa[3];
+    cross3(t, pb, txp);                  // d(a x t) . pb / da = t x pb
+    cross3(pb, av, pxa);                 // d(a x t) . pb / dt = pb x a
+    for (int k = 0; k < 3; ++k) { ab[k] += pb[k] + txp[k]; tb[k] += pxa[k]; }
+    // t2 = t . t
+    for (int k = 0; k < 3; ++k) tb[k] += 2 * t2b * t[k];
+    // t = qh B / gm
+    double gmb = 0;
+    for (int k = 0; k < 3; ++k) {
+        Bb[k] = qh * tb[k] / gm;
+        qhb += tb[k] * B[k] / gm;
+        gmb -= tb[k] * t[k] / gm;
+    }
+    // gm = sqrt(1 + a . a)
+    for (int k = 0; k < 3; ++k) ab[k] += gmb * av[k] / gm;
+    // a = u + qh E
+    for (int k = 0; k < 3; ++k) { Eb[k] += qh * ab[k]; qhb += ab[k] * E[k]; }
+    g.gux[p] = ab[0]; g.guy[p] = ab[1]; g.guz[p] = ab[2];
+    g.gEx[p] = Eb[0]; g.gEy[p] = Eb[1]; g.gEz[p] = Eb[2];
+    g.gBx[p] = Bb[0]; g.gBy[p] = Bb[1]; g.gBz[p] = Bb[2];
+    g.gqm[p] = 0.5 * dt * qhb;
+    g.gdt[p] = 0.5 * qm * qhb;
+}
+
 }  // namespace
 }  // namespace g3d
 
 extern "C" {
+int g3d_tsc_deposit(const g3d_tsc_ctx* c, const double* q, double* plane) {
+    return g3d::guard([&] {
+        if (!c || !c->x || !c->y || !q || !plane || c->nx < 1 || c->ny < 1 || c->np < 0)
+            throw std::invalid_argument("tsc deposit: bad context");
+        if (c->np == 0) return;
+        hipLaunchKernelGGL(g3d::k_tsc_deposit, dim3(g3d::blocks_for(c->np, 256)), dim3(256), 0, g3d::stream(), *c, q,
+                           plane);
+        g3d::launch_check("tsc deposit");
+    });
+}
+int g3d_tsc_gather(const g3d_tsc_ctx* c, const double* plane, double* out, double* gx, double* gy) {
+    return g3d::guard([&] {
+        if (!c || !c->x || !c->y || !plane || !out || c->nx < 1 || c->ny < 1 || c->np < 0 || (!gx) != (!gy))
+            throw std::invalid_argument("tsc gather: bad context");
+        if (c->np == 0) return;
+        hipLaunchKernelGGL(g3d::k_tsc_gather, dim3(g3d::blocks_for(c->np, 256)), dim3(256), 0, g3d::stream(), *c, plane,
+                           out, gx, gy);
+        g3d::launch_check("tsc gather");
+    });
+}
+int g3d_boris(const g3d_boris_args* a) {
+    return g3d::guard([&] {
+        if (!a || a->np < 0) throw std::invalid_argument("boris: bad arguments");
+        if (a->np == 0) return;
+        hipLaunchKernelGGL(g3d::k_boris, dim3(g3d::blocks_for(a->np, 256)), dim3(256), 0, g3d::stream(), *a);
+        g3d::launch_check("boris");
+    });
+}
+int g3d_boris_vjp(const g3d_boris_args* a, const g3d_boris_adj* g) {
+    return g3d::guard([&] {
+        if (!a || !g || a->np < 0) throw std::invalid_argument("boris vjp: bad arguments");
+        if (a->np == 0) return;
+        hipLaunchKernelGGL(g3d::k_boris_vjp, dim3(g3d::blocks_for(a->np, 256)), dim3(256), 0, g3d::stream(), *a, *g);
+        g3d::launch_check("boris vjp");
+    });
+}
 int g3d_march_adjoint(const g3d_march_adj_ctx* m, int lo, int hi) {
     return g3d::guard([&] {
         if (!m || !m->Rx || !m->Ry || !m->Ez || !m->Bz || !m->Lx || !m->Ly || !m->j0x || !m->j0y || !m->jkx ||

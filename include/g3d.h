@@ -155,6 +155,34 @@ typedef struct {
 } g3d_march_adj_ctx;
 int g3d_march_adjoint(const g3d_march_adj_ctx* c, int lo, int hi);
 
+/* Adjoint pairs of the particle-grid couplings (no counterpart in the reference; the pieces of the reverse march
+ * for fixed trajectories).  TSC on the periodic Fourier-y grid, nodes at (i dx, j dy), weights of
+ * src/sim_kernels.hpp:21-31: g3d_tsc_deposit adds q_p W_p to the plane, g3d_tsc_gather returns sum_k W_p(k) P[k]
+ * (its transpose) and, when gx, gy are set, the derivatives of that sum in x_p and y_p. */
+typedef struct {
+    const double *x, *y;          /* box coordinates of np particles (device) */
+    long np;
+    int nx, ny;
+    double dx, dy;
+} g3d_tsc_ctx;
+int g3d_tsc_deposit(const g3d_tsc_ctx* c, const double* q, double* plane);
+int g3d_tsc_gather(const g3d_tsc_ctx* c, const double* plane, double* out, double* gx, double* gy);
+
+/* The relativistic Boris rotation of the streaming push (src/sim_stream.hpp:8-18), in place on u, per particle
+ * fields E, B, charge-to-mass qm and step dt; and its vector-Jacobian product: given the adjoint of the pushed u
+ * (ub*), the adjoints of u, E, B, qm and dt at the state before the push (u* must still hold that state). */
+typedef struct {
+    double *ux, *uy, *uz;
+    const double *Ex, *Ey, *Ez, *Bx, *By, *Bz, *qm, *dt;
+    long np;
+} g3d_boris_args;
+typedef struct {
+    const double *ubx, *uby, *ubz;                 /* in: adjoint of the pushed momentum */
+    double *gux, *guy, *guz, *gEx, *gEy, *gEz, *gBx, *gBy, *gBz, *gqm, *gdt;   /* out */
+} g3d_boris_adj;
+int g3d_boris(const g3d_boris_args* a);
+int g3d_boris_vjp(const g3d_boris_args* a, const g3d_boris_adj* g);
+
 /* ------------------------------------------------------------------ engine: the Simulation of the march
  * g3d_sim mirrors the reference's Simulation (src/sim.hpp:222-360) on one GPU: init() and step() and the
  * member functions of the five drop-in files, each over the kernels of this library. */

@@ -108,3 +108,81 @@ def test_march_adjoint_front_slice():
     out = ga.march(geom, xs[:4], xs[4:])
     torch.sum(torch.real(out[2][ns - 1])).backward()
     assert all(float(torch.abs(t.grad).max()) == 0.0 for t in xs)
+
+
+# ------------------------------------------------------------------------------ the particle-grid pieces
+NX, NY, DX, DY = 8, 8, 0.7, 0.9
+
+
+def particles(r, n):
+    x = torch.from_numpy(r.uniform(0, NX * DX, n)).cuda()
+    y = torch.from_numpy(r.uniform(0, NY * DY, n)).cuda()
+    return x, y
+
+
+def test_tsc_pair_dot_product():
+    """<deposit(q), P> = <q, gather(P)>: the gather is the deposit's transpose"""
+    r = np.random.default_rng(21)
+    x, y = particles(r, 300)
+    q = torch.from_numpy(r.standard_normal(300)).cuda()
+    P = torch.from_numpy(r.standard_normal(NX * NY)).cuda()
+    lhs = float(torch.dot(ga.tsc_deposit_raw(x, y, q, NX, NY, DX, DY), P))
+    rhs = float(torch.dot(q, ga.tsc_gather_raw(x, y, P, NX, NY, DX, DY)))
+    assert abs(lhs - rhs) <= 1e-12 * max(1.0, abs(lhs))
+
+
+def test_tsc_deposit_weights():
+    """the deposit of unit charges sums to the particle count, and one particle's weights are the closed-form TSC
+    weights (src/sim_kernels.hpp:21-31; periodic, nodes at i dx)"""
+    r = np.random.default_rng(22)
+    x, y = particles(r, 50)
+    plane = ga.tsc_deposit_raw(x, y, torch.ones(50, dtype=torch.float64, device="cuda"), NX, NY, DX, DY)
+    assert abs(float(plane.sum()) - 50) < 1e-12
+    xi, yi = 2.3 * DX, 5.9 * DY            # one particle: weights 1/2 (1/2 - f)^2, 3/4 - f^2, 1/2 (1/2 + f)^2
+    one = ga.tsc_deposit_raw(torch.tensor([xi], dtype=torch.float64, device="cuda"),
+                             torch.tensor([yi], dtype=torch.float64, device="cuda"),
+                             torch.ones(1, dtype=torch.float64, device="cuda"), NX, NY, DX, DY).cpu().numpy()
+    def w(s):
+        i0 = int(np.floor(s + 0.5)); f = s - i0
+        return i0, [0.5 * (0.5 - f) ** 2, 0.75 - f * f, 0.5 * (0.5 + f) ** 2]
+    ix, wx = w(xi / DX); jy, wy = w(yi / DY)
+    want = np.zeros(NX * NY)
+    for v in range(3):
+        for u in range(3):
+            want[((jy - 1 + v) % NY) * NX + (ix - 1 + u) % NX] += wx[u] * wy[v]
+    np.testing.assert_allclose(one, want, rtol=0, atol=1e-15)
+
+
+def test_tsc_gradcheck():
+    """deposit and gather differentiated in the particle positions, the values and the plane: the hand-written
+    backward against torch's central differences"""
+    r = np.random.default_rng(23)
+    x, y = particles(r, 6)
+    q = torch.from_numpy(r.standard_normal(6)).cuda()
+    P = torch.from_numpy(r.standard_normal(NX * NY)).cuda()
+    for t in (x, y, q, P):
+        t.requires_grad_()
+    assert torch.autograd.gradcheck(lambda a, b, c: ga.TscDeposit.apply(a, b, c, NX, NY, DX, DY), (x, y, q),
+                                    eps=1e-6, atol=1e-8, rtol=1e-6)
+    assert torch.autograd.gradcheck(lambda a, b, c: ga.TscGather.apply(a, b, c, NX, NY, DX, DY), (x, y, P),
+                                    eps=1e-6, atol=1e-8, rtol=1e-6)
+
+
+def test_boris_vjp_gradcheck():
+    """the reverse Boris rotation against central differences of the forward one (every input: u, E, B, qm, dt),
+    and the forward against the oracle's boris bit for bit"""
+    r = np.random.default_rng(24)
+    n = 5
+    ins = [torch.from_numpy(r.standard_normal(n) * s).cuda().requires_grad_()
+           for s in (2.0, 2.0, 2.0, 0.5, 0.5, 0.5, 0.3, 0.3, 0.3)]
+    qm = torch.from_numpy(-np.ones(n) + 0.1 * r.standard_normal(n)).cuda().requires_grad_()
+    dt = torch.from_numpy(r.uniform(0.1, 0.5, n)).cuda().requires_grad_()
+    assert torch.autograd.gradcheck(ga.BorisFunction.apply, (*ins, qm, dt), eps=1e-6, atol=1e-8, rtol=1e-6)
+    import ctypes as C
+    out = ga.BorisFunction.apply(*ins, qm, dt)
+    for p in range(n):
+        u = np.array([float(t[p]) for t in ins[:3]])
+        E = np.array([float(t[p]) for t in ins[3:6]]); B = np.array([float(t[p]) for t in ins[6:9]])
+        orc.lib().or_boris(u.ctypes.data_as(C.c_void_p), E.ctypes.data_as(C.c_void_p), B.ctypes.data_as(C.c_void_p),
+                           float(qm[p]), float(dt[p]))
+        assert [float(o[p]) for o in out] == list(u)
