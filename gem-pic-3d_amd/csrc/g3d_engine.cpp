@@ -1003,6 +1003,9 @@ void Sim::march_plasma(double s0, double s1) {
     TileGeom tg = n_tiled > 0 ? tiles : TileGeom{};
     tg.idx = nullptr;
     tg.chunk = 0; tg.max_extra = 0; tg.xtile = nullptr; tg.xstart = nullptr; tg.xcount = nullptr;
+    // FP32-mixed: the tile windows go to the current planes with float atomics instead of slabs (3 % faster at
+    // 256 x 256: the row kernel then reads one plane set); G3D_STREAM_AFLUSH=0 keeps the slabs
+    { const char* e = getenv("G3D_STREAM_AFLUSH"); tg.aflush = mx && !(e && atoi(e) == 0); }
     // FP32-mixed: tiles denser than one workgroup's lanes are streamed in chunks (G3D_STREAM_CHUNK=0: off)
     const char* ech = getenv("G3D_STREAM_CHUNK");
     if (mx && tg.ntile && !(ech && atoi(ech) == 0)) {

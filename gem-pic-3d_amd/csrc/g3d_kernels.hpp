@@ -50,6 +50,7 @@ struct TileGeom {
     // further chunks of a tile, xtile/xstart[0 .. *xcount), run in max_extra more workgroups after the tiles and
     // add their window to the current planes with atomics
     int chunk, max_extra;
+    int aflush;                       // FP32-mixed: every window added to the current planes with atomics (no slabs)
     const int* xtile;
     const long* xstart;
     const int* xcount;

@@ -324,7 +324,7 @@ k_stream_mixed(StreamArgs c, float* J, float* slab, Parts S, Parts D, long n, Ti
             }
         }
         __syncthreads();
-        if (!extra) {
+        if (!extra && !tg.aflush) {
             float* sl = slab + t * long(wn);
             for (int r = threadIdx.x; r < wn; r += blockDim.x) sl[r] = float(dep[r]);
         } else
@@ -553,7 +553,9 @@ void pass_mixed(const ModeArgs& m, const FftPlan& p, float* J, const float* slab
     if (cm && (p.ny > kColMax || p.ny < 2)) throw std::runtime_error("plasma pass: column kernel needs 2 <= ny <= 1024");
     if (kind != 1) {
         const size_t lds = (size_t(2 * rpw) * (p.nx + p.nx / 8) + p.nx / 2) * sizeof(cxf);
-        hipLaunchKernelGGL(k_rows_fwd_mixed, dim3(p.ny / rpw), dim3(lanes(rpw * p.nx)), lds, stream(), J, slab, tg, p.nx,
+        TileGeom tr = tg;
+        if (tg.aflush) tr.ntile = 0;      // the windows are already in J
+        hipLaunchKernelGGL(k_rows_fwd_mixed, dim3(p.ny / rpw), dim3(lanes(rpw * p.nx)), lds, stream(), J, slab, tr, p.nx,
                            p.ny, p.lx, rpw, p.twx, X, rl, rli, i0, j0, cm);
     }
     const size_t li = (size_t(3 * rpw) * (p.nx + p.nx / 8) + p.nx / 2) * sizeof(cxf);
