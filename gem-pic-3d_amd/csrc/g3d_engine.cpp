@@ -966,13 +966,13 @@ void Sim::march_plasma(double s0, double s1) {
     const SpecGeom g = geom();
     jpred.zero();
     jsr.zero();
-    // the FP32-mixed pass: Fourier y, and no ion species in the list (ions and ionization stay FP64).  Its modes
-    // run column-major (one fused column launch per pass) unless G3D_PASS_CM=0
+    // the FP32-mixed pass: Fourier y, and no ion species in the list (ions and ionization stay FP64).  With Fourier
+    // y the modes of either form run column-major (one fused column launch per pass) unless G3D_PASS_CM=0
     bool ions = false;
     for (const auto& is : cfg.ions) ions = ions || cfg.loaded(is);
     const bool mx = cfg.mixed && yf && !ions;
     const char* ecm = getenv("G3D_PASS_CM");
-    const bool cm = mx && !(ecm && atoi(ecm) == 0) && cfg.ny <= 1024 && cfg.nx % 16 == 0 && cfg.ny % 16 == 0;
+    const bool cm = yf && !(ecm && atoi(ecm) == 0) && cfg.ny <= 1024 && cfg.nx % 16 == 0 && cfg.ny % 16 == 0;
     ensure_layout(cm, cm);
     ModeArgs m{};
     m.mv = mview(D);
@@ -988,13 +988,14 @@ void Sim::march_plasma(double s0, double s1) {
     static const bool fused = [] { const char* e = getenv("G3D_PASS_FUSED"); return e && atoi(e) > 0; }();
     auto pass_work = [&](cx* X, int kind) {
         if (!yf) pass_lines(m, L, X, tp3.data(), nullptr, kind);
+        else if (cm) pass_colmodes(m, fft, X, tp3.data(), kind);
         else if (fused) pass_cols(m, fft, X, tp3.data(), nullptr, kind);
         else pass_modes(m, fft, X, tp3.data(), nullptr, kind);
     };
     const long fpl = long(ns + 2) * nk;
     auto real_fields = [&](int n) {
         tic(F_IFFT);
-        pass_rows_inv(tp3.data(), F.data() + long(n - a) * nk, fpl, fft);
+        pass_rows_inv(tp3.data(), F.data() + long(n - a) * nk, fpl, fft, cm);
         toc(F_IFFT);
     };
     StreamArgs sc = stream_args(s0, s1);
@@ -1080,7 +1081,7 @@ void Sim::march_plasma(double s0, double s1) {
             launch_timed(F_STREAM, lt);
             tic(F_SFFT);
             pass_rows_fwd(jsr.data(), tg, fft, sp3.data(), last ? rho_lines.data() + long(n) * (nx + ny) : nullptr,
-                          last && rho_lines_i.size() ? rho_lines_i.data() + long(n) * (nx + ny) : nullptr, i0, j0);
+                          last && rho_lines_i.size() ? rho_lines_i.data() + long(n) * (nx + ny) : nullptr, i0, j0, cm);
             toc(F_SFFT);
             tic(F_MODES);
             pass_work(sp3.data(), 0);

@@ -74,6 +74,13 @@ __device__ __forceinline__ void load_twiddles(C* tws, const cx* tw, int hn, bool
 
 __device__ __forceinline__ int padded(int n) { return n + (n >> 3); }
 
+// The row block of workgroup b of nb when the row blocks of one XCD are to be consecutive: workgroup b runs on XCD
+// b mod 8, so block (b mod 8) nb/8 + b/8 puts the blocks [x nb/8, (x + 1) nb/8) on XCD x, and the transposed
+// 8-byte accesses of neighbouring rows (mode ix ny + j) meet in that XCD's L2 as whole lines instead of partial
+// ones (measured at 256 x 256: the rows-forward kernel writes 2.9 MB instead of 7.6 MB)
+__device__ __forceinline__ int row_block(int b, int nb, int xcd) {
+    return (xcd && (nb & 7) == 0) ? (b & 7) * (nb >> 3) + (b >> 3) : b;
+}
 
 }  // namespace fftcore
 }  // namespace g3d

@@ -170,9 +170,14 @@ struct ModeArgs {
 };
 // the fused kernels of a plasma pass (g3d_plasma.hip): rows forward (jsr + slabs -> X, 3 planes), columns
 // (kind 0 pass, 1 front slice, 2 spectra only into jsn; X -> T, 3 planes), rows inverse (T -> F planes of a slice)
-void pass_rows_fwd(double* J, const TileGeom& tg, const FftPlan& p, cx* X, double* rl, double* rli, int i0, int j0);
+void pass_rows_fwd(double* J, const TileGeom& tg, const FftPlan& p, cx* X, double* rl, double* rli, int i0, int j0,
+                   int tr = 0);
 void pass_cols(const ModeArgs& m, const FftPlan& p, const cx* X, cx* T, cx* jsn, int kind);
-void pass_rows_inv(const cx* T, float* Fn, long fpl, const FftPlan& p);
+void pass_rows_inv(const cx* T, float* Fn, long fpl, const FftPlan& p, int tr = 0);
+// tr = 1: X written and T read column-major (mode ix ny + j), for pass_colmodes: the y transforms, the per-mode
+// work and the inverse y transforms in one launch, one workgroup per k_x column (needs m.mv.cm, 2 <= ny <= 1024)
+void pass_colmodes(const ModeArgs& m, const FftPlan& p, const cx* X, cx* T, int kind);
+int xcd_rows();                     // the row blocks of one XCD consecutive in the column-major row kernels
 int rows_per_wg(int ny);            // rows per workgroup of the row kernels
 // the FP32-mixed plasma pass (g3d_mixed.hip): the streaming half-cell with FP32 deposit (jsr and slabs float), and
 // the field pass of slice m.n (rows forward, y transforms, per-mode work, inverse transforms, F); X, T: 3 planes

@@ -347,14 +347,14 @@ k_stream_mixed(StreamArgs c, float* J, float* slab, Parts S, Parts D, long n, Ti
 // tr: X column-major (element (kx, j) at kx ny + j), for the fused column kernel
 __global__ void __launch_bounds__(kT) k_rows_fwd_mixed(float* J, const float* slab, TileGeom tg, int nx, int ny, int lx,
                                                        int rpw, const cx* twx, cxf* X, double* rl, double* rli, int i0,
-                                                       int j0, int tr) {
+                                                       int j0, int tr, int xcd) {
     extern __shared__ cxf smf[];
     const int sp = padded(nx);
     cxf* buf = smf;
     cxf* tws = smf + size_t(2 * rpw) * sp;
     load_twiddles(tws, twx, nx >> 1, false);
     const long nk = long(nx) * ny;
-    const int row0 = blockIdx.x * rpw;
+    const int row0 = row_block(blockIdx.x, gridDim.x, xcd) * rpw;
     const int tot = rpw * nx;
     for (int e = threadIdx.x; e < tot; e += blockDim.x) {
         const int r = e / nx, ixx = e - r * nx, j = row0 + r;
@@ -451,14 +451,14 @@ __global__ void __launch_bounds__(kT) k_modes_mixed(ModeArgs m, const cxf* X, cx
 
 // ---------------------------------------------------------------- rows inverse (FP32) and F of the slice
 __global__ void __launch_bounds__(kT) k_rows_inv_mixed(const cxf* T, float* Fn, long fpl, int nx, int ny, int lx,
-                                                       int rpw, const cx* twx, float scale, int tr) {
+                                                       int rpw, const cx* twx, float scale, int tr, int xcd) {
     extern __shared__ cxf smf[];
     const int sp = padded(nx);
     cxf* buf = smf;
     cxf* tws = smf + size_t(3 * rpw) * sp;
     load_twiddles(tws, twx, nx >> 1, true);
     const long nk = long(nx) * ny;
-    const int row0 = blockIdx.x * rpw;
+    const int row0 = row_block(blockIdx.x, gridDim.x, xcd) * rpw;
     const int tot = rpw * nx;
     for (int e = threadIdx.x; e < 3 * tot; e += blockDim.x) {
         const int k = e / tot, rem = e - k * tot, r = rem / nx, ixx = rem - r * nx;
@@ -556,7 +556,7 @@ void pass_mixed(const ModeArgs& m, const FftPlan& p, float* J, const float* slab
         TileGeom tr = tg;
         if (tg.aflush) tr.ntile = 0;      // the windows are already in J
         hipLaunchKernelGGL(k_rows_fwd_mixed, dim3(p.ny / rpw), dim3(lanes(rpw * p.nx)), lds, stream(), J, slab, tr, p.nx,
-                           p.ny, p.lx, rpw, p.twx, X, rl, rli, i0, j0, cm);
+                           p.ny, p.lx, rpw, p.twx, X, rl, rli, i0, j0, cm, cm && xcd_rows());
     }
     const size_t li = (size_t(3 * rpw) * (p.nx + p.nx / 8) + p.nx / 2) * sizeof(cxf);
     if (cm) {       // modes column-major: one fused launch between the row transforms
@@ -565,7 +565,7 @@ void pass_mixed(const ModeArgs& m, const FftPlan& p, float* J, const float* slab
         hipLaunchKernelGGL(kc, dim3(unsigned(p.nx)), dim3(unsigned(p.ny)), lm, stream(), m, X, T, p.ny, p.ly, p.twy,
                            kind);
         hipLaunchKernelGGL(k_rows_inv_mixed, dim3(p.ny / rpw), dim3(lanes(rpw * p.nx)), li, stream(), T, Fn, fpl, p.nx,
-                           p.ny, p.lx, rpw, p.twx, float(1.0 / (double(p.nx) * p.ny)), 1);
+                           p.ny, p.lx, rpw, p.twx, float(1.0 / (double(p.nx) * p.ny)), 1, xcd_rows());
         launch_check("plasma pass (mixed, column-major modes)");
         return;
     }
@@ -582,7 +582,7 @@ void pass_mixed(const ModeArgs& m, const FftPlan& p, float* J, const float* slab
     hipLaunchKernelGGL(k_cols_mixed, dim3(unsigned(3 * (p.nx / nc))), dim3(tc), lc, stream(), T, p.nx, p.ny, p.ly, nc,
                        p.twy, 1);
     hipLaunchKernelGGL(k_rows_inv_mixed, dim3(p.ny / rpw), dim3(lanes(rpw * p.nx)), li, stream(), T, Fn, fpl, p.nx, p.ny,
-                       p.lx, rpw, p.twx, float(1.0 / (double(p.nx) * p.ny)), 0);
+                       p.lx, rpw, p.twx, float(1.0 / (double(p.nx) * p.ny)), 0, 0);
     launch_check("plasma pass (mixed)");
 }
 

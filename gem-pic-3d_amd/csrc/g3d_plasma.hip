@@ -26,15 +26,17 @@ constexpr int kT = 256;
 constexpr int kMaxModes = 4;          // modes per lane in k_pass_cols (nc * ny <= kMaxModes * kT)
 
 // ---------------------------------------------------------------- rows forward
+// tr: the spectra written column-major (mode ix ny + j) for k_colmodes; xcd: the row blocks of one XCD consecutive
 __global__ void __launch_bounds__(kT) k_pass_rows_fwd(double* J, TileGeom tg, int nx, int ny, int lx, int rpw,
-                                                     const cx* twx, cx* X, double* rl, double* rli, int i0, int j0) {
+                                                     const cx* twx, cx* X, double* rl, double* rli, int i0, int j0,
+                                                     int tr, int xcd) {
     extern __shared__ cx sm[];
     const int sp = padded(nx);
     cx* buf = sm;                                    // 2 sequences per row: j_x + i j_y, j_z
     cx* tws = sm + size_t(2 * rpw) * sp;
     load_twiddles(tws, twx, nx >> 1, false);
     const long nk = long(nx) * ny;
-    const int row0 = blockIdx.x * rpw;
+    const int row0 = row_block(blockIdx.x, gridDim.x, xcd) * rpw;
     const int tot = rpw * nx;
     for (int e = threadIdx.x; e < tot; e += blockDim.x) {
         const int r = e / nx, ixx = e - r * nx, j = row0 + r;
@@ -78,7 +80,7 @@ __global__ void __launch_bounds__(kT) k_pass_rows_fwd(double* J, TileGeom tg, in
     lds_fft(buf, nx, lx, 2 * rpw, sp, tws);
     for (int e = threadIdx.x; e < tot; e += blockDim.x) {
         const int r = e / nx, kx = e - r * nx, j = row0 + r;
-        const long i = long(j) * nx + kx;
+        const long i = tr ? long(kx) * ny + j : long(j) * nx + kx;
         const cx z = buf[(2 * r) * sp + lpos(kx)], zm = conjg(buf[(2 * r) * sp + lpos((nx - kx) & (nx - 1))]);
         const cx d = z - zm;
         X[i] = 0.5 * (z + zm);
@@ -176,20 +178,67 @@ __global__ void __launch_bounds__(kT) k_pass_modes(ModeArgs m, const cx* X, cx* 
     for (int k = 0; k < 3; ++k) T[k * nk + i] = t[k];
 }
 
+// Columns + modes + inverse columns with the modes column-major (mode ix ny + j, the layout k_pass_rows_fwd writes
+// with tr): one workgroup per k_x column, one lane per mode.  The lane issues every read of its mode's state first
+// (mode_load, in flight during the forward y transforms), then the same operations as k_fft_cols + k_pass_modes +
+// k_fft_cols in the same order (the same transform core and twiddles), so the results are those of the split pass.
+constexpr int kColMax = 1024;
+template <int BS>
+__global__ void __launch_bounds__(BS) k_colmodes(ModeArgs m, const cx* X, cx* T, int ny, int ly, const cx* twy, int kind) {
+    extern __shared__ cx sm[];
+    const int sp = padded(ny);
+    cx* buf = sm;                                    // the three planes of the column
+    cx* twf = sm + 3 * sp;
+    cx* twi = twf + (ny >> 1);
+    load_twiddles(twf, twy, ny >> 1, false);
+    load_twiddles(twi, twy, ny >> 1, true);
+    const long nk = m.mv.nk;
+    const int kx = blockIdx.x;
+    const long c0 = long(kx) * ny;
+    const int jj = threadIdx.x;                      // this lane's mode (kx, jj)
+    const long i = c0 + jj;
+    ModeLoad L;
+    if constexpr (BS <= 256) L = mode_load<false>(m, i);
+    if (kind != 1)
+        for (int e = jj; e < 3 * ny; e += blockDim.x) {
+            const int p = e / ny, j = e - p * ny;
+            buf[p * sp + lpos(bitrev(j, ly))] = X[p * nk + c0 + j];
+        }
+    __syncthreads();
+    if (kind != 1) lds_fft(buf, ny, ly, 3, sp, twf);
+    cx u[3] = {mk(0), mk(0), mk(0)};
+    if (kind != 1) {
+        const double k2 = m.kxs[kx] * m.kxs[kx] + m.kys[jj] * m.kys[jj];
+        const double f = (kx == 0 && jj == 0 && !m.keep0) ? 0.0 : m.filt[kx] * m.filty[jj] / (1 + m.a2 * k2);
+        for (int k = 0; k < 3; ++k) u[k] = f * buf[k * sp + lpos(jj)];
+    }
+    cx t[3];
+    if constexpr (BS <= 256) mode_compute<false>(m, i, L, u, t);
+    else mode_work<false>(m, i, u, t);
+    __syncthreads();                                 // every lane has read its X before T overwrites the buffer
+    const int b = lpos(bitrev(jj, ly));
+    for (int k = 0; k < 3; ++k) buf[k * sp + b] = t[k];
+    __syncthreads();
+    lds_fft(buf, ny, ly, 3, sp, twi);
+    for (int k = 0; k < 3; ++k) T[k * nk + i] = buf[k * sp + lpos(jj)];
+}
+
 // ---------------------------------------------------------------- rows inverse and the real fields of slice n
+// tr: T column-major (mode ix ny + j); xcd as k_pass_rows_fwd
 __global__ void __launch_bounds__(kT) k_pass_rows_inv(const cx* T, float* Fn, long fpl, int nx, int ny, int lx, int rpw,
-                                                     const cx* twx, double scale) {
+                                                     const cx* twx, double scale, int tr, int xcd) {
     extern __shared__ cx sm[];
     const int sp = padded(nx);
     cx* buf = sm;                                    // 3 sequences per row: R, L, E_z/B_z packed
     cx* tws = sm + size_t(3 * rpw) * sp;
     load_twiddles(tws, twx, nx >> 1, true);
     const long nk = long(nx) * ny;
-    const int row0 = blockIdx.x * rpw;
+    const int row0 = row_block(blockIdx.x, gridDim.x, xcd) * rpw;
     const int tot = rpw * nx;
     for (int e = threadIdx.x; e < 3 * tot; e += blockDim.x) {
         const int k = e / tot, rem = e - k * tot, r = rem / nx, ixx = rem - r * nx;
-        buf[(3 * r + k) * sp + lpos(bitrev(ixx, lx))] = T[k * nk + long(row0 + r) * nx + ixx];
+        const long i = tr ? long(ixx) * ny + row0 + r : long(row0 + r) * nx + ixx;
+        buf[(3 * r + k) * sp + lpos(bitrev(ixx, lx))] = T[k * nk + i];
     }
     __syncthreads();
     lds_fft(buf, nx, lx, 3 * rpw, sp, tws);
@@ -227,13 +276,26 @@ int rows_per_wg(int ny) {
     while (r < 8 && ny / (2 * r) >= 256) r <<= 1;
     return r;
 }
-void pass_rows_fwd(double* J, const TileGeom& tg, const FftPlan& p, cx* X, double* rl, double* rli, int i0, int j0) {
+// the row blocks of one XCD consecutive in the row kernels whose spectra are column-major (G3D_XCD_ROWS=0: off)
+int xcd_rows() {
+    static const int on = [] { const char* e = getenv("G3D_XCD_ROWS"); return e ? atoi(e) : 1; }();
+    return on;
+}
+void pass_rows_fwd(double* J, const TileGeom& tg, const FftPlan& p, cx* X, double* rl, double* rli, int i0, int j0,
+                   int tr) {
     const int rpw = rows_per_wg(p.ny);
     if (p.ny % rpw) throw std::runtime_error("pass rows: ny not a multiple of the row tile");
     const size_t lds = (size_t(2 * rpw) * (p.nx + p.nx / 8) + p.nx / 2) * sizeof(cx);
     hipLaunchKernelGGL(k_pass_rows_fwd, dim3(p.ny / rpw), dim3(lanes(rpw * p.nx)), lds, stream(), J, tg, p.nx, p.ny, p.lx,
-                       rpw, p.twx, X, rl, rli, i0, j0);
+                       rpw, p.twx, X, rl, rli, i0, j0, tr, tr && xcd_rows());
     launch_check("pass rows forward");
+}
+void pass_colmodes(const ModeArgs& m, const FftPlan& p, const cx* X, cx* T, int kind) {
+    if (p.ny > kColMax || p.ny < 2 || !m.mv.cm) throw std::runtime_error("plasma pass: column kernel needs 2 <= ny <= 1024, modes column-major");
+    const size_t lds = (size_t(3) * (p.ny + p.ny / 8) + p.ny) * sizeof(cx);
+    auto* kc = p.ny <= 256 ? k_colmodes<256> : k_colmodes<kColMax>;
+    hipLaunchKernelGGL(kc, dim3(unsigned(p.nx)), dim3(unsigned(p.ny)), lds, stream(), m, X, T, p.ny, p.ly, p.twy, kind);
+    launch_check("plasma pass (column-major modes)");
 }
 void pass_cols(const ModeArgs& m, const FftPlan& p, const cx* X, cx* T, cx* jsn, int kind) {
     const int nc = cols_per_wg(p.nx, p.ny);
@@ -249,11 +311,11 @@ void pass_modes(const ModeArgs& m, const FftPlan& p, cx* X, cx* T, cx* jsn, int 
     launch_check("pass modes");
     if (kind != 2) fft_cols(p, T, 3, true, 1.0);
 }
-void pass_rows_inv(const cx* T, float* Fn, long fpl, const FftPlan& p) {
+void pass_rows_inv(const cx* T, float* Fn, long fpl, const FftPlan& p, int tr) {
     const int rpw = rows_per_wg(p.ny);
     const size_t lds = (size_t(3 * rpw) * (p.nx + p.nx / 8) + p.nx / 2) * sizeof(cx);
     hipLaunchKernelGGL(k_pass_rows_inv, dim3(p.ny / rpw), dim3(lanes(rpw * p.nx)), lds, stream(), T, Fn, fpl, p.nx, p.ny,
-                       p.lx, rpw, p.twx, p.two_d ? 1.0 / (double(p.nx) * p.ny) : 1.0 / p.nx);
+                       p.lx, rpw, p.twx, p.two_d ? 1.0 / (double(p.nx) * p.ny) : 1.0 / p.nx, tr, tr && xcd_rows());
     launch_check("pass rows inverse");
 }
 

@@ -82,9 +82,11 @@ def test_plasma_init():
     compare_fields(gpu, ref)
 
 
-@pytest.mark.parametrize("passes,extrap", [(2, 1), (1, 1), (2, 0)])
-def test_plasma_steps(passes, extrap):
-    """init + 3 steps of the streaming-plasma march (src/sim_plasma.cpp:228-470)."""
+@pytest.mark.parametrize("passes,extrap,layout", [(2, 1, "cm"), (1, 1, "cm"), (2, 0, "cm"), (2, 1, "split")])
+def test_plasma_steps(passes, extrap, layout, monkeypatch):
+    """init + 3 steps of the streaming-plasma march (src/sim_plasma.cpp:228-470), with the modes column-major and
+    one fused column launch per pass (cm), and with the reference's layout and the split column pass (split)"""
+    monkeypatch.setenv("G3D_PASS_CM", "1" if layout == "cm" else "0")
     gpu, ref = pair(plasma_deck, passes=passes, extrap=extrap)
     gpu.init(); ref.init()
     for _ in range(3):
