@@ -236,3 +236,95 @@ class BorisFunction(torch.autograd.Function):
             setattr(g, n, t.data_ptr())
         _call(g3d_amd.lib().g3d_boris_vjp, C.byref(a), C.byref(g))
         return tuple(outs)
+
+
+# ---------------------------------------------------------------------------------------------- streaming push
+_STATE = ("x", "y", "zeta", "z", "px", "py", "pz")
+
+
+class SpushCtx(C.Structure):
+    """g3d_spush_ctx (include/g3d.h)"""
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("dx", C.c_double), ("dy", C.c_double), ("zeta0", C.c_double),
+                ("ihz", C.c_double), ("n", C.c_int), ("zn", C.c_double), ("zn1", C.c_double),
+                ("Delta", C.c_double), ("s1", C.c_double), ("sTrap", C.c_double), ("vz_beam", C.c_double),
+                ("ia", C.c_double), ("F0", C.c_void_p), ("F1", C.c_void_p)] + \
+               [(n, C.c_void_p) for n in _STATE] + \
+               [("q", C.c_void_p), ("qm", C.c_void_p), ("w", C.c_void_p), ("alive", C.c_void_p), ("J", C.c_void_p),
+                ("np", C.c_long)]
+
+
+class SpushAdj(C.Structure):
+    """g3d_spush_adj (include/g3d.h)"""
+    _fields_ = [("b" + n, C.c_void_p) for n in _STATE] + [("bJ", C.c_void_p)] + \
+               [("g" + n, C.c_void_p) for n in _STATE] + [("gF0", C.c_void_p), ("gF1", C.c_void_p)]
+
+
+class HalfCell:
+    """The half-cell and grid of a streaming push: a periodic nx x ny grid of steps dx, dy; slices n and n + 1 of a
+    uniform ζ axis (slice k at zeta0 + k h); the step length Delta, its end s1, the trapping limit sTrap, the beam
+    threshold vz_beam and the inverse cell-area factor ia of the deposit (src/sim_stream.hpp:20-29)"""
+    def __init__(self, nx, ny, dx, dy, zeta0, h, n, Delta, s1, sTrap, vz_beam, ia):
+        self.nx, self.ny, self.dx, self.dy = nx, ny, dx, dy
+        self.zeta0, self.h, self.n = zeta0, h, n
+        self.Delta, self.s1, self.sTrap, self.vz_beam, self.ia = Delta, s1, sTrap, vz_beam, ia
+
+    def ctx(self, F0, F1, state, q, qm, w, alive, J):
+        c = SpushCtx()
+        c.nx, c.ny, c.dx, c.dy = self.nx, self.ny, self.dx, self.dy
+        c.zeta0, c.ihz, c.n = self.zeta0, 1.0 / self.h, self.n
+        c.zn, c.zn1 = self.zeta0 + self.n * self.h, self.zeta0 + (self.n + 1) * self.h
+        c.Delta, c.s1, c.sTrap, c.vz_beam, c.ia = self.Delta, self.s1, self.sTrap, self.vz_beam, self.ia
+        c.F0, c.F1 = F0.data_ptr(), F1.data_ptr()
+        for nm, t in zip(_STATE, state):
+            setattr(c, nm, t.data_ptr())
+        c.q, c.qm, c.w, c.alive = q.data_ptr(), qm.data_ptr(), w.data_ptr(), alive.data_ptr()
+        c.J = J.data_ptr() if J is not None else None
+        c.np = state[0].numel()
+        return c
+
+
+def stream_push_raw(hc, F0, F1, state, q, qm, w, alive):
+    """the pushed state (new tensors), the alive codes after the push and the deposit J [5][ny nx]"""
+    out = [t.clone() for t in state]
+    al = alive.clone()
+    J = torch.zeros(5 * hc.nx * hc.ny, dtype=torch.float64, device=F0.device)
+    c = hc.ctx(F0, F1, out, q, qm, w, al, J)
+    _call(g3d_amd.lib().g3d_stream_push, C.byref(c))
+    return out, al, J
+
+
+class StreamPush(torch.autograd.Function):
+    """(x, y, ζ, z, p_x, p_y, p_z) of the electrons after half-cell n, and their deposit J [5 ny nx], as functions of
+    the state before it and of the fields F0, F1 [6 ny nx] of slices n and n + 1 (alive codes, q, q/m and w held
+    fixed)"""
+    @staticmethod
+    def forward(ctx, hc, F0, F1, x, y, zeta, z, px, py, pz, q, qm, w, alive):
+        state = [_f64(t, "state") for t in (x, y, zeta, z, px, py, pz)]
+        F0, F1 = _f64(F0, "F0"), _f64(F1, "F1")
+        q, qm, w = _f64(q, "q"), _f64(qm, "qm"), _f64(w, "w")
+        ctx.hc = hc
+        ctx.alive = alive.contiguous()
+        ctx.save_for_backward(F0, F1, q, qm, w, *state)
+        out, _, J = stream_push_raw(hc, F0, F1, state, q, qm, w, ctx.alive)
+        return (*out, J)
+
+    @staticmethod
+    def backward(ctx, *grads):
+        F0, F1, q, qm, w, *state = ctx.saved_tensors
+        hc = ctx.hc
+        zeros = torch.zeros_like(state[0])
+        bs = [(g if g is not None else zeros).contiguous() for g in grads[:7]]
+        bJ = (grads[7] if grads[7] is not None else torch.zeros(5 * hc.nx * hc.ny, dtype=torch.float64,
+                                                                 device=F0.device)).contiguous()
+        gs = [torch.empty_like(state[0]) for _ in _STATE]
+        gF0, gF1 = torch.zeros_like(F0), torch.zeros_like(F1)
+        c = hc.ctx(F0, F1, list(state), q, qm, w, ctx.alive, None)
+        g = SpushAdj()
+        for nm, t in zip(_STATE, bs):
+            setattr(g, "b" + nm, t.data_ptr())
+        g.bJ = bJ.data_ptr()
+        for nm, t in zip(_STATE, gs):
+            setattr(g, "g" + nm, t.data_ptr())
+        g.gF0, g.gF1 = gF0.data_ptr(), gF1.data_ptr()
+        _call(g3d_amd.lib().g3d_stream_push_vjp, C.byref(c), C.byref(g))
+        return (None, gF0, gF1, *gs, None, None, None, None)

@@ -183,6 +183,39 @@ typedef struct {
 int g3d_boris(const g3d_boris_args* a);
 int g3d_boris_vjp(const g3d_boris_args* a, const g3d_boris_adj* g);
 
+/* One streaming electron through half-cell n (from slice n at zn towards slice n + 1 at zn1) on a periodic Fourier-y
+ * grid: the push of src/sim_stream.hpp:104-171 for an electron with no ionization and no analytic laser (the gather
+ * at the midpoint, the two Boris rotations, the trapping branch, the TSC deposit of the current and, for a trapped
+ * electron, of its beam current), with the fields of the two slices given in FP64.  g3d_stream_push runs it in
+ * place on the state and adds the deposit to J; g3d_stream_push_vjp is its reverse: given the adjoints of the
+ * pushed state and of J, the adjoints of the state before the push (the state arrays must still hold it) and of the
+ * two field planes (added to).  The branch decisions (alive, trapped, the sign of the charge) are those of the
+ * forward push and are held fixed.  The reverse has no counterpart in the reference. */
+typedef struct {
+    int nx, ny;
+    double dx, dy;                 /* periodic grid: Lx = nx dx, Ly = ny dy; node (ix, j) at (ix dx, j dy) */
+    double zeta0, ihz;             /* uniform ζ axis: slice k at zeta0 + k / ihz */
+    int n;                         /* the half-cell */
+    double zn, zn1;                /* ζ of slices n and n + 1 */
+    double Delta, s1, sTrap, vz_beam, ia;   /* step length, end of the step, trapping limit, beam threshold,
+                                               inverse cell-area factor of the deposit */
+    const double* F0;              /* E_x E_y E_z B_x B_y B_z of slice n, [6][ny][nx] */
+    const double* F1;              /* the same of slice n + 1 */
+    double *x, *y, *zeta, *z, *px, *py, *pz;
+    const double *q, *qm, *w;
+    int* alive;                    /* 1: streaming (pushed); 5: trapped by this push; other codes are left alone */
+    double* J;                     /* j_x j_y j_z rho- rho+, [5][ny][nx], added to (forward) */
+    long np;
+} g3d_spush_ctx;
+typedef struct {
+    const double *bx, *by, *bzeta, *bz, *bpx, *bpy, *bpz;   /* in: adjoints of the pushed state */
+    const double* bJ;                                       /* in: adjoint of J, [5][ny][nx] */
+    double *gx, *gy, *gzeta, *gz, *gpx, *gpy, *gpz;         /* out: adjoints of the state before the push */
+    double *gF0, *gF1;                                      /* out: adjoints of F0, F1 (added to) */
+} g3d_spush_adj;
+int g3d_stream_push(const g3d_spush_ctx* c);
+int g3d_stream_push_vjp(const g3d_spush_ctx* c, const g3d_spush_adj* g);
+
 /* ------------------------------------------------------------------ engine: the Simulation of the march
  * g3d_sim mirrors the reference's Simulation (src/sim.hpp:222-360) on one GPU: init() and step() and the
  * member functions of the five drop-in files, each over the kernels of this library. */

@@ -186,3 +186,62 @@ def test_boris_vjp_gradcheck():
         orc.lib().or_boris(u.ctypes.data_as(C.c_void_p), E.ctypes.data_as(C.c_void_p), B.ctypes.data_as(C.c_void_p),
                            float(qm[p]), float(dt[p]))
         assert [float(o[p]) for o in out] == list(u)
+
+
+# ------------------------------------------------------------------------------ the streaming push
+def spush_setup():
+    """the golden stream input (64 electrons of half-cell STREAM_N of the 8 x 8 x 12 deck, random fields) as
+    FP64 planes of slices n, n + 1 and the state tensors"""
+    deck = gi.small_deck()
+    sim = g3d_amd.Sim(deck)
+    sim.init()
+    F, P = gi.stream_input(sim)
+    info = sim.info()
+    ns, nk = info["ns"], info["nk"]
+    n = gi.STREAM_N
+    Fv = F.reshape(6, ns + 2, nk).astype(np.float64)
+    g = orc.Grid(deck)
+    c = deck.c
+    hc = ga.HalfCell(g.nx, g.ny, g.dx, g.dy, 0.0, 1 / g.ih, n, 5.0, 5.0, 5.0 + (c.trap_advance - 1) * 5.0, c.vz_beam,
+                     g.iA[0])
+    return deck, sim, F, P, hc, gpu(Fv[:, n].ravel()), gpu(Fv[:, n + 1].ravel()), info
+
+
+def test_stream_push_against_oracle():
+    """the forward streaming push (g3d_stream_push) against the oracle's stream pass on the golden input: the state
+    and alive codes of every electron (waiting, entering, trapped, dead), the deposit to 1e-13 of its largest value"""
+    deck, sim, F, P, hc, F0, F1, info = spush_setup()
+    want, J_want = orc.stream_halfcell(orc.Grid(deck), deck, F, P, gi.STREAM_N, 0.0, 5.0, info["X0"], info["Y0"])
+    state = [gpu(P[f].astype(np.float64)) for f in ("x", "y", "zeta", "z", "px", "py", "pz")]
+    out, alive, J = ga.stream_push_raw(hc, F0, F1, state, gpu(P["q"]), gpu(P["qm"]), gpu(P["w"]),
+                                       gpu(P["alive"].astype(np.int32)))
+    for f, t in zip(("x", "y", "zeta", "z", "px", "py", "pz"), out):
+        got = t.cpu().numpy()
+        assert np.allclose(got, want[f], rtol=1e-14, atol=1e-14), (f, np.abs(got - want[f]).max())
+    assert np.array_equal(alive.cpu().numpy(), want["alive"])
+    J = J.cpu().numpy()
+    assert np.abs(J - J_want).max() <= 1e-13 * np.abs(J_want).max()
+
+
+def test_stream_push_gradcheck():
+    """the reverse streaming push against central differences of the forward one, in every input (the state and
+    both field planes): electrons inside the half-cell (ζ between the slices), a fast one that is trapped in the
+    push and deposits its beam current, slow ones with a strong transverse kick"""
+    deck, sim, F, P, hc, F0, F1, info = spush_setup()
+    r = np.random.default_rng(31)
+    m = 5
+    zn, h = hc.zeta0 + hc.n * hc.h, hc.h
+    x = gpu(r.uniform(0, hc.nx * hc.dx, m)); y = gpu(r.uniform(0, hc.ny * hc.dy, m))
+    zeta = gpu(zn + h * r.uniform(0.2, 0.8, m)); z = gpu(r.uniform(0.0, 1.0, m))
+    px = gpu(0.3 * r.standard_normal(m)); py = gpu(0.3 * r.standard_normal(m)); pz = gpu(0.2 * r.standard_normal(m))
+    pz[0] = 30.0
+    ones = torch.ones(m, dtype=torch.float64, device="cuda")
+    q, qm, w = -ones, -ones, 1e-2 * ones
+    alive = torch.ones(m, dtype=torch.int32, device="cuda")
+    ins = [F0.clone().requires_grad_(), F1.clone().requires_grad_()] + \
+          [t.clone().requires_grad_() for t in (x, y, zeta, z, px, py, pz)]
+    out, al, _ = ga.stream_push_raw(hc, ins[0].detach(), ins[1].detach(), [t.detach() for t in ins[2:]], q, qm, w,
+                                    alive)
+    assert int(al[0]) == 5 and (al[1:] == 1).all()          # the fast one is trapped, the others stream
+    f = lambda *a: ga.StreamPush.apply(hc, *a, q, qm, w, alive)
+    assert torch.autograd.gradcheck(f, tuple(ins), eps=1e-6, atol=1e-8, rtol=1e-5)
