@@ -11,6 +11,7 @@
 // and of the deposit weights enters through the derivatives of the TSC weights.  Branches (entry of a not yet
 // streaming electron, trapping, the clamp of v_z, the sign of the charge) follow the values, as in the forward
 // push, and are held fixed.  No counterpart of the reverse in the reference.
+#include "g3d_dual.hpp"
 #include "g3d_kernels.hpp"
 #include "g3d_runtime.hpp"
 
@@ -19,31 +20,8 @@ namespace {
 constexpr int kT = 256;
 constexpr int kNd = 13;             // x y ζ z px py pz | Ex Ey Ez Bx By Bz
 
-// ---------------------------------------------------------------- dual numbers (forward-mode derivatives)
-struct Dn {
-    double v, d[kNd];
-};
-__device__ __forceinline__ Dn dconst(double v) { Dn r; r.v = v; for (int k = 0; k < kNd; ++k) r.d[k] = 0; return r; }
-__device__ __forceinline__ Dn dvar(double v, int k) { Dn r = dconst(v); r.d[k] = 1; return r; }
-__device__ __forceinline__ Dn operator+(const Dn& a, const Dn& b) { Dn r; r.v = a.v + b.v; for (int k = 0; k < kNd; ++k) r.d[k] = a.d[k] + b.d[k]; return r; }
-__device__ __forceinline__ Dn operator-(const Dn& a, const Dn& b) { Dn r; r.v = a.v - b.v; for (int k = 0; k < kNd; ++k) r.d[k] = a.d[k] - b.d[k]; return r; }
-__device__ __forceinline__ Dn operator*(const Dn& a, const Dn& b) { Dn r; r.v = a.v * b.v; for (int k = 0; k < kNd; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k]; return r; }
-__device__ __forceinline__ Dn operator/(const Dn& a, const Dn& b) { Dn r; r.v = a.v / b.v; for (int k = 0; k < kNd; ++k) r.d[k] = (a.d[k] - r.v * b.d[k]) / b.v; return r; }
-__device__ __forceinline__ Dn operator+(const Dn& a, double b) { Dn r = a; r.v = a.v + b; return r; }
-__device__ __forceinline__ Dn operator+(double a, const Dn& b) { Dn r = b; r.v = a + b.v; return r; }
-__device__ __forceinline__ Dn operator-(const Dn& a, double b) { Dn r = a; r.v = a.v - b; return r; }
-__device__ __forceinline__ Dn operator-(double a, const Dn& b) { Dn r; r.v = a - b.v; for (int k = 0; k < kNd; ++k) r.d[k] = -b.d[k]; return r; }
-__device__ __forceinline__ Dn operator*(const Dn& a, double b) { Dn r; r.v = a.v * b; for (int k = 0; k < kNd; ++k) r.d[k] = a.d[k] * b; return r; }
-__device__ __forceinline__ Dn operator*(double a, const Dn& b) { Dn r; r.v = a * b.v; for (int k = 0; k < kNd; ++k) r.d[k] = a * b.d[k]; return r; }
-__device__ __forceinline__ Dn operator/(const Dn& a, double b) { Dn r; r.v = a.v / b; for (int k = 0; k < kNd; ++k) r.d[k] = a.d[k] / b; return r; }
-__device__ __forceinline__ Dn operator/(double a, const Dn& b) { Dn r; r.v = a / b.v; for (int k = 0; k < kNd; ++k) r.d[k] = -r.v * b.d[k] / b.v; return r; }
-__device__ __forceinline__ Dn tsqrt(const Dn& a) { Dn r; r.v = sqrt(a.v); for (int k = 0; k < kNd; ++k) r.d[k] = a.d[k] * (0.5 / r.v); return r; }
-__device__ __forceinline__ double tsqrt(double a) { return sqrt(a); }
-__device__ __forceinline__ double val(double a) { return a; }
-__device__ __forceinline__ double val(const Dn& a) { return a.v; }
-template <typename T> __device__ __forceinline__ T cst(double v);
-template <> __device__ __forceinline__ double cst<double>(double v) { return v; }
-template <> __device__ __forceinline__ Dn cst<Dn>(double v) { return dconst(v); }
+using Dn = Dual<kNd>;
+
 // a periodic coordinate moved into [0, L): the same shift on the value and no change of its derivatives
 __device__ __forceinline__ double twrap(double x, double L) { return wrap_mod(x, L); }
 __device__ __forceinline__ Dn twrap(const Dn& x, double L) { return x + (wrap_mod(x.v, L) - x.v); }
@@ -208,10 +186,10 @@ __global__ void __launch_bounds__(kT) k_spush_vjp(g3d_spush_ctx c, g3d_spush_adj
     const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= c.np) return;
     const long nk = long(c.nx) * c.ny;
-    SP<Dn> p{dvar(c.x[i], 0), dvar(c.y[i], 1), dvar(c.zeta[i], 2), dvar(c.z[i], 3), dvar(c.px[i], 4),
-             dvar(c.py[i], 5), dvar(c.pz[i], 6)};
+    SP<Dn> p{dvar<kNd>(c.x[i], 0), dvar<kNd>(c.y[i], 1), dvar<kNd>(c.zeta[i], 2), dvar<kNd>(c.z[i], 3),
+             dvar<kNd>(c.px[i], 4), dvar<kNd>(c.py[i], 5), dvar<kNd>(c.pz[i], 6)};
     int alive = c.alive[i];
-    Dn S = dconst(0);
+    Dn S = dconst<kNd>(0);
     GatherPoint gp{};
     bool gathered = false;
     auto gather = [&](const Dn& xg, const Dn& yg, const Dn& zm, Dn E[3], Dn B[3]) {
@@ -220,7 +198,7 @@ __global__ void __launch_bounds__(kT) k_spush_vjp(g3d_spush_ctx c, g3d_spush_adj
         for (int comp = 0; comp < 6; ++comp) {
             double gx, gy, gzeta;
             const double s = gather_comp(c, gp, comp, &gx, &gy, &gzeta);
-            Dn r = dvar(s, 7 + comp);
+            Dn r = dvar<kNd>(s, 7 + comp);
             for (int k = 0; k < 7; ++k) r.d[k] = gx * xg.d[k] + gy * yg.d[k] + gzeta * zm.d[k];
             if (comp < 3) E[comp] = r; else B[comp - 3] = r;
         }
@@ -233,7 +211,7 @@ __global__ void __launch_bounds__(kT) k_spush_vjp(g3d_spush_ctx c, g3d_spush_adj
         for (int v = 0; v < 3; ++v)
             for (int u = 0; u < 3; ++u) {
                 const long k = long(jn[v]) * c.nx + ix[u];
-                Dn W = dconst(wx[u] * wn[v]);
+                Dn W = dconst<kNd>(wx[u] * wn[v]);
                 for (int t = 0; t < kNd; ++t) W.d[t] = dwx[u] * wn[v] * x.d[t] + wx[u] * dwn[v] * y.d[t];
                 const Dn ww = f * W * c.ia;
                 const double* bJ = a.bJ;
@@ -270,7 +248,8 @@ int g3d_stream_push(const g3d_spush_ctx* c) {
 }
 int g3d_stream_push_vjp(const g3d_spush_ctx* c, const g3d_spush_adj* g) {
     return g3d::guard([&] {
-        if (!c || !g || c->nx < 1 || c->ny < 1 || c->np < 0) throw std::invalid_argument("stream push vjp: bad context");
+        if (!c || !g || c->nx < 1 || c->ny < 1 || c->np < 0)
+            throw std::invalid_argument("stream push vjp: bad context");
         if (c->np == 0) return;
         hipLaunchKernelGGL(k_spush_vjp, dim3(blocks_for(c->np, kT)), dim3(kT), 0, stream(), *c, *g);
         launch_check("stream push vjp");

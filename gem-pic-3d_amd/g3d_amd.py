@@ -251,6 +251,7 @@ def _load():
         "g3d_tsc_deposit": ([P, P, P], C.c_int), "g3d_tsc_gather": ([P, P, P, P, P], C.c_int),
         "g3d_boris": ([P], C.c_int), "g3d_boris_vjp": ([P, P], C.c_int),
         "g3d_stream_push": ([P], C.c_int), "g3d_stream_push_vjp": ([P, P], C.c_int),
+        "g3d_bunch_push_piece": ([P], C.c_int), "g3d_bunch_push_vjp": ([P, P], C.c_int),
         "g3d_sim_create": ([P, C.POINTER(P)], C.c_int), "g3d_sim_destroy": ([P], C.c_int),
         "g3d_sim_info_get": ([P, P], C.c_int), "g3d_sim_init": ([P], C.c_int), "g3d_sim_step": ([P], C.c_int),
         "g3d_sim_call": ([P, C.c_char_p, C.POINTER(C.c_double), C.c_int], C.c_int),

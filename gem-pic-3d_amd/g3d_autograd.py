@@ -328,3 +328,85 @@ class StreamPush(torch.autograd.Function):
         g.gF0, g.gF1 = gF0.data_ptr(), gF1.data_ptr()
         _call(g3d_amd.lib().g3d_stream_push_vjp, C.byref(c), C.byref(g))
         return (None, gF0, gF1, *gs, None, None, None, None)
+
+
+# ---------------------------------------------------------------------------------------------- RK4 bunch push
+_BSTATE = ("x", "y", "zeta", "px", "py", "pz")
+
+
+class BpushCtx(C.Structure):
+    """g3d_bpush_ctx (include/g3d.h)"""
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nsl", C.c_int)] + \
+               [(n, C.c_double) for n in ("dx", "dy", "zeta0", "ihz")] + [("F", C.c_void_p)] + \
+               [("laser_on", C.c_int), ("laser_pol", C.c_int)] + \
+               [(n, C.c_double) for n in ("a0", "kL", "tau", "cep", "eta_c", "kx", "kz", "X0", "zc", "s0", "s1",
+                                          "dz_push", "push_phase", "hmin", "zhand", "zmax", "ylo", "yhi", "vz_min")] + \
+               [(n, C.c_void_p) for n in _BSTATE + ("z", "rem", "qm", "alive")] + [("np", C.c_long)]
+
+
+class BpushAdj(C.Structure):
+    """g3d_bpush_adj (include/g3d.h)"""
+    _fields_ = [("b" + n, C.c_void_p) for n in _BSTATE] + [("g" + n, C.c_void_p) for n in _BSTATE] + \
+               [("gF", C.c_void_p)]
+
+
+class PushGeometry:
+    """The grid, ζ axis, laser and step of an RK4 bunch push: a periodic nx x ny grid of steps dx, dy; nsl slices
+    at zeta0 + k h; the plane wave (laser_on, laser_pol, a0, kL, tau, cep, eta_c, kx, kz, as g3d_config) with the
+    window offset X0 and centre zc; the step s0 -> s1 with dz_push, push_phase, hmin, zhand, zmax, ylo, yhi, vz_min
+    (src/sim_kernels.hpp:10-17)"""
+    def __init__(self, **kw):
+        self.kw = kw
+
+    def ctx(self, F, state, z, rem, qm, alive):
+        c = BpushCtx()
+        kw = dict(self.kw)
+        c.ihz = 1.0 / kw.pop("h")
+        for k, v in kw.items():
+            setattr(c, k, v)
+        c.F = F.data_ptr()
+        for nm, t in zip(_BSTATE, state):
+            setattr(c, nm, t.data_ptr())
+        c.z, c.rem, c.qm, c.alive = z.data_ptr(), rem.data_ptr(), qm.data_ptr(), alive.data_ptr()
+        c.np = state[0].numel()
+        return c
+
+
+def bunch_push_raw(geo, F, state, z, rem, qm, alive):
+    """the pushed state (new tensors) and the new z, rem, alive"""
+    out = [t.clone() for t in state]
+    z2, rem2, al = z.clone(), rem.clone(), alive.clone()
+    c = geo.ctx(F, out, z2, rem2, qm, al)
+    _call(g3d_amd.lib().g3d_bunch_push_piece, C.byref(c))
+    return out, z2, rem2, al
+
+
+class BunchPush(torch.autograd.Function):
+    """(x, y, ζ, p_x, p_y, p_z) of beam particles after the RK4 push of a step, as functions of the state before it
+    and of the field planes F [6 nsl ny nx] (z, rem, q/m and the alive codes held fixed)"""
+    @staticmethod
+    def forward(ctx, geo, F, x, y, zeta, px, py, pz, z, rem, qm, alive):
+        state = [_f64(t, "state") for t in (x, y, zeta, px, py, pz)]
+        F = _f64(F, "F")
+        ctx.geo = geo
+        ctx.fixed = (z.contiguous(), rem.contiguous(), qm.contiguous(), alive.contiguous())
+        ctx.save_for_backward(F, *state)
+        out, _, _, _ = bunch_push_raw(geo, F, state, *ctx.fixed)
+        return tuple(out)
+
+    @staticmethod
+    def backward(ctx, *grads):
+        F, *state = ctx.saved_tensors
+        zeros = torch.zeros_like(state[0])
+        bs = [(g if g is not None else zeros).contiguous() for g in grads]
+        gs = [torch.empty_like(state[0]) for _ in _BSTATE]
+        gF = torch.zeros_like(F)
+        c = ctx.geo.ctx(F, list(state), *ctx.fixed)
+        g = BpushAdj()
+        for nm, t in zip(_BSTATE, bs):
+            setattr(g, "b" + nm, t.data_ptr())
+        for nm, t in zip(_BSTATE, gs):
+            setattr(g, "g" + nm, t.data_ptr())
+        g.gF = gF.data_ptr()
+        _call(g3d_amd.lib().g3d_bunch_push_vjp, C.byref(c), C.byref(g))
+        return (None, gF, *gs, None, None, None, None)

@@ -216,6 +216,35 @@ typedef struct {
 int g3d_stream_push(const g3d_spush_ctx* c);
 int g3d_stream_push_vjp(const g3d_spush_ctx* c, const g3d_spush_adj* g);
 
+/* The RK4 push in z of beam particles from s0 (alive 1; alive 3: from s1 - rem) to s1 (src/sim_kernels.hpp:75-149,
+ * src/sim_particles.cpp:6-24): the adaptive sub-steps, the gather at every stage from the slices of a uniform ζ
+ * axis on a periodic Fourier-y grid (fields in FP64), the analytic plane-wave laser, the hand-over, slow and lost
+ * particle codes.  g3d_bunch_push_piece runs it in place; g3d_bunch_push_vjp is its reverse: given the adjoints of
+ * the pushed (x, y, ζ, p_x, p_y, p_z), the adjoints of the state before the push (the arrays must still hold it)
+ * and of the field planes (added to).  The sub-step lengths and every branch are those of the forward push and
+ * are held fixed; a particle whose push fails keeps its state (gradient: the identity).  The reverse has no
+ * counterpart in the reference. */
+typedef struct {
+    int nx, ny, nsl;
+    double dx, dy, zeta0, ihz;     /* periodic grid; slice k (k = 0 .. nsl-1) at zeta0 + k / ihz */
+    const double* F;               /* E_x E_y E_z B_x B_y B_z, [6][nsl][ny][nx] */
+    int laser_on, laser_pol;       /* the plane wave of g3d_config (code units) */
+    double a0, kL, tau, cep, eta_c, kx, kz;
+    double X0, zc;                 /* window offset in x; ζ of the laser centre */
+    double s0, s1, dz_push, push_phase, hmin, zhand, zmax, ylo, yhi, vz_min;
+    double *x, *y, *zeta, *px, *py, *pz, *z, *rem;
+    const double* qm;
+    int* alive;
+    long np;
+} g3d_bpush_ctx;
+typedef struct {
+    const double *bx, *by, *bzeta, *bpx, *bpy, *bpz;       /* in: adjoints of the pushed state */
+    double *gx, *gy, *gzeta, *gpx, *gpy, *gpz;             /* out: adjoints of the state before the push */
+    double* gF;                                            /* out: adjoint of F (added to) */
+} g3d_bpush_adj;
+int g3d_bunch_push_piece(const g3d_bpush_ctx* c);
+int g3d_bunch_push_vjp(const g3d_bpush_ctx* c, const g3d_bpush_adj* g);
+
 /* ------------------------------------------------------------------ engine: the Simulation of the march
  * g3d_sim mirrors the reference's Simulation (src/sim.hpp:222-360) on one GPU: init() and step() and the
  * member functions of the five drop-in files, each over the kernels of this library. */

@@ -245,3 +245,63 @@ def test_stream_push_gradcheck():
     assert int(al[0]) == 5 and (al[1:] == 1).all()          # the fast one is trapped, the others stream
     f = lambda *a: ga.StreamPush.apply(hc, *a, q, qm, w, alive)
     assert torch.autograd.gradcheck(f, tuple(ins), eps=1e-6, atol=1e-8, rtol=1e-5)
+
+
+# ------------------------------------------------------------------------------ the RK4 bunch push
+def bpush_geometry(deck, sim, s0, s1):
+    g = orc.Grid(deck)
+    c = deck.c
+    L = orc.laser_of(deck)
+    info = sim.info()
+    return ga.PushGeometry(nx=g.nx, ny=g.ny, nsl=info["ns"] + 2, dx=g.dx, dy=g.dy, zeta0=0.0, h=1 / g.ih,
+                           laser_on=L.on, laser_pol=L.pol, a0=L.a0, kL=L.kL, tau=L.tau, cep=L.cep, eta_c=L.eta_c,
+                           kx=L.kx, kz=L.kz, X0=info["X0"], zc=c.zeta_center, s0=s0, s1=s1,
+                           dz_push=c.dz_push or c.dz, push_phase=c.push_phase,
+                           hmin=float(np.min(np.diff(g.zeta))), zhand=-1e300, zmax=float(g.zeta[-1]), ylo=-1e300,
+                           yhi=1e300, vz_min=0.0)
+
+
+def test_bunch_push_piece_against_oracle():
+    """the forward RK4 piece (g3d_bunch_push_piece) against the oracle's bunch push on the golden input (32 beam
+    particles with the oblique plane wave, one slow with many sub-steps); the laser's exp, cos and sin may differ
+    from glibc's in the last bit"""
+    deck = gi.bunch_small_deck()
+    sim = g3d_amd.Sim(deck)
+    sim.init()
+    F, P = gi.push_input(sim)
+    geo = bpush_geometry(deck, sim, 0.0, 20.0)
+    info = sim.info()
+    want = orc.bunch_push(orc.Grid(deck), deck, F, P, 0.0, 20.0, info["X0"], info["Y0"])
+    state = [gpu(P[f].astype(np.float64)) for f in ("x", "y", "zeta", "px", "py", "pz")]
+    out, z, rem, alive = ga.bunch_push_raw(geo, gpu(F.astype(np.float64)), state, gpu(P["z"]), gpu(P["rem"]),
+                                           gpu(P["qm"]), gpu(P["alive"].astype(np.int32)))
+    for f, t in zip(("x", "y", "zeta", "px", "py", "pz"), out):
+        assert np.allclose(t.cpu().numpy(), want[f], rtol=1e-13, atol=0), f
+    assert np.array_equal(alive.cpu().numpy(), want["alive"])
+    assert np.allclose(z.cpu().numpy(), want["z"], rtol=0, atol=0)
+
+
+def test_bunch_push_gradcheck():
+    """the reverse RK4 push against central differences of the forward one, in the state and every field value:
+    four particles from p_z = 3 (many laser-limited sub-steps) to 1000 (one), in the oblique plane wave.  atol
+    1e-7: the central difference of an output near 1000 with step 1e-6 carries about 1e-7 of rounding"""
+    deck = gi.bunch_small_deck()
+    sim = g3d_amd.Sim(deck)
+    sim.init()
+    info = sim.info()
+    geo = bpush_geometry(deck, sim, 0.0, 20.0)
+    r = np.random.default_rng(41)
+    nsl, nk = info["ns"] + 2, info["nk"]
+    F = gpu(0.01 * r.standard_normal(6 * nsl * nk)).requires_grad_()
+    m = 4
+    x = gpu(8 + r.standard_normal(m)); y = gpu(8 + r.standard_normal(m))
+    zeta = gpu(3.75 + 0.5 * r.standard_normal(m))
+    px = gpu(1e-2 * r.standard_normal(m)); py = gpu(1e-2 * r.standard_normal(m))
+    pz = gpu(np.array([3.0, 10.0, 50.0, 1000.0]))
+    fixed = (torch.zeros(m, dtype=torch.float64, device="cuda"), torch.zeros(m, dtype=torch.float64, device="cuda"),
+             -torch.ones(m, dtype=torch.float64, device="cuda"), torch.ones(m, dtype=torch.int32, device="cuda"))
+    ins = (F,) + tuple(t.requires_grad_() for t in (x, y, zeta, px, py, pz))
+    out, _, _, al = ga.bunch_push_raw(geo, F.detach(), [t.detach() for t in ins[1:]], *fixed)
+    assert (al == 1).all()
+    f = lambda *a: ga.BunchPush.apply(geo, *a, *fixed)
+    assert torch.autograd.gradcheck(f, ins, eps=1e-6, atol=1e-7, rtol=1e-5)
