@@ -82,10 +82,12 @@ def measured_traffic(kernel):
     """HBM bytes per launch of the kernel from the committed byte-counter profile (FETCH_SIZE and WRITE_SIZE runs of
     rocprofv3, scaled by a plain copy of known size), or None"""
     try:
-        t = json.load(open(TRAFFIC_FILE))[kernel]
-        return t["bytes_per_launch"]
-    except (OSError, KeyError, ValueError):
+        t = json.load(open(TRAFFIC_FILE))
+    except (OSError, ValueError):
         return None
+    # the entry of the kernel itself or of its one template instance in the profile (k_stream_mixed<4, false>)
+    hits = [v for k, v in t.items() if k == kernel or k.startswith(kernel + "<")]
+    return hits[0]["bytes_per_launch"] if len(hits) == 1 else None
 
 
 def run_form(args, mixed, torch):
