@@ -4,6 +4,10 @@ slice, two passes per half-cell, Gaussian grid laser a0 = 2, w0 = 30/k0, c tau =
 24/k0, dz = 20/k0).  A step is one s-step of the march (currents, the 1023 half-cells x 2 passes of the streaming
 plasma, sponge and real fields).  Metric: streaming-particle pushes per second including deposit and gather.
 
+--workload vacuum: the vacuum deck instead (64 x 64 modes, 256 slices, h = 0.5/k0, 2^18 beam electrons at
+gamma 1000 in the oblique plane wave a0 = 1, dz = 20/k0); a step is the RK4 push of every beam electron, its
+deposit and currents, the march of all slices, sponge and real fields; metric: beam pushes per second.
+
 Prints one JSON line.  --gpus above 1 is refused: this build runs on one GPU.
 """
 import argparse
@@ -33,6 +37,118 @@ def lwfa_deck(nx=256, ny=256, nz=1024, h=0.25):
           ppc_x=2, ppc_y=2, smooth_length=0.0, iterations=2)
     d.add_grid_laser(a0=2.0, waist=30.0, tau=15.0, zeta_c=min(200.0, 0.78 * (nz - 1) * h), z_focus=0.0)
     return d
+
+
+def vacuum_deck(nx=64, nz=256, h=0.5, nmacro=1 << 18):
+    """the vacuum deck: bunch of 11.4014 pC (2^18 electrons, peak 0.01 n_c for sigma 2, 2, 4/k0), gamma 1000,
+    energy spread 1e-3, emittance 2e-4 um, seed 1; plane wave a0 = 1 at 2 mrad, p-polarised, c tau = 30/k0"""
+    import g3d_amd
+    d = g3d_amd.Deck(LAMBDA0)
+    d.c.nx = nx; d.c.Lx = 80.0
+    d.y_uniform(nx, 80.0)
+    d.zeta_uniform(nz, (nz - 1) * h)
+    Lw = (nz - 1) * h
+    d.set(dz=20.0, z_end=1000.0, sponge=8.0, sponge_y=8.0, move_x=1, move_y=1, charge=11.4014089e-12, gamma=1000.0,
+          sigx=2.0, sigy=2.0, sigz=4.0, emitx=2e-4, emity=2e-4, espread=1e-3, zeta_center=0.5 * Lw, nmacro=nmacro,
+          seed=1, laser_on=1, laser_pol=1, a0=1.0, theta=2e-3, tau=30.0, start_fwhm=0.0, stop_fwhm=0.0)
+    return d
+
+
+# FP64 operations of one RK4 sub-step of the bunch push (k_bunch_push): four right-hand sides of about 355 each
+# (the gather of six components over nine nodes and two slices 285, the plane wave 25, gamma and the derivatives
+# 45) and the stage and update arithmetic (72); one sub-step per push on the vacuum deck (gamma 1000: neither the
+# slice spacing nor the laser phase limits the 20/k0 step)
+RK4_FLOP_PER_SUBSTEP = 4 * 355 + 72
+# HBM bytes of one push: read x y zeta px py pz w, write the six coordinates
+PUSH_BYTES = 104.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def run_vacuum(args, torch):
+    import g3d_amd
+    sim = g3d_amd.Sim(vacuum_deck())
+    t0 = time.perf_counter()
+    sim.init()
+    t_init = time.perf_counter() - t0
+    print("vacuum: init %.2f s" % t_init, file=sys.stderr, flush=True)
+    for k in range(args.warmup):
+        sim.step()
+    info = sim.info()
+    N = info["N"]
+    torch.cuda.synchronize()
+    sim.timing(["beam_push"])
+    t0 = time.perf_counter()
+    pushes = 0
+    for k in range(args.steps):
+        np_ = sim.info()["np"]
+        sim.step()
+        pushes += np_
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms, cnt = sim.timers()["beam_push"]
+    avg_s = ms / 1000 / max(cnt, 1)
+    np_ = sim.info()["np"]
+    sim.timing(True)
+    sim.step()
+    torch.cuda.synchronize()
+    fam = {k: {"ms_total": round(v[0], 3), "launches": v[1], "avg_us": round(1000 * v[0] / v[1], 2)}
+           for k, v in sim.timers().items() if v[1]}
+    sim.timing(False)
+    return {"value": pushes / dt, "ms_step": 1000 * dt / args.steps, "N": N, "np": np_, "avg_us": avg_s * 1e6,
+            "bpl": np_ * PUSH_BYTES, "flop": np_ * RK4_FLOP_PER_SUBSTEP, "fam": fam, "t_init": t_init,
+            "slice_solves": N * args.steps / dt}
+
+
+def cpu_baseline_vacuum(threads):
+    """the reference's CPU build on the vacuum deck itself, 40 steps after init (about 5 s on 16 cores)"""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import refpy
+    if not refpy.available():
+        return None
+    refpy.set_threads(threads)
+    ref = refpy.RefSim(vacuum_deck())
+    ref.init()
+    steps = 40
+    t = time.perf_counter()
+    pushes = 0
+    for _ in range(steps):
+        pushes += ref.info()["np"]
+        ref.step()
+    dt = time.perf_counter() - t
+    return {"value": pushes / dt, "unit": "pushes/s", "cores": threads, "kind": "reference",
+            "sample": "reference CPU build (g++ -O3 -fopenmp, %d threads), the vacuum deck, %d steps: %.2f s"
+                      % (threads, steps, dt)}
+
+
+def vacuum_main(args, torch):
+    r = run_vacuum(args, torch)
+    cpu = None
+    if not args.no_cpu:
+        thr = args.cpu_threads or min(16, os.cpu_count() or 1)
+        try:
+            cpu = cpu_baseline_vacuum(thr)
+        except Exception as e:
+            print("cpu baseline failed: %s" % e, file=sys.stderr)
+    avg_s = r["avg_us"] * 1e-6
+    out = {
+        "metric": "beam particle pushes/s (RK4 push with gathers and plane wave, deposit, field solve)",
+        "value": r["value"], "unit": "pushes/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": r["ms_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "vacuum bunch + plane wave 64x64x256, 2^18 beam electrons, FP64", "N": r["N"],
+                   "np": r["np"], "slice_solves_per_s": r["slice_solves"]},
+        "roofline": {"kernel": "k_bunch_push", "bound": "hbm", "achieved": r["bpl"] / avg_s / 1e9,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": r["bpl"] / avg_s / 1e9 / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("k_bunch_push"), "bytes_per_launch": r["bpl"],
+                     "avg_launch_us": r["avg_us"],
+                     "fp64_valu": {"achieved_tflops": r["flop"] / avg_s / 1e12, "peak_tflops": FP64_PEAK_TFLOPS,
+                                   "frac": r["flop"] / avg_s / 1e12 / FP64_PEAK_TFLOPS,
+                                   "flop_per_push": RK4_FLOP_PER_SUBSTEP}},
+        "cpu_baseline": cpu,
+        "kernel_families_one_extra_step": r["fam"],
+        "init_s": r["t_init"],
+    }
+    print(json.dumps(out))
 
 
 def stream_bytes_per_launch(nstrm, nk):
@@ -160,6 +276,8 @@ def main():
     ap.add_argument("--precision", choices=("both", "mixed", "fp64"), default="both",
                     help="both (default): the FP64 and the FP32-mixed march on the same deck; the line reports the "
                          "mixed one and its field error against FP64")
+    ap.add_argument("--workload", choices=("lwfa", "vacuum"), default="lwfa",
+                    help="lwfa (default): the plasma march of the headline metric; vacuum: the beam-push deck")
     ap.add_argument("--copy-calib", type=float, default=0.0,
                     help="only copy a buffer of this many MiB a few times (the reference kernel that scales the HBM "
                          "byte counters of a profile) and exit")
@@ -173,6 +291,9 @@ def main():
         sys.exit(2)
     import torch
     assert torch.cuda.is_available(), "no GPU"
+    if args.workload == "vacuum":
+        vacuum_main(args, torch)
+        return
 
     forms = {"fp64": [False], "mixed": [True], "both": [False, True]}[args.precision]
     runs = {}

@@ -8,8 +8,11 @@
 //      18 on the free-hand grid: 9 node cells with j_x, j_z and 9 half-node cells with j_y), the cell key
 //      (group, half-cell, point) and the values; cells a particle does not touch get the key ~0;
 //   2. a stable radix sort of (key, r) groups each cell's records in particle order;
-//   3. one lane per cell (the head of its run of keys) sums the run in that order, from zero, as the reference
-//      does, and stores the sums into jw.
+//   3. the values are copied into that order (one lane per record), so that
+//   4. one lane per cell (the head of its run of keys), which finds the end of its run by bisection, sums the run
+//      from contiguous memory in that order, from zero, as the reference does, and stores the sums into jw.  (A
+//      cell at the core of a dense bunch collects thousands of records: read through the record index, each
+//      addition waited for two dependent loads.)
 // The result is the reference's single-thread jw, bit for bit.
 #include <rocprim/device/device_radix_sort.hpp>
 
@@ -71,18 +74,46 @@ __global__ void __launch_bounds__(kT) k_dep_emit(DepositArgs c, Parts P, long np
     }
 }
 
+// the values of the sorted records in sorted order
+__global__ void __launch_bounds__(kT) k_dep_order(const unsigned long long* key, const unsigned* rec, const double* val,
+                                                  long n, double* vs) {
+    const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n || key[t] == kNone) return;
+    const double* v = val + 3 * long(rec[t]);
+    vs[3 * t] = v[0]; vs[3 * t + 1] = v[1]; vs[3 * t + 2] = v[2];
+}
+
 // one lane per sorted record; the head of each run of equal keys sums the run in record (= particle) order
-__global__ void __launch_bounds__(kT) k_dep_sum(const unsigned long long* key, const unsigned* rec, const double* val,
-                                                long n, double* J, long nh, long nk, int yf) {
+__global__ void __launch_bounds__(kT) k_dep_sum(const unsigned long long* key, const double* vs, long n, double* J,
+                                                long nh, long nk, int yf) {
     const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const unsigned long long k = key[i];
     if (k == kNone || (i > 0 && key[i - 1] == k)) return;
-    double s0 = 0, s1 = 0, s2 = 0;
-    for (long t = i; t < n && key[t] == k; ++t) {
-        const double* v = val + 3 * long(rec[t]);
-        s0 += v[0]; s1 += v[1]; s2 += v[2];
+    long lo = i + 1, hi = n;                        // the run ends at the first record with another key
+    while (lo < hi) {
+        const long mid = lo + ((hi - lo) >> 1);
+        if (key[mid] == k) lo = mid + 1; else hi = mid;
     }
+    // in blocks of kU records, the next block's loads in flight during this block's additions (the additions
+    // themselves stay in record order)
+    constexpr int kU = 8;
+    double s0 = 0, s1 = 0, s2 = 0;
+    long t = i;
+    double a[3 * kU], b[3 * kU];
+    if (t + kU <= lo)
+        for (int u = 0; u < 3 * kU; ++u) a[u] = vs[3 * t + u];
+    while (t + kU <= lo) {
+        const long tn = t + kU;
+        const bool more = tn + kU <= lo;
+        if (more)
+            for (int u = 0; u < 3 * kU; ++u) b[u] = vs[3 * tn + u];
+        for (int u = 0; u < kU; ++u) { s0 += a[3 * u]; s1 += a[3 * u + 1]; s2 += a[3 * u + 2]; }
+        t = tn;
+        if (more)
+            for (int u = 0; u < 3 * kU; ++u) a[u] = b[u];
+    }
+    for (; t < lo; ++t) { s0 += vs[3 * t]; s1 += vs[3 * t + 1]; s2 += vs[3 * t + 2]; }
     const bool half = (k >> kCellBits) & 1;
     const long cell = long(k & ((1ull << kCellBits) - 1));
     const long hw = cell / nk, p = cell - hw * nk;
@@ -102,6 +133,7 @@ void bunch_deposit(const DepositArgs& a, Parts p, long np, DepositScratch& s) {
     if (s.key0.size() < size_t(n)) {
         const size_t cap = size_t(n) + size_t(n) / 4;
         s.key0.resize(cap); s.key1.resize(cap); s.rec0.resize(cap); s.rec1.resize(cap); s.val.resize(3 * cap);
+        s.vs.resize(3 * cap);
     }
     hipLaunchKernelGGL(k_dep_emit, dim3(blocks_for(np, kT)), dim3(kT), 0, stream(), a, p, np, R, s.key0.data(),
                        s.rec0.data(), s.val.data());
@@ -115,8 +147,10 @@ void bunch_deposit(const DepositArgs& a, Parts p, long np, DepositScratch& s) {
     bytes = s.tmp.size();
     hip_check(rocprim::radix_sort_pairs(s.tmp.data(), bytes, s.key0.data(), s.key1.data(), s.rec0.data(),
                                         s.rec1.data(), n, 0, end_bit, stream()), "beam deposit sort");
-    hipLaunchKernelGGL(k_dep_sum, dim3(blocks_for(n, kT)), dim3(kT), 0, stream(), s.key1.data(), s.rec1.data(),
-                       s.val.data(), n, a.J, long(a.nh), a.nk, a.yf);
+    hipLaunchKernelGGL(k_dep_order, dim3(blocks_for(n, kT)), dim3(kT), 0, stream(), s.key1.data(), s.rec1.data(),
+                       s.val.data(), n, s.vs.data());
+    hipLaunchKernelGGL(k_dep_sum, dim3(blocks_for(n, kT)), dim3(kT), 0, stream(), s.key1.data(), s.vs.data(), n, a.J,
+                       long(a.nh), a.nk, a.yf);
     launch_check("beam deposit sums");
 }
 

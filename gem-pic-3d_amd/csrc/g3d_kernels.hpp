@@ -81,7 +81,7 @@ struct DepositArgs {
 struct DepositScratch {
     DevBuf<unsigned long long> key0, key1;
     DevBuf<unsigned> rec0, rec1;
-    DevBuf<double> val;
+    DevBuf<double> val, vs;          // the record values, and the same in sorted order
     DevBuf<unsigned char> tmp;
 };
 // the reference's single-thread summation order, bit for bit (g3d_deposit.hip)

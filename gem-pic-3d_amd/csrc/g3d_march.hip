@@ -13,22 +13,32 @@ __device__ __forceinline__ void store6(const MarchView& c, long o, const Mode6& 
     c.Rx[o] = m.Rx; c.Ry[o] = m.Ry; c.Ez[o] = m.Ez; c.Bz[o] = m.Bz; c.Lx[o] = m.Lx; c.Ly[o] = m.Ly;
 }
 
-// slices hi-1 ... lo of every mode (src/sim_fields.cpp:7-31, Fourier y)
-__global__ void __launch_bounds__(256) k_march_all(MarchView c, int lo, int hi) {
+// workgroups of one wave for the march of every mode, so that a small grid (64 x 64 modes: 64 waves) spreads
+// over as many compute units
+constexpr int kMarchT = 64;
+// slices hi-1 ... lo of every mode (src/sim_fields.cpp:7-31, Fourier y), one lane per mode
+__global__ void __launch_bounds__(kMarchT) k_march_all(MarchView c, int lo, int hi) {
     const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= c.nk) return;
     Mode6 up{};
     if (hi < c.N) up = load6(c, long(hi - c.a) * c.nk + i);
-    // prefetched explicit part of the next slice
+    // the explicit part and the currents of the next slice are loaded while this slice is marched (the march of
+    // a mode is a chain over the slices: each load on it would add its latency to every slice)
     long o = long(hi - 1 - c.a) * c.nk + i;
     Mode6 old{c.Rx[o], c.Ry[o], c.Ez[o], c.Bz[o], {}, {}};
+    Cur9 cur = load_cur(c, i, hi - 1);
     for (int n = hi - 1; n >= lo; --n) {
         Mode6 nxt{};
+        Cur9 curn{};
         const long on = o - c.nk;
-        if (n > lo) { nxt.Rx = c.Rx[on]; nxt.Ry = c.Ry[on]; nxt.Ez = c.Ez[on]; nxt.Bz = c.Bz[on]; }
-        up = march_mode(c, i, n, old, up);
+        if (n > lo) {
+            nxt.Rx = c.Rx[on]; nxt.Ry = c.Ry[on]; nxt.Ez = c.Ez[on]; nxt.Bz = c.Bz[on];
+            curn = load_cur(c, i, n - 1);
+        }
+        up = march_mode(c, i, n, old, up, cur);
         store6(c, o, up);
         old = nxt;
+        cur = curn;
         o = on;
     }
 }
@@ -72,7 +82,7 @@ LineView lines_of(const g3d_march_ctx& m) {
 
 void march_all(const MarchView& v, int lo, int hi) {
     if (hi <= lo) return;
-    hipLaunchKernelGGL(k_march_all, dim3(blocks_for(v.nk, 256)), dim3(256), 0, stream(), v, lo, hi);
+    hipLaunchKernelGGL(k_march_all, dim3(blocks_for(v.nk, kMarchT)), dim3(kMarchT), 0, stream(), v, lo, hi);
     launch_check("march");
 }
 void march_one(const MarchView& v, int n, cx* bRx, cx* bRy, cx* bEz, cx* bBz, int restore) {
