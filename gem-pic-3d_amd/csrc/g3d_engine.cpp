@@ -1007,10 +1007,12 @@ void Sim::march_plasma(double s0, double s1) {
     // FP32-mixed: the tile windows go to the current planes with float atomics instead of slabs (3 % faster at
     // 256 x 256: the row kernel then reads one plane set); G3D_STREAM_AFLUSH=0 keeps the slabs
     { const char* e = getenv("G3D_STREAM_AFLUSH"); tg.aflush = mx && !(e && atoi(e) == 0); }
-    // FP32-mixed: tiles denser than one workgroup's lanes are streamed in chunks (G3D_STREAM_CHUNK=0: off)
+    // FP32-mixed: tiles denser than G3D_STREAM_CHUNK (default 1) times one workgroup's lanes are streamed in
+    // chunks of that many particles (0: off)
     const char* ech = getenv("G3D_STREAM_CHUNK");
-    if (mx && tg.ntile && !(ech && atoi(ech) == 0)) {
-        tg.chunk = tg.block;
+    const int chunk_mult = ech ? atoi(ech) : 1;
+    if (mx && tg.ntile && chunk_mult > 0) {
+        tg.chunk = chunk_mult * tg.block;
         tg.max_extra = int((nstrm + tg.chunk - 1) / tg.chunk);
         if (chunk_tile.size() < size_t(tg.max_extra)) { chunk_tile.resize(size_t(tg.max_extra)); chunk_start.resize(size_t(tg.max_extra)); }
         if (chunk_cnt.size() < 1) chunk_cnt.resize(1);
@@ -1068,7 +1070,7 @@ void Sim::march_plasma(double s0, double s1) {
             if (passes > 1) { m.bRx = bk.data(); m.bRy = bk.data() + pl; m.bEz = bk.data() + 2 * pl; m.bBz = bk.data() + 3 * pl; }
             if (mx) {
                 const LaunchTimer lt = launch_timer(F_STREAM);
-                stream_mixed(sc, jsr_f.data(), slab_f.data(), strmA.view(), strmB.view(&strmA), nlist, tg, lt);
+                stream_mixed(sc, jsr_f.data(), slab_f.data(), strmA.view(), strmB.view(&strmA), nlist, tg, lt, last);
                 launch_timed(F_STREAM, lt);
                 tic(F_MODES);
                 pass_mixed(m, fft, jsr_f.data(), slab_f.data(), tg, Xf, Tf, F.data() + long(n - a) * nk, fpl,

@@ -182,8 +182,9 @@ int rows_per_wg(int ny);            // rows per workgroup of the row kernels
 // the FP32-mixed plasma pass (g3d_mixed.hip): the streaming half-cell with FP32 deposit (jsr and slabs float), and
 // the field pass of slice m.n (rows forward, y transforms, per-mode work, inverse transforms, F); X, T: 3 planes
 // of cxf scratch; kind as pass_cols
+// rho: deposit the charge density planes too (the density lines read the last pass's)
 void stream_mixed(const StreamArgs& a, float* J, float* slab, Parts src, Parts dst, long nstrm, const TileGeom& tg,
-                  const LaunchTimer& lt = {});
+                  const LaunchTimer& lt = {}, bool rho = true);
 void pass_mixed(const ModeArgs& m, const FftPlan& p, float* J, const float* slab, const TileGeom& tg, cxf* X, cxf* T,
                 float* Fn, long fpl, double* rl, double* rli, int i0, int j0, int kind, int rpw);
 // the same columns step as three launches: y transforms of X (in place), the per-mode work (one lane per mode),

@@ -132,9 +132,10 @@ __device__ __forceinline__ void gather_f(const FieldView& c, const InvF& iv, con
     E[0] = v[0]; E[1] = v[1]; E[2] = v[2]; B[0] = v[3]; B[1] = v[4]; B[2] = v[5];
 }
 
-// TSC deposit of f v into the FP32 current planes (window or jsr), as the FP64 deposit
+// TSC deposit of f v into the FP32 current planes (window or jsr), as the FP64 deposit; the charge density planes
+// (rho-, rho+) only when rho (their one reader, the density lines, takes the last pass's)
 __device__ __forceinline__ void sdeposit_f(const StreamArgs& c, const InvF& iv, float* J, const WinF* win, double x,
-                                           double y, float f, float vx, float vy, float vz) {
+                                           double y, float f, float vx, float vy, float vz, bool rho) {
     const FieldView& P = c.fv;
     int ix[3], jn[3]; float wx[3], wn[3];
     tsc_f(x, iv.ix, P.nx, ix, wx);
@@ -157,7 +158,7 @@ __device__ __forceinline__ void sdeposit_f(const StreamArgs& c, const InvF& iv, 
                         atomicAdd(L + 1, double(ww * vy));
                         atomicAdd(L + 2, double(ww * vz));
                     }
-                    atomicAdd(L + cq, double(ww));
+                    if (rho) atomicAdd(L + cq, double(ww));
                 }
             return;
         }
@@ -173,7 +174,7 @@ __device__ __forceinline__ void sdeposit_f(const StreamArgs& c, const InvF& iv, 
                 atomicAdd(&J[nk + k], ww * vy);
                 atomicAdd(&J[2 * nk + k], ww * vz);
             }
-            atomicAdd(&J[cq * nk + k], ww);
+            if (rho) atomicAdd(&J[cq * nk + k], ww);
         }
 }
 
@@ -186,14 +187,26 @@ __device__ __forceinline__ MState load_mstate(const Parts& S, long i) {
     return MState{S.x[i], S.y[i], S.zeta[i], S.z[i], S.px[i], S.py[i], S.pz[i], S.q[i], S.qm[i], S.w[i], S.alive[i]};
 }
 
+// the values of a launch that every electron uses: the half-cell's ends and length, the FP32 factor
+// 1 / (Delta h dx) of the deposit, and whether the density planes are deposited
+struct HalfCellF {
+    double zn, zn1, h;
+    float iq;
+    bool rho;
+};
+__device__ __forceinline__ HalfCellF half_cell(const StreamArgs& c, int rho) {
+    const double zn = c.zs[c.n], zn1 = c.zs[c.n + 1], h = zn1 - zn;
+    return HalfCellF{zn, zn1, h, rcp_f(float(c.Delta * h * c.fv.dx)), rho != 0};
+}
+
 // one electron through half-cell n (src/sim_stream.hpp:104-171), the physics of the FP64 pass in FP32; s: its state
-// at the start (loaded by the caller), written to D[i]
+// at the start (loaded by the caller), written to D[i].  The TSC positions are not wrapped into the box first: the
+// node indices are (wrap_index), and the FP32 fraction does not see the difference.
 // LASER: the deck has the analytic laser (its evaluation would otherwise set the register budget of every launch)
 template <bool LASER>
-__device__ __forceinline__ void stream_one_mixed(const StreamArgs& c, const InvF& iv, float* J, const WinF* win,
-                                                 const MState& s, Parts D, long i) {
-    const int n = c.n;
-    const double zn = c.zs[n], zn1 = c.zs[n + 1], h = zn1 - zn;
+__device__ __forceinline__ void stream_one_mixed(const StreamArgs& c, const HalfCellF& hc, const InvF& iv, float* J,
+                                                 const WinF* win, const MState& s, Parts D, long i) {
+    const double zn = hc.zn, zn1 = hc.zn1, h = hc.h;
     double x = s.x, y = s.y, zeta = s.zeta, z = s.z;
     const double pxd = s.px, pyd = s.py, pzd = s.pz;
     const double q = s.q, qmd = s.qm, w = s.w;
@@ -217,7 +230,7 @@ __device__ __forceinline__ void stream_one_mixed(const StreamArgs& c, const InvF
     const double xm = x + double(hs * px), ym = y + double(hs * py);
     const double zm = z + double(hs * pz), zetam = zeta - 0.5 * hl;
     float E[3], B[3];
-    gather_f(P, iv, win, wrap_mod(xm, P.Lx), wrap_mod(ym, P.Ly), zetam, E, B);
+    gather_f(P, iv, win, xm, ym, zetam, E, B);
     if (LASER && c.laser.on) {
         double Ee[3], Be[3];
         c.laser.field(xm + c.X0, zm, zm - (zetam - c.zc), Ee, Be);
@@ -251,15 +264,13 @@ __device__ __forceinline__ void stream_one_mixed(const StreamArgs& c, const InvF
     z = trapped ? c.s1 : z + double(vz * dt);
     zeta = zeta_end;
     // q w dt / (Delta h dx), the cell factors in FP32
-    const float qw = float(q * w) * rcp_f(float(c.Delta * h * P.dx));
+    const float qw = float(q * w) * hc.iq;
     const float f = qw * dt;
-    if (f != 0)
-        sdeposit_f(c, iv, J, win, wrap_mod(0.5 * (xo + x), P.Lx), wrap_mod(0.5 * (yo + y), P.Ly), f, vx, vy, vz);
+    if (f != 0) sdeposit_f(c, iv, J, win, 0.5 * (xo + x), 0.5 * (yo + y), f, vx, vy, vz, hc.rho);
     if (trapped) {
         alive = 5;
         const float vze = uz * ig1;
-        if (vze >= float(c.vz_beam))
-            sdeposit_f(c, iv, J, win, wrap_mod(x, P.Lx), wrap_mod(y, P.Ly), qw * rcp_f(vze), ux * ig1, uy * ig1, vze);
+        if (vze >= float(c.vz_beam)) sdeposit_f(c, iv, J, win, x, y, qw * rcp_f(vze), ux * ig1, uy * ig1, vze, hc.rho);
     }
     store(ux, uy, uz);
 }
@@ -267,7 +278,7 @@ __device__ __forceinline__ void stream_one_mixed(const StreamArgs& c, const InvF
 // WPE: waves per SIMD the register allocation must allow (0: the compiler's choice)
 template <int WPE, bool LASER>
 __global__ void __launch_bounds__(kStreamBlockMax) __attribute__((amdgpu_waves_per_eu(WPE ? WPE : 1)))
-k_stream_mixed(StreamArgs c, float* J, float* slab, Parts S, Parts D, long n, TileGeom tg) {
+k_stream_mixed(StreamArgs c, float* J, float* slab, Parts S, Parts D, long n, TileGeom tg, int rho) {
     extern __shared__ float4 winf4[];
     const long b = blockIdx.x;
     const long nt = tg.ntile + tg.max_extra;
@@ -315,8 +326,9 @@ k_stream_mixed(StreamArgs c, float* J, float* slab, Parts S, Parts D, long n, Ti
         __syncthreads();
         WinF w{dep, F.self ? winf4 : nullptr, x0, y0, tg.WX, tg.WY, fl};
         const InvF iv{1.0 / F.dx, 1.0 / F.dy, float(c.ia[0])};
+        const HalfCellF hc = half_cell(c, rho);
         while (k < hi) {
-            stream_one_mixed<LASER>(c, iv, J, &w, st, D, i);
+            stream_one_mixed<LASER>(c, hc, iv, J, &w, st, D, i);
             k += blockDim.x;
             if (k < hi) {
                 i = tg.idx ? long(tg.idx[k]) : k;
@@ -339,8 +351,8 @@ k_stream_mixed(StreamArgs c, float* J, float* slab, Parts S, Parts D, long n, Ti
     }
     const long i = tg.n_tiled + (b - nt) * blockDim.x + threadIdx.x;
     if (i < n)
-        stream_one_mixed<LASER>(c, InvF{1.0 / c.fv.dx, 1.0 / c.fv.dy, float(c.ia[0])}, J, nullptr, load_mstate(S, i), D,
-                                i);
+        stream_one_mixed<LASER>(c, half_cell(c, rho), InvF{1.0 / c.fv.dx, 1.0 / c.fv.dy, float(c.ia[0])}, J, nullptr,
+                                load_mstate(S, i), D, i);
 }
 
 // ---------------------------------------------------------------- rows forward (FP32)
@@ -533,7 +545,7 @@ int lanes(int work) { int t = 64; while (t < work && t < kT) t <<= 1; return t; 
 }  // namespace
 
 void stream_mixed(const StreamArgs& a, float* J, float* slab, Parts src, Parts dst, long nstrm, const TileGeom& tg,
-                  const LaunchTimer& lt) {
+                  const LaunchTimer& lt, bool rho) {
     if (nstrm <= 0) return;
     const int bs = tg.ntile ? tg.block : kT;
     if (bs < 64 || bs > kStreamBlockMax || bs % 64) throw std::runtime_error("stream half-cell (mixed): bad block size");
@@ -542,8 +554,7 @@ void stream_mixed(const StreamArgs& a, float* J, float* slab, Parts src, Parts d
     const dim3 grid(unsigned(tg.ntile + tg.max_extra + tail_blocks)), blk(bs);
     auto* k = a.laser.on ? (tg.wpe == 4 ? k_stream_mixed<4, true> : k_stream_mixed<0, true>)
                          : (tg.wpe == 4 ? k_stream_mixed<4, false> : k_stream_mixed<0, false>);
-    launch_timed_kernel(lt, k, grid, blk, unsigned(lds), a, J, slab,
-                        src, dst, nstrm, tg);
+    launch_timed_kernel(lt, k, grid, blk, unsigned(lds), a, J, slab, src, dst, nstrm, tg, int(rho));
     launch_check("stream half-cell (mixed)");
 }
 void pass_mixed(const ModeArgs& m, const FftPlan& p, float* J, const float* slab, const TileGeom& tg, cxf* X, cxf* T,
