@@ -36,7 +36,8 @@ struct InvF {
 
 // TSC weights in FP32 on a periodic uniform axis: the cell from the FP64 index coordinate x / d (x * (1/d): the
 // cell may differ from the FP64 pass where x / d lies within an ulp of a half-integer), the fraction in FP32.
-__device__ __forceinline__ void tsc_f(double x, double inv, int n, int idx[3], float w[3]) {
+// Returns the middle node's index, not wrapped (the window path wraps only its offset; tsc_nodes wraps all three).
+__device__ __forceinline__ int tsc_f(double x, double inv, float w[3]) {
     const double xi = x * inv;
     const int i0 = int(floor(xi + 0.5));
     const float f = float(xi - i0);
@@ -44,6 +45,9 @@ __device__ __forceinline__ void tsc_f(double x, double inv, int n, int idx[3], f
     w[0] = 0.5f * a * a;
     w[1] = 0.75f - f * f;
     w[2] = 0.5f * b * b;
+    return i0;
+}
+__device__ __forceinline__ void tsc_nodes(int i0, int n, int idx[3]) {
     idx[0] = wrap_index(i0 - 1, n); idx[1] = wrap_index(i0, n); idx[2] = wrap_index(i0 + 1, n);
 }
 
@@ -82,9 +86,8 @@ __device__ __forceinline__ void gather_f(const FieldView& c, const InvF& iv, con
     if (n == c.a - 1 && fz > 1 - 1e-9) { n = c.a; fz = 0; }
     const int fl = n - c.a;
     if (fl < 0 || fl > c.ns) return;
-    int ix[3], jn[3]; float wx[3], wn[3];
-    tsc_f(x, iv.ix, c.nx, ix, wx);
-    tsc_f(y, iv.iy, c.ny, jn, wn);
+    float wx[3], wn[3];
+    const int i0 = tsc_f(x, iv.ix, wx), j0 = tsc_f(y, iv.iy, wn);
     const float f1 = float(fz), f0 = 1.0f - f1;
     float wt[9];
 #pragma unroll
@@ -93,7 +96,7 @@ __device__ __forceinline__ void gather_f(const FieldView& c, const InvF& iv, con
         for (int i = 0; i < 3; ++i) wt[3 * j + i] = wx[i] * wn[j];
     float v[6];
     if (w && w->fld && fl == w->fl) {
-        const int u0 = wrap_index(ix[0] - w->x0, c.nx), v0 = wrap_index(jn[0] - w->y0, c.ny);
+        const int u0 = wrap_index(i0 - 1 - w->x0, c.nx), v0 = wrap_index(j0 - 1 - w->y0, c.ny);
         if (u0 + 2 < w->WX && v0 + 2 < w->WY) {
             float s0[6] = {0, 0, 0, 0, 0, 0}, s1[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
@@ -113,6 +116,9 @@ __device__ __forceinline__ void gather_f(const FieldView& c, const InvF& iv, con
             return;
         }
     }
+    int ix[3], jn[3];
+    tsc_nodes(i0, c.nx, ix);
+    tsc_nodes(j0, c.ny, jn);
     const long cs = long(c.ns + 2) * c.nk;
     const float* p0 = c.F + long(fl) * c.nk;
 #pragma unroll
@@ -137,32 +143,40 @@ __device__ __forceinline__ void gather_f(const FieldView& c, const InvF& iv, con
 __device__ __forceinline__ void sdeposit_f(const StreamArgs& c, const InvF& iv, float* J, const WinF* win, double x,
                                            double y, float f, float vx, float vy, float vz, bool rho) {
     const FieldView& P = c.fv;
-    int ix[3], jn[3]; float wx[3], wn[3];
-    tsc_f(x, iv.ix, P.nx, ix, wx);
-    tsc_f(y, iv.iy, P.ny, jn, wn);
+    float wx[3], wn[3];
+    const int i0 = tsc_f(x, iv.ix, wx), j0 = tsc_f(y, iv.iy, wn);
     const long nk = P.nk;
     const int cq = f < 0 ? 3 : 4;
     const bool mv = vx != 0 || vy != 0 || vz != 0;
     const float fi = f * iv.ia0;
     if (win) {
-        const int u0 = wrap_index(ix[0] - win->x0, P.nx), v0 = wrap_index(jn[0] - win->y0, P.ny);
+        const int u0 = wrap_index(i0 - 1 - win->x0, P.nx), v0 = wrap_index(j0 - 1 - win->y0, P.ny);
         if (u0 + 2 < win->WX && v0 + 2 < win->WY) {
+            // the products in FP64 from the FP32 weights and velocity: the LDS sums are FP64, and an FP64 multiply
+            // issues as fast as a conversion
+            double dx[3], dy[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) { dx[k] = double(fi * wx[k]); dy[k] = double(wn[k]); }
+            const double dvx = vx, dvy = vy, dvz = vz;
 #pragma unroll
             for (int v = 0; v < 3; ++v)
 #pragma unroll
                 for (int u = 0; u < 3; ++u) {
                     double* L = win->lds + kDepNode * ((v0 + v) * win->WX + (u0 + u));
-                    const float ww = fi * wx[u] * wn[v];
+                    const double ww = dx[u] * dy[v];
                     if (mv) {
-                        atomicAdd(L, double(ww * vx));
-                        atomicAdd(L + 1, double(ww * vy));
-                        atomicAdd(L + 2, double(ww * vz));
+                        atomicAdd(L, ww * dvx);
+                        atomicAdd(L + 1, ww * dvy);
+                        atomicAdd(L + 2, ww * dvz);
                     }
-                    if (rho) atomicAdd(L + cq, double(ww));
+                    if (rho) atomicAdd(L + cq, ww);
                 }
             return;
         }
     }
+    int ix[3], jn[3];
+    tsc_nodes(i0, P.nx, ix);
+    tsc_nodes(j0, P.ny, jn);
 #pragma unroll
     for (int v = 0; v < 3; ++v)
 #pragma unroll
