@@ -16,15 +16,40 @@ namespace {
 constexpr int kT = 64;
 
 // ---------------------------------------------------------------- march of one line
-// Slice n of line ik at level s1, with the reference's operations and operand order (bit-identical results).
-// Fused form: one sweep up the line forms the explicit parts of R (from slice n at s0) and L (from slice n+1 at
-// s1) and runs the forward eliminations of both systems, E_z on the nodes 1 .. ny-1 and B_z on the half-nodes
-// 0 .. ny-1, the latter one node behind; the explicit parts wait in the R and L slots of slice n, the eliminated
-// right-hand sides in the E_z and B_z slots, the factors in scp.  One sweep down back-substitutes both systems
-// and finishes R and L one node behind.  Every value of slice n at s0 is read before its slot is rewritten.
-__device__ void march_line(const LineView& L, int ik, int n) {
+// Slice n of line ik at level s1 by the line's workgroup (kLW lanes), with the reference's operations and operand
+// order (bit-identical results).  Of the march only the two tridiagonal eliminations (E_z on the nodes 1 .. ny-1,
+// B_z on the half-nodes 0 .. ny-1) and their back substitutions are recurrences along y; they run on one lane each
+// (lanes 0 and 1 together) over coefficients in LDS.  Everything else is per node and spread over the lanes:
+//   1. the explicit parts of R (from slice n at s0) and of L (from slice n+1 at s1);
+//   2. from those of the node and its neighbours, the right-hand sides and the fixed parts of the pivots;
+//   3. (lanes 0, 1) the eliminations and back substitutions;
+//   4. E_z, B_z and R, L of every node.
+// Every value of slice n at s0 is read (1) before any is rewritten (4).
+constexpr int kLW = 64;
+// per node (half-node for B_z): the explicit parts, the right-hand sides (then the eliminated ones, then the
+// solution), the fixed pivot parts and the off-diagonals (the upper one then the elimination factor)
+struct LineLds {
+    cx *sRx, *sRy, *sLx, *sLy, *rE, *rB;
+    double *bE, *lE, *uE, *bB, *lB, *uB;
+};
+constexpr size_t kLineLdsNode = 6 * sizeof(cx) + 6 * sizeof(double);
+__device__ LineLds line_lds(void* base, int ny) {
+    cx* z = reinterpret_cast<cx*>(base);
+    LineLds s;
+    s.sRx = z; s.sRy = z + ny; s.sLx = z + 2 * ny; s.sLy = z + 3 * ny; s.rE = z + 4 * ny; s.rB = z + 5 * ny;
+    double* d = reinterpret_cast<double*>(z + 6 * ny);
+    s.bE = d; s.lE = d + ny; s.uE = d + 2 * ny; s.bB = d + 3 * ny; s.lB = d + 4 * ny; s.uB = d + 5 * ny;
+    return s;
+}
+// the lanes of the workgroup have written global memory that the other lanes read next
+__device__ __forceinline__ void line_sync() {
+    __threadfence_block();
+    __syncthreads();
+}
+
+__device__ void march_line(const LineView& L, int ik, int n, const LineLds& s) {
     const MarchView& c = L.mv;
-    const int nx = c.nx, ny = L.ny;
+    const int nx = c.nx, ny = L.ny, t0 = threadIdx.x;
     const long nk = c.nk;
     const long P0 = long(n - c.a) * nk + ik;        // slice n = plane of half-cell n-1 in the currents
     const long P1 = P0 + nk;                        // slice n+1 = half-cell n
@@ -32,8 +57,6 @@ __device__ void march_line(const LineView& L, int ik, int n) {
     const double kx2 = c.kx[ik] * c.kx[ik], D = c.D, wl = c.wlo[n], wu = c.whi[n];
     const double* iD = L.iD;
     const double* iA = L.iA;
-    double* cpE = L.scp;
-    double* cpB = L.scp + nk;
     // explicit part of R at node / half-node j from slice n at level s0 and the currents of both levels
     auto known_R = [&](int j, cx ez0, cx ezu, cx bz0, cx bzm, cx& sx, cx& sy) {
         const long o = P0 + long(j) * nx;
@@ -46,110 +69,113 @@ __device__ void march_line(const LineView& L, int ik, int n) {
         const cx Jy = wl * (c.j0y[o] + c.jky[o]) + wu * (c.j0y[o + nk] + c.jky[o + nk]);
         sy = c.Ry[o] + D * ((ezu - ez0) * iD[j] - a * bz0) - D * Jy;
     };
-    cx ez0 = c.Ez[P0], bzm = mk(0);
+    auto explicit_R = [&](int j) {
+        const long o = P0 + long(j) * nx;
+        const cx ez0 = c.Ez[o], ezu = j + 1 < ny ? c.Ez[o + nx] : mk(0), bz0 = c.Bz[o];
+        const cx bzm = j > 0 ? c.Bz[o - nx] : mk(0);
+        known_R(j, ez0, ezu, bz0, bzm, s.sRx[j], s.sRy[j]);
+    };
     if (n == c.N - 1) {                              // front of the window: L = E_z = B_z = 0
-        for (int j = 0; j < ny; ++j) {
+        for (int j = t0; j < ny; j += kLW) explicit_R(j);
+        line_sync();
+        for (int j = t0; j < ny; j += kLW) {
             const long o = P0 + long(j) * nx;
-            const cx ezu = j + 1 < ny ? c.Ez[o + nx] : mk(0), bz0 = c.Bz[o];
-            cx sx, sy;
-            known_R(j, ez0, ezu, bz0, bzm, sx, sy);
-            c.Rx[o] = sx; c.Ry[o] = sy;
+            c.Rx[o] = s.sRx[j]; c.Ry[o] = s.sRy[j];
             c.Ez[o] = c.Bz[o] = c.Lx[o] = c.Ly[o] = mk(0);
-            ez0 = ezu; bzm = bz0;
         }
+        line_sync();
         return;
     }
     const double h = c.hcg[n + 2], qh = 0.25 * h, hh = 0.5 * h, cc = qh * (D + qh);
-    auto P = [&](const cx* A, int j) { return A[P1 + long(j) * nx]; };
-    // slice n+1 at node j (and j-1: ...m), explicit parts of node j-1, elimination state of both systems
-    cx pRx = P(c.Rx, 0), pRy = P(c.Ry, 0), pLx = P(c.Lx, 0), pLy = P(c.Ly, 0), pBz = P(c.Bz, 0), pEz = mk(0);
-    cx pRxm = mk(0), pRym = mk(0), pLxm = mk(0), pLym = mk(0), pBzm = mk(0);
-    cx sRxm = mk(0), sRym = mk(0), sLxm = mk(0), sLym = mk(0);
-    double cE = 0, cB = 0;
-    cx dE = mk(0), dB = mk(0);
-    // forward elimination of B_z at half-node jb (fu, su: the R + L sums of node jb+1, zero above the last)
-    auto elim_B = [&](int jb, cx fu, cx f0, cx ry, cx ly, cx pbz, cx su, cx s0, cx sry, cx sly) {
-        const cx Cp = 0.5 * (a * (ry + ly) - (fu - f0) * iD[jb]);
-        const cx SB = a * (sry + sly) - (su - s0) * iD[jb];
-        const cx r = pbz - hh * Cp - qh * SB;
-        const double al = jb > 0 ? iA[jb] : 0, au = jb < ny - 1 ? iA[jb + 1] : 0;
-        const double lo = -cc * iD[jb] * al, up = -cc * iD[jb] * au;
-        const double den = 1 + cc * kx2 + cc * iD[jb] * (al + au) - lo * cB;
-        dB = (r - lo * dB) / den;
-        cB = up / den;
-        c.Bz[P0 + long(jb) * nx] = dB;
-        cpB[long(jb) * nx + ik] = cB;
-    };
-    for (int j = 0; j < ny; ++j) {
-        const long o = P0 + long(j) * nx, u = P1 + long(j) * nx;
-        const cx ezu = j + 1 < ny ? c.Ez[o + nx] : mk(0), bz0 = c.Bz[o];
-        cx sRx, sRy;
-        known_R(j, ez0, ezu, bz0, bzm, sRx, sRy);
-        const cx pEzu = j + 1 < ny ? c.Ez[u + nx] : mk(0);
-        cx sLx;
-        if (j == 0) sLx = mk(0);
+    // 1. explicit parts
+    for (int j = t0; j < ny; j += kLW) {
+        explicit_R(j);
+        const long u = P1 + long(j) * nx;
+        const cx pBz = c.Bz[u], pBzm = j > 0 ? c.Bz[u - nx] : mk(0);
+        const cx pEz = j > 0 ? c.Ez[u] : mk(0), pEzu = j + 1 < ny ? c.Ez[u + nx] : mk(0);
+        if (j == 0) s.sLx[j] = mk(0);
         else {
             const cx bm = (pBz - pBzm) * iA[j];
-            sLx = pLx - qh * (a * pEz - bm) - hh * c.jkx[u];
+            s.sLx[j] = c.Lx[u] - qh * (a * pEz - bm) - hh * c.jkx[u];
         }
         const cx bp = (pEzu - pEz) * iD[j];
-        const cx sLy = pLy - qh * (a * pBz + bp) - hh * c.jky[u];
-        c.Rx[o] = sRx; c.Ry[o] = sRy; c.Lx[o] = sLx; c.Ly[o] = sLy;
+        s.sLy[j] = c.Ly[u] - qh * (a * pBz + bp) - hh * c.jky[u];
+    }
+    __syncthreads();
+    // 2. right-hand sides and fixed pivot parts: E_z at node j (j >= 1), B_z at half-node j
+    for (int j = t0; j < ny; j += kLW) {
+        const long u = P1 + long(j) * nx;
+        const cx pRx = c.Rx[u], pRy = c.Ry[u], pLx = c.Lx[u], pLy = c.Ly[u];
+        const cx sRx = s.sRx[j], sRy = s.sRy[j], sLx = s.sLx[j], sLy = s.sLy[j];
         if (j >= 1) {
-            // E_z at node j
+            const cx pRym = c.Ry[u - nx], pLym = c.Ly[u - nx];
+            const cx sRym = s.sRy[j - 1], sLym = s.sLy[j - 1];
+            const cx pEz = c.Ez[u];
             const cx Qp = 0.5 * (-(a * (pRx - pLx)) + ((pLy - pRy) - (pLym - pRym)) * iA[j]);
             const cx SE = -(a * (sRx - sLx)) + ((sLy - sRy) - (sLym - sRym)) * iA[j];
-            const cx r = pEz - hh * Qp - qh * SE - h * c.jkz[u];
-            const double lo = j > 1 ? -cc * iA[j] * iD[j - 1] : 0, up = j < ny - 1 ? -cc * iA[j] * iD[j] : 0;
-            const double den = 1 + cc * kx2 + cc * iA[j] * (iD[j - 1] + iD[j]) - lo * cE;
-            dE = (r - lo * dE) / den;
-            cE = up / den;
-            c.Ez[o] = dE;
-            cpE[long(j) * nx + ik] = cE;
-            // B_z at half-node j-1
-            elim_B(j - 1, pRx + pLx, pRxm + pLxm, pRym, pLym, pBzm, sRx + sLx, sRxm + sLxm, sRym, sLym);
+            s.rE[j] = pEz - hh * Qp - qh * SE - h * c.jkz[u];
+            s.lE[j] = j > 1 ? -cc * iA[j] * iD[j - 1] : 0;
+            s.uE[j] = j < ny - 1 ? -cc * iA[j] * iD[j] : 0;
+            s.bE[j] = 1 + cc * kx2 + cc * iA[j] * (iD[j - 1] + iD[j]);
         }
-        ez0 = ezu; bzm = bz0;
-        pRxm = pRx; pRym = pRy; pLxm = pLx; pLym = pLy; pBzm = pBz;
-        sRxm = sRx; sRym = sRy; sLxm = sLx; sLym = sLy;
-        pEz = pEzu;
-        if (j + 1 < ny) {
-            pRx = P(c.Rx, j + 1); pRy = P(c.Ry, j + 1); pLx = P(c.Lx, j + 1); pLy = P(c.Ly, j + 1);
-            pBz = P(c.Bz, j + 1);
-        }
+        const bool top = j + 1 >= ny;
+        const cx fu = top ? mk(0) : c.Rx[u + nx] + c.Lx[u + nx], f0 = pRx + pLx;
+        const cx su = top ? mk(0) : s.sRx[j + 1] + s.sLx[j + 1], s0 = sRx + sLx;
+        const cx Cp = 0.5 * (a * (pRy + pLy) - (fu - f0) * iD[j]);
+        const cx SB = a * (sRy + sLy) - (su - s0) * iD[j];
+        s.rB[j] = c.Bz[u] - hh * Cp - qh * SB;
+        const double al = j > 0 ? iA[j] : 0, au = j < ny - 1 ? iA[j + 1] : 0;
+        s.lB[j] = -cc * iD[j] * al;
+        s.uB[j] = -cc * iD[j] * au;
+        s.bB[j] = 1 + cc * kx2 + cc * iD[j] * (al + au);
     }
-    elim_B(ny - 1, mk(0), pRxm + pLxm, pRym, pLym, pBzm, mk(0), sRxm + sLxm, sRym, sLym);
-    // back substitution of both systems; R and L of node jj once E_z(jj), E_z(jj+1), B_z(jj), B_z(jj-1) are known
-    auto finish = [&](int jj, cx ez, cx ezu, cx bz, cx bzl) {
-        const long o = P0 + long(jj) * nx;
-        c.Ez[o] = ez; c.Bz[o] = bz;
-        if (jj == 0) { c.Rx[o] = mk(0); c.Lx[o] = mk(0); }
-        else {
-            const cx bm = (bz - bzl) * iA[jj];
-            c.Rx[o] = c.Rx[o] + D * (a * ez + bm);
-            c.Lx[o] = c.Lx[o] - qh * (a * ez - bm);
+    __syncthreads();
+    // 3. the recurrences, side by side in one instruction stream: lane 0 E_z (nodes 1 ..), lane 1 B_z (half-nodes
+    //    0 ..); the solution replaces the eliminated right-hand side
+    if (t0 < 2) {
+        const bool hb = t0 == 1;
+        const int j0 = hb ? 0 : 1;
+        cx* d = hb ? s.rB : s.rE;
+        const double *bs = hb ? s.bB : s.bE, *ls = hb ? s.lB : s.lE;
+        double* cp = hb ? s.uB : s.uE;
+        double cv = 0;
+        cx dv = mk(0);
+        for (int j = j0; j < ny; ++j) {
+            const double lo = ls[j];
+            const double den = bs[j] - lo * cv;
+            dv = (d[j] - lo * dv) / den;
+            cv = cp[j] / den;
+            d[j] = dv; cp[j] = cv;
         }
-        const cx bp = (ezu - ez) * iD[jj];
-        c.Ry[o] = c.Ry[o] + D * (bp - a * bz);
-        c.Ly[o] = c.Ly[o] - qh * (a * bz + bp);
-    };
-    cx nE = mk(0), nB = mk(0);
-    cx e1 = mk(0), e2 = mk(0), b1 = mk(0);          // E_z(j+1), E_z(j+2), B_z(j+1)
-    for (int j = ny - 1; j >= 0; --j) {
+        cx nv = mk(0);
+        for (int j = ny - 1; j >= j0; --j) { nv = d[j] - cp[j] * nv; d[j] = nv; }
+        if (!hb) s.rE[0] = mk(0);
+    }
+    __syncthreads();
+    // 4. E_z, B_z and R, L of every node
+    for (int j = t0; j < ny; j += kLW) {
         const long o = P0 + long(j) * nx;
-        cx e0 = mk(0);
-        if (j >= 1) { nE = c.Ez[o] - cpE[long(j) * nx + ik] * nE; e0 = nE; }
-        nB = c.Bz[o] - cpB[long(j) * nx + ik] * nB;
-        if (j + 1 < ny) finish(j + 1, e1, e2, b1, nB);
-        e2 = e1; e1 = e0; b1 = nB;
+        const cx ez = s.rE[j], ezu = j + 1 < ny ? s.rE[j + 1] : mk(0), bz = s.rB[j];
+        const cx bzl = j > 0 ? s.rB[j - 1] : mk(0);
+        c.Ez[o] = ez; c.Bz[o] = bz;
+        if (j == 0) { c.Rx[o] = mk(0); c.Lx[o] = mk(0); }
+        else {
+            const cx bm = (bz - bzl) * iA[j];
+            c.Rx[o] = s.sRx[j] + D * (a * ez + bm);
+            c.Lx[o] = s.sLx[j] - qh * (a * ez - bm);
+        }
+        const cx bp = (ezu - ez) * iD[j];
+        c.Ry[o] = s.sRy[j] + D * (bp - a * bz);
+        c.Ly[o] = s.sLy[j] - qh * (a * bz + bp);
     }
-    finish(0, e1, e2, b1, mk(0));
+    line_sync();
 }
 
-__global__ void __launch_bounds__(kT) k_march_lines(LineView L, int lo, int hi) {
-    const int ik = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ik >= L.mv.nx) return;
-    for (int n = hi - 1; n >= lo; --n) march_line(L, ik, n);
+// slices hi-1 ... lo of line blockIdx.x
+__global__ void __launch_bounds__(kLW) k_march_lines(LineView L, int lo, int hi) {
+    extern __shared__ double lds_line[];
+    const LineLds s = line_lds(lds_line, L.ny);
+    for (int n = hi - 1; n >= lo; --n) march_line(L, blockIdx.x, n, s);
 }
 
 // ---------------------------------------------------------------- filters of a line of currents
@@ -320,15 +346,18 @@ __global__ void k_fh_comoving_R(const cx* psi, const double* kx, const double* i
 }
 
 // ---------------------------------------------------------------- one pass of a plasma half-cell, per line
-// Per line ik: the y filters, the x filter, node 0 and the smoothing (1 + a^2 (k_x^2 - Lap_y))^-1 of the streaming
-// current of this pass (X: the per-row unpacked x transforms, modified in place); kind 2 stops there (jsn).
-// Otherwise per point the bookkeeping of the pass and the pass backup of slice n, the march of the line, on the
-// last pass the correction of R at slice n+1, and the three packed planes of slice n for the inverse transforms.
-__global__ void __launch_bounds__(kT) k_pass_lines(ModeArgs m, LineView L, cx* X, cx* T, cx* jsn, int kind) {
-    const int ik = blockIdx.x * blockDim.x + threadIdx.x;
+// Per line ik (one workgroup of kLW lanes): the y filters, the x filter, node 0 and the smoothing
+// (1 + a^2 (k_x^2 - Lap_y))^-1 of the streaming current of this pass (X: the per-row unpacked x transforms, modified
+// in place); kind 2 stops there (jsn).  Otherwise per point the bookkeeping of the pass and the pass backup of slice
+// n, the march of the line, on the last pass the correction of R at slice n+1, and the three packed planes of
+// slice n for the inverse transforms.  The per-node work is spread over the lanes; the smoothing eliminations are
+// recurrences along y and run one system per lane (three lanes).
+__global__ void __launch_bounds__(kLW) k_pass_lines(ModeArgs m, LineView L, cx* X, cx* T, cx* jsn, int kind) {
+    extern __shared__ double lds_line[];
+    const LineLds s = line_lds(lds_line, L.ny);
+    const int ik = blockIdx.x, t0 = threadIdx.x;
     const MarchView& c = m.mv;
     const int nx = c.nx, ny = L.ny;
-    if (ik >= nx) return;
     const long nk = c.nk;
     const int n = m.n;
     const double* ia = L.iA;
@@ -341,29 +370,41 @@ __global__ void __launch_bounds__(kT) k_pass_lines(ModeArgs m, LineView L, cx* X
     if (kind != 1) {
         const double fx = m.filt[ik];
         if (m.kf) {
-            Filter121 fxn{mk(0), at(ox, 1) / ia[1]}, fzn{mk(0), at(oz, 1) / ia[1]};
-            Filter121 fyh{at(oy, 0) / id[0], at(oy, 0) / id[0]};
-            for (int j = 0; j < ny; ++j) {
-                const bool more = j + 1 < ny;
-                const cx y1 = more ? at(oy, j + 1) / id[j + 1] : fyh.q0;
-                cx x1 = mk(0), z1 = mk(0);
-                if (more && j >= 1) { x1 = at(ox, j + 1) / ia[j + 1]; z1 = at(oz, j + 1) / ia[j + 1]; }
-                at(oy, j) = fx * fyh.step(y1, id[j]);
-                if (j >= 1) { at(ox, j) = fx * fxn.step(x1, ia[j]); at(oz, j) = fx * fzn.step(z1, ia[j]); }
+            // (1,2,1)/4 of the charges q = value / weight: the charges of every node first (they are read by the
+            // neighbours), q = 0 beyond the walls for the nodes, mirrored for the half-nodes
+            cx* qx = s.sRx;
+            cx* qy = s.sRy;
+            cx* qz = s.sLx;
+            for (int j = t0; j < ny; j += kLW) {
+                qy[j] = at(oy, j) / id[j];
+                if (j >= 1) { qx[j] = at(ox, j) / ia[j]; qz[j] = at(oz, j) / ia[j]; }
+            }
+            __syncthreads();
+            for (int j = t0; j < ny; j += kLW) {
+                const cx ym = j > 0 ? qy[j - 1] : qy[0], yp = j + 1 < ny ? qy[j + 1] : qy[j];
+                at(oy, j) = fx * ((0.25 * ym + 0.5 * qy[j] + 0.25 * yp) * id[j]);
+                if (j >= 1) {
+                    const cx xm = j > 1 ? qx[j - 1] : mk(0), xp = j + 1 < ny ? qx[j + 1] : mk(0);
+                    const cx zm = j > 1 ? qz[j - 1] : mk(0), zp = j + 1 < ny ? qz[j + 1] : mk(0);
+                    at(ox, j) = fx * ((0.25 * xm + 0.5 * qx[j] + 0.25 * xp) * ia[j]);
+                    at(oz, j) = fx * ((0.25 * zm + 0.5 * qz[j] + 0.25 * zp) * ia[j]);
+                }
             }
         } else
-            for (int j = 0; j < ny; ++j) {
+            for (int j = t0; j < ny; j += kLW) {
                 if (j >= 1) { at(ox, j) = fx * at(ox, j); at(oz, j) = fx * at(oz, j); }
                 at(oy, j) = fx * at(oy, j);
             }
-        at(ox, 0) = fx * mk(0); at(oz, 0) = fx * mk(0);
+        if (t0 == 0) { at(ox, 0) = fx * mk(0); at(oz, 0) = fx * mk(0); }
+        line_sync();
         const double a2 = m.a2;
-        if (a2 > 0) {
+        if (a2 > 0 && t0 < 3) {
+            // lane 0: j_x, lane 1: j_z on the nodes (zero at the walls, as E_z in the march); lane 2: j_y on the
+            // half-nodes (as B_z in the march)
             const double kx2 = m.kxs[ik] * m.kxs[ik];
-            double* cpv = L.scp;
-            // nodes (j_x, j_z): zero at the walls, as E_z in the march
-            for (int t = 0; t < 2; ++t) {
-                cx* A = t ? oz : ox;
+            double* cpv = t0 == 0 ? s.bE : (t0 == 1 ? s.lE : s.uE);
+            if (t0 < 2) {
+                cx* A = t0 ? oz : ox;
                 double cp = 0;
                 cx dp = mk(0);
                 for (int j = 1; j < ny; ++j) {
@@ -371,34 +412,35 @@ __global__ void __launch_bounds__(kT) k_pass_lines(ModeArgs m, LineView L, cx* X
                     const double den = 1 + a2 * kx2 + a2 * ia[j] * (id[j - 1] + id[j]) - lo * cp;
                     dp = (at(A, j) - lo * dp) / den;
                     cp = up / den;
-                    at(A, j) = dp; cpv[o + long(j) * nx] = cp;
+                    at(A, j) = dp; cpv[j] = cp;
                 }
                 cx next = mk(0);
-                for (int j = ny - 1; j >= 1; --j) { next = at(A, j) - cpv[o + long(j) * nx] * next; at(A, j) = next; }
+                for (int j = ny - 1; j >= 1; --j) { next = at(A, j) - cpv[j] * next; at(A, j) = next; }
+            } else {
+                double cp = 0;
+                cx dp = mk(0);
+                for (int j = 0; j < ny; ++j) {
+                    const double al = j > 0 ? ia[j] : 0, au = j < ny - 1 ? ia[j + 1] : 0;
+                    const double lo = -a2 * id[j] * al, up = -a2 * id[j] * au;
+                    const double den = 1 + a2 * kx2 + a2 * id[j] * (al + au) - lo * cp;
+                    dp = (at(oy, j) - lo * dp) / den;
+                    cp = up / den;
+                    at(oy, j) = dp; cpv[j] = cp;
+                }
+                cx next = mk(0);
+                for (int j = ny - 1; j >= 0; --j) { next = at(oy, j) - cpv[j] * next; at(oy, j) = next; }
             }
-            // half-nodes (j_y): as B_z in the march
-            double cp = 0;
-            cx dp = mk(0);
-            for (int j = 0; j < ny; ++j) {
-                const double al = j > 0 ? ia[j] : 0, au = j < ny - 1 ? ia[j + 1] : 0;
-                const double lo = -a2 * id[j] * al, up = -a2 * id[j] * au;
-                const double den = 1 + a2 * kx2 + a2 * id[j] * (al + au) - lo * cp;
-                dp = (at(oy, j) - lo * dp) / den;
-                cp = up / den;
-                at(oy, j) = dp; cpv[o + long(j) * nx] = cp;
-            }
-            cx next = mk(0);
-            for (int j = ny - 1; j >= 0; --j) { next = at(oy, j) - cpv[o + long(j) * nx] * next; at(oy, j) = next; }
         }
+        if (a2 > 0) line_sync();
         if (kind == 2) {
-            for (int j = 0; j < ny; ++j) {
+            for (int j = t0; j < ny; j += kLW) {
                 const long i = o + long(j) * nx;
                 jsn[i] = ox[i]; jsn[nk + i] = oy[i]; jsn[2 * nk + i] = oz[i];
             }
             return;
         }
     }
-    for (int j = 0; j < ny; ++j) {
+    for (int j = t0; j < ny; j += kLW) {
         const long i = o + long(j) * nx, so = long(n - c.a) * nk + i;
         cx u[3] = {mk(0), mk(0), mk(0)};
         if (kind != 1) { u[0] = ox[i]; u[1] = oy[i]; u[2] = oz[i]; }
@@ -408,8 +450,9 @@ __global__ void __launch_bounds__(kT) k_pass_lines(ModeArgs m, LineView L, cx* X
             else { c.Rx[so] = m.bRx[i]; c.Ry[so] = m.bRy[i]; c.Ez[so] = m.bEz[i]; c.Bz[so] = m.bBz[i]; }
         }
     }
-    march_line(L, ik, n);
-    for (int j = 0; j < ny; ++j) {
+    line_sync();
+    march_line(L, ik, n, s);
+    for (int j = t0; j < ny; j += kLW) {
         const long i = o + long(j) * nx, so = long(n - c.a) * nk + i;
         if (m.last && !m.top) {
             const cx u[3] = {ox[i], oy[i], oz[i]};
@@ -428,7 +471,17 @@ template <class K, class... A> void launch(K k, long n, int t, const char* what,
 }
 }  // namespace
 
-void march_lines(const LineView& L, int lo, int hi) { launch(k_march_lines, L.mv.nx, kT, "march lines", L, lo, hi); }
+// one workgroup per line with the line's LDS
+static size_t line_lds_bytes(int ny) {
+    const size_t b = size_t(ny) * kLineLdsNode;
+    if (b > 64 * 1024) throw std::runtime_error("free-hand lines: ny " + std::to_string(ny) + " above the LDS of a line");
+    return b;
+}
+void march_lines(const LineView& L, int lo, int hi) {
+    if (L.mv.nx <= 0 || hi <= lo) return;
+    hipLaunchKernelGGL(k_march_lines, dim3(unsigned(L.mv.nx)), dim3(kLW), line_lds_bytes(L.ny), stream(), L, lo, hi);
+    launch_check("march lines");
+}
 void fh_currents(const cx* S1, cx* jkx, cx* jky, cx* jkz, int planes, const LineView& L, const double* filt, int kf,
                  int keep0, double Ly) {
     launch(k_fh_currents, long(planes) * L.mv.nx, kT, "free-hand currents", S1, jkx, jky, jkz, planes, L, filt, kf,
@@ -455,7 +508,10 @@ void fh_comoving_R(const cx* psi, const double* kx, const double* iD, double bet
     launch(k_fh_comoving_R, tot, 256, "free-hand comoving R", psi, kx, iD, beta, Rx, Ry, Ez, Bz, tot, nx, ny, nk);
 }
 void pass_lines(const ModeArgs& m, const LineView& L, cx* X, cx* T, cx* jsn, int kind) {
-    launch(k_pass_lines, m.mv.nx, kT, "pass lines", m, L, X, T, jsn, kind);
+    if (m.mv.nx <= 0) return;
+    hipLaunchKernelGGL(k_pass_lines, dim3(unsigned(m.mv.nx)), dim3(kLW), line_lds_bytes(L.ny), stream(), m, L, X, T,
+                       jsn, kind);
+    launch_check("pass lines");
 }
 
 }  // namespace g3d
