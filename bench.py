@@ -4,6 +4,11 @@ slice, two passes per half-cell, Gaussian grid laser a0 = 2, w0 = 30/k0, c tau =
 24/k0, dz = 20/k0).  A step is one s-step of the march (currents, the 1023 half-cells x 2 passes of the streaming
 plasma, sponge and real fields).  Metric: streaming-particle pushes per second including deposit and gather.
 
+--workload freehand: config 3 instead (512 Fourier x modes by 386 free-hand y cells between conducting walls,
+2048 slices on a zeta table from 0.1 to 0.5/k0, electrons 2 x 1 per cell, mobile hydrogen, nitrogen with ADK
+ionization, a0 = 2.5 Gaussian laser); metric: streaming-particle pushes per second (ions and freed electrons
+included), the plain FP64 form.
+
 --workload vacuum: the vacuum deck instead (64 x 64 modes, 256 slices, h = 0.5/k0, 2^18 beam electrons at
 gamma 1000 in the oblique plane wave a0 = 1, dz = 20/k0); a step is the RK4 push of every beam electron, its
 deposit and currents, the march of all slices, sponge and real fields; metric: beam pushes per second.
@@ -151,6 +156,99 @@ def vacuum_main(args, torch):
     print(json.dumps(out))
 
 
+N_IP = [14.53413, 29.60125, 47.4453, 77.4735, 97.8901, 552.06733, 667.04612]   # nitrogen, NIST [eV]
+AMU = 1822.888486
+
+
+def freehand_deck(Lw=615.5):
+    """config 3: 2048 slices at Lw = 615.5/k0 (38 at Lw = 8, the CPU sample)"""
+    import g3d_amd
+    d = g3d_amd.Deck(LAMBDA0)
+    d.c.nx = 512; d.c.Lx = 400.0
+    d.y_table([0.0, 18.4, 30.4], [0.2, 0.2, 2.0], 200.0)
+    if Lw > 300:
+        d.zeta_table([0.0, 20.0, 60.0, 100.0, 200.0, 300.0], [0.1, 0.1, 0.15, 0.2, 0.3, 0.5], Lw)
+    else:
+        d.zeta_table([0.0, 2.0, 4.0], [0.1, 0.1, 0.5], Lw)
+    Lz = float(d.zeta[-1])
+    d.set(dz=5.0, z_end=1000.0, plasma_on=1, n0=1e-3, ppc_x=2, ppc_y=1, move_x=0, sponge=8.0, sponge_y=8.0,
+          iterations=2)
+    d.add_grid_laser(a0=2.5, waist=25.0, tau=12.0, zeta_c=Lz - 40.0 if Lz > 100 else 0.6 * Lz)
+    d.add_ion(1.00794 * AMU, 1, 1, atoms=1.0, mobile=True, ionization=False)
+    d.add_ion(14.0067 * AMU, 0, 7, atoms=0.01, mobile=True, ionization=True, ip_eV=N_IP)
+    return d
+
+
+def freehand_main(args, torch):
+    import g3d_amd
+    sim = g3d_amd.Sim(freehand_deck())
+    t0 = time.perf_counter()
+    sim.init()
+    t_init = time.perf_counter() - t0
+    print("freehand: init %.2f s" % t_init, file=sys.stderr, flush=True)
+    for k in range(args.warmup):
+        sim.step()
+        print("freehand: warmup step %d done" % k, file=sys.stderr, flush=True)
+    N, nk = sim.info()["N"], sim.info()["nk"]
+    torch.cuda.synchronize()
+    sim.timing(["stream_halfcell"])
+    sim.timing_every(TIME_EVERY)
+    t0 = time.perf_counter()
+    pushes = 0
+    for k in range(args.steps):
+        sim.step()
+        pushes += sim.info()["nstrm"] * (N - 1) * 2
+        print("freehand: step %d done" % k, file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms, cnt = sim.timers()["stream_halfcell"]
+    avg_s = ms / 1000 / max(cnt, 1)
+    sim.timing_every(1)
+    nstrm = sim.info()["nstrm"]
+    bpl = stream_bytes_per_launch(nstrm, nk)
+    sim.timing(True)
+    sim.step()
+    torch.cuda.synchronize()
+    fam = {k: {"ms_total": round(v[0], 3), "launches": v[1], "avg_us": round(1000 * v[0] / v[1], 2)}
+           for k, v in sim.timers().items() if v[1]}
+    sim.timing(False)
+    cpu = None
+    if not args.no_cpu:
+        thr = args.cpu_threads or min(16, os.cpu_count() or 1)
+        try:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import refpy
+            if refpy.available():
+                refpy.set_threads(thr)
+                ref = refpy.RefSim(freehand_deck(Lw=8.0))
+                ref.init()
+                t = time.perf_counter()
+                ref.step()
+                dtc = time.perf_counter() - t
+                n_ref = ref.info()["N"]
+                cpu = {"value": ref.info()["nstrm"] * (n_ref - 1) * 2 / dtc, "unit": "pushes/s", "cores": thr,
+                       "kind": "reference",
+                       "sample": "reference CPU build (g++ -O3 -fopenmp, %d threads), the same transverse deck with "
+                                 "%d slices, 1 step: %.2f s" % (thr, n_ref, dtc)}
+        except Exception as e:
+            print("cpu baseline failed: %s" % e, file=sys.stderr)
+    out = {
+        "metric": "particle pushes/s incl. deposit+gather+ADK, free-hand y",
+        "value": pushes / dt, "unit": "pushes/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1000 * dt / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "free-hand y plasma 512x386x2048 with H and N (ADK), FP64", "N": N, "nstrm": nstrm,
+                   "slice_solves_per_s": N * args.steps / dt},
+        "roofline": {"kernel": "k_stream_halfcell", "bound": "hbm", "achieved": bpl / avg_s / 1e9,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": bpl / avg_s / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": bpl, "avg_launch_us": avg_s * 1e6},
+        "cpu_baseline": cpu,
+        "kernel_families_one_extra_step": fam,
+        "init_s": t_init,
+    }
+    print(json.dumps(out))
+
+
 def stream_bytes_per_launch(nstrm, nk):
     """Algorithmic HBM bytes of one streaming half-cell pass (k_stream_halfcell).
     Per particle: read x y z zeta px py pz w q qm (80 B) + alive sp (8 B), write x y z zeta px py pz q qm (72 B)
@@ -276,8 +374,9 @@ def main():
     ap.add_argument("--precision", choices=("both", "mixed", "fp64"), default="both",
                     help="both (default): the FP64 and the FP32-mixed march on the same deck; the line reports the "
                          "mixed one and its field error against FP64")
-    ap.add_argument("--workload", choices=("lwfa", "vacuum"), default="lwfa",
-                    help="lwfa (default): the plasma march of the headline metric; vacuum: the beam-push deck")
+    ap.add_argument("--workload", choices=("lwfa", "vacuum", "freehand"), default="lwfa",
+                    help="lwfa (default): the plasma march of the headline metric; vacuum: the beam-push deck; "
+                         "freehand: config 3 (free-hand y, ions, ADK)")
     ap.add_argument("--copy-calib", type=float, default=0.0,
                     help="only copy a buffer of this many MiB a few times (the reference kernel that scales the HBM "
                          "byte counters of a profile) and exit")
@@ -293,6 +392,9 @@ def main():
     assert torch.cuda.is_available(), "no GPU"
     if args.workload == "vacuum":
         vacuum_main(args, torch)
+        return
+    if args.workload == "freehand":
+        freehand_main(args, torch)
         return
 
     forms = {"fp64": [False], "mixed": [True], "both": [False, True]}[args.precision]
