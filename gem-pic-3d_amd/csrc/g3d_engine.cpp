@@ -664,10 +664,20 @@ void Sim::sponge_and_real_fields() {
         fft_run(fft, S1.data(), ns, false);
         if (!yf) fh_sponge_unpack(fs, L, S1.data(), S2.data(), Rx.data(), Ry.data());
     }
+    // the FP32-mixed form: the three planes of the real fields transformed in FP32 (F is float in both forms)
+    bool ions = false;
+    for (const auto& is : cfg.ions) ions = ions || cfg.loaded(is);
+    const bool f32 = cfg.mixed && yf && !ions && g.cm;
+    cxf* T3f = f32 ? reinterpret_cast<cxf*>(T3.data()) : nullptr;
     fields_pack3(g, sp && yf ? S1.data() : nullptr, S2.data(), opb, Rx.data(), Ry.data(), Lx.data(), Ly.data(),
-                 Ez.data(), Bz.data(), T3.data(), ts, ns);
-    fft_run(fft, T3.data(), 3 * ns, true);
-    fields_to_F(T3.data(), ts, F.data(), long(ns + 2) * nk, nk, ns);
+                 Ez.data(), Bz.data(), T3.data(), ts, ns, T3f);
+    if (f32) {
+        fft_run_f(fft, T3f, 3 * ns, true);
+        fields_to_F_f(T3f, ts, F.data(), long(ns + 2) * nk, nk, ns);
+    } else {
+        fft_run(fft, T3.data(), 3 * ns, true);
+        fields_to_F(T3.data(), ts, F.data(), long(ns + 2) * nk, nk, ns);
+    }
     toc(F_SPONGE);
 }
 

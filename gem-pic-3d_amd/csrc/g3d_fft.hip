@@ -37,17 +37,20 @@ using namespace fftcore;
 constexpr int kThreads = 256;
 constexpr int kTile = 2048;          // complex values per workgroup tile (32 KiB of LDS)
 
-__global__ void __launch_bounds__(kThreads) k_fft_rows(cx* data, long nrows, int n, int logn, int rpw, const cx* tw,
-                                                       int inv, double scale) {
-    extern __shared__ cx sm[];
+// C: cx (FP64) or cxf (FP32); S: its real type
+template <class C, class S>
+__global__ void __launch_bounds__(kThreads) k_fft_rows(C* data, long nrows, int n, int logn, int rpw, const cx* tw,
+                                                       int inv, S scale) {
+    extern __shared__ __align__(16) unsigned char smraw[];
+    C* sm = reinterpret_cast<C*>(smraw);
     const int sp = padded(n);
-    cx* buf = sm;
-    cx* tws = sm + size_t(rpw) * sp;
+    C* buf = sm;
+    C* tws = sm + size_t(rpw) * sp;
     const long row0 = long(blockIdx.x) * rpw;
     const long left = nrows - row0;
     const int nr = left < rpw ? int(left) : rpw;
     load_twiddles(tws, tw, n >> 1, inv);
-    cx* g = data + row0 * n;
+    C* g = data + row0 * n;
     const int tot = nr * n;
     for (int e = threadIdx.x; e < tot; e += blockDim.x) {
         const int r = e >> logn, i = e & (n - 1);
@@ -57,22 +60,24 @@ __global__ void __launch_bounds__(kThreads) k_fft_rows(cx* data, long nrows, int
     lds_fft(buf, n, logn, nr, sp, tws);
     for (int e = threadIdx.x; e < tot; e += blockDim.x) {
         const int r = e >> logn, i = e & (n - 1);
-        const cx v = buf[r * sp + lpos(i)];
-        g[e] = scale == 1.0 ? v : scale * v;
+        const C v = buf[r * sp + lpos(i)];
+        g[e] = scale == S(1) ? v : scale * v;
     }
 }
 
 // columns c0 .. c0+nc-1 of plane p; LDS holds column c at [c * padded(ny)]
-__global__ void __launch_bounds__(kThreads) k_fft_cols(cx* data, int nx, int ny, int logny, int nc, const cx* tw,
-                                                       int inv, double scale) {
-    extern __shared__ cx sm[];
+template <class C, class S>
+__global__ void __launch_bounds__(kThreads) k_fft_cols(C* data, int nx, int ny, int logny, int nc, const cx* tw,
+                                                       int inv, S scale) {
+    extern __shared__ __align__(16) unsigned char smraw[];
+    C* sm = reinterpret_cast<C*>(smraw);
     const int sp = padded(ny);
-    cx* buf = sm;
-    cx* tws = sm + size_t(nc) * sp;
+    C* buf = sm;
+    C* tws = sm + size_t(nc) * sp;
     const int tiles = nx / nc;
     const long p = blockIdx.x / tiles;
     const int c0 = (blockIdx.x % tiles) * nc;
-    cx* g = data + p * long(nx) * ny + c0;
+    C* g = data + p * long(nx) * ny + c0;
     load_twiddles(tws, tw, ny >> 1, inv);
     const int tot = nc * ny;
     for (int e = threadIdx.x; e < tot; e += blockDim.x) {
@@ -83,8 +88,8 @@ __global__ void __launch_bounds__(kThreads) k_fft_cols(cx* data, int nx, int ny,
     lds_fft(buf, ny, logny, nc, sp, tws);
     for (int e = threadIdx.x; e < tot; e += blockDim.x) {
         const int j = e / nc, c = e - j * nc;
-        const cx v = buf[c * sp + lpos(j)];
-        g[long(j) * nx + c] = scale == 1.0 ? v : scale * v;
+        const C v = buf[c * sp + lpos(j)];
+        g[long(j) * nx + c] = scale == S(1) ? v : scale * v;
     }
 }
 
@@ -120,19 +125,21 @@ void fft_plan_free(FftPlan& p) {
     p.twx = p.twy = nullptr;
 }
 
-void fft_rows(const FftPlan& p, cx* data, long nrows, bool inverse, double scale) {
+template <class C, class S>
+void fft_rows_t(const FftPlan& p, C* data, long nrows, bool inverse, S scale) {
     if (nrows <= 0) return;
     const int n = p.nx;
     int rpw = kTile / n;
     if (rpw < 1) rpw = 1;
     while (rpw > 1 && (nrows + rpw - 1) / rpw < 512) rpw >>= 1;
-    const size_t lds = (size_t(rpw) * (n + n / 8) + n / 2) * sizeof(cx);
+    const size_t lds = (size_t(rpw) * (n + n / 8) + n / 2) * sizeof(C);
     const int thr = threads_for(rpw * (n / 8 > 0 ? n / 8 : 1));
-    hipLaunchKernelGGL(k_fft_rows, dim3(blocks_for(nrows, rpw)), dim3(thr), lds, stream(), data, nrows, n, p.lx,
+    hipLaunchKernelGGL((k_fft_rows<C, S>), dim3(blocks_for(nrows, rpw)), dim3(thr), lds, stream(), data, nrows, n, p.lx,
                        rpw, p.twx, int(inverse), scale);
     launch_check("fft rows");
 }
-void fft_cols(const FftPlan& p, cx* data, int count, bool inverse, double scale) {
+template <class C, class S>
+void fft_cols_t(const FftPlan& p, C* data, int count, bool inverse, S scale) {
     if (count <= 0) return;
     static const int forced = [] { const char* e = getenv("G3D_FFT_COLS_NC"); return e ? atoi(e) : 0; }();
     int nc = kTile / p.ny;
@@ -140,12 +147,18 @@ void fft_cols(const FftPlan& p, cx* data, int count, bool inverse, double scale)
     while (nc > 4 && long(count) * (p.nx / nc) < 512) nc >>= 1;   // >= 4 columns: 64-byte row segments
     if (nc < 1) nc = 1;
     if (forced > 0 && forced <= p.nx && p.nx % forced == 0) nc = forced;
-    const size_t lds = (size_t(nc) * (p.ny + p.ny / 8) + p.ny / 2) * sizeof(cx);
+    const size_t lds = (size_t(nc) * (p.ny + p.ny / 8) + p.ny / 2) * sizeof(C);
     const long blocks = long(count) * (p.nx / nc);
     const int thr = threads_for(nc * (p.ny / 8 > 0 ? p.ny / 8 : 1));
-    hipLaunchKernelGGL(k_fft_cols, dim3(unsigned(blocks)), dim3(thr), lds, stream(), data, p.nx, p.ny, p.ly, nc,
+    hipLaunchKernelGGL((k_fft_cols<C, S>), dim3(unsigned(blocks)), dim3(thr), lds, stream(), data, p.nx, p.ny, p.ly, nc,
                        p.twy, int(inverse), scale);
     launch_check("fft columns");
+}
+void fft_rows(const FftPlan& p, cx* data, long nrows, bool inverse, double scale) {
+    fft_rows_t(p, data, nrows, inverse, scale);
+}
+void fft_cols(const FftPlan& p, cx* data, int count, bool inverse, double scale) {
+    fft_cols_t(p, data, count, inverse, scale);
 }
 void fft_run(const FftPlan& p, cx* data, int count, bool inverse) {
     if (count <= 0) return;
@@ -153,6 +166,14 @@ void fft_run(const FftPlan& p, cx* data, int count, bool inverse) {
     if (!p.two_d) { fft_rows(p, data, rows, inverse, inverse ? 1.0 / p.nx : 1.0); return; }
     fft_rows(p, data, rows, inverse, 1.0);
     fft_cols(p, data, count, inverse, inverse ? 1.0 / (double(p.nx) * p.ny) : 1.0);
+}
+// the same transforms in FP32 (the same butterflies and twiddles, rounded to float)
+void fft_run_f(const FftPlan& p, cxf* data, int count, bool inverse) {
+    if (count <= 0) return;
+    const long rows = long(count) * p.ny;
+    if (!p.two_d) { fft_rows_t(p, data, rows, inverse, inverse ? float(1.0 / p.nx) : 1.0f); return; }
+    fft_rows_t(p, data, rows, inverse, 1.0f);
+    fft_cols_t(p, data, count, inverse, inverse ? float(1.0 / (double(p.nx) * p.ny)) : 1.0f);
 }
 
 }  // namespace g3d

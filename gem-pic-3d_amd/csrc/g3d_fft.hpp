@@ -23,6 +23,8 @@ void fft_plan_init(FftPlan& p, int nx, int ny, bool two_d);
 void fft_plan_free(FftPlan& p);
 // count planes starting at data, in place, on the library stream
 void fft_run(const FftPlan& p, cx* data, int count, bool inverse);
+// the same in FP32 (FP32-mixed form)
+void fft_run_f(const FftPlan& p, cxf* data, int count, bool inverse);
 // rows only (x transforms) / columns only (y transforms) of count planes; scale multiplies the result
 void fft_rows(const FftPlan& p, cx* data, long nrows, bool inverse, double scale);
 void fft_cols(const FftPlan& p, cx* data, int count, bool inverse, double scale);

@@ -157,8 +157,12 @@ __global__ void __launch_bounds__(256) k_sponge_pack_cm(SpecGeom g, const cx* rx
     const cx u = trx[b][a] - bx, v = tr_y[b][a] - by;
     s1[idx] = mk(u.re - v.im, u.im + v.re);
 }
+// TO: cx, or cxf for the FP32 transforms of the FP32-mixed form
+__device__ __forceinline__ void put(cx* T, long i, cx v) { T[i] = v; }
+__device__ __forceinline__ void put(cxf* T, long i, cx v) { T[i] = to_f(v); }
+template <class TO>
 __global__ void __launch_bounds__(256) k_pack3_cm(SpecGeom g, const cx* s1, const cx* s2, double opb, cx* rx, cx* ry,
-                                                  const cx* lx, const cx* ly, const cx* ez, const cx* bz, cx* T,
+                                                  const cx* lx, const cx* ly, const cx* ez, const cx* bz, TO* T,
                                                   long ts) {
     __shared__ cx tile[6][kTT][kTT + 1];
     const int ntx = g.nx / kTT, nty = g.ny / kTT;
@@ -180,9 +184,9 @@ __global__ void __launch_bounds__(256) k_pack3_cm(SpecGeom g, const cx* s1, cons
         q = Y - opb * (ik(g.ky[j]) * s2[idx]);
     }
     const cx lv = tile[2][b][a], m = tile[3][b][a], e = tile[4][b][a], bv = tile[5][b][a];
-    T[idx] = mk(r.re - q.im, r.im + q.re);
-    T[ts + idx] = mk(lv.re - m.im, lv.im + m.re);
-    T[2 * ts + idx] = mk(e.re - bv.im, e.im + bv.re);
+    put(T, idx, mk(r.re - q.im, r.im + q.re));
+    put(T, ts + idx, mk(lv.re - m.im, lv.im + m.re));
+    put(T, 2 * ts + idx, mk(e.re - bv.im, e.im + bv.re));
     if (!s1) return;
     __syncthreads();
     tile[0][b][a] = r; tile[1][b][a] = q;
@@ -191,6 +195,17 @@ __global__ void __launch_bounds__(256) k_pack3_cm(SpecGeom g, const cx* s1, cons
 }
 // E_x = (R_x + L_x)/2, E_y = (R_y + L_y)/2, B_x = (L_y - R_y)/2, B_y = (R_x - L_x)/2, E_z, B_z
 // (src/sim_fields.cpp:280-293) into float F
+__global__ void k_to_F_f(const cxf* T, long ts, float* F, long fpl, long tot) {
+    const long idx = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= tot) return;
+    const cxf R = T[idx], L = T[ts + idx], Z = T[2 * ts + idx];
+    F[0 * fpl + idx] = 0.5f * (R.re + L.re);
+    F[1 * fpl + idx] = 0.5f * (R.im + L.im);
+    F[3 * fpl + idx] = 0.5f * (L.im - R.im);
+    F[4 * fpl + idx] = 0.5f * (R.re - L.re);
+    F[2 * fpl + idx] = Z.re;
+    F[5 * fpl + idx] = Z.im;
+}
 __global__ void k_to_F(const cx* T, long ts, float* F, long fpl, long tot) {
     const long idx = long(blockIdx.x) * blockDim.x + threadIdx.x;
     if (idx >= tot) return;
@@ -444,13 +459,18 @@ void sponge_mask(const SpecGeom& g, cx* S1, const double* mx, const double* mn, 
     G3D_LAUNCH(k_sponge_mask, long(count) * g.nk, g, S1, mx, mn, mh, long(count) * g.nk);
 }
 void fields_pack3(const SpecGeom& g, const cx* S1, const cx* S2, double opb, cx* Rx, cx* Ry, const cx* Lx,
-                  const cx* Ly, const cx* Ez, const cx* Bz, cx* T, long ts, int count) {
+                  const cx* Ly, const cx* Ez, const cx* Bz, cx* T, long ts, int count, cxf* Tf) {
+    if (Tf && !g.cm) throw std::runtime_error("pack: FP32 planes need the column-major modes");
     if (g.cm) {
         if (g.nx % kTT || g.ny % kTT) throw std::runtime_error("pack: column-major modes need nx, ny multiples of 16");
         const long nb = long(count) * (g.nx / kTT) * (g.ny / kTT);
         if (nb <= 0) return;
-        hipLaunchKernelGGL(k_pack3_cm, dim3(unsigned(nb)), dim3(256), 0, stream(), g, S1, S2, opb, Rx, Ry, Lx, Ly, Ez, Bz,
-                           T, ts);
+        if (Tf)
+            hipLaunchKernelGGL(k_pack3_cm<cxf>, dim3(unsigned(nb)), dim3(256), 0, stream(), g, S1, S2, opb, Rx, Ry, Lx,
+                               Ly, Ez, Bz, Tf, ts);
+        else
+            hipLaunchKernelGGL(k_pack3_cm<cx>, dim3(unsigned(nb)), dim3(256), 0, stream(), g, S1, S2, opb, Rx, Ry, Lx,
+                               Ly, Ez, Bz, T, ts);
         launch_check("fields pack (column-major)");
         return;
     }
@@ -458,6 +478,9 @@ void fields_pack3(const SpecGeom& g, const cx* S1, const cx* S2, double opb, cx*
 }
 void fields_to_F(const cx* T, long ts, float* F, long fpl, long nk, int count) {
     G3D_LAUNCH(k_to_F, long(count) * nk, T, ts, F, fpl, long(count) * nk);
+}
+void fields_to_F_f(const cxf* T, long ts, float* F, long fpl, long nk, int count) {
+    G3D_LAUNCH(k_to_F_f, long(count) * nk, T, ts, F, fpl, long(count) * nk);
 }
 void sspec_pack(double* J, long nk, int nx, int ny, cx* pk, double* rl, double* rli, int i0, int j0,
                 const TileGeom& tg) {

@@ -143,11 +143,13 @@ void currents_unpack(const SpecGeom& g, const cx* S1, cx* jkx, cx* jky, cx* jkz,
 void sponge_pack(const SpecGeom& g, const cx* Rx, const cx* Ry, const cx* jkz, const double* hcg, const double* wlo,
                  const double* whi, double g2, double opb, cx* S1, cx* S2);
 void sponge_mask(const SpecGeom& g, cx* S1, const double* mx, const double* mn, const double* mh, int count);
-// unpack the sponged R and add R_b (when S1 != null), then pack R, L, E_z/B_z for the inverse transforms
+// unpack the sponged R and add R_b (when S1 != null), then pack R, L, E_z/B_z for the inverse transforms (into
+// Tf as complex float instead when Tf != null: the FP32 transforms of the FP32-mixed form, column-major modes)
 void fields_pack3(const SpecGeom& g, const cx* S1, const cx* S2, double opb, cx* Rx, cx* Ry, const cx* Lx,
-                  const cx* Ly, const cx* Ez, const cx* Bz, cx* T, long tstride, int count);
-// real fields from the three inverse transforms into F planes fl0 ... fl0+count-1
+                  const cx* Ly, const cx* Ez, const cx* Bz, cx* T, long tstride, int count, cxf* Tf = nullptr);
+// real fields from the three inverse transforms into F planes fl0 ... fl0+count-1 (FP64 or FP32 planes)
 void fields_to_F(const cx* T, long tstride, float* F, long fpl, long nk, int count);
+void fields_to_F_f(const cxf* T, long tstride, float* F, long fpl, long nk, int count);
 // streaming current of one half-cell: the sum of jsr and of the tile slabs covering each point (tg.ntile > 0),
 // packed as (jx + i jy, jz); the density lines (when rl != null); jsr is zeroed for the next pass
 void sspec_pack(double* J, long nk, int nx, int ny, cx* pk, double* rl, double* rli, int i0, int j0,
