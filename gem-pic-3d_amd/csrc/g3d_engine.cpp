@@ -216,7 +216,7 @@ Sim::Sim(const Config& c) : cfg(c) {
     counters.resize(4);
     cent_dev.resize(3);
     setup_ions();
-    if (cfg.plasma_on && yf) setup_tiles();
+    if (cfg.plasma_on) setup_tiles();
 }
 
 // Tiles of TX x TY cells (16 x 8 on large grids) with a margin of M nodes around the TSC reach of the tile; a
@@ -228,19 +228,23 @@ void Sim::setup_tiles() {
         if (sscanf(t, "%dx%d", &px, &py) != 2 || px < 8 || py < 8)
             throw std::runtime_error("G3D_TILE: expected <TX>x<TY>, both at least 8");
     }
-    const int TX = pick(cfg.nx, px), TY = pick(cfg.ny, py);
+    // free-hand y: rows of py cells from the first node (the last row may be short), the windows end at the walls
+    const int TX = pick(cfg.nx, px), TY = yf ? pick(cfg.ny, py) : (cfg.ny >= 2 * py ? py : 0);
     if (!TX || !TY) return;
     int M = 3;                                // G3D_TILE_MARGIN: nodes of drift a window allows (sweeps)
     if (const char* e = getenv("G3D_TILE_MARGIN")) M = std::max(0, atoi(e));
-    M = std::min({M, TX - 1, TY - 1, (cfg.nx - TX - 3) / 2, (cfg.ny - TY - 3) / 2});
+    M = std::min({M, TX - 1, TY - 1, (cfg.nx - TX - 3) / 2});
+    if (yf) M = std::min(M, (cfg.ny - TY - 3) / 2);
     if (M < 0) return;
     tiles.TX = TX; tiles.TY = TY; tiles.M = M; tiles.WX = TX + 3 + 2 * M; tiles.WY = TY + 3 + 2 * M;
-    tiles.ntx = cfg.nx / TX; tiles.nty = cfg.ny / TY; tiles.ntile = long(tiles.ntx) * tiles.nty;
+    tiles.ntx = cfg.nx / TX; tiles.nty = (cfg.ny + TY - 1) / TY; tiles.ntile = long(tiles.ntx) * tiles.nty;
+    tiles.fh = !yf;
+    tiles.fy = yax;
     tile_off.resize(tiles.ntile + 1);
     slab.resize(size_t(tiles.ntile) * 5 * tiles.WX * tiles.WY);
     tiles.off = tile_off.data(); tiles.slab = slab.data(); tiles.n_tiled = 0;
     const char* e = getenv("G3D_STREAM_FWIN");
-    tiles.fwin = e ? atoi(e) : 1;
+    tiles.fwin = yf ? (e ? atoi(e) : 1) : 0;
     const char* b = getenv("G3D_STREAM_BLOCK");  // lanes per tile workgroup (about the particles of a tile)
     tiles.block = b ? atoi(b) : 512;
     const char* w = getenv("G3D_STREAM_WPE");     // register cap of the FP32-mixed stream kernel (0 or 4)
@@ -1092,7 +1096,9 @@ void Sim::march_plasma(double s0, double s1) {
             stream_halfcell(sc, strmA.view(), strmB.view(&strmA), nlist, tg, lt);
             launch_timed(F_STREAM, lt);
             tic(F_SFFT);
-            pass_rows_fwd(jsr.data(), tg, fft, sp3.data(), last ? rho_lines.data() + long(n) * (nx + ny) : nullptr,
+            TileGeom tr = tg;
+            if (tg.fh) tr.ntile = 0;             // free-hand: the windows are already in jsr
+            pass_rows_fwd(jsr.data(), tr, fft, sp3.data(), last ? rho_lines.data() + long(n) * (nx + ny) : nullptr,
                           last && rho_lines_i.size() ? rho_lines_i.data() + long(n) * (nx + ny) : nullptr, i0, j0, cm);
             toc(F_SFFT);
             tic(F_MODES);

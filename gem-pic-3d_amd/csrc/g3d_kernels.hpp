@@ -54,6 +54,11 @@ struct TileGeom {
     const int* xtile;
     const long* xstart;
     const int* xcount;
+    // free-hand y: the tile rows follow the node table (cell j of a particle from fy; the last row of tiles may be
+    // short), the windows stop at the walls, and the tile workgroups add their deposit window to the current
+    // planes with FP64 atomics instead of slabs (no field window)
+    int fh;
+    Axis fy;
     long ntile, n_tiled;              // tiles; list entries binned (= off[ntile]); from n_tiled on: untiled tail
     const long* off;
     const int* idx;                   // list indices grouped by tile (null: the list itself is tile-ordered)

@@ -191,6 +191,29 @@ __device__ __forceinline__ void sdeposit(const StreamArgs& c, const Win* win, do
         return;
     }
     tsc_freehand(P.yax, y, jn, wn, jh, wh);
+    if (win) {
+        // the window's rows are node (and half-node) indices from y0, not wrapped: the walls bound the grid
+        const int u0 = wrap_index(ix[0] - win->x0, P.nx), vn = jn[0] - win->y0, vh = jh[0] - win->y0;
+        if (u0 + 2 < win->WX && vn >= 0 && vn + 2 < win->WY && vh >= 0 && vh + 2 < win->WY) {
+            const long wpl = long(win->WX) * win->WY;
+            for (int v = 0; v < 3; ++v) {
+                const bool nod = jn[v] >= 1 && jn[v] <= P.ny - 1, hal = jh[v] >= 0 && jh[v] <= P.ny - 1;
+                for (int u = 0; u < 3; ++u) {
+                    if (nod) {
+                        double* L = win->lds + long(vn + v) * win->WX + (u0 + u);
+                        const double w = f * wx[u] * wn[v] * c.ia[jn[v]];
+                        if (mv) { atomicAdd(L, w * vx); atomicAdd(L + 2 * wpl, w * vz); }
+                        atomicAdd(L + cq * wpl, w);
+                    }
+                    if (hal && mv) {
+                        double* L = win->lds + long(vh + v) * win->WX + (u0 + u);
+                        atomicAdd(L + wpl, f * wx[u] * wh[v] * c.id[jh[v]] * vy);
+                    }
+                }
+            }
+            return;
+        }
+    }
     for (int v = 0; v < 3; ++v) {
         const bool nod = jn[v] >= 1 && jn[v] <= P.ny - 1, hal = jh[v] >= 0 && jh[v] <= P.ny - 1;
         for (int u = 0; u < 3; ++u) {
@@ -362,7 +385,7 @@ __global__ void __launch_bounds__(kStreamBlockMax) k_stream_halfcell(StreamArgs 
         const int fl = c.n - F.a;
         float* fw = reinterpret_cast<float*>(win_lds + wn);
         for (long r = threadIdx.x; r < wn; r += blockDim.x) win_lds[r] = 0;
-        const bool fwin = tg.fwin && F.self;
+        const bool fwin = YF == 1 && tg.fwin && F.self;
         if (fwin)        // slices fl, fl + 1 of the 6 components: [slice][comp][v][u]; one node per lane, 12 loads
             for (int r = threadIdx.x; r < pl; r += blockDim.x) {
                 const int v = r / tg.WX, u = r - v * tg.WX;
@@ -386,8 +409,18 @@ __global__ void __launch_bounds__(kStreamBlockMax) k_stream_halfcell(StreamArgs 
             }
         }
         __syncthreads();
-        double* sl = tg.slab + b * wn;
-        for (long r = threadIdx.x; r < wn; r += blockDim.x) sl[r] = win_lds[r];
+        if (YF == 1) {
+            double* sl = tg.slab + b * wn;
+            for (long r = threadIdx.x; r < wn; r += blockDim.x) sl[r] = win_lds[r];
+        } else                  // free-hand y: into the current planes (rows outside the walls hold nothing)
+            for (long r = threadIdx.x; r < wn; r += blockDim.x) {
+                const double v = win_lds[r];
+                if (v == 0) continue;
+                const int cc = int(r / pl), rr = int(r - long(cc) * pl), vv = rr / tg.WX, u = rr - vv * tg.WX;
+                const int gy = y0 + vv;
+                if (gy < 0 || gy >= F.ny) continue;
+                atomicAdd(&c.J[cc * F.nk + long(gy) * F.nx + wrap_index(x0 + u, F.nx)], v);
+            }
         return;
     }
     const long i = tg.n_tiled + (b - tg.ntile) * blockDim.x + threadIdx.x;
@@ -400,9 +433,16 @@ __global__ void __launch_bounds__(kStreamBlockMax) k_stream_halfcell(StreamArgs 
 // and gather from LDS windows.  Only live particles (alive 1, waiting 6) are binned: the others are final for the
 // rest of the step, and their state is copied into the second buffer here so that both buffers agree.
 __device__ __forceinline__ int tile_of(const TileGeom& tg, double x, double y, double dx, double dy, int nx, int ny) {
-    int ix = int(floor(x / dx)) % nx, iy = int(floor(y / dy)) % ny;
+    int ix = int(floor(x / dx)) % nx, iy;
     if (ix < 0) ix += nx;
-    if (iy < 0) iy += ny;
+    if (tg.fh) {                // free-hand y: the cell of the node table, clamped to the grid
+        double f;
+        tg.fy.locate(y, iy, f);
+        iy = iy < 0 ? 0 : (iy >= ny ? ny - 1 : iy);
+    } else {
+        iy = int(floor(y / dy)) % ny;
+        if (iy < 0) iy += ny;
+    }
     return (iy / tg.TY) * tg.ntx + ix / tg.TX;
 }
 __global__ void __launch_bounds__(kT) k_bin_count(Parts S, Parts D, long n, TileGeom tg, double dx, double dy,
