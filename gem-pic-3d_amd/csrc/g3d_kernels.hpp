@@ -169,7 +169,7 @@ struct ModeArgs {
     cx *jsn, *sin_, *si0, *pin, *dl;  // 3 planes each
     cx *js0x, *js0y, *js0z;
     cx *jkx, *jky, *jkz;              // writable views of the march's jk
-    cx *bRx, *bRy, *bEz, *bBz;        // backup of slice n for the passes
+    cx *bRx, *bRy, *bEz, *bBz;        // several passes: set (the free-hand line kernels keep slice n at s0 here)
     cx* T;                            // 3 packed planes for the inverse transforms
     const double *kxs, *kys, *filt, *filty;
     double wex, qcorr, a2;

@@ -122,11 +122,8 @@ __device__ __forceinline__ ModeLoad mode_load(const ModeArgs& m, long i) {
     }
     if (m.last && !m.top && !m.first) { L.si0[0] = bk_load<MIXED>(m.si0, i); L.si0[1] = bk_load<MIXED>(m.si0, nk + i); }
     const long o = long(n - c.a) * nk + i, on = o + nk;
-    if (!m.first && m.bRx) {
-        L.old.Rx = m.bRx[i]; L.old.Ry = m.bRy[i]; L.old.Ez = bk_load<MIXED>(m.bEz, i); L.old.Bz = bk_load<MIXED>(m.bBz, i);
-    } else {
-        L.old.Rx = c.Rx[o]; L.old.Ry = c.Ry[o]; L.old.Ez = c.Ez[o]; L.old.Bz = c.Bz[o];
-    }
+    // slice n at s0: the passes before the last leave it in place (below), so every pass reads it there
+    L.old.Rx = c.Rx[o]; L.old.Ry = c.Ry[o]; L.old.Ez = c.Ez[o]; L.old.Bz = c.Bz[o];
     L.up = Mode6{};
     if (n < c.N - 1) L.up = Mode6{c.Rx[on], c.Ry[on], c.Ez[on], c.Bz[on], c.Lx[on], c.Ly[on]};
     L.j0xo = c.j0x[o]; L.j0xn = c.j0x[on]; L.j0yo = c.j0y[o]; L.j0yn = c.j0y[on];
@@ -194,15 +191,16 @@ __device__ __forceinline__ void mode_compute(const ModeArgs& m, long i, const Mo
     }
     const long o = long(n - c.a) * nk + i;
     Mode6 old = L.old;
-    if (!(!m.first && m.bRx) && m.bRx) {
-        m.bRx[i] = old.Rx; m.bRy[i] = old.Ry; bk_store<MIXED>(m.bEz, i, old.Ez); bk_store<MIXED>(m.bBz, i, old.Bz);
-        if constexpr (MIXED) { old.Ez = to_d(to_f(old.Ez)); old.Bz = to_d(to_f(old.Bz)); }   // as later passes read them
-    }
+    // FP32-mixed with several passes: E_z and B_z of slice n enter every pass rounded to float (the precision of the
+    // pass bookkeeping)
+    if constexpr (MIXED) if (m.bRx) { old.Ez = to_d(to_f(old.Ez)); old.Bz = to_d(to_f(old.Bz)); }
     const Mode6& up = L.up;
     const long on = o + nk;
     const Cur9 J{L.j0xo, L.j0xn, L.j0yo, L.j0yn, q[0], pp[0], q[1], pp[1], pp[2]};
     const Mode6 r = MIXED ? march_mode_mixed(c, i, n, old, up, J) : march_mode(c, i, n, old, up, J);
-    c.Rx[o] = r.Rx; c.Ry[o] = r.Ry; c.Ez[o] = r.Ez; c.Bz[o] = r.Bz; c.Lx[o] = r.Lx; c.Ly[o] = r.Ly;
+    // a pass before the last (m.bRx set: several passes) leaves slice n at s0 for the next pass to start from; only
+    // its packed fields (T) go on to the particles
+    if (m.last || !m.bRx) { c.Rx[o] = r.Rx; c.Ry[o] = r.Ry; c.Ez[o] = r.Ez; c.Bz[o] = r.Bz; c.Lx[o] = r.Lx; c.Ly[o] = r.Ly; }
     if (m.last && !m.top) {                         // src/sim_plasma.cpp:381-396
         if (!have_si0) { si0[0] = L.si0[0]; si0[1] = L.si0[1]; }
         c.Rx[on] = up.Rx - m.qcorr * (u[0] - si0[0]);

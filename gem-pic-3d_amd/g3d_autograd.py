@@ -399,14 +399,70 @@ class BunchPush(torch.autograd.Function):
         F, *state = ctx.saved_tensors
         zeros = torch.zeros_like(state[0])
         bs = [(g if g is not None else zeros).contiguous() for g in grads]
-        gs = [torch.empty_like(state[0]) for _ in _BSTATE]
         gF = torch.zeros_like(F)
-        c = ctx.geo.ctx(F, list(state), *ctx.fixed)
-        g = BpushAdj()
-        for nm, t in zip(_BSTATE, bs):
-            setattr(g, "b" + nm, t.data_ptr())
-        for nm, t in zip(_BSTATE, gs):
-            setattr(g, "g" + nm, t.data_ptr())
-        g.gF = gF.data_ptr()
-        _call(g3d_amd.lib().g3d_bunch_push_vjp, C.byref(c), C.byref(g))
+        gs = bunch_push_vjp_raw(ctx.geo, F, state, *ctx.fixed, bs, gF)
         return (None, gF, *gs, None, None, None, None)
+
+
+def bunch_push_vjp_raw(geo, F, state, z, rem, qm, alive, bs, gF):
+    """the adjoints of the state before the push (new tensors) from those of the pushed state bs; the adjoint of F
+    is added to gF.  state, z, rem and alive are the ones before the push (left unchanged)"""
+    gs = [torch.empty_like(state[0]) for _ in _BSTATE]
+    c = geo.ctx(F, [t.clone() for t in state], z.clone(), rem.clone(), qm, alive.clone())
+    g = BpushAdj()
+    for nm, t in zip(_BSTATE, bs):
+        setattr(g, "b" + nm, t.data_ptr())
+    for nm, t in zip(_BSTATE, gs):
+        setattr(g, "g" + nm, t.data_ptr())
+    g.gF = gF.data_ptr()
+    _call(g3d_amd.lib().g3d_bunch_push_vjp, C.byref(c), C.byref(g))
+    return gs
+
+
+class BunchMarch(torch.autograd.Function):
+    """(x, y, ζ, p_x, p_y, p_z) of beam particles after len(geos) RK4 pushes, one PushGeometry per s-step, in the
+    field planes F [6 nsl ny nx] held over the steps.  The forward keeps the particle state only at every
+    `every`-th step; the reverse sweep recomputes each segment of steps from its checkpoint, then runs the steps'
+    reverse pushes rear to front (memory: one state per checkpoint plus one segment).  z, rem and the alive codes
+    move with the steps and are not differentiated."""
+    @staticmethod
+    def forward(ctx, geos, every, F, x, y, zeta, px, py, pz, z, rem, qm, alive):
+        if every < 1:
+            raise ValueError("BunchMarch: every must be >= 1")
+        state = [_f64(t, "state") for t in (x, y, zeta, px, py, pz)]
+        F = _f64(F, "F")
+        fix = (z.contiguous(), rem.contiguous(), alive.contiguous())
+        qm = qm.contiguous()
+        ckpts = []
+        for t, geo in enumerate(geos):
+            if t % every == 0:
+                ckpts.append((t, [u.clone() for u in state], *[u.clone() for u in fix]))
+            state, *fix = _push_step(geo, F, state, fix, qm)
+        ctx.geos, ctx.every, ctx.ckpts, ctx.np = list(geos), every, ckpts, state[0].numel()
+        ctx.save_for_backward(F, qm)
+        return tuple(state)
+
+    @staticmethod
+    def backward(ctx, *grads):
+        F, qm = ctx.saved_tensors
+        geos, every = ctx.geos, ctx.every
+        zeros = torch.zeros(ctx.np, dtype=torch.float64, device=F.device)
+        bs = [(g if g is not None else zeros).contiguous() for g in grads]
+        gF = torch.zeros_like(F)
+        for t0, state, z, rem, alive in reversed(ctx.ckpts):
+            seg = []                                   # the inputs of each step of the segment
+            fix = (z, rem, alive)
+            for t in range(t0, min(t0 + every, len(geos))):
+                seg.append((state, fix))
+                state, *fix = _push_step(geos[t], F, state, fix, qm)
+            for k in range(len(seg) - 1, -1, -1):
+                st, (zz, rr, al) = seg[k]
+                bs = bunch_push_vjp_raw(geos[t0 + k], F, st, zz, rr, qm, al, bs, gF)
+        ctx.ckpts = None
+        return (None, None, gF, *bs, None, None, None, None)
+
+
+def _push_step(geo, F, state, fix, qm):
+    z, rem, alive = fix
+    out, z2, rem2, al = bunch_push_raw(geo, F, state, z, rem, qm, alive)
+    return out, z2, rem2, al

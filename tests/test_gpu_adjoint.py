@@ -305,3 +305,73 @@ def test_bunch_push_gradcheck():
     assert (al == 1).all()
     f = lambda *a: ga.BunchPush.apply(geo, *a, *fixed)
     assert torch.autograd.gradcheck(f, ins, eps=1e-6, atol=1e-7, rtol=1e-5)
+
+
+def test_bunch_march_checkpoints():
+    """six RK4 pushes of the 32 golden beam particles (the oblique plane wave, one slow particle with many
+    sub-steps) with particle checkpoints every k steps (ga.BunchMarch): the gradient of a random linear functional
+    of the final state is the same for every k and equals the step-by-step autograd chain; in the initial state it
+    matches central differences (step 1e-6) of the oracle's chained bunch push within 1e-6 relative, and in the
+    field planes central differences of the forward chain"""
+    deck = gi.bunch_small_deck()
+    sim = g3d_amd.Sim(deck)
+    sim.init()
+    info = sim.info()
+    F32, P = gi.push_input(sim)
+    S, ds = 6, 20.0
+    geos = [bpush_geometry(deck, sim, ds * t, ds * (t + 1)) for t in range(S)]
+    names = ("x", "y", "zeta", "px", "py", "pz")
+    r = np.random.default_rng(77)
+    cot = [r.standard_normal(len(P)) for _ in names]
+    F = gpu(F32.astype(np.float64))
+    fixed = (gpu(P["z"]), gpu(P["rem"]), gpu(P["qm"]), gpu(P["alive"].astype(np.int32)))
+    z, rem, qm, alive = fixed
+
+    def grads(every):
+        ins = [F.clone().requires_grad_()] + [gpu(P[n].astype(np.float64)).requires_grad_() for n in names]
+        if every is None:                             # one autograd node per step
+            st, zz, rr, al = ins[1:], z, rem, alive
+            for geo in geos:
+                st2 = ga.BunchPush.apply(geo, ins[0], *st, zz, rr, qm, al)
+                _, zz, rr, al = ga.bunch_push_raw(geo, ins[0].detach(), [t.detach() for t in st], zz, rr, qm, al)
+                st = st2
+        else:
+            st = ga.BunchMarch.apply(geos, every, ins[0], *ins[1:], z, rem, qm, alive)
+        loss = sum((t * gpu(c)).sum() for t, c in zip(st, cot))
+        g = torch.autograd.grad(loss, ins)
+        return float(loss), [t.cpu().numpy() for t in g]
+
+    L0, g_chain = grads(None)
+    for every in (1, 2, 4, S, S + 3):
+        L, g = grads(every)
+        assert L == L0
+        for a, b in zip(g, g_chain):
+            assert np.allclose(a, b, rtol=1e-12, atol=1e-15 * np.abs(b).max()), every
+
+    def oracle_loss(Pin):
+        p = Pin
+        for t in range(S):
+            p = orc.bunch_push(orc.Grid(deck), deck, F32, p, ds * t, ds * (t + 1), info["X0"], info["Y0"])
+        assert (p["alive"] == 1).all()
+        return sum(float(np.dot(p[n], c)) for n, c in zip(names, cot))
+
+    d = [r.standard_normal(len(P)) for _ in names]
+    eps = 1e-6
+    Pp, Pm = P.copy(), P.copy()
+    for n, v in zip(names, d):
+        Pp[n] += eps * v
+        Pm[n] -= eps * v
+    fd = (oracle_loss(Pp) - oracle_loss(Pm)) / (2 * eps)
+    an = sum(float(np.dot(gs, v)) for gs, v in zip(g_chain[1:], d))
+    assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
+
+    dF = r.standard_normal(F32.size)
+    def gpu_loss(Fv):
+        st, zz, rr, al = [gpu(P[n].astype(np.float64)) for n in names], z, rem, alive
+        for geo in geos:
+            st, zz, rr, al = ga.bunch_push_raw(geo, gpu(Fv), st, zz, rr, qm, al)
+        return sum(float((t * gpu(c)).sum()) for t, c in zip(st, cot))
+    F64 = F32.astype(np.float64)
+    fdF = (gpu_loss(F64 + eps * dF) - gpu_loss(F64 - eps * dF)) / (2 * eps)
+    anF = float(np.dot(g_chain[0], dF))
+    assert abs(fdF - anF) <= 1e-6 * abs(anF), (fdF, anF)
