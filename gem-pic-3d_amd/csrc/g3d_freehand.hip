@@ -47,6 +47,41 @@ __device__ __forceinline__ void line_sync() {
     __syncthreads();
 }
 
+// One tridiagonal elimination and back substitution over LDS on nodes j0 .. n-1: lower off-diagonal lo, pivot
+// part bs, upper off-diagonal up, right-hand side v (replaced by the solution); the elimination factors go to cpo
+// (which may be up itself).  den = bs - lo c'(j-1), v'(j) = (v(j) - lo v'(j-1)) / den, c'(j) = up / den, then
+// v(j) = v'(j) - c'(j) v(j+1).  The operands of the next node are loaded while this one is eliminated, so that the
+// recurrence waits on its own arithmetic only.
+__device__ __forceinline__ void lds_thomas(cx* v, const double* lo, const double* bs, const double* up, double* cpo,
+                                           int j0, int n) {
+    if (j0 >= n) return;
+    double cv = 0;
+    cx dv = mk(0);
+    double l = lo[j0], b = bs[j0], u = up[j0];
+    cx r = v[j0];
+    for (int j = j0; j < n; ++j) {
+        const int jn = j + 1 < n ? j + 1 : j;
+        const double ln = lo[jn], bn = bs[jn], un = up[jn];
+        const cx rn = v[jn];
+        const double den = b - l * cv;
+        dv = (r - l * dv) / den;
+        cv = u / den;
+        v[j] = dv; cpo[j] = cv;
+        l = ln; b = bn; u = un; r = rn;
+    }
+    cx nv = mk(0);
+    cx rj = v[n - 1];
+    double cj = cpo[n - 1];
+    for (int j = n - 1; j >= j0; --j) {
+        const int jp = j > j0 ? j - 1 : j;
+        const cx rp = v[jp];
+        const double cp = cpo[jp];
+        nv = rj - cj * nv;
+        v[j] = nv;
+        rj = rp; cj = cp;
+    }
+}
+
 __device__ void march_line(const LineView& L, int ik, int n, const LineLds& s) {
     const MarchView& c = L.mv;
     const int nx = c.nx, ny = L.ny, t0 = threadIdx.x;
@@ -138,17 +173,7 @@ __device__ void march_line(const LineView& L, int ik, int n, const LineLds& s) {
         cx* d = hb ? s.rB : s.rE;
         const double *bs = hb ? s.bB : s.bE, *ls = hb ? s.lB : s.lE;
         double* cp = hb ? s.uB : s.uE;
-        double cv = 0;
-        cx dv = mk(0);
-        for (int j = j0; j < ny; ++j) {
-            const double lo = ls[j];
-            const double den = bs[j] - lo * cv;
-            dv = (d[j] - lo * dv) / den;
-            cv = cp[j] / den;
-            d[j] = dv; cp[j] = cv;
-        }
-        cx nv = mk(0);
-        for (int j = ny - 1; j >= j0; --j) { nv = d[j] - cp[j] * nv; d[j] = nv; }
+        lds_thomas(d, ls, bs, cp, cp, j0, ny);
         if (!hb) s.rE[0] = mk(0);
     }
     __syncthreads();
@@ -171,11 +196,18 @@ __device__ void march_line(const LineView& L, int ik, int n, const LineLds& s) {
     line_sync();
 }
 
-// slices hi-1 ... lo of line blockIdx.x
-__global__ void __launch_bounds__(kLW) k_march_lines(LineView L, int lo, int hi) {
+// the k_x line of workgroup b: workgroup b runs on XCD b mod 8, so with xcd the lines of one XCD are consecutive
+// (neighbouring lines share the cache lines of every access strided by nx, and each XCD has its own L2)
+__device__ __forceinline__ int line_of(int b, int nb, int xcd) {
+    return (xcd && (nb & 7) == 0) ? (b & 7) * (nb >> 3) + (b >> 3) : b;
+}
+
+// slices hi-1 ... lo of line line_of(blockIdx.x)
+__global__ void __launch_bounds__(kLW) k_march_lines(LineView L, int lo, int hi, int xcd) {
     extern __shared__ double lds_line[];
     const LineLds s = line_lds(lds_line, L.ny);
-    for (int n = hi - 1; n >= lo; --n) march_line(L, blockIdx.x, n, s);
+    const int ik = line_of(blockIdx.x, gridDim.x, xcd);
+    for (int n = hi - 1; n >= lo; --n) march_line(L, ik, n, s);
 }
 
 // ---------------------------------------------------------------- filters of a line of currents
@@ -352,10 +384,11 @@ __global__ void k_fh_comoving_R(const cx* psi, const double* kx, const double* i
 // n, the march of the line, on the last pass the correction of R at slice n+1, and the three packed planes of
 // slice n for the inverse transforms.  The per-node work is spread over the lanes; the smoothing eliminations are
 // recurrences along y and run one system per lane (three lanes).
-__global__ void __launch_bounds__(kLW) k_pass_lines(ModeArgs m, LineView L, cx* X, cx* T, cx* jsn, int kind) {
+__global__ void __launch_bounds__(kLW) k_pass_lines(ModeArgs m, LineView L, cx* X, cx* T, cx* jsn, int kind,
+                                                    int xcd) {
     extern __shared__ double lds_line[];
     const LineLds s = line_lds(lds_line, L.ny);
-    const int ik = blockIdx.x, t0 = threadIdx.x;
+    const int ik = line_of(blockIdx.x, gridDim.x, xcd), t0 = threadIdx.x;
     const MarchView& c = m.mv;
     const int nx = c.nx, ny = L.ny;
     const long nk = c.nk;
@@ -398,40 +431,35 @@ __global__ void __launch_bounds__(kLW) k_pass_lines(ModeArgs m, LineView L, cx* 
         if (t0 == 0) { at(ox, 0) = fx * mk(0); at(oz, 0) = fx * mk(0); }
         line_sync();
         const double a2 = m.a2;
-        if (a2 > 0 && t0 < 3) {
-            // lane 0: j_x, lane 1: j_z on the nodes (zero at the walls, as E_z in the march); lane 2: j_y on the
-            // half-nodes (as B_z in the march)
+        if (a2 > 0) {
+            // the three columns and the coefficients of their systems into LDS, over the lanes (the filters' charge
+            // arrays are free again): node coefficients for j_x and j_z (zero at the walls, as E_z in the march),
+            // half-node ones for j_y (as B_z in the march)
             const double kx2 = m.kxs[ik] * m.kxs[ik];
-            double* cpv = t0 == 0 ? s.bE : (t0 == 1 ? s.lE : s.uE);
-            if (t0 < 2) {
-                cx* A = t0 ? oz : ox;
-                double cp = 0;
-                cx dp = mk(0);
-                for (int j = 1; j < ny; ++j) {
-                    const double lo = j > 1 ? -a2 * ia[j] * id[j - 1] : 0, up = j < ny - 1 ? -a2 * ia[j] * id[j] : 0;
-                    const double den = 1 + a2 * kx2 + a2 * ia[j] * (id[j - 1] + id[j]) - lo * cp;
-                    dp = (at(A, j) - lo * dp) / den;
-                    cp = up / den;
-                    at(A, j) = dp; cpv[j] = cp;
+            cx *vx = s.rE, *vz = s.rB, *vy = s.sLy;
+            for (int j = t0; j < ny; j += kLW) {
+                vx[j] = at(ox, j); vz[j] = at(oz, j); vy[j] = at(oy, j);
+                if (j >= 1) {
+                    s.lE[j] = j > 1 ? -a2 * ia[j] * id[j - 1] : 0;
+                    s.uE[j] = j < ny - 1 ? -a2 * ia[j] * id[j] : 0;
+                    s.bE[j] = 1 + a2 * kx2 + a2 * ia[j] * (id[j - 1] + id[j]);
                 }
-                cx next = mk(0);
-                for (int j = ny - 1; j >= 1; --j) { next = at(A, j) - cpv[j] * next; at(A, j) = next; }
-            } else {
-                double cp = 0;
-                cx dp = mk(0);
-                for (int j = 0; j < ny; ++j) {
-                    const double al = j > 0 ? ia[j] : 0, au = j < ny - 1 ? ia[j + 1] : 0;
-                    const double lo = -a2 * id[j] * al, up = -a2 * id[j] * au;
-                    const double den = 1 + a2 * kx2 + a2 * id[j] * (al + au) - lo * cp;
-                    dp = (at(oy, j) - lo * dp) / den;
-                    cp = up / den;
-                    at(oy, j) = dp; cpv[j] = cp;
-                }
-                cx next = mk(0);
-                for (int j = ny - 1; j >= 0; --j) { next = at(oy, j) - cpv[j] * next; at(oy, j) = next; }
+                const double al = j > 0 ? ia[j] : 0, au = j < ny - 1 ? ia[j + 1] : 0;
+                s.lB[j] = -a2 * id[j] * al;
+                s.uB[j] = -a2 * id[j] * au;
+                s.bB[j] = 1 + a2 * kx2 + a2 * id[j] * (al + au);
             }
+            __syncthreads();
+            // lane 0: j_x, lane 1: j_z (nodes 1 .. ny-1, the elimination factors of each in its own array), lane 2:
+            // j_y (half-nodes 0 .. ny-1)
+            if (t0 < 2)
+                lds_thomas(t0 ? vz : vx, s.lE, s.bE, s.uE, reinterpret_cast<double*>(t0 ? s.sRy : s.sRx), 1, ny);
+            else if (t0 == 2)
+                lds_thomas(vy, s.lB, s.bB, s.uB, s.uB, 0, ny);
+            __syncthreads();
+            for (int j = t0; j < ny; j += kLW) { at(ox, j) = vx[j]; at(oz, j) = vz[j]; at(oy, j) = vy[j]; }
+            line_sync();
         }
-        if (a2 > 0) line_sync();
         if (kind == 2) {
             for (int j = t0; j < ny; j += kLW) {
                 const long i = o + long(j) * nx;
@@ -479,7 +507,8 @@ static size_t line_lds_bytes(int ny) {
 }
 void march_lines(const LineView& L, int lo, int hi) {
     if (L.mv.nx <= 0 || hi <= lo) return;
-    hipLaunchKernelGGL(k_march_lines, dim3(unsigned(L.mv.nx)), dim3(kLW), line_lds_bytes(L.ny), stream(), L, lo, hi);
+    hipLaunchKernelGGL(k_march_lines, dim3(unsigned(L.mv.nx)), dim3(kLW), line_lds_bytes(L.ny), stream(), L, lo, hi,
+                       xcd_rows());
     launch_check("march lines");
 }
 void fh_currents(const cx* S1, cx* jkx, cx* jky, cx* jkz, int planes, const LineView& L, const double* filt, int kf,
@@ -510,7 +539,7 @@ void fh_comoving_R(const cx* psi, const double* kx, const double* iD, double bet
 void pass_lines(const ModeArgs& m, const LineView& L, cx* X, cx* T, cx* jsn, int kind) {
     if (m.mv.nx <= 0) return;
     hipLaunchKernelGGL(k_pass_lines, dim3(unsigned(m.mv.nx)), dim3(kLW), line_lds_bytes(L.ny), stream(), m, L, X, T,
-                       jsn, kind);
+                       jsn, kind, xcd_rows());
     launch_check("pass lines");
 }
 

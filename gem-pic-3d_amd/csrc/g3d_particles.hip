@@ -369,17 +369,20 @@ __global__ void __launch_bounds__(kStreamBlockMax) k_stream_halfcell(StreamArgs 
     extern __shared__ double win_lds[];
     const long b = blockIdx.x;
     if (b < tg.ntile) {
+        // free-hand y (every gather reads F): the tiles of one XCD a band of consecutive tile rows, so that
+        // neighbouring tiles meet in one L2 (workgroup b runs on XCD b mod 8)
+        const long t = (tg.fh && (tg.ntile & 7) == 0) ? (b & 7) * (tg.ntile >> 3) + (b >> 3) : b;
         // the particles binned into this tile (tg.idx: indices into the list; null: the contiguous load ranges).
         // The range, this lane's first particle and its state first: their dependent loads overlap those of the
         // field window.
-        const long lo = tg.off[b], hi = tg.off[b + 1];
+        const long lo = tg.off[t], hi = tg.off[t + 1];
         long k = lo + threadIdx.x;
         long i = k < hi ? (tg.idx ? long(tg.idx[k]) : k) : 0;
         PState p{};
         if (k < hi) p = load_pstate(S, i);
         const int pl = tg.WX * tg.WY;
         const long wn = 5L * pl;
-        const int tx = int(b % tg.ntx), ty = int(b / tg.ntx);
+        const int tx = int(t % tg.ntx), ty = int(t / tg.ntx);
         const int x0 = tx * tg.TX - 1 - tg.M, y0 = ty * tg.TY - 1 - tg.M;
         const FieldView& F = c.fv;
         const int fl = c.n - F.a;
@@ -410,7 +413,7 @@ __global__ void __launch_bounds__(kStreamBlockMax) k_stream_halfcell(StreamArgs 
         }
         __syncthreads();
         if (YF == 1) {
-            double* sl = tg.slab + b * wn;
+            double* sl = tg.slab + t * wn;
             for (long r = threadIdx.x; r < wn; r += blockDim.x) sl[r] = win_lds[r];
         } else                  // free-hand y: into the current planes (rows outside the walls hold nothing)
             for (long r = threadIdx.x; r < wn; r += blockDim.x) {

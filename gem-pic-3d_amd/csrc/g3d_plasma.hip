@@ -276,7 +276,8 @@ int rows_per_wg(int ny) {
     while (r < 8 && ny / (2 * r) >= 256) r <<= 1;
     return r;
 }
-// the row blocks of one XCD consecutive in the row kernels whose spectra are column-major (G3D_XCD_ROWS=0: off)
+// the row blocks of one XCD consecutive in the row kernels whose spectra are column-major, and the k_x lines of one
+// XCD consecutive in the free-hand line kernels (G3D_XCD_ROWS=0: off)
 int xcd_rows() {
     static const int on = [] { const char* e = getenv("G3D_XCD_ROWS"); return e ? atoi(e) : 1; }();
     return on;
