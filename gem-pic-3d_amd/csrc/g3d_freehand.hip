@@ -25,7 +25,7 @@ constexpr int kT = 64;
 //   3. (lanes 0, 1) the eliminations and back substitutions;
 //   4. E_z, B_z and R, L of every node.
 // Every value of slice n at s0 is read (1) before any is rewritten (4).
-constexpr int kLW = 64;
+constexpr int kLW = 256;
 // per node (half-node for B_z): the explicit parts, the right-hand sides (then the eliminated ones, then the
 // solution), the fixed pivot parts and the off-diagonals (the upper one then the elimination factor)
 struct LineLds {
