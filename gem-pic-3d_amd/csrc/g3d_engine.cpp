@@ -244,7 +244,7 @@ void Sim::setup_tiles() {
     slab.resize(size_t(tiles.ntile) * 5 * tiles.WX * tiles.WY);
     tiles.off = tile_off.data(); tiles.slab = slab.data(); tiles.n_tiled = 0;
     const char* e = getenv("G3D_STREAM_FWIN");
-    tiles.fwin = yf ? (e ? atoi(e) : 1) : 0;
+    tiles.fwin = e ? atoi(e) : 1;
     const char* b = getenv("G3D_STREAM_BLOCK");  // lanes per tile workgroup (about the particles of a tile)
     tiles.block = b ? atoi(b) : 512;
     const char* w = getenv("G3D_STREAM_WPE");     // register cap of the FP32-mixed stream kernel (0 or 4)

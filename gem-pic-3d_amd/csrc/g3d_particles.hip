@@ -132,6 +132,49 @@ __device__ __forceinline__ bool gather_window(const FieldView& c, const Win& w, 
     return true;
 }
 
+// E, B from the tile's field window on the free-hand y grid (window rows y0 .. y0+WY-1 are node indices; the
+// half-nodes share them): the rows beyond the walls skipped and the same weights and order of the sums as
+// gather_t<0>, so the values are identical; false when a row or column of the stencil or the slice pair is not in
+// the window (then the caller gathers from F)
+__device__ __forceinline__ bool gather_window_fh(const FieldView& c, const Win& w, double x, double y, double zeta,
+                                                 double E[3], double B[3]) {
+    if (!c.self) return false;
+    int n; double fz;
+    c.zax.locate(zeta, n, fz);
+    if (n == c.a - 1 && fz > 1 - 1e-9) { n = c.a; fz = 0; }
+    if (n - c.a != w.fl) return false;
+    int ix[3], jn[3], jh[3]; double wx[3], wn[3], wh[3];
+    tsc_periodic(x, c.dx, c.nx, ix, wx);
+    const int u0 = wrap_index(ix[0] - w.x0, c.nx);
+    if (u0 + 2 >= w.WX) return false;
+    tsc_freehand(c.yax, y, jn, wn, jh, wh);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        if (jn[j] >= 0 && jn[j] < c.ny && (jn[j] < w.y0 || jn[j] >= w.y0 + w.WY)) return false;
+        if (jh[j] >= 0 && jh[j] < c.ny && (jh[j] < w.y0 || jh[j] >= w.y0 + w.WY)) return false;
+    }
+    const double gz = 1 - fz;
+    const int pl = w.WX * w.WY;
+#pragma unroll
+    for (int comp = 0; comp < 6; ++comp) {
+        const float* q0 = w.fld + comp * pl;
+        const float* q1 = w.fld + (6 + comp) * pl;
+        const bool hf = half_comp(comp);
+        double v = 0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const int jj = hf ? jh[j] : jn[j];
+            const double wy = hf ? wh[j] : wn[j];
+            if (jj < 0 || jj >= c.ny) continue;
+            const int r = (jj - w.y0) * w.WX + u0;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) v += wx[i] * wy * (gz * double(q0[r + i]) + fz * double(q1[r + i]));
+        }
+        if (comp < 3) E[comp] = v; else B[comp - 3] = v;
+    }
+    return true;
+}
+
 // TSC deposit of f (dt/Delta) v/(h dx dy) (src/sim_stream.hpp:33-64) into plane comp of the half-cell current:
 // the tile's LDS window when the whole 3 x 3 stencil lies inside it, else jsr with global atomics.  Zero
 // contributions are skipped (adding +-0 to a sum that starts at +0 never changes it).
@@ -305,7 +348,12 @@ __device__ __forceinline__ void stream_one(const StreamArgs& c, const Win* win, 
     const double zm = p.z + 0.5 * dt0 * p.pz / g0, zetam = p.zeta - 0.5 * hl;
     double E[3], B[3], Ee[3], Be[3];
     const double xg = wrap_mod(xm, P.Lx), yg = YF ? wrap_mod(ym, P.Ly) : ym;
-    if (!(YF && win && win->fld && gather_window(P, *win, xg, yg, zetam, E, B))) gather_t<YF>(P, xg, yg, zetam, E, B);
+    bool got = false;
+    if (win && win->fld) {
+        if constexpr (YF == 1) got = gather_window(P, *win, xg, yg, zetam, E, B);
+        else got = gather_window_fh(P, *win, xg, yg, zetam, E, B);
+    }
+    if (!got) gather_t<YF>(P, xg, yg, zetam, E, B);
     if (LASER) {
         c.laser.field(xm + c.X0, zm, zm - (zetam - c.zc), Ee, Be);
         for (int k = 0; k < 3; ++k) { E[k] += Ee[k]; B[k] += Be[k]; }
@@ -388,17 +436,20 @@ __global__ void __launch_bounds__(kStreamBlockMax) k_stream_halfcell(StreamArgs 
         const int fl = c.n - F.a;
         float* fw = reinterpret_cast<float*>(win_lds + wn);
         for (long r = threadIdx.x; r < wn; r += blockDim.x) win_lds[r] = 0;
-        const bool fwin = YF == 1 && tg.fwin && F.self;
-        if (fwin)        // slices fl, fl + 1 of the 6 components: [slice][comp][v][u]; one node per lane, 12 loads
+        const bool fwin = tg.fwin && F.self;
+        // slices fl, fl + 1 of the 6 components: [slice][comp][v][u]; one node per lane, 12 loads.  Free-hand y: the
+        // rows beyond the walls are never read (zero)
+        if (fwin)
             for (int r = threadIdx.x; r < pl; r += blockDim.x) {
                 const int v = r / tg.WX, u = r - v * tg.WX;
-                const int gx = wrap_index(x0 + u, F.nx), gy = wrap_index(y0 + v, F.ny);
-                const float* src = F.F + long(fl) * F.nk + long(gy) * F.nx + gx;
+                const int gx = wrap_index(x0 + u, F.nx), gy = YF ? wrap_index(y0 + v, F.ny) : y0 + v;
+                const bool in = YF || (gy >= 0 && gy < F.ny);
+                const float* src = F.F + long(fl) * F.nk + long(in ? gy : 0) * F.nx + gx;
                 const long cs = long(F.ns + 2) * F.nk;
 #pragma unroll
                 for (int comp = 0; comp < 6; ++comp) {
-                    fw[comp * pl + r] = src[comp * cs];
-                    fw[(6 + comp) * pl + r] = src[comp * cs + F.nk];
+                    fw[comp * pl + r] = in ? src[comp * cs] : 0.0f;
+                    fw[(6 + comp) * pl + r] = in ? src[comp * cs + F.nk] : 0.0f;
                 }
             }
         __syncthreads();
