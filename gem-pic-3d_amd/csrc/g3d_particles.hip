@@ -274,10 +274,25 @@ __device__ __forceinline__ void sdeposit(const StreamArgs& c, const Win* win, do
     }
 }
 
+// E, B at (x, y, zeta) (x, y wrapped as the grid wants): from the tile's field window when it holds the stencil,
+// else from F; the values are the same either way
+template <int YF>
+__device__ __forceinline__ void gather_stream(const FieldView& P, const Win* win, double x, double y, double zeta,
+                                              double E[3], double B[3]) {
+    bool got = false;
+    if (win && win->fld) {
+        if constexpr (YF == 1) got = gather_window(P, *win, x, y, zeta, E, B);
+        else got = gather_window_fh(P, *win, x, y, zeta, E, B);
+    }
+    if (!got) gather_t<YF>(P, x, y, zeta, E, B);
+}
+
 // |E| of the grid and the analytic laser (src/sim_stream.hpp:67-74)
-__device__ __forceinline__ double field_abs(const StreamArgs& c, double x, double y, double z, double zeta) {
+template <int YF>
+__device__ __forceinline__ double field_abs(const StreamArgs& c, const Win* win, double x, double y, double z,
+                                            double zeta) {
     double E[3], B[3], Ee[3], Be[3];
-    gather(c.fv, wrap_mod(x, c.fv.Lx), c.fv.yf ? wrap_mod(y, c.fv.Ly) : y, zeta, E, B);
+    gather_stream<YF>(c.fv, win, wrap_mod(x, c.fv.Lx), YF ? wrap_mod(y, c.fv.Ly) : y, zeta, E, B);
     c.laser.field(x + c.X0, z, z - (zeta - c.zc), Ee, Be);
     for (int k = 0; k < 3; ++k) E[k] += Ee[k];
     return sqrt(E[0] * E[0] + E[1] * E[1] + E[2] * E[2]);
@@ -335,7 +350,7 @@ __device__ __forceinline__ void stream_one(const StreamArgs& c, const Win* win, 
     if (ion && !c.ions[p.sp - SP_ION].mobile) {
         const double f = p.q * p.w * hl / (c.Delta * h * P.dx);
         if (f != 0) sdeposit<YF>(c, win, p.x, p.y, f, 0, 0, 0);
-        const double E1 = field_abs(c, p.x, p.y, p.z, p.zeta), E0 = field_abs(c, p.x, p.y, p.z, zn);
+        const double E1 = field_abs<YF>(c, win, p.x, p.y, p.z, p.zeta), E0 = field_abs<YF>(c, win, p.x, p.y, p.z, zn);
         p.zeta = zn;
         ionize(c, p, E1, E0, hl);
         return;
@@ -348,12 +363,7 @@ __device__ __forceinline__ void stream_one(const StreamArgs& c, const Win* win, 
     const double zm = p.z + 0.5 * dt0 * p.pz / g0, zetam = p.zeta - 0.5 * hl;
     double E[3], B[3], Ee[3], Be[3];
     const double xg = wrap_mod(xm, P.Lx), yg = YF ? wrap_mod(ym, P.Ly) : ym;
-    bool got = false;
-    if (win && win->fld) {
-        if constexpr (YF == 1) got = gather_window(P, *win, xg, yg, zetam, E, B);
-        else got = gather_window_fh(P, *win, xg, yg, zetam, E, B);
-    }
-    if (!got) gather_t<YF>(P, xg, yg, zetam, E, B);
+    gather_stream<YF>(P, win, xg, yg, zetam, E, B);
     if (LASER) {
         c.laser.field(xm + c.X0, zm, zm - (zetam - c.zc), Ee, Be);
         for (int k = 0; k < 3; ++k) { E[k] += Ee[k]; B[k] += Be[k]; }
@@ -388,7 +398,7 @@ __device__ __forceinline__ void stream_one(const StreamArgs& c, const Win* win, 
     if (f != 0) sdeposit<YF>(c, win, 0.5 * (xo + p.x), 0.5 * (yo + p.y), f, vx, vy, vz);
     if (!YF && (p.y < c.ylo || p.y > c.yhi)) { p.alive = 0; return; }
     if (ion && !trapped && c.ions[p.sp - SP_ION].ionizing)
-        ionize(c, p, field_abs(c, xm, ym, zm, zeta0), field_abs(c, xm, ym, zm, zn), dt);
+        ionize(c, p, field_abs<YF>(c, win, xm, ym, zm, zeta0), field_abs<YF>(c, win, xm, ym, zm, zn), dt);
     if (trapped) {
         p.alive = 5;
         const double vze = uz / g1;
