@@ -423,7 +423,19 @@ MarchView Sim::mview(double D) {
 }
 LineView Sim::lview(double D) {
     LineView L;
-    L.mv = mview(D); L.iD = iD.data(); L.iA = iA.data(); L.scp = scp.data(); L.ny = cfg.ny;
+    L.mv = mview(D); L.iD = iD.data(); L.iA = iA.data(); L.scp = scp.data(); L.ny = cfg.ny; L.piv = nullptr;
+    // free-hand y march (D != 0): the pivots and elimination factors of every line and slice, formed once per D
+    // (up to 24 GiB; G3D_LINE_PIVOTS=0: formed inside the march as it goes)
+    static const bool on = [] { const char* e = getenv("G3D_LINE_PIVOTS"); return !(e && atoi(e) == 0); }();
+    const size_t need = size_t(N) * 4 * size_t(cfg.ny) * size_t(cfg.nx);
+    if (!yf && D != 0 && on && N >= 2 && need * sizeof(double) <= (size_t(24) << 30)) {
+        if (!(piv_ok && piv_D == D)) {
+            if (piv.size() < need) piv.resize(need);
+            line_pivots(L, piv.data());
+            piv_D = D; piv_ok = true;
+        }
+        L.piv = piv.data();
+    }
     return L;
 }
 FieldView Sim::fview() {

@@ -119,6 +119,9 @@ public:
     std::vector<double> wlo_h, hcg_h, hDel, hA;
     // free-hand y: node table, LU factors of k_x^2 - Lap_y, eigenvectors of -Lap_y, Thomas factors of the lines
     DevBuf<double> yn, pcp, pid, evf, evb, evl, scp;
+    DevBuf<double> piv;                    // free-hand y: the march's pivots of every line and slice (lview)
+    double piv_D = 0;
+    bool piv_ok = false;
     Axis zax{}, yax{};
     FftPlan fft;
     PartStore beam, strmA, strmB, born;

@@ -82,6 +82,51 @@ __device__ __forceinline__ void lds_thomas(cx* v, const double* lo, const double
     }
 }
 
+// The coefficients of the march's systems at node j (src/sim_kernels.hpp:262-279): E_z on the nodes 1 .. ny-1
+// (lower, upper off-diagonal, pivot part), B_z on the half-nodes 0 .. ny-1; cc = h/4 (D + h/4), kx2 = k_x^2
+__device__ __forceinline__ void coef_E(const double* iA, const double* iD, int ny, double cc, double kx2, int j,
+                                       double& lo, double& up, double& bs) {
+    lo = j > 1 ? -cc * iA[j] * iD[j - 1] : 0;
+    up = j < ny - 1 ? -cc * iA[j] * iD[j] : 0;
+    bs = 1 + cc * kx2 + cc * iA[j] * (iD[j - 1] + iD[j]);
+}
+__device__ __forceinline__ void coef_B(const double* iA, const double* iD, int ny, double cc, double kx2, int j,
+                                       double& lo, double& up, double& bs) {
+    const double al = j > 0 ? iA[j] : 0, au = j < ny - 1 ? iA[j + 1] : 0;
+    lo = -cc * iD[j] * al;
+    up = -cc * iD[j] * au;
+    bs = 1 + cc * kx2 + cc * iD[j] * (al + au);
+}
+
+// lds_thomas with the pivots den(j) = bs - lo c'(j-1) and the factors c'(j) given (bs, cp): the same operations on
+// the right-hand side, one complex division per node
+__device__ __forceinline__ void lds_thomas_fixed(cx* v, const double* lo, const double* den, const double* cp, int j0,
+                                                 int n) {
+    if (j0 >= n) return;
+    cx dv = mk(0);
+    double l = lo[j0], dn = den[j0];
+    cx r = v[j0];
+    for (int j = j0; j < n; ++j) {
+        const int jn = j + 1 < n ? j + 1 : j;
+        const double ln = lo[jn], dnn = den[jn];
+        const cx rn = v[jn];
+        dv = (r - l * dv) / dn;
+        v[j] = dv;
+        l = ln; dn = dnn; r = rn;
+    }
+    cx nv = mk(0);
+    cx rj = v[n - 1];
+    double cj = cp[n - 1];
+    for (int j = n - 1; j >= j0; --j) {
+        const int jp = j > j0 ? j - 1 : j;
+        const cx rp = v[jp];
+        const double c = cp[jp];
+        nv = rj - cj * nv;
+        v[j] = nv;
+        rj = rp; cj = c;
+    }
+}
+
 __device__ void march_line(const LineView& L, int ik, int n, const LineLds& s) {
     const MarchView& c = L.mv;
     const int nx = c.nx, ny = L.ny, t0 = threadIdx.x;
@@ -122,6 +167,7 @@ __device__ void march_line(const LineView& L, int ik, int n, const LineLds& s) {
         return;
     }
     const double h = c.hcg[n + 2], qh = 0.25 * h, hh = 0.5 * h, cc = qh * (D + qh);
+    const long pv = long(n) * 4 * ny * nx + ik;        // this line's pivots of slice n (L.piv)
     // 1. explicit parts
     for (int j = t0; j < ny; j += kLW) {
         explicit_R(j);
@@ -149,9 +195,8 @@ __device__ void march_line(const LineView& L, int ik, int n, const LineLds& s) {
             const cx Qp = 0.5 * (-(a * (pRx - pLx)) + ((pLy - pRy) - (pLym - pRym)) * iA[j]);
             const cx SE = -(a * (sRx - sLx)) + ((sLy - sRy) - (sLym - sRym)) * iA[j];
             s.rE[j] = pEz - hh * Qp - qh * SE - h * c.jkz[u];
-            s.lE[j] = j > 1 ? -cc * iA[j] * iD[j - 1] : 0;
-            s.uE[j] = j < ny - 1 ? -cc * iA[j] * iD[j] : 0;
-            s.bE[j] = 1 + cc * kx2 + cc * iA[j] * (iD[j - 1] + iD[j]);
+            coef_E(iA, iD, ny, cc, kx2, j, s.lE[j], s.uE[j], s.bE[j]);
+            if (L.piv) { s.bE[j] = L.piv[pv + long(j) * nx]; s.uE[j] = L.piv[pv + long(ny + j) * nx]; }
         }
         const bool top = j + 1 >= ny;
         const cx fu = top ? mk(0) : c.Rx[u + nx] + c.Lx[u + nx], f0 = pRx + pLx;
@@ -159,10 +204,8 @@ __device__ void march_line(const LineView& L, int ik, int n, const LineLds& s) {
         const cx Cp = 0.5 * (a * (pRy + pLy) - (fu - f0) * iD[j]);
         const cx SB = a * (sRy + sLy) - (su - s0) * iD[j];
         s.rB[j] = c.Bz[u] - hh * Cp - qh * SB;
-        const double al = j > 0 ? iA[j] : 0, au = j < ny - 1 ? iA[j + 1] : 0;
-        s.lB[j] = -cc * iD[j] * al;
-        s.uB[j] = -cc * iD[j] * au;
-        s.bB[j] = 1 + cc * kx2 + cc * iD[j] * (al + au);
+        coef_B(iA, iD, ny, cc, kx2, j, s.lB[j], s.uB[j], s.bB[j]);
+        if (L.piv) { s.bB[j] = L.piv[pv + long(2 * ny + j) * nx]; s.uB[j] = L.piv[pv + long(3 * ny + j) * nx]; }
     }
     __syncthreads();
     // 3. the recurrences, side by side in one instruction stream: lane 0 E_z (nodes 1 ..), lane 1 B_z (half-nodes
@@ -173,7 +216,8 @@ __device__ void march_line(const LineView& L, int ik, int n, const LineLds& s) {
         cx* d = hb ? s.rB : s.rE;
         const double *bs = hb ? s.bB : s.bE, *ls = hb ? s.lB : s.lE;
         double* cp = hb ? s.uB : s.uE;
-        lds_thomas(d, ls, bs, cp, cp, j0, ny);
+        if (L.piv) lds_thomas_fixed(d, ls, bs, cp, j0, ny);
+        else lds_thomas(d, ls, bs, cp, cp, j0, ny);
         if (!hb) s.rE[0] = mk(0);
     }
     __syncthreads();
@@ -492,6 +536,31 @@ __global__ void __launch_bounds__(kLW) k_pass_lines(ModeArgs m, LineView L, cx* 
     }
 }
 
+// thread (ik, n): the pivots and elimination factors of line ik, slice n, as march_line's eliminations form them
+__global__ void __launch_bounds__(64) k_line_pivots(LineView L, double* piv) {
+    const MarchView& c = L.mv;
+    const int ik = blockIdx.x * blockDim.x + threadIdx.x, n = blockIdx.y, nx = c.nx, ny = L.ny;
+    if (ik >= nx) return;
+    const double h = c.hcg[n + 2], qh = 0.25 * h, cc = qh * (c.D + qh), kx2 = c.kx[ik] * c.kx[ik];
+    double* p = piv + long(n) * 4 * ny * nx + ik;
+    double cv = 0;
+    for (int j = 1; j < ny; ++j) {
+        double lo, up, bs;
+        coef_E(L.iA, L.iD, ny, cc, kx2, j, lo, up, bs);
+        const double den = bs - lo * cv;
+        cv = up / den;
+        p[long(j) * nx] = den; p[long(ny + j) * nx] = cv;
+    }
+    cv = 0;
+    for (int j = 0; j < ny; ++j) {
+        double lo, up, bs;
+        coef_B(L.iA, L.iD, ny, cc, kx2, j, lo, up, bs);
+        const double den = bs - lo * cv;
+        cv = up / den;
+        p[long(2 * ny + j) * nx] = den; p[long(3 * ny + j) * nx] = cv;
+    }
+}
+
 template <class K, class... A> void launch(K k, long n, int t, const char* what, A... a) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k, dim3(blocks_for(n, t)), dim3(t), 0, stream(), a...);
@@ -504,6 +573,12 @@ static size_t line_lds_bytes(int ny) {
     const size_t b = size_t(ny) * kLineLdsNode;
     if (b > 64 * 1024) throw std::runtime_error("free-hand lines: ny " + std::to_string(ny) + " above the LDS of a line");
     return b;
+}
+void line_pivots(const LineView& L, double* piv) {
+    if (L.mv.N < 2 || L.mv.nx <= 0) return;
+    hipLaunchKernelGGL(k_line_pivots, dim3(unsigned((L.mv.nx + 63) / 64), unsigned(L.mv.N - 1)), dim3(64), 0, stream(),
+                       L, piv);
+    launch_check("line pivots");
 }
 void march_lines(const LineView& L, int lo, int hi) {
     if (L.mv.nx <= 0 || hi <= lo) return;

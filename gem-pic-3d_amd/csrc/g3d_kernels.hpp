@@ -232,7 +232,12 @@ struct LineView {
     const double *iD, *iA;            // 1/Delta_j (half-nodes), 1/A_j (nodes)
     double* scp;                      // 2 nk: Thomas factors of the E_z and B_z systems (march, smoothing)
     int ny;
+    // null, or the march's pivots and elimination factors, [N][4][ny][nx]: E_z pivot, E_z factor, B_z pivot, B_z
+    // factor (they depend on the line, the y grid, h of the slice and D only; line_pivots)
+    const double* piv;
 };
+// the pivots and elimination factors of the march's eliminations for every line and slice 0 .. N-2 (L.piv unused)
+void line_pivots(const LineView& L, double* piv);
 // slices hi-1 ... lo of every k_x line (src/sim_fields.cpp:7-31, src/sim_kernels.hpp:213-313)
 void march_lines(const LineView& L, int lo, int hi);
 // the pair of x transforms (S1: j_x + i j_y per half-cell plane) unpacked per row, the (1,2,1)/4 y filters of the

@@ -75,7 +75,7 @@ void require_one_gpu(const g3d_march_ctx& m) {
 }
 LineView lines_of(const g3d_march_ctx& m) {
     LineView L;
-    L.mv = view_of(m); L.iD = m.iD; L.iA = m.iA; L.scp = m.scp; L.ny = m.ny;
+    L.mv = view_of(m); L.iD = m.iD; L.iA = m.iA; L.scp = m.scp; L.ny = m.ny; L.piv = nullptr;
     return L;
 }
 }  // namespace
