@@ -77,6 +77,9 @@ struct Axis {
     const double* X;
     int n, uniform;
     double x0, ih;
+    // search hint (0 <= hlo < hhi <= n-1, or hhi = 0: none): when X[hlo] <= x < X[hhi] the bisection starts from
+    // [hlo, hhi]; the cell with X[i] <= x < X[i+1] is unique, so the result is the same
+    int hlo = 0, hhi = 0;
     G3D_HD void locate(double x, int& i, double& f) const {
         if (uniform) {
             const double t = (x - x0) * ih;
@@ -95,6 +98,7 @@ struct Axis {
             return;
         }
         int lo = 0, hi = n - 1;
+        if (hhi > hlo && X[hlo] <= x && x < X[hhi]) { lo = hlo; hi = hhi; }
         while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
             if (X[mid] <= x) lo = mid; else hi = mid;

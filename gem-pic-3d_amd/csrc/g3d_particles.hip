@@ -464,8 +464,16 @@ __global__ void __launch_bounds__(kStreamBlockMax) k_stream_halfcell(StreamArgs 
             }
         __syncthreads();
         Win w{win_lds, fwin ? fw : nullptr, x0, y0, tg.WX, tg.WY, fl};
+        // free-hand y: the y locates of this tile's particles search the window's rows first
+        StreamArgs cl = c;
+        if (YF == 0 && !c.fv.yax.uniform && c.fv.yax.n >= 3) {
+            const int nn = c.fv.yax.n;
+            const int lo_r = y0 < 0 ? 0 : (y0 > nn - 2 ? nn - 2 : y0);
+            const int hi_r = y0 + tg.WY > nn - 1 ? nn - 1 : (y0 + tg.WY < lo_r + 1 ? lo_r + 1 : y0 + tg.WY);
+            cl.fv.yax.hlo = lo_r; cl.fv.yax.hhi = hi_r;
+        }
         while (k < hi) {
-            stream_index<YF, ION, LASER>(c, &w, p, D, i);
+            stream_index<YF, ION, LASER>(cl, &w, p, D, i);
             k += blockDim.x;
             if (k < hi) {
                 i = tg.idx ? long(tg.idx[k]) : k;
