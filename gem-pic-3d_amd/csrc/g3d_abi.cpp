@@ -36,6 +36,11 @@ bool lookup(g3d::Sim& s, const std::string& n, void*& dev, size_t& bytes) {
     dev = it->second.first; bytes = it->second.second;
     return true;
 }
+// the Simulation of a handle; a null handle is an error, not a fault
+g3d::Sim& S(g3d_sim* h) {
+    if (!h || !h->s) throw std::invalid_argument("null simulation handle");
+    return *h->s;
+}
 int check_size(size_t want, size_t got) {
     if (want != got) { g3d::set_error("size mismatch: " + std::to_string(got) + " bytes, array has " + std::to_string(want)); return 2; }
     return 0;
@@ -57,20 +62,20 @@ int g3d_sim_destroy(g3d_sim* sim) {
 }
 int g3d_sim_info_get(g3d_sim* sim, g3d_sim_info* o) {
     return g3d::guard([&] {
-        g3d::Sim& s = *sim->s;
+        g3d::Sim& s = S(sim);
         o->N = s.N; o->ns = s.ns; o->nk = s.nk; o->np = s.np; o->nstrm = s.nstrm; o->istep = s.istep;
         o->born_cap = s.born_cap; o->s = s.s; o->dx = s.dx; o->dy = s.dy; o->X0 = s.X0; o->Y0 = s.Y0;
         o->beta_b = s.beta_b; o->gm2_b = s.gm2_b; o->trapped_total = s.trapped_total;
         o->restreamed_total = s.restreamed_total; o->lost_total = s.lost_total;
     });
 }
-int g3d_sim_init(g3d_sim* sim) { return g3d::guard([&] { sim->s->init(); g3d::hip_check(hipStreamSynchronize(g3d::stream()), "synchronize"); }); }
-int g3d_sim_step(g3d_sim* sim) { return g3d::guard([&] { sim->s->step(); }); }
+int g3d_sim_init(g3d_sim* sim) { return g3d::guard([&] { S(sim).init(); g3d::hip_check(hipStreamSynchronize(g3d::stream()), "synchronize"); }); }
+int g3d_sim_step(g3d_sim* sim) { return g3d::guard([&] { S(sim).step(); }); }
 
 int g3d_sim_call(g3d_sim* sim, const char* op_, const double* a, int na) {
     int rc = 0;
     const int g = g3d::guard([&] {
-        g3d::Sim& s = *sim->s;
+        g3d::Sim& s = S(sim);
         const std::string op(op_);
         auto need = [&](int k) {
             if (na < k) throw std::runtime_error(op + ": needs " + std::to_string(k) + " arguments");
@@ -94,6 +99,7 @@ int g3d_sim_call(g3d_sim* sim, const char* op_, const double* a, int na) {
         else if (op == "set_s") { need(2); s.s = a[0]; s.istep = long(a[1]); }
         else if (op == "set_starting") { need(1); s.starting_ = a[0] != 0; }
         else if (op == "j0_from_jk") s.j0_from_jk();
+        else if (op == "line_pivots") { need(1); s.line_pivots_on = a[0] != 0; s.piv_ok = false; }
         else { g3d::set_error("unknown op " + op); rc = 3; return; }
         g3d::hip_check(hipStreamSynchronize(g3d::stream()), "synchronize");
     });
@@ -103,7 +109,7 @@ int g3d_sim_call(g3d_sim* sim, const char* op_, const double* a, int na) {
 int g3d_sim_array_size(g3d_sim* sim, const char* name, size_t* bytes) {
     int rc = 0;
     const int g = g3d::guard([&] {
-        g3d::Sim& s = *sim->s;
+        g3d::Sim& s = S(sim);
         const std::string n(name);
         void* d;
         if (n == "parts") *bytes = size_t(s.np) * sizeof(g3d_particle);
@@ -115,7 +121,7 @@ int g3d_sim_array_size(g3d_sim* sim, const char* name, size_t* bytes) {
 int g3d_sim_get(g3d_sim* sim, const char* name, void* host, size_t bytes) {
     int rc = 0;
     const int g = g3d::guard([&] {
-        g3d::Sim& s = *sim->s;
+        g3d::Sim& s = S(sim);
         s.ensure_layout(false, false);      // arrays leave and enter in the reference's layout
         const std::string n(name);
         if (n == "parts" || n == "strm") {
@@ -136,9 +142,10 @@ int g3d_sim_get(g3d_sim* sim, const char* name, void* host, size_t bytes) {
 int g3d_sim_set(g3d_sim* sim, const char* name, const void* host, size_t bytes) {
     int rc = 0;
     const int g = g3d::guard([&] {
-        g3d::Sim& s = *sim->s;
+        g3d::Sim& s = S(sim);
         s.ensure_layout(false, false);
         s.jw_zero = false;
+        s.piv_ok = false;          // the march geometry (kx, hcg) may change: the pivot table is formed again
         if (std::string(name) == "strm") {
             const g3d_particle* p = static_cast<const g3d_particle*>(host);
             s.set_strm(std::vector<g3d_particle>(p, p + bytes / sizeof(g3d_particle)));
@@ -154,15 +161,15 @@ int g3d_sim_set(g3d_sim* sim, const char* name, const void* host, size_t bytes) 
     return g ? g : rc;
 }
 int g3d_sim_set_particles(g3d_sim* sim, const g3d_particle* p, long n) {
-    return g3d::guard([&] { sim->s->set_particles(std::vector<g3d_particle>(p, p + n)); });
+    return g3d::guard([&] { S(sim).set_particles(std::vector<g3d_particle>(p, p + n)); });
 }
 int g3d_sim_set_joining(g3d_sim* sim, const g3d_particle* p, long n) {
-    return g3d::guard([&] { sim->s->joining_.assign(p, p + n); });
+    return g3d::guard([&] { S(sim).joining_.assign(p, p + n); });
 }
 int g3d_sim_get_joining(g3d_sim* sim, g3d_particle* p, long cap, long* n) {
     int rc = 0;
     const int g = g3d::guard([&] {
-        const auto& j = sim->s->joining_;
+        const auto& j = S(sim).joining_;
         *n = long(j.size());
         if (*n > cap) { g3d::set_error("joining list larger than the buffer"); rc = 2; return; }
         std::memcpy(p, j.data(), j.size() * sizeof(g3d_particle));
@@ -171,7 +178,7 @@ int g3d_sim_get_joining(g3d_sim* sim, g3d_particle* p, long cap, long* n) {
 }
 int g3d_sim_timing(g3d_sim* sim, int enable) {
     return g3d::guard([&] {
-        g3d::Sim& s = *sim->s;
+        g3d::Sim& s = S(sim);
         s.flush_timers();
         s.timing = unsigned(enable);
         for (int k = 0; k < g3d::Sim::kFam; ++k) { s.fam_ms[k] = 0; s.fam_n[k] = 0; s.fam_seen[k] = 0; }
@@ -180,12 +187,12 @@ int g3d_sim_timing(g3d_sim* sim, int enable) {
 int g3d_sim_timing_every(g3d_sim* sim, int every) {
     return g3d::guard([&] {
         if (every < 1) throw std::invalid_argument("timing: every must be at least 1");
-        sim->s->timing_every = every;
+        S(sim).timing_every = every;
     });
 }
 int g3d_sim_timers(g3d_sim* sim, double* ms, long* counts, int nfam, char* names, int name_len) {
     return g3d::guard([&] {
-        g3d::Sim& s = *sim->s;
+        g3d::Sim& s = S(sim);
         s.flush_timers();
         for (int k = 0; k < nfam && k < g3d::Sim::kFam; ++k) {
             ms[k] = s.fam_ms[k]; counts[k] = s.fam_n[k];

@@ -71,7 +71,7 @@ G3D_HD void tsc_periodic(double x, double d, int n, int idx[3], double w[3]) {
     for (int k = 0; k < 3; ++k) idx[k] = wrap_index(idx[k], n);
 }
 
-// Node table X[0..n-1] with the direct path for uniform grids (src/grid.hpp:130-146): X[i] <= x < X[i+1],
+// Node table X[0..n-1] with the direct path for uniform grids (src/grid.hpp:35-46): X[i] <= x < X[i+1],
 // end cells continued outside.
 struct Axis {
     const double* X;

@@ -92,7 +92,7 @@ Parts PartStore::view(const PartStore* sh) {
     return p;
 }
 
-// ---------------------------------------------------------------- ADK tables (src/ionization.cpp:55-99)
+// ---------------------------------------------------------------- ADK tables (src/ionization.cpp:9-53)
 namespace {
 constexpr double kUH = 13.605693122994, kEa = 5.14220674763e11, kWa = 4.1341373335e16;
 void adk_coefficients(double U, int Zafter, double& logpre, double& power, double& expo) {
@@ -215,6 +215,7 @@ Sim::Sim(const Config& c) : cfg(c) {
     }
     counters.resize(4);
     cent_dev.resize(3);
+    { const char* e = getenv("G3D_LINE_PIVOTS"); line_pivots_on = !(e && atoi(e) == 0); }
     setup_ions();
     if (cfg.plasma_on) setup_tiles();
 }
@@ -425,12 +426,20 @@ LineView Sim::lview(double D) {
     LineView L;
     L.mv = mview(D); L.iD = iD.data(); L.iA = iA.data(); L.scp = scp.data(); L.ny = cfg.ny; L.piv = nullptr;
     // free-hand y march (D != 0): the pivots and elimination factors of every line and slice, formed once per D
-    // (up to 24 GiB; G3D_LINE_PIVOTS=0: formed inside the march as it goes)
-    static const bool on = [] { const char* e = getenv("G3D_LINE_PIVOTS"); return !(e && atoi(e) == 0); }();
+    // and geometry (4 doubles per node of every line and slice, at most 24 GiB and only while the device has that
+    // much free memory beside 1 GiB; otherwise, or with line_pivots off, march_line forms them as it goes)
     const size_t need = size_t(N) * 4 * size_t(cfg.ny) * size_t(cfg.nx);
-    if (!yf && D != 0 && on && N >= 2 && need * sizeof(double) <= (size_t(24) << 30)) {
+    if (!yf && D != 0 && line_pivots_on && N >= 2 && need * sizeof(double) <= (size_t(24) << 30)) {
         if (!(piv_ok && piv_D == D)) {
-            if (piv.size() < need) piv.resize(need);
+            if (piv.size() < need) {
+                piv.resize(0);
+                size_t fr = 0, tot = 0;
+                if (hipMemGetInfo(&fr, &tot) != hipSuccess || need * sizeof(double) + (size_t(1) << 30) > fr) {
+                    (void)hipGetLastError();
+                    return L;
+                }
+                piv.resize(need);
+            }
             line_pivots(L, piv.data());
             piv_D = D; piv_ok = true;
         }

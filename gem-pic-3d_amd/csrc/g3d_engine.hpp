@@ -121,7 +121,8 @@ public:
     DevBuf<double> yn, pcp, pid, evf, evb, evl, scp;
     DevBuf<double> piv;                    // free-hand y: the march's pivots of every line and slice (lview)
     double piv_D = 0;
-    bool piv_ok = false;
+    bool piv_ok = false;              // piv holds the pivots of D = piv_D and the current march geometry
+    bool line_pivots_on = true;       // form the pivot table (false: march_line forms the pivots as it goes)
     Axis zax{}, yax{};
     FftPlan fft;
     PartStore beam, strmA, strmB, born;

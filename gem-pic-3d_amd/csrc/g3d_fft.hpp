@@ -5,7 +5,7 @@
 // row by row so that every global access is a contiguous run of nc complex values).  Inside a workgroup the data
 // sits in LDS and a radix-2 decimation-in-time transform runs in place after a bit-reversed load, with the
 // twiddles exp(-2 pi i k/n) of the reference's own FFT (src/fft.hpp:156-163) staged in LDS; the butterfly is
-// u + v w, u - v w as in src/fft.hpp:203-214.  Forward has the sign -1; the inverse is scaled by 1/(nx ny) (2D)
+// u + v w, u - v w as in src/fft.hpp:74-85.  Forward has the sign -1; the inverse is scaled by 1/(nx ny) (2D)
 // or 1/nx (rows only) in the last pass (an exact power of two, so the place of the scaling does not round).
 #pragma once
 #include "g3d_common.hpp"

@@ -15,7 +15,7 @@ __device__ __forceinline__ int lpos(int i) { return i + (i >> 3); }
 
 // K radix-2 DIT stages s0 .. s0+K-1 for one closed group of 2^K elements held in registers: the members
 // h 2^(s0-1+K) + m 2^(s0-1) + r, m = 0 .. 2^K-1.  Same butterflies, same twiddles, same order of operations as
-// the reference's iterative transform (src/fft.hpp:203-214), so the result is bit for bit the same.
+// the reference's iterative transform (src/fft.hpp:30-33, 74-85), so the result is bit for bit the same.
 template <int K, class C>
 __device__ __forceinline__ void group_stages(C* seq, int n, int s0, int h, int r, const C* tws) {
     constexpr int G = 1 << K;

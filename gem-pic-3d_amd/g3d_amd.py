@@ -289,7 +289,7 @@ def check(rc):
 
 ARRAY_DTYPES = {"F": np.float32, "jw": np.float64, "jsr": np.float64, "rho_lines": np.float64, "kx": np.float64,
                 "ky": np.float64, "hcg": np.float64, "wlo": np.float64, "whi": np.float64,
-                "parts": PARTICLE, "strm": PARTICLE}
+                "parts": PARTICLE, "strm": PARTICLE, "born": PARTICLE}
 
 
 class SimBase:

@@ -444,6 +444,9 @@ class BunchMarch(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, *grads):
+        if ctx.ckpts is None:
+            raise RuntimeError("BunchMarch: a second backward through the same graph is not supported (the particle "
+                               "checkpoints are released by the first); run the forward again")
         F, qm = ctx.saved_tensors
         geos, every = ctx.geos, ctx.every
         zeros = torch.zeros(ctx.np, dtype=torch.float64, device=F.device)

@@ -16,7 +16,7 @@ static cx dvs(cx a, double s) { return mk(a.re / s, a.im / s); }
 static cx cjg(cx a) { return mk(a.re, -a.im); }
 static cx ik(double k) { return mk(0, k); }
 
-/* ---------------------------------------------------------------- FFT (src/fft.hpp:153-214, fftmany.hpp:95-118) */
+/* ---------------------------------------------------------------- FFT (src/fft.hpp:21-85, fftmany.hpp:95-118) */
 void or_fft1(cx* a, int n, int inverse) {
     int lg = 0;
     while ((1 << lg) < n) ++lg;
@@ -75,7 +75,7 @@ static void tsc_per(double x, double d, int n, int idx[3], double w[3]) {   /* s
     or_tsc_index(x / d, idx, w);
     for (int k = 0; k < 3; ++k) idx[k] = (idx[k] % n + n) % n;
 }
-void or_axis_locate(const or_axis* ax, double x, int* i, double* f) {      /* src/grid.hpp:134-145 */
+void or_axis_locate(const or_axis* ax, double x, int* i, double* f) {      /* src/grid.hpp:35-46 */
     if (ax->uniform) { const double t = (x - ax->x0) * ax->ih; *i = (int)floor(t); *f = t - *i; return; }
     const double* X = ax->X;
     const int n = ax->n;
@@ -263,12 +263,12 @@ double or_adk_rate(const or_adk* L, double E) {                        /* src/io
     E = E < L->Epeak ? E : L->Epeak;
     return exp(L->logpre - L->power * log(E) + L->expo / E);
 }
-static double adk_raw(const or_adk* L, double E) {                     /* src/ionization.cpp:63-66 */
+static double adk_raw(const or_adk* L, double E) {                     /* src/ionization.cpp:17-20 */
     E = E < L->Epeak ? E : L->Epeak;
     return E > 0 ? exp(L->logpre - L->power * log(E) + L->expo / E) : 0.0;
 }
 void or_adk_levels(const double* ip, int n_ip, int Z0, int Zmax, double E0, double omega0, double T, or_adk* lev) {
-    /* src/ionization.cpp:55-62 (coefficients) and 76-99 (code units, peak, skip threshold by bisection) */
+    /* src/ionization.cpp:9-16 (coefficients) and 30-53 (code units, peak, skip threshold by bisection) */
     const double UH = 13.605693122994, Ea = 5.14220674763e11, wa = 4.1341373335e16;
     for (int k = 0; k < Zmax; ++k) { lev[k].logpre = lev[k].power = lev[k].expo = 0; lev[k].Epeak = lev[k].Emin = 1e300; }
     for (int k = Z0; k < Zmax && k < n_ip; ++k) {

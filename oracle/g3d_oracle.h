@@ -51,10 +51,10 @@ typedef struct {
     long nk;
 } or_march_ctx;
 
-/* FFT (src/fft.hpp:203-214 radix 2; src/fftmany.hpp:95-118 rows then columns, inverse scaled) */
+/* FFT (src/fft.hpp:30-33, 74-85 radix 2; src/fftmany.hpp:95-118 rows then columns, inverse scaled) */
 void or_fft1(g3d_cpx* a, int n, int inverse);
 void or_fft_many(int nx, int ny, int two_d, g3d_cpx* data, int count, int inverse);
-/* grids (src/sim_kernels.hpp:21-40, src/grid.hpp:134-145) */
+/* grids (src/sim_kernels.hpp:21-40, src/grid.hpp:35-46) */
 void or_tsc_index(double xi, int idx[3], double w[3]);
 void or_axis_locate(const or_axis* ax, double x, int* i, double* f);
 /* fields at a particle (src/sim_kernels.hpp:44-72; src/sim.hpp:196-209) */
@@ -68,7 +68,7 @@ void or_bunch_deposit(const g3d_particle* p, long n, const or_axis* zax, const d
                       double dy, int nx, int ny, int a, int ns, int N, double* J);
 /* one streaming half-cell over a list, in list order (src/sim_stream.hpp:33-171) */
 void or_stream_halfcell(const or_stream_ctx* c, g3d_particle* p, long n);
-/* ADK and the counter-based draws (src/ionization.hpp:25-46, src/ionization.cpp:55-99) */
+/* ADK and the counter-based draws (src/ionization.hpp:25-46, src/ionization.cpp:9-53) */
 double or_adk_rate(const or_adk* L, double E);
 void or_adk_levels(const double* ip_eV, int n_ip, int Z0, int Zmax, double E0, double omega0, double T, or_adk* out);
 uint64_t or_mix64(uint64_t z);

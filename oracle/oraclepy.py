@@ -136,13 +136,17 @@ class Grid:
         self.iD = np.full(ny, 1 / self.dy)
         self.iA = np.full(ny, 1 / self.dy)
         self.freehand = not c.y_fourier
-        if self.freehand:                    # src/sim_setup.cpp:86-102: walls at the end nodes
+        self.ynode = None
+        if self.freehand:                    # src/sim_setup.cpp:86-108: walls at the end nodes
             Y = np.array(deck.ynode, dtype=np.float64)
             Dl = np.diff(Y)
             A = np.empty(ny)
             A[0] = 0.5 * Dl[0]
             A[1:] = 0.5 * (Dl[:-1] + Dl[1:])
             self.iD, self.iA = 1 / Dl, 1 / A
+            self.ynode = Y
+            self.y_uniform = bool(np.all(Dl == Dl[0]))
+            self.y_ih = 1 / Dl[0]
         Z = np.array(deck.zeta, dtype=np.float64)
         self.zeta = Z
         N = self.N
@@ -159,11 +163,16 @@ class Grid:
     def zaxis(self):
         return Axis(dptr(self.zeta), self.N, int(self.uniform), 0.0, self.ih)
 
+    def yaxis(self):
+        if self.freehand:                    # the node table (src/sim_setup.cpp:105-108)
+            return Axis(dptr(self.ynode), self.ny + 1, int(self.y_uniform), float(self.ynode[0]), self.y_ih)
+        return Axis(None, self.ny + 1, 1, 0.0, 1 / self.dy)
+
     def fields(self, F, ns=None):
         ns = self.N if ns is None else ns
         self._keep.append(F)
-        return Fields(F.ctypes.data_as(C.POINTER(C.c_float)), 0, ns, self.nx, self.ny, 1, 1, self.nk, self.dx, self.dy,
-                      self.Lx, self.Ly, self.zaxis(), Axis(None, self.ny + 1, 1, 0.0, 1 / self.dy))
+        return Fields(F.ctypes.data_as(C.POINTER(C.c_float)), 0, ns, self.nx, self.ny, int(not self.freehand), 1,
+                      self.nk, self.dx, self.dy, self.Lx, self.Ly, self.zaxis(), self.yaxis())
 
 
 def laser_of(deck):
@@ -203,23 +212,71 @@ def stream_spectra(g, J, a2, keep0):
     return out
 
 
-def stream_halfcell(g, deck, F, parts, n, s0, s1, X0, Y0, J=None):
-    """one streaming pass through half-cell n (Fourier y, no ions) in list order; returns (particles, J)"""
+def ion_tables(deck):
+    """the per-species constants and the ADK levels of the deck's ion blocks, as setup_ions forms them
+    (src/sim_ions.cpp:68-87): code units of the deck's wavelength, T = twice the window length"""
+    c = deck.c
+    Lu = c.lambda0 / (2 * math.pi)
+    omega0 = g3d_amd.C_SI / Lu
+    E0 = g3d_amd.ME_SI * g3d_amd.C_SI * omega0 / g3d_amd.E_SI
+    T = 2 * float(deck.zeta[-1])
+    ions, levels = [], []
+    for k in range(c.n_ions):
+        s = c.ions[k]
+        off, Emin = len(levels), 1e300
+        if s.ionization:
+            L = adk_levels([s.ip_eV[i] for i in range(s.n_ip)], s.Z0, s.Zmax, E0, omega0, T)
+            Emin = min(float(L[z][4]) for z in range(s.Z0, s.Zmax))
+            levels.extend(L)
+        ions.append((s.mass, int(s.mobile), int(s.ionization), s.Zmax, off, Emin))
+    return ions, np.array(levels).reshape(-1, 5)
+
+
+def stream_halfcell(g, deck, F, parts, n, s0, s1, X0, Y0, J=None, key=0, bcap=0):
+    """one streaming pass through half-cell n in list order (Fourier or free-hand y; with the deck's ion species,
+    their ADK levels and the draw key, room for bcap freed electrons); returns (particles, J, freed electrons,
+    freed-electron count)"""
     c = deck.c
     p = np.ascontiguousarray(parts, dtype=g3d_amd.PARTICLE).copy()
     J = np.zeros(5 * g.nk) if J is None else J
     F = np.ascontiguousarray(F, dtype=np.float32)
     zs = np.ascontiguousarray(g.zeta)
     cnt = C.c_int(0)
+    born = np.zeros(max(bcap, 1), dtype=g3d_amd.PARTICLE)
     sc = StreamCtx()
     sc.f = g.fields(F)
     sc.laser = laser_of(deck)
-    sc.X0, sc.Y0, sc.zc, sc.ylo, sc.yhi = X0, Y0, c.zeta_center, -1e300, 1e300
+    ylo, yhi = (float(g.ynode[1]), float(g.ynode[-2])) if g.freehand else (-1e300, 1e300)
+    sc.X0, sc.Y0, sc.zc, sc.ylo, sc.yhi = X0, Y0, c.zeta_center, ylo, yhi
     sc.zs = dptr(zs); sc.J = dptr(J); sc.ia = dptr(g.iA); sc.id = dptr(g.iD)
     sc.Delta = s1 - s0; sc.s1 = s1; sc.sTrap = s1 + (c.trap_advance - 1) * (s1 - s0); sc.vz_beam = c.vz_beam
-    sc.n = n; sc.bcnt = C.pointer(cnt); sc.bcap = 0; sc.key = 0
+    sc.n = n; sc.bcnt = C.pointer(cnt); sc.bcap = bcap; sc.key = key
+    ions, levels = ion_tables(deck)
+    ion_arr = (Ion * max(1, len(ions)))(*[Ion(*t) for t in ions])
+    adk_arr = (Adk * max(1, len(levels)))(*[Adk(*map(float, L)) for L in levels])
+    sc.ions = C.cast(ion_arr, C.POINTER(Ion))
+    sc.adk = C.cast(adk_arr, C.POINTER(Adk))
+    sc.born = born.ctypes.data_as(C.c_void_p)
     lib().or_stream_halfcell(C.byref(sc), ptr(p), len(p))
-    return p, J
+    return p, J, born[:min(cnt.value, bcap)].copy(), cnt.value
+
+
+def bunch_deposit(g, parts):
+    """the beam deposit jw [3][N+3][nk] of a particle list on one block (src/sim_particles.cpp:66-89)"""
+    p = np.ascontiguousarray(parts, dtype=g3d_amd.PARTICLE)
+    J = np.zeros(3 * (g.N + 3) * g.nk)
+    za = g.zaxis()
+    lib().or_bunch_deposit(ptr(p), len(p), C.byref(za), ptr(g.hcg), ptr(g.iA), g.dx, g.dy, g.nx, g.ny, 0, g.N, g.N,
+                           ptr(J))
+    return J
+
+
+def key_chain(seed, step, n, pass_, ids):
+    """the draw bases ion_key_particle(ion_key_slice(ion_key_step(seed, step), n, pass), id) (src/ionization.hpp:44-46)"""
+    L = lib()
+    ks = L.or_key_step(seed, step)
+    kl = L.or_key_slice(ks, n, pass_)
+    return ks, kl, [L.or_key_particle(kl, int(i)) for i in ids]
 
 
 def bunch_push(g, deck, F, parts, s0, s1, X0, Y0, plasma=False):

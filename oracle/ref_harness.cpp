@@ -101,6 +101,14 @@ Config make_config(const g3d_config& g) {
     return c;
 }
 
+// the freed electrons stored by the last stream pass (the count bcnt, capped at the born buffer)
+size_t born_count(Simulation& s) {
+    if (s.bcnt.size() == 0) return 0;
+    int c = 0;
+    s.bcnt.to_host(&c, 0, 1);
+    return std::min(size_t(c < 0 ? 0 : c), s.born.size());
+}
+
 template <class T> int copy_out(const Buf<T>& b, void* out, size_t bytes) {
     if (bytes != b.size() * sizeof(T)) { err_ = "size mismatch"; return 2; }
     b.to_host(static_cast<T*>(out), 0, b.size());
@@ -159,6 +167,18 @@ int ref_sim_call(void* h, const char* op_, const double* a, int na) {
         else if (op == "set_s") { need(2); s.s = a[0]; s.istep = long(a[1]); }
         else if (op == "set_starting") { need(1); s.starting_ = a[0] != 0; }
         else if (op == "stream_pass") { need(4); stream_pass(s, int(a[0]), int(a[1]), a[2], a[3]); }
+        else if (op == "reserve_born") {
+            // room for `cap` freed electrons behind the streaming list, as load_plasma leaves it
+            // (src/sim_plasma.cpp:82-88): born_cap, the born buffer, and the list's spare capacity
+            need(1);
+            const long cap = long(a[0]);
+            std::vector<Particle> v(size_t(s.nstrm));
+            s.strm.to_host(v.data(), 0, v.size());
+            s.strm.resize(size_t(s.nstrm + cap)); s.strm_bk.resize(s.strm.size());
+            s.strm.from_host(v.data(), 0, v.size());
+            if (s.born.size() < size_t(cap)) s.born.resize(size_t(cap));
+            s.born_cap = cap;
+        }
         else if (op == "j0_from_jk") {
             const long n = long(s.ns + 1) * s.nk;
             std::vector<cpx> t(n);
@@ -179,7 +199,8 @@ int ref_sim_array_size(void* h, const char* name_, size_t* bytes) {
         {"js0x", s.js0x.size() * 16}, {"js0y", s.js0y.size() * 16}, {"js0z", s.js0z.size() * 16},
         {"jw", s.jw.size() * 8}, {"F", s.F.size() * 4}, {"jsr", s.jsr.size() * 8}, {"jsn", s.jsn.size() * 16},
         {"rho_lines", s.rho_lines.size() * 8},
-        {"parts", size_t(s.np) * sizeof(Particle)}, {"strm", size_t(s.nstrm) * sizeof(Particle)}};
+        {"parts", size_t(s.np) * sizeof(Particle)}, {"strm", size_t(s.nstrm) * sizeof(Particle)},
+        {"born", born_count(s) * sizeof(Particle)}};
     auto it = m.find(n);
     if (it == m.end()) { err_ = "unknown array " + n; return 3; }
     *bytes = it->second;
@@ -217,6 +238,12 @@ int ref_sim_get(void* h, const char* name_, void* out, size_t bytes) {
         if (n == "strm") {
             if (bytes != size_t(s.nstrm) * sizeof(Particle)) { err_ = "size mismatch"; return 2; }
             s.strm.to_host(static_cast<Particle*>(out), 0, size_t(s.nstrm));
+            return 0;
+        }
+        if (n == "born") {
+            const size_t c = born_count(s);
+            if (bytes != c * sizeof(Particle)) { err_ = "size mismatch"; return 2; }
+            s.born.to_host(static_cast<Particle*>(out), 0, c);
             return 0;
         }
         err_ = "unknown array " + n;
@@ -288,7 +315,7 @@ int ref_fft_many(int nx, int ny, int two_d, g3d_cpx* data, int count, int invers
     } catch (const std::exception& e) { return fail(e); }
 }
 
-// ADK table of one species (src/ionization.cpp:76-99): 5 doubles per level Z0 ... Zmax-1
+// ADK table of one species (src/ionization.cpp:30-53): 5 doubles per level Z0 ... Zmax-1
 int ref_adk_levels(const double* ip_eV, int n_ip, int Z0, int Zmax, double E0, double omega0, double T, double* out) {
     try {
         std::vector<double> ip(ip_eV, ip_eV + n_ip);
@@ -302,5 +329,9 @@ int ref_adk_levels(const double* ip_eV, int n_ip, int Z0, int Zmax, double E0, d
 }
 double ref_adk_rate_SI(double U_eV, int Zafter, double E) { return adk_rate_SI(U_eV, Zafter, E); }
 unsigned long long ref_mix64(unsigned long long z) { return mix64(z); }
+// the key chain of the ionization draws (src/ionization.hpp:44-46)
+unsigned long long ref_key_step(unsigned long long seed, long step) { return ion_key_step(seed, step); }
+unsigned long long ref_key_slice(unsigned long long ks, int n, int pass) { return ion_key_slice(ks, n, pass); }
+unsigned long long ref_key_particle(unsigned long long kl, unsigned long long id) { return ion_key_particle(kl, id); }
 double ref_uniform01(unsigned long long k) { return uniform01(k); }
 }

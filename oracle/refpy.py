@@ -41,6 +41,9 @@ def lib():
             "ref_adk_levels": ([P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, P], C.c_int),
             "ref_adk_rate_SI": ([C.c_double, C.c_int, C.c_double], C.c_double),
             "ref_mix64": ([C.c_ulonglong], C.c_ulonglong), "ref_uniform01": ([C.c_ulonglong], C.c_double),
+            "ref_key_step": ([C.c_ulonglong, C.c_long], C.c_ulonglong),
+            "ref_key_slice": ([C.c_ulonglong, C.c_int, C.c_int], C.c_ulonglong),
+            "ref_key_particle": ([C.c_ulonglong, C.c_ulonglong], C.c_ulonglong),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)

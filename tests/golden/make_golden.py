@@ -18,7 +18,7 @@ import refpy  # noqa: E402
 import golden_inputs as gi  # noqa: E402
 
 
-def main():
+def main(only=None):
     refpy.set_threads(1)
     out = {}
     # batched FFT (src/fftmany.hpp): 2D planes and rows, both directions
@@ -69,16 +69,64 @@ def main():
     ref.set_particles(P)
     ref.call("push", 0.0, 20.0)
     out["push"] = dict(F=F, parts_in=P, parts_out=ref.array("parts"), X0=ref.info()["X0"], Y0=ref.info()["Y0"])
-    # ADK levels of nitrogen (src/ionization.cpp:76-99), code units of lambda0 = 0.8 um
+    # ADK levels of nitrogen (src/ionization.cpp:30-53), code units of lambda0 = 0.8 um
     out["adk"] = dict(ip=gi.N_IP, table=gi.ref_adk_table())
     # SplitMix64 finaliser and the draw keys (src/ionization.hpp:37-46)
     keys = np.array([0, 1, 1234567, 0x9e3779b97f4a7c15, 2 ** 63 + 12345, 2 ** 64 - 1], dtype=np.uint64)
     out["mix64"] = dict(keys=keys, mix=np.array([refpy.lib().ref_mix64(int(k)) for k in keys], dtype=np.uint64),
                         u01=np.array([refpy.lib().ref_uniform01(int(k)) for k in keys]))
+    # one streaming pass with ionizing nitrogen, mobile and fixed, from the neutral atom (src/sim_stream.hpp:67-171):
+    # the charge states, the freed electrons (in list order on one thread) and the deposit
+    deck = gi.ion_deck()
+    ref = refpy.RefSim(deck)
+    ref.call("init")
+    F, P, cap = gi.ion_stream_input(ref)
+    seed, step, n, pas = gi.ION_KEY
+    ref.set("F", F)
+    ref.set("strm", P)
+    ref.call("set_s", 0.0, step)
+    ref.call("reserve_born", cap)
+    ref.call("stream_pass", n, pas, 0.0, 5.0)
+    L = refpy.lib()
+    key = L.ref_key_slice(L.ref_key_step(seed, step), n, pas)
+    out["stream_ions"] = dict(F=F, parts_in=P, parts_out=ref.array("strm"), jsr=ref.array("jsr"), born=ref.array("born"),
+                              cap=cap, key=np.uint64(key), X0=ref.info()["X0"], Y0=ref.info()["Y0"])
+    # one streaming pass on a free-hand y grid and a free-hand zeta table, with the plane wave (src/sim_stream.hpp:
+    # 33-171, the table locates of src/grid.hpp:35-46): walls, nodes and half-nodes of the deposit
+    deck = gi.fh_stream_deck()
+    ref = refpy.RefSim(deck)
+    ref.call("init")
+    F, P = gi.fh_stream_input(ref, deck)
+    ref.set("F", F)
+    ref.set("strm", P)
+    ref.call("stream_pass", gi.FH_STREAM_N, 0, 0.0, 5.0)
+    out["stream_fh"] = dict(F=F, parts_in=P, parts_out=ref.array("strm"), jsr=ref.array("jsr"), ynode=deck.ynode,
+                            zeta=deck.zeta, X0=ref.info()["X0"], Y0=ref.info()["Y0"])
+    # the beam deposit of currents() (src/sim_particles.cpp:56-107): jw on one thread, in list order
+    deck = gi.bunch_small_deck()
+    ref = refpy.RefSim(deck)
+    P = gi.deposit_input()
+    ref.set_particles(P)
+    ref.call("currents")
+    out["bunch_deposit"] = dict(parts=P, jw=ref.array("jw"))
+    # the key chain of the ionization draws (src/ionization.hpp:44-46)
+    ks, kl, kp = [], [], []
+    for seed, step, n, pas in gi.KEY_CASES:
+        a = L.ref_key_step(seed, step)
+        b = L.ref_key_slice(a, n, pas)
+        ks.append(a); kl.append(b); kp.append([L.ref_key_particle(b, int(i)) for i in gi.KEY_IDS])
+    out["keys"] = dict(seed=np.array([c[0] for c in gi.KEY_CASES], dtype=np.uint64),
+                       step=np.array([c[1] for c in gi.KEY_CASES], dtype=np.int64),
+                       n=np.array([c[2] for c in gi.KEY_CASES], dtype=np.int64),
+                       pas=np.array([c[3] for c in gi.KEY_CASES], dtype=np.int64), ids=gi.KEY_IDS,
+                       key_step=np.array(ks, dtype=np.uint64), key_slice=np.array(kl, dtype=np.uint64),
+                       key_particle=np.array(kp, dtype=np.uint64))
     for name, d in out.items():
+        if only and name not in only:
+            continue
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **d)
         print("wrote", name, os.path.getsize(os.path.join(HERE, name + ".npz")), "bytes")
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -211,7 +211,7 @@ def test_stream_push_against_oracle():
     """the forward streaming push (g3d_stream_push) against the oracle's stream pass on the golden input: the state
     and alive codes of every electron (waiting, entering, trapped, dead), the deposit to 1e-13 of its largest value"""
     deck, sim, F, P, hc, F0, F1, info = spush_setup()
-    want, J_want = orc.stream_halfcell(orc.Grid(deck), deck, F, P, gi.STREAM_N, 0.0, 5.0, info["X0"], info["Y0"])
+    want, J_want, _, _ = orc.stream_halfcell(orc.Grid(deck), deck, F, P, gi.STREAM_N, 0.0, 5.0, info["X0"], info["Y0"])
     state = [gpu(P[f].astype(np.float64)) for f in ("x", "y", "zeta", "z", "px", "py", "pz")]
     out, alive, J = ga.stream_push_raw(hc, F0, F1, state, gpu(P["q"]), gpu(P["qm"]), gpu(P["w"]),
                                        gpu(P["alive"].astype(np.int32)))

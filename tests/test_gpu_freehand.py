@@ -87,10 +87,13 @@ def test_fh_grid_tables():
     assert hz.min() > 0.4 - 1e-12 and hz.max() < 0.7 + 1e-12 and b.zeta[0] == 0
 
 
-def test_fh_march_random_bit_exact():
+@pytest.mark.parametrize("pivots", [1, 0])
+def test_fh_march_random_bit_exact(pivots):
     """Simulation::march(D) on the lines (src/sim_kernels.hpp:213-313): the same operations in the same order,
-    so the spectra agree bit for bit"""
+    so the spectra agree bit for bit, with the pivot table formed once per D (pivots=1) and with the pivots formed
+    inside the eliminations (pivots=0, the path of the kernel-level g3d_march)"""
     gpu, ref = pair(fh_plasma_deck, nx=16, nz=24)
+    gpu.call("line_pivots", pivots)
     arrs = random_spectra(gpu, SPEC + ["j0x", "j0y", "jkx", "jky", "jkz"], 21)
     for sim in (gpu, ref):
         for k, v in arrs.items():
@@ -100,9 +103,11 @@ def test_fh_march_random_bit_exact():
         np.testing.assert_array_equal(gpu.array(k), ref.array(k), err_msg=k)
 
 
-def test_fh_march_slice_bit_exact():
-    """Simulation::march_slice(n, D) on the lines, the front slice and an inner one"""
+@pytest.mark.parametrize("pivots", [1, 0])
+def test_fh_march_slice_bit_exact(pivots):
+    """Simulation::march_slice(n, D) on the lines, the front slice and an inner one, both pivot paths"""
     gpu, ref = pair(fh_plasma_deck, nx=16, nz=24)
+    gpu.call("line_pivots", pivots)
     arrs = random_spectra(gpu, SPEC + ["j0x", "j0y", "jkx", "jky", "jkz"], 22)
     for sim in (gpu, ref):
         for k, v in arrs.items():
@@ -111,6 +116,28 @@ def test_fh_march_slice_bit_exact():
         sim.call("march_slice", 11, 1.5)
     for k in SPEC:
         np.testing.assert_array_equal(gpu.array(k), ref.array(k), err_msg=k)
+
+
+def test_fh_march_after_geometry_change():
+    """The pivot table depends on the half-cell lengths and k_x as well as D: after hcg is set anew, a march with
+    the same D must use the new geometry (the same spectra as a fresh simulation given that geometry first, and as
+    the march without the table)"""
+    names = SPEC + ["j0x", "j0y", "jkx", "jky", "jkz"]
+    sims = [g3d_amd.Sim(fh_plasma_deck(nx=16, nz=24)) for _ in range(3)]
+    arrs = random_spectra(sims[0], names, 24)
+    hcg = sims[0].array("hcg") * 1.1
+    for k, v in arrs.items():
+        sims[0].set(k, v)
+    sims[0].call("march", 2.5)                 # the table of the old geometry
+    sims[2].call("line_pivots", 0)
+    for s in sims:
+        s.set("hcg", hcg)
+        for k, v in arrs.items():
+            s.set(k, v)
+        s.call("march", 2.5)
+    for k in SPEC:
+        np.testing.assert_array_equal(sims[0].array(k), sims[1].array(k), err_msg=k)
+        np.testing.assert_array_equal(sims[0].array(k), sims[2].array(k), err_msg=k)
 
 
 def test_fh_sponge_and_real_fields_random():

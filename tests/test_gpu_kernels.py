@@ -74,7 +74,7 @@ def test_stream_pass_against_oracle():
     sim.set("strm", P)
     sim.call("stream_pass", gi.STREAM_N, 0, 0.0, 5.0)
     info = sim.info()
-    p_want, J_want = orc.stream_halfcell(orc.Grid(deck), deck, F, P, gi.STREAM_N, 0.0, 5.0, info["X0"], info["Y0"])
+    p_want, J_want, _, _ = orc.stream_halfcell(orc.Grid(deck), deck, F, P, gi.STREAM_N, 0.0, 5.0, info["X0"], info["Y0"])
     p = sim.array("strm")
     for f in ("x", "y", "zeta", "px", "py", "pz", "z", "alive"):
         assert np.array_equal(p[f], p_want[f]), f

@@ -62,3 +62,24 @@ def test_checkpoint_bookkeeping(monkeypatch):
         g = torch.autograd.grad(sum((a * c).sum() for a, c in zip(out, cot)), ins)
         for a, b in zip(g, ref):
             assert torch.allclose(a, b, rtol=1e-13, atol=1e-14), every
+
+
+def test_second_backward_is_refused(monkeypatch):
+    """the checkpoints are released by the first reverse sweep: a second backward through the same graph
+    (retain_graph=True) raises a clear error instead of failing inside the sweep"""
+    import pytest
+    monkeypatch.setattr(ga, "bunch_push_raw", toy_raw)
+    monkeypatch.setattr(ga, "bunch_push_vjp_raw", toy_vjp)
+    monkeypatch.setattr(ga, "_f64", lambda t, what: t.contiguous())
+    n = 3
+    geos = [ga.PushGeometry(s0=0.1 * t, s1=0.1 * (t + 1)) for t in range(4)]
+    F = torch.ones(4, dtype=torch.float64, requires_grad=True)
+    st = [torch.full((n,), 0.5, dtype=torch.float64, requires_grad=True) for _ in range(6)]
+    z, rem, qm = torch.zeros(n, dtype=torch.float64), torch.ones(n, dtype=torch.float64), -torch.ones(
+        n, dtype=torch.float64)
+    alive = torch.ones(n, dtype=torch.int32)
+    out = ga.BunchMarch.apply(geos, 2, F, *st, z, rem, qm, alive)
+    loss = sum(o.sum() for o in out)
+    loss.backward(retain_graph=True)
+    with pytest.raises(RuntimeError, match="second backward"):
+        loss.backward()

@@ -92,7 +92,7 @@ def test_stream_pass_golden():
     g = gold("stream_pass")
     deck = gi.small_deck()
     grid = orc.Grid(deck)
-    p, J = orc.stream_halfcell(grid, deck, g["F"], g["parts_in"], gi.STREAM_N, 0.0, 5.0, float(g["X0"]), float(g["Y0"]))
+    p, J, _, _ = orc.stream_halfcell(grid, deck, g["F"], g["parts_in"], gi.STREAM_N, 0.0, 5.0, float(g["X0"]), float(g["Y0"]))
     ref = g["parts_out"]
     for f in ("x", "y", "zeta", "px", "py", "pz", "z", "alive"):
         assert np.array_equal(p[f], ref[f]), f
@@ -208,3 +208,96 @@ def test_real_fields_matches_numpy():
             0.5 * (re["Rx"] - re["Lx"]), re["Bz"]]
     for c in range(6):
         assert np.abs(F[c] - want[c]).max() <= 1e-6 * np.abs(want[c]).max()
+
+
+# ---------------------------------------------------------------- the paths pinned by the later goldens
+def test_stream_ions_golden():
+    """one streaming pass with ADK nitrogen, mobile and fixed ions (src/sim_stream.hpp:67-171): charge states, the
+    freed electrons in list order and their count, and the deposit bit for bit against the reference"""
+    g = gold("stream_ions")
+    deck = gi.ion_deck()
+    grid = orc.Grid(deck)
+    seed, step, n, pas = gi.ION_KEY
+    key = orc.lib().or_key_slice(orc.lib().or_key_step(seed, step), n, pas)
+    assert key == int(g["key"])
+    cap = int(g["cap"])
+    p, J, born, cnt = orc.stream_halfcell(grid, deck, g["F"], g["parts_in"], n, 0.0, 5.0, float(g["X0"]),
+                                          float(g["Y0"]), key=key, bcap=cap)
+    ref = g["parts_out"]
+    for f in ("x", "y", "zeta", "px", "py", "pz", "z", "q", "qm", "alive", "sp"):
+        assert np.array_equal(p[f], ref[f]), f
+    assert np.array_equal(J, g["jsr"])
+    assert cnt == len(g["born"]) and np.array_equal(born, g["born"])
+    assert cnt >= 10 and (p["q"] != g["parts_in"]["q"]).sum() >= 10        # the deck ionizes
+    # every freed electron is the clone of an ion (same id and position, q = qm = -1, species 50 + species index)
+    ions = {int(i): s for i, s in zip(p["id"], p["sp"]) if 3 <= s < 50}
+    assert all(b["q"] == -1 and b["qm"] == -1 and b["sp"] == 50 + ions[int(b["id"])] - 3 for b in born)
+    # charge is conserved: the charge gained by the ions equals the electrons freed
+    assert (p["q"] - g["parts_in"]["q"])[p["sp"] >= 3].sum() == cnt
+
+
+def test_stream_freehand_golden():
+    """one streaming pass on a free-hand y grid between walls and a free-hand zeta table (the table branches of
+    Axis::locate, src/grid.hpp:35-46; the node and half-node deposit, the wall loss): bit for bit"""
+    g = gold("stream_fh")
+    deck = gi.fh_stream_deck()
+    assert np.array_equal(deck.ynode, g["ynode"]) and np.array_equal(deck.zeta, g["zeta"])
+    grid = orc.Grid(deck)
+    assert not grid.uniform and not grid.y_uniform
+    p, J, _, _ = orc.stream_halfcell(grid, deck, g["F"], g["parts_in"], gi.FH_STREAM_N, 0.0, 5.0, float(g["X0"]),
+                                     float(g["Y0"]))
+    ref = g["parts_out"]
+    for f in ("x", "y", "zeta", "px", "py", "pz", "z", "alive"):
+        assert np.array_equal(p[f], ref[f]), f
+    assert (p["alive"] == 0).sum() >= 2                     # lost through the walls
+    assert np.array_equal(J, g["jsr"])
+
+
+def test_axis_locate_table_brute_force():
+    """the table locate (src/grid.hpp:35-46) against a linear scan: X[i] <= x < X[i+1] inside, the end cells
+    continued outside, f = (x - X[i]) / (X[i+1] - X[i])"""
+    import ctypes as C
+    X = np.array(gi.fh_stream_deck().zeta)
+    ax = orc.Axis(orc.dptr(X), len(X), 0, 0.0, 1.0)
+    i, f = C.c_int(), C.c_double()
+    r = np.random.default_rng(9)
+    for x in np.concatenate([r.uniform(X[0] - 2, X[-1] + 2, 300), X, X[:-1] + 1e-13]):
+        orc.lib().or_axis_locate(C.byref(ax), float(x), C.byref(i), C.byref(f))
+        if X[0] <= x < X[-1]:
+            k = max(j for j in range(len(X) - 1) if X[j] <= x)
+            assert i.value == k and abs(f.value - (x - X[k]) / (X[k + 1] - X[k])) < 1e-13
+        else:
+            lo = 0 if x < X[0] else len(X) - 2
+            t = (x - X[lo if x < X[0] else -1]) / (X[lo + 1] - X[lo])
+            assert i.value + f.value == pytest.approx((0 if x < X[0] else len(X) - 1) + t, abs=1e-12)
+
+
+def test_bunch_deposit_golden():
+    """the beam deposit of currents() (src/sim_particles.cpp:56-107): jw bit for bit in list order, and the charge
+    of the deposited particles recovered from j_z (sum j_z h dx dy = sum q w over the particles in the window)"""
+    g = gold("bunch_deposit")
+    deck = gi.bunch_small_deck()
+    grid = orc.Grid(deck)
+    J = orc.bunch_deposit(grid, g["parts"])
+    assert np.array_equal(J, g["jw"])
+    P = g["parts"]
+    N, nk = grid.N, grid.nk
+    jz = J.reshape(3, N + 3, nk)[2]
+    live = (P["alive"] == 1) & (P["zeta"] >= 0) & (P["zeta"] < grid.zeta[-1])
+    h = grid.hcg[2]
+    assert abs(jz.sum() * h * grid.dx * grid.dy - (P["q"] * P["w"])[live].sum()) <= 1e-12 * np.abs(P["w"]).sum()
+    assert live.sum() < len(P)                              # dead and outside particles are left out
+
+
+def test_key_chain_golden():
+    """ion_key_step / ion_key_slice / ion_key_particle (src/ionization.hpp:44-46) against the reference, including
+    the wrap of the seed product and a negative step"""
+    g = gold("keys")
+    for c in range(len(g["seed"])):
+        ks, kl, kp = orc.key_chain(int(g["seed"][c]), int(g["step"][c]), int(g["n"][c]), int(g["pas"][c]),
+                                   g["ids"])
+        assert ks == int(g["key_step"][c]) and kl == int(g["key_slice"][c])
+        assert kp == [int(v) for v in g["key_particle"][c]]
+    # the keys of neighbouring particles, slices and steps differ (a fixed point would repeat every draw)
+    flat = g["key_particle"].reshape(-1)
+    assert len(set(int(v) for v in flat)) == flat.size
