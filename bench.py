@@ -1,8 +1,14 @@
-"""Benchmark of the per-slice march on one MI355X: the laser-wakefield deck (256 x 256 Fourier modes, 1024 uniform
-zeta slices, h = 0.25/k0, uniform plasma 1e-3 n_c streamed as 2 x 2 electrons per cell per slice = 262144 per
-slice, two passes per half-cell, Gaussian grid laser a0 = 2, w0 = 30/k0, c tau = 15/k0 at zeta = 200/k0, sponge
-24/k0, dz = 20/k0).  A step is one s-step of the march (currents, the 1023 half-cells x 2 passes of the streaming
-plasma, sponge and real fields).  Metric: streaming-particle pushes per second including deposit and gather.
+"""Benchmark of the per-slice march on one MI355X: the laser-wakefield deck with a trailing witness bunch
+(256 x 256 Fourier modes, 1024 uniform zeta slices, h = 0.25/k0, uniform plasma 1e-3 n_c streamed as 2 x 2
+electrons per cell per slice = 262144 per slice, two passes per half-cell, Gaussian grid laser a0 = 2, w0 = 30/k0,
+c tau = 15/k0 at zeta = 200/k0, sponge 24/k0, dz = 20/k0; witness of 2^20 electrons, sigma_r = 1.5/k0,
+sigma_zeta = 3/k0, 0.005 n_c peak = 2.40 pC, gamma = 200, at zeta = 100/k0 in the first bucket behind the laser).
+A step is one s-step of the march: the RK4 push of the witness, its deposit and current spectra, the 1023
+half-cells x 2 passes of the streaming plasma, sponge and real fields.  It runs in the FP64 form and in the
+FP32-mixed form from the same start.  Metric: live pushes per second (streaming particles through a half-cell in a
+pass, counted on the device, plus witness particles advanced by a step) including deposit and gather; the line
+reports the mixed form, with its per-slice field error against FP64 and the witness's energy-gain and emittance
+errors from the reference's centred moments.
 
 --workload freehand: config 3 instead (512 Fourier x modes by 386 free-hand y cells between conducting walls,
 2048 slices on a zeta table from 0.1 to 0.5/k0, electrons 2 x 1 per cell, mobile hydrogen, nitrogen with ADK
@@ -32,16 +38,38 @@ TIME_EVERY = 16
 LAMBDA0 = 2 * np.pi * 1e-6
 
 
-def lwfa_deck(nx=256, ny=256, nz=1024, h=0.25):
+def lwfa_deck(nx=256, ny=256, nz=1024, h=0.25, witness=True):
+    """the laser-wakefield deck; witness: the trailing bunch of 2^20 electrons (config 2)"""
     import g3d_amd
     d = g3d_amd.Deck(LAMBDA0)
     d.c.nx = nx; d.c.Lx = 240.0
     d.y_uniform(ny, 240.0)
     d.zeta_uniform(nz, (nz - 1) * h)
-    d.set(dz=20.0, z_end=2000.0, sponge=24.0, sponge_y=24.0, move_x=0, move_y=0, plasma_on=1, n0=1e-3,
+    d.set(dz=20.0, z_end=4000.0, sponge=24.0, sponge_y=24.0, move_x=0, move_y=0, plasma_on=1, n0=1e-3,
           ppc_x=2, ppc_y=2, smooth_length=0.0, iterations=2)
     d.add_grid_laser(a0=2.0, waist=30.0, tau=15.0, zeta_c=min(200.0, 0.78 * (nz - 1) * h), z_focus=0.0)
+    if witness:
+        # peak 0.005 n_c: charge 0.005 (2 pi)^1.5 1.5^2 3 n_c/k0^3 = 2.40 pC at lambda0 = 2 pi um
+        d.set(charge=2.40e-12, gamma=200.0, sigx=1.5, sigy=1.5, sigz=3.0, emitx=1e-3, emity=1e-3, espread=0.0,
+              zeta_center=min(100.0, 0.39 * (nz - 1) * h), nmacro=1 << 20, seed=7)
     return d
+
+
+def bunch_moments(p):
+    """the beam moments of the reference's bunches.txt (src/sim_bunch_diag.cpp:9-61) for the live particles of a
+    list (one group): weights w; first moments, then second moments centred on them (two passes).  Returns
+    (w, <gamma>, eps_nx, eps_ny) in code units."""
+    live = p[p["alive"] == 1]
+    w = live["w"]
+    W = w.sum()
+    g = np.sqrt(1 + live["px"] ** 2 + live["py"] ** 2 + live["pz"] ** 2)
+    x, y, ux, uy = live["x"], live["y"], live["px"], live["py"]
+    m = {k: (w * v).sum() / W for k, v in (("x", x), ("y", y), ("ux", ux), ("uy", uy), ("g", g))}
+    dx, dy, dux, duy = x - m["x"], y - m["y"], ux - m["ux"], uy - m["uy"]
+    var = lambda a, b: (w * a * b).sum() / W
+    ex = np.sqrt(max(0.0, var(dx, dx) * var(dux, dux) - var(dx, dux) ** 2))
+    ey = np.sqrt(max(0.0, var(dy, dy) * var(duy, duy) - var(dy, duy) ** 2))
+    return W, m["g"], ex, ey
 
 
 def vacuum_deck(nx=64, nz=256, h=0.5, nmacro=1 << 18):
@@ -257,36 +285,62 @@ def stream_bytes_per_launch(nstrm, nk):
     return nstrm * 164.0 + nk * 88.0
 
 
-def cpu_baseline(threads):
+def cpu_model():
+    """the host CPU's model name and logical CPU count"""
+    name = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return name, os.cpu_count()
+
+
+def cpu_baseline(threads, witness=True):
     """The reference's own CPU build (oracle/_ref) on a bounded sample of the same workload: the full 256 x 256
-    transverse grid and 262144 streaming electrons per slice, 256 slices instead of 1024 (the per-half-cell work does
-    not depend on the window length), 2 steps after init: about 10 s on 16 cores."""
+    transverse grid, 262144 streaming electrons per slice and the 2^20-electron witness, 256 slices instead of 1024
+    (the per-half-cell work does not depend on the window length), 2 steps after init: about 15 s on 16 cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import refpy
     if not refpy.available():
         return None
     refpy.set_threads(threads)
     nz = 256
-    d = lwfa_deck(nz=nz)
+    d = lwfa_deck(nz=nz, witness=witness)
     ref = refpy.RefSim(d)
     ref.init()
     t = time.perf_counter()
     steps = 2
+    pushes = 0
     for _ in range(steps):
+        pushes += ref.info()["np"]
         ref.step()
+        pushes += ref.info()["nstrm"] * (nz - 1) * 2
     dt = time.perf_counter() - t
-    nstrm = ref.info()["nstrm"]
-    pushes = nstrm * (nz - 1) * 2 * steps
+    model, ncpu = cpu_model()
     return {"value": pushes / dt, "unit": "pushes/s", "cores": threads, "kind": "reference",
+            "host_cpu": model, "host_logical_cpus": ncpu,
             "sample": "reference CPU build (g++ -O3 -fopenmp, %d threads), 256x256 modes, %d streaming electrons per "
-                      "slice, %d slices, 2 passes, %d steps: %.2f s" % (threads, nstrm, nz, steps, dt)}
+                      "slice, %d slices, 2 passes, %s witness, %d steps: %.2f s"
+                      % (threads, ref.info()["nstrm"], nz, "2^20" if witness else "no", steps, dt)}
 
 
 def stream_bytes_mixed(nstrm, nk):
-    """Algorithmic HBM bytes of one launch of k_stream_mixed: per particle 160 B (read x y zeta z px py pz w q qm +
-    alive, write x y zeta z px py pz q qm + alive; the state stays FP64 in HBM), per point the F gather of slices
-    n, n+1 (48 B) and the 5 float current planes (20 B)."""
+    """Design HBM bytes of one launch of k_stream_mixed (what the kernel reads and writes): per particle 160 B (read
+    x y zeta z px py pz w q qm + alive, write x y zeta z px py pz q qm + alive; the state stays FP64 in HBM), per
+    point the F gather of slices n, n+1 (48 B) and the 5 float current planes (20 B)."""
     return nstrm * 160.0 + nk * 68.0
+
+
+# Algorithmic HBM bytes of the streaming half-cell pass (the compulsory traffic of the march): per push of a
+# streaming particle (2 passes restored from the start state: the state read twice and written once, 8 words and a
+# 4-byte state word) 98 B in FP64 and 64 B in FP32-mixed (z and zeta FP64); per grid point the gather of F of the
+# slice (6 floats, slice n + 1 stays in the caches) 24 B and the deposit of j_x, j_y, j_z 24 B (FP64) or 12 B (FP32)
+ALGO_PUSH_B = {"fp64": 98.0, "mixed": 64.0}
+ALGO_POINT_B = {"fp64": 48.0, "mixed": 36.0}
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stream_traffic.json")
@@ -306,7 +360,7 @@ def measured_traffic(kernel):
 
 def run_form(args, mixed, torch):
     import g3d_amd
-    d = lwfa_deck(nz=args.nz)
+    d = lwfa_deck(nz=args.nz, witness=args.witness)
     d.set(mixed=int(mixed))
     sim = g3d_amd.Sim(d)
     t0 = time.perf_counter()
@@ -324,19 +378,25 @@ def run_form(args, mixed, torch):
     # cost a few microseconds of launch spacing; a uniform sample over the slices keeps that out of the step time)
     sim.timing(["stream_halfcell"])
     sim.timing_every(TIME_EVERY)
+    i0 = sim.info()
     t0 = time.perf_counter()
-    pushes = 0
     for k in range(args.steps):
         sim.step()
-        pushes += sim.info()["nstrm"] * (N - 1) * 2
         print("%s: step %d done" % (tag, k), file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    i1 = sim.info()
+    ps, pb = i1["pushes_stream"] - i0["pushes_stream"], i1["pushes_beam"] - i0["pushes_beam"]
+    pushes = ps + pb
     ms, cnt = sim.timers()["stream_halfcell"]
     avg_s = ms / 1000 / max(cnt, 1)
     sim.timing_every(1)
     nstrm = sim.info()["nstrm"]
-    bpl = stream_bytes_mixed(nstrm, nk) if mixed else stream_bytes_per_launch(nstrm, nk)
+    form = "mixed" if mixed else "fp64"
+    design_bpl = stream_bytes_mixed(nstrm, nk) if mixed else stream_bytes_per_launch(nstrm, nk)
+    # algorithmic bytes of one stream launch: the live pushes of an average launch and the grid points
+    push_per_launch = ps / (args.steps * (N - 1) * sim_passes(sim))
+    bpl = push_per_launch * ALGO_PUSH_B[form] + nk * ALGO_POINT_B[form]
     kernel = "k_stream_mixed" if mixed else "k_stream_halfcell"
     # one more step, outside the timed region, with every kernel family timed (the per-kernel breakdown)
     sim.timing(True)
@@ -346,9 +406,14 @@ def run_form(args, mixed, torch):
            for k, v in sim.timers().items() if v[1]}
     sim.timing(False)
     return {"sim": sim, "value": pushes / dt, "ms_step": 1000 * dt / args.steps, "N": N, "nstrm": nstrm,
-            "pushes_per_step": pushes // args.steps, "kernel": kernel, "bpl": bpl, "avg_us": avg_s * 1e6,
-            "achieved": bpl / avg_s / 1e9, "traffic": measured_traffic(kernel), "fam": fam, "t_init": t_init,
-            "timed_launches": cnt}
+            "pushes_per_step": pushes // args.steps, "stream_pushes_per_step": ps // args.steps,
+            "beam_pushes_per_step": pb // args.steps, "kernel": kernel, "bpl": bpl, "design_bpl": design_bpl,
+            "push_per_launch": push_per_launch, "avg_us": avg_s * 1e6, "achieved": bpl / avg_s / 1e9,
+            "traffic": measured_traffic(kernel), "fam": fam, "t_init": t_init, "timed_launches": cnt}
+
+
+def sim_passes(sim):
+    return int(sim._deck.c.iterations)
 
 
 def copy_calibration(mib):
@@ -369,6 +434,8 @@ def main():
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nz", type=int, default=1024)
+    ap.add_argument("--no-witness", dest="witness", action="store_false",
+                    help="the laser-wakefield deck without the witness bunch (config 1)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--precision", choices=("both", "mixed", "fp64"), default="both",
@@ -404,30 +471,15 @@ def main():
     head = runs.get("mixed") or runs["fp64"]
     compare = None
     if len(runs) == 2:
-        # the two runs took the same steps from the same start: the real fields of every slice compared
-        import numpy as np
-        info = runs["fp64"]["sim"].info()
-        ns, nk = info["ns"], info["nk"]
-        A = runs["mixed"]["sim"].array("F").astype(np.float64).reshape(6, ns + 2, nk)[:, :ns]
-        B = runs["fp64"]["sim"].array("F").astype(np.float64).reshape(6, ns + 2, nk)[:, :ns]
-        nb = np.sqrt((B ** 2).sum(axis=(0, 2)))
-        de = np.sqrt(((A - B) ** 2).sum(axis=(0, 2)))
-        err = de / np.maximum(nb, 1e-6 * nb.max())
-        worst = int(err.argmax())
-        big = nb >= 1e-2 * nb.max()          # slices carrying at least 1 % of the largest slice's field
-        compare = {"F_rel_l2_all_slices": float(np.sqrt((de ** 2).sum() / (nb ** 2).sum())),
-                   "F_rel_l2_max_over_slices": float(err.max()), "F_rel_l2_mean": float(err.mean()),
-                   "F_rel_l2_max_over_slices_above_1pct": float(err[big].max()),
-                   "slices_above_1pct": int(big.sum()), "slices": int(len(nb)),
-                   "worst_slice": worst, "worst_slice_norm_over_max": float(nb[worst] / nb.max()),
-                   "speedup_mixed_over_fp64": runs["mixed"]["value"] / runs["fp64"]["value"],
-                   "fp64_value": runs["fp64"]["value"], "fp64_ms_per_step": runs["fp64"]["ms_step"],
-                   "fp64_stream_us": runs["fp64"]["avg_us"]}
+        compare = compare_forms(runs["fp64"], runs["mixed"])
+        compare.update({"speedup_mixed_over_fp64": runs["mixed"]["value"] / runs["fp64"]["value"],
+                        "fp64_value": runs["fp64"]["value"], "fp64_ms_per_step": runs["fp64"]["ms_step"],
+                        "fp64_stream_us": runs["fp64"]["avg_us"]})
     cpu = None
     if not args.no_cpu:
         thr = args.cpu_threads or min(16, os.cpu_count() or 1)
         try:
-            cpu = cpu_baseline(thr)
+            cpu = cpu_baseline(thr, args.witness)
         except Exception as e:  # the baseline is reported, never the target
             print("cpu baseline failed: %s" % e, file=sys.stderr)
     mixed = "mixed" in runs
@@ -436,12 +488,16 @@ def main():
         "value": head["value"], "unit": "pushes/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": head["ms_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32-mixed" if mixed else "f64", "data": "synthetic",
-        "config": {"workload": "LWFA plasma march 256x256x%d, 262144 streaming e- per slice, 2 passes, %s form"
-                               % (args.nz, "FP32-mixed" if mixed else "FP64"),
-                   "N": head["N"], "nstrm": head["nstrm"], "pushes_per_step": head["pushes_per_step"]},
+        "config": {"workload": "LWFA plasma march 256x256x%d, 262144 streaming e- per slice, 2 passes, %s"
+                               "witness 2^20 e- at gamma 200, %s form"
+                               % (args.nz, "" if args.witness else "no ", "FP32-mixed" if mixed else "FP64"),
+                   "N": head["N"], "nstrm": head["nstrm"], "pushes_per_step": head["pushes_per_step"],
+                   "stream_pushes_per_step": head["stream_pushes_per_step"],
+                   "witness_pushes_per_step": head["beam_pushes_per_step"]},
         "roofline": {"kernel": head["kernel"], "bound": "hbm", "achieved": head["achieved"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": head["achieved"] / HBM_PEAK_GBS, "traffic": head["traffic"],
-                     "bytes_per_launch": head["bpl"], "avg_launch_us": head["avg_us"],
+                     "bytes_per_launch": head["bpl"], "pushes_per_launch": head["push_per_launch"],
+                     "design_bytes_per_launch": head["design_bpl"], "avg_launch_us": head["avg_us"],
                      "timed_launches": head["timed_launches"], "timed_one_in": TIME_EVERY},
         "cpu_baseline": cpu,
         "mixed_vs_fp64": compare,
@@ -449,6 +505,38 @@ def main():
         "init_s": head["t_init"],
     }
     print(json.dumps(out))
+
+
+def compare_forms(r64, rmx):
+    """the two runs took the same steps from the same start: the real fields F of every slice (E and B of the slice
+    together, relative L2 against FP64, every slice), and the witness moments of the reference's bunches.txt"""
+    s64, smx = r64["sim"], rmx["sim"]
+    info = s64.info()
+    ns, nk = info["ns"], info["nk"]
+    A = smx.array("F").astype(np.float64).reshape(6, ns + 2, nk)[:, :ns]
+    B = s64.array("F").astype(np.float64).reshape(6, ns + 2, nk)[:, :ns]
+    nb = np.sqrt((B ** 2).sum(axis=(0, 2)))
+    de = np.sqrt(((A - B) ** 2).sum(axis=(0, 2)))
+    del A, B
+    err = de / np.maximum(nb, 1e-300)
+    worst = int(err.argmax())
+    big = nb >= 1e-2 * nb.max()          # slices carrying at least 1 % of the largest slice's field
+    out = {"F_rel_l2_max_over_slices": float(err.max()), "worst_slice": worst,
+           "worst_slice_norm_over_max": float(nb[worst] / nb.max()),
+           "F_rel_l2_p50_p99_over_slices": [float(v) for v in np.percentile(err, [50, 99])],
+           "F_rel_l2_all_slices": float(np.sqrt((de ** 2).sum() / (nb ** 2).sum())),
+           "F_rel_l2_max_over_slices_above_1pct": float(err[big].max()), "slices": int(len(nb))}
+    p64, pmx = s64.array("parts"), smx.array("parts")
+    if len(p64):
+        g0 = s64._deck.c.gamma
+        W64, g64, ex64, ey64 = bunch_moments(p64)
+        Wmx, gmx, exmx, eymx = bunch_moments(pmx)
+        out.update({"witness_gain_fp64": g64 - g0, "witness_gain_mixed": gmx - g0,
+                    "witness_gain_rel_err": abs((gmx - g0) - (g64 - g0)) / abs(g64 - g0),
+                    "witness_emit_nx_fp64": ex64, "witness_emit_nx_rel_err": abs(exmx - ex64) / ex64,
+                    "witness_emit_ny_rel_err": abs(eymx - ey64) / ey64,
+                    "witness_live": [int((p64["alive"] == 1).sum()), int((pmx["alive"] == 1).sum())]})
+    return out
 
 
 if __name__ == "__main__":

@@ -1,5 +1,6 @@
 // C ABI of the engine (include/g3d.h): create a Simulation of the march, run init/step or single member
 // functions, and move its arrays between host and device in the reference's layouts.
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <stdexcept>
@@ -30,11 +31,24 @@ bool lookup(g3d::Sim& s, const std::string& n, void*& dev, size_t& bytes) {
         // half-cell lengths and the two-point current weights of each slice
         {"kx", {s.kxd.data(), bytes_of(s.kxd)}}, {"ky", {s.kyd.data(), bytes_of(s.kyd)}},
         {"hcg", {s.hcg.data(), bytes_of(s.hcg)}}, {"wlo", {s.wlo.data(), bytes_of(s.wlo)}},
-        {"whi", {s.whi.data(), bytes_of(s.whi)}}};
+        {"whi", {s.whi.data(), bytes_of(s.whi)}},
+        // the filters, phase-shift k vectors, slice positions and area factors of the same constructor
+        {"kxs", {s.kxs.data(), bytes_of(s.kxs)}}, {"kys", {s.kys.data(), bytes_of(s.kys)}},
+        {"filt", {s.filt.data(), bytes_of(s.filt)}}, {"filty", {s.filty.data(), bytes_of(s.filty)}},
+        {"zs", {s.zs.data(), bytes_of(s.zs)}}, {"iA", {s.iA.data(), bytes_of(s.iA)}},
+        {"iD", {s.iD.data(), bytes_of(s.iD)}}, {"wsl", {s.wsl.data(), bytes_of(s.wsl)}}};
     auto it = m.find(n);
     if (it == m.end()) return false;
     dev = it->second.first; bytes = it->second.second;
     return true;
+}
+// "ion_statsK": the ion statistics of ion species K after the last step (src/sim_plasma.cpp:438-446: the weight
+// of each charge state 0 .. Zmax of the ions still streaming, then the freed charge), a host array
+const std::vector<double>* ion_stats_of(g3d::Sim& s, const std::string& n) {
+    if (n.rfind("ion_stats", 0) != 0 || n.size() != 10) return nullptr;
+    const int k = n[9] - '0';
+    if (k < 0 || k >= int(s.ion_stats.size())) return nullptr;
+    return &s.ion_stats[size_t(k)];
 }
 // the Simulation of a handle; a null handle is an error, not a fault
 g3d::Sim& S(g3d_sim* h) {
@@ -67,6 +81,7 @@ int g3d_sim_info_get(g3d_sim* sim, g3d_sim_info* o) {
         o->born_cap = s.born_cap; o->s = s.s; o->dx = s.dx; o->dy = s.dy; o->X0 = s.X0; o->Y0 = s.Y0;
         o->beta_b = s.beta_b; o->gm2_b = s.gm2_b; o->trapped_total = s.trapped_total;
         o->restreamed_total = s.restreamed_total; o->lost_total = s.lost_total;
+        o->pushes_stream = s.pushes_stream; o->pushes_beam = s.pushes_beam;
     });
 }
 int g3d_sim_init(g3d_sim* sim) { return g3d::guard([&] { S(sim).init(); g3d::hip_check(hipStreamSynchronize(g3d::stream()), "synchronize"); }); }
@@ -114,6 +129,7 @@ int g3d_sim_array_size(g3d_sim* sim, const char* name, size_t* bytes) {
         void* d;
         if (n == "parts") *bytes = size_t(s.np) * sizeof(g3d_particle);
         else if (n == "strm") *bytes = size_t(s.nstrm) * sizeof(g3d_particle);
+        else if (const std::vector<double>* h = ion_stats_of(s, n)) *bytes = h->size() * sizeof(double);
         else if (!lookup(s, n, d, *bytes)) { g3d::set_error("unknown array " + n); rc = 3; }
     });
     return g ? g : rc;
@@ -128,6 +144,11 @@ int g3d_sim_get(g3d_sim* sim, const char* name, void* host, size_t bytes) {
             const std::vector<g3d_particle> v = n == "parts" ? s.host_particles() : s.host_strm();
             if ((rc = check_size(v.size() * sizeof(g3d_particle), bytes))) return;
             std::memcpy(host, v.data(), bytes);
+            return;
+        }
+        if (const std::vector<double>* h = ion_stats_of(s, n)) {
+            if ((rc = check_size(h->size() * sizeof(double), bytes))) return;
+            std::memcpy(host, h->data(), bytes);
             return;
         }
         void* d; size_t b;
@@ -175,6 +196,49 @@ int g3d_sim_get_joining(g3d_sim* sim, g3d_particle* p, long cap, long* n) {
         std::memcpy(p, j.data(), j.size() * sizeof(g3d_particle));
     });
     return g ? g : rc;
+}
+static FILE* open_or_throw(const char* path, const char* mode) {
+    if (!path) throw std::invalid_argument("missing path");
+    FILE* f = std::fopen(path, mode);
+    if (!f) throw std::runtime_error(std::string("cannot write ") + path);
+    return f;
+}
+int g3d_sim_write_bunches(g3d_sim* sim, const char* path, int header) {
+    return g3d::guard([&] {
+        g3d::Sim& s = S(sim);
+        FILE* f = open_or_throw(path, "a");
+        try { s.write_bunches(f, header != 0); } catch (...) { std::fclose(f); throw; }
+        std::fclose(f);
+    });
+}
+int g3d_sim_write_slices(g3d_sim* sim, const char* path, double len) {
+    return g3d::guard([&] { if (!path) throw std::invalid_argument("missing path"); S(sim).write_slices(path, len); });
+}
+int g3d_sim_diagnostics(g3d_sim* sim, const char* path, int header) {
+    return g3d::guard([&] {
+        g3d::Sim& s = S(sim);
+        FILE* f = open_or_throw(path, "a");
+        try { s.diagnostics(f, header != 0); } catch (...) { std::fclose(f); throw; }
+        std::fclose(f);
+    });
+}
+int g3d_sim_field_energy(g3d_sim* sim, double* out) {
+    return g3d::guard([&] { if (!out) throw std::invalid_argument("missing out"); *out = S(sim).total_field_energy(); });
+}
+int g3d_sim_dump_particles(g3d_sim* sim, const char* path) {
+    return g3d::guard([&] { if (!path) throw std::invalid_argument("missing path"); S(sim).dump_particles(path); });
+}
+int g3d_sim_write_frame(g3d_sim* sim, const char* path, int comps) {
+    return g3d::guard([&] { if (!path) throw std::invalid_argument("missing path"); S(sim).write_frame(path, comps); });
+}
+int g3d_sim_write_axis(g3d_sim* sim, const char* path) {
+    return g3d::guard([&] { if (!path) throw std::invalid_argument("missing path"); S(sim).write_axis(path); });
+}
+int g3d_sim_write_checkpoint(g3d_sim* sim, const char* dir) {
+    return g3d::guard([&] { if (!dir) throw std::invalid_argument("missing dir"); S(sim).write_checkpoint(dir); });
+}
+int g3d_sim_read_checkpoint(g3d_sim* sim, const char* dir, long step) {
+    return g3d::guard([&] { if (!dir) throw std::invalid_argument("missing dir"); S(sim).read_checkpoint(dir, step); });
 }
 int g3d_sim_timing(g3d_sim* sim, int enable) {
     return g3d::guard([&] {

@@ -11,6 +11,7 @@
 
 namespace g3d {
 
+
 const char* const kFamilyNames[Sim::kFam] = {"stream_halfcell", "pass_rows_fwd", "pass_cols", "pass_rows_inv",
                                              "unused", "beam_push", "currents", "sponge_real_fields"};
 enum { F_STREAM = 0, F_SFFT, F_MODES, F_IFFT, F_TOF, F_PUSH, F_CUR, F_SPONGE };
@@ -615,6 +616,7 @@ void Sim::j0_from_jk() {
 // ---------------------------------------------------------------- beam push (src/sim_particles.cpp:6-24)
 void Sim::push(double s0, double s1) {
     if (np <= 0) return;
+    pushes_beam += np;
     PushArgs c;
     c.fv = fview(); c.laser = laser; c.X0 = X0; c.Y0 = Y0; c.zc = cfg.zeta_center; c.dz_push = cfg.dz_push;
     c.push_phase = cfg.push_phase; c.zhand = -1e300; c.zmax = cfg.zeta[N - 1]; c.s1 = s1;
@@ -960,6 +962,7 @@ StreamArgs Sim::stream_args(double s0, double s1) {
     sc.Delta = Delta; sc.s1 = s1; sc.sTrap = s1 + (cfg.trap_advance - 1) * Delta; sc.vz_beam = cfg.vz_beam;
     sc.ions = ion_dev.data(); sc.adk = adk_dev.data(); sc.born = born.cap ? born.view() : Parts{};
     sc.bcnt = counters.data(); sc.bcap = 0; sc.key = 0; sc.n = 0;
+    sc.npush = nullptr; sc.npush_cap = 0;
     return sc;
 }
 
@@ -1059,6 +1062,12 @@ void Sim::march_plasma(double s0, double s1) {
         if (tiles.ntile && slab_f.size() < size_t(tiles.ntile) * 5 * tiles.WX * tiles.WY)
             slab_f.resize(size_t(tiles.ntile) * 5 * tiles.WX * tiles.WY);
     }
+    // the stream launches count their pushes per workgroup: at most every tile, every further chunk and the
+    // untiled rest of the list in workgroups of at least 64 lanes
+    const long npc = tg.ntile + tg.max_extra + (long(strmA.cap) + 63) / 64 + 1;
+    if (npush.size() < size_t(npc)) npush.resize(size_t(npc));
+    else npush.zero();
+    sc.npush = npush.data(); sc.npush_cap = long(npush.size());
     cxf* Xf = reinterpret_cast<cxf*>(sp3.data());
     cxf* Tf = reinterpret_cast<cxf*>(tp3.data());
     const int rpw = rows_per_wg(cfg.ny);
@@ -1144,6 +1153,11 @@ void Sim::march_plasma(double s0, double s1) {
     }
     (void)born_base;
     nstrm = nlist;
+    {
+        std::vector<unsigned long long> cnt(npush.size());
+        npush.to_host(cnt.data(), 0, cnt.size());
+        for (unsigned long long c : cnt) pushes_stream += long(c);
+    }
     // trapped particles (alive 5): fast ones join the beam list, slow ones stream again in the next step
     std::vector<g3d_particle> sv;
     bool need_all = false;

@@ -4,6 +4,7 @@
 // stream, with no host synchronisation inside the half-cell loop.
 #pragma once
 #include <array>
+#include <cstdio>
 #include <deque>
 #include <string>
 #include <vector>
@@ -67,7 +68,7 @@ public:
     ~Sim();
     void init();
     void step();
-    bool finished() const { return s >= cfg.z_end - 1e-9; }
+    bool finished() const { return s >= cfg.z_end - 1e-9 || stop_; }
 
     // the reference's member functions of the five drop-in files
     void push(double s0, double s1);
@@ -93,6 +94,21 @@ public:
     void shift_window(int nxs, int nys);
     void j0_from_jk();
 
+    // output and checkpoints in the reference's formats (g3d_output.cpp)
+    std::vector<std::string> beam_names() const;
+    int beam_index(int sp) const;
+    void write_bunches(FILE* f, bool header);
+    void write_slices(const std::string& file, double slice_length);
+    void diagnostics(FILE* f, bool header);
+    double total_field_energy();
+    void dump_particles(const std::string& file);
+    void write_frame(const std::string& file, int comps = 255);
+    void write_axis(const std::string& file);
+    std::string checkpoint_file(const std::string& dir, long step) const;
+    void write_checkpoint(const std::string& dir);
+    void read_checkpoint(const std::string& dir, long step);
+    bool stop_ = false;               // the laser has left the bunch by stop_fwhm (diagnostics)
+
     std::vector<g3d_particle> host_particles();
     void set_particles(const std::vector<g3d_particle>& v);
     std::vector<g3d_particle> host_strm();
@@ -104,6 +120,10 @@ public:
     bool yf = true, starting_ = false, comoving_c_ = false;
     long istep = 0, np = 0, nstrm = 0, born_cap = 0, fresh_last = 0;
     long trapped_total = 0, restreamed_total = 0, lost_total = 0;
+    // pushes done since the simulation was made: streaming particles through a half-cell in one pass, and beam
+    // particles from s0 to s1 (the particles of the list at the start of push)
+    long pushes_stream = 0, pushes_beam = 0;
+    DevBuf<unsigned long long> npush;     // per-workgroup push counts of the stream launches of a step
     PlaneWave laser{};
     std::vector<g3d_particle> joining_;
     std::vector<std::vector<double>> ion_stats;

@@ -179,7 +179,6 @@ void fft_run_f(const FftPlan& p, cxf* data, int count, bool inverse) {
 }  // namespace g3d
 
 // ------------------------------------------------------------------ C ABI: runtime, memory, FFT
-struct g3d_fft { g3d::FftPlan p; };
 
 extern "C" {
 int g3d_init(int device) {

@@ -30,3 +30,6 @@ void fft_rows(const FftPlan& p, cx* data, long nrows, bool inverse, double scale
 void fft_cols(const FftPlan& p, cx* data, int count, bool inverse, double scale);
 
 }  // namespace g3d
+
+// the plan behind the C ABI handle g3d_fft (include/g3d.h)
+struct g3d_fft { g3d::FftPlan p; };

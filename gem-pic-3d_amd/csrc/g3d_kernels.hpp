@@ -35,6 +35,10 @@ struct StreamArgs {
     int* bcnt;
     int bcap;
     uint64_t key;
+    // pushes of the launch: workgroup b adds the number of particles it pushed through the half-cell to npush[b]
+    // (npush_cap slots; null: not counted)
+    unsigned long long* npush;
+    long npush_cap;
 };
 
 // Transverse tiles of the streaming list: the fresh plasma is loaded tile by tile (TX x TY cells), particles
@@ -105,6 +109,24 @@ void launch_timed_kernel(const LaunchTimer& lt, void (*k)(KArgs...), dim3 grid, 
     else hipLaunchKernelGGL(k, grid, block, lds, stream(), args...);
 }
 constexpr int kStreamBlockMax = 512;
+// a list entry is pushed through half-cell n (from slice n + 1 towards slice n) when it streams (alive 1) or when it
+// waits (alive 6) above slice n (src/sim_stream.hpp:107-111)
+__device__ __forceinline__ bool pushed_through(int alive, double zeta, double zn) {
+    return alive == 1 || (alive == 6 && zeta > zn);
+}
+// the workgroup's pushed particles (cnt per lane) added to npush[blockIdx.x]; every lane of the workgroup calls it
+// once (it has a barrier), at most kStreamBlockMax lanes
+__device__ __forceinline__ void add_pushes(unsigned long long* npush, unsigned cnt) {
+    __shared__ unsigned part[kStreamBlockMax / 64];
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_down(cnt, o, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (unsigned w = 0; w < (blockDim.x + 63) / 64; ++w) t += part[w];
+        npush[blockIdx.x] += t;
+    }
+}
 void stream_halfcell(const StreamArgs& a, Parts src, Parts dst, long nstrm, const TileGeom& tg,
                      const LaunchTimer& lt = {});
 // append the freed electrons of the last pass (count on the device) to both buffers; nstrm_dev += count

@@ -1,13 +1,15 @@
-// The FP32-mixed form of the plasma pass (Fourier y, electron streams): the deposit, the transforms and the
-// per-mode implicit update run in FP32 on the vector ALUs.  FP64 is kept for
-//   - the particle coordinates x, y, zeta, z and their accumulation along s (the increments are FP32),
+// The FP32-mixed form of the plasma pass (Fourier y, electron streams): the deposit into the current planes, the
+// transforms and the per-mode implicit update run in FP32 on the vector ALUs.  FP64 is kept for
+//   - the push of the streaming electrons (gather, Boris rotations, positions and momenta and their accumulation
+//     along s; see stream_one_mixed) and the sums of the tile's current window,
 //   - the trapezoidal s-integration of R (R and its explicit part R + D (F - J)),
 //   - the running currents jk, j0, js0 and the prediction bookkeeping (the charge totals),
 // and the spectra stay FP64 in HBM, so the FP64 and mixed passes share every array outside the pass.
 //
-//   k_stream_mixed      one half-cell pass of the streaming electrons (src/sim_stream.hpp:104-171): gather and
-//                       deposit through the tile's LDS windows in FP32, FP32 Boris rotations
-//   k_rows_fwd_mixed    jsr + slabs (float) -> x transforms in FP32 -> pair unpacked per row
+//   k_stream_mixed      one half-cell pass of the streaming electrons (src/sim_stream.hpp:104-171): gather from
+//                       the tile's LDS field window, FP64 Boris rotations, deposit into the tile's current window,
+//                       added to the FP32 current planes
+//   k_rows_fwd_mixed    current planes (float) -> x transforms in FP32 -> pair unpacked per row
 //   k_cols_mixed        y transforms of three planes in FP32 (forward or inverse)
 //   k_modes_mixed       filters, bookkeeping (FP64), march with the FP32 implicit solve, packed planes
 //   k_rows_inv_mixed    x inverse transforms in FP32 and E, B of the slice into F
@@ -24,60 +26,79 @@ constexpr int kT = 256;
 // contraction off everywhere else)
 #pragma clang fp contract(fast)
 
-// FP32 reciprocal and square root of the hardware (v_rcp_f32, v_sqrt_f32: 1 ulp), for the FP32 physics
-__device__ __forceinline__ float rcp_f(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ __forceinline__ float sqrt_f(float x) { return __builtin_amdgcn_sqrtf(x); }
+// The push of the streaming electrons runs in FP64 arithmetic in this form too: FP32 rounding in the gather and
+// the Boris step perturbs every trajectory by about 1e-7 per half-cell, and behind the wake, where electron orbits
+// cross, the march amplifies such perturbations by a few hundred (measured at 256 x 256 x 1024: a per-slice field
+// error of 1.7e-4 with an FP32 push against 6.5e-5 with an FP64 push, both with the FP32 field pass).  Without
+// packed FP32 the FP64 and FP32 vector rates of gfx950 are equal; what FP64 adds is the conversions of the float
+// fields (done once per window node) and its division and square root, which this form takes from the hardware
+// estimates (v_rcp_f64, v_rsq_f64) refined by two Newton steps each (the relative error of the estimate is squared
+// by each step) instead of the IEEE sequences of the plain FP64 form.
+__device__ __forceinline__ double rcp_d(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+__device__ __forceinline__ double rsq_d(double x) {       // 1 / sqrt(x), x > 0
+    double r = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    double e = fma(-h * r, r, 0.5);
+    r = fma(r, e, r);
+    e = fma(-h * r, r, 0.5);
+    return fma(r, e, r);
+}
 
-// 1/dx, 1/dy and the first inverse area factor, once per lane
-struct InvF {
+// 1/dx and 1/dy, once per lane
+struct InvD {
     double ix, iy;
-    float ia0;
 };
 
-// TSC weights in FP32 on a periodic uniform axis: the cell from the FP64 index coordinate x / d (x * (1/d): the
-// cell may differ from the FP64 pass where x / d lies within an ulp of a half-integer), the fraction in FP32.
-// Returns the middle node's index, not wrapped (the window path wraps only its offset; tsc_nodes wraps all three).
-__device__ __forceinline__ int tsc_f(double x, double inv, float w[3]) {
+// TSC weights on a periodic uniform axis from the index coordinate x / d (taken as x * (1/d): the cell may differ
+// from the plain FP64 form where x / d lies within an ulp of a half-integer).  Returns the middle node's index, not
+// wrapped (the window path wraps only its offset; tsc_nodes wraps all three).
+__device__ __forceinline__ int tsc_d(double x, double inv, double w[3]) {
     const double xi = x * inv;
     const int i0 = int(floor(xi + 0.5));
-    const float f = float(xi - i0);
-    const float a = 0.5f - f, b = 0.5f + f;
-    w[0] = 0.5f * a * a;
-    w[1] = 0.75f - f * f;
-    w[2] = 0.5f * b * b;
+    const double f = xi - i0;
+    const double a = 0.5 - f, b = 0.5 + f;
+    w[0] = 0.5 * a * a;
+    w[1] = 0.75 - f * f;
+    w[2] = 0.5 * b * b;
     return i0;
 }
 __device__ __forceinline__ void tsc_nodes(int i0, int n, int idx[3]) {
     idx[0] = wrap_index(i0 - 1, n); idx[1] = wrap_index(i0, n); idx[2] = wrap_index(i0 + 1, n);
 }
 
-__device__ __forceinline__ void boris_f(float& ux, float& uy, float& uz, const float E[3], const float B[3], float qm,
-                                        float dt) {
-    const float qh = 0.5f * qm * dt;
-    const float ax = ux + qh * E[0], ay = uy + qh * E[1], az = uz + qh * E[2];
-    const float qg = qh * rcp_f(sqrt_f(1.0f + ax * ax + ay * ay + az * az));
-    const float tx = qg * B[0], ty = qg * B[1], tz = qg * B[2];
-    const float t2 = tx * tx + ty * ty + tz * tz;
-    const float px = ax + (ay * tz - az * ty), py = ay + (az * tx - ax * tz), pz = az + (ax * ty - ay * tx);
-    const float sf = 2.0f * rcp_f(1.0f + t2);
-    const float bx = ax + sf * (py * tz - pz * ty), by = ay + sf * (pz * tx - px * tz), bz = az + sf * (px * ty - py * tx);
+// the relativistic Boris step of src/sim_stream.hpp:8-18 with the fast reciprocal and inverse square root
+__device__ __forceinline__ void boris_d(double& ux, double& uy, double& uz, const double E[3], const double B[3],
+                                        double qm, double dt) {
+    const double qh = 0.5 * qm * dt;
+    const double ax = ux + qh * E[0], ay = uy + qh * E[1], az = uz + qh * E[2];
+    const double qg = qh * rsq_d(1.0 + ax * ax + ay * ay + az * az);
+    const double tx = qg * B[0], ty = qg * B[1], tz = qg * B[2];
+    const double t2 = tx * tx + ty * ty + tz * tz;
+    const double px = ax + (ay * tz - az * ty), py = ay + (az * tx - ax * tz), pz = az + (ax * ty - ay * tx);
+    const double sf = 2.0 * rcp_d(1.0 + t2);
+    const double bx = ax + sf * (py * tz - pz * ty), by = ay + sf * (pz * tx - px * tz), bz = az + sf * (px * ty - py * tx);
     ux = bx + qh * E[0]; uy = by + qh * E[1]; uz = bz + qh * E[2];
 }
 
-// LDS windows of a tile, node-major so that one node is one address: the field window first (12 floats per node:
-// E, B at slice fl then fl + 1, three 16-byte reads) and the current window after it (5 doubles per node: j_x, j_y,
-// j_z, rho-, rho+; FP64 sums of the FP32 contributions, since the FP32 LDS atomic add serialises on the
-// same-address conflicts of a tile's particles and the FP64 one does not)
+// LDS windows of a tile, node-major so that one node is one address: the field window first (12 doubles per node:
+// E, B at slice fl then fl + 1, the float fields converted once when the window is loaded; six 16-byte reads per
+// stencil node) and the current window after it (5 doubles per node: j_x, j_y, j_z, rho-, rho+)
 constexpr int kFwNode = 12, kDepNode = 5;
-struct WinF {
+struct WinD {
     double* lds;              // kDepNode doubles per node
-    const float4* fld;        // 3 float4 per node, or null
+    const double2* fld;       // 6 double2 per node, or null
     int x0, y0, WX, WY, fl;
 };
 
 // E, B at a point: the tile's field window when the stencil and the slice pair are in it, else F
-__device__ __forceinline__ void gather_f(const FieldView& c, const InvF& iv, const WinF* w, double x, double y,
-                                         double zeta, float E[3], float B[3]) {
+__device__ __forceinline__ void gather_d(const FieldView& c, const InvD& iv, const WinD* w, double x, double y,
+                                         double zeta, double E[3], double B[3]) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) { E[k] = 0; B[k] = 0; }
     if (!c.self) return;
@@ -86,30 +107,29 @@ __device__ __forceinline__ void gather_f(const FieldView& c, const InvF& iv, con
     if (n == c.a - 1 && fz > 1 - 1e-9) { n = c.a; fz = 0; }
     const int fl = n - c.a;
     if (fl < 0 || fl > c.ns) return;
-    float wx[3], wn[3];
-    const int i0 = tsc_f(x, iv.ix, wx), j0 = tsc_f(y, iv.iy, wn);
-    const float f1 = float(fz), f0 = 1.0f - f1;
-    float wt[9];
+    double wx[3], wn[3];
+    const int i0 = tsc_d(x, iv.ix, wx), j0 = tsc_d(y, iv.iy, wn);
+    const double f1 = fz, f0 = 1.0 - fz;
+    double wt[9];
 #pragma unroll
     for (int j = 0; j < 3; ++j)
 #pragma unroll
         for (int i = 0; i < 3; ++i) wt[3 * j + i] = wx[i] * wn[j];
-    float v[6];
     if (w && w->fld && fl == w->fl) {
         const int u0 = wrap_index(i0 - 1 - w->x0, c.nx), v0 = wrap_index(j0 - 1 - w->y0, c.ny);
         if (u0 + 2 < w->WX && v0 + 2 < w->WY) {
-            float s0[6] = {0, 0, 0, 0, 0, 0}, s1[6] = {0, 0, 0, 0, 0, 0};
+            double s0[6] = {0, 0, 0, 0, 0, 0}, s1[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
             for (int j = 0; j < 3; ++j)
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
-                    const float4* q = w->fld + 3 * ((v0 + j) * w->WX + u0 + i);
-                    const float4 a = q[0], b = q[1], d = q[2];
-                    const float t = wt[3 * j + i];
-                    s0[0] += t * a.x; s0[1] += t * a.y; s0[2] += t * a.z; s0[3] += t * a.w;
-                    s0[4] += t * b.x; s0[5] += t * b.y;
-                    s1[0] += t * b.z; s1[1] += t * b.w;
-                    s1[2] += t * d.x; s1[3] += t * d.y; s1[4] += t * d.z; s1[5] += t * d.w;
+                    const double2* q = w->fld + 6 * ((v0 + j) * w->WX + u0 + i);
+                    const double2 a = q[0], b = q[1], d = q[2], e = q[3], g = q[4], h = q[5];
+                    const double t = wt[3 * j + i];
+                    s0[0] += t * a.x; s0[1] += t * a.y; s0[2] += t * b.x; s0[3] += t * b.y;
+                    s0[4] += t * d.x; s0[5] += t * d.y;
+                    s1[0] += t * e.x; s1[1] += t * e.y; s1[2] += t * g.x; s1[3] += t * g.y;
+                    s1[4] += t * h.x; s1[5] += t * h.y;
                 }
             E[0] = f0 * s0[0] + f1 * s1[0]; E[1] = f0 * s0[1] + f1 * s1[1]; E[2] = f0 * s0[2] + f1 * s1[2];
             B[0] = f0 * s0[3] + f1 * s1[3]; B[1] = f0 * s0[4] + f1 * s1[4]; B[2] = f0 * s0[5] + f1 * s1[5];
@@ -121,53 +141,50 @@ __device__ __forceinline__ void gather_f(const FieldView& c, const InvF& iv, con
     tsc_nodes(j0, c.ny, jn);
     const long cs = long(c.ns + 2) * c.nk;
     const float* p0 = c.F + long(fl) * c.nk;
+    double v[6];
 #pragma unroll
     for (int comp = 0; comp < 6; ++comp) {
         const float* q0 = p0 + comp * cs;
         const float* q1 = q0 + c.nk;
-        float s = 0;
+        double sum = 0;
 #pragma unroll
         for (int j = 0; j < 3; ++j)
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
                 const long k = long(jn[j]) * c.nx + ix[i];
-                s += wt[3 * j + i] * (f0 * q0[k] + f1 * q1[k]);
+                sum += wt[3 * j + i] * (f0 * double(q0[k]) + f1 * double(q1[k]));
             }
-        v[comp] = s;
+        v[comp] = sum;
     }
     E[0] = v[0]; E[1] = v[1]; E[2] = v[2]; B[0] = v[3]; B[1] = v[4]; B[2] = v[5];
 }
 
-// TSC deposit of f v into the FP32 current planes (window or jsr), as the FP64 deposit; the charge density planes
-// (rho-, rho+) only when rho (their one reader, the density lines, takes the last pass's)
-__device__ __forceinline__ void sdeposit_f(const StreamArgs& c, const InvF& iv, float* J, const WinF* win, double x,
-                                           double y, float f, float vx, float vy, float vz, bool rho) {
+// TSC deposit of f v into the current window (FP64 sums) or, outside it, into the FP32 current planes with float
+// atomics (src/sim_stream.hpp:33-48); f includes the inverse area factor.  The charge density planes (rho-, rho+)
+// only when rho (their one reader, the density lines, takes the last pass's)
+__device__ __forceinline__ void sdeposit_d(const StreamArgs& c, const InvD& iv, float* J, const WinD* win, double x,
+                                           double y, double f, double vx, double vy, double vz, bool rho) {
     const FieldView& P = c.fv;
-    float wx[3], wn[3];
-    const int i0 = tsc_f(x, iv.ix, wx), j0 = tsc_f(y, iv.iy, wn);
+    double wx[3], wn[3];
+    const int i0 = tsc_d(x, iv.ix, wx), j0 = tsc_d(y, iv.iy, wn);
     const long nk = P.nk;
     const int cq = f < 0 ? 3 : 4;
     const bool mv = vx != 0 || vy != 0 || vz != 0;
-    const float fi = f * iv.ia0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) wx[k] *= f;
     if (win) {
         const int u0 = wrap_index(i0 - 1 - win->x0, P.nx), v0 = wrap_index(j0 - 1 - win->y0, P.ny);
         if (u0 + 2 < win->WX && v0 + 2 < win->WY) {
-            // the products in FP64 from the FP32 weights and velocity: the LDS sums are FP64, and an FP64 multiply
-            // issues as fast as a conversion
-            double dx[3], dy[3];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) { dx[k] = double(fi * wx[k]); dy[k] = double(wn[k]); }
-            const double dvx = vx, dvy = vy, dvz = vz;
 #pragma unroll
             for (int v = 0; v < 3; ++v)
 #pragma unroll
                 for (int u = 0; u < 3; ++u) {
                     double* L = win->lds + kDepNode * ((v0 + v) * win->WX + (u0 + u));
-                    const double ww = dx[u] * dy[v];
+                    const double ww = wx[u] * wn[v];
                     if (mv) {
-                        atomicAdd(L, ww * dvx);
-                        atomicAdd(L + 1, ww * dvy);
-                        atomicAdd(L + 2, ww * dvz);
+                        atomicAdd(L, ww * vx);
+                        atomicAdd(L + 1, ww * vy);
+                        atomicAdd(L + 2, ww * vz);
                     }
                     if (rho) atomicAdd(L + cq, ww);
                 }
@@ -182,13 +199,13 @@ __device__ __forceinline__ void sdeposit_f(const StreamArgs& c, const InvF& iv, 
 #pragma unroll
         for (int u = 0; u < 3; ++u) {
             const long k = long(jn[v]) * P.nx + ix[u];
-            const float ww = fi * wx[u] * wn[v];
+            const double ww = wx[u] * wn[v];
             if (mv) {
-                atomicAdd(&J[k], ww * vx);
-                atomicAdd(&J[nk + k], ww * vy);
-                atomicAdd(&J[2 * nk + k], ww * vz);
+                atomicAdd(&J[k], float(ww * vx));
+                atomicAdd(&J[nk + k], float(ww * vy));
+                atomicAdd(&J[2 * nk + k], float(ww * vz));
             }
-            if (rho) atomicAdd(&J[cq * nk + k], ww);
+            if (rho) atomicAdd(&J[cq * nk + k], float(ww));
         }
 }
 
@@ -201,90 +218,87 @@ __device__ __forceinline__ MState load_mstate(const Parts& S, long i) {
     return MState{S.x[i], S.y[i], S.zeta[i], S.z[i], S.px[i], S.py[i], S.pz[i], S.q[i], S.qm[i], S.w[i], S.alive[i]};
 }
 
-// the values of a launch that every electron uses: the half-cell's ends and length, the FP32 factor
-// 1 / (Delta h dx) of the deposit, and whether the density planes are deposited
-struct HalfCellF {
-    double zn, zn1, h;
-    float iq;
+// the values of a launch that every electron uses: the half-cell's ends and length, the deposit factor
+// ia / (Delta h dx) of the streaming current and ia / (h dx) of a trapped electron's beam current, and whether the
+// density planes are deposited
+struct HalfCellD {
+    double zn, zn1, h, iq, iqb;
     bool rho;
 };
-__device__ __forceinline__ HalfCellF half_cell(const StreamArgs& c, int rho) {
+__device__ __forceinline__ HalfCellD half_cell(const StreamArgs& c, int rho) {
     const double zn = c.zs[c.n], zn1 = c.zs[c.n + 1], h = zn1 - zn;
-    return HalfCellF{zn, zn1, h, rcp_f(float(c.Delta * h * c.fv.dx)), rho != 0};
+    return HalfCellD{zn, zn1, h, c.ia[0] / (c.Delta * h * c.fv.dx), c.ia[0] / (h * c.fv.dx), rho != 0};
 }
 
-// one electron through half-cell n (src/sim_stream.hpp:104-171), the physics of the FP64 pass in FP32; s: its state
-// at the start (loaded by the caller), written to D[i].  The TSC positions are not wrapped into the box first: the
-// node indices are (wrap_index), and the FP32 fraction does not see the difference.
+// one electron through half-cell n (src/sim_stream.hpp:104-171); s: its state at the start (loaded by the caller),
+// written to D[i].  The TSC positions are not wrapped into the box first: the node indices are (wrap_index).
 // LASER: the deck has the analytic laser (its evaluation would otherwise set the register budget of every launch)
 template <bool LASER>
-__device__ __forceinline__ void stream_one_mixed(const StreamArgs& c, const HalfCellF& hc, const InvF& iv, float* J,
-                                                 const WinF* win, const MState& s, Parts D, long i) {
+__device__ __forceinline__ void stream_one_mixed(const StreamArgs& c, const HalfCellD& hc, const InvD& iv, float* J,
+                                                 const WinD* win, const MState& s, Parts D, long i) {
     const double zn = hc.zn, zn1 = hc.zn1, h = hc.h;
     double x = s.x, y = s.y, zeta = s.zeta, z = s.z;
-    const double pxd = s.px, pyd = s.py, pzd = s.pz;
-    const double q = s.q, qmd = s.qm, w = s.w;
+    const double px = s.px, py = s.py, pz = s.pz;
+    const double q = s.q, qm = s.qm, w = s.w;
     int alive = s.alive;
     auto store = [&](double ux, double uy, double uz) {
         D.x[i] = x; D.y[i] = y; D.zeta[i] = zeta; D.px[i] = ux; D.py[i] = uy; D.pz[i] = uz;
-        D.z[i] = z; D.q[i] = q; D.qm[i] = qmd; D.alive[i] = alive;
+        D.z[i] = z; D.q[i] = q; D.qm[i] = qm; D.alive[i] = alive;
     };
     if (alive == 6) {
         if (zeta > zn) alive = 1;
-        else { store(pxd, pyd, pzd); return; }
+        else { store(px, py, pz); return; }
     }
-    if (alive != 1) { store(pxd, pyd, pzd); return; }
+    if (alive != 1) { store(px, py, pz); return; }
     const FieldView& P = c.fv;
     const double hl = zeta >= zn1 - 1e-9 * h ? h : (zeta - zn > 0 ? zeta - zn : 0.0);
-    const float hf = float(hl), qm = float(qmd);
-    const float px = float(pxd), py = float(pyd), pz = float(pzd);
-    const float g0 = sqrt_f(1 + px * px + py * py + pz * pz), ig0 = rcp_f(g0);
-    const float omv0 = (1 + px * px + py * py) * rcp_f(g0 * (g0 + pz));
-    const float dt0 = hf * rcp_f(omv0), hs = 0.5f * dt0 * ig0;
-    const double xm = x + double(hs * px), ym = y + double(hs * py);
-    const double zm = z + double(hs * pz), zetam = zeta - 0.5 * hl;
-    float E[3], B[3];
-    gather_f(P, iv, win, xm, ym, zetam, E, B);
+    const double a0 = 1 + px * px + py * py, s0 = a0 + pz * pz;
+    const double ig0 = rsq_d(s0), g0 = s0 * ig0;
+    const double omv0 = a0 * rcp_d(g0 * (g0 + pz));
+    const double dt0 = hl * rcp_d(omv0), hs = 0.5 * dt0 * ig0;
+    const double xm = x + hs * px, ym = y + hs * py, zm = z + hs * pz, zetam = zeta - 0.5 * hl;
+    double E[3], B[3];
+    gather_d(P, iv, win, xm, ym, zetam, E, B);
     if (LASER && c.laser.on) {
         double Ee[3], Be[3];
         c.laser.field(xm + c.X0, zm, zm - (zetam - c.zc), Ee, Be);
-        for (int k = 0; k < 3; ++k) { E[k] += float(Ee[k]); B[k] += float(Be[k]); }
+        for (int k = 0; k < 3; ++k) { E[k] += Ee[k]; B[k] += Be[k]; }
     }
-    float ux = px, uy = py, uz = pz;
-    boris_f(ux, uy, uz, E, B, qm, dt0);
-    float g1 = sqrt_f(1 + ux * ux + uy * uy + uz * uz);
-    float dt = 2 * hf * rcp_f(omv0 + (1 + ux * ux + uy * uy) * rcp_f(g1 * (g1 + uz)));
+    double ux = px, uy = py, uz = pz;
+    boris_d(ux, uy, uz, E, B, qm, dt0);
+    double a1 = 1 + ux * ux + uy * uy, s1 = a1 + uz * uz;
+    double ig1 = rsq_d(s1), g1 = s1 * ig1;
+    double dt = 2 * hl * rcp_d(omv0 + a1 * rcp_d(g1 * (g1 + uz)));
     ux = px; uy = py; uz = pz;
-    boris_f(ux, uy, uz, E, B, qm, dt);
-    g1 = sqrt_f(1 + ux * ux + uy * uy + uz * uz);
-    float ig1 = rcp_f(g1);
-    float vx = 0.5f * (px * ig0 + ux * ig1), vy = 0.5f * (py * ig0 + uy * ig1), vz = 0.5f * (pz * ig0 + uz * ig1);
+    boris_d(ux, uy, uz, E, B, qm, dt);
+    ig1 = rsq_d(1 + ux * ux + uy * uy + uz * uz);
+    double vx = 0.5 * (px * ig0 + ux * ig1), vy = 0.5 * (py * ig0 + uy * ig1), vz = 0.5 * (pz * ig0 + uz * ig1);
     double zeta_end = zn;
     bool trapped = false;
-    if (z + double(vz * dt) >= c.sTrap) {
-        const float vzc = vz > 1e-3f ? vz : 1e-3f;
-        const float dtp = float((c.sTrap - z) / vzc) > 0 ? float((c.sTrap - z) / vzc) : 0.0f;
+    if (z + vz * dt >= c.sTrap) {
+        const double vzc = vz > 1e-3 ? vz : 1e-3;
+        const double dtp = (c.sTrap - z) * rcp_d(vzc) > 0 ? (c.sTrap - z) * rcp_d(vzc) : 0.0;
         ux = px; uy = py; uz = pz;
-        boris_f(ux, uy, uz, E, B, qm, dtp);
-        g1 = sqrt_f(1 + ux * ux + uy * uy + uz * uz);
-        ig1 = rcp_f(g1);
-        vx = 0.5f * (px * ig0 + ux * ig1); vy = 0.5f * (py * ig0 + uy * ig1); vz = 0.5f * (pz * ig0 + uz * ig1);
+        boris_d(ux, uy, uz, E, B, qm, dtp);
+        ig1 = rsq_d(1 + ux * ux + uy * uy + uz * uz);
+        vx = 0.5 * (px * ig0 + ux * ig1); vy = 0.5 * (py * ig0 + uy * ig1); vz = 0.5 * (pz * ig0 + uz * ig1);
         dt = dtp;
-        zeta_end = zeta - double((1 - vz) * dtp);
+        zeta_end = zeta - (1 - vz) * dtp;
         trapped = true;
     }
     const double xo = x, yo = y;
-    x += double(vx * dt); y += double(vy * dt);
-    z = trapped ? c.s1 : z + double(vz * dt);
+    x += vx * dt; y += vy * dt;
+    z = trapped ? c.s1 : z + vz * dt;
     zeta = zeta_end;
-    // q w dt / (Delta h dx), the cell factors in FP32
-    const float qw = float(q * w) * hc.iq;
-    const float f = qw * dt;
-    if (f != 0) sdeposit_f(c, iv, J, win, 0.5 * (xo + x), 0.5 * (yo + y), f, vx, vy, vz, hc.rho);
+    // the streaming current q w dt / (Delta h dx) of the half-cell
+    const double qw = q * w;
+    const double f = qw * hc.iq * dt;
+    if (f != 0) sdeposit_d(c, iv, J, win, 0.5 * (xo + x), 0.5 * (yo + y), f, vx, vy, vz, hc.rho);
     if (trapped) {
         alive = 5;
-        const float vze = uz * ig1;
-        if (vze >= float(c.vz_beam)) sdeposit_f(c, iv, J, win, x, y, qw * rcp_f(vze), ux * ig1, uy * ig1, vze, hc.rho);
+        const double vze = uz * ig1;
+        // joins the beam list: its beam current q w / (h dx v_z) of level s1 from now on
+        if (vze >= c.vz_beam) sdeposit_d(c, iv, J, win, x, y, qw * hc.iqb * rcp_d(vze), ux * ig1, uy * ig1, vze, hc.rho);
     }
     store(ux, uy, uz);
 }
@@ -293,7 +307,7 @@ __device__ __forceinline__ void stream_one_mixed(const StreamArgs& c, const Half
 template <int WPE, bool LASER>
 __global__ void __launch_bounds__(kStreamBlockMax) __attribute__((amdgpu_waves_per_eu(WPE ? WPE : 1)))
 k_stream_mixed(StreamArgs c, float* J, float* slab, Parts S, Parts D, long n, TileGeom tg, int rho) {
-    extern __shared__ float4 winf4[];
+    extern __shared__ double2 wind2[];
     const long b = blockIdx.x;
     const long nt = tg.ntile + tg.max_extra;
     if (b < nt) {
@@ -319,7 +333,7 @@ k_stream_mixed(StreamArgs c, float* J, float* slab, Parts S, Parts D, long n, Ti
         const int x0 = tx * tg.TX - 1 - tg.M, y0 = ty * tg.TY - 1 - tg.M;
         const FieldView& F = c.fv;
         const int fl = c.n - F.a;
-        double* dep = reinterpret_cast<double*>(winf4 + 3 * pl);
+        double* dep = reinterpret_cast<double*>(wind2 + 6 * pl);
         for (int r = threadIdx.x; r < wn; r += blockDim.x) dep[r] = 0;
         if (F.self)      // one node per lane: its 12 values from the 6 component planes of slices fl, fl + 1
             for (int r = threadIdx.x; r < pl; r += blockDim.x) {
@@ -327,21 +341,22 @@ k_stream_mixed(StreamArgs c, float* J, float* slab, Parts S, Parts D, long n, Ti
                 const int gx = wrap_index(x0 + u, F.nx), gy = wrap_index(y0 + v, F.ny);
                 const float* src = F.F + long(fl) * F.nk + long(gy) * F.nx + gx;
                 const long cs = long(F.ns + 2) * F.nk;
-                float q[kFwNode];
+                double q[kFwNode];
 #pragma unroll
                 for (int comp = 0; comp < 6; ++comp) {
                     q[comp] = src[comp * cs];
                     q[6 + comp] = src[comp * cs + F.nk];
                 }
-                winf4[3 * r] = make_float4(q[0], q[1], q[2], q[3]);
-                winf4[3 * r + 1] = make_float4(q[4], q[5], q[6], q[7]);
-                winf4[3 * r + 2] = make_float4(q[8], q[9], q[10], q[11]);
+#pragma unroll
+                for (int k = 0; k < 6; ++k) wind2[6 * r + k] = make_double2(q[2 * k], q[2 * k + 1]);
             }
         __syncthreads();
-        WinF w{dep, F.self ? winf4 : nullptr, x0, y0, tg.WX, tg.WY, fl};
-        const InvF iv{1.0 / F.dx, 1.0 / F.dy, float(c.ia[0])};
-        const HalfCellF hc = half_cell(c, rho);
+        WinD w{dep, F.self ? wind2 : nullptr, x0, y0, tg.WX, tg.WY, fl};
+        const InvD iv{1.0 / F.dx, 1.0 / F.dy};
+        const HalfCellD hc = half_cell(c, rho);
+        unsigned cnt = 0;
         while (k < hi) {
+            cnt += pushed_through(st.alive, st.zeta, hc.zn);
             stream_one_mixed<LASER>(c, hc, iv, J, &w, st, D, i);
             k += blockDim.x;
             if (k < hi) {
@@ -361,12 +376,18 @@ k_stream_mixed(StreamArgs c, float* J, float* slab, Parts S, Parts D, long n, Ti
                 const long g = long(wrap_index(y0 + vv, F.ny)) * F.nx + wrap_index(x0 + u, F.nx);
                 atomicAdd(&J[cc * F.nk + g], float(v));
             }
+        if (c.npush) add_pushes(c.npush, cnt);
         return;
     }
     const long i = tg.n_tiled + (b - nt) * blockDim.x + threadIdx.x;
-    if (i < n)
-        stream_one_mixed<LASER>(c, half_cell(c, rho), InvF{1.0 / c.fv.dx, 1.0 / c.fv.dy, float(c.ia[0])}, J, nullptr,
-                                load_mstate(S, i), D, i);
+    unsigned cnt = 0;
+    if (i < n) {
+        const HalfCellD hc = half_cell(c, rho);
+        const MState st = load_mstate(S, i);
+        cnt = pushed_through(st.alive, st.zeta, hc.zn);
+        stream_one_mixed<LASER>(c, hc, InvD{1.0 / c.fv.dx, 1.0 / c.fv.dy}, J, nullptr, st, D, i);
+    }
+    if (c.npush) add_pushes(c.npush, cnt);
 }
 
 // ---------------------------------------------------------------- rows forward (FP32)
@@ -564,8 +585,9 @@ void stream_mixed(const StreamArgs& a, float* J, float* slab, Parts src, Parts d
     const int bs = tg.ntile ? tg.block : kT;
     if (bs < 64 || bs > kStreamBlockMax || bs % 64) throw std::runtime_error("stream half-cell (mixed): bad block size");
     const long tail_blocks = nstrm > tg.n_tiled ? long(blocks_for(nstrm - tg.n_tiled, bs)) : 0;
-    const size_t lds = tg.ntile ? size_t(tg.WX) * tg.WY * (kDepNode * sizeof(double) + kFwNode * sizeof(float)) : 0;
+    const size_t lds = tg.ntile ? size_t(tg.WX) * tg.WY * (kDepNode + kFwNode) * sizeof(double) : 0;
     const dim3 grid(unsigned(tg.ntile + tg.max_extra + tail_blocks)), blk(bs);
+    if (a.npush && long(grid.x) > a.npush_cap) throw std::runtime_error("stream half-cell (mixed): push counter too small");
     auto* k = a.laser.on ? (tg.wpe == 4 ? k_stream_mixed<4, true> : k_stream_mixed<0, true>)
                          : (tg.wpe == 4 ? k_stream_mixed<4, false> : k_stream_mixed<0, false>);
     launch_timed_kernel(lt, k, grid, blk, unsigned(lds), a, J, slab, src, dst, nstrm, tg, int(rho));

@@ -472,7 +472,10 @@ __global__ void __launch_bounds__(kStreamBlockMax) k_stream_halfcell(StreamArgs 
             const int hi_r = y0 + tg.WY > nn - 1 ? nn - 1 : (y0 + tg.WY < lo_r + 1 ? lo_r + 1 : y0 + tg.WY);
             cl.fv.yax.hlo = lo_r; cl.fv.yax.hhi = hi_r;
         }
+        const double zn = c.zs[c.n];
+        unsigned cnt = 0;
         while (k < hi) {
+            cnt += pushed_through(p.alive, p.zeta, zn);
             stream_index<YF, ION, LASER>(cl, &w, p, D, i);
             k += blockDim.x;
             if (k < hi) {
@@ -493,10 +496,17 @@ __global__ void __launch_bounds__(kStreamBlockMax) k_stream_halfcell(StreamArgs 
                 if (gy < 0 || gy >= F.ny) continue;
                 atomicAdd(&c.J[cc * F.nk + long(gy) * F.nx + wrap_index(x0 + u, F.nx)], v);
             }
+        if (c.npush) add_pushes(c.npush, cnt);
         return;
     }
     const long i = tg.n_tiled + (b - tg.ntile) * blockDim.x + threadIdx.x;
-    if (i < n) stream_index<YF, ION, LASER>(c, nullptr, load_pstate(S, i), D, i);
+    unsigned cnt = 0;
+    if (i < n) {
+        const PState p = load_pstate(S, i);
+        cnt = pushed_through(p.alive, p.zeta, c.zs[c.n]);
+        stream_index<YF, ION, LASER>(c, nullptr, p, D, i);
+    }
+    if (c.npush) add_pushes(c.npush, cnt);
 }
 
 // ---------------------------------------------------------------- binning of the streaming list by tile
@@ -672,6 +682,7 @@ void stream_halfcell(const StreamArgs& a, Parts src, Parts dst, long nstrm, cons
     // the deposit window (5 double planes) and the field window (12 float planes)
     const size_t lds = tg.ntile ? size_t(tg.WX) * tg.WY * (5 * sizeof(double) + (tg.fwin ? 12 * sizeof(float) : 0)) : 0;
     const dim3 grid(unsigned(tg.ntile + tail_blocks)), blk(bs);
+    if (a.npush && long(grid.x) > a.npush_cap) throw std::runtime_error("stream half-cell: push counter too small");
     const bool ion = a.ions != nullptr;
     const unsigned sh = unsigned(lds);
     auto* k = a.laser.on

@@ -29,7 +29,7 @@ class Glaser(C.Structure):
 
 class Ion(C.Structure):
     _fields_ = [("mass", C.c_double), ("atoms", C.c_double), ("Z0", C.c_int), ("Zmax", C.c_int), ("mobile", C.c_int),
-                ("ionization", C.c_int), ("n_ip", C.c_int), ("ip_eV", C.c_double * 16)]
+                ("ionization", C.c_int), ("n_ip", C.c_int), ("ip_eV", C.c_double * 16), ("name", C.c_char * 16)]
 
 
 class Config(C.Structure):
@@ -57,7 +57,8 @@ class Config(C.Structure):
 class SimInfo(C.Structure):
     _fields_ = [(n, C.c_long) for n in ("N", "ns", "nk", "np", "nstrm", "istep", "born_cap")] + \
                [(n, C.c_double) for n in ("s", "dx", "dy", "X0", "Y0", "beta_b", "gm2_b")] + \
-               [(n, C.c_long) for n in ("trapped_total", "restreamed_total", "lost_total")]
+               [(n, C.c_long) for n in ("trapped_total", "restreamed_total", "lost_total", "pushes_stream",
+                                         "pushes_beam")]
 
 
 # the 104-byte particle record (src/sim.hpp:217)
@@ -160,13 +161,14 @@ class Deck:
         self.c.n_glasers += 1
         return self
 
-    def add_ion(self, mass, Z0, Zmax, atoms=1.0, mobile=True, ionization=False, ip_eV=()):
+    def add_ion(self, mass, Z0, Zmax, atoms=1.0, mobile=True, ionization=False, ip_eV=(), name=""):
         """an ion species (src/sim.hpp IonSpecies): mass in electron masses, atoms per plasma electron, charge
         states Z0 .. Zmax, ionization potentials [eV] of the levels Z0 ... Zmax-1 (indexed by charge state)"""
         if self.c.n_ions >= 4:
             raise ValueError("at most 4 ion species")
         s = self.c.ions[self.c.n_ions]
         s.mass = mass; s.atoms = atoms; s.Z0 = Z0; s.Zmax = Zmax; s.mobile = int(mobile); s.ionization = int(ionization)
+        s.name = name.encode()[:15]
         s.n_ip = len(ip_eV)
         for k, v in enumerate(ip_eV):
             s.ip_eV[k] = v
@@ -252,6 +254,15 @@ def _load():
         "g3d_boris": ([P], C.c_int), "g3d_boris_vjp": ([P, P], C.c_int),
         "g3d_stream_push": ([P], C.c_int), "g3d_stream_push_vjp": ([P, P], C.c_int),
         "g3d_bunch_push_piece": ([P], C.c_int), "g3d_bunch_push_vjp": ([P, P], C.c_int),
+        "g3d_bunch_push": ([P, P, C.c_long, C.c_long, C.c_double], C.c_int),
+        "g3d_stream_halfcell": ([P, P, P, C.c_long], C.c_int),
+        "g3d_stream_spectra": ([P, P, P, P], C.c_int),
+        "g3d_real_fields_slice": ([P, P, P, P, P, P, P, P, C.c_long, P], C.c_int),
+        "g3d_bunch_currents": ([P, P, P, P, C.c_double, C.c_double, C.c_int, C.c_int, P, C.c_long, P, P, P, P, P],
+                               C.c_int),
+        "g3d_bunch_centroid": ([P, C.c_long, C.c_double, C.c_double, P], C.c_int),
+        "g3d_window_shift": ([P, C.c_int, C.c_int, C.c_double, C.c_double, P, P, C.c_int, P, P, P, C.c_long],
+                             C.c_int),
         "g3d_sim_create": ([P, C.POINTER(P)], C.c_int), "g3d_sim_destroy": ([P], C.c_int),
         "g3d_sim_info_get": ([P, P], C.c_int), "g3d_sim_init": ([P], C.c_int), "g3d_sim_step": ([P], C.c_int),
         "g3d_sim_call": ([P, C.c_char_p, C.POINTER(C.c_double), C.c_int], C.c_int),
@@ -261,6 +272,15 @@ def _load():
         "g3d_sim_set_particles": ([P, P, C.c_long], C.c_int), "g3d_sim_set_joining": ([P, P, C.c_long], C.c_int),
         "g3d_sim_get_joining": ([P, P, C.c_long, C.POINTER(C.c_long)], C.c_int),
         "g3d_sim_timing": ([P, C.c_int], C.c_int), "g3d_sim_timing_every": ([P, C.c_int], C.c_int),
+        "g3d_sim_write_bunches": ([P, C.c_char_p, C.c_int], C.c_int),
+        "g3d_sim_write_slices": ([P, C.c_char_p, C.c_double], C.c_int),
+        "g3d_sim_diagnostics": ([P, C.c_char_p, C.c_int], C.c_int),
+        "g3d_sim_field_energy": ([P, C.POINTER(C.c_double)], C.c_int),
+        "g3d_sim_dump_particles": ([P, C.c_char_p], C.c_int),
+        "g3d_sim_write_frame": ([P, C.c_char_p, C.c_int], C.c_int),
+        "g3d_sim_write_axis": ([P, C.c_char_p], C.c_int),
+        "g3d_sim_write_checkpoint": ([P, C.c_char_p], C.c_int),
+        "g3d_sim_read_checkpoint": ([P, C.c_char_p, C.c_long], C.c_int),
         "g3d_sim_timers": ([P, C.POINTER(C.c_double), C.POINTER(C.c_long), C.c_int, C.c_char_p, C.c_int], C.c_int),
     }
     for name, (args, res) in sig.items():
@@ -288,7 +308,10 @@ def check(rc):
 
 
 ARRAY_DTYPES = {"F": np.float32, "jw": np.float64, "jsr": np.float64, "rho_lines": np.float64, "kx": np.float64,
-                "ky": np.float64, "hcg": np.float64, "wlo": np.float64, "whi": np.float64,
+                "ky": np.float64, "hcg": np.float64, "wlo": np.float64, "whi": np.float64, "kxs": np.float64,
+                "kys": np.float64, "filt": np.float64, "filty": np.float64, "zs": np.float64, "iA": np.float64,
+                "iD": np.float64, "wsl": np.float64, "ion_stats0": np.float64, "ion_stats1": np.float64,
+                "ion_stats2": np.float64, "ion_stats3": np.float64,
                 "parts": PARTICLE, "strm": PARTICLE, "born": PARTICLE}
 
 
@@ -368,6 +391,37 @@ class Sim(SimBase):
     def _info(self, i):
         check(lib().g3d_sim_info_get(self.h, C.byref(i)))
 
+    # output and checkpoints in the reference's formats (src/sim_bunch_diag.cpp, src/sim_output.cpp,
+    # src/sim_checkpoint.cpp)
+    def write_bunches(self, path, header=False):
+        check(lib().g3d_sim_write_bunches(self.h, str(path).encode(), int(header)))
+
+    def write_slices(self, path, slice_length):
+        check(lib().g3d_sim_write_slices(self.h, str(path).encode(), float(slice_length)))
+
+    def diagnostics(self, path, header=False):
+        check(lib().g3d_sim_diagnostics(self.h, str(path).encode(), int(header)))
+
+    def field_energy(self):
+        e = C.c_double()
+        check(lib().g3d_sim_field_energy(self.h, C.byref(e)))
+        return e.value
+
+    def dump_particles(self, path):
+        check(lib().g3d_sim_dump_particles(self.h, str(path).encode()))
+
+    def write_frame(self, path, comps=255):
+        check(lib().g3d_sim_write_frame(self.h, str(path).encode(), int(comps)))
+
+    def write_axis(self, path):
+        check(lib().g3d_sim_write_axis(self.h, str(path).encode()))
+
+    def write_checkpoint(self, directory):
+        check(lib().g3d_sim_write_checkpoint(self.h, str(directory).encode()))
+
+    def read_checkpoint(self, directory, step):
+        check(lib().g3d_sim_read_checkpoint(self.h, str(directory).encode(), int(step)))
+
     FAMILIES = ("stream_halfcell", "pass_rows_fwd", "pass_cols", "pass_rows_inv", "unused", "beam_push", "currents",
                 "sponge_real_fields")
 
@@ -396,3 +450,111 @@ class Sim(SimBase):
             nm = names.raw[k * 32:(k + 1) * 32].split(b"\0")[0].decode()
             out[nm] = (ms[k], cnt[k])
         return out
+
+
+# ---------------------------------------------------------------- kernel level (include/g3d.h): the reference's
+# device contexts over caller-owned device buffers
+class Axis(C.Structure):
+    """src/grid.hpp:20-46 Axis: uniform nodes x0 + i/ih, or a device node table X[0 .. n-1]"""
+    _fields_ = [("X", C.c_void_p), ("n", C.c_int), ("uniform", C.c_int), ("x0", C.c_double), ("ih", C.c_double)]
+
+
+class PlaneWave(C.Structure):
+    _fields_ = [("on", C.c_int), ("pol", C.c_int)] + \
+               [(n, C.c_double) for n in ("a0", "kL", "tau", "cep", "eta_c", "kx", "kz")]
+
+
+class PushCtx(C.Structure):
+    """src/sim_kernels.hpp:10-17 PushCtx without the external elements"""
+    _fields_ = [("F", C.c_void_p)] + [(n, C.c_int) for n in ("a", "ns", "N", "nx", "ny", "self", "yf")] + \
+               [("nk", C.c_long)] + [(n, C.c_double) for n in ("dx", "dy", "Lx", "Ly")] + \
+               [("zax", Axis), ("yax", Axis), ("laser", PlaneWave)] + \
+               [(n, C.c_double) for n in ("X0", "Y0", "zc", "dz_push", "push_phase", "zhand", "zmax", "ylo", "yhi",
+                                          "s1", "vz_min", "hmin")] + [("freeze", C.c_int)]
+
+
+class Parts(C.Structure):
+    """g3d_parts: a device particle list as structure of arrays"""
+    _fields_ = [(n, C.c_void_p) for n in ("x", "y", "zeta", "px", "py", "pz", "w", "z", "q", "qm", "rem", "alive",
+                                          "sp", "id")]
+
+
+class StreamCtx(C.Structure):
+    """src/sim_stream.hpp:20-29 StreamCtx"""
+    _fields_ = [("pc", PushCtx), ("zs", C.c_void_p), ("J", C.c_void_p), ("ia", C.c_void_p), ("id", C.c_void_p)] + \
+               [(n, C.c_double) for n in ("Delta", "s1", "sTrap", "vz_beam")] + [("n", C.c_int)] + \
+               [("ions", C.c_void_p), ("adk", C.c_void_p), ("born", Parts), ("bcnt", C.c_void_p), ("bcap", C.c_int),
+                ("key", C.c_uint64)]
+
+
+class SpecCtx(C.Structure):
+    """the spectral geometry of the constructor (src/sim_setup.cpp:5-60, 145-164), device tables"""
+    _fields_ = [(n, C.c_int) for n in ("nx", "ny", "ns", "N", "a")] + [("nk", C.c_long)] + \
+               [(n, C.c_void_p) for n in ("kx", "ky", "kxs", "kys", "filt", "filty", "hcg", "wlo", "whi")] + \
+               [("a2", C.c_double), ("keep0", C.c_int), ("plan", C.c_void_p)]
+
+
+class DevArray:
+    """a device buffer from g3d_malloc holding a copy of a host array (or `nbytes` zero bytes)"""
+
+    def __init__(self, host=None, nbytes=None):
+        self.ptr = C.c_void_p()
+        self.nbytes = int(host.nbytes if host is not None else nbytes)
+        check(lib().g3d_malloc(C.byref(self.ptr), max(self.nbytes, 8)))
+        if host is not None and self.nbytes:
+            h = np.ascontiguousarray(host)
+            check(lib().g3d_memcpy_h2d(self.ptr, h.ctypes.data_as(C.c_void_p), self.nbytes))
+
+    def get(self, dtype, count=None):
+        out = np.empty(self.nbytes // np.dtype(dtype).itemsize if count is None else count, dtype=dtype)
+        if out.nbytes:
+            check(lib().g3d_memcpy_d2h(out.ctypes.data_as(C.c_void_p), self.ptr, out.nbytes))
+        return out
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().g3d_free(self.ptr)
+                self.ptr = C.c_void_p()
+        except Exception:
+            pass
+
+
+SOA_FIELDS = ("x", "y", "zeta", "px", "py", "pz", "w", "z", "q", "qm", "rem", "alive", "sp", "id")
+
+
+class DeviceParts:
+    """a particle list on the device as structure of arrays (g3d_parts), made from 104-byte records"""
+
+    def __init__(self, records=None, n=None):
+        recs = np.zeros(n, dtype=PARTICLE) if records is None else np.ascontiguousarray(records, dtype=PARTICLE)
+        self.n = len(recs)
+        self.arrays = {f: DevArray(np.ascontiguousarray(recs[f])) for f in SOA_FIELDS}
+        self.c = Parts(**{f: self.arrays[f].ptr.value for f in SOA_FIELDS})
+
+    def records(self):
+        out = np.zeros(self.n, dtype=PARTICLE)
+        for f in SOA_FIELDS:
+            out[f] = self.arrays[f].get(PARTICLE[f], self.n)
+        return out
+
+
+class FftPlan:
+    """FftMany(nx, ny, two_d) of the C ABI"""
+
+    def __init__(self, nx, ny, two_d=True):
+        self.h = C.c_void_p()
+        check(lib().g3d_fft_plan(nx, ny, int(two_d), C.byref(self.h)))
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().g3d_fft_destroy(self.h)
+                self.h = C.c_void_p()
+        except Exception:
+            pass
+
+
+def kernel_call(name, *args):
+    """call a kernel-level entry point of the C ABI and raise on a nonzero status"""
+    check(getattr(lib(), name)(*args))

@@ -49,6 +49,7 @@ typedef struct {
     double mass, atoms;
     int Z0, Zmax, mobile, ionization, n_ip;
     double ip_eV[16];
+    char name[16];            /* the ion block's name (the output's inj_<name>); empty: ion<k> */
 } g3d_ion;
 
 /* run configuration: the fields of the reference's Config (src/sim.hpp:36-187) that the march uses, in code
@@ -245,6 +246,111 @@ typedef struct {
 int g3d_bunch_push_piece(const g3d_bpush_ctx* c);
 int g3d_bunch_push_vjp(const g3d_bpush_ctx* c, const g3d_bpush_adj* g);
 
+/* ------------------------------------------------------------------ kernel level: the device contexts of the
+ * reference (src/sim_kernels.hpp:10-17 PushCtx, src/sim_stream.hpp:20-29 StreamCtx) and the member functions of the
+ * drop-in files over caller-owned device buffers in the reference's layouts.  Fourier y unless stated; free-hand y
+ * runs through the engine (g3d_sim_*). */
+
+/* a grid axis (src/grid.hpp:20-46 Axis): uniform nodes x0 + i / ih (uniform = 1), or the node table X[0 .. n-1]
+ * on the device; outside the table the end cells are continued */
+typedef struct {
+    const double* X;
+    int n, uniform;
+    double x0, ih;
+} g3d_axis;
+
+/* the analytic plane-wave laser in code units (src/sim.hpp:191-210 Laser): pol 0 s, 1 p, 2 circular; eta_c the
+ * centre of the envelope; (kx, kz) the unit propagation vector */
+typedef struct {
+    int on, pol;
+    double a0, kL, tau, cep, eta_c, kx, kz;
+} g3d_plane_wave;
+
+/* PushCtx (src/sim_kernels.hpp:10-17) without the external elements (undulators, magnets) */
+typedef struct {
+    const float* F;               /* real fields [6][ns+2][nk] */
+    int a, ns, N, nx, ny, self, yf;
+    long nk;
+    double dx, dy, Lx, Ly;
+    g3d_axis zax, yax;
+    g3d_plane_wave laser;
+    double X0, Y0, zc, dz_push, push_phase, zhand, zmax, ylo, yhi, s1, vz_min, hmin;
+    int freeze;
+} g3d_push_ctx;
+
+/* Simulation::push (src/sim_particles.cpp:6-24): particles [i0, i1) of the beam list to the plane s1, those with
+ * alive = 1 from s0, adopted ones (alive = 3) from s1 - rem; RK4 in z with the gathers of src/sim_kernels.hpp:75-149 */
+int g3d_bunch_push(const g3d_push_ctx* c, const g3d_parts* p, long i0, long i1, double s0);
+
+/* ADK level (src/ionization.hpp:25-30) and the device constants of an ion species (src/sim_ions.cpp:68-87) */
+typedef struct { double logpre, power, expo, Epeak, Emin; } g3d_adk_level;
+typedef struct { double mass; int mobile, ionizing, Zmax, off; double Emin; } g3d_ion_dev;
+
+/* StreamCtx (src/sim_stream.hpp:20-29) */
+typedef struct {
+    g3d_push_ctx pc;              /* fields, laser, grids */
+    const double* zs;             /* slice positions, N */
+    double* J;                    /* jsr: j_x j_y j_z rho- rho+ of the half-cell, [5][nk] (added to) */
+    const double *ia, *id;        /* 1/A_j, 1/Delta_j (Fourier y: 1/dy) */
+    double Delta, s1, sTrap, vz_beam;
+    int n;                        /* the half-cell */
+    const g3d_ion_dev* ions;      /* ion species (null: no ions), their ADK levels */
+    const g3d_adk_level* adk;
+    g3d_parts born;               /* electrons freed in this pass at slots *bcnt (atomic), at most bcap */
+    int* bcnt;
+    int bcap;
+    uint64_t key;                 /* ion_key_slice of the pass (src/sim_plasma.cpp:293,331) */
+} g3d_stream_ctx;
+
+/* the streaming loop of march_plasma (src/sim_plasma.cpp:334-337, stream_one of src/sim_stream.hpp:104-171): every
+ * particle [0, nstrm) of src through half-cell c->n, the pushed state written to dst (dst may be src), the deposit
+ * added to c->J with FP64 atomics (summation order as free as the reference's OpenMP atomics) */
+int g3d_stream_halfcell(const g3d_stream_ctx* c, const g3d_parts* src, const g3d_parts* dst, long nstrm);
+
+/* the spectral geometry of the constructor (src/sim_setup.cpp:5-60, 145-164) as device tables, for the member
+ * functions below (Fourier y) */
+typedef struct {
+    int nx, ny, ns, N, a;
+    long nk;
+    const double *kx, *ky;        /* k vectors of the march, the Nyquist mode zeroed (nx, ny) */
+    const double *kxs, *kys;      /* k vectors of the filters and phase shifts (nx, ny) */
+    const double *filt, *filty;   /* cos^2 (k d/2) filters, Nyquist zeroed, all ones without k_filter (nx, ny) */
+    const double *hcg, *wlo, *whi;   /* half-cell lengths (hcg[m+2] = h_m) and current weights of the slices */
+    double a2;                    /* smooth_length^2 of the streaming current */
+    int keep0;                    /* keep the k = 0 mode of the streaming and beam currents (box_mean) */
+    g3d_fft* plan;                /* FftMany(nx, ny, two_d = 1) */
+} g3d_spec_ctx;
+
+/* Simulation::stream_spectra (src/sim_plasma.cpp:94-186): jsr [5][nk] -> jsn [3][nk] (j_x, j_y, j_z spectra with the
+ * transverse filters and the smoothing); jsr is zeroed; scratch: 6 nk complex values */
+int g3d_stream_spectra(const g3d_spec_ctx* c, double* jsr, g3d_cpx* jsn, g3d_cpx* scratch);
+
+/* Simulation::real_fields_slice (src/sim_plasma.cpp:200-220): the spectra of one slice (plane pointers of R_x R_y
+ * L_x L_y E_z B_z) -> its six real field planes E_x E_y E_z B_x B_y B_z at F_slice + comp * fpl (float);
+ * scratch: 3 nk complex values */
+int g3d_real_fields_slice(const g3d_spec_ctx* c, const g3d_cpx* Rx, const g3d_cpx* Ry, const g3d_cpx* Lx,
+                          const g3d_cpx* Ly, const g3d_cpx* Ez, const g3d_cpx* Bz, float* F_slice, long fpl,
+                          g3d_cpx* scratch);
+
+/* Simulation::currents (src/sim_particles.cpp:56-207): the beam deposit of np particles into jw [3][ns+3][nk]
+ * (summed in the reference's single-thread order: the same doubles), the (1,2,1)/4 zeta filter, the spectra
+ * jk{x,y,z} [ns+1][nk] with the transverse filter and the k = 0 rule (keep0: plasma with box_mean, not starting);
+ * zfilter, zuniform: cfg.zeta_filter and a uniform zeta axis; ia, id as g3d_stream_ctx; zax: the zeta axis;
+ * scratch: (ns+1) nk complex values.  The deposit's sort buffers are held by the library between calls. */
+int g3d_bunch_currents(const g3d_spec_ctx* c, const g3d_axis* zax, const double* ia, const double* id, double dx,
+                       double dy, int zfilter, int zuniform, const g3d_parts* p, long np, double* jw, g3d_cpx* jkx,
+                       g3d_cpx* jky, g3d_cpx* jkz, g3d_cpx* scratch);
+
+/* the window centroid of src/sim_setup.cpp:221-234: sums over the live bunch particles of w (x + X0), w (y + Y0)
+ * and w into out[3] (host), in a fixed tree order; the partial sums are library scratch */
+int g3d_bunch_centroid(const g3d_parts* p, long np, double X0, double Y0, double* out);
+
+/* the window shift of src/sim_setup.cpp:248-280 by (nxs, nys) cells: the phase factor e^{i (k_x sx + k_y sy)} on
+ * count planes of each spectral array in planes[0 .. nplanes) (R, E_z, B_z, j0, js0 ...), the index roll of F
+ * [6][ns+2][nk] (scratch: F's size in floats), and the beam particles moved by (-sx, -sy) */
+int g3d_window_shift(const g3d_spec_ctx* c, int nxs, int nys, double dx, double dy, g3d_cpx* const* planes,
+                     const long* counts, int nplanes, float* F, float* F_scratch, const g3d_parts* p, long np);
+
 /* ------------------------------------------------------------------ engine: the Simulation of the march
  * g3d_sim mirrors the reference's Simulation (src/sim.hpp:222-360) on one GPU: init() and step() and the
  * member functions of the five drop-in files, each over the kernels of this library. */
@@ -253,6 +359,10 @@ typedef struct {
     long N, ns, nk, np, nstrm, istep, born_cap;
     double s, dx, dy, X0, Y0, beta_b, gm2_b;
     long trapped_total, restreamed_total, lost_total;
+    /* pushes done since g3d_sim_create (no counterpart in the reference: the measure of the march's throughput):
+     * streaming particles through one half-cell in one pass, counted on the device by the stream launches, and
+     * beam particles advanced from s0 to s1 (the list at the start of each push) */
+    long pushes_stream, pushes_beam;
 } g3d_sim_info;
 
 int g3d_sim_create(const g3d_config* cfg, g3d_sim** out);
@@ -275,6 +385,25 @@ int g3d_sim_set(g3d_sim* sim, const char* name, const void* host, size_t bytes);
 int g3d_sim_set_particles(g3d_sim* sim, const g3d_particle* p, long n);
 int g3d_sim_set_joining(g3d_sim* sim, const g3d_particle* p, long n);
 int g3d_sim_get_joining(g3d_sim* sim, g3d_particle* p, long cap, long* n);
+/* output and checkpoints in the reference's formats, from the simulation's state after a step:
+ * bunches.txt lines (Simulation::write_bunches, src/sim_bunch_diag.cpp:63-77) and diag.txt lines
+ * (Simulation::diagnostics, src/sim_output.cpp:5-52) appended to the file at path, with the header line when
+ * header != 0; slices_NNNNNN.txt (write_slices, src/sim_bunch_diag.cpp:79-97); the field energy in code units
+ * (total_field_energy, src/sim_output.cpp:54-79); the particle dump (dump_particles, 81-97); the frame /
+ * fields_NNNNNN.bin file of the planes through the axis (write_frame, 103-198; comps: bits ex ey ez bx by bz ne ni)
+ * and axis_NNNNNN.txt (write_axis, 200-245) */
+int g3d_sim_write_bunches(g3d_sim* sim, const char* path, int header);
+int g3d_sim_write_slices(g3d_sim* sim, const char* path, double slice_length);
+int g3d_sim_diagnostics(g3d_sim* sim, const char* path, int header);
+int g3d_sim_field_energy(g3d_sim* sim, double* out);
+int g3d_sim_dump_particles(g3d_sim* sim, const char* path);
+int g3d_sim_write_frame(g3d_sim* sim, const char* path, int comps);
+int g3d_sim_write_axis(g3d_sim* sim, const char* path);
+/* the G3DCKPT1 checkpoint (src/sim_checkpoint.cpp:9-105): written as dir/checkpoint_NNNNNN_r0000.bin after a step
+ * (write_checkpoint), read instead of init() (read_checkpoint); the same header and records as the reference's,
+ * one rank */
+int g3d_sim_write_checkpoint(g3d_sim* sim, const char* dir);
+int g3d_sim_read_checkpoint(g3d_sim* sim, const char* dir, long step);
 /* HIP event timing of the kernel families: bit f of mask times family f (-1: all, 0: off); clears the sums.
  * g3d_sim_timers reads the accumulated milliseconds and launch counts with the family names. */
 int g3d_sim_timing(g3d_sim* sim, int mask);

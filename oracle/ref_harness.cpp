@@ -92,7 +92,7 @@ Config make_config(const g3d_config& g) {
     for (int k = 0; k < g.n_ions; ++k) {
         Config::IonSpecies s;
         const g3d_ion& i = g.ions[k];
-        s.name = "ion" + std::to_string(k + 1);
+        s.name = i.name[0] ? std::string(i.name, strnlen(i.name, sizeof i.name)) : "ion" + std::to_string(k + 1);
         s.mass = i.mass; s.atoms = i.atoms; s.Z0 = i.Z0; s.Zmax = i.Zmax; s.mobile = i.mobile; s.ionization = i.ionization;
         s.ip_eV.assign(i.ip_eV, i.ip_eV + i.n_ip);
         c.ions.push_back(s);
@@ -141,6 +141,7 @@ int ref_sim_info(void* h, g3d_sim_info* o) {
     o->born_cap = s.born_cap; o->s = s.s; o->dx = s.dx; o->dy = s.dy; o->X0 = s.X0; o->Y0 = s.Y0;
     o->beta_b = s.beta_b; o->gm2_b = s.gm2_b; o->trapped_total = s.trapped_total;
     o->restreamed_total = s.restreamed_total; o->lost_total = s.lost_total;
+    o->pushes_stream = 0; o->pushes_beam = 0;      // not counted by the reference
     return 0;
 }
 
@@ -167,6 +168,7 @@ int ref_sim_call(void* h, const char* op_, const double* a, int na) {
         else if (op == "set_s") { need(2); s.s = a[0]; s.istep = long(a[1]); }
         else if (op == "set_starting") { need(1); s.starting_ = a[0] != 0; }
         else if (op == "stream_pass") { need(4); stream_pass(s, int(a[0]), int(a[1]), a[2], a[3]); }
+        else if (op == "shift_window") { need(2); s.shift_window(int(a[0]), int(a[1])); }
         else if (op == "reserve_born") {
             // room for `cap` freed electrons behind the streaming list, as load_plasma leaves it
             // (src/sim_plasma.cpp:82-88): born_cap, the born buffer, and the list's spare capacity
@@ -185,6 +187,30 @@ int ref_sim_call(void* h, const char* op_, const double* a, int na) {
             s.jkx.to_host(t.data(), 0, n); s.j0x.from_host(t.data(), 0, n);
             s.jky.to_host(t.data(), 0, n); s.j0y.from_host(t.data(), 0, n);
         } else { err_ = "unknown op " + op; return 3; }
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
+}
+
+// the reference's output and checkpoint member functions: what = bunches | diag (appended to path, arg: header),
+// slices (darg: slice length), energy (*out), particles, frame (arg: comps), axis, write_ckpt / read_ckpt (path: the
+// directory, arg: the step)
+int ref_sim_output(void* h, const char* what_, const char* path, long arg, double darg, double* out) {
+    try {
+        Simulation& s = *static_cast<Ref*>(h)->sim;
+        const std::string what(what_);
+        if (what == "bunches" || what == "diag") {
+            FILE* f = std::fopen(path, "a");
+            if (!f) { err_ = std::string("cannot write ") + path; return 1; }
+            if (what == "bunches") s.write_bunches(f, arg != 0); else s.diagnostics(f, arg != 0);
+            std::fclose(f);
+        } else if (what == "slices") { s.cfg.slice_length = darg; s.write_slices(path); }
+        else if (what == "energy") *out = s.total_field_energy();
+        else if (what == "particles") s.dump_particles(path);
+        else if (what == "frame") s.write_frame(path, int(arg));
+        else if (what == "axis") s.write_axis(path);
+        else if (what == "write_ckpt") { s.cfg.dir = path; s.write_checkpoint(); }
+        else if (what == "read_ckpt") { s.cfg.dir = path; s.read_checkpoint(arg); }
+        else { err_ = "unknown output " + what; return 3; }
         return 0;
     } catch (const std::exception& e) { return fail(e); }
 }

@@ -32,6 +32,7 @@ def lib():
             "ref_last_error": ([], C.c_char_p), "ref_set_threads": ([C.c_int], None),
             "ref_sim_create": ([P], P), "ref_sim_destroy": ([P], None), "ref_sim_info": ([P, P], C.c_int),
             "ref_sim_call": ([P, C.c_char_p, C.POINTER(C.c_double), C.c_int], C.c_int),
+            "ref_sim_output": ([P, C.c_char_p, C.c_char_p, C.c_long, C.c_double, C.POINTER(C.c_double)], C.c_int),
             "ref_sim_array_size": ([P, C.c_char_p, C.POINTER(C.c_size_t)], C.c_int),
             "ref_sim_get": ([P, C.c_char_p, P, C.c_size_t], C.c_int),
             "ref_sim_set": ([P, C.c_char_p, P, C.c_size_t], C.c_int),
@@ -102,6 +103,38 @@ class RefSim(g3d_amd.SimBase):
     def set(self, name, arr):
         arr = np.ascontiguousarray(arr)
         check(lib().ref_sim_set(self.h, name.encode(), arr.ctypes.data_as(C.c_void_p), arr.nbytes))
+
+    def _output(self, what, path="", arg=0, darg=0.0):
+        out = C.c_double()
+        check(lib().ref_sim_output(self.h, what.encode(), str(path).encode(), int(arg), float(darg), C.byref(out)))
+        return out.value
+
+    def write_bunches(self, path, header=False):
+        self._output("bunches", path, int(header))
+
+    def write_slices(self, path, slice_length):
+        self._output("slices", path, 0, slice_length)
+
+    def diagnostics(self, path, header=False):
+        self._output("diag", path, int(header))
+
+    def field_energy(self):
+        return self._output("energy")
+
+    def dump_particles(self, path):
+        self._output("particles", path)
+
+    def write_frame(self, path, comps=255):
+        self._output("frame", path, comps)
+
+    def write_axis(self, path):
+        self._output("axis", path)
+
+    def write_checkpoint(self, directory):
+        self._output("write_ckpt", directory)
+
+    def read_checkpoint(self, directory, step):
+        self._output("read_ckpt", directory, step)
 
     def set_particles(self, p):
         p = np.ascontiguousarray(p, dtype=g3d_amd.PARTICLE)

@@ -71,3 +71,46 @@ def particle_parity(a, b, deck, tol=1e-12):
     p = np.sqrt(b["px"] ** 2 + b["py"] ** 2 + b["pz"] ** 2)
     dp = np.sqrt((a["px"] - b["px"]) ** 2 + (a["py"] - b["py"]) ** 2 + (a["pz"] - b["pz"]) ** 2)
     assert np.linalg.norm(dp) <= tol * np.linalg.norm(p), np.linalg.norm(dp) / np.linalg.norm(p)
+
+
+def config2_deck(mixed=0, nz=1024):
+    """The laser-wakefield deck at full size (256 x 256 modes, 1024 slices of 0.25/k0, 262144 streaming electrons
+    per slice, two passes, Gaussian grid laser a0 = 2, w0 = 30/k0, c tau = 15/k0 at zeta = 200/k0) with the trailing
+    witness bunch (2^20 electrons, sigma_r = 1.5/k0, sigma_zeta = 3/k0, 0.005 n_c peak = 2.40 pC, gamma = 200, at
+    zeta = 100/k0 in the first bucket behind the laser: the plasma wavelength is 2 pi / sqrt(1e-3) = 199/k0)."""
+    d = g3d_amd.Deck(LAMBDA0)
+    d.c.nx = 256; d.c.Lx = 240.0
+    d.y_uniform(256, 240.0)
+    d.zeta_uniform(nz, (nz - 1) * 0.25)
+    d.set(dz=20.0, z_end=4000.0, sponge=24.0, sponge_y=24.0, move_x=0, move_y=0, plasma_on=1, n0=1e-3,
+          ppc_x=2, ppc_y=2, smooth_length=0.0, iterations=2, mixed=mixed)
+    d.add_grid_laser(a0=2.0, waist=30.0, tau=15.0, zeta_c=200.0, z_focus=0.0)
+    d.set(charge=2.40e-12, gamma=200.0, sigx=1.5, sigy=1.5, sigz=3.0, emitx=1e-3, emity=1e-3, espread=0.0,
+          zeta_center=100.0, nmacro=1 << 20, seed=7)
+    return d
+
+
+def bunch_moments(p):
+    """The beam moments of the reference's bunches.txt for the live particles of a list, one group
+    (src/sim_bunch_diag.cpp:9-61): the weighted first moments, then the second moments centred on them (two passes);
+    returns the total weight, <gamma> and the normalised emittances eps_nx, eps_ny (code units)."""
+    live = p[p["alive"] == 1]
+    w = live["w"]
+    W = w.sum()
+    g = np.sqrt(1 + live["px"] ** 2 + live["py"] ** 2 + live["pz"] ** 2)
+    x, y, ux, uy = live["x"], live["y"], live["px"], live["py"]
+    mx, my, mux, muy = [(w * v).sum() / W for v in (x, y, ux, uy)]
+    dx, dy, dux, duy = x - mx, y - my, ux - mux, uy - muy
+    var = lambda a, b: (w * a * b).sum() / W
+    ex = np.sqrt(max(0.0, var(dx, dx) * var(dux, dux) - var(dx, dux) ** 2))
+    ey = np.sqrt(max(0.0, var(dy, dy) * var(duy, duy) - var(dy, duy) ** 2))
+    return W, float((w * g).sum() / W), float(ex), float(ey)
+
+
+def F_slice_errors(Fa, Fb, ns, nk):
+    """relative L2 of F per slice (E and B of the slice together) over every own slice, and the slice norms"""
+    A = np.asarray(Fa, dtype=np.float64).reshape(6, ns + 2, nk)[:, :ns]
+    B = np.asarray(Fb, dtype=np.float64).reshape(6, ns + 2, nk)[:, :ns]
+    nb = np.sqrt((B ** 2).sum(axis=(0, 2)))
+    de = np.sqrt(((A - B) ** 2).sum(axis=(0, 2)))
+    return de / np.maximum(nb, 1e-300), nb

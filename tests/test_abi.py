@@ -43,3 +43,31 @@ def test_status_strings():
     lib.g3d_status_string.restype = ctypes.c_char_p
     lib.g3d_status_string.argtypes = [ctypes.c_int]
     assert lib.g3d_status_string(0)
+
+
+def test_ctypes_mirrors_match_the_c_layout(tmp_path):
+    """every field offset and the size of the Python mirrors of the C-ABI structs equal the C compiler's (a small C
+    program compiled here prints them)"""
+    import subprocess
+    import g3d_amd
+    mirrors = {"g3d_axis": g3d_amd.Axis, "g3d_plane_wave": g3d_amd.PlaneWave, "g3d_push_ctx": g3d_amd.PushCtx,
+               "g3d_parts": g3d_amd.Parts, "g3d_stream_ctx": g3d_amd.StreamCtx, "g3d_spec_ctx": g3d_amd.SpecCtx,
+               "g3d_config": g3d_amd.Config, "g3d_sim_info": g3d_amd.SimInfo}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "%s"' % HEADER, "int main(void) {"]
+    for cname, py in mirrors.items():
+        lines.append('printf("%s size %%zu\\n", sizeof(%s));' % (cname, cname))
+        for f, _ in py._fields_:
+            lines.append('printf("%s %s %%zu\\n", offsetof(%s, %s));' % (cname, f, cname, f))
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-o", str(exe), str(src)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        a, b, v = line.split()
+        got[(a, b)] = int(v)
+    for cname, py in mirrors.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)

@@ -1,0 +1,329 @@
+"""The reference's analytic checks of the march, re-expressed on the MI355X engine (FP64 form) with the reference's
+decks, references and tolerances: vacuum dispersion of the reduced system (tests/test_fields.cpp:103-148), the 1D
+linear wake against cold-fluid theory (tests/test_plasma.cpp:7-53), the smoothing commuting with the divergence on
+the free-hand y grid (tests/test_plasma.cpp:136-176), Gauss's law for a long bunch and a bunch in equilibrium in
+its own field (tests/test_bunch.cpp:36-110), and field ionization against the rate equations
+(tests/test_ionization.cpp:30-90).  The checks the engine cannot run are left out: grids whose mode numbers are not
+powers of two (its transforms take powers of two)."""
+import math
+
+import numpy as np
+import pytest
+
+import g3d_amd
+
+pytestmark = pytest.mark.gpu
+LAMBDA_TEST = 10e-6          # the reference tests' unit system (Config::base_units(10e-6))
+
+
+def close(val, ref, rtol, what):
+    assert abs(val - ref) <= rtol * abs(ref), "%s: %.6g against %.6g (rtol %g)" % (what, val, ref, rtol)
+
+
+def base(nx, ny, nz, Lx, Ly, Lw, lam=LAMBDA_TEST):
+    """tests/test_util.hpp:36-43: Fourier x and y, nz uniform slices over Lw, no sponge, fixed window"""
+    return g3d_amd.Deck(lam).grid(nx, ny, nz, Lx, Ly, Lw)
+
+
+def field(F, ns, nk, comp, n):
+    return np.asarray(F, dtype=np.float64).reshape(6, ns + 2, nk)[comp, n]
+
+
+@pytest.mark.parametrize("kind", ["uniform", "zeta", "y"])
+def test_vacuum_dispersion(kind):
+    """a wave tilted in x, R_y ~ cos(k_x x) cos(kappa zeta), evolves as exp(-i omega s) with
+    omega = 2 kappa k^2 / (4 kappa^2 - k^2) and keeps its amplitude (tests/test_fields.cpp:103-148)"""
+    kx, kappa = 0.1, 1.0
+    h = 2 * math.pi / kappa / 32
+    d = base(16, 4, 1025, 2 * math.pi / kx, 10.0, h * 1024)
+    w = 6 * 2 * math.pi
+    zc0 = 0.5 * h * 1024
+    if kind == "zeta":
+        a1 = 40.0
+        a2 = a1 + 30 * h * 4
+        b1 = a2 + 8 * w
+        b2 = b1 + 30 * h * 4
+        d.zeta_table([0, a1, a2, b1, b2], [4 * h, 4 * h, h, h, 4 * h], b2 + 60)
+        zc0 = 0.5 * (a2 + b1)
+    if kind == "y":
+        d.y_table([0.0], [2.5], 5.0)
+    d.set(self_fields=1, dz=10.0, nmacro=0)
+    sim = g3d_amd.Sim(d)
+    info = sim.info()
+    N, nk, nx, ny = info["N"], info["nk"], d.c.nx, d.c.ny
+    fourier = bool(d.c.y_fourier)
+    R = np.zeros(N * nk, dtype=np.complex128)
+    for n, z in enumerate(d.zeta):
+        v = 0.5 * math.exp(-((z - zc0) / w) ** 2) * math.cos(kappa * z)
+        for j in range(1 if fourier else ny):
+            R[n * nk + j * nx + 1] = R[n * nk + j * nx + nx - 1] = v
+    sim.set("Ry", R)
+    sim.call("solve_fields", 0.0)
+    ws = sim.array("wsl")
+    jm = 0 if fourier else 1
+    ph = np.exp(-1j * kappa * d.zeta)
+
+    def demod():
+        r = sim.array("Ry").reshape(N, nk)[:, jm * nx + 1]
+        return complex((ws * r * ph).sum())
+
+    a0 = demod()
+    steps = 40
+    for _ in range(steps):
+        sim.call("solve_fields", 0.5 * d.c.dz)
+    a1 = demod()
+    om = -np.angle(a1 / a0) / (steps * d.c.dz)
+    k2 = kx * kx
+    close(om, 2 * kappa * k2 / (4 * kappa * kappa - k2), 0.01, "omega of a tilted forward wave (%s)" % kind)
+    close(abs(a1), abs(a0), 2e-3, "amplitude kept over 40 steps (%s)" % kind)
+
+
+@pytest.mark.parametrize("extrap", [1, 0])
+def test_plasma_wake_1d(extrap):
+    """1D linear wake of a Gaussian pulse (a0 = 0.05, n0 = 0.01 n_c, transversely uniform) against the cold-fluid
+    theory E'' + kp^2 E = -kp^2 d/dzeta <a^2>/2: amplitude and plasma wavelength behind the pulse after 3 steps
+    (tests/test_plasma.cpp:7-53)"""
+    h = 2 * math.pi / 32
+    N = int(300 / h) + 1
+    d = g3d_amd.Deck(LAMBDA_TEST)
+    d.c.nx = 4; d.c.Lx = 40.0
+    d.y_uniform(4, 40.0)
+    d.zeta_uniform(N, h * (N - 1))
+    Lw = h * (N - 1)
+    d.set(sponge=0.0, sponge_y=0.0, move_x=0, move_y=0, plasma_on=1, n0=0.01, ppc_x=1, ppc_y=1,
+          predictor_extrapolate=extrap, nmacro=0, dz=25.0, dz_push=25.0, zeta_center=0.75 * Lw)
+    tau = 2 * math.pi / math.sqrt(0.01) * 0.4
+    zeta_c = 240.0
+    d.add_grid_laser(a0=0.05, waist=1e7, tau=tau, zeta_c=zeta_c)
+    sim = g3d_amd.Sim(d)
+    sim.init()
+    for _ in range(3):
+        sim.step()
+    info = sim.info()
+    ns, nk = info["ns"], info["nk"]
+    F = sim.array("F")
+    kp = math.sqrt(0.01)
+    zeta = d.zeta
+    ez = np.array([field(F, ns, nk, 2, n).mean() for n in range(N)])
+    emax = np.abs(ez[zeta < 200]).max()
+
+    def a2(z):
+        dd = z - zeta_c
+        e = np.exp(-2 * math.log(2.0) * dd * dd / (tau * tau))
+        return 0.5 * 0.05 * 0.05 * e * e
+
+    rmax = 0.0
+    d2 = h / 4
+    for n in range(0, N, 4):
+        z = zeta[n]
+        if z >= 200:
+            break
+        zp = np.arange(z, Lw, d2)
+        E = (np.sin(kp * (zp - z)) / kp * (-kp * kp * (a2(zp + 1e-3) - a2(zp - 1e-3)) / 2e-3 / 2) * d2).sum()
+        rmax = max(rmax, abs(E))
+    z1 = z2 = -1.0
+    cnt = 0
+    for n in range(N - 2, -1, -1):
+        if zeta[n] > 200:
+            continue
+        if (ez[n] > 0) != (ez[n + 1] > 0):
+            zx = zeta[n] + (zeta[n + 1] - zeta[n]) * ez[n] / (ez[n] - ez[n + 1])
+            if z1 < 0:
+                z1 = zx
+            z2 = zx
+            cnt += 1
+    lam = 2 * (z1 - z2) / (cnt - 1)
+    close(emax, rmax, 0.02 if extrap else 0.03, "1D wake amplitude against linear theory")
+    close(lam, 2 * math.pi / kp, 0.005, "plasma wavelength 2 pi / k_p")
+
+
+def test_smoothing_commutes_with_divergence():
+    """the smoothing (1 - a^2 lap_perp)^-1 of the streaming current commutes with the divergence on the free-hand y
+    grid: N div(J_a) = div(J_0) with N = 1 + a^2 (k_x^2 - b-b+) on the nodes (tests/test_plasma.cpp:136-176)"""
+    def deck(a):
+        d = g3d_amd.Deck(LAMBDA_TEST)
+        d.c.nx = 8; d.c.Lx = 20.0
+        d.y_table([0.0, 5.0, 15.0], [1.0, 1.0, 3.0], 20.0)
+        d.zeta_uniform(8, 7.0)
+        d.set(plasma_on=1, n0=0.01, k_filter=0, sponge=0.0, sponge_y=0.0, move_x=0, nmacro=0, dz=10.0,
+              smooth_length=a)
+        return d
+
+    a = 2.0
+    d0 = deck(0.0)
+    nx, ny = d0.c.nx, d0.c.ny
+    nk = nx * ny
+    src = np.random.default_rng(5).uniform(-1, 1, 5 * nk)
+    src[3 * nk:] = 0.0
+    J = []
+    for s in (0.0, a):
+        sim = g3d_amd.Sim(deck(s))
+        sim.set("jsr", src)
+        sim.call("stream_spectra")
+        J.append(sim.array("jsn"))
+    kx = sim.array("kxs")
+    ia, idl = sim.array("iA"), sim.array("iD")
+    res = scale = 0.0
+    for i in range(nx):
+        def divv(A, j):      # node j: i k_x J_x + (J_y(j) - J_y(j-1)) / A_j
+            return 1j * kx[i] * A[j * nx + i] + (A[nk + j * nx + i] - A[nk + (j - 1) * nx + i]) * ia[j]
+        d0v = np.zeros(ny + 1, dtype=complex)
+        dav = np.zeros(ny + 1, dtype=complex)
+        for j in range(1, ny):
+            d0v[j] = divv(J[0], j)
+            dav[j] = divv(J[1], j)
+        for j in range(1, ny):
+            lap = ((dav[j + 1] - dav[j]) * idl[j] - (dav[j] - dav[j - 1]) * idl[j - 1]) * ia[j]
+            Nd = dav[j] + a * a * (kx[i] * kx[i] * dav[j] - lap)
+            res = max(res, abs(Nd - d0v[j]))
+            scale = max(scale, abs(d0v[j]))
+    assert scale > 0
+    assert res < 1e-12 * scale, res / scale
+
+
+@pytest.mark.parametrize("kind", ["uniform", "y", "zeta"])
+def test_gauss_law_long_bunch(kind):
+    """the field of a long round Gaussian bunch against E_r = lambda (1 - exp(-r^2/2 sigma^2)) / (2 pi r), with the
+    TSC widening (tests/test_bunch.cpp:36-86)"""
+    sig, sz = 6.0, 60.0
+    Lw = 8 * sz
+    d = base(128, 128, 129, 40 * sig, 40 * sig, Lw)
+    if kind == "y":
+        d.y_table([0.0, 4.0, 30.0], [1.5, 1.5, 6.0], 20 * sig)
+    if kind == "zeta":
+        d.zeta_table([0.0, 200.0, 300.0], [4.0, 4.0, 1.5], Lw)
+    d.set(gamma=1000.0, sigx=sig, sigy=sig, sigz=sz, charge=1e-12, nmacro=400000, zeta_center=0.5 * Lw, dz=10.0,
+          k_filter=0, emitx=0.0, emity=0.0, espread=0.0)
+    sim = g3d_amd.Sim(d)
+    sim.init()
+    p = sim.array("parts")
+    Qcode = -p["w"].sum()
+    lam = Qcode / (math.sqrt(2 * math.pi) * sz)
+    info = sim.info()
+    ns, nk, nx, ny, dx = info["ns"], info["nk"], d.c.nx, d.c.ny, info["dx"]
+    zeta, ynode = d.zeta, d.ynode
+    n0 = int(np.argmin(np.abs(zeta - 0.5 * Lw)))
+    lam_n = sum(0.2 * lam * math.exp(-(zeta[n] - 0.5 * Lw) ** 2 / (2 * sz * sz)) for n in range(n0 - 2, n0 + 3))
+    F = sim.array("F")
+
+    def at(comp, i, j):
+        return sum(0.2 * field(F, ns, nk, comp, n)[j * nx + i] for n in range(n0 - 2, n0 + 3))
+
+    def Er(r):
+        j = int(np.argmin(np.abs(ynode - r)))
+        dyl = ynode[min(j + 1, ny)] - ynode[max(j - 1, 0)]
+        s2 = sig * sig + (dx * dx + 0.25 * dyl * dyl) / 8
+        return lam_n * (1 - math.exp(-r * r / (2 * s2))) / (2 * math.pi * r)
+
+    jc, ic = ny // 2, nx // 2
+    fourier = bool(d.c.y_fourier)
+    yh = np.array([ynode[j] if fourier else 0.5 * (ynode[j] + ynode[j + 1]) for j in range(ny)])
+    for rr in (1.0, 2.0):
+        k = int(round(rr * sig / dx))
+        Ex = 0.5 * (at(0, ic + k, jc) - at(0, ic - k, jc))
+        close(Ex, Er(k * dx), 0.02, "%s: E_x at x = %.0f sigma" % (kind, rr))
+        jp = int(np.argmin(np.abs(yh - rr * sig)))
+        jm = ny - jp if fourier else ny - 1 - jp
+        Ey = 0.5 * (at(1, ic, jp) - at(1, ic, jm))
+        close(Ey, Er(yh[jp]), 0.02, "%s: E_y at y = %.0f sigma" % (kind, rr))
+    k = int(round(sig / dx))
+    By = 0.5 * (at(4, ic + k, jc) - at(4, ic - k, jc))
+    close(By, Er(k * dx), 0.02, "%s: B_y at x = 1 sigma" % kind)
+
+
+@pytest.mark.parametrize("freehand", [False, True])
+def test_bunch_equilibrium(freehand):
+    """a relativistic bunch in its own field (no laser) keeps its mean energy over 100 steps (tests/test_bunch.cpp:
+    88-110; the field energy of that test is a diagnostic the engine does not output, so its mean gamma is checked)"""
+    sig = sz = 6.0
+    Lw = 16 * sz
+    d = base(64, 64, 257, 30 * sig, 30 * sig, Lw)
+    if freehand:
+        d.y_table([0.0, 15.0, 40.0], [1.5, 1.5, 6.0], 15 * sig)
+        d.zeta_table([0.0, 30.0, 45.0], [1.0, 1.0, 0.375], Lw)
+    d.set(gamma=1000.0, sigx=sig, sigy=sig, sigz=sz, charge=10e-12, nmacro=50000, zeta_center=0.6 * Lw, dz=50.0,
+          dz_push=50.0, z_end=1e9, emitx=0.0, emity=0.0, espread=0.0)
+    sim = g3d_amd.Sim(d)
+    sim.init()
+
+    def mean_g():
+        p = sim.array("parts")
+        live = p[p["alive"] == 1]
+        return float(np.sqrt(1 + live["px"] ** 2 + live["py"] ** 2 + live["pz"] ** 2).mean())
+
+    g0 = mean_g()
+    for _ in range(100):
+        sim.step()
+    g1 = mean_g()
+    assert abs(g1 - g0) / g0 < 1e-6, (g1 - g0) / g0
+
+
+# NIST ionization energies [eV] (the reference's src/atomic_data.cpp:29-36) and masses [u]
+IP = {"H": [13.598434599702], "N": [14.53413, 29.60125, 47.4453, 77.4735, 97.8901, 552.06733, 667.04612]}
+MASS_U = {"H": 1.00794, "N": 14.0067}
+
+
+def ionization_run(el, Z0, Zmax, mobile, a0):
+    """tests/test_ionization.cpp:38-82: atoms at 1e-6 n_c loaded at the front of the window, the analytic
+    s-polarised plane-wave pulse (c tau = 30/k0, lambda0 = 0.8 um) centred in the window; after one step the
+    fractions of the charge states against the rate equations along the path of an atom at rest"""
+    import refpy
+    tau = 30.0
+    h = 2 * math.pi / 40
+    N = int(6 * tau / h) + 1
+    d = g3d_amd.Deck(0.8e-6)
+    d.c.nx = 8; d.c.Lx = 16.0
+    d.y_uniform(8, 16.0)
+    d.zeta_uniform(N, h * (N - 1))
+    Lw = h * (N - 1)
+    d.set(sponge=0.0, sponge_y=0.0, move_x=0, move_y=0, plasma_on=1, n0=1e-6, ppc_x=8, ppc_y=8,
+          free_electrons=int(Z0 > 0), nmacro=0, dz=10.0, dz_push=10.0, z_end=10.0, laser_on=1, a0=a0, kL=1.0,
+          tau=tau, theta=0.0, laser_pol=0, cep=0.0, zeta_center=0.5 * Lw, start_fwhm=0.0, stop_fwhm=0.0)
+    d.add_ion(MASS_U[el] * 1822.888486, Z0, Zmax, atoms=1.0 / Z0 if Z0 > 0 else 1.0, mobile=mobile, ionization=True,
+              ip_eV=IP[el])
+    sim = g3d_amd.Sim(d)
+    sim.init()
+    sim.step()
+    h0 = sim.array("ion_stats0")
+    tot = h0[:Zmax + 1].sum()
+    frac = [h0[z] / tot for z in range(Z0, Zmax + 1)]
+    c = 299792458.0
+    Lu = 0.8e-6 / (2 * math.pi)
+    omega0 = c / Lu
+    E0 = 9.1093837015e-31 * c * omega0 / 1.602176634e-19
+
+    def A(et):
+        return a0 * np.exp(-2 * math.log(2.0) * et * et / (tau * tau)) * np.cos(et)
+
+    n = np.zeros(Zmax - Z0 + 1)
+    n[0] = 1
+    dt, eps = h / 50, 1e-4
+    ets = np.arange(-(Lw - 0.5 * Lw), 0.5 * Lw, dt)
+    es = np.abs(-(A(ets + 0.5 * dt + eps) - A(ets + 0.5 * dt - eps)) / (2 * eps)) * E0
+    rate = refpy.lib().ref_adk_rate_SI          # the reference's ADK rate in SI (checked against Landau's form)
+    for e in es:
+        w = [rate(IP[el][k], k + 1, float(e)) * dt / omega0 for k in range(Z0, Zmax)]
+        if Zmax - Z0 == 1:
+            dd = n[0] * (1 - math.exp(-w[0]))
+            n[0] -= dd
+            n[1] += dd
+        else:
+            a, b = w[0], w[1]
+            ea, eb = math.exp(-a), math.exp(-b)
+            n1 = n[1] * eb + (n[0] * a * (ea - eb) / (b - a) if abs(b - a) > 1e-12 else n[0] * a * ea)
+            n0n = n[0] * ea
+            n[2] += (n[0] - n0n) + (n[1] - n1)
+            n[0] = n0n
+            n[1] = n1
+    return frac, n
+
+
+def test_field_ionization_rate_equations():
+    """Monte Carlo ADK ionization against the rate equations (tests/test_ionization.cpp:83-90): hydrogen, fixed
+    atoms, a0 = 0.01; N5+ -> N6+ -> N7+, mobile ions, a0 = 2"""
+    f, ref = ionization_run("H", 0, 1, False, 0.01)
+    close(f[1], ref[1], 0.03, "H ionized fraction (fixed atoms, a0 = 0.01)")
+    f, ref = ionization_run("N", 5, 7, True, 2.0)
+    close(f[0], ref[0], 0.05, "N5+ fraction after the pulse (a0 = 2)")
+    close(f[1], ref[1], 0.03, "N6+ fraction after the pulse")
+    close(f[2], ref[2], 0.10, "N7+ fraction after the pulse")

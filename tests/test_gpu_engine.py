@@ -170,3 +170,26 @@ def test_device_lattice_matches_host(monkeypatch):
     assert len(pa) == len(pb) == 32 * 32 * 4
     for f in pa.dtype.names:
         np.testing.assert_array_equal(pa[f], pb[f], err_msg=f)
+
+
+def test_window_shift_follows_bunch():
+    """the transverse window follows an off-centre bunch (src/sim_setup.cpp:221-280): window_lag 1, the bunch moved
+    4.6 cells in x and -3.2 cells in y after init, so step 2 shifts the window by (5, -3) cells (phase factors on the
+    spectra, the roll of F, the particles moved); X0, Y0, particles and fields after 4 steps against the reference"""
+    mk = lambda: bunch_deck().set(window_lag=1)
+    gpu, ref = g3d_amd.Sim(mk()), refpy.RefSim(mk())
+    gpu.init(); ref.init()
+    i0 = ref.info()
+    p = ref.array("parts")
+    p["x"] += 4.6 * i0["dx"]
+    p["y"] -= 3.2 * i0["dy"]
+    gpu.set_particles(p); ref.set_particles(p)
+    for _ in range(4):
+        gpu.step(); ref.step()
+    ig, ir = gpu.info(), ref.info()
+    assert ir["X0"] == i0["X0"] + 5 * i0["dx"] and ir["Y0"] == i0["Y0"] - 3 * i0["dy"], (ir["X0"], ir["Y0"])
+    assert (ig["X0"], ig["Y0"]) == (ir["X0"], ir["Y0"])
+    compare_fields(gpu, ref)
+    pa, pb = gpu.array("parts"), ref.array("parts")
+    np.testing.assert_array_equal(pa["alive"], pb["alive"])
+    particle_parity(pa, pb, mk())
