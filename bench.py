@@ -329,10 +329,11 @@ def cpu_baseline(threads, witness=True):
 
 
 def stream_bytes_mixed(nstrm, nk):
-    """Design HBM bytes of one launch of k_stream_mixed (what the kernel reads and writes): per particle 160 B (read
-    x y zeta z px py pz w q qm + alive, write x y zeta z px py pz q qm + alive; the state stays FP64 in HBM), per
-    point the F gather of slices n, n+1 (48 B) and the 5 float current planes (20 B)."""
-    return nstrm * 160.0 + nk * 68.0
+    """Design HBM bytes of one launch of k_stream_mixed (what the kernel reads and writes): per particle 144 B (read
+    x y zeta z px py pz w q qm + alive, write x y zeta z px py pz + alive; the state stays FP64 in HBM, q and qm are
+    constant for its electrons), per point the F gather of slices n, n+1 (48 B) and the 5 float current planes
+    (20 B)."""
+    return nstrm * 144.0 + nk * 68.0
 
 
 # Algorithmic HBM bytes of the streaming half-cell pass (the compulsory traffic of the march): per push of a

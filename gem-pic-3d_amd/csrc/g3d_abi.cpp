@@ -27,6 +27,7 @@ bool lookup(g3d::Sim& s, const std::string& n, void*& dev, size_t& bytes) {
         {"jw", {s.jw.data(), bytes_of(s.jw)}}, {"F", {s.F.data(), bytes_of(s.F)}},
         {"jsr", {s.jsr.data(), bytes_of(s.jsr)}}, {"jsn", {s.jsn.data(), bytes_of(s.jsn)}},
         {"rho_lines", {s.rho_lines.data(), bytes_of(s.rho_lines)}},
+        {"rho_lines_i", {s.rho_lines_i.data(), bytes_of(s.rho_lines_i)}},
         // the march geometry (src/sim_setup.cpp:5-60, 148-164): k vectors with the Nyquist mode zeroed, the
         // half-cell lengths and the two-point current weights of each slice
         {"kx", {s.kxd.data(), bytes_of(s.kxd)}}, {"ky", {s.kyd.data(), bytes_of(s.kyd)}},
@@ -36,7 +37,8 @@ bool lookup(g3d::Sim& s, const std::string& n, void*& dev, size_t& bytes) {
         {"kxs", {s.kxs.data(), bytes_of(s.kxs)}}, {"kys", {s.kys.data(), bytes_of(s.kys)}},
         {"filt", {s.filt.data(), bytes_of(s.filt)}}, {"filty", {s.filty.data(), bytes_of(s.filty)}},
         {"zs", {s.zs.data(), bytes_of(s.zs)}}, {"iA", {s.iA.data(), bytes_of(s.iA)}},
-        {"iD", {s.iD.data(), bytes_of(s.iD)}}, {"wsl", {s.wsl.data(), bytes_of(s.wsl)}}};
+        {"iD", {s.iD.data(), bytes_of(s.iD)}}, {"wsl", {s.wsl.data(), bytes_of(s.wsl)}},
+        {"maskx", {s.maskx.data(), bytes_of(s.maskx)}}, {"masky", {s.masky_n.data(), bytes_of(s.masky_n)}}};
     auto it = m.find(n);
     if (it == m.end()) return false;
     dev = it->second.first; bytes = it->second.second;
@@ -130,6 +132,7 @@ int g3d_sim_array_size(g3d_sim* sim, const char* name, size_t* bytes) {
         if (n == "parts") *bytes = size_t(s.np) * sizeof(g3d_particle);
         else if (n == "strm") *bytes = size_t(s.nstrm) * sizeof(g3d_particle);
         else if (const std::vector<double>* h = ion_stats_of(s, n)) *bytes = h->size() * sizeof(double);
+        else if (n == "laser") *bytes = 9 * sizeof(double);
         else if (!lookup(s, n, d, *bytes)) { g3d::set_error("unknown array " + n); rc = 3; }
     });
     return g ? g : rc;
@@ -144,6 +147,13 @@ int g3d_sim_get(g3d_sim* sim, const char* name, void* host, size_t bytes) {
             const std::vector<g3d_particle> v = n == "parts" ? s.host_particles() : s.host_strm();
             if ((rc = check_size(v.size() * sizeof(g3d_particle), bytes))) return;
             std::memcpy(host, v.data(), bytes);
+            return;
+        }
+        if (n == "laser") {          // the analytic plane wave: on, pol, a0, kL, tau, cep, eta_c, kx, kz
+            const g3d::PlaneWave& L = s.laser;
+            const double v[9] = {double(L.on), double(L.pol), L.a0, L.kL, L.tau, L.cep, L.eta_c, L.kx, L.kz};
+            if ((rc = check_size(sizeof v, bytes))) return;
+            std::memcpy(host, v, sizeof v);
             return;
         }
         if (const std::vector<double>* h = ion_stats_of(s, n)) {

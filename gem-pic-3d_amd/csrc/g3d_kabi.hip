@@ -159,6 +159,44 @@ int g3d_bunch_currents(const g3d_spec_ctx* c, const g3d_axis* zax, const double*
     });
 }
 
+int g3d_sponge_real_fields(const g3d_spec_ctx* c, int sponge, const double* maskx, const double* masky,
+                           double beta_b, double gm2_b, g3d_cpx* Rx, g3d_cpx* Ry, const g3d_cpx* Lx, const g3d_cpx* Ly,
+                           const g3d_cpx* Ez, const g3d_cpx* Bz, const g3d_cpx* jkz, float* F, g3d_cpx* scratch) {
+    return g3d::guard([&] {
+        g3d::need(c, "spectral context"); g3d::need(Rx, "Rx"); g3d::need(Ry, "Ry"); g3d::need(Lx, "Lx");
+        g3d::need(Ly, "Ly"); g3d::need(Ez, "Ez"); g3d::need(Bz, "Bz"); g3d::need(F, "F"); g3d::need(scratch, "scratch");
+        if (sponge) { g3d::need(jkz, "jkz"); g3d::need(maskx, "maskx"); g3d::need(masky, "masky"); g3d::need(c->hcg, "hcg"); }
+        const g3d::SpecGeom g = g3d::geom_of(*c);
+        const long ts = long(g.ns) * g.nk;
+        g3d::cx* S1 = scratch;
+        g3d::cx* S2 = scratch + ts;
+        g3d::cx* T3 = scratch + 2 * ts;
+        const double opb = 1 + beta_b;
+        if (sponge) {
+            g3d::sponge_pack(g, Rx, Ry, jkz, c->hcg, c->wlo, c->whi, gm2_b, opb, S1, S2);
+            g3d::fft_run(c->plan->p, S1, g.ns, true);
+            g3d::sponge_mask(g, S1, maskx, masky, masky, g.ns);
+            g3d::fft_run(c->plan->p, S1, g.ns, false);
+        }
+        g3d::fields_pack3(g, sponge ? S1 : nullptr, S2, opb, Rx, Ry, Lx, Ly, Ez, Bz, T3, ts, g.ns);
+        g3d::fft_run(c->plan->p, T3, 3 * g.ns, true);
+        g3d::fields_to_F(T3, ts, F, long(g.ns + 2) * g.nk, g.nk, g.ns);
+    });
+}
+
+int g3d_comoving(const g3d_spec_ctx* c, const g3d_cpx* jkz, double beta, double gm2, g3d_cpx* Rx, g3d_cpx* Ry,
+                 g3d_cpx* Ez, g3d_cpx* Bz, g3d_cpx* scratch) {
+    return g3d::guard([&] {
+        g3d::need(c, "spectral context"); g3d::need(jkz, "jkz"); g3d::need(Rx, "Rx"); g3d::need(Ry, "Ry");
+        g3d::need(Ez, "Ez"); g3d::need(Bz, "Bz"); g3d::need(scratch, "scratch"); g3d::need(c->hcg, "hcg");
+        const g3d::SpecGeom g = g3d::geom_of(*c);
+        g3d::cx* S1 = scratch;
+        g3d::cx* S2 = scratch + long(g.ns) * g.nk;
+        g3d::comoving_solve(g, jkz, c->hcg, c->wlo, c->whi, beta, gm2, S1, S2);
+        g3d::comoving_R(g, S2, beta, Rx, Ry, Ez, Bz);
+    });
+}
+
 int g3d_bunch_centroid(const g3d_parts* p, long np, double X0, double Y0, double* out) {
     return g3d::guard([&] {
         g3d::need(p, "particles");

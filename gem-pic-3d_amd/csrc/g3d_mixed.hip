@@ -241,9 +241,11 @@ __device__ __forceinline__ void stream_one_mixed(const StreamArgs& c, const Half
     const double px = s.px, py = s.py, pz = s.pz;
     const double q = s.q, qm = s.qm, w = s.w;
     int alive = s.alive;
+    // q and qm are not written: this form streams electrons without ionization, so no push changes them, and every
+    // loader (the lattice, the host list, the joining particles) writes the same values into both state buffers
     auto store = [&](double ux, double uy, double uz) {
         D.x[i] = x; D.y[i] = y; D.zeta[i] = zeta; D.px[i] = ux; D.py[i] = uy; D.pz[i] = uz;
-        D.z[i] = z; D.q[i] = q; D.qm[i] = qm; D.alive[i] = alive;
+        D.z[i] = z; D.alive[i] = alive;
     };
     if (alive == 6) {
         if (zeta > zn) alive = 1;

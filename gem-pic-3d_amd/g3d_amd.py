@@ -261,6 +261,8 @@ def _load():
         "g3d_bunch_currents": ([P, P, P, P, C.c_double, C.c_double, C.c_int, C.c_int, P, C.c_long, P, P, P, P, P],
                                C.c_int),
         "g3d_bunch_centroid": ([P, C.c_long, C.c_double, C.c_double, P], C.c_int),
+        "g3d_sponge_real_fields": ([P, C.c_int, P, P, C.c_double, C.c_double, P, P, P, P, P, P, P, P, P], C.c_int),
+        "g3d_comoving": ([P, P, C.c_double, C.c_double, P, P, P, P, P], C.c_int),
         "g3d_window_shift": ([P, C.c_int, C.c_int, C.c_double, C.c_double, P, P, C.c_int, P, P, P, C.c_long],
                              C.c_int),
         "g3d_sim_create": ([P, C.POINTER(P)], C.c_int), "g3d_sim_destroy": ([P], C.c_int),
@@ -307,11 +309,11 @@ def check(rc):
         raise RuntimeError(lib().g3d_last_error().decode() or lib().g3d_status_string(rc).decode())
 
 
-ARRAY_DTYPES = {"F": np.float32, "jw": np.float64, "jsr": np.float64, "rho_lines": np.float64, "kx": np.float64,
+ARRAY_DTYPES = {"F": np.float32, "jw": np.float64, "jsr": np.float64, "rho_lines": np.float64, "rho_lines_i": np.float64, "kx": np.float64,
                 "ky": np.float64, "hcg": np.float64, "wlo": np.float64, "whi": np.float64, "kxs": np.float64,
                 "kys": np.float64, "filt": np.float64, "filty": np.float64, "zs": np.float64, "iA": np.float64,
-                "iD": np.float64, "wsl": np.float64, "ion_stats0": np.float64, "ion_stats1": np.float64,
-                "ion_stats2": np.float64, "ion_stats3": np.float64,
+                "iD": np.float64, "wsl": np.float64, "maskx": np.float64, "masky": np.float64, "ion_stats0": np.float64, "ion_stats1": np.float64,
+                "ion_stats2": np.float64, "ion_stats3": np.float64, "laser": np.float64,
                 "parts": PARTICLE, "strm": PARTICLE, "born": PARTICLE}
 
 

@@ -341,6 +341,22 @@ int g3d_bunch_currents(const g3d_spec_ctx* c, const g3d_axis* zax, const double*
                        double dy, int zfilter, int zuniform, const g3d_parts* p, long np, double* jw, g3d_cpx* jkx,
                        g3d_cpx* jky, g3d_cpx* jkz, g3d_cpx* scratch);
 
+/* Simulation::sponge_and_real_fields (src/sim_fields.cpp:156-307, Fourier y): with sponge != 0 the bound field of
+ * the beam (jkz, the beam's co-moving beta_b and gamma_b^-2 = gm2_b) is taken out of R, R is damped by the real-space
+ * masks maskx [nx] and masky [ny] (sim_setup.cpp: 1 - rate (1 - d/w)^2 in the sponge layers) and the bound field
+ * added back; then the real fields of all ns slices into F [6][ns+2][nk].  R_x, R_y: ns planes (rewritten); L_x,
+ * L_y, E_z, B_z: ns planes; jkz: ns + 1; scratch: 5 ns nk complex values */
+int g3d_sponge_real_fields(const g3d_spec_ctx* c, int sponge, const double* maskx, const double* masky,
+                           double beta_b, double gm2_b, g3d_cpx* Rx, g3d_cpx* Ry, const g3d_cpx* Lx, const g3d_cpx* Ly,
+                           const g3d_cpx* Ez, const g3d_cpx* Bz, const g3d_cpx* jkz, float* F, g3d_cpx* scratch);
+
+/* Simulation::comoving_field (src/sim_fields.cpp:44-151, Fourier y): the start field of the beam's half-cell charge
+ * jkz (ns + 1 planes) moving at beta (gm2 = 1 - beta^2; the reference's beta is the weighted mean v_z of the live
+ * beam particles, computed on the host): the Thomas solve along zeta per mode of -(k^2 + gm2 d^2/dzeta^2) psi = rho
+ * and R = (1 + beta) (-grad psi), E_z, B_z of the slices; scratch: 2 ns nk complex values */
+int g3d_comoving(const g3d_spec_ctx* c, const g3d_cpx* jkz, double beta, double gm2, g3d_cpx* Rx, g3d_cpx* Ry,
+                 g3d_cpx* Ez, g3d_cpx* Bz, g3d_cpx* scratch);
+
 /* the window centroid of src/sim_setup.cpp:221-234: sums over the live bunch particles of w (x + X0), w (y + Y0)
  * and w into out[3] (host), in a fixed tree order; the partial sums are library scratch */
 int g3d_bunch_centroid(const g3d_parts* p, long np, double X0, double Y0, double* out);

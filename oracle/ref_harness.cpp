@@ -2,8 +2,11 @@
 // g3d_config POD, owns a reference Simulation and exposes its member functions and arrays over a C ABI, so that
 // the tests can run the reference and the MI355X library on the same inputs.  Compiled against the sources under
 // /root/reference (never copied); only tests/, smoke() and bench.py's CPU baseline load the result.
+#include "deck.hpp"
 #include "sim.hpp"
 #include "sim_stream.hpp"
+#include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <stdexcept>
@@ -354,6 +357,71 @@ int ref_adk_levels(const double* ip_eV, int n_ip, int Z0, int Zmax, double E0, d
     } catch (const std::exception& e) { return fail(e); }
 }
 double ref_adk_rate_SI(double U_eV, int Zafter, double E) { return adk_rate_SI(U_eV, Zafter, E); }
+// the reference's own deck reader (src/deck.cpp Deck::read, Deck::override_from; src/sim_config.cpp:69-343
+// Config::from_deck) into the g3d_config POD; zeta (zcap) and ynode (ycap) receive the grid tables, out[] the
+// main output group: every, beam_every, fields, beams, axis, particles, frames, slice_length, checkpoint_every,
+// walltime_hours; dir (dcap bytes) the output folder
+int ref_deck_config(const char* path, const char* const* ov, int nov, g3d_config* g, double* zeta, int zcap,
+                    double* ynode, int ycap, double* out, char* dir, int dcap) {
+    try {
+        Deck d = Deck::read(path);
+        for (int i = 0; i < nov; ++i) d.override_from(ov[i]);
+        const Config c = Config::from_deck(d);
+        if (c.nzeta > zcap || c.ny + 1 > ycap) throw std::runtime_error("grid tables larger than the buffers");
+        if (c.glasers.size() > 4 || c.ions.size() > 4 || c.pz_points.size() > 16)
+            throw std::runtime_error("more lasers, ions or ramp points than the POD holds");
+        std::memset(g, 0, sizeof *g);
+        g->lambda0 = c.lambda0;
+        g->nx = c.nx; g->ny = c.ny; g->nzeta = c.nzeta; g->Lx = c.Lx; g->Ly = c.Ly; g->y_fourier = c.y_fourier;
+        std::copy(c.zeta.begin(), c.zeta.end(), zeta); std::copy(c.ynode.begin(), c.ynode.end(), ynode);
+        g->zeta = zeta; g->ynode = ynode;
+        g->dz = c.dz; g->z_end = c.z_end; g->dz_push = c.dz_push;
+        g->sponge = c.sponge; g->sponge_y = c.sponge_y; g->sponge_rate = c.sponge_rate;
+        g->move_x = c.move_x; g->move_y = c.move_y; g->window_lag = c.window_lag;
+        g->self_fields = c.self_fields; g->zeta_filter = c.zeta_filter; g->k_filter = c.k_filter;
+        g->push_phase = c.push_phase;
+        g->laser_on = c.laser_on; g->laser_pol = c.pol == "s" ? 0 : c.pol == "p" ? 1 : 2;
+        g->a0 = c.a0; g->kL = c.kL; g->theta = c.theta; g->tau = c.tau; g->cep = c.cep;
+        g->start_fwhm = c.start_fwhm; g->stop_fwhm = c.stop_fwhm;
+        g->charge = c.charge; g->gamma = c.gamma; g->sigx = c.sigx; g->sigy = c.sigy; g->sigz = c.sigz;
+        g->emitx = c.emitx; g->emity = c.emity; g->espread = c.espread; g->zeta_center = c.zeta_center;
+        g->nmacro = c.nmacro; g->seed = c.seed;
+        g->n_glasers = int(c.glasers.size());
+        for (int k = 0; k < g->n_glasers; ++k) {
+            const Config::GridLaser& L = c.glasers[k];
+            g3d_glaser& s = g->glasers[k];
+            s.a0 = L.a0; s.kL = L.kL; s.waist = L.waist; s.tau = L.tau; s.zeta_c = L.zeta_c; s.z_focus = L.z_focus;
+            s.cep = L.cep; s.pol_angle = L.pol_angle; s.circular = L.circular; s.helicity = L.helicity;
+        }
+        g->plasma_on = c.plasma_on; g->n0 = c.n0; g->r_max = c.r_max; g->ppc_x = c.ppc_x; g->ppc_y = c.ppc_y;
+        g->smooth_length = c.smooth_length;
+        g->n_pz = int(c.pz_points.size());
+        std::copy(c.pz_points.begin(), c.pz_points.end(), g->pz_points);
+        std::copy(c.pz_values.begin(), c.pz_values.end(), g->pz_values);
+        g->parabolic = c.parabolic; g->channel_depth = c.channel_depth; g->r_channel = c.r_channel;
+        g->iterations = c.iterations; g->vz_min = c.vz_min; g->vz_beam = c.vz_beam; g->trap_advance = c.trap_advance;
+        g->predictor_extrapolate = c.predictor_extrapolate; g->free_electrons = c.free_electrons;
+        g->n_ions = int(c.ions.size());
+        for (int k = 0; k < g->n_ions; ++k) {
+            const Config::IonSpecies& s = c.ions[k];
+            g3d_ion& i = g->ions[k];
+            if (s.ip_eV.size() > 16) throw std::runtime_error("more ionization levels than the POD holds");
+            i.mass = s.mass; i.atoms = s.atoms; i.Z0 = s.Z0; i.Zmax = s.Zmax; i.mobile = s.mobile;
+            i.ionization = s.ionization; i.n_ip = int(s.ip_eV.size());
+            std::copy(s.ip_eV.begin(), s.ip_eV.end(), i.ip_eV);
+            std::strncpy(i.name, s.name.c_str(), sizeof i.name - 1);
+        }
+        g->freeze_bunch = c.freeze_bunch; g->start_relax = c.start_relax; g->box_mean = c.box_mean;
+        const auto& o = c.groups.at(0);
+        const double v[10] = {double(c.every), double(o.beam_every), double(o.fields), double(o.beams),
+                              double(o.axis), double(c.dump_particles), double(c.frames), c.slice_length,
+                              double(c.checkpoint_every), c.walltime_hours};
+        std::copy(v, v + 10, out);
+        std::snprintf(dir, dcap, "%s", c.dir.c_str());
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
+}
+
 unsigned long long ref_mix64(unsigned long long z) { return mix64(z); }
 // the key chain of the ionization draws (src/ionization.hpp:44-46)
 unsigned long long ref_key_step(unsigned long long seed, long step) { return ion_key_step(seed, step); }

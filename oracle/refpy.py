@@ -45,6 +45,8 @@ def lib():
             "ref_key_step": ([C.c_ulonglong, C.c_long], C.c_ulonglong),
             "ref_key_slice": ([C.c_ulonglong, C.c_int, C.c_int], C.c_ulonglong),
             "ref_key_particle": ([C.c_ulonglong, C.c_ulonglong], C.c_ulonglong),
+            "ref_deck_config": ([C.c_char_p, C.POINTER(C.c_char_p), C.c_int, P, P, C.c_int, P, C.c_int, P, C.c_char_p,
+                                 C.c_int], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -61,6 +63,22 @@ def check(rc):
 
 def set_threads(n):
     lib().ref_set_threads(int(n))
+
+
+def deck_config(path, overrides=(), zcap=1 << 16, ycap=1 << 14):
+    """the reference's own reading of a deck file (Deck::read, overrides, Config::from_deck): (g3d_config, zeta,
+    ynode, output settings as a dict)"""
+    cfg = g3d_amd.Config()
+    zeta, ynode, out = np.zeros(zcap), np.zeros(ycap), np.zeros(10)
+    d = C.create_string_buffer(4096)
+    ov = (C.c_char_p * max(1, len(overrides)))(*[o.encode() for o in overrides])
+    check(lib().ref_deck_config(str(path).encode(), ov, len(overrides), C.byref(cfg), zeta.ctypes.data, zcap,
+                                ynode.ctypes.data, ycap, out.ctypes.data, d, len(d)))
+    keys = ("every", "beam_every", "fields", "beams", "axis", "particles", "frames", "slice_length",
+            "checkpoint_every", "walltime_hours")
+    o = {k: float(v) for k, v in zip(keys, out)}
+    o["dir"] = d.value.decode()
+    return cfg, zeta[:cfg.nzeta].copy(), ynode[:cfg.ny + 1].copy(), o
 
 
 class RefSim(g3d_amd.SimBase):

@@ -114,3 +114,18 @@ def F_slice_errors(Fa, Fb, ns, nk):
     nb = np.sqrt((B ** 2).sum(axis=(0, 2)))
     de = np.sqrt(((A - B) ** 2).sum(axis=(0, 2)))
     return de / np.maximum(nb, 1e-300), nb
+
+
+def config0_deck():
+    """Config 0 at full size: the vacuum deck of 64 x 64 modes over 80/k0, 256 slices of 0.5/k0, 50 steps of 20/k0;
+    the bunch of 2^18 electrons (11.4014 pC: peak 0.01 n_c for sigma = 2, 2, 4/k0), gamma 1000, energy spread 1e-3,
+    emittance 2e-4 um (u_x rms 1e-4), seed 1; the plane wave a0 = 1, p-polarised, tilted 2 mrad, c tau = 30/k0"""
+    d = g3d_amd.Deck(LAMBDA0)
+    d.c.nx = 64; d.c.Lx = 80.0
+    d.y_uniform(64, 80.0)
+    d.zeta_uniform(256, 255 * 0.5)
+    d.set(dz=20.0, z_end=1000.0, sponge=8.0, sponge_y=8.0, move_x=1, move_y=1, charge=11.4014089e-12, gamma=1000.0,
+          sigx=2.0, sigy=2.0, sigz=4.0, emitx=2e-4, emity=2e-4, espread=1e-3, zeta_center=0.5 * 255 * 0.5,
+          nmacro=1 << 18, seed=1, laser_on=1, laser_pol=1, a0=1.0, theta=2e-3, tau=30.0, start_fwhm=0.0,
+          stop_fwhm=0.0)
+    return d

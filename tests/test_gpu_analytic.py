@@ -327,3 +327,170 @@ def test_field_ionization_rate_equations():
     close(f[0], ref[0], 0.05, "N5+ fraction after the pulse (a0 = 2)")
     close(f[1], ref[1], 0.03, "N6+ fraction after the pulse")
     close(f[2], ref[2], 0.10, "N7+ fraction after the pulse")
+
+
+def gamma_of(p):
+    return np.sqrt(1 + p["px"] ** 2 + p["py"] ** 2 + p["pz"] ** 2)
+
+
+@pytest.mark.parametrize("pol", [0, 1])
+def test_plane_wave_invariants(pol):
+    """one electron in the analytic plane wave, no self-fields: h = gamma - p.khat is conserved, the peak gamma follows
+    from h and p_perp + q A, and the electron leaves the pulse with its initial momentum (tests/test_bunch.cpp:5-35;
+    pol 0 s, 1 p)"""
+    d = base(4, 8, 4, 100.0, 200.0, 1e7)
+    d.set(self_fields=0, laser_on=1, a0=5.0, kL=1.0, theta=0.1, laser_pol=pol, tau=150.0, start_fwhm=3.5,
+          stop_fwhm=3.5, push_phase=0.025, gamma=50.0, charge=1e-15, nmacro=1, zeta_center=0.5e7, dz=30.0,
+          dz_push=15.0, z_end=1e9, sigx=1.0, sigy=1.0, sigz=1.0, emitx=1e-3, emity=1e-3, espread=1e-3)
+    sim = g3d_amd.Sim(d)
+    sim.init()
+    on, _, a0, kL, tau, cep, eta_c, kx, kz = sim.array("laser")
+    hlf = lambda p: float(gamma_of(p)[0] - p["px"][0] * kx - p["pz"][0] * kz)
+    p = sim.array("parts")
+    h0, pz0, px0 = hlf(p), float(p["pz"][0]), float(p["px"][0])
+    pp0 = px0 * kz - pz0 * kx
+    dh = gmax = 0.0
+    for _ in range(100000):
+        sim.step()
+        p = sim.array("parts")
+        dh = max(dh, abs(hlf(p) - h0))
+        gmax = max(gmax, float(gamma_of(p)[0]))
+        info = sim.info()
+        s = info["s"]
+        et = (s - (p["zeta"][0] - 0.5e7)) - kx * (p["x"][0] + info["X0"]) - kz * s
+        if et - eta_c > 3.5 * tau:
+            break
+    a = 5.0
+    pmax2 = pp0 * pp0 + a * a if pol == 0 else (abs(pp0) + a) ** 2
+    gref = 0.5 * ((1 + pmax2) / h0 + h0)
+    assert p["alive"][0] == 1 and dh / h0 < 1e-5, dh / h0
+    close(gmax, gref, 2e-3, "peak gamma from h and p_perp - A")
+    close(float(p["pz"][0]), pz0, 1e-5, "p_z after the pulse (Lawson-Woodward)")
+    assert abs(float(p["px"][0]) - px0) < 1e-4
+
+
+@pytest.mark.parametrize("freehand", [False, True])
+def test_sponge_does_not_drag_the_bunch(freehand):
+    """a dense gamma = 50 bunch evolves the same with and without the absorbing sponge (tests/test_bunch.cpp:111-130)"""
+    pz = []
+    for k in range(2):
+        d = base(32, 32, 129, 100.0, 100.0, 64.0)
+        if freehand:
+            d.y_table([0.0, 15.0, 30.0], [2.0, 2.0, 6.0], 90.0)
+        d.set(gamma=50.0, sigx=6.0, sigy=6.0, sigz=6.0, charge=160e-12, nmacro=20000, zeta_center=40.0, dz=30.0,
+              dz_push=30.0, sponge=15.0 if k else 0.0, sponge_y=(30.0 if freehand else 15.0) if k else 0.0,
+              sponge_rate=0.1, z_end=1e9)
+        sim = g3d_amd.Sim(d)
+        sim.init()
+        for _ in range(40):
+            sim.step()
+        p = sim.array("parts")
+        pz.append(float(p["pz"][p["alive"] == 1].mean()))
+    assert abs(pz[1] - pz[0]) / pz[0] < 2e-6, (pz[1] - pz[0]) / pz[0]
+
+
+@pytest.mark.parametrize("yfourier", [True, False])
+def test_constant_step_table_equals_uniform_zeta(yfourier):
+    """a zeta table with a constant step reproduces the uniform zeta grid (tests/test_bunch.cpp:132-153)"""
+    res = []
+    for k in range(2):
+        d = base(32, 32, 129, 100.0, 64.0, 64.0)
+        if not yfourier:
+            d.y_table([0.0, 10.0, 20.0], [2.0, 2.0, 4.0], 40.0)
+        if k:
+            d.zeta_table([0.0], [0.5], 64.0)
+        d.set(gamma=50.0, sigx=6.0, sigy=6.0, sigz=6.0, charge=160e-12, nmacro=4000, zeta_center=40.0, dz=30.0,
+              dz_push=30.0, sponge=15.0, sponge_y=10.0, laser_on=1, a0=5.0, theta=0.1, tau=100.0, start_fwhm=1.0,
+              z_end=1e9)
+        sim = g3d_amd.Sim(d)
+        sim.init()
+        for _ in range(10):
+            sim.step()
+        res.append(sim.array("parts"))
+    assert len(res[0]) == len(res[1])
+    dd = max(np.abs(res[0][f] - res[1][f]).max() for f in ("pz", "y", "zeta"))
+    assert dd < 1e-10, dd
+
+
+def test_streaming_and_trapping_kinematics():
+    """in vacuum (plasma block, no plasma within r_max = 0): an electron with v_z = 0.995 joining the march at zeta0
+    is trapped when it reaches z = s1, at zeta0 - (1 - v_z) dz / v_z, and moves on in the beam list; one with
+    v_z = 0.707 < vz_min leaves the beam list and keeps streaming (tests/test_plasma.cpp:178-207)"""
+    d = g3d_amd.Deck(LAMBDA_TEST)
+    d.c.nx = 8; d.c.Lx = 40.0
+    d.y_uniform(8, 40.0)
+    d.zeta_uniform(201, 200.0)
+    d.set(plasma_on=1, n0=0.01, r_max=0.0, sponge=0.0, sponge_y=0.0, move_x=0, move_y=0, nmacro=0, dz=50.0,
+          dz_push=50.0, z_end=1e9)
+    sim = g3d_amd.Sim(d)
+    sim.init()
+    f = np.zeros(1, dtype=g3d_amd.PARTICLE)
+    f["x"], f["y"], f["zeta"], f["pz"], f["w"], f["q"], f["qm"], f["sp"], f["z"], f["alive"] = \
+        20.0, 20.0, 150.3, 10.0, 1e-12, -1.0, -1.0, 2, 0.0, 6
+    sl = f.copy()
+    sl["pz"], sl["zeta"], sl["alive"], sl["sp"] = 1.0, 120.7, 1, 0
+    sim.set_joining(f)
+    sim.set_particles(sl)
+    sim.step()
+    vf, vs = 10 / math.sqrt(101.0), 1 / math.sqrt(2.0)
+    shift_f, shift_s = (1 - vf) * 50.0 / vf, (1 - vs) * 50.0 / vs
+    b = sim.array("parts")
+    zref = 150.3 - shift_f
+    assert len(b) == 1 and b["sp"][0] == 1 and abs(b["zeta"][0] - zref) < 1e-9, (b["sp"], b["zeta"], zref)
+    j = sim.joining()
+    assert len(j) == 1 and abs(j["zeta"][0] - (120.7 - shift_s)) < 1e-9, (j["zeta"], 120.7 - shift_s)
+    sim.step()
+    b = sim.array("parts")
+    assert len(b) == 1 and abs(b["zeta"][0] - (zref - shift_f)) < 1e-6
+
+
+def test_ions_immobile_limit():
+    """mobile ions of mass 1e15 u give the fields of the run without ions, the 1D laser wake of the wake test at a0 =
+    0.3 (tests/test_ions.cpp:4-31)"""
+    def run(ions):
+        h = 2 * math.pi / 32
+        N = int(300 / h) + 1
+        d = g3d_amd.Deck(LAMBDA_TEST)
+        d.c.nx = 4; d.c.Lx = 40.0
+        d.y_uniform(4, 40.0)
+        d.zeta_uniform(N, h * (N - 1))
+        d.set(sponge=0.0, sponge_y=0.0, move_x=0, move_y=0, plasma_on=1, n0=0.01, ppc_x=1, ppc_y=1, nmacro=0,
+              dz=25.0, dz_push=25.0, zeta_center=0.75 * h * (N - 1))
+        d.add_grid_laser(a0=0.3, waist=1e7, tau=2 * math.pi / math.sqrt(0.01) * 0.4, zeta_c=240.0)
+        if ions:
+            d.add_ion(1e15 * 1822.888486, 1, 1, atoms=1.0, mobile=True, ionization=False)
+        sim = g3d_amd.Sim(d)
+        sim.init()
+        for _ in range(3):
+            sim.step()
+        return sim.array("F").astype(np.float64)
+    a, b = run(False), run(True)
+    assert np.abs(a - b).max() <= 1e-6 * np.abs(a).max(), np.abs(a - b).max() / np.abs(a).max()
+
+
+def test_ions_neutral_pairs_at_rest():
+    """a parabolic channel with a z ramp, H+ (fraction 0.5) and He2+ (0.25 atoms), no driver: the fields stay zero,
+    no particle moves, and Z n_i equals n_e on the lines through the axis (tests/test_ions.cpp:33-59)"""
+    d = g3d_amd.Deck(LAMBDA_TEST)
+    d.c.nx = 16; d.c.Lx = 40.0
+    d.y_uniform(16, 40.0)
+    d.zeta_uniform(41, 40.0)
+    d.set(sponge=0.0, sponge_y=0.0, move_x=0, move_y=0, plasma_on=1, n0=0.01, parabolic=1, channel_depth=0.5,
+          r_channel=10.0, r_max=15.0, nmacro=0, dz=20.0, dz_push=20.0, z_end=1e9)
+    d.c.n_pz = 2
+    d.c.pz_points[0], d.c.pz_points[1] = 0.0, 100.0
+    d.c.pz_values[0], d.c.pz_values[1] = 0.2, 1.0
+    d.add_ion(1822.888486, 1, 1, atoms=0.5, mobile=True, ionization=False)
+    d.add_ion(4.0026 * 1822.888486, 2, 2, atoms=0.25, mobile=True, ionization=False)
+    sim = g3d_amd.Sim(d)
+    sim.init()
+    for _ in range(2):
+        sim.step()
+    fm = float(np.abs(sim.array("F")).max())
+    sp = sim.array("strm")
+    pm = float(max(np.abs(sp["px"]).max(), np.abs(sp["py"]).max(), np.abs(sp["pz"]).max()))
+    ni = int((sp["sp"] >= 3).sum())
+    re, ri = sim.array("rho_lines"), sim.array("rho_lines_i")
+    em, dm = float(np.abs(re).max()), float(np.abs(re + ri).max())
+    assert fm == 0 and pm == 0 and ni > 0, (fm, pm, ni)
+    assert em > 0 and dm <= 1e-12 * em, dm / em

@@ -1,9 +1,10 @@
 """The kernel-level C ABI (include/g3d.h: g3d_stream_halfcell, g3d_stream_spectra, g3d_real_fields_slice,
-g3d_bunch_push, g3d_bunch_currents, g3d_bunch_centroid, g3d_window_shift) driven as the reference's own member
+g3d_bunch_push, g3d_bunch_currents, g3d_sponge_real_fields, g3d_comoving, g3d_bunch_centroid, g3d_window_shift) driven as the reference's own member
 functions drive their device code, on caller-owned device buffers in the reference's layouts, against the
 reference's CPU build (oracle/_ref) on one thread: the same inputs, the same outputs.  The geometry tables of the
 constructor come from the engine (their values are held to the reference by the engine tests)."""
 import ctypes as C
+import math
 
 import numpy as np
 import pytest
@@ -165,3 +166,58 @@ def test_bunch_push_currents_and_window_shift_against_reference():
     got, want = parts.records(), ref.array("parts")
     np.testing.assert_array_equal(got["x"], want["x"])
     np.testing.assert_array_equal(got["y"], want["y"])
+
+
+def test_sponge_and_comoving_against_reference():
+    """sponge_and_real_fields on random Hermitian spectra (sponge layers on, the bound field of the bunch taken out and
+    added back), and comoving_field of the bunch's half-cell charge, through the kernel level"""
+    mk = lambda: bunch_deck(nx=32, ny=32, nz=64, nmacro=4096, laser=False, sponge=4.0)
+    deck = mk()
+    refpy.set_threads(1)
+    ref = refpy.RefSim(mk())
+    ref.init()
+    tabs = Tables(g3d_amd.Sim(mk()))
+    info = ref.info()
+    ns, nk, nx, ny = info["ns"], info["nk"], deck.c.nx, deck.c.ny
+    plan = g3d_amd.FftPlan(nx, ny)
+    spec = tabs.spec(deck, plan)
+    rng = np.random.default_rng(11)
+    arrs = {}
+    for n in ("Rx", "Ry", "Lx", "Ly", "Ez", "Bz", "jkz"):
+        planes = ns + 1 if n == "jkz" else ns
+        a = np.fft.fft2(rng.standard_normal((planes, ny, nx))).reshape(-1)
+        ref.set(n, a)
+        arrs[n] = D(a)
+    masks = {k: D(tabs.sim.array(k)) for k in ("maskx", "masky")}
+    F = D(nbytes=6 * (ns + 2) * nk * 4)
+    scr = D(nbytes=5 * ns * nk * 16)
+    ref.call("sponge_and_real_fields")
+    g3d_amd.kernel_call("g3d_sponge_real_fields", C.byref(spec), 1, masks["maskx"].ptr, masks["masky"].ptr,
+                        info["beta_b"], info["gm2_b"], *[arrs[n].ptr for n in ("Rx", "Ry", "Lx", "Ly", "Ez", "Bz", "jkz")],
+                        F.ptr, scr.ptr)
+    for n in ("Rx", "Ry"):
+        assert rel(arrs[n].get(np.complex128), ref.array(n)) <= 1e-13, n
+    Fr = ref.array("F").reshape(6, ns + 2, nk)[:, :ns]
+    Fg = F.get(np.float32).reshape(6, ns + 2, nk)[:, :ns]
+    assert rel(Fg, Fr) <= 1.2e-7
+    # the co-moving start field of the bunch's half-cell charge
+    ref2 = ref                       # its bunch as init loaded it; currents recomputes jkz from it
+    ref2.call("set_starting", 1)
+    ref2.call("currents")
+    jkz = D(ref2.array("jkz"))
+    p = ref2.array("parts")
+    assert len(p) > 0
+    s0 = s1 = 0.0
+    for q in p:
+        if q["alive"] == 1:
+            s0 += q["w"] * q["pz"] / math.sqrt(1 + q["px"] * q["px"] + q["py"] * q["py"] + q["pz"] * q["pz"])
+            s1 += q["w"]
+    beta = s0 / s1
+    ref2.call("comoving_field")
+    outs = {n: D(nbytes=ns * nk * 16) for n in ("Rx", "Ry", "Ez", "Bz")}
+    scr2 = D(nbytes=2 * ns * nk * 16)
+    g3d_amd.kernel_call("g3d_comoving", C.byref(spec), jkz.ptr, beta, 1 - beta * beta,
+                        *[outs[n].ptr for n in ("Rx", "Ry", "Ez", "Bz")], scr2.ptr)
+    assert ref2.info()["beta_b"] == beta
+    for n in ("Rx", "Ry", "Ez", "Bz"):
+        assert rel(outs[n].get(np.complex128), ref2.array(n)) <= 1e-13, n
