@@ -256,7 +256,7 @@ def _load():
         "g3d_bunch_push_piece": ([P], C.c_int), "g3d_bunch_push_vjp": ([P, P], C.c_int),
         "g3d_bunch_push": ([P, P, C.c_long, C.c_long, C.c_double], C.c_int),
         "g3d_stream_halfcell": ([P, P, P, C.c_long], C.c_int),
-        "g3d_stream_spectra": ([P, P, P, P], C.c_int),
+        "g3d_stream_spectra": ([P, P, P, P], C.c_int), "g3d_stream_spectra_static": ([P, P, P], C.c_int),
         "g3d_real_fields_slice": ([P, P, P, P, P, P, P, P, C.c_long, P], C.c_int),
         "g3d_bunch_currents": ([P, P, P, P, C.c_double, C.c_double, C.c_int, C.c_int, P, C.c_long, P, P, P, P, P],
                                C.c_int),

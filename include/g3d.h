@@ -325,6 +325,13 @@ typedef struct {
  * transverse filters and the smoothing); jsr is zeroed; scratch: 6 nk complex values */
 int g3d_stream_spectra(const g3d_spec_ctx* c, double* jsr, g3d_cpx* jsn, g3d_cpx* scratch);
 
+/* No counterpart in the reference: g3d_stream_spectra with its workspace in a file-scope device array instead of a
+ * caller's buffer, for callers that cannot allocate device memory (code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc).  It calls no allocating routine.  Limit: nk <= 65536 (256 x 256 modes), refused beyond; the
+ * workspace is 6 MiB per device, held until the process ends.  Every call on one GPU shares that GPU's workspace: calls
+ * that overlap on one GPU race. */
+int g3d_stream_spectra_static(const g3d_spec_ctx* c, double* jsr, g3d_cpx* jsn);
+
 /* Simulation::real_fields_slice (src/sim_plasma.cpp:200-220): the spectra of one slice (plane pointers of R_x R_y
  * L_x L_y E_z B_z) -> its six real field planes E_x E_y E_z B_x B_y B_z at F_slice + comp * fpl (float);
  * scratch: 3 nk complex values */
