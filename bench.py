@@ -208,38 +208,29 @@ def freehand_deck(Lw=615.5):
 
 
 def freehand_main(args, torch):
-    import g3d_amd
-    sim = g3d_amd.Sim(freehand_deck())
-    t0 = time.perf_counter()
-    sim.init()
-    t_init = time.perf_counter() - t0
-    print("freehand: init %.2f s" % t_init, file=sys.stderr, flush=True)
-    for k in range(args.warmup):
-        sim.step()
-        print("freehand: warmup step %d done" % k, file=sys.stderr, flush=True)
-    N, nk = sim.info()["N"], sim.info()["nk"]
-    torch.cuda.synchronize()
-    sim.timing(["stream_halfcell"])
-    sim.timing_every(TIME_EVERY)
-    t0 = time.perf_counter()
-    pushes = 0
-    for k in range(args.steps):
-        sim.step()
-        pushes += sim.info()["nstrm"] * (N - 1) * 2
-        print("freehand: step %d done" % k, file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    ms, cnt = sim.timers()["stream_halfcell"]
-    avg_s = ms / 1000 / max(cnt, 1)
-    sim.timing_every(1)
-    nstrm = sim.info()["nstrm"]
-    bpl = stream_bytes_per_launch(nstrm, nk)
-    sim.timing(True)
-    sim.step()
-    torch.cuda.synchronize()
-    fam = {k: {"ms_total": round(v[0], 3), "launches": v[1], "avg_us": round(1000 * v[0] / v[1], 2)}
-           for k, v in sim.timers().items() if v[1]}
-    sim.timing(False)
+    forms = {"fp64": [False], "mixed": [True], "both": [False, True]}[args.precision]
+    runs = {}
+    for mixed in forms:
+        r = run_form(args, mixed, torch, deck=freehand_deck())
+        # one form's device state is about 130 GB at this size: keep what the comparison reads on the host and
+        # release the device memory before the next form
+        sim = r.pop("sim")
+        info = sim.info()
+        r["host"] = {"F": sim.array("F"), "ns": info["ns"], "nk": info["nk"], "nstrm": info["nstrm"],
+                     "ions": [float(v) for v in sim.array("ion_stats1")]}
+        sim.close()
+        del sim
+        runs["mixed" if mixed else "fp64"] = r
+    head = runs.get("mixed") or runs["fp64"]
+    compare = None
+    if len(runs) == 2:
+        h64, hmx = runs["fp64"]["host"], runs["mixed"]["host"]
+        compare = slice_errors(hmx["F"], h64["F"], h64["ns"], h64["nk"])
+        compare.update({"speedup_mixed_over_fp64": runs["mixed"]["value"] / runs["fp64"]["value"],
+                        "fp64_value": runs["fp64"]["value"], "fp64_ms_per_step": runs["fp64"]["ms_step"],
+                        "fp64_stream_us": runs["fp64"]["avg_us"],
+                        "streaming_list_fp64_mixed": [h64["nstrm"], hmx["nstrm"]],
+                        "nitrogen_charge_states_fp64": h64["ions"], "nitrogen_charge_states_mixed": hmx["ions"]})
     cpu = None
     if not args.no_cpu:
         thr = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -253,26 +244,33 @@ def freehand_main(args, torch):
                 t = time.perf_counter()
                 ref.step()
                 dtc = time.perf_counter() - t
-                n_ref = ref.info()["N"]
-                cpu = {"value": ref.info()["nstrm"] * (n_ref - 1) * 2 / dtc, "unit": "pushes/s", "cores": thr,
-                       "kind": "reference",
+                model, ncpu = cpu_model()
+                # the reference counts no pushes: every list entry through every half-cell and pass
+                cpu = {"value": ref.info()["nstrm"] * (ref.info()["N"] - 1) * 2 / dtc, "unit": "pushes/s", "cores": thr,
+                       "kind": "reference", "host_cpus": ncpu, "cpu_model": model,
                        "sample": "reference CPU build (g++ -O3 -fopenmp, %d threads), the same transverse deck with "
-                                 "%d slices, 1 step: %.2f s" % (thr, n_ref, dtc)}
+                                 "%d slices, 1 step: %.2f s" % (thr, ref.info()["N"], dtc)}
         except Exception as e:
             print("cpu baseline failed: %s" % e, file=sys.stderr)
+    mixed = "mixed" in runs
     out = {
-        "metric": "particle pushes/s incl. deposit+gather+ADK, free-hand y",
-        "value": pushes / dt, "unit": "pushes/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": 1000 * dt / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "free-hand y plasma 512x386x2048 with H and N (ADK), FP64", "N": N, "nstrm": nstrm,
-                   "slice_solves_per_s": N * args.steps / dt},
-        "roofline": {"kernel": "k_stream_halfcell", "bound": "hbm", "achieved": bpl / avg_s / 1e9,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": bpl / avg_s / 1e9 / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_launch": bpl, "avg_launch_us": avg_s * 1e6},
+        "metric": "particle pushes/s incl. deposit+gather+ADK, free-hand y; FP32-mixed vs FP64",
+        "value": head["value"], "unit": "pushes/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": head["ms_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32-mixed" if mixed else "f64", "data": "synthetic",
+        "config": {"workload": "free-hand y plasma 512x386x2048 with H and N (ADK), %s form"
+                               % ("FP32-mixed" if mixed else "FP64"),
+                   "N": head["N"], "nstrm": head["nstrm"], "pushes_per_step": head["pushes_per_step"],
+                   "slice_solves_per_s": head["N"] * 1000.0 / head["ms_step"]},
+        "roofline": {"kernel": head["kernel"], "bound": "hbm", "achieved": head["achieved"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": head["achieved"] / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": head["bpl"], "pushes_per_launch": head["push_per_launch"],
+                     "design_bytes_per_launch": head["design_bpl"], "avg_launch_us": head["avg_us"],
+                     "timed_launches": head["timed_launches"], "timed_one_in": TIME_EVERY},
         "cpu_baseline": cpu,
-        "kernel_families_one_extra_step": fam,
-        "init_s": t_init,
+        "mixed_vs_fp64": compare,
+        "kernel_families_one_extra_step": head["fam"],
+        "init_s": head["t_init"],
     }
     print(json.dumps(out))
 
@@ -359,9 +357,9 @@ def measured_traffic(kernel):
     return hits[0]["bytes_per_launch"] if len(hits) == 1 else None
 
 
-def run_form(args, mixed, torch):
+def run_form(args, mixed, torch, deck=None):
     import g3d_amd
-    d = lwfa_deck(nz=args.nz, witness=args.witness)
+    d = deck if deck is not None else lwfa_deck(nz=args.nz, witness=args.witness)
     d.set(mixed=int(mixed))
     sim = g3d_amd.Sim(d)
     t0 = time.perf_counter()
@@ -393,12 +391,16 @@ def run_form(args, mixed, torch):
     avg_s = ms / 1000 / max(cnt, 1)
     sim.timing_every(1)
     nstrm = sim.info()["nstrm"]
-    form = "mixed" if mixed else "fp64"
-    design_bpl = stream_bytes_mixed(nstrm, nk) if mixed else stream_bytes_per_launch(nstrm, nk)
+    # the mixed stream kernel runs for Fourier y without ions; with ions or the free-hand y grid the mixed form
+    # streams with the FP64 kernel (FP64 particle state, float current planes)
+    c = d.c
+    mixed_kernel = mixed and c.y_fourier and not any(c.ions[k].atoms > 0 for k in range(c.n_ions))
+    form = "mixed" if mixed_kernel else "fp64"
+    design_bpl = stream_bytes_mixed(nstrm, nk) if mixed_kernel else stream_bytes_per_launch(nstrm, nk)
     # algorithmic bytes of one stream launch: the live pushes of an average launch and the grid points
     push_per_launch = ps / (args.steps * (N - 1) * sim_passes(sim))
     bpl = push_per_launch * ALGO_PUSH_B[form] + nk * ALGO_POINT_B[form]
-    kernel = "k_stream_mixed" if mixed else "k_stream_halfcell"
+    kernel = "k_stream_mixed" if mixed_kernel else "k_stream_halfcell"
     # one more step, outside the timed region, with every kernel family timed (the per-kernel breakdown)
     sim.timing(True)
     sim.step()
@@ -508,17 +510,15 @@ def main():
     print(json.dumps(out))
 
 
-def compare_forms(r64, rmx):
-    """the two runs took the same steps from the same start: the real fields F of every slice (E and B of the slice
-    together, relative L2 against FP64, every slice), and the witness moments of the reference's bunches.txt"""
-    s64, smx = r64["sim"], rmx["sim"]
-    info = s64.info()
-    ns, nk = info["ns"], info["nk"]
-    A = smx.array("F").astype(np.float64).reshape(6, ns + 2, nk)[:, :ns]
-    B = s64.array("F").astype(np.float64).reshape(6, ns + 2, nk)[:, :ns]
-    nb = np.sqrt((B ** 2).sum(axis=(0, 2)))
-    de = np.sqrt(((A - B) ** 2).sum(axis=(0, 2)))
-    del A, B
+def slice_errors(Fmx, F64, ns, nk):
+    """the real fields F of every slice of two runs (E and B of the slice together), relative L2 against FP64"""
+    A, B = Fmx.reshape(6, ns + 2, nk), F64.reshape(6, ns + 2, nk)
+    nb, de = np.zeros(ns), np.zeros(ns)
+    for m0 in range(0, ns, 64):                      # in blocks of slices (F of config 3 is 10 GB)
+        a = A[:, m0:min(m0 + 64, ns)].astype(np.float64)
+        b = B[:, m0:min(m0 + 64, ns)].astype(np.float64)
+        nb[m0:m0 + a.shape[1]] = np.sqrt((b ** 2).sum(axis=(0, 2)))
+        de[m0:m0 + a.shape[1]] = np.sqrt(((a - b) ** 2).sum(axis=(0, 2)))
     err = de / np.maximum(nb, 1e-300)
     worst = int(err.argmax())
     big = nb >= 1e-2 * nb.max()          # slices carrying at least 1 % of the largest slice's field
@@ -527,6 +527,15 @@ def compare_forms(r64, rmx):
            "F_rel_l2_p50_p99_over_slices": [float(v) for v in np.percentile(err, [50, 99])],
            "F_rel_l2_all_slices": float(np.sqrt((de ** 2).sum() / (nb ** 2).sum())),
            "F_rel_l2_max_over_slices_above_1pct": float(err[big].max()), "slices": int(len(nb))}
+    return out
+
+
+def compare_forms(r64, rmx):
+    """the two runs took the same steps from the same start: the real fields F of every slice (relative L2 against
+    FP64, every slice), and the witness moments of the reference's bunches.txt"""
+    s64, smx = r64["sim"], rmx["sim"]
+    info = s64.info()
+    out = slice_errors(smx.array("F"), s64.array("F"), info["ns"], info["nk"])
     p64, pmx = s64.array("parts"), smx.array("parts")
     if len(p64):
         g0 = s64._deck.c.gamma

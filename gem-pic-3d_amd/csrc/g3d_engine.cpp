@@ -17,6 +17,18 @@ const char* const kFamilyNames[Sim::kFam] = {"stream_halfcell", "pass_rows_fwd",
 enum { F_STREAM = 0, F_SFFT, F_MODES, F_IFFT, F_TOF, F_PUSH, F_CUR, F_SPONGE };
 
 // ---------------------------------------------------------------- configuration
+// a piecewise-linear function as the sum of its hat functions v[k] max(0, 1 - |x - x_k| / h), x_k = x0 + k h, summed
+// in order: the arithmetic of the reference's expression of this form (an envelope_function or profile_z)
+double hat_sum(const double* v, int n, double x0, double h, double x) {
+    double s = 0;
+    for (int k = 0; k < n; ++k) {
+        const double xk = x0 + k * h;
+        const double t = v[k] * std::max(0.0, 1 - std::fabs(x - xk) / h);
+        s = k ? s + t : t;
+    }
+    return s;
+}
+
 Config Config::from_abi(const g3d_config& g) {
     Config c;
     c.lambda0 = g.lambda0;
@@ -37,6 +49,12 @@ Config Config::from_abi(const g3d_config& g) {
     c.cep = g.cep; c.start_fwhm = g.start_fwhm; c.stop_fwhm = g.stop_fwhm;
     c.charge = g.charge; c.gamma = g.gamma; c.sigx = g.sigx; c.sigy = g.sigy; c.sigz = g.sigz; c.emitx = g.emitx;
     c.emity = g.emity; c.espread = g.espread; c.zeta_center = g.zeta_center; c.nmacro = g.nmacro; c.seed = g.seed;
+    if (g.n_prof < 0 || g.n_prof > 16 || g.n_prof == 1 || (g.n_prof && !(g.prof_zmax > g.prof_zmin)))
+        throw std::invalid_argument("bunch profile: 2 ... 16 points on zeta_min < zeta_max");
+    c.prof.assign(g.prof, g.prof + g.n_prof); c.prof_zmin = g.prof_zmin; c.prof_zmax = g.prof_zmax;
+    for (int k = 0; k < g.n_glasers; ++k)
+        if (g.glasers[k].n_env < 0 || g.glasers[k].n_env > 32 || (g.glasers[k].n_env && !(g.glasers[k].env_dz > 0)))
+            throw std::invalid_argument("grid laser envelope: 1 ... 32 points with a positive spacing");
     c.glasers.assign(g.glasers, g.glasers + g.n_glasers);
     c.plasma_on = g.plasma_on; c.n0 = g.n0; c.r_max = g.r_max; c.ppc_x = g.ppc_x; c.ppc_y = g.ppc_y;
     c.smooth_length = g.smooth_length;
@@ -540,7 +558,8 @@ void Sim::load_bunch() {
     const double ylo = yf ? -1e300 : cfg.ynode[1], yhi = yf ? 1e300 : cfg.ynode[cfg.ny - 1];
     const double yoff = yf ? 0.5 * cfg.Ly : 0;   // Fourier y: box coordinate, physical y = p.y + Y0
     std::vector<g3d_particle> mine;
-    if (cfg.nmacro > 0) {
+    if (cfg.nmacro > 0 && !cfg.prof.empty()) load_bunch_profile(mine, ylo, yhi, yoff);
+    else if (cfg.nmacro > 0) {
         std::mt19937_64 rng(cfg.seed);
         std::normal_distribution<double> G(0, 1);
         const double Ne = cfg.charge / kE;
@@ -570,6 +589,60 @@ void Sim::load_bunch() {
     }
     set_particles(mine);
     X0 = -0.5 * cfg.Lx; Y0 = -yoff;
+}
+
+// the bunch with a current profile along zeta (distribution_z = function, src/sim_bunches.cpp:115-132, 160-177):
+// zeta from the inverse of the profile's cumulative distribution (4096 trapezoids), the draws in the reference's
+// order (uniform for zeta, then x, y, the transverse momenta if there is an emittance, the energy spread)
+void Sim::load_bunch_profile(std::vector<g3d_particle>& mine, double ylo, double yhi, double yoff) {
+    std::mt19937_64 rng(cfg.seed);
+    std::normal_distribution<double> G(0, 1);
+    std::uniform_real_distribution<double> U(-0.5, 0.5);
+    const double Ne = cfg.charge / kE;
+    const double W = (Ne / cfg.nmacro) / (cfg.nc * std::pow(cfg.Lu, 3));
+    const double zmin = cfg.prof_zmin, zmax = cfg.prof_zmax, zc = cfg.zeta_center;
+    const int np = int(cfg.prof.size());
+    const double hk = (zmax - zmin) / (np - 1);
+    auto prof = [&](double z) { return hat_sum(cfg.prof.data(), np, zmin, hk, z); };
+    const int K = 4096;
+    std::vector<double> cz(K + 1, 0.0);
+    double prev = std::max(0.0, prof(zmin));
+    for (int k = 1; k <= K; ++k) {
+        const double f = std::max(0.0, prof(zmin + (zmax - zmin) * k / K));
+        cz[k] = cz[k - 1] + 0.5 * (prev + f); prev = f;
+    }
+    if (!(cz[K] > 0)) throw std::runtime_error("bunch: profile_z is zero on [zeta_min, zeta_max]");
+    auto sample_zeta = [&](double u01) {
+        const double t = u01 * cz[K];
+        const int k = int(std::upper_bound(cz.begin(), cz.end(), t) - cz.begin()) - 1;
+        const int kk = std::min(std::max(k, 0), K - 1);
+        const double f = cz[kk + 1] > cz[kk] ? (t - cz[kk]) / (cz[kk + 1] - cz[kk]) : 0.5;
+        return zmin + (zmax - zmin) * (kk + f) / K;
+    };
+    Axis zh = zax; zh.X = cfg.zeta.data();
+    for (long ib = 0; ib < cfg.nmacro; ++ib) {
+        const double zf = sample_zeta(U(rng) + 0.5);
+        const double xr = cfg.sigx * G(rng), yr = cfg.sigy * G(rng);
+        const double x = 0 + xr, y = 0 + yr;
+        const double px = cfg.emitx > 0 && cfg.sigx > 0 ? cfg.emitx / cfg.sigx * G(rng) : 0;
+        const double py = cfg.emity > 0 && cfg.sigy > 0 ? cfg.emity / cfg.sigy * G(rng) : 0;
+        const double ptot = std::sqrt(cfg.gamma * cfg.gamma - 1) * (1 + cfg.espread * G(rng));
+        if (px * px + py * py >= ptot * ptot) throw std::runtime_error("bunch: transverse momentum above the total");
+        const double pz = std::sqrt(ptot * ptot - px * px - py * py);
+        const double z = (zf - zc) * pz / std::sqrt(1 + ptot * ptot);
+        const double g = std::sqrt(1 + px * px + py * py + pz * pz);
+        const double vx = px / g, vy = py / g, vz = pz / g;
+        const double zz = z + 0.0;
+        g3d_particle p{};
+        p.x = x - vx * zz / vz + 0.5 * cfg.Lx; p.y = y - vy * zz / vz + yoff;
+        p.zeta = zc + zz / vz;
+        p.px = px; p.py = py; p.pz = pz; p.w = W; p.alive = 1;
+        p.q = -1; p.qm = -1; p.sp = 0; p.z = 0;
+        int hc; double f;
+        zh.locate(p.zeta, hc, f);
+        if (p.zeta < 0 || hc > N - 2 || p.y < ylo || p.y > yhi) { ++lost_total; continue; }
+        mine.push_back(p);
+    }
 }
 
 // ---------------------------------------------------------------- time step (src/sim_setup.cpp:178-217)
@@ -770,6 +843,15 @@ void Sim::add_grid_lasers() {
         la.rc = std::cos(Lz.pol_angle); la.rs = std::sin(Lz.pol_angle); la.dx = dx; la.dy = dy; la.X0 = X0; la.Y0 = Y0;
         la.circular = Lz.circular; la.helicity = Lz.helicity; la.nx = cfg.nx; la.ny = cfg.ny; la.nk = nk;
         la.zeta = zs.data(); la.yn = yf ? nullptr : yn.data();
+        la.env = nullptr;
+        DevBuf<double> envd;
+        if (Lz.n_env > 0) {          // envelope_function: a(zeta) of each slice (src/sim_fields.cpp:335)
+            std::vector<double> e(static_cast<size_t>(N));
+            for (int n = 0; n < N; ++n) e[size_t(n)] = hat_sum(Lz.env, Lz.n_env, Lz.env_z0, Lz.env_dz, cfg.zeta[n]);
+            envd.resize(size_t(N));
+            envd.from_host(e.data(), 0, size_t(N));
+            la.env = envd.data();
+        }
         grid_laser_fill(la, T3.data(), a, ns);
         fft_run(fft, T3.data(), ns, false);
         grid_laser_add(g, T3.data(), Rx.data(), Ry.data(), ns, !yf);
@@ -962,7 +1044,7 @@ StreamArgs Sim::stream_args(double s0, double s1) {
     sc.Delta = Delta; sc.s1 = s1; sc.sTrap = s1 + (cfg.trap_advance - 1) * Delta; sc.vz_beam = cfg.vz_beam;
     sc.ions = ion_dev.data(); sc.adk = adk_dev.data(); sc.born = born.cap ? born.view() : Parts{};
     sc.bcnt = counters.data(); sc.bcap = 0; sc.key = 0; sc.n = 0;
-    sc.npush = nullptr; sc.npush_cap = 0;
+    sc.npush = nullptr; sc.npush_cap = 0; sc.Jf = nullptr; sc.skip_state = 0;
     return sc;
 }
 
@@ -1004,11 +1086,15 @@ void Sim::march_plasma(double s0, double s1) {
     const SpecGeom g = geom();
     jpred.zero();
     jsr.zero();
-    // the FP32-mixed pass: Fourier y, and no ion species in the list (ions and ionization stay FP64).  With Fourier
-    // y the modes of either form run column-major (one fused column launch per pass) unless G3D_PASS_CM=0
+    // the FP32-mixed pass: with Fourier y and electrons only the mixed stream kernel (mx); with ion species or the
+    // free-hand y grid (mxi) the FP64 stream kernel (the ion push and ionization as in the FP64 form) depositing into
+    // float current planes, the FP32 forward row transforms, and the mixed column pass with its FP32 inverse rows
+    // (Fourier y) or the FP64 line pass and FP64 inverse rows (free-hand y: the elimination stays FP64).  With Fourier y the modes of either form run column-major (one
+    // fused column launch per pass) unless G3D_PASS_CM=0
     bool ions = false;
     for (const auto& is : cfg.ions) ions = ions || cfg.loaded(is);
     const bool mx = cfg.mixed && yf && !ions;
+    const bool mxi = cfg.mixed && !mx;
     const char* ecm = getenv("G3D_PASS_CM");
     const bool cm = yf && !(ecm && atoi(ecm) == 0) && cfg.ny <= 1024 && cfg.nx % 16 == 0 && cfg.ny % 16 == 0;
     ensure_layout(cm, cm);
@@ -1044,7 +1130,7 @@ void Sim::march_plasma(double s0, double s1) {
     tg.chunk = 0; tg.max_extra = 0; tg.xtile = nullptr; tg.xstart = nullptr; tg.xcount = nullptr;
     // FP32-mixed: the tile windows go to the current planes with float atomics instead of slabs (3 % faster at
     // 256 x 256: the row kernel then reads one plane set); G3D_STREAM_AFLUSH=0 keeps the slabs
-    { const char* e = getenv("G3D_STREAM_AFLUSH"); tg.aflush = mx && !(e && atoi(e) == 0); }
+    { const char* e = getenv("G3D_STREAM_AFLUSH"); tg.aflush = (mx && !(e && atoi(e) == 0)) || mxi; }
     // FP32-mixed: tiles denser than G3D_STREAM_CHUNK (default 1) times one workgroup's lanes are streamed in
     // chunks of that many particles (0: off)
     const char* ech = getenv("G3D_STREAM_CHUNK");
@@ -1056,6 +1142,10 @@ void Sim::march_plasma(double s0, double s1) {
         if (chunk_cnt.size() < 1) chunk_cnt.resize(1);
         tg.xtile = chunk_tile.data(); tg.xstart = chunk_start.data(); tg.xcount = chunk_cnt.data();
         tile_chunks(tg, chunk_tile.data(), chunk_start.data(), chunk_cnt.data());
+    }
+    if (mxi) {
+        if (jsr_f.size() < 5 * pl) jsr_f.resize(5 * pl);
+        sc.Jf = jsr_f.data();
     }
     if (mx) {
         if (jsr_f.size() < 5 * pl) jsr_f.resize(5 * pl);
@@ -1075,11 +1165,12 @@ void Sim::march_plasma(double s0, double s1) {
     m.pk = nullptr; m.n = N - 1; m.first = 1; m.last = 1; m.top = 1; m.wex = 0; m.qcorr = 0;
     m.bRx = m.bRy = m.bEz = m.bBz = nullptr;
     tic(F_MODES);
-    if (mx) pass_mixed(m, fft, jsr_f.data(), slab_f.data(), tg, Xf, Tf, F.data() + long(N - 1 - a) * nk, fpl, nullptr,
-                       nullptr, 0, 0, 1, rpw);
+    if (mx || (mxi && yf))
+        pass_mixed(m, fft, jsr_f.data(), slab_f.data(), tg, Xf, Tf, F.data() + long(N - 1 - a) * nk, fpl, nullptr,
+                   nullptr, 0, 0, 1, rpw);
     else pass_work(sp3.data(), 1);
     toc(F_MODES);
-    if (!mx) real_fields(N - 1);
+    if (!mx && !(mxi && yf)) real_fields(N - 1);
     // every rebin half-cells the live particles are binned again by their current tile (0: never)
     static const int rebin = [] { const char* e = getenv("G3D_REBIN"); return e ? atoi(e) : 8; }();
     const uint64_t kst = key_step(cfg.seed, istep);
@@ -1109,6 +1200,7 @@ void Sim::march_plasma(double s0, double s1) {
                 sc.bcap = int(std::min<long>(long(born.cap), long(strmA.cap) - nlist));
             }
             const bool last = pass == passes - 1;
+            sc.skip_state = !last;
             m.n = n; m.first = pass == 0; m.last = last; m.top = 0; m.wex = wex;
             m.qcorr = D * wlo_h[n + 1];
             if (passes > 1) { m.bRx = bk.data(); m.bRy = bk.data() + pl; m.bEz = bk.data() + 2 * pl; m.bBz = bk.data() + 3 * pl; }
@@ -1125,6 +1217,25 @@ void Sim::march_plasma(double s0, double s1) {
             const LaunchTimer lt = launch_timer(F_STREAM);
             stream_halfcell(sc, strmA.view(), strmB.view(&strmA), nlist, tg, lt);
             launch_timed(F_STREAM, lt);
+            if (mxi) {
+                double* rl = last ? rho_lines.data() + long(n) * (nx + ny) : nullptr;
+                double* rli = last && rho_lines_i.size() ? rho_lines_i.data() + long(n) * (nx + ny) : nullptr;
+                tic(F_MODES);
+                if (yf) {
+                    TileGeom tr = tg;
+                    tr.ntile = 0;                   // the windows are already in the float planes
+                    pass_mixed(m, fft, jsr_f.data(), slab_f.data(), tr, Xf, Tf, F.data() + long(n - a) * nk, fpl, rl,
+                               rli, i0, j0, 0, rpw);
+                } else {
+                    // free-hand y: the inverse x transforms stay FP64 (F from FP32 transforms moved the rear slices
+                    // of config 3 to 3e-4 in 5 steps, FP64 ones keep them at 2e-5)
+                    rows_fwd_mixed_fh(fft, jsr_f.data(), sp3.data(), rl, rli, i0, j0, rpw);
+                    pass_work(sp3.data(), 0);
+                    real_fields(n);
+                }
+                toc(F_MODES);
+                continue;
+            }
             tic(F_SFFT);
             TileGeom tr = tg;
             if (tg.fh) tr.ntile = 0;             // free-hand: the windows are already in jsr

@@ -33,6 +33,8 @@ struct Config {
     double charge = 0, gamma = 1, sigx = 0, sigy = 0, sigz = 0, emitx = 0, emity = 0, espread = 0, zeta_center = 0;
     long nmacro = 0;
     unsigned long seed = 1;
+    std::vector<double> prof;         // profile_z of the bunch (empty: Gaussian in zeta), on prof_zmin ... prof_zmax
+    double prof_zmin = 0, prof_zmax = 0;
     std::vector<g3d_glaser> glasers;
     bool plasma_on = false;
     double n0 = 0, r_max = 1e300, channel_depth = 0, r_channel = 1, smooth_length = 0;
@@ -89,6 +91,7 @@ public:
     StreamArgs stream_args(double s0, double s1);
     void collect_restreaming();
     void load_bunch();
+    void load_bunch_profile(std::vector<g3d_particle>& mine, double ylo, double yhi, double yoff);
     void window_shift_start();
     void window_shift_post();
     void shift_window(int nxs, int nys);

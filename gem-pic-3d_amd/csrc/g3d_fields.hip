@@ -278,7 +278,7 @@ __global__ void k_sspec_unpack(SpecGeom g, const cx* pk, cx* ox, double a2, int 
 // the paraxial Gaussian at transverse point (xp, yp) of slice n: 2 E_x, 2 E_y
 __device__ void glaser_field(const GridLaserArgs& a, int n, double xp, double yp, double& ex, double& ey) {
     const double d = a.zeta[n] - a.zc;
-    const double env = exp(-2 * 0.69314718055994531 * d * d / (a.tau * a.tau));
+    const double env = a.env ? a.env[n] : exp(-2 * 0.69314718055994531 * d * d / (a.tau * a.tau));
     // 1/q with q = 1 + i qi
     const double qi = (0.0 - a.zfoc) / a.zR;
     const double qq = 1 + qi * qi;
@@ -302,7 +302,7 @@ __global__ void k_glaser_fill(GridLaserArgs a, cx* T, int n0, long tot) {
     const long i = idx % a.nk;
     const int ixx = int(i % a.nx), j = int(i / a.nx), n = n0 + l;
     const double d = a.zeta[n] - a.zc;
-    const double env = exp(-2 * 0.69314718055994531 * d * d / (a.tau * a.tau));
+    const double env = a.env ? a.env[n] : exp(-2 * 0.69314718055994531 * d * d / (a.tau * a.tau));
     if (fabs(env) < 1e-12) { T[idx] = mk(0); return; }
     const double xp = ixx * a.dx + a.X0;
     double ex, ey;

@@ -26,6 +26,9 @@ struct StreamArgs {
     double X0, Y0, zc, ylo, yhi;
     const double* zs;
     double* J;                       // jsr: jx jy jz rho- rho+ (5 nk)
+    float* Jf;                       // FP32-mixed form: the current planes in float (same layout); null: J
+    int skip_state;                  // a pass before the last of a half-cell: the particle state is not written (the
+                                     // next pass restarts from the same start state and writes it)
     const double *ia, *id;
     double Delta, s1, sTrap, vz_beam;
     int n;
@@ -216,6 +219,7 @@ void stream_mixed(const StreamArgs& a, float* J, float* slab, Parts src, Parts d
                   const LaunchTimer& lt = {}, bool rho = true);
 void pass_mixed(const ModeArgs& m, const FftPlan& p, float* J, const float* slab, const TileGeom& tg, cxf* X, cxf* T,
                 float* Fn, long fpl, double* rl, double* rli, int i0, int j0, int kind, int rpw);
+void rows_fwd_mixed_fh(const FftPlan& p, float* J, cx* X, double* rl, double* rli, int i0, int j0, int rpw);
 // the same columns step as three launches: y transforms of X (in place), the per-mode work (one lane per mode),
 // inverse y transforms of T
 void pass_modes(const ModeArgs& m, const FftPlan& p, cx* X, cx* T, cx* jsn, int kind);
@@ -234,6 +238,7 @@ struct GridLaserArgs {
     long nk;
     const double* zeta;
     const double* yn;                 // free-hand y: the node table (null: Fourier y)
+    const double* env;                // envelope_function: a(zeta) per slice (null: the Gaussian of tau)
 };
 void grid_laser_fill(const GridLaserArgs& a, cx* T, int n0, int count);
 void grid_laser_add(const SpecGeom& g, const cx* T, cx* Rx, cx* Ry, int count, int fh);

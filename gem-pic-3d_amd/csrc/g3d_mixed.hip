@@ -244,6 +244,7 @@ __device__ __forceinline__ void stream_one_mixed(const StreamArgs& c, const Half
     // q and qm are not written: this form streams electrons without ionization, so no push changes them, and every
     // loader (the lattice, the host list, the joining particles) writes the same values into both state buffers
     auto store = [&](double ux, double uy, double uz) {
+        if (c.skip_state) return;
         D.x[i] = x; D.y[i] = y; D.zeta[i] = zeta; D.px[i] = ux; D.py[i] = uy; D.pz[i] = uz;
         D.z[i] = z; D.alive[i] = alive;
     };
@@ -392,10 +393,16 @@ k_stream_mixed(StreamArgs c, float* J, float* slab, Parts S, Parts D, long n, Ti
     if (c.npush) add_pushes(c.npush, cnt);
 }
 
+// the spectra of the FP32 forward transforms are stored in float (cxf) or, for the FP64 line pass of the free-hand
+// grid, in double (cx)
+__device__ __forceinline__ void put(cxf* d, cxf v) { *d = v; }
+__device__ __forceinline__ void put(cx* d, cxf v) { *d = to_d(v); }
+
 // ---------------------------------------------------------------- rows forward (FP32)
 // tr: X column-major (element (kx, j) at kx ny + j), for the fused column kernel
+template <class XT>
 __global__ void __launch_bounds__(kT) k_rows_fwd_mixed(float* J, const float* slab, TileGeom tg, int nx, int ny, int lx,
-                                                       int rpw, const cx* twx, cxf* X, double* rl, double* rli, int i0,
+                                                       int rpw, const cx* twx, XT* X, double* rl, double* rli, int i0,
                                                        int j0, int tr, int xcd) {
     extern __shared__ cxf smf[];
     const int sp = padded(nx);
@@ -450,9 +457,9 @@ __global__ void __launch_bounds__(kT) k_rows_fwd_mixed(float* J, const float* sl
         const long i = tr ? long(kx) * ny + j : long(j) * nx + kx;
         const cxf z = buf[(2 * r) * sp + lpos(kx)], zm = conjg(buf[(2 * r) * sp + lpos((nx - kx) & (nx - 1))]);
         const cxf d = z - zm;
-        X[i] = 0.5f * (z + zm);
-        X[nk + i] = 0.5f * mkf(d.im, -d.re);
-        X[2 * nk + i] = buf[(2 * r + 1) * sp + lpos(kx)];
+        put(X + i, 0.5f * (z + zm));
+        put(X + nk + i, 0.5f * mkf(d.im, -d.re));
+        put(X + 2 * nk + i, buf[(2 * r + 1) * sp + lpos(kx)]);
     }
 }
 
@@ -604,7 +611,7 @@ void pass_mixed(const ModeArgs& m, const FftPlan& p, float* J, const float* slab
         const size_t lds = (size_t(2 * rpw) * (p.nx + p.nx / 8) + p.nx / 2) * sizeof(cxf);
         TileGeom tr = tg;
         if (tg.aflush) tr.ntile = 0;      // the windows are already in J
-        hipLaunchKernelGGL(k_rows_fwd_mixed, dim3(p.ny / rpw), dim3(lanes(rpw * p.nx)), lds, stream(), J, slab, tr, p.nx,
+        hipLaunchKernelGGL(k_rows_fwd_mixed<cxf>, dim3(p.ny / rpw), dim3(lanes(rpw * p.nx)), lds, stream(), J, slab, tr, p.nx,
                            p.ny, p.lx, rpw, p.twx, X, rl, rli, i0, j0, cm, cm && xcd_rows());
     }
     const size_t li = (size_t(3 * rpw) * (p.nx + p.nx / 8) + p.nx / 2) * sizeof(cxf);
@@ -633,6 +640,16 @@ void pass_mixed(const ModeArgs& m, const FftPlan& p, float* J, const float* slab
     hipLaunchKernelGGL(k_rows_inv_mixed, dim3(p.ny / rpw), dim3(lanes(rpw * p.nx)), li, stream(), T, Fn, fpl, p.nx, p.ny,
                        p.lx, rpw, p.twx, float(1.0 / (double(p.nx) * p.ny)), 0, 0);
     launch_check("plasma pass (mixed)");
+}
+
+// free-hand y: the x transforms of the current planes in FP32 into the double spectra X of the line pass (the lines
+// and the inverse transforms stay FP64)
+void rows_fwd_mixed_fh(const FftPlan& p, float* J, cx* X, double* rl, double* rli, int i0, int j0, int rpw) {
+    TileGeom tr{};
+    const size_t lds = (size_t(2 * rpw) * (p.nx + p.nx / 8) + p.nx / 2) * sizeof(cxf);
+    hipLaunchKernelGGL(k_rows_fwd_mixed<cx>, dim3(p.ny / rpw), dim3(lanes(rpw * p.nx)), lds, stream(), J,
+                       static_cast<const float*>(nullptr), tr, p.nx, p.ny, p.lx, rpw, p.twx, X, rl, rli, i0, j0, 0, 0);
+    launch_check("plasma rows forward (mixed, free-hand)");
 }
 
 }  // namespace g3d

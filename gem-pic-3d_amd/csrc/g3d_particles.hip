@@ -178,6 +178,12 @@ __device__ __forceinline__ bool gather_window_fh(const FieldView& c, const Win& 
 // TSC deposit of f (dt/Delta) v/(h dx dy) (src/sim_stream.hpp:33-64) into plane comp of the half-cell current:
 // the tile's LDS window when the whole 3 x 3 stencil lies inside it, else jsr with global atomics.  Zero
 // contributions are skipped (adding +-0 to a sum that starts at +0 never changes it).
+// a deposit into the current planes: float atomics in the FP32-mixed form (c.Jf), else FP64
+__device__ __forceinline__ void jadd(const StreamArgs& c, long k, double v) {
+    if (c.Jf) atomicAdd(&c.Jf[k], float(v));
+    else atomicAdd(&c.J[k], v);
+}
+
 template <int YF>
 __device__ __forceinline__ void sdeposit(const StreamArgs& c, const Win* win, double x, double y, double f, double vx,
                                          double vy, double vz) {
@@ -225,11 +231,11 @@ __device__ __forceinline__ void sdeposit(const StreamArgs& c, const Win* win, do
                 const long k = long(jn[v]) * P.nx + ix[u];
                 const double ww = w[3 * v + u];
                 if (mv) {
-                    atomicAdd(&c.J[k], ww * vx);
-                    atomicAdd(&c.J[nk + k], ww * vy);
-                    atomicAdd(&c.J[2 * nk + k], ww * vz);
+                    jadd(c, k, ww * vx);
+                    jadd(c, nk + k, ww * vy);
+                    jadd(c, 2 * nk + k, ww * vz);
                 }
-                atomicAdd(&c.J[cq * nk + k], ww);
+                jadd(c, cq * nk + k, ww);
             }
         return;
     }
@@ -263,12 +269,12 @@ __device__ __forceinline__ void sdeposit(const StreamArgs& c, const Win* win, do
             if (nod) {
                 const long k = long(jn[v]) * P.nx + ix[u];
                 const double w = f * wx[u] * wn[v] * c.ia[jn[v]];
-                if (mv) { atomicAdd(&c.J[k], w * vx); atomicAdd(&c.J[2 * nk + k], w * vz); }
-                atomicAdd(&c.J[cq * nk + k], w);
+                if (mv) { jadd(c, k, w * vx); jadd(c, 2 * nk + k, w * vz); }
+                jadd(c, cq * nk + k, w);
             }
             if (hal && mv) {
                 const long k = long(jh[v]) * P.nx + ix[u];
-                atomicAdd(&c.J[nk + k], f * wx[u] * wh[v] * c.id[jh[v]] * vy);
+                jadd(c, nk + k, f * wx[u] * wh[v] * c.id[jh[v]] * vy);
             }
         }
     }
@@ -415,6 +421,7 @@ __device__ __forceinline__ PState load_pstate(const Parts& S, long i) {
 template <int YF, bool ION, bool LASER>
 __device__ __forceinline__ void stream_index(const StreamArgs& c, const Win* win, PState p, Parts D, long i) {
     stream_one<YF, ION, LASER>(c, win, p);
+    if (c.skip_state) return;
     D.x[i] = p.x; D.y[i] = p.y; D.zeta[i] = p.zeta; D.px[i] = p.px; D.py[i] = p.py; D.pz[i] = p.pz;
     D.z[i] = p.z; D.q[i] = p.q; D.qm[i] = p.qm; D.alive[i] = p.alive;
 }
@@ -484,17 +491,17 @@ __global__ void __launch_bounds__(kStreamBlockMax) k_stream_halfcell(StreamArgs 
             }
         }
         __syncthreads();
-        if (YF == 1) {
+        if (YF == 1 && !c.Jf) {
             double* sl = tg.slab + t * wn;
             for (long r = threadIdx.x; r < wn; r += blockDim.x) sl[r] = win_lds[r];
-        } else                  // free-hand y: into the current planes (rows outside the walls hold nothing)
+        } else                  // into the current planes (free-hand y: rows outside the walls hold nothing)
             for (long r = threadIdx.x; r < wn; r += blockDim.x) {
                 const double v = win_lds[r];
                 if (v == 0) continue;
                 const int cc = int(r / pl), rr = int(r - long(cc) * pl), vv = rr / tg.WX, u = rr - vv * tg.WX;
-                const int gy = y0 + vv;
+                const int gy = YF ? wrap_index(y0 + vv, F.ny) : y0 + vv;
                 if (gy < 0 || gy >= F.ny) continue;
-                atomicAdd(&c.J[cc * F.nk + long(gy) * F.nx + wrap_index(x0 + u, F.nx)], v);
+                jadd(c, cc * F.nk + long(gy) * F.nx + wrap_index(x0 + u, F.nx), v);
             }
         if (c.npush) add_pushes(c.npush, cnt);
         return;

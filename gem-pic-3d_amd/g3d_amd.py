@@ -24,7 +24,8 @@ C_SI, E_SI, ME_SI, EPS0 = 299792458.0, 1.602176634e-19, 9.1093837015e-31, 8.8541
 
 class Glaser(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("a0", "kL", "waist", "tau", "zeta_c", "z_focus", "cep", "pol_angle")] + \
-               [("circular", C.c_int), ("helicity", C.c_int)]
+               [("circular", C.c_int), ("helicity", C.c_int), ("n_env", C.c_int), ("env_z0", C.c_double),
+                ("env_dz", C.c_double), ("env", C.c_double * 32)]
 
 
 class Ion(C.Structure):
@@ -44,7 +45,8 @@ class Config(C.Structure):
         ("start_fwhm", C.c_double), ("stop_fwhm", C.c_double), ("charge", C.c_double), ("gamma", C.c_double),
         ("sigx", C.c_double), ("sigy", C.c_double), ("sigz", C.c_double), ("emitx", C.c_double),
         ("emity", C.c_double), ("espread", C.c_double), ("zeta_center", C.c_double), ("nmacro", C.c_long),
-        ("seed", C.c_ulong), ("n_glasers", C.c_int), ("glasers", Glaser * 4), ("plasma_on", C.c_int),
+        ("seed", C.c_ulong), ("n_prof", C.c_int), ("prof_zmin", C.c_double), ("prof_zmax", C.c_double),
+        ("prof", C.c_double * 16), ("n_glasers", C.c_int), ("glasers", Glaser * 4), ("plasma_on", C.c_int),
         ("n0", C.c_double), ("r_max", C.c_double), ("ppc_x", C.c_int), ("ppc_y", C.c_int),
         ("smooth_length", C.c_double), ("n_pz", C.c_int), ("pz_points", C.c_double * 16),
         ("pz_values", C.c_double * 16), ("parabolic", C.c_int), ("channel_depth", C.c_double),
@@ -159,6 +161,29 @@ class Deck:
         g.a0 = a0; g.kL = kL; g.waist = waist; g.tau = tau; g.zeta_c = zeta_c; g.z_focus = z_focus; g.cep = cep
         g.pol_angle = pol_angle; g.circular = 0; g.helicity = 1
         self.c.n_glasers += 1
+        return self
+
+    def set_envelope(self, k, values, z0, dz):
+        """grid laser k: the piecewise-linear envelope a(zeta) through values at z0 + i dz in place of the Gaussian
+        of tau (the reference's envelope_function as a sum of hat functions)"""
+        if not 1 <= len(values) <= 32:
+            raise ValueError("an envelope has 1 ... 32 points")
+        g = self.c.glasers[k]
+        g.n_env = len(values); g.env_z0 = z0; g.env_dz = dz
+        for i, v in enumerate(values):
+            g.env[i] = v
+        return self
+
+    def set_profile(self, values, zmin, zmax):
+        """the bunch's current profile along zeta: the piecewise-linear profile_z through values at evenly spaced
+        points from zmin to zmax (the reference's distribution_z = function), sampled by its inverse CDF"""
+        if not 2 <= len(values) <= 16:
+            raise ValueError("a profile has 2 ... 16 points")
+        c = self.c
+        c.n_prof = len(values); c.prof_zmin = zmin; c.prof_zmax = zmax
+        for i, v in enumerate(values):
+            c.prof[i] = v
+        c.zeta_center = 0.5 * (zmin + zmax)
         return self
 
     def add_ion(self, mass, Z0, Zmax, atoms=1.0, mobile=True, ionization=False, ip_eV=(), name=""):

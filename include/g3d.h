@@ -42,6 +42,11 @@ typedef struct {
 typedef struct {
     double a0, kL, waist, tau, zeta_c, z_focus, cep, pol_angle;
     int circular, helicity;
+    /* envelope_function a(zeta) in place of the Gaussian of tau (src/sim_config.cpp:146-158, sim_fields.cpp:335): the
+     * piecewise-linear function through env[k] at zeta = env_z0 + k env_dz, k < n_env (0: Gaussian), zero beyond
+     * one step outside; evaluated as the sum of the hat functions env[k] max(0, 1 - |zeta - zeta_k| / env_dz) */
+    int n_env;
+    double env_z0, env_dz, env[32];
 } g3d_glaser;
 
 /* ion species (ion blocks), src/sim.hpp:118-127 */
@@ -73,6 +78,11 @@ typedef struct {
     double charge, gamma, sigx, sigy, sigz, emitx, emity, espread, zeta_center;
     long nmacro;
     unsigned long seed;
+    /* distribution_z = function (src/sim_bunches.cpp:115-132, 160): profile_z the piecewise-linear function through
+     * prof[k] at zeta = prof_zmin + k (prof_zmax - prof_zmin) / (n_prof - 1), k < n_prof (0: Gaussian in zeta of
+     * sigz), sampled by the inverse of its cumulative distribution; zeta_center is then the middle of the range */
+    int n_prof;
+    double prof_zmin, prof_zmax, prof[16];
     /* grid lasers */
     int n_glasers;
     g3d_glaser glasers[4];

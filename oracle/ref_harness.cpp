@@ -59,6 +59,22 @@ void stream_pass(Simulation& s, int n, int pass, double s0, double s1) {
     par_for(s.nstrm, [=](long i) { stream_one(scc, P[i]); });
 }
 
+// the text of a piecewise-linear function as the sum of its hat functions, the numbers printed to round-trip:
+// v[0]*max(0,1-abs(zeta-x_0)/h)+v[1]*max(...)+...
+std::string hat_text(const double* v, int n, double x0, double h) {
+    std::string t;
+    char b[256];
+    for (int k = 0; k < n; ++k) {
+        const double xk = x0 + k * h;
+        std::snprintf(b, sizeof b, "%s(%.17g)*max(0,1-abs(zeta-(%.17g))/(%.17g))", k ? "+" : "", v[k], xk, h);
+        t += b;
+    }
+    return t;
+}
+Expr hat_expr(const double* v, int n, double x0, double h) {
+    return Expr(hat_text(v, n, x0, h), {"zeta"}, {}, [](const std::string&, double&) { return false; });
+}
+
 Config make_config(const g3d_config& g) {
     Config c = Config::base_units(g.lambda0);
     c.nx = g.nx; c.Lx = g.Lx;
@@ -83,6 +99,7 @@ Config make_config(const g3d_config& g) {
         const g3d_glaser& s = g.glasers[k];
         L.a0 = s.a0; L.kL = s.kL; L.waist = s.waist; L.tau = s.tau; L.zeta_c = s.zeta_c; L.z_focus = s.z_focus;
         L.cep = s.cep; L.pol_angle = s.pol_angle; L.circular = s.circular; L.helicity = s.helicity;
+        if (s.n_env > 0) L.envelope = hat_expr(s.env, s.n_env, s.env_z0, s.env_dz);
         c.glasers.push_back(L);
     }
     c.plasma_on = g.plasma_on; c.n0 = g.n0; c.r_max = g.r_max; c.ppc_x = g.ppc_x; c.ppc_y = g.ppc_y;
@@ -101,6 +118,16 @@ Config make_config(const g3d_config& g) {
         c.ions.push_back(s);
     }
     c.freeze_bunch = g.freeze_bunch; c.start_relax = g.start_relax; c.box_mean = g.box_mean;
+    if (g.n_prof > 0) {              // distribution_z = function: the bunch list of one bunch with profile_z
+        Config::Bunch u;
+        u.charge = g.charge; u.gamma = g.gamma; u.sigx = g.sigx; u.sigy = g.sigy; u.sigz = g.sigz;
+        u.emitx = g.emitx; u.emity = g.emity; u.espread = g.espread; u.nmacro = g.nmacro; u.seed = g.seed;
+        u.zeta_min = g.prof_zmin; u.zeta_max = g.prof_zmax;
+        u.profile_z = hat_expr(g.prof, g.n_prof, g.prof_zmin, (g.prof_zmax - g.prof_zmin) / (g.n_prof - 1));
+        u.zeta_center = 0.5 * (u.zeta_min + u.zeta_max);
+        c.bunches.push_back(u);
+        c.zeta_center = u.zeta_center;
+    }
     return c;
 }
 

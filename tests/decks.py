@@ -129,3 +129,36 @@ def config0_deck():
           nmacro=1 << 18, seed=1, laser_on=1, laser_pol=1, a0=1.0, theta=2e-3, tau=30.0, start_fwhm=0.0,
           stop_fwhm=0.0)
     return d
+
+
+def config4_params(seed=4):
+    """the parameters of config 4: the 32 laser amplitudes a0(zeta) = 1.5 + N(0, 0.05) (seeded) and the 16 values of
+    the witness current profile (ones)"""
+    rng = np.random.default_rng(seed)
+    return 1.5 + 0.05 * rng.standard_normal(32), np.ones(16)
+
+
+def config4_deck(env=None, prof=None, n=128, nz=512, nwit=1 << 18, steps=20, mixed=0):
+    """Config 4: 128 x 128 modes over 120/k0, 512 slices of 0.5/k0, 20 steps of 10/k0; uniform plasma at 2e-3 n_c,
+    2 x 1 electrons per cell; the Gaussian driver (waist 25/k0) whose amplitude along zeta is the piecewise-linear
+    a0(zeta) through 32 values 2/k0 apart (its front 10/k0 behind the window's front); the witness of nwit electrons
+    at gamma 100 (sigma_r 2/k0, 5 pC) whose current profile is the piecewise-linear profile through 16 values over
+    zeta 90 ... 120/k0.  Smaller n, nz and nwit give the same deck on a coarser grid (the tests against the
+    reference)."""
+    e0, p0 = config4_params()
+    env = e0 if env is None else env
+    prof = p0 if prof is None else prof
+    h = 0.5 * 512 / nz
+    Lw = (nz - 1) * h
+    d = g3d_amd.Deck(LAMBDA0)
+    d.c.nx = n; d.c.Lx = 120.0
+    d.y_uniform(n, 120.0)
+    d.zeta_uniform(nz, Lw)
+    d.set(dz=10.0, z_end=10.0 * steps, sponge=8.0, sponge_y=8.0, move_x=0, move_y=0, plasma_on=1, n0=2e-3,
+          ppc_x=2, ppc_y=1, iterations=2, mixed=mixed, charge=5e-12, gamma=100.0, sigx=2.0, sigy=2.0, sigz=5.0,
+          emitx=0.0, emity=0.0, espread=0.0, nmacro=nwit, seed=11)
+    z_front = Lw - 10.0
+    d.add_grid_laser(a0=1.0, waist=25.0, tau=1.0, zeta_c=z_front - 31.0, z_focus=0.0)
+    d.set_envelope(0, list(env), z_front - 62.0, 2.0)
+    d.set_profile(list(prof), 90.0 * Lw / 255.5, 120.0 * Lw / 255.5)
+    return d

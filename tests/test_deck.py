@@ -81,6 +81,11 @@ plasma { density = 0.01 n_c }
 """
 
 
+def plain(v):
+    """a ctypes array as a list (compared by value)"""
+    return list(v) if isinstance(v, C.Array) else v
+
+
 def compare(path, overrides=()):
     mine = g3d_deck.read_deck(path, overrides)
     cfg, zeta, ynode, out = refpy.deck_config(path, overrides)
@@ -92,7 +97,7 @@ def compare(path, overrides=()):
         if name == "glasers":
             for k in range(cfg.n_glasers):
                 for f, _ in type(a[k])._fields_:
-                    assert getattr(a[k], f) == getattr(b[k], f), "glasers[%d].%s" % (k, f)
+                    assert plain(getattr(a[k], f)) == plain(getattr(b[k], f)), "glasers[%d].%s" % (k, f)
         elif name == "ions":
             for k in range(cfg.n_ions):
                 for f, _ in type(a[k])._fields_:
@@ -103,7 +108,7 @@ def compare(path, overrides=()):
         elif name in ("pz_points", "pz_values"):
             assert list(a)[:cfg.n_pz] == list(b)[:cfg.n_pz], name
         else:
-            assert a == b, "%s: %r (reader) against %r (reference)" % (name, a, b)
+            assert plain(a) == plain(b), "%s: %r (reader) against %r (reference)" % (name, a, b)
     np.testing.assert_array_equal(mine.zeta, zeta)
     np.testing.assert_array_equal(mine.ynode, ynode)
     for k, v in out.items():
