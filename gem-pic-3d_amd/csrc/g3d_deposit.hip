@@ -9,10 +9,12 @@
 //      (group, half-cell, point) and the values; cells a particle does not touch get the key ~0;
 //   2. a stable radix sort of (key, r) groups each cell's records in particle order;
 //   3. the values are copied into that order (one lane per record), so that
-//   4. one lane per cell (the head of its run of keys), which finds the end of its run by bisection, sums the run
-//      from contiguous memory in that order, from zero, as the reference does, and stores the sums into jw.  (A
-//      cell at the core of a dense bunch collects thousands of records: read through the record index, each
-//      addition waited for two dependent loads.)
+//   4. one lane per cell (the head of its run of keys) finds the end of its run by bisection; a short run it sums
+//      itself from contiguous memory in that order, from zero, as the reference does, and stores the sums into jw;
+//   5. a long run (a cell at the core of a dense bunch collects thousands of records) goes to a list, and one wave
+//      per long run sums it: the wave loads 64 records at a time with coalesced loads into LDS (the next chunk's
+//      loads in flight meanwhile) and one lane adds them in record order (the additions are the same sequence; a
+//      lane alone on a long run waited for its loads every few records).
 // The result is the reference's single-thread jw, bit for bit.
 #include <rocprim/device/device_radix_sort.hpp>
 
@@ -83,9 +85,24 @@ __global__ void __launch_bounds__(kT) k_dep_order(const unsigned long long* key,
     vs[3 * t] = v[0]; vs[3 * t + 1] = v[1]; vs[3 * t + 2] = v[2];
 }
 
-// one lane per sorted record; the head of each run of equal keys sums the run in record (= particle) order
+constexpr int kShortRun = 32;                      // longer runs are summed by a wave each (k_dep_sum_long)
+
+// the sums of one cell into jw
+__device__ __forceinline__ void dep_store(unsigned long long k, double s0, double s1, double s2, double* J, long nh,
+                                          long nk, int yf) {
+    const bool half = (k >> kCellBits) & 1;
+    const long cell = long(k & ((1ull << kCellBits) - 1));
+    const long hw = cell / nk, p = cell - hw * nk;
+    if (half) { J[(1 * nh + hw) * nk + p] = s1; return; }
+    J[(0 * nh + hw) * nk + p] = s0;
+    if (yf) J[(1 * nh + hw) * nk + p] = s1;
+    J[(2 * nh + hw) * nk + p] = s2;
+}
+
+// one lane per sorted record; the head of each run of equal keys sums a short run in record (= particle) order and
+// lists a long one
 __global__ void __launch_bounds__(kT) k_dep_sum(const unsigned long long* key, const double* vs, long n, double* J,
-                                                long nh, long nk, int yf) {
+                                                long nh, long nk, int yf, long* runs, int* nruns, int cap) {
     const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const unsigned long long k = key[i];
@@ -94,6 +111,10 @@ __global__ void __launch_bounds__(kT) k_dep_sum(const unsigned long long* key, c
     while (lo < hi) {
         const long mid = lo + ((hi - lo) >> 1);
         if (key[mid] == k) lo = mid + 1; else hi = mid;
+    }
+    if (lo - i > kShortRun) {
+        const int r = atomicAdd(nruns, 1);
+        if (r < cap) { runs[2 * r] = i; runs[2 * r + 1] = lo; return; }
     }
     // in blocks of kU records, the next block's loads in flight during this block's additions (the additions
     // themselves stay in record order)
@@ -114,13 +135,57 @@ __global__ void __launch_bounds__(kT) k_dep_sum(const unsigned long long* key, c
             for (int u = 0; u < 3 * kU; ++u) a[u] = b[u];
     }
     for (; t < lo; ++t) { s0 += vs[3 * t]; s1 += vs[3 * t + 1]; s2 += vs[3 * t + 2]; }
-    const bool half = (k >> kCellBits) & 1;
-    const long cell = long(k & ((1ull << kCellBits) - 1));
-    const long hw = cell / nk, p = cell - hw * nk;
-    if (half) { J[(1 * nh + hw) * nk + p] = s1; return; }
-    J[(0 * nh + hw) * nk + p] = s0;
-    if (yf) J[(1 * nh + hw) * nk + p] = s1;
-    J[(2 * nh + hw) * nk + p] = s2;
+    dep_store(k, s0, s1, s2, J, nh, nk, yf);
+}
+
+// one wave per listed long run (persistent: wave w takes runs w, w + waves, ...; every wave leaves when the list is
+// done): chunks of 64 records through LDS, lane 0 adds them in record order from zero
+constexpr int kRunWaves = 4;                        // waves per workgroup
+__global__ void __launch_bounds__(64 * kRunWaves) k_dep_sum_long(const unsigned long long* key, const double* vs,
+                                                                 const long* runs, const int* nruns, int cap,
+                                                                 double* J, long nh, long nk, int yf) {
+    __shared__ double buf[kRunWaves][2][3 * 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const long waves = long(gridDim.x) * kRunWaves;
+    const int nr = min(*nruns, cap);
+    for (long r = long(blockIdx.x) * kRunWaves + wv; r < nr; r += waves) {
+        const long i = runs[2 * r], lo = runs[2 * r + 1];
+        double s0 = 0, s1 = 0, s2 = 0;
+        int cur = 0;
+        // the first chunk into LDS
+        for (int c = lane; c < 3 * 64; c += 64) {
+            const long t = i + c / 3;
+            buf[wv][0][c] = t < lo ? vs[3 * i + c] : 0.0;
+        }
+        __builtin_amdgcn_wave_barrier();
+        for (long c0 = i; c0 < lo; c0 += 64) {
+            // the next chunk's loads in flight while lane 0 adds this one
+            double nx[3];
+            const long n0 = c0 + 64;
+            for (int q = 0; q < 3; ++q) {
+                const int c = lane + 64 * q;
+                nx[q] = n0 + c / 3 < lo ? vs[3 * n0 + c] : 0.0;
+            }
+            if (lane == 0) {
+                const int m = int(min(long(64), lo - c0));
+                const double* b = buf[wv][cur];
+                int u = 0;
+                // 16 records at a time: their 48 values into registers with one LDS round trip, then the additions
+                for (; u + 16 <= m; u += 16) {
+                    double w[48];
+#pragma unroll
+                    for (int q = 0; q < 48; ++q) w[q] = b[3 * u + q];
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) { s0 += w[3 * q]; s1 += w[3 * q + 1]; s2 += w[3 * q + 2]; }
+                }
+                for (; u < m; ++u) { s0 += b[3 * u]; s1 += b[3 * u + 1]; s2 += b[3 * u + 2]; }
+            }
+            for (int q = 0; q < 3; ++q) buf[wv][cur ^ 1][lane + 64 * q] = nx[q];
+            cur ^= 1;
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (lane == 0) dep_store(key[i], s0, s1, s2, J, nh, nk, yf);
+    }
 }
 }  // namespace
 
@@ -149,8 +214,15 @@ void bunch_deposit(const DepositArgs& a, Parts p, long np, DepositScratch& s) {
                                         s.rec1.data(), n, 0, end_bit, stream()), "beam deposit sort");
     hipLaunchKernelGGL(k_dep_order, dim3(blocks_for(n, kT)), dim3(kT), 0, stream(), s.key1.data(), s.rec1.data(),
                        s.val.data(), n, s.vs.data());
+    // long runs: at most one per kShortRun + 1 records
+    const int cap = int(n / (kShortRun + 1) + 1);
+    if (s.runs.size() < size_t(2 * cap)) s.runs.resize(size_t(2 * cap));
+    if (s.nruns.size() < 1) s.nruns.resize(1);
+    hip_check(hipMemsetAsync(s.nruns.data(), 0, sizeof(int), stream()), "hipMemsetAsync");
     hipLaunchKernelGGL(k_dep_sum, dim3(blocks_for(n, kT)), dim3(kT), 0, stream(), s.key1.data(), s.vs.data(), n, a.J,
-                       long(a.nh), a.nk, a.yf);
+                       long(a.nh), a.nk, a.yf, s.runs.data(), s.nruns.data(), cap);
+    hipLaunchKernelGGL(k_dep_sum_long, dim3(1024), dim3(64 * kRunWaves), 0, stream(), s.key1.data(), s.vs.data(),
+                       s.runs.data(), s.nruns.data(), cap, a.J, long(a.nh), a.nk, a.yf);
     launch_check("beam deposit sums");
 }
 

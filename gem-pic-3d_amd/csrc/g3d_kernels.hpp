@@ -95,6 +95,8 @@ struct DepositScratch {
     DevBuf<unsigned> rec0, rec1;
     DevBuf<double> val, vs;          // the record values, and the same in sorted order
     DevBuf<unsigned char> tmp;
+    DevBuf<long> runs;               // the long runs of equal keys: (first record, end) pairs
+    DevBuf<int> nruns;
 };
 // the reference's single-thread summation order, bit for bit (g3d_deposit.hip)
 void bunch_deposit(const DepositArgs& a, Parts p, long np, DepositScratch& s);
