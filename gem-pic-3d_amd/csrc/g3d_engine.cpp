@@ -52,6 +52,18 @@ Config Config::from_abi(const g3d_config& g) {
     if (g.n_prof < 0 || g.n_prof > 16 || g.n_prof == 1 || (g.n_prof && !(g.prof_zmax > g.prof_zmin)))
         throw std::invalid_argument("bunch profile: 2 ... 16 points on zeta_min < zeta_max");
     c.prof.assign(g.prof, g.prof + g.n_prof); c.prof_zmin = g.prof_zmin; c.prof_zmax = g.prof_zmax;
+    if (g.n_bunch_more < 0 || g.n_bunch_more > 3) throw std::invalid_argument("bunches: at most 4");
+    c.bunches.push_back(Config::Bunch{g.charge, g.gamma, g.sigx, g.sigy, g.sigz, g.emitx, g.emity, g.espread,
+                                      g.zeta_center, g.nmacro, g.seed, "bunch"});
+    for (int k = 0; k < g.n_bunch_more; ++k) {
+        const g3d_bunch& b = g.bunch_more[k];
+        std::string nm(b.name, strnlen(b.name, sizeof b.name));
+        if (nm.empty()) nm = "bunch" + std::to_string(k + 2);
+        for (const auto& o : c.bunches)
+            if (o.name == nm) throw std::invalid_argument("bunches: the name '" + nm + "' given twice");
+        c.bunches.push_back(Config::Bunch{b.charge, b.gamma, b.sigx, b.sigy, b.sigz, b.emitx, b.emity, b.espread,
+                                          b.zeta_center, b.nmacro, b.seed, nm});
+    }
     for (int k = 0; k < g.n_glasers; ++k)
         if (g.glasers[k].n_env < 0 || g.glasers[k].n_env > 32 || (g.glasers[k].n_env && !(g.glasers[k].env_dz > 0)))
             throw std::invalid_argument("grid laser envelope: 1 ... 32 points with a positive spacing");
@@ -558,29 +570,32 @@ void Sim::load_bunch() {
     const double ylo = yf ? -1e300 : cfg.ynode[1], yhi = yf ? 1e300 : cfg.ynode[cfg.ny - 1];
     const double yoff = yf ? 0.5 * cfg.Ly : 0;   // Fourier y: box coordinate, physical y = p.y + Y0
     std::vector<g3d_particle> mine;
-    if (cfg.nmacro > 0 && !cfg.prof.empty()) load_bunch_profile(mine, ylo, yhi, yoff);
-    else if (cfg.nmacro > 0) {
-        std::mt19937_64 rng(cfg.seed);
+    // the bunches one after the other, each from its own seed (species 0, -1, -2, ...; src/sim_bunches.cpp:92-217)
+    for (size_t kb = 0; kb < cfg.bunches.size(); ++kb) {
+        const Config::Bunch& B = cfg.bunches[kb];
+        if (B.nmacro <= 0) continue;
+        if (kb == 0 && !cfg.prof.empty()) { load_bunch_profile(mine, ylo, yhi, yoff); continue; }
+        std::mt19937_64 rng(B.seed);
         std::normal_distribution<double> G(0, 1);
-        const double Ne = cfg.charge / kE;
-        const double W = (Ne / cfg.nmacro) / (cfg.nc * std::pow(cfg.Lu, 3));
-        const double p0 = std::sqrt(cfg.gamma * cfg.gamma - 1);
+        const double Ne = B.charge / kE;
+        const double W = (Ne / B.nmacro) / (cfg.nc * std::pow(cfg.Lu, 3));
+        const double p0 = std::sqrt(B.gamma * B.gamma - 1);
         Axis zh = zax; zh.X = cfg.zeta.data();
-        for (long ib = 0; ib < cfg.nmacro; ++ib) {
+        for (long ib = 0; ib < B.nmacro; ++ib) {
             double x, y, z, px, py, pz;
-            x = 0 + cfg.sigx * G(rng); y = 0 + cfg.sigy * G(rng);
-            z = cfg.sigz * G(rng);
-            px = cfg.sigx > 0 ? cfg.emitx / cfg.sigx * G(rng) : 0;
-            py = cfg.sigy > 0 ? cfg.emity / cfg.sigy * G(rng) : 0;
-            pz = p0 * (1 + cfg.espread * G(rng));
+            x = 0 + B.sigx * G(rng); y = 0 + B.sigy * G(rng);
+            z = B.sigz * G(rng);
+            px = B.sigx > 0 ? B.emitx / B.sigx * G(rng) : 0;
+            py = B.sigy > 0 ? B.emity / B.sigy * G(rng) : 0;
+            pz = p0 * (1 + B.espread * G(rng));
             const double g = std::sqrt(1 + px * px + py * py + pz * pz);
             const double vx = px / g, vy = py / g, vz = pz / g;
             const double zz = z + 0.0;
             g3d_particle p{};
             p.x = x - vx * zz / vz + 0.5 * cfg.Lx; p.y = y - vy * zz / vz + yoff;
-            p.zeta = cfg.zeta_center + zz / vz;
+            p.zeta = B.zeta_center + zz / vz;
             p.px = px; p.py = py; p.pz = pz; p.w = W; p.alive = 1;
-            p.q = -1; p.qm = -1; p.sp = 0; p.z = 0;
+            p.q = -1; p.qm = -1; p.sp = -int(kb); p.z = 0;
             int hc; double f;
             zh.locate(p.zeta, hc, f);
             if (p.zeta < 0 || hc > N - 2 || p.y < ylo || p.y > yhi) { ++lost_total; continue; }
@@ -652,8 +667,8 @@ void Sim::init() {
     s = 0; istep = 0;
     if (cfg.self_fields) {
         starting_ = true;
-        currents();
-        comoving_field();
+        if (cfg.bunches.size() > 1) comoving_fields_per_bunch();
+        else { currents(); comoving_field(); }
         if (cfg.start_relax) relax_start_field();
         add_grid_lasers();
         march(0.0);
@@ -810,6 +825,34 @@ void Sim::comoving_field() {
     comoving_solve_ev(g, T3.data(), evl.data(), hcg.data(), wlo.data(), whi.data(), beta, gm2, S1.data(), S2.data());
     fh_eigen_back(S2.data(), evb.data(), S1.data(), ns, nx, ny, nk);
     fh_comoving_R(S1.data(), kxd.data(), iD.data(), beta, Rx.data(), Ry.data(), Ez.data(), Bz.data(), ns, nx, ny, nk);
+}
+
+// several bunches: the co-moving field of each with its own beta, the forward fields R summed (the equation is
+// linear), then the currents of all; the sponge's bound field keeps the mean beta of all bunch particles
+// (src/sim_bunches.cpp:218-248)
+void Sim::comoving_fields_per_bunch() {
+    ensure_layout(false, false);
+    const std::vector<g3d_particle> all = host_particles();
+    const long n = long(ns) * nk;
+    DevBuf<cx> sx(static_cast<size_t>(n)), sy(static_cast<size_t>(n));
+    for (int k = 0; k < int(cfg.bunches.size()); ++k) {
+        std::vector<g3d_particle> one;
+        for (const auto& p : all) if (p.sp == -k) one.push_back(p);
+        set_particles(one);
+        currents();
+        comoving_field();
+        axpy_planes(sx.data(), Rx.data(), 1, n);
+        axpy_planes(sy.data(), Ry.data(), 1, n);
+    }
+    set_particles(all);
+    currents();
+    copy_planes(Rx.data(), sx.data(), n);
+    copy_planes(Ry.data(), sy.data(), n);
+    double m[2] = {0, 0};
+    for (const auto& p : all)
+        if (p.alive == 1) { m[0] += p.w * p.pz / std::sqrt(1 + p.px * p.px + p.py * p.py + p.pz * p.pz); m[1] += p.w; }
+    beta_b = m[1] > 0 ? m[0] / m[1] : 1;
+    gm2_b = 1 - beta_b * beta_b;
 }
 
 void Sim::relax_start_field() {

@@ -33,7 +33,14 @@ struct Config {
     double charge = 0, gamma = 1, sigx = 0, sigy = 0, sigz = 0, emitx = 0, emity = 0, espread = 0, zeta_center = 0;
     long nmacro = 0;
     unsigned long seed = 1;
-    std::vector<double> prof;         // profile_z of the bunch (empty: Gaussian in zeta), on prof_zmin ... prof_zmax
+    struct Bunch {                    // the bunch blocks (src/sim.hpp:61-80): [0] the single-bunch fields above
+        double charge, gamma, sigx, sigy, sigz, emitx, emity, espread, zeta_center;
+        long nmacro;
+        unsigned long seed;
+        std::string name;
+    };
+    std::vector<Bunch> bunches;
+    std::vector<double> prof;         // profile_z of the first bunch (empty: Gaussian in zeta), prof_zmin ... prof_zmax
     double prof_zmin = 0, prof_zmax = 0;
     std::vector<g3d_glaser> glasers;
     bool plasma_on = false;
@@ -92,6 +99,7 @@ public:
     void collect_restreaming();
     void load_bunch();
     void load_bunch_profile(std::vector<g3d_particle>& mine, double ylo, double yhi, double yoff);
+    void comoving_fields_per_bunch();
     void window_shift_start();
     void window_shift_post();
     void shift_window(int nxs, int nys);

@@ -85,14 +85,19 @@ template <class T> std::vector<T> host_copy(const DevBuf<T>& b) {
 // beam-list species of the output (src/sim.hpp:132-143): the bunch, the trapped plasma electrons, then the electrons
 // injected by the ionization of each ion species
 std::vector<std::string> Sim::beam_names() const {
-    std::vector<std::string> v{"bunch", "trapped"};
+    std::vector<std::string> v;
+    for (const auto& b : cfg.bunches) v.push_back(b.name);
+    v.push_back("trapped");
     for (size_t k = 0; k < cfg.ions.size(); ++k) {
         const char* nm = cfg.ions[k].name;
         v.push_back("inj_" + (nm[0] ? std::string(nm, strnlen(nm, sizeof(cfg.ions[k].name))) : "ion" + std::to_string(k + 1)));
     }
     return v;
 }
-int Sim::beam_index(int sp) const { return is_bunch(sp) ? -sp : sp >= SP_INJECTED ? 2 + sp - SP_INJECTED : 1; }
+int Sim::beam_index(int sp) const {
+    const int nb = int(cfg.bunches.size());
+    return is_bunch(sp) ? -sp : sp >= SP_INJECTED ? nb + 1 + sp - SP_INJECTED : nb;
+}
 
 static Moments beam_moments(Sim& S, int nsl, double len) {
     const std::vector<g3d_particle> P = S.host_particles();

@@ -28,6 +28,12 @@ class Glaser(C.Structure):
                 ("env_dz", C.c_double), ("env", C.c_double * 32)]
 
 
+class Bunch(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("charge", "gamma", "sigx", "sigy", "sigz", "emitx", "emity", "espread",
+                                          "zeta_center")] + \
+               [("nmacro", C.c_long), ("seed", C.c_ulong), ("name", C.c_char * 16)]
+
+
 class Ion(C.Structure):
     _fields_ = [("mass", C.c_double), ("atoms", C.c_double), ("Z0", C.c_int), ("Zmax", C.c_int), ("mobile", C.c_int),
                 ("ionization", C.c_int), ("n_ip", C.c_int), ("ip_eV", C.c_double * 16), ("name", C.c_char * 16)]
@@ -46,7 +52,7 @@ class Config(C.Structure):
         ("sigx", C.c_double), ("sigy", C.c_double), ("sigz", C.c_double), ("emitx", C.c_double),
         ("emity", C.c_double), ("espread", C.c_double), ("zeta_center", C.c_double), ("nmacro", C.c_long),
         ("seed", C.c_ulong), ("n_prof", C.c_int), ("prof_zmin", C.c_double), ("prof_zmax", C.c_double),
-        ("prof", C.c_double * 16), ("n_glasers", C.c_int), ("glasers", Glaser * 4), ("plasma_on", C.c_int),
+        ("prof", C.c_double * 16), ("n_bunch_more", C.c_int), ("bunch_more", Bunch * 3), ("n_glasers", C.c_int), ("glasers", Glaser * 4), ("plasma_on", C.c_int),
         ("n0", C.c_double), ("r_max", C.c_double), ("ppc_x", C.c_int), ("ppc_y", C.c_int),
         ("smooth_length", C.c_double), ("n_pz", C.c_int), ("pz_points", C.c_double * 16),
         ("pz_values", C.c_double * 16), ("parabolic", C.c_int), ("channel_depth", C.c_double),
@@ -172,6 +178,19 @@ class Deck:
         g.n_env = len(values); g.env_z0 = z0; g.env_dz = dz
         for i, v in enumerate(values):
             g.env[i] = v
+        return self
+
+    def add_bunch(self, name, charge, gamma, sigx, sigy, sigz, zeta_center, nmacro, seed, emitx=0.0, emity=0.0,
+                  espread=0.0):
+        """a further Gaussian electron bunch (the reference's bunch2, bunch3, ...; species -1, -2, -3), in code
+        units; the first bunch is the deck's single-bunch fields"""
+        if self.c.n_bunch_more >= 3:
+            raise ValueError("at most 4 bunches")
+        b = self.c.bunch_more[self.c.n_bunch_more]
+        b.charge, b.gamma, b.sigx, b.sigy, b.sigz = charge, gamma, sigx, sigy, sigz
+        b.emitx, b.emity, b.espread, b.zeta_center = emitx, emity, espread, zeta_center
+        b.nmacro, b.seed, b.name = nmacro, seed, name.encode()[:15]
+        self.c.n_bunch_more += 1
         return self
 
     def set_profile(self, values, zmin, zmax):

@@ -5,10 +5,10 @@ and `bunch` blocks named laser2, ion2, ...; values with physical units; command-
 
 `read_deck(path)` returns the engine's run configuration (g3d_amd.Deck) for the blocks and keys the march uses:
 grid (Fourier or table y, uniform or table zeta), laser (plane_wave, gaussian), plasma (uniform or parabolic, the z
-ramp), ion (by element or by mass, ADK), one Gaussian electron bunch, numerics and the native output settings.
-Decks that need what the engine does not have (expressions and the block `constants`, file lasers, profile and
-envelope functions, density functions, several or flat-top bunches, undulators and magnets, openPMD output, diag
-groups) are refused with an error that names the key; so are unknown keys, as in the reference
+ramp), ion (by element or by mass, ADK), Gaussian electron bunches (bunch, bunch2, ... up to 4), numerics and the
+native output settings.  Decks that need what the engine does not have (expressions and the block `constants`, file
+lasers, profile and envelope functions, density functions, flat-top or positron bunches, undulators and magnets,
+openPMD output, diag groups) are refused with an error that names the key; so are unknown keys, as in the reference
 (Deck::check_unused, src/deck.cpp:143-148).
 """
 import math
@@ -234,7 +234,7 @@ def deck_from_text(text, overrides=(), name="(deck)"):
     for b in blocks:
         base = b.rstrip("0123456789")
         if b in ("constants", "diag") or b.startswith("diag.") or base in (
-                "undulator", "quadrupole", "plasma_lens", "solenoid", "chicane") or (base == "bunch" and b != "bunch"):
+                "undulator", "quadrupole", "plasma_lens", "solenoid", "chicane"):
             raise DeckError("%s: block '%s' is not supported by this engine" % (name, b))
     lam = d.num("grid", "lambda0", "length",
                 d.num("laser", "wavelength", "length") if d.has("laser", "wavelength") else 1e-6)
@@ -514,43 +514,64 @@ def _ions(d, deck, nc):
 
 
 def _bunch(d, deck, L, Ld):
-    """the bunch block: one Gaussian electron bunch with random shot noise (src/sim_bunches.cpp:6-82)"""
+    """the bunch blocks bunch, bunch2, ...: Gaussian electron bunches with random shot noise (src/sim_bunches.cpp:
+    6-82); the first sets the single-bunch fields, the others are the further bunches (species -1, -2, -3)"""
     c = deck.c
     lw = float(deck.zeta[-1])
-    b = "bunch"
-    if not d.has_block(b):
+    if not d.has_block("bunch"):
+        if d.has_block("bunch2"):
+            raise DeckError("bunch2 without a block bunch")
         c.nmacro, c.charge, c.gamma, c.zeta_center = 0, 0.0, 1.0, 0.75 * lw
         return
-    if d.str(b, "name", b) != b:
-        raise DeckError("bunch.name: this engine names its one bunch 'bunch'")
-    for k in ("file", "gamma_function", "ux_function", "uy_function", "profile_z", "profile_r", "noise_wavelength",
-              "beamlet"):
-        d.refuse(b, k)
-    if d.str(b, "species", "electron") != "electron":
-        raise DeckError("bunch.species: this engine loads electron bunches")
-    c.charge = abs(d.num(b, "charge", "charge"))
-    if d.has(b, "energy") + d.has(b, "gamma") != 1:
-        raise DeckError(b + ": give one of energy, gamma")
-    c.gamma = d.num(b, "gamma", None) if d.has(b, "gamma") else 1 + d.num(b, "energy", "energy") / MC2_EV
-    if not c.gamma > 1:
-        raise DeckError(b + ": gamma must be > 1")
-    if d.str(b, "distribution_r", "gaussian") != "gaussian":
-        raise DeckError("bunch.distribution_r: this engine loads Gaussian bunches")
-    c.sigx = L(b, "sigma_x")
-    c.sigy = Ld(b, "sigma_y", c.sigx)
-    if d.str(b, "distribution_z", "gaussian") != "gaussian":
-        raise DeckError("bunch.distribution_z: this engine loads Gaussian bunches")
-    c.sigz = L(b, "sigma_z")
-    c.emitx = Ld(b, "emittance_x", 0.0)
-    c.emity = Ld(b, "emittance_y", c.emitx)
-    c.espread = d.num(b, "energy_spread", None, 0.0)
-    c.zeta_center = Ld(b, "zeta_center", 0.75 * lw)
-    if Ld(b, "x0", 0.0) != 0 or Ld(b, "y0", 0.0) != 0:
-        raise DeckError("bunch: x0 and y0 other than 0 are not supported by this engine")
-    c.nmacro = d.integer(b, "macroparticles", 100000)
-    if d.str(b, "shot_noise", "random") != "random":
-        raise DeckError("bunch.shot_noise: this engine draws random shot noise")
-    c.seed = d.integer(b, "seed", 1)
+    names = []
+    k = 1
+    while d.has_block("bunch" if k == 1 else "bunch%d" % k):
+        b = "bunch" if k == 1 else "bunch%d" % k
+        name = d.str(b, "name", b)
+        if k == 1 and name != "bunch":
+            raise DeckError("bunch.name: this engine names its first bunch 'bunch'")
+        if name in names:
+            raise DeckError("%s.name: '%s' given twice" % (b, name))
+        names.append(name)
+        for key in ("file", "gamma_function", "ux_function", "uy_function", "profile_z", "profile_r",
+                    "noise_wavelength", "beamlet"):
+            d.refuse(b, key)
+        if d.str(b, "species", "electron") != "electron":
+            raise DeckError(b + ".species: this engine loads electron bunches")
+        charge = abs(d.num(b, "charge", "charge"))
+        if d.has(b, "energy") + d.has(b, "gamma") != 1:
+            raise DeckError(b + ": give one of energy, gamma")
+        gamma = d.num(b, "gamma", None) if d.has(b, "gamma") else 1 + d.num(b, "energy", "energy") / MC2_EV
+        if not gamma > 1:
+            raise DeckError(b + ": gamma must be > 1")
+        if d.str(b, "distribution_r", "gaussian") != "gaussian":
+            raise DeckError(b + ".distribution_r: this engine loads Gaussian bunches")
+        sigx = L(b, "sigma_x")
+        sigy = Ld(b, "sigma_y", sigx)
+        if d.str(b, "distribution_z", "gaussian") != "gaussian":
+            raise DeckError(b + ".distribution_z: this engine loads Gaussian bunches")
+        sigz = L(b, "sigma_z")
+        emitx = Ld(b, "emittance_x", 0.0)
+        emity = Ld(b, "emittance_y", emitx)
+        espread = d.num(b, "energy_spread", None, 0.0)
+        zc = Ld(b, "zeta_center", 0.75 * lw)
+        if Ld(b, "x0", 0.0) != 0 or Ld(b, "y0", 0.0) != 0:
+            raise DeckError(b + ": x0 and y0 other than 0 are not supported by this engine")
+        nmacro = d.integer(b, "macroparticles", 100000)
+        if d.str(b, "shot_noise", "random") != "random":
+            raise DeckError(b + ".shot_noise: this engine draws random shot noise")
+        seed = d.integer(b, "seed", k)
+        if k == 1:
+            c.charge, c.gamma, c.sigx, c.sigy, c.sigz = charge, gamma, sigx, sigy, sigz
+            c.emitx, c.emity, c.espread, c.zeta_center, c.nmacro, c.seed = emitx, emity, espread, zc, nmacro, seed
+        else:
+            if k > 4:
+                raise DeckError(b + ": at most 4 bunches in this engine")
+            if len(name.encode()) > 15:
+                raise DeckError(b + ".name: at most 15 characters in this engine")
+            deck.add_bunch(name, charge, gamma, sigx, sigy, sigz, zc, nmacro, seed, emitx=emitx, emity=emity,
+                           espread=espread)
+        k += 1
 
 
 def _output(d, Lu):

@@ -49,6 +49,15 @@ typedef struct {
     double env_z0, env_dz, env[32];
 } g3d_glaser;
 
+/* a bunch block after the first (bunch2, bunch3, ...; src/sim.hpp:61-80 Bunch): Gaussian electrons with their own
+ * seed and name; the first bunch is the single-bunch fields of g3d_config, named "bunch" */
+typedef struct {
+    double charge, gamma, sigx, sigy, sigz, emitx, emity, espread, zeta_center;
+    long nmacro;
+    unsigned long seed;
+    char name[16];
+} g3d_bunch;
+
 /* ion species (ion blocks), src/sim.hpp:118-127 */
 typedef struct {
     double mass, atoms;
@@ -83,6 +92,10 @@ typedef struct {
      * sigz), sampled by the inverse of its cumulative distribution; zeta_center is then the middle of the range */
     int n_prof;
     double prof_zmin, prof_zmax, prof[16];
+    /* further bunches (species -1, -2, -3): their co-moving start fields are solved each with its own beta and
+     * summed (src/sim_bunches.cpp:218-248) */
+    int n_bunch_more;
+    g3d_bunch bunch_more[3];
     /* grid lasers */
     int n_glasers;
     g3d_glaser glasers[4];

@@ -118,15 +118,27 @@ Config make_config(const g3d_config& g) {
         c.ions.push_back(s);
     }
     c.freeze_bunch = g.freeze_bunch; c.start_relax = g.start_relax; c.box_mean = g.box_mean;
-    if (g.n_prof > 0) {              // distribution_z = function: the bunch list of one bunch with profile_z
-        Config::Bunch u;
+    if (g.n_prof > 0 || g.n_bunch_more > 0) {   // the bunch list: the first bunch (distribution_z = function
+        Config::Bunch u;                            // with profile_z), then the further bunch blocks
         u.charge = g.charge; u.gamma = g.gamma; u.sigx = g.sigx; u.sigy = g.sigy; u.sigz = g.sigz;
         u.emitx = g.emitx; u.emity = g.emity; u.espread = g.espread; u.nmacro = g.nmacro; u.seed = g.seed;
-        u.zeta_min = g.prof_zmin; u.zeta_max = g.prof_zmax;
-        u.profile_z = hat_expr(g.prof, g.n_prof, g.prof_zmin, (g.prof_zmax - g.prof_zmin) / (g.n_prof - 1));
-        u.zeta_center = 0.5 * (u.zeta_min + u.zeta_max);
+        u.zeta_center = g.zeta_center;
+        if (g.n_prof > 0) {
+            u.zeta_min = g.prof_zmin; u.zeta_max = g.prof_zmax;
+            u.profile_z = hat_expr(g.prof, g.n_prof, g.prof_zmin, (g.prof_zmax - g.prof_zmin) / (g.n_prof - 1));
+            u.zeta_center = 0.5 * (u.zeta_min + u.zeta_max);
+            c.zeta_center = u.zeta_center;
+        }
         c.bunches.push_back(u);
-        c.zeta_center = u.zeta_center;
+        for (int k = 0; k < g.n_bunch_more; ++k) {
+            const g3d_bunch& b = g.bunch_more[k];
+            Config::Bunch v;
+            v.name = b.name[0] ? std::string(b.name, strnlen(b.name, sizeof b.name)) : "bunch" + std::to_string(k + 2);
+            v.charge = b.charge; v.gamma = b.gamma; v.sigx = b.sigx; v.sigy = b.sigy; v.sigz = b.sigz;
+            v.emitx = b.emitx; v.emity = b.emity; v.espread = b.espread; v.zeta_center = b.zeta_center;
+            v.nmacro = b.nmacro; v.seed = b.seed;
+            c.bunches.push_back(v);
+        }
     }
     return c;
 }
@@ -439,6 +451,16 @@ int ref_deck_config(const char* path, const char* const* ov, int nov, g3d_config
             std::strncpy(i.name, s.name.c_str(), sizeof i.name - 1);
         }
         g->freeze_bunch = c.freeze_bunch; g->start_relax = c.start_relax; g->box_mean = c.box_mean;
+        if (c.bunches.size() > 4) throw std::runtime_error("more bunches than the POD holds");
+        g->n_bunch_more = c.bunches.empty() ? 0 : int(c.bunches.size()) - 1;
+        for (int k = 0; k < g->n_bunch_more; ++k) {
+            const Config::Bunch& u = c.bunches[size_t(k) + 1];
+            g3d_bunch& b = g->bunch_more[k];
+            b.charge = u.charge; b.gamma = u.gamma; b.sigx = u.sigx; b.sigy = u.sigy; b.sigz = u.sigz;
+            b.emitx = u.emitx; b.emity = u.emity; b.espread = u.espread; b.zeta_center = u.zeta_center;
+            b.nmacro = u.nmacro; b.seed = u.seed;
+            std::strncpy(b.name, u.name.c_str(), sizeof b.name - 1);
+        }
         const auto& o = c.groups.at(0);
         const double v[10] = {double(c.every), double(o.beam_every), double(o.fields), double(o.beams),
                               double(o.axis), double(c.dump_particles), double(c.frames), c.slice_length,

@@ -75,6 +75,15 @@ ion { mass = 2 u ; ionization_potentials = 10, 20 eV ; charge_state = 1 ; ioniza
 ion { element = H ; fraction = 0.05 ; mobile = true }
 """
 
+BUNCHES = """
+grid { nx = 16 ; ny = 16 ; Lx = 50 um ; nzeta = 33 ; window_length = 30 um ; dz = 1 um ; z_end = 10 um }
+bunch { charge = 10 pC ; gamma = 1000 ; sigma_x = 2 um ; sigma_z = 2 um ; zeta_center = 20 um
+  macroparticles = 1000 }
+bunch { charge = 3 pC ; energy = 50 MeV ; sigma_x = 1 um ; sigma_y = 1.5 um ; sigma_z = 1 um ; zeta_center = 8 um
+  macroparticles = 500 ; name = witness ; emittance_x = 0.1 um ; energy_spread = 0.01 }
+bunch { charge = 1 pC ; gamma = 200 ; sigma_x = 1 um ; sigma_z = 1 um ; seed = 9 }
+"""
+
 MINIMAL = """
 grid { nx = 8 ; ny = 8 ; Lx = 50 um ; nzeta = 9 ; window_length = 30 um ; dz = 1 um ; z_end = 10 um }
 plasma { density = 0.01 n_c }
@@ -105,6 +114,10 @@ def compare(path, overrides=()):
                     if f == "ip_eV":
                         va, vb = list(va)[:a[k].n_ip], list(vb)[:b[k].n_ip]
                     assert va == vb, "ions[%d].%s: %r %r" % (k, f, va, vb)
+        elif name == "bunch_more":
+            for k in range(cfg.n_bunch_more):
+                for f, _ in type(a[k])._fields_:
+                    assert plain(getattr(a[k], f)) == plain(getattr(b[k], f)), "bunch_more[%d].%s" % (k, f)
         elif name in ("pz_points", "pz_values"):
             assert list(a)[:cfg.n_pz] == list(b)[:cfg.n_pz], name
         else:
@@ -123,7 +136,8 @@ def write(tmp_path, text, name="d.deck"):
     return p
 
 
-@pytest.mark.parametrize("text", [FOURIER, FREEHAND, MINIMAL], ids=["fourier", "freehand", "minimal"])
+@pytest.mark.parametrize("text", [FOURIER, FREEHAND, MINIMAL, BUNCHES], ids=["fourier", "freehand", "minimal",
+                                                                        "bunches"])
 def test_reader_matches_reference(tmp_path, text):
     compare(write(tmp_path, text))
 
@@ -187,8 +201,8 @@ def test_both_refuse(tmp_path, case):
 UNSUPPORTED = {
     "expression": (MINIMAL.replace("Lx = 50 um", "Lx = 2 * 25 um"), "expressions"),
     "constants": (MINIMAL + "constants { w = 5 um }", "constants"),
-    "second bunch": (MINIMAL + "bunch { charge = 1 pC ; gamma = 10 ; sigma_x = 1 um ; sigma_z = 1 um }\n"
-                     "bunch { charge = 1 pC ; gamma = 10 ; sigma_x = 1 um ; sigma_z = 1 um }", "bunch2"),
+    "fifth bunch": (MINIMAL + "bunch { charge = 1 pC ; gamma = 10 ; sigma_x = 1 um ; sigma_z = 1 um }\n" * 5,
+                    "at most 4"),
     "flat-top": (MINIMAL + "bunch { charge = 1 pC ; gamma = 10 ; sigma_x = 1 um ; distribution_z = flattop ; "
                  "length = 3 um }", "distribution_z"),
     "envelope": (MINIMAL + "laser { type = gaussian ; a0 = 1 ; waist = 5 um ; envelope_function = 1 }",
