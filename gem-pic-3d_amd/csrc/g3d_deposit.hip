@@ -25,10 +25,11 @@ namespace g3d {
 namespace {
 constexpr int kT = 256;
 constexpr unsigned long long kNone = ~0ull;
-constexpr int kCellBits = 40;                       // cell = hw nk + point < 2^40; bit 40: the half-node group
+// key of a record: cell = hw nk + point in the low cb bits (cb: the bits of nh nk), the half-node group in bit cb;
+// the sort looks at bits 0 .. cb+1 only, where the empty key ~0 has all ones and so follows every cell
 
-__global__ void __launch_bounds__(kT) k_dep_emit(DepositArgs c, Parts P, long np, int R, unsigned long long* key,
-                                                 unsigned* rec, double* val) {
+__global__ void __launch_bounds__(kT) k_dep_emit(DepositArgs c, Parts P, long np, int R, int cb,
+                                                 unsigned long long* key, unsigned* rec, double* val) {
     const long ip = long(blockIdx.x) * blockDim.x + threadIdx.x;
     if (ip >= np) return;
     const long r0 = ip * R;
@@ -68,7 +69,7 @@ __global__ void __launch_bounds__(kT) k_dep_emit(DepositArgs c, Parts P, long np
                 V[3 * s] = w * vx / vz; V[3 * s + 1] = 0; V[3 * s + 2] = w;
             }
             if (hal) {
-                key[r0 + 9 + s] = (1ull << kCellBits) | (base + (unsigned long long)(long(jh[v]) * c.nx + ix[u]));
+                key[r0 + 9 + s] = (1ull << cb) | (base + (unsigned long long)(long(jh[v]) * c.nx + ix[u]));
                 V[3 * (9 + s)] = 0; V[3 * (9 + s) + 2] = 0;
                 V[3 * (9 + s) + 1] = f * wx[u] * wh[v] * c.id[jh[v]] * vy / vz;
             }
@@ -88,10 +89,10 @@ __global__ void __launch_bounds__(kT) k_dep_order(const unsigned long long* key,
 constexpr int kShortRun = 32;                      // longer runs are summed by a wave each (k_dep_sum_long)
 
 // the sums of one cell into jw
-__device__ __forceinline__ void dep_store(unsigned long long k, double s0, double s1, double s2, double* J, long nh,
-                                          long nk, int yf) {
-    const bool half = (k >> kCellBits) & 1;
-    const long cell = long(k & ((1ull << kCellBits) - 1));
+__device__ __forceinline__ void dep_store(unsigned long long k, int cb, double s0, double s1, double s2, double* J,
+                                          long nh, long nk, int yf) {
+    const bool half = (k >> cb) & 1;
+    const long cell = long(k & ((1ull << cb) - 1));
     const long hw = cell / nk, p = cell - hw * nk;
     if (half) { J[(1 * nh + hw) * nk + p] = s1; return; }
     J[(0 * nh + hw) * nk + p] = s0;
@@ -102,7 +103,7 @@ __device__ __forceinline__ void dep_store(unsigned long long k, double s0, doubl
 // one lane per sorted record; the head of each run of equal keys sums a short run in record (= particle) order and
 // lists a long one
 __global__ void __launch_bounds__(kT) k_dep_sum(const unsigned long long* key, const double* vs, long n, double* J,
-                                                long nh, long nk, int yf, long* runs, int* nruns, int cap) {
+                                                long nh, long nk, int yf, int cb, long* runs, int* nruns, int cap) {
     const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const unsigned long long k = key[i];
@@ -135,7 +136,7 @@ __global__ void __launch_bounds__(kT) k_dep_sum(const unsigned long long* key, c
             for (int u = 0; u < 3 * kU; ++u) a[u] = b[u];
     }
     for (; t < lo; ++t) { s0 += vs[3 * t]; s1 += vs[3 * t + 1]; s2 += vs[3 * t + 2]; }
-    dep_store(k, s0, s1, s2, J, nh, nk, yf);
+    dep_store(k, cb, s0, s1, s2, J, nh, nk, yf);
 }
 
 // one wave per listed long run (persistent: wave w takes runs w, w + waves, ...; every wave leaves when the list is
@@ -143,7 +144,7 @@ __global__ void __launch_bounds__(kT) k_dep_sum(const unsigned long long* key, c
 constexpr int kRunWaves = 4;                        // waves per workgroup
 __global__ void __launch_bounds__(64 * kRunWaves) k_dep_sum_long(const unsigned long long* key, const double* vs,
                                                                  const long* runs, const int* nruns, int cap,
-                                                                 double* J, long nh, long nk, int yf) {
+                                                                 double* J, long nh, long nk, int yf, int cb) {
     __shared__ double buf[kRunWaves][2][3 * 64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const long waves = long(gridDim.x) * kRunWaves;
@@ -184,7 +185,7 @@ __global__ void __launch_bounds__(64 * kRunWaves) k_dep_sum_long(const unsigned 
             cur ^= 1;
             __builtin_amdgcn_wave_barrier();
         }
-        if (lane == 0) dep_store(key[i], s0, s1, s2, J, nh, nk, yf);
+        if (lane == 0) dep_store(key[i], cb, s0, s1, s2, J, nh, nk, yf);
     }
 }
 }  // namespace
@@ -194,17 +195,19 @@ void bunch_deposit(const DepositArgs& a, Parts p, long np, DepositScratch& s) {
     const int R = a.yf ? 9 : 18;
     const long n = np * R;
     if (n > long(0xffffffffu)) throw std::runtime_error("beam deposit: too many contribution records");
-    if (long((1ull << kCellBits) / (unsigned long long)a.nk) < a.nh) throw std::runtime_error("beam deposit: grid too large");
+    int cb = 1;                                     // the bits of the cells hw nk + point
+    while ((1ull << cb) < (unsigned long long)a.nh * (unsigned long long)a.nk) ++cb;
+    if (cb > 60) throw std::runtime_error("beam deposit: grid too large");
     if (s.key0.size() < size_t(n)) {
         const size_t cap = size_t(n) + size_t(n) / 4;
         s.key0.resize(cap); s.key1.resize(cap); s.rec0.resize(cap); s.rec1.resize(cap); s.val.resize(3 * cap);
         s.vs.resize(3 * cap);
     }
-    hipLaunchKernelGGL(k_dep_emit, dim3(blocks_for(np, kT)), dim3(kT), 0, stream(), a, p, np, R, s.key0.data(),
+    hipLaunchKernelGGL(k_dep_emit, dim3(blocks_for(np, kT)), dim3(kT), 0, stream(), a, p, np, R, cb, s.key0.data(),
                        s.rec0.data(), s.val.data());
     launch_check("beam deposit records");
-    // bits 0 .. 41: the cell, the group bit, and above every valid key the low bits of the empty key ~0
-    const unsigned end_bit = kCellBits + 2;
+    // bits 0 .. cb+1: the cell, the group bit, and above every valid key the low bits of the empty key ~0
+    const unsigned end_bit = unsigned(cb) + 2;
     size_t bytes = 0;
     hip_check(rocprim::radix_sort_pairs(nullptr, bytes, s.key0.data(), s.key1.data(), s.rec0.data(), s.rec1.data(),
                                         n, 0, end_bit, stream()), "beam deposit sort size");
@@ -220,9 +223,9 @@ void bunch_deposit(const DepositArgs& a, Parts p, long np, DepositScratch& s) {
     if (s.nruns.size() < 1) s.nruns.resize(1);
     hip_check(hipMemsetAsync(s.nruns.data(), 0, sizeof(int), stream()), "hipMemsetAsync");
     hipLaunchKernelGGL(k_dep_sum, dim3(blocks_for(n, kT)), dim3(kT), 0, stream(), s.key1.data(), s.vs.data(), n, a.J,
-                       long(a.nh), a.nk, a.yf, s.runs.data(), s.nruns.data(), cap);
+                       long(a.nh), a.nk, a.yf, cb, s.runs.data(), s.nruns.data(), cap);
     hipLaunchKernelGGL(k_dep_sum_long, dim3(1024), dim3(64 * kRunWaves), 0, stream(), s.key1.data(), s.vs.data(),
-                       s.runs.data(), s.nruns.data(), cap, a.J, long(a.nh), a.nk, a.yf);
+                       s.runs.data(), s.nruns.data(), cap, a.J, long(a.nh), a.nk, a.yf, cb);
     launch_check("beam deposit sums");
 }
 
