@@ -117,6 +117,7 @@ int g3d_sim_call(g3d_sim* sim, const char* op_, const double* a, int na) {
         else if (op == "set_starting") { need(1); s.starting_ = a[0] != 0; }
         else if (op == "j0_from_jk") s.j0_from_jk();
         else if (op == "line_pivots") { need(1); s.line_pivots_on = a[0] != 0; s.piv_ok = false; }
+        else if (op == "bpush_windows") { need(1); s.bpush_windows = a[0] != 0; }
         else { g3d::set_error("unknown op " + op); rc = 3; return; }
         g3d::hip_check(hipStreamSynchronize(g3d::stream()), "synchronize");
     });

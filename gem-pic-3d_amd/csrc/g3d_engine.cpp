@@ -247,6 +247,7 @@ Sim::Sim(const Config& c) : cfg(c) {
     counters.resize(4);
     cent_dev.resize(3);
     { const char* e = getenv("G3D_LINE_PIVOTS"); line_pivots_on = !(e && atoi(e) == 0); }
+    { const char* e = getenv("G3D_BPUSH_WIN"); bpush_windows = !(e && atoi(e) == 0); }
     setup_ions();
     if (cfg.plasma_on) setup_tiles();
 }
@@ -712,8 +713,10 @@ void Sim::push(double s0, double s1) {
     c.hmin = 1e300;
     for (int m = 0; m + 1 < N; ++m) c.hmin = std::min(c.hmin, cfg.zeta[m + 1] - cfg.zeta[m]);
     c.ylo = yf ? -1e300 : cfg.ynode[1]; c.yhi = yf ? 1e300 : cfg.ynode[cfg.ny - 1];
+    // the binned push with field windows (off: one lane per particle gathering from F; the same results)
     tic(F_PUSH);
-    bunch_push(c, beam.view(), np, s0);
+    if (bpush_windows && np >= 4096) bunch_push_windowed(c, beam.view(), np, s0, bpush_scratch);
+    else bunch_push(c, beam.view(), np, s0);
     toc(F_PUSH);
 }
 

@@ -142,6 +142,8 @@ public:
     // device arrays in the reference's layouts
     DevBuf<cx> Rx, Ry, Ez, Bz, Lx, Ly, j0x, j0y, jkx, jky, jkz, S1, S2, T3;
     DepositScratch dep_scratch;      // ordered beam deposit
+    BpushScratch bpush_scratch;      // binned beam push with field windows
+    bool bpush_windows = true;       // the beam push bins its particles and stages field windows (G3D_BPUSH_WIN=0: off)
     DevBuf<cx> js0x, js0y, js0z, jsn, jsin, jsi0, jpred, dlt, bk, sp3, tp3;
     DevBuf<double> jw, jsr, rho_lines, rho_lines_i;
     DevBuf<float> jsr_f, slab_f;      // FP32-mixed pass: the half-cell current and the tile slabs

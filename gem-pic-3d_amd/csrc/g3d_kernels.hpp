@@ -80,6 +80,16 @@ void bin_tiles(Parts S, Parts D, long n, const TileGeom& tg, double dx, double d
 void tile_chunks(const TileGeom& tg, int* xtile, long* xstart, int* xcount);
 
 void bunch_push(const PushArgs& a, Parts p, long np, double s0);
+// the same push with the particles binned by slice pair and transverse tile and the fields of each bin staged in LDS
+// (Fourier y with self-fields; otherwise the plain push); the results are the plain push's, bit for bit
+struct BpushScratch {
+    DevBuf<unsigned long long> key0, key1;
+    DevBuf<unsigned> idx0, idx1;
+    DevBuf<unsigned char> tmp;
+    DevBuf<long> chunk;
+    DevBuf<int> nchunk;
+};
+void bunch_push_windowed(const PushArgs& a, Parts p, long np, double s0, BpushScratch& s);
 // beam deposit into jw [3][ns+3][nk] (src/sim_particles.cpp:66-106)
 struct DepositArgs {
     Axis zax, yax;

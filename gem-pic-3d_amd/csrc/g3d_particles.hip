@@ -3,87 +3,14 @@
 // One lane per particle over structure-of-arrays streams.
 #include <algorithm>
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "g3d_kernels.hpp"
 #include "g3d_runtime.hpp"
 
 namespace g3d {
 namespace {
 constexpr int kT = 256;
-
-// ---------------------------------------------------------------- RK4 in z (src/sim_kernels.hpp:73-149)
-struct St { double x, y, zeta, px, py, pz; };
-
-__device__ __forceinline__ bool rhs(const PushArgs& c, const St& Y, double z, double qm, St& d) {
-    const double g = sqrt(1 + Y.px * Y.px + Y.py * Y.py + Y.pz * Y.pz);
-    if (Y.pz < 0.3 * g) return false;
-    const double gm = (1 + Y.px * Y.px + Y.py * Y.py) / (g + Y.pz);
-    double E[3], B[3], Ee[3], Be[3];
-    gather(c.fv, wrap_mod(Y.x, c.fv.Lx), c.fv.yf ? wrap_mod(Y.y, c.fv.Ly) : Y.y, Y.zeta, E, B);
-    c.laser.field(Y.x + c.X0, z, z - (Y.zeta - c.zc), Ee, Be);
-    for (int k = 0; k < 3; ++k) { E[k] += Ee[k]; B[k] += Be[k]; }
-    const double ip = 1 / Y.pz;
-    d.x = Y.px * ip; d.y = Y.py * ip; d.zeta = -gm * ip;
-    d.px = qm * (g * E[0] + Y.py * B[2] - Y.pz * B[1]) * ip;
-    d.py = qm * (g * E[1] + Y.pz * B[0] - Y.px * B[2]) * ip;
-    d.pz = qm * (g * E[2] + Y.px * B[1] - Y.py * B[0]) * ip;
-    return true;
-}
-__device__ __forceinline__ St axpy(const St& a, double h, const St& b) {
-    return St{a.x + h * b.x, a.y + h * b.y, a.zeta + h * b.zeta, a.px + h * b.px, a.py + h * b.py, a.pz + h * b.pz};
-}
-
-__global__ void __launch_bounds__(kT) k_bunch_push(PushArgs c, Parts P, long np, double z0) {
-    const long ip = long(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (ip >= np) return;
-    const int alive = P.alive[ip];
-    if (alive != 1 && alive != 3) return;
-    double z = alive == 3 ? c.s1 - P.rem[ip] : z0;
-    if (c.freeze && is_bunch(P.sp[ip])) { P.z[ip] = c.s1; return; }
-    St Y{P.x[ip], P.y[ip], P.zeta[ip], P.px[ip], P.py[ip], P.pz[ip]};
-    const double qm = P.qm[ip];
-    auto store = [&](const St& s) {
-        P.x[ip] = s.x; P.y[ip] = s.y; P.zeta[ip] = s.zeta; P.px[ip] = s.px; P.py[ip] = s.py; P.pz[ip] = s.pz;
-    };
-    while (z < c.s1 - 1e-12 * fabs(c.s1)) {
-        if (Y.zeta < c.zhand) { store(Y); P.alive[ip] = 2; P.rem[ip] = c.s1 - z; return; }
-        if (c.vz_min > 0 && Y.pz < c.vz_min * sqrt(1 + Y.px * Y.px + Y.py * Y.py + Y.pz * Y.pz)) {
-            store(Y); P.z[ip] = z; P.alive[ip] = 4;
-            return;
-        }
-        double ds = c.dz_push < c.s1 - z ? c.dz_push : c.s1 - z;
-        {
-            const double g = sqrt(1 + Y.px * Y.px + Y.py * Y.py + Y.pz * Y.pz);
-            const double lz = 0.5 * c.hmin * Y.pz * (g + Y.pz) / (1 + Y.px * Y.px + Y.py * Y.py);
-            if (lz < ds) ds = (c.s1 - z) / ceil((c.s1 - z) / lz);
-        }
-        if (c.laser.on) {
-            const double g = sqrt(1 + Y.px * Y.px + Y.py * Y.py + Y.pz * Y.pz);
-            const double hl = g - Y.px * c.laser.kx - Y.pz * c.laser.kz;
-            const double lim = c.push_phase * Y.pz / (c.laser.kL * (hl > 1e-12 ? hl : 1e-12));
-            if (lim < ds) ds = (c.s1 - z) / ceil((c.s1 - z) / lim);
-        }
-        St k1, k2, k3, k4;
-        const double hd = 0.5 * ds;
-        if (!(rhs(c, Y, z, qm, k1) && rhs(c, axpy(Y, hd, k1), z + hd, qm, k2) && rhs(c, axpy(Y, hd, k2), z + hd, qm, k3) &&
-              rhs(c, axpy(Y, ds, k3), z + ds, qm, k4))) {
-            P.alive[ip] = 0;
-            return;
-        }
-        const double w = ds / 6;
-        Y.x += w * (k1.x + 2 * k2.x + 2 * k3.x + k4.x);
-        Y.y += w * (k1.y + 2 * k2.y + 2 * k3.y + k4.y);
-        Y.zeta += w * (k1.zeta + 2 * k2.zeta + 2 * k3.zeta + k4.zeta);
-        Y.px += w * (k1.px + 2 * k2.px + 2 * k3.px + k4.px);
-        Y.py += w * (k1.py + 2 * k2.py + 2 * k3.py + k4.py);
-        Y.pz += w * (k1.pz + 2 * k2.pz + 2 * k3.pz + k4.pz);
-        z += ds;
-    }
-    store(Y);
-    P.z[ip] = c.s1;
-    int al = (Y.zeta < 0 || Y.zeta >= c.zmax || Y.y < c.ylo || Y.y > c.yhi) ? 0 : 1;
-    if (al == 1 && Y.zeta < c.zhand) { al = 2; P.rem[ip] = 0; }
-    P.alive[ip] = al;
-}
 
 // ---------------------------------------------------------------- streaming particles (src/sim_stream.hpp)
 // LDS windows of a tile, WX x WY nodes from node (x0, y0) (periodic); null for untiled particles:
@@ -173,6 +100,172 @@ __device__ __forceinline__ bool gather_window_fh(const FieldView& c, const Win& 
         if (comp < 3) E[comp] = v; else B[comp - 3] = v;
     }
     return true;
+}
+
+// ---------------------------------------------------------------- RK4 in z (src/sim_kernels.hpp:73-149)
+struct St { double x, y, zeta, px, py, pz; };
+
+// E, B at a beam particle: from the field window of its workgroup when the stencil and the slice pair lie in it (the
+// same values: gather_window sums as gather does), else from F
+__device__ __forceinline__ bool rhs(const PushArgs& c, const Win* win, const St& Y, double z, double qm, St& d) {
+    const double g = sqrt(1 + Y.px * Y.px + Y.py * Y.py + Y.pz * Y.pz);
+    if (Y.pz < 0.3 * g) return false;
+    const double gm = (1 + Y.px * Y.px + Y.py * Y.py) / (g + Y.pz);
+    double E[3], B[3], Ee[3], Be[3];
+    const double xw = wrap_mod(Y.x, c.fv.Lx), yw = c.fv.yf ? wrap_mod(Y.y, c.fv.Ly) : Y.y;
+    if (!(win && gather_window(c.fv, *win, xw, yw, Y.zeta, E, B))) gather(c.fv, xw, yw, Y.zeta, E, B);
+    c.laser.field(Y.x + c.X0, z, z - (Y.zeta - c.zc), Ee, Be);
+    for (int k = 0; k < 3; ++k) { E[k] += Ee[k]; B[k] += Be[k]; }
+    const double ip = 1 / Y.pz;
+    d.x = Y.px * ip; d.y = Y.py * ip; d.zeta = -gm * ip;
+    d.px = qm * (g * E[0] + Y.py * B[2] - Y.pz * B[1]) * ip;
+    d.py = qm * (g * E[1] + Y.pz * B[0] - Y.px * B[2]) * ip;
+    d.pz = qm * (g * E[2] + Y.px * B[1] - Y.py * B[0]) * ip;
+    return true;
+}
+__device__ __forceinline__ St axpy(const St& a, double h, const St& b) {
+    return St{a.x + h * b.x, a.y + h * b.y, a.zeta + h * b.zeta, a.px + h * b.px, a.py + h * b.py, a.pz + h * b.pz};
+}
+
+// beam particle ip from s0 (or from its hand-over point) to s1 (src/sim_kernels.hpp:98-149)
+__device__ __forceinline__ void push_beam_one(const PushArgs& c, const Win* win, Parts P, long ip, double z0) {
+    const int alive = P.alive[ip];
+    if (alive != 1 && alive != 3) return;
+    double z = alive == 3 ? c.s1 - P.rem[ip] : z0;
+    if (c.freeze && is_bunch(P.sp[ip])) { P.z[ip] = c.s1; return; }
+    St Y{P.x[ip], P.y[ip], P.zeta[ip], P.px[ip], P.py[ip], P.pz[ip]};
+    const double qm = P.qm[ip];
+    auto store = [&](const St& s) {
+        P.x[ip] = s.x; P.y[ip] = s.y; P.zeta[ip] = s.zeta; P.px[ip] = s.px; P.py[ip] = s.py; P.pz[ip] = s.pz;
+    };
+    while (z < c.s1 - 1e-12 * fabs(c.s1)) {
+        if (Y.zeta < c.zhand) { store(Y); P.alive[ip] = 2; P.rem[ip] = c.s1 - z; return; }
+        if (c.vz_min > 0 && Y.pz < c.vz_min * sqrt(1 + Y.px * Y.px + Y.py * Y.py + Y.pz * Y.pz)) {
+            store(Y); P.z[ip] = z; P.alive[ip] = 4;
+            return;
+        }
+        double ds = c.dz_push < c.s1 - z ? c.dz_push : c.s1 - z;
+        {
+            const double g = sqrt(1 + Y.px * Y.px + Y.py * Y.py + Y.pz * Y.pz);
+            const double lz = 0.5 * c.hmin * Y.pz * (g + Y.pz) / (1 + Y.px * Y.px + Y.py * Y.py);
+            if (lz < ds) ds = (c.s1 - z) / ceil((c.s1 - z) / lz);
+        }
+        if (c.laser.on) {
+            const double g = sqrt(1 + Y.px * Y.px + Y.py * Y.py + Y.pz * Y.pz);
+            const double hl = g - Y.px * c.laser.kx - Y.pz * c.laser.kz;
+            const double lim = c.push_phase * Y.pz / (c.laser.kL * (hl > 1e-12 ? hl : 1e-12));
+            if (lim < ds) ds = (c.s1 - z) / ceil((c.s1 - z) / lim);
+        }
+        St k1, k2, k3, k4;
+        const double hd = 0.5 * ds;
+        if (!(rhs(c, win, Y, z, qm, k1) && rhs(c, win, axpy(Y, hd, k1), z + hd, qm, k2) &&
+              rhs(c, win, axpy(Y, hd, k2), z + hd, qm, k3) && rhs(c, win, axpy(Y, ds, k3), z + ds, qm, k4))) {
+            P.alive[ip] = 0;
+            return;
+        }
+        const double w = ds / 6;
+        Y.x += w * (k1.x + 2 * k2.x + 2 * k3.x + k4.x);
+        Y.y += w * (k1.y + 2 * k2.y + 2 * k3.y + k4.y);
+        Y.zeta += w * (k1.zeta + 2 * k2.zeta + 2 * k3.zeta + k4.zeta);
+        Y.px += w * (k1.px + 2 * k2.px + 2 * k3.px + k4.px);
+        Y.py += w * (k1.py + 2 * k2.py + 2 * k3.py + k4.py);
+        Y.pz += w * (k1.pz + 2 * k2.pz + 2 * k3.pz + k4.pz);
+        z += ds;
+    }
+    store(Y);
+    P.z[ip] = c.s1;
+    int al = (Y.zeta < 0 || Y.zeta >= c.zmax || Y.y < c.ylo || Y.y > c.yhi) ? 0 : 1;
+    if (al == 1 && Y.zeta < c.zhand) { al = 2; P.rem[ip] = 0; }
+    P.alive[ip] = al;
+}
+
+__global__ void __launch_bounds__(kT) k_bunch_push(PushArgs c, Parts P, long np, double z0) {
+    const long ip = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (ip >= np) return;
+    push_beam_one(c, nullptr, P, ip, z0);
+}
+
+// ---- the beam push with field windows (Fourier y): the particles binned by (slice pair, transverse tile of kBT x kBT
+// cells) through an index list (the list itself keeps its order), a workgroup per chunk of at most kBChunk
+// particles of a bin with the bin's window of both slices in LDS (a chain of four dependent gathers per RK4 sub-step
+// then waits on LDS instead of memory)
+constexpr int kBT = 8, kBM = 1, kBW = kBT + 3 + 2 * kBM, kBChunk = 1024;
+constexpr unsigned long long kBNone = ~0ull;
+
+// the bin of every particle that moves (alive 1 or 3, not frozen); the bin nbin (no window) for one outside the
+// slices of F; ~0 for the others (a frozen bunch particle only takes the new z here, as the plain push does)
+__global__ void __launch_bounds__(kT) k_bpush_keys(PushArgs c, Parts P, long np, int ntx, int nty, long nbin,
+                                                   unsigned long long* key, unsigned* idx) {
+    const long ip = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (ip >= np) return;
+    idx[ip] = unsigned(ip);
+    key[ip] = kBNone;
+    const int alive = P.alive[ip];
+    if (alive != 1 && alive != 3) return;
+    if (c.freeze && is_bunch(P.sp[ip])) { P.z[ip] = c.s1; return; }
+    const FieldView& F = c.fv;
+    int n; double fz;
+    F.zax.locate(P.zeta[ip], n, fz);
+    if (n == F.a - 1 && fz > 1 - 1e-9) { n = F.a; fz = 0; }
+    const int fl = n - F.a;
+    if (fl < 0 || fl >= F.ns + 1) { key[ip] = (unsigned long long)nbin; return; }
+    int ix = int(floor(wrap_mod(P.x[ip], F.Lx) / F.dx)), iy = int(floor(wrap_mod(P.y[ip], F.Ly) / F.dy));
+    ix = ix < 0 ? 0 : (ix >= F.nx ? F.nx - 1 : ix);
+    iy = iy < 0 ? 0 : (iy >= F.ny ? F.ny - 1 : iy);
+    key[ip] = (unsigned long long)((long(fl) * nty + iy / kBT) * ntx + ix / kBT);
+}
+// the chunks of the bins: the head of each run of equal keys lists its run in chunks of at most kBChunk
+__global__ void __launch_bounds__(kT) k_bpush_chunks(const unsigned long long* key, long n, long* chunk, int* nchunk,
+                                                     int cap) {
+    const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned long long k = key[i];
+    if (k == kBNone || (i > 0 && key[i - 1] == k)) return;
+    long lo = i + 1, hi = n;
+    while (lo < hi) {
+        const long mid = lo + ((hi - lo) >> 1);
+        if (key[mid] == k) lo = mid + 1; else hi = mid;
+    }
+    const int m = int((lo - i + kBChunk - 1) / kBChunk);
+    const int r = atomicAdd(nchunk, m);
+    for (int q = 0; q < m && r + q < cap; ++q) {
+        chunk[2 * (r + q)] = i + long(q) * kBChunk;
+        chunk[2 * (r + q) + 1] = min(lo, i + long(q + 1) * kBChunk);
+    }
+}
+// persistent: workgroup g takes the chunks g, g + grid, ...; every workgroup leaves when the list is done
+__global__ void __launch_bounds__(kT) k_bunch_push_win(PushArgs c, Parts P, const unsigned long long* key,
+                                                       const unsigned* idx, const long* chunk, const int* nchunk,
+                                                       int cap, int ntx, int nty, long nbin, double z0) {
+    __shared__ float win[12 * kBW * kBW];
+    const FieldView& F = c.fv;
+    const int nc = min(*nchunk, cap);
+    for (int q = blockIdx.x; q < nc; q += gridDim.x) {
+        const long lo = chunk[2 * q], hi = chunk[2 * q + 1];
+        const unsigned long long k = key[lo];
+        if (k == (unsigned long long)nbin) {         // outside the slices of F: no window
+            for (long t = lo + threadIdx.x; t < hi; t += blockDim.x) push_beam_one(c, nullptr, P, long(idx[t]), z0);
+            continue;
+        }
+        const int tx = int(k % (unsigned long long)ntx), ty = int((k / ntx) % (unsigned long long)nty);
+        const int fl = int(k / ((unsigned long long)ntx * nty));
+        const int x0 = tx * kBT - 1 - kBM, y0 = ty * kBT - 1 - kBM;
+        constexpr int pl = kBW * kBW;
+        const long cs = long(F.ns + 2) * F.nk;
+        __syncthreads();                             // the previous chunk's particles are done with the window
+        for (int r = threadIdx.x; r < pl; r += blockDim.x) {
+            const int v = r / kBW, u = r - v * kBW;
+            const float* src = F.F + long(fl) * F.nk + long(wrap_index(y0 + v, F.ny)) * F.nx + wrap_index(x0 + u, F.nx);
+#pragma unroll
+            for (int comp = 0; comp < 6; ++comp) {
+                win[comp * pl + r] = src[comp * cs];
+                win[(6 + comp) * pl + r] = src[comp * cs + F.nk];
+            }
+        }
+        __syncthreads();
+        const Win w{nullptr, win, x0, y0, kBW, kBW, fl};
+        for (long t = lo + threadIdx.x; t < hi; t += blockDim.x) push_beam_one(c, &w, P, long(idx[t]), z0);
+    }
 }
 
 // TSC deposit of f (dt/Delta) v/(h dx dy) (src/sim_stream.hpp:33-64) into plane comp of the half-cell current:
@@ -678,6 +771,41 @@ void bunch_push(const PushArgs& a, Parts p, long np, double s0) {
     if (np <= 0) return;
     hipLaunchKernelGGL(k_bunch_push, dim3(blocks_for(np, kT)), dim3(kT), 0, stream(), a, p, np, s0);
     launch_check("bunch push");
+}
+void bunch_push_windowed(const PushArgs& a, Parts p, long np, double s0, BpushScratch& s) {
+    if (np <= 0) return;
+    const FieldView& F = a.fv;
+    if (!F.yf || !F.self || F.nx < 2 * kBT || F.ny < 2 * kBT || np > long(0xffffffffu)) {
+        bunch_push(a, p, np, s0);
+        return;
+    }
+    const int ntx = (F.nx + kBT - 1) / kBT, nty = (F.ny + kBT - 1) / kBT;
+    const long nbin = long(F.ns + 1) * nty * ntx;
+    int eb = 1;                                      // bits of the bins and the bin nbin, one more for ~0
+    while ((1ll << eb) <= nbin) ++eb;
+    ++eb;
+    if (s.key0.size() < size_t(np)) {
+        const size_t cap = size_t(np) + size_t(np) / 4;
+        s.key0.resize(cap); s.key1.resize(cap); s.idx0.resize(cap); s.idx1.resize(cap);
+    }
+    const int cap = int(np / 1 + 1);                 // chunks: at most one per particle
+    if (s.chunk.size() < size_t(2 * cap)) s.chunk.resize(size_t(2 * cap));
+    if (s.nchunk.size() < 1) s.nchunk.resize(1);
+    hipLaunchKernelGGL(k_bpush_keys, dim3(blocks_for(np, kT)), dim3(kT), 0, stream(), a, p, np, ntx, nty, nbin,
+                       s.key0.data(), s.idx0.data());
+    size_t bytes = 0;
+    hip_check(rocprim::radix_sort_pairs(nullptr, bytes, s.key0.data(), s.key1.data(), s.idx0.data(), s.idx1.data(),
+                                        np, 0, unsigned(eb), stream()), "beam push sort size");
+    if (s.tmp.size() < bytes) s.tmp.resize(bytes + bytes / 4 + 256);
+    bytes = s.tmp.size();
+    hip_check(rocprim::radix_sort_pairs(s.tmp.data(), bytes, s.key0.data(), s.key1.data(), s.idx0.data(),
+                                        s.idx1.data(), np, 0, unsigned(eb), stream()), "beam push sort");
+    hip_check(hipMemsetAsync(s.nchunk.data(), 0, sizeof(int), stream()), "hipMemsetAsync");
+    hipLaunchKernelGGL(k_bpush_chunks, dim3(blocks_for(np, kT)), dim3(kT), 0, stream(), s.key1.data(), np,
+                       s.chunk.data(), s.nchunk.data(), cap);
+    hipLaunchKernelGGL(k_bunch_push_win, dim3(2048), dim3(kT), 0, stream(), a, p, s.key1.data(), s.idx1.data(),
+                       s.chunk.data(), s.nchunk.data(), cap, ntx, nty, nbin, s0);
+    launch_check("bunch push (windows)");
 }
 void stream_halfcell(const StreamArgs& a, Parts src, Parts dst, long nstrm, const TileGeom& tg,
                      const LaunchTimer& lt) {

@@ -193,3 +193,38 @@ def test_window_shift_follows_bunch():
     pa, pb = gpu.array("parts"), ref.array("parts")
     np.testing.assert_array_equal(pa["alive"], pb["alive"])
     particle_parity(pa, pb, mk())
+
+
+def test_windowed_beam_push_equals_plain():
+    """the beam push with its particles binned by slice pair and tile and the fields of each bin staged in LDS
+    equals the one-lane-per-particle push that gathers from F, bit for bit: the config-0 style bunch in the plane
+    wave over 3 steps (particles and fields; the vacuum step is deterministic), and the 2^20-electron witness of the
+    laser-wakefield deck at 512 slices pushed twice through the same developed fields (the plasma pass sums with
+    FP64 atomics, so two runs of it differ in the last bits; here both pushes start from one state)"""
+    from decks import config2_deck
+    a, b = g3d_amd.Sim(bunch_deck(nmacro=16384)), g3d_amd.Sim(bunch_deck(nmacro=16384))
+    b.call("bpush_windows", 0)
+    a.init(); b.init()
+    for _ in range(3):
+        a.step(); b.step()
+    pa, pb = a.array("parts"), b.array("parts")
+    assert len(pa) > 4096
+    for f in pa.dtype.names:
+        np.testing.assert_array_equal(pa[f], pb[f], err_msg=f)
+    np.testing.assert_array_equal(a.array("F"), b.array("F"))
+    a.close(); b.close()
+    a, b = g3d_amd.Sim(config2_deck(0, nz=512)), g3d_amd.Sim(config2_deck(0, nz=512))
+    b.call("bpush_windows", 0)
+    a.init(); b.init()
+    for _ in range(2):
+        a.step()
+    b.set_particles(a.array("parts"))
+    b.set("F", a.array("F"))
+    s, dz = a.info()["s"], a._deck.c.dz
+    for k in range(2):
+        a.call("push", s + k * dz, s + (k + 1) * dz)
+        b.call("push", s + k * dz, s + (k + 1) * dz)
+    pa, pb = a.array("parts"), b.array("parts")
+    assert (pa["alive"] == 1).sum() > 1_000_000
+    for f in pa.dtype.names:
+        np.testing.assert_array_equal(pa[f], pb[f], err_msg=f)
