@@ -494,3 +494,181 @@ def test_ions_neutral_pairs_at_rest():
     em, dm = float(np.abs(re).max()), float(np.abs(re + ri).max())
     assert fm == 0 and pm == 0 and ni > 0, (fm, pm, ni)
     assert em > 0 and dm <= 1e-12 * em, dm / em
+
+
+def beam_wake_axis(freehand):
+    """tests/test_plasma.cpp:66-90: a frozen narrow bunch (sigma 5/k0, n_b = 0.01 n0, gamma 1000) 30 steps in a
+    uniform plasma n0 = 0.01 n_c, 64 x 64 modes over 120/k0 (or the free-hand y table of 1.875/k0 steps), 301
+    slices of 0.5/k0; E_z on the axis"""
+    sig, sz, Lw, h = 5.0, 5.0, 150.0, 0.5
+    d = g3d_amd.Deck(LAMBDA_TEST)
+    d.c.nx = 64; d.c.Lx = 120.0
+    if freehand:
+        d.y_table([0.0], [1.875], 60.0)
+    else:
+        d.y_uniform(64, 120.0)
+    d.zeta_uniform(int(Lw / h) + 1, Lw)
+    n0 = 0.01
+    Ne = 0.01 * n0 * (2 * math.pi) ** 1.5 * sig * sig * sz
+    d.set(sponge=0.0, sponge_y=0.0, move_x=0, move_y=0, k_filter=0, plasma_on=1, n0=n0, ppc_x=1, ppc_y=1,
+          gamma=1000.0, sigx=sig, sigy=sig, sigz=sz, nmacro=200000, zeta_center=120.0,
+          charge=Ne * d.nc * d.Lu ** 3 * 1.602176634e-19, dz=10.0, dz_push=10.0, freeze_bunch=1)
+    sim = g3d_amd.Sim(d)
+    sim.init()
+    for _ in range(30):
+        sim.step()
+    info = sim.info()
+    ns, nk, nx, ny = info["ns"], info["nk"], d.c.nx, d.c.ny
+    F = np.asarray(sim.array("F"), dtype=np.float64).reshape(6, ns + 2, nk)
+    ez = F[2, :d.c.nzeta, (ny // 2) * nx + nx // 2]
+    return d, ez, info["dx"], info["dy"], Ne
+
+
+def test_beam_wake_3d_linear_theory():
+    """the 3D linear wake of the frozen narrow bunch on the axis against the linear theory of the scheme (per mode
+    the plasma frequency k_p S(k) with the TSC form factor S = sinc^3), rms over 20 < zeta < 100 within 3 %, and
+    the free-hand y grid against the Fourier one within 3 % (tests/test_plasma.cpp:91-135)"""
+    d, ez, dx, dy, Ne = beam_wake_axis(False)
+    ef = beam_wake_axis(True)[1]
+    c = d.c
+    kp, sig, sz, Lx, Ly = math.sqrt(c.n0), c.sigx, c.sigz, c.Lx, c.Ly
+    lam0 = Ne / (math.sqrt(2 * math.pi) * sz)
+    zs = np.arange(0.0, float(d.zeta[-1]), 0.05)
+    g = lam0 * np.exp(-(zs - c.zeta_center) ** 2 / (2 * sz * sz)) * 0.05
+    m = np.arange(-c.nx // 2 + 1, c.nx // 2)
+    l = np.arange(-c.ny // 2 + 1, c.ny // 2)
+    kx = (2 * math.pi * m / Lx)[:, None]
+    ky = (2 * math.pi * l / Ly)[None, :]
+    k2 = (kx * kx + ky * ky).ravel()
+    S = (np.sinc(0.5 * kx * dx / math.pi) * np.sinc(0.5 * ky * dy / math.pi)).ravel() ** 3
+    z = np.asarray(d.zeta)
+    out = {}
+    for cont in (0, 1):
+        Sk = np.ones_like(S) if cont else S
+        K = kp * Sk
+        A = np.exp(-0.5 * k2 * sig * sig) * Sk * K * K / (k2 + K * K) / (Lx * Ly)
+        Cc = np.cos(np.outer(K, zs)) @ g
+        Ss = np.sin(np.outer(K, zs)) @ g
+        out[cont] = (A[:, None] * (np.cos(np.outer(K, z)) * Cc[:, None] + np.sin(np.outer(K, z)) * Ss[:, None])).sum(0)
+    E = out[0]
+    sel = (z <= 100) & (z >= 20)
+    dev = math.sqrt(((ez[sel] - E[sel]) ** 2).sum() / (E[sel] ** 2).sum())
+    dfh = math.sqrt(((ef[sel] - ez[sel]) ** 2).sum() / (ez[sel] ** 2).sum())
+    dc = math.sqrt(((ez[sel] - out[1][sel]) ** 2).sum() / (E[sel] ** 2).sum())
+    print("3D beam wake: rms against the scheme's theory %.4f, against the continuum %.4f; free-hand y against "
+          "Fourier y %.4f" % (dev, dc, dfh))
+    assert dev < 0.03, dev
+    assert dfh < 0.03, dfh
+
+
+def two_bunch_fields(bunches):
+    """tests/test_bunches.cpp:9-27: 32 x 32 modes over 60/k0, 121 slices over 120/k0, frozen bunches, init and two
+    steps of 20/k0; the real fields F"""
+    d = base(32, 32, 121, 60.0, 60.0, 120.0)
+    d.set(dz=20.0, dz_push=20.0, freeze_bunch=1, move_x=0, move_y=0)
+    first = bunches[0]
+    d.set(charge=1e-12, gamma=first[1], sigx=4.0, sigy=4.0, sigz=6.0, zeta_center=first[0], nmacro=20000,
+          seed=first[2])
+    for k, (zc, gamma, seed) in enumerate(bunches[1:]):
+        d.add_bunch("b%d" % (k + 2), 1e-12, gamma, 4.0, 4.0, 6.0, zc, 20000, seed)
+    sim = g3d_amd.Sim(d)
+    sim.init()
+    for _ in range(2):
+        sim.step()
+    return np.asarray(sim.array("F"), dtype=np.float64)
+
+
+def test_two_bunch_superposition():
+    """the fields of two frozen electron bunches (other zeta and gamma, each from its own seed) together equal the
+    sum of their fields alone, to the float output (tests/test_bunches.cpp:35-42; the reference's second bunch is a
+    positron bunch offset in x, which this engine does not load)"""
+    e, b = (60.0, 200.0, 7), (35.0, 50.0, 3)
+    fe, fb, fs = two_bunch_fields([e]), two_bunch_fields([b]), two_bunch_fields([e, b])
+    ms = np.abs(fs).max()
+    ds = np.abs(fs - fe - fb).max()
+    print("two bunches: max |F(e+b) - F(e) - F(b)| / max |F(e+b)| = %.3e" % (ds / ms))
+    assert ms > 0 and ds <= 1e-6 * ms
+
+
+def ion_collapse_run(mass, nx=320, Lx=80.0, hz=1.0, steps=3, dz=100.0, n0=0.002):
+    """tests/test_ions.cpp:68-132: a rigid Gaussian bunch (n_b = 100 n0, sigma_r 2.5, sigma_z 80/k0, gamma 1e5) in
+    a plasma with mobile ions of the given mass, 320 x 320 modes over 80/k0, 641 slices of 1/k0, 3 steps of
+    100/k0; the on-axis ion density Z n_i / n0 at the half-cell centres, and the radial test-ion model (ion shells
+    in E_r = (Q_i(r) - Q_b(r)) / r with full blowout, the bunch width s^2 = sigma^2 + dx^2/2 of the TSC deposit and
+    gather, the on-axis density as the TSC average of the shells at the axis node) on the same points"""
+    sig, sz, nb, Lw, zc = 2.5, 80.0, 0.2, 640.0, 400.0
+    d = g3d_amd.Deck(LAMBDA_TEST)
+    d.c.nx = nx; d.c.Lx = Lx
+    d.y_uniform(nx, Lx)
+    d.zeta_uniform(int(Lw / hz) + 1, Lw)
+    Ne = nb * (2 * math.pi) ** 1.5 * sig * sig * sz
+    d.set(sponge=0.0, sponge_y=0.0, move_x=0, move_y=0, k_filter=0, plasma_on=1, n0=n0, ppc_x=1, ppc_y=1,
+          gamma=1e5, sigx=sig, sigy=sig, sigz=sz, nmacro=400000, zeta_center=zc,
+          charge=Ne * d.nc * d.Lu ** 3 * 1.602176634e-19, dz=dz, dz_push=dz, freeze_bunch=1)
+    d.add_ion(mass, 1, 1, atoms=1.0, mobile=True)
+    sim = g3d_amd.Sim(d)
+    sim.init()
+    for _ in range(steps):
+        sim.step()
+    N, L, i0 = d.c.nzeta, nx + nx, nx // 2
+    rl = np.asarray(sim.array("rho_lines_i"), dtype=np.float64)
+    zeta = 0.5 * (d.zeta[:-1] + d.zeta[1:])
+    ni = np.array([rl[m * L + i0] / n0 for m in range(N - 1)])
+    dx = sim.info()["dx"]
+    s2 = sig * sig + 0.5 * dx * dx
+    R, dr0 = 6 * sig, 0.01
+    K = int(R / dr0)
+    rr = (np.arange(K) + 0.5) * dr0
+    vv = np.zeros(K)
+    q = n0 * rr * dr0
+    th = 2 * math.pi * (np.arange(64) + 0.5) / 64
+
+    def tsc1(x):
+        a = np.abs(x) / dx
+        return np.where(a < 0.5, 0.75 - a * a, np.where(a < 1.5, 0.5 * (1.5 - a) ** 2, 0.0))
+
+    def axis_density():
+        a = np.abs(rr)
+        near = a <= 1.5 * math.sqrt(2.0) * dx
+        w = (tsc1(np.outer(a[near], np.cos(th))) * tsc1(np.outer(a[near], np.sin(th)))).sum(1)
+        return (2 * math.pi * q[near] * w / 64 / (dx * dx)).sum() / n0
+
+    dt, z = 0.05, float(d.zeta[-1])
+    model = np.zeros(len(zeta))
+    out = len(zeta)
+    while out > 0:
+        order = np.argsort(np.abs(rr), kind="stable")
+        a = np.abs(rr[order])
+        Qi = np.cumsum(q[order]) - 0.5 * q[order]            # the charge inside each shell (half its own)
+        nbz = nb * math.exp(-0.5 * (z - zc) ** 2 / (sz * sz))
+        e = (Qi - nbz * s2 * (1 - np.exp(-a * a / (2 * s2)))) / np.maximum(a, 1e-9)
+        E = np.empty(K)
+        E[order] = np.where(rr[order] < 0, -e, e)
+        while out > 0 and zeta[out - 1] >= z - 0.5 * dt:
+            out -= 1
+            model[out] = axis_density()
+        vv += E / mass * dt
+        rr += vv * dt
+        z -= dt
+    return zeta, ni, model
+
+
+@pytest.mark.parametrize("ion", ["H", "D"])
+def test_ion_collapse_test_ion_model(ion):
+    """the on-axis ion density of the ion collapse in a dense bunch against the test-ion model within 3 % over the
+    compression phase (model 3 ... 30 n0 from the front of the window), at least 10 points
+    (tests/test_ions.cpp:56-150; Rosenzweig et al., PRL 95, 195002 (2005))"""
+    mass = (1.00784 if ion == "H" else 2.01410) * 1822.888486
+    zeta, ni, model = ion_collapse_run(mass)
+    worst, cnt, compressed = 0.0, 0, False
+    for k in range(len(zeta) - 1, -1, -1):
+        if model[k] >= 30:
+            compressed = True
+        if compressed:
+            continue
+        if model[k] > 3:
+            worst = max(worst, abs(ni[k] / model[k] - 1))
+            cnt += 1
+    print("ion collapse (%s): worst deviation from the test-ion model %.4f over %d points, peak Z n_i/n0 %.1f"
+          % (ion, worst, cnt, ni.max()))
+    assert cnt > 10 and worst < 0.03, (cnt, worst)
