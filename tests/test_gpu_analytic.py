@@ -2,9 +2,12 @@
 decks, references and tolerances: vacuum dispersion of the reduced system (tests/test_fields.cpp:103-148), the 1D
 linear wake against cold-fluid theory (tests/test_plasma.cpp:7-53), the smoothing commuting with the divergence on
 the free-hand y grid (tests/test_plasma.cpp:136-176), Gauss's law for a long bunch and a bunch in equilibrium in
-its own field (tests/test_bunch.cpp:36-110), and field ionization against the rate equations
-(tests/test_ionization.cpp:30-90).  The checks the engine cannot run are left out: grids whose mode numbers are not
-powers of two (its transforms take powers of two)."""
+its own field (tests/test_bunch.cpp:36-110), field ionization against the rate equations
+(tests/test_ionization.cpp:30-90), the plane-wave invariants, the sponge, a constant step table, streaming and
+trapping kinematics, the immobile and neutral-pair limits of the ions (tests/test_ions.cpp), the 3D wake of a narrow
+bunch against the scheme's own linear theory on both y grids, the superposition of two bunches
+(tests/test_bunches.cpp) and the ion collapse in a dense bunch against the test-ion model.  The checks the engine
+cannot run are left out: grids whose mode numbers are not powers of two (its transforms take powers of two)."""
 import math
 
 import numpy as np
@@ -590,10 +593,10 @@ def test_two_bunch_superposition():
     assert ms > 0 and ds <= 1e-6 * ms
 
 
-def ion_collapse_run(mass, nx=320, Lx=80.0, hz=1.0, steps=3, dz=100.0, n0=0.002):
+def ion_collapse_run(mass, nx=512, Lx=128.0, hz=1.0, steps=3, dz=100.0, n0=0.002):
     """tests/test_ions.cpp:68-132: a rigid Gaussian bunch (n_b = 100 n0, sigma_r 2.5, sigma_z 80/k0, gamma 1e5) in
-    a plasma with mobile ions of the given mass, 320 x 320 modes over 80/k0, 641 slices of 1/k0, 3 steps of
-    100/k0; the on-axis ion density Z n_i / n0 at the half-cell centres, and the radial test-ion model (ion shells
+    a plasma with mobile ions of the given mass, 641 slices of 1/k0, 3 steps of 100/k0, the reference's cell
+    dx = sigma_r / 10 on 512 x 512 modes over 128/k0 (its 320 x 320 over 80/k0 is not a power of two); the on-axis ion density Z n_i / n0 at the half-cell centres, and the radial test-ion model (ion shells
     in E_r = (Q_i(r) - Q_b(r)) / r with full blowout, the bunch width s^2 = sigma^2 + dx^2/2 of the TSC deposit and
     gather, the on-axis density as the TSC average of the shells at the axis node) on the same points"""
     sig, sz, nb, Lw, zc = 2.5, 80.0, 0.2, 640.0, 400.0
