@@ -17,7 +17,8 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "lib", "libg3d.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 SOURCES = ["g3d_fft.hip", "g3d_march.hip", "g3d_particles.hip", "g3d_fields.hip", "g3d_plasma.hip",
-           "g3d_freehand.hip", "g3d_deposit.hip", "g3d_mixed.hip", "g3d_adjoint.hip", "g3d_spush.hip", "g3d_bpush.hip", "g3d_kabi.hip", "g3d_static.hip", "g3d_engine.cpp", "g3d_output.cpp", "g3d_abi.cpp"]
+           "g3d_freehand.hip", "g3d_deposit.hip", "g3d_mixed.hip", "g3d_adjoint.hip", "g3d_spush.hip", "g3d_bpush.hip", "g3d_kabi.hip", "g3d_static.hip", "g3d_grad.hip",
+           "g3d_engine.cpp", "g3d_output.cpp", "g3d_grad.cpp", "g3d_abi.cpp"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-result"]
 

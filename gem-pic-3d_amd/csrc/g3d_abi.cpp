@@ -251,6 +251,13 @@ int g3d_sim_write_checkpoint(g3d_sim* sim, const char* dir) {
 int g3d_sim_read_checkpoint(g3d_sim* sim, const char* dir, long step) {
     return g3d::guard([&] { if (!dir) throw std::invalid_argument("missing dir"); S(sim).read_checkpoint(dir, step); });
 }
+int g3d_sim_gradient(g3d_sim* sim, int steps, int every, double* loss, double* genv, int nenv, double* gprof,
+                     int nprof) {
+    return g3d::guard([&] {
+        if (!loss || (nenv > 0 && !genv) || (nprof > 0 && !gprof)) throw std::invalid_argument("gradient: missing output");
+        S(sim).gradient(steps, every, loss, genv, nenv, gprof, nprof);
+    });
+}
 int g3d_sim_timing(g3d_sim* sim, int enable) {
     return g3d::guard([&] {
         g3d::Sim& s = S(sim);

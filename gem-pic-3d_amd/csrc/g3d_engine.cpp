@@ -637,9 +637,39 @@ void Sim::load_bunch_profile(std::vector<g3d_particle>& mine, double ylo, double
         const double f = cz[kk + 1] > cz[kk] ? (t - cz[kk]) / (cz[kk + 1] - cz[kk]) : 0.5;
         return zmin + (zmax - zmin) * (kk + f) / K;
     };
+    // for the gradient in the profile values (record_dprof_): d cz[k] / d prof[m] of the trapezoid sums, and per
+    // loaded particle d zeta_f / d prof[m] at its draw (the trapezoid k it falls in held fixed)
+    std::vector<double> dcz;
+    if (record_dprof_) {
+        dcz.assign(size_t(K + 1) * np, 0.0);
+        auto hat = [&](int m, double z) { return std::max(0.0, 1 - std::fabs(z - (zmin + m * hk)) / hk); };
+        for (int k = 1; k <= K; ++k) {
+            const double z0 = zmin + (zmax - zmin) * (k - 1) / K, z1 = zmin + (zmax - zmin) * k / K;
+            const bool p0 = prof(z0) > 0, p1 = prof(z1) > 0;
+            for (int m = 0; m < np; ++m)
+                dcz[size_t(k) * np + m] = dcz[size_t(k - 1) * np + m] + 0.5 * ((p0 ? hat(m, z0) : 0) + (p1 ? hat(m, z1) : 0));
+        }
+    }
+    auto dzeta_dprof = [&](double u01, double* out) {
+        const double t = u01 * cz[K];
+        const int k = int(std::upper_bound(cz.begin(), cz.end(), t) - cz.begin()) - 1;
+        const int kk = std::min(std::max(k, 0), K - 1);
+        const double dl = cz[kk + 1] - cz[kk];
+        for (int m = 0; m < np; ++m) {
+            double df = 0;
+            if (dl > 0) {
+                const double dt = u01 * dcz[size_t(K) * np + m], d0 = dcz[size_t(kk) * np + m], d1 = dcz[size_t(kk + 1) * np + m];
+                df = (dt - d0) / dl - (t - cz[kk]) * (d1 - d0) / (dl * dl);
+            }
+            out[m] = (zmax - zmin) * df / K;
+        }
+    };
     Axis zh = zax; zh.X = cfg.zeta.data();
+    std::vector<double> dz(static_cast<size_t>(np));
     for (long ib = 0; ib < cfg.nmacro; ++ib) {
-        const double zf = sample_zeta(U(rng) + 0.5);
+        const double u01 = U(rng) + 0.5;
+        const double zf = sample_zeta(u01);
+        if (record_dprof_) dzeta_dprof(u01, dz.data());
         const double xr = cfg.sigx * G(rng), yr = cfg.sigy * G(rng);
         const double x = 0 + xr, y = 0 + yr;
         const double px = cfg.emitx > 0 && cfg.sigx > 0 ? cfg.emitx / cfg.sigx * G(rng) : 0;
@@ -660,6 +690,9 @@ void Sim::load_bunch_profile(std::vector<g3d_particle>& mine, double ylo, double
         zh.locate(p.zeta, hc, f);
         if (p.zeta < 0 || hc > N - 2 || p.y < ylo || p.y > yhi) { ++lost_total; continue; }
         mine.push_back(p);
+        // zeta = zc + (zeta_f - zc) (pz / sqrt(1 + ptot^2)) / vz
+        if (record_dprof_)
+            for (int m = 0; m < np; ++m) dprof_.push_back(dz[size_t(m)] * (pz / std::sqrt(1 + ptot * ptot)) / vz);
     }
 }
 
@@ -686,6 +719,7 @@ void Sim::step() {
     window_shift_start();
     push(s, s + cfg.dz);
     if (cfg.plasma_on) collect_restreaming();
+    if (tape_) tape_beam(*tape_, false);
     s += cfg.dz; ++istep;
     if (cfg.self_fields && cfg.plasma_on) {
         currents();
@@ -1229,8 +1263,38 @@ void Sim::march_plasma(double s0, double s1) {
     bool have_dl = false;
     long born_base = nstrm;
     long nlist = nstrm;          // particles in the list (freed electrons are appended on the device)
+    // the record for the reverse of the step (Sim::gradient): the streaming particles before every half-cell, the
+    // real fields of every slice after each pass but the last, and after the march
+    auto tape_copy = [&](void* dst, const void* src, size_t bytes) {
+        hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream()), "hipMemcpy D2D");
+    };
+    if (tape_) {
+        StepTape& t = *tape_;
+        const size_t np7 = size_t(N - 1) * 7 * size_t(nlist);
+        t.nstrm = nlist;
+        if (t.ps.size() != np7) {
+            t.ps.resize(np7); t.pa.resize(size_t(N - 1) * size_t(nlist)); t.pq.resize(3 * size_t(nlist));
+            t.ids.resize(size_t(nlist));
+        }
+        tape_copy(t.ids.data(), strmA.id.data(), size_t(nlist) * sizeof(unsigned long long));
+        t.added.clear();
+        if (t.Fmid.size() != size_t(std::max(passes - 1, 0)) * F.size()) t.Fmid.resize(size_t(std::max(passes - 1, 0)) * F.size());
+        if (t.Fmarch.size() != F.size()) t.Fmarch.resize(F.size());
+        const size_t b = size_t(nlist) * sizeof(double);
+        tape_copy(t.pq.data(), strmA.q.data(), b);
+        tape_copy(t.pq.data() + nlist, strmA.qm.data(), b);
+        tape_copy(t.pq.data() + 2 * nlist, strmA.w.data(), b);
+    }
     for (int n = N - 2; n >= a; --n) {
         const double wex = (n >= 1 && n + 1 <= N - 2 && have_dl) ? (cen[n] - cen[n - 1]) / (cen[n + 1] - cen[n]) : 0.0;
+        if (tape_) {
+            StepTape& t = *tape_;
+            const size_t b = size_t(nlist) * sizeof(double);
+            double* dst = t.ps.data() + size_t(n) * 7 * size_t(nlist);
+            const DevBuf<double>* src[7] = {&strmA.x, &strmA.y, &strmA.zeta, &strmA.z, &strmA.px, &strmA.py, &strmA.pz};
+            for (int c = 0; c < 7; ++c) tape_copy(dst + size_t(c) * nlist, src[c]->data(), b);
+            tape_copy(t.pa.data() + size_t(n) * nlist, strmA.alive.data(), size_t(nlist) * sizeof(int));
+        }
         if (tg.ntile && rebin > 0 && n < N - 2 && (N - 2 - n) % rebin == 0) {
             if (bin_idx.size() < strmA.cap) bin_idx.resize(strmA.cap);
             if (bin_cnt.size() < size_t(tg.ntile)) { bin_cnt.resize(size_t(tg.ntile)); bin_cur.resize(size_t(tg.ntile)); }
@@ -1294,6 +1358,10 @@ void Sim::march_plasma(double s0, double s1) {
             pass_work(sp3.data(), 0);
             toc(F_MODES);
             real_fields(n);
+            if (tape_ && !last)
+                for (int c = 0; c < 6; ++c)
+                    tape_copy(tape_->Fmid.data() + size_t(pass) * F.size() + size_t(c) * fpl + size_t(n - a) * nk,
+                              F.data() + size_t(c) * fpl + size_t(n - a) * nk, size_t(nk) * sizeof(float));
         }
         // the passes wrote the new state into strmB: it becomes the list
         std::swap(strmA, strmB);
@@ -1311,6 +1379,7 @@ void Sim::march_plasma(double s0, double s1) {
         have_dl = true;
     }
     (void)born_base;
+    if (tape_) tape_copy(tape_->Fmarch.data(), F.data(), F.size() * sizeof(float));
     nstrm = nlist;
     {
         std::vector<unsigned long long> cnt(npush.size());
@@ -1363,6 +1432,7 @@ void Sim::march_plasma(double s0, double s1) {
         } else { p.alive = 6; joining_.push_back(p); }
     }
     if (!add.empty()) {
+        if (tape_) for (const auto& p : add) tape_->added.push_back(p.id);
         std::vector<g3d_particle> all = host_particles(), keep;
         keep.reserve(all.size() + add.size());
         for (auto& p : all) if (p.alive == 1) keep.push_back(p);

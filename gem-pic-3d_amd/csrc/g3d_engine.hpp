@@ -71,6 +71,24 @@ struct PartStore {
     Parts view(const PartStore* shared = nullptr);
 };
 
+// The record of one step kept for the reverse of the march (Sim::gradient, g3d_grad.cpp): the beam before and
+// after its push, the real fields before the step, after the march and after every pass but the last of each
+// half-cell, and the streaming particles before every half-cell.  step() and march_plasma fill it while
+// Sim::tape_ points to it.
+struct StepTape {
+    double s0 = 0;
+    long np = 0, nstrm = 0;
+    DevBuf<double> beam0, beam1;      // x y zeta px py pz z rem, [8][np]: before and after the push
+    DevBuf<int> alive0, alive1;
+    DevBuf<double> qm;
+    DevBuf<float> F0, Fmarch, Fmid;   // Fmid: [passes - 1][6][ns + 2][nk]
+    DevBuf<double> ps;                // [N - 1][7][nstrm]: x y zeta z px py pz before half-cell n
+    DevBuf<int> pa;                   // [N - 1][nstrm]
+    DevBuf<double> pq;                // q qm w, [3][nstrm]
+    DevBuf<unsigned long long> ids;   // the ids of the streaming list (its order is that of the whole step)
+    std::vector<unsigned long long> added;   // ids of the electrons trapped into the beam at the end of the march
+};
+
 class Sim {
 public:
     explicit Sim(const Config& c);
@@ -119,6 +137,16 @@ public:
     void write_checkpoint(const std::string& dir);
     void read_checkpoint(const std::string& dir, long step);
     bool stop_ = false;               // the laser has left the bunch by stop_fwhm (diagnostics)
+
+    // The gradient of the mean p_z of the live species-0 beam after `steps` steps from init() in the envelope
+    // values of the first grid laser and in the values of the bunch's current profile (the reverse of the march,
+    // g3d_grad.cpp; no counterpart in the reference).  State checkpoints every `every` steps; each segment is
+    // run again with its tape and reversed.
+    void gradient(int steps, int every, double* loss, double* genv, int nenv, double* gprof, int nprof);
+    StepTape* tape_ = nullptr;
+    void tape_beam(StepTape& t, bool before);     // the beam state into t.beam0 (before the push) or t.beam1
+    bool record_dprof_ = false;       // load_bunch_profile records the profile derivatives below
+    std::vector<double> dprof_;       // per loaded particle: d zeta_f / d prof[m] (nprof), then dx, dy, dzeta / d zeta_f
 
     std::vector<g3d_particle> host_particles();
     void set_particles(const std::vector<g3d_particle>& v);

@@ -1,0 +1,444 @@
+// The gradient of a march (no counterpart in the reference): the mean p_z of the live bunch after a number of
+// steps, differentiated in the envelope values of the grid laser and the values of the bunch's current profile,
+// by the reverse of every step from the last to the first.
+//
+// Forward: init(), then steps with a state checkpoint (device copies of the G3DCKPT1 arrays) every `every` steps.
+// Reverse, segment by segment from the last: restore the segment's checkpoint, run its steps again with a tape
+// (StepTape: the beam before and after its push, the streaming particles before every half-cell, the real fields
+// that every pass read), then reverse its steps.  The reverse of a step undoes, in the opposite order, what step()
+// did: j0 = jk; the sponge and the real fields; the march of the plasma (jk_z -= js0_z, then the slices rear to
+// front and each slice's passes last to first: the real fields of the pass, its per-mode bookkeeping and march,
+// the forward transform of its current, the streaming push through the half-cell with its deposit); the beam
+// currents; the beam push.  The field side is linear for fixed particles and is reversed by its transpose
+// (g3d_grad.hip, g3d_march_adj.hpp); the particle side by the dual-number reverses of the streaming push
+// (g3d_spush.hip) and of the beam push (g3d_bpush.hip).  The decisions of the forward (trapping, particle codes,
+// the half-cell of each beam particle's deposit, the sub-steps of the beam push) are held fixed; a run in which a
+// particle is trapped, re-streamed or lost from the lists is refused.  Then init() in reverse as far as the
+// parameters: j0 = jk, the sponge and real fields, the march of D = 0; the laser's spectra are linear in the
+// envelope values (the hat functions at each slice), the bunch's zeta in the profile values (the inverse of the
+// cumulative distribution, each particle in its trapezoid).
+//
+// FP64 form, Fourier y, a uniform zeta grid, the whole window on this GPU, electrons only, one bunch, no window
+// motion.
+#include <algorithm>
+#include <cmath>
+#include <deque>
+#include <unordered_map>
+
+#include "g3d_engine.hpp"
+#include "g3d_grad.hpp"
+#include "g3d_runtime.hpp"
+
+namespace g3d {
+
+void Sim::tape_beam(StepTape& t, bool before) {
+    t.np = np;
+    DevBuf<double>& d = before ? t.beam0 : t.beam1;
+    DevBuf<int>& al = before ? t.alive0 : t.alive1;
+    if (d.size() != 8 * size_t(np)) d.resize(8 * size_t(np));
+    if (al.size() != size_t(np)) al.resize(size_t(np));
+    if (before && t.qm.size() != size_t(np)) t.qm.resize(size_t(np));
+    if (!np) return;
+    const DevBuf<double>* src[8] = {&beam.x, &beam.y, &beam.zeta, &beam.px, &beam.py, &beam.pz, &beam.z, &beam.rem};
+    const size_t b = size_t(np) * sizeof(double);
+    for (int c = 0; c < 8; ++c)
+        hip_check(hipMemcpyAsync(d.data() + size_t(c) * np, src[c]->data(), b, hipMemcpyDeviceToDevice, stream()), "hipMemcpy D2D");
+    hip_check(hipMemcpyAsync(al.data(), beam.alive.data(), size_t(np) * sizeof(int), hipMemcpyDeviceToDevice, stream()), "hipMemcpy D2D");
+    if (before) hip_check(hipMemcpyAsync(t.qm.data(), beam.qm.data(), b, hipMemcpyDeviceToDevice, stream()), "hipMemcpy D2D");
+}
+
+namespace {
+
+void d2d(void* dst, const void* src, size_t bytes) {
+    if (bytes) hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream()), "hipMemcpy D2D");
+}
+
+// the state of a simulation between steps, in device memory: the arrays of the G3DCKPT1 checkpoint
+struct Snapshot {
+    long istep = 0, lost = 0, trapped = 0, restreamed = 0;
+    double s = 0, X0 = 0, Y0 = 0;
+    bool stop = false;
+    std::vector<DevBuf<cx>> planes;
+    DevBuf<float> F;
+    std::vector<g3d_particle> parts, joining;
+    std::deque<std::array<double, 3>> cent;
+};
+
+// the adjoints of the state and the scratch of the reverse (complex planes in the reference's layout)
+struct Adjoints {
+    DevBuf<cx> R[6], jk[3], j0[2], js0[3], pin, sin, si0, dl;
+    DevBuf<double> F, Fm;                  // real fields after the step / after the march, [6][ns + 2][nk]
+    DevBuf<double> W, Wn;                  // beam: x y zeta px py pz, [6][np]
+    DevBuf<cx> T3, X, Y, P, S, lT, lU;
+    DevBuf<double> jw, Fd, lFp, gF0, gF1, F0d, F1d, bJ;
+    DevBuf<double> lP, lPacc, gP, zeros;   // streaming particles: x y zeta z px py pz, [7][nstrm]
+};
+
+}  // namespace
+
+void Sim::gradient(int steps, int every, double* loss, double* genv, int nenv, double* gprof, int nprof) {
+    bool ions = false;
+    for (const auto& is : cfg.ions) ions = ions || cfg.loaded(is);
+    if (!yf) throw std::runtime_error("gradient: the Fourier y grid only");
+    if (cfg.mixed) throw std::runtime_error("gradient: the FP64 form only (mixed = 0)");
+    if (a != 0 || b != N) throw std::runtime_error("gradient: the whole window on this GPU");
+    if (!cfg.plasma_on || !cfg.self_fields) throw std::runtime_error("gradient: a plasma with self fields");
+    if (ions) throw std::runtime_error("gradient: electrons only (no mobile or ionizing ions)");
+    if (cfg.move_x || cfg.move_y) throw std::runtime_error("gradient: a fixed window (move_x = move_y = 0)");
+    if (!zax.uniform) throw std::runtime_error("gradient: a uniform zeta grid");
+    if (cfg.bunches.size() != 1 || cfg.freeze_bunch) throw std::runtime_error("gradient: one bunch, not frozen");
+    if (nenv > 0 && (cfg.glasers.empty() || cfg.glasers[0].n_env != nenv))
+        throw std::runtime_error("gradient: the first grid laser has no envelope of " + std::to_string(nenv) + " values");
+    if (nprof > 0 && int(cfg.prof.size()) != nprof)
+        throw std::runtime_error("gradient: the bunch has no profile of " + std::to_string(nprof) + " values");
+    if (nprof > 0 && (cfg.bunches[0].emitx > 0 || cfg.bunches[0].emity > 0))
+        throw std::runtime_error("gradient: the profile gradient needs a bunch without emittance");
+    if (steps < 1 || every < 1) throw std::runtime_error("gradient: steps and every must be positive");
+    if (istep != 0 || s != 0 || np != 0) throw std::runtime_error("gradient: runs from init() on a new simulation");
+
+    record_dprof_ = nprof > 0;
+    dprof_.clear();
+    init();
+    record_dprof_ = false;
+    const long np0 = np;
+    auto check_lists = [&] {
+        if (restreamed_total || np != np0 + trapped_total)
+            throw std::runtime_error("gradient: beam particles were re-streamed (" + std::to_string(restreamed_total) +
+                                     ") or dropped from the beam list (" + std::to_string(np0) + " + " +
+                                     std::to_string(trapped_total) + " trapped -> " + std::to_string(np) +
+                                     "); the reverse follows the electrons held over to the next step and those "
+                                     "trapped into the beam, but not these");
+    };
+    auto take = [&](Snapshot& sn) {
+        ensure_layout(false, false);
+        sn.istep = istep; sn.lost = lost_total; sn.trapped = trapped_total; sn.restreamed = restreamed_total;
+        sn.s = s; sn.X0 = X0; sn.Y0 = Y0; sn.stop = stop_;
+        DevBuf<cx>* src[11] = {&Rx, &Ry, &Ez, &Bz, &Lx, &Ly, &j0x, &j0y, &js0x, &js0y, &js0z};
+        sn.planes.clear();
+        for (DevBuf<cx>* p : src) {
+            sn.planes.emplace_back(p->size());
+            d2d(sn.planes.back().data(), p->data(), p->size() * sizeof(cx));
+        }
+        sn.F.resize(F.size());
+        d2d(sn.F.data(), F.data(), F.size() * sizeof(float));
+        sn.parts = host_particles();
+        sn.joining = joining_;
+        sn.cent = cent_;
+    };
+    auto restore = [&](const Snapshot& sn) {
+        ensure_layout(false, false);
+        istep = sn.istep; lost_total = sn.lost; trapped_total = sn.trapped; restreamed_total = sn.restreamed;
+        s = sn.s; X0 = sn.X0; Y0 = sn.Y0; stop_ = sn.stop;
+        DevBuf<cx>* dst[11] = {&Rx, &Ry, &Ez, &Bz, &Lx, &Ly, &j0x, &j0y, &js0x, &js0y, &js0z};
+        for (int k = 0; k < 11; ++k) d2d(dst[k]->data(), sn.planes[size_t(k)].data(), dst[k]->size() * sizeof(cx));
+        d2d(F.data(), sn.F.data(), F.size() * sizeof(float));
+        set_particles(sn.parts);
+        joining_ = sn.joining;
+        cent_ = sn.cent;
+        jw_zero = false;
+    };
+
+    // forward, with the checkpoints
+    std::vector<Snapshot> snaps;
+    for (int k = 0; k < steps; ++k) {
+        if (k % every == 0) { snaps.emplace_back(); take(snaps.back()); }
+        step();
+        check_lists();
+    }
+    {
+        double sum = 0;
+        long cnt = 0;
+        for (const auto& p : host_particles())
+            if (p.alive == 1 && p.sp == 0) { sum += p.pz; ++cnt; }
+        if (!cnt) throw std::runtime_error("gradient: no live bunch particle at the end");
+        *loss = sum / double(cnt);
+    }
+
+    // the adjoint state
+    const long pf = long(ns) * nk, pj = long(ns + 1) * nk, fpl = long(ns + 2) * nk, Fsz = 6 * fpl;
+    Adjoints A;
+    for (auto& r : A.R) r.resize(size_t(pf));
+    for (auto& r : A.jk) r.resize(size_t(pj));
+    for (auto& r : A.j0) r.resize(size_t(pj));
+    for (auto& r : A.js0) r.resize(size_t(pj));
+    A.pin.resize(3 * size_t(nk)); A.sin.resize(3 * size_t(nk)); A.si0.resize(3 * size_t(nk)); A.dl.resize(3 * size_t(nk));
+    A.F.resize(size_t(Fsz)); A.Fm.resize(size_t(Fsz)); A.Fd.resize(size_t(Fsz));
+    const long npm = np;                          // the beam only grows (trapped electrons are appended)
+    A.W.resize(6 * size_t(npm)); A.Wn.resize(6 * size_t(npm));
+    A.T3.resize(3 * size_t(pf)); A.X.resize(size_t(pj)); A.Y.resize(size_t(pj)); A.P.resize(size_t(pj));
+    A.S.resize(size_t(pj)); A.lT.resize(3 * size_t(nk)); A.lU.resize(3 * size_t(nk));
+    A.jw.resize(3 * size_t(ns + 3) * size_t(nk));
+    for (DevBuf<double>* p : {&A.lFp, &A.gF0, &A.gF1, &A.F0d, &A.F1d}) p->resize(6 * size_t(nk));
+    A.bJ.resize(5 * size_t(nk));
+    {                                            // the loss: the mean p_z of the live bunch at the end
+
+        long cnt = 0;
+        for (const auto& p : host_particles()) if (p.alive == 1 && p.sp == 0) ++cnt;
+        seed_mean_pz(beam.view(), np, 0, 1.0 / double(cnt), A.W.data() + 5 * npm);
+    }
+
+    // constants of the reverse
+    const int P = cfg.iterations;
+    const double D = 0.5 * cfg.dz, h = cfg.zeta[1] - cfg.zeta[0];
+    std::vector<double> cen(size_t(N - 1));
+    for (int m = 0; m < N - 1; ++m) cen[size_t(m)] = 0.5 * (cfg.zeta[m] + cfg.zeta[m + 1]);
+    double ia0 = 0;
+    iA.to_host(&ia0, 0, 1);
+    double hmin = 1e300;
+    for (int m = 0; m + 1 < N; ++m) hmin = std::min(hmin, cfg.zeta[m + 1] - cfg.zeta[m]);
+    SpecGeom g = geom();
+    g.cm = 0;
+    const double sc = 1.0 / (double(cfg.nx) * cfg.ny);
+
+    PassAdj pb{};
+    pb.lRx = A.R[0].data(); pb.lRy = A.R[1].data(); pb.lEz = A.R[2].data(); pb.lBz = A.R[3].data();
+    pb.lLx = A.R[4].data(); pb.lLy = A.R[5].data();
+    pb.ljkx = A.jk[0].data(); pb.ljky = A.jk[1].data(); pb.ljkz = A.jk[2].data();
+    pb.lj0x = A.j0[0].data(); pb.lj0y = A.j0[1].data();
+    pb.ljs0x = A.js0[0].data(); pb.ljs0y = A.js0[1].data(); pb.ljs0z = A.js0[2].data();
+    pb.lpin = A.pin.data(); pb.lsin = A.sin.data(); pb.lsi0 = A.si0.data(); pb.ldl = A.dl.data();
+    pb.lT = A.lT.data(); pb.lU = A.lU.data();
+    pb.kx = kxd.data(); pb.ky = kyd.data(); pb.hcg = hcg.data(); pb.wlo = wlo.data(); pb.whi = whi.data();
+    pb.kxs = kxs.data(); pb.kys = kys.data(); pb.filt = filt.data(); pb.filty = filty.data();
+    pb.D = D; pb.sc = sc; pb.a2 = cfg.smooth_length * cfg.smooth_length;
+    pb.a = a; pb.N = N; pb.nx = cfg.nx; pb.ny = cfg.ny; pb.extrap = cfg.predictor_extrapolate; pb.keep0 = cfg.box_mean;
+    pb.nk = nk;
+
+    SpongeAdj sa{};
+    sa.g = g;
+    sa.lRx = A.R[0].data(); sa.lRy = A.R[1].data(); sa.lEz = A.R[2].data(); sa.lBz = A.R[3].data();
+    sa.lLx = A.R[4].data(); sa.lLy = A.R[5].data(); sa.ljkz = A.jk[2].data();
+    sa.hc = hcg.data(); sa.wl = wlo.data(); sa.wu = whi.data(); sa.mx = maskx.data(); sa.mn = masky_n.data();
+    sa.mh = masky_h.data(); sa.g2 = gm2_b; sa.opb = 1 + beta_b; sa.sponge = cfg.sponge > 0 || cfg.sponge_y > 0;
+
+    auto sponge_rev = [&] {
+        // j0 = jk (x, y) after the fields; then the sponge and the real fields
+        axpy_planes(A.jk[0].data(), A.j0[0].data(), 1, pj);
+        axpy_planes(A.jk[1].data(), A.j0[1].data(), 1, pj);
+        A.j0[0].zero(); A.j0[1].zero();
+        sponge_real_fields_adj(sa, fft, A.F.data(), fpl, A.T3.data(), A.X.data(), A.Y.data(), A.P.data());
+        A.F.zero();
+    };
+
+    std::unordered_map<unsigned long long, std::array<double, 6>> carry;   // x y zeta px py pz by id
+    auto march_rev = [&](StepTape& t) {
+        const long ns_ = t.nstrm;
+        if (A.lP.size() != 7 * size_t(ns_)) {
+            A.lP.resize(7 * size_t(ns_)); A.lPacc.resize(7 * size_t(ns_)); A.gP.resize(7 * size_t(ns_));
+            A.zeros.resize(7 * size_t(ns_));
+        }
+        A.Fm.zero(); A.pin.zero(); A.sin.zero(); A.si0.zero(); A.dl.zero();
+        // electrons held over to the next step (trapped at z = s1 below the beam threshold) start its list with
+        // their state at the end of this march: their adjoints from the next step's reverse, by id, start this
+        // one's (z is set to s1 and carries none)
+        std::vector<unsigned long long> ids(static_cast<size_t>(ns_));
+        std::vector<int> alive0(static_cast<size_t>(ns_));
+        t.ids.to_host(ids.data(), 0, size_t(ns_));
+        t.pa.to_host(alive0.data(), size_t(N - 2) * size_t(ns_), size_t(ns_));
+        {
+            std::vector<double> lp(7 * size_t(ns_), 0.0);
+            for (long i = 0; i < ns_; ++i) {
+                const auto it = carry.find(ids[size_t(i)]);
+                if (it == carry.end()) continue;
+                const int slot[6] = {0, 1, 2, 4, 5, 6};
+                for (int c6 = 0; c6 < 6; ++c6) lp[size_t(slot[c6]) * ns_ + i] = it->second[size_t(c6)];
+            }
+            A.lP.from_host(lp.data(), 0, lp.size());
+        }
+        axpy_planes(A.js0[2].data(), A.jk[2].data(), -1, pj);       // jk_z -= js0_z after the slices
+        g3d_spush_ctx c{};
+        c.nx = cfg.nx; c.ny = cfg.ny; c.dx = dx; c.dy = dy; c.zeta0 = cfg.zeta[0]; c.ihz = 1 / h;
+        c.Delta = cfg.dz; c.s1 = t.s0 + cfg.dz; c.sTrap = c.s1 + (cfg.trap_advance - 1) * cfg.dz;
+        c.vz_beam = cfg.vz_beam; c.ia = ia0; c.F0 = A.F0d.data(); c.F1 = A.F1d.data(); c.J = nullptr;
+        c.q = t.pq.data(); c.qm = t.pq.data() + ns_; c.w = t.pq.data() + 2 * ns_; c.np = ns_;
+        g3d_spush_adj ad{};
+        ad.bJ = A.bJ.data(); ad.gF0 = A.gF0.data(); ad.gF1 = A.gF1.data();
+        double* gp = A.gP.data();
+        ad.gx = gp; ad.gy = gp + ns_; ad.gzeta = gp + 2 * ns_; ad.gz = gp + 3 * ns_; ad.gpx = gp + 4 * ns_;
+        ad.gpy = gp + 5 * ns_; ad.gpz = gp + 6 * ns_;
+        for (int n = a; n <= N - 2; ++n) {
+            double* st = t.ps.data() + size_t(n) * 7 * size_t(ns_);
+            c.n = n; c.zn = cfg.zeta[n]; c.zn1 = cfg.zeta[n + 1];
+            c.x = st; c.y = st + ns_; c.zeta = st + 2 * ns_; c.z = st + 3 * ns_; c.px = st + 4 * ns_;
+            c.py = st + 5 * ns_; c.pz = st + 6 * ns_; c.alive = t.pa.data() + size_t(n) * ns_;
+            A.lPacc.zero();
+            for (int p = P - 1; p >= 0; --p) {
+                if (p == P - 1) fields_to_F_adj(A.Fm.data() + long(n - a) * nk, fpl, nk, 1, A.lT.data(), nk);
+                else fields_to_F_adj(A.lFp.data(), nk, nk, 1, A.lT.data(), nk);
+                fft_run(fft, A.lT.data(), 3, false);
+                PassAdj pa = pb;
+                pa.n = n; pa.first = p == 0; pa.last = p == P - 1; pa.top = 0; pa.bk = P > 1;
+                pa.wex = (n >= 1 && n + 1 <= N - 2 && n != N - 2) ? (cen[size_t(n)] - cen[size_t(n - 1)]) / (cen[size_t(n + 1)] - cen[size_t(n)]) : 0.0;
+                pa.qcorr = D * wlo_h[size_t(n + 1)];
+                pass_mode_adj(pa);
+                fft_rows(fft, A.lU.data(), 3L * cfg.ny, true, 1.0);
+                fft_cols(fft, A.lU.data(), 3, true, 1.0);
+                real_part_planes(A.lU.data(), A.bJ.data(), 1.0, nk);
+                const float* f0 = p == 0 ? t.F0.data() + long(n - a) * nk
+                                         : t.Fmid.data() + size_t(p - 1) * size_t(Fsz) + long(n - a) * nk;
+                planes_to_double(f0, fpl, A.F0d.data(), nk, nk, 6);
+                planes_to_double(t.Fmarch.data() + long(n + 1 - a) * nk, fpl, A.F1d.data(), nk, nk, 6);
+                A.gF0.zero(); A.gF1.zero();
+                const double* bs = p == P - 1 ? A.lP.data() : A.zeros.data();
+                ad.bx = bs; ad.by = bs + ns_; ad.bzeta = bs + 2 * ns_; ad.bz = bs + 3 * ns_; ad.bpx = bs + 4 * ns_;
+                ad.bpy = bs + 5 * ns_; ad.bpz = bs + 6 * ns_;
+                if (g3d_stream_push_vjp(&c, &ad)) throw std::runtime_error(last_error());
+                add_values(A.lPacc.data(), A.gP.data(), 7 * ns_);
+                if (p == 0) add_planes(A.F.data() + long(n - a) * nk, fpl, A.gF0.data(), nk, nk, 6);
+                else d2d(A.lFp.data(), A.gF0.data(), 6 * size_t(nk) * sizeof(double));
+                add_planes(A.Fm.data() + long(n + 1 - a) * nk, fpl, A.gF1.data(), nk, nk, 6);
+            }
+            A.lP.swap(A.lPacc);
+        }
+        // the adjoints of the list's start state: those of the electrons held over from the step before go on
+        {
+            std::vector<double> lp(7 * size_t(ns_));
+            A.lP.to_host(lp.data(), 0, lp.size());
+            carry.clear();
+            for (long i = 0; i < ns_; ++i) {
+                if (alive0[size_t(i)] != 6) continue;
+                std::array<double, 6>& v = carry[ids[size_t(i)]];
+                const int slot[6] = {0, 1, 2, 4, 5, 6};
+                for (int c6 = 0; c6 < 6; ++c6) v[size_t(c6)] = lp[size_t(slot[c6]) * ns_ + i];
+            }
+        }
+        // the front slice
+        fields_to_F_adj(A.Fm.data() + long(N - 1 - a) * nk, fpl, nk, 1, A.lT.data(), nk);
+        fft_run(fft, A.lT.data(), 3, false);
+        PassAdj pa = pb;
+        pa.n = N - 1; pa.first = 1; pa.last = 1; pa.top = 1; pa.bk = 0; pa.wex = 0; pa.qcorr = 0;
+        pass_mode_adj(pa);
+    };
+
+    CurrentsAdj ca{};
+    ca.g = g; ca.ljkx = A.jk[0].data(); ca.ljky = A.jk[1].data(); ca.ljkz = A.jk[2].data(); ca.hcg = hcg.data();
+    ca.zf = cfg.zeta_filter; ca.zu = zax.uniform; ca.keep0 = cfg.box_mean;
+    ca.d.zax = zax; ca.d.yax = yax; ca.d.hcg = hcg.data(); ca.d.ia = iA.data(); ca.d.id = iD.data(); ca.d.J = nullptr;
+    ca.d.dx = dx; ca.d.dy = dy; ca.d.nh = ns + 3; ca.d.a = a; ca.d.N = N; ca.d.nx = cfg.nx; ca.d.ny = cfg.ny;
+    ca.d.yf = 1; ca.d.nk = nk;
+
+    auto beam_parts = [&](StepTape& t, bool after) {
+        Parts p = beam.view();
+        double* d = after ? t.beam1.data() : t.beam0.data();
+        p.x = d; p.y = d + t.np; p.zeta = d + 2 * t.np; p.px = d + 3 * t.np; p.py = d + 4 * t.np;
+        p.pz = d + 5 * t.np; p.z = d + 6 * t.np; p.rem = d + 7 * t.np;
+        p.alive = after ? t.alive1.data() : t.alive0.data();
+        p.qm = t.qm.data();
+        return p;
+    };
+
+    auto step_rev = [&](StepTape& t) {
+        sponge_rev();
+        // electrons trapped into the beam at the end of the march (appended after the t.np particles of the step):
+        // their adjoints start the march's reverse from their streaming state, by id
+        if (!t.added.empty()) {
+            std::vector<double> w(6 * size_t(npm));
+            A.W.to_host(w.data(), 0, w.size());
+            for (size_t k = 0; k < t.added.size(); ++k) {
+                std::array<double, 6>& v = carry[t.added[k]];
+                for (int c6 = 0; c6 < 6; ++c6) v[size_t(c6)] = w[size_t(c6) * npm + size_t(t.np) + k];
+            }
+        }
+        march_rev(t);
+        // the beam currents of the pushed beam
+        double* W = A.W.data();
+        currents_adj(ca, fft, beam_parts(t, true), t.np, W, W + npm, W + 3 * npm, W + 4 * npm, W + 5 * npm,
+                     A.X.data(), A.Y.data(), A.P.data(), A.S.data(), A.jw.data());
+        // the beam push (src/sim_particles.cpp:6-24) in the fields before the step
+        planes_to_double(t.F0.data(), F.size(), A.Fd.data(), Fsz, Fsz, 1);
+        const Parts b0 = beam_parts(t, false);
+        g3d_bpush_ctx bc{};
+        bc.nx = cfg.nx; bc.ny = cfg.ny; bc.nsl = ns + 2; bc.dx = dx; bc.dy = dy; bc.zeta0 = cfg.zeta[0]; bc.ihz = 1 / h;
+        bc.F = A.Fd.data();
+        bc.laser_on = laser.on; bc.laser_pol = laser.pol; bc.a0 = laser.a0; bc.kL = laser.kL; bc.tau = laser.tau;
+        bc.cep = laser.cep; bc.eta_c = laser.eta_c; bc.kx = laser.kx; bc.kz = laser.kz;
+        bc.X0 = X0; bc.zc = cfg.zeta_center; bc.s0 = t.s0; bc.s1 = t.s0 + cfg.dz; bc.dz_push = cfg.dz_push;
+        bc.push_phase = cfg.push_phase; bc.hmin = hmin; bc.zhand = -1e300; bc.zmax = cfg.zeta[N - 1];
+        bc.ylo = -1e300; bc.yhi = 1e300; bc.vz_min = cfg.vz_min;
+        bc.x = b0.x; bc.y = b0.y; bc.zeta = b0.zeta; bc.px = b0.px; bc.py = b0.py; bc.pz = b0.pz; bc.z = b0.z;
+        bc.rem = b0.rem; bc.qm = b0.qm; bc.alive = b0.alive; bc.np = t.np;
+        double* Wn = A.Wn.data();
+        g3d_bpush_adj ba{};
+        ba.bx = W; ba.by = W + npm; ba.bzeta = W + 2 * npm; ba.bpx = W + 3 * npm; ba.bpy = W + 4 * npm;
+        ba.bpz = W + 5 * npm;
+        ba.gx = Wn; ba.gy = Wn + npm; ba.gzeta = Wn + 2 * npm; ba.gpx = Wn + 3 * npm; ba.gpy = Wn + 4 * npm;
+        ba.gpz = Wn + 5 * npm; ba.gF = A.F.data();
+        if (g3d_bunch_push_vjp(&bc, &ba)) throw std::runtime_error(last_error());
+        A.W.swap(A.Wn);
+    };
+
+    // reverse, segment by segment
+    std::vector<StepTape> tapes(size_t(std::min(every, steps)));
+    for (int seg = int(snaps.size()) - 1; seg >= 0; --seg) {
+        const int k0 = seg * every, k1 = std::min(steps, k0 + every);
+        restore(snaps[size_t(seg)]);
+        for (int k = k0; k < k1; ++k) {
+            StepTape& t = tapes[size_t(k - k0)];
+            t.s0 = s;
+            tape_beam(t, true);
+            if (t.F0.size() != F.size()) t.F0.resize(F.size());
+            d2d(t.F0.data(), F.data(), F.size() * sizeof(float));
+            tape_ = &t;
+            try { step(); } catch (...) { tape_ = nullptr; throw; }
+            tape_ = nullptr;
+            check_lists();
+        }
+        for (int k = k1 - 1; k >= k0; --k) step_rev(tapes[size_t(k - k0)]);
+    }
+    tapes.clear();
+
+    // init() in reverse as far as the parameters: j0 = jk, the sponge and real fields, the march of D = 0
+    sponge_rev();
+    {
+        g3d_march_adj_ctx mc{};
+        mc.Rx = reinterpret_cast<g3d_cpx*>(A.R[0].data()); mc.Ry = reinterpret_cast<g3d_cpx*>(A.R[1].data());
+        mc.Ez = reinterpret_cast<g3d_cpx*>(A.R[2].data()); mc.Bz = reinterpret_cast<g3d_cpx*>(A.R[3].data());
+        mc.Lx = reinterpret_cast<g3d_cpx*>(A.R[4].data()); mc.Ly = reinterpret_cast<g3d_cpx*>(A.R[5].data());
+        mc.up = nullptr;
+        mc.j0x = reinterpret_cast<g3d_cpx*>(A.j0[0].data()); mc.j0y = reinterpret_cast<g3d_cpx*>(A.j0[1].data());
+        mc.jkx = reinterpret_cast<g3d_cpx*>(A.jk[0].data()); mc.jky = reinterpret_cast<g3d_cpx*>(A.jk[1].data());
+        mc.jkz = reinterpret_cast<g3d_cpx*>(A.jk[2].data());
+        mc.kx = kxd.data(); mc.ky = kyd.data(); mc.hcg = hcg.data(); mc.wlo = wlo.data(); mc.whi = whi.data();
+        mc.D = 0; mc.a = a; mc.N = N; mc.nx = cfg.nx; mc.ny = cfg.ny; mc.nk = nk;
+        if (g3d_march_adjoint(&mc, a, b)) throw std::runtime_error(last_error());
+    }
+    // the laser: R += the spectra of the laser field, linear in the envelope at each slice
+    if (nenv > 0) {
+        const auto& Lz = cfg.glasers[0];
+        GridLaserArgs la;
+        la.amp = Lz.a0 * Lz.kL; la.waist = Lz.waist; la.zR = 0.5 * Lz.kL * Lz.waist * Lz.waist; la.zfoc = Lz.z_focus;
+        la.zc = Lz.zeta_c; la.tau = Lz.tau; la.kL = Lz.kL; la.cep = Lz.cep;
+        la.rc = std::cos(Lz.pol_angle); la.rs = std::sin(Lz.pol_angle); la.dx = dx; la.dy = dy; la.X0 = X0; la.Y0 = Y0;
+        la.circular = Lz.circular; la.helicity = Lz.helicity; la.nx = cfg.nx; la.ny = cfg.ny; la.nk = nk;
+        la.zeta = zs.data(); la.yn = nullptr;
+        std::vector<double> ones(size_t(N), 1.0);
+        DevBuf<double> envd(static_cast<size_t>(N));
+        envd.from_host(ones.data(), 0, size_t(N));
+        la.env = envd.data();
+        grid_laser_fill(la, T3.data(), a, ns);
+        fft_run(fft, T3.data(), ns, false);
+        DevBuf<double> proj(static_cast<size_t>(ns));
+        laser_projection(g, T3.data(), A.R[0].data(), A.R[1].data(), ns, proj.data());
+        std::vector<double> ph(static_cast<size_t>(ns));
+        proj.to_host(ph.data(), 0, size_t(ns));
+        for (int k = 0; k < nenv; ++k) {
+            double gk = 0;
+            const double zk = Lz.env_z0 + k * Lz.env_dz;
+            for (int n = 0; n < ns; ++n)
+                gk += std::max(0.0, 1 - std::fabs(cfg.zeta[a + n] - zk) / Lz.env_dz) * ph[size_t(n)];
+            genv[k] = gk;
+        }
+    }
+    // the profile: zeta of each bunch particle
+    if (nprof > 0) {
+        if (dprof_.size() != size_t(np0) * size_t(nprof)) throw std::runtime_error("gradient: profile derivatives missing");
+        std::vector<double> lz(static_cast<size_t>(np0));
+        A.W.to_host(lz.data(), 2 * size_t(npm), size_t(np0));
+        for (int m = 0; m < nprof; ++m) gprof[m] = 0;
+        for (long i = 0; i < np0; ++i)
+            for (int m = 0; m < nprof; ++m) gprof[m] += lz[size_t(i)] * dprof_[size_t(i) * nprof + m];
+    }
+    hip_check(hipStreamSynchronize(stream()), "synchronize");
+}
+
+}  // namespace g3d

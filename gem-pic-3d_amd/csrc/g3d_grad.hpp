@@ -1,0 +1,67 @@
+// The reverse of a march step for gradients (the adjoint of Sim::step on the Fourier-y grid, FP64 form): the
+// device pieces used by Sim::gradient (g3d_grad.cpp).  No counterpart in the reference.
+#pragma once
+#include "g3d_kernels.hpp"
+
+namespace g3d {
+
+// Adjoints of the per-mode state of the plasma march (complex, the reference's layout j nx + ix): the spectral
+// fields (ns planes), the currents jk, j0 and js0 (ns + 1 planes) and the bookkeeping planes of a march
+// (3 planes each).  lU: the adjoint of the filtered streaming current of a pass, times its filter.
+struct PassAdj {
+    cx *lRx, *lRy, *lEz, *lBz, *lLx, *lLy;
+    cx *ljkx, *ljky, *ljkz, *lj0x, *lj0y, *ljs0x, *ljs0y, *ljs0z;
+    cx *lpin, *lsin, *lsi0, *ldl;
+    const cx* lT;                     // the 3 packed planes after the forward transform of the real-field adjoint
+    cx* lU;
+    const double *kx, *ky, *hcg, *wlo, *whi, *kxs, *kys, *filt, *filty;
+    double D, sc, wex, qcorr, a2;
+    int n, a, N, nx, ny, first, last, top, extrap, bk, keep0;
+    long nk;
+};
+
+// lT (3 planes per slice, ts apart) = the transpose of E, B = f(R, L, E_z + i B_z) for count slices of lF
+// ([6][.][nk] with planes fpl apart, slice l at l nk)
+void fields_to_F_adj(const double* lF, long fpl, long nk, int count, cx* lT, long ts);
+// one pass of half-cell n (or the front slice, c.top): the transposed bookkeeping and march (lT read, lU written)
+void pass_mode_adj(const PassAdj& c);
+// J[c] = scale Re X[c] for 3 planes; J[3], J[4] = 0 (the rho planes of the streaming deposit)
+void real_part_planes(const cx* X, double* J, double scale, long nk);
+// float planes -> double: count planes of n values, sstride / dstride apart
+void planes_to_double(const float* src, long sstride, double* dst, long dstride, long n, int count);
+// y[c dstride + i] += x[c sstride + i] for count planes of n values
+void add_planes(double* y, long dstride, const double* x, long sstride, long n, int count);
+// y += x over n values
+void add_values(double* y, const double* x, long n);
+
+// The sponge and the real fields (Sim::sponge_and_real_fields) reversed: lF (ns slices, consumed) -> the adjoints
+// of R, L, E_z, B_z (the sponge rewrites R in place) and of jkz (through the bound field); Fourier y.
+struct SpongeAdj {
+    SpecGeom g;
+    cx *lRx, *lRy, *lEz, *lBz, *lLx, *lLy, *ljkz;
+    const double *hc, *wl, *wu, *mx, *mn, *mh;
+    double g2, opb;
+    int sponge;
+};
+void sponge_real_fields_adj(const SpongeAdj& c, const FftPlan& p, double* lF, long fpl, cx* T, cx* X, cx* Y,
+                            cx* P);
+
+// The beam currents (Sim::currents) reversed: the adjoints of jkx, jky, jkz (ns + 1 planes, consumed) -> those of
+// the beam particles' x, y, p_x, p_y, p_z (added to).  ζ moves a particle's whole current from one half-cell to the
+// next: its derivative is zero, held fixed.
+struct CurrentsAdj {
+    SpecGeom g;
+    cx *ljkx, *ljky, *ljkz;
+    const double* hcg;
+    int zf, zu, keep0;
+    DepositArgs d;
+};
+void currents_adj(const CurrentsAdj& c, const FftPlan& p, Parts P, long np, double* lx, double* ly, double* lpx,
+                  double* lpy, double* lpz, cx* X, cx* Y, cx* Z, cx* S, double* ljw);
+
+// the loss seed: lpz = 1 / count for the live particles of species sp0 (alive 1), 0 elsewhere
+void seed_mean_pz(Parts P, long np, int sp0, double inv_count, double* lpz);
+// per slice l < count: out[l] += Re sum_i conj(lRx) X + conj(lRy) Y, (X, Y) the unpacked pair of T (grid laser)
+void laser_projection(const SpecGeom& g, const cx* T, const cx* lRx, const cx* lRy, int count, double* out);
+
+}  // namespace g3d

@@ -327,6 +327,8 @@ def _load():
         "g3d_sim_write_axis": ([P, C.c_char_p], C.c_int),
         "g3d_sim_write_checkpoint": ([P, C.c_char_p], C.c_int),
         "g3d_sim_read_checkpoint": ([P, C.c_char_p, C.c_long], C.c_int),
+        "g3d_sim_gradient": ([P, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int,
+                              C.POINTER(C.c_double), C.c_int], C.c_int),
         "g3d_sim_timers": ([P, C.POINTER(C.c_double), C.POINTER(C.c_long), C.c_int, C.c_char_p, C.c_int], C.c_int),
     }
     for name, (args, res) in sig.items():
@@ -467,6 +469,20 @@ class Sim(SimBase):
 
     def read_checkpoint(self, directory, step):
         check(lib().g3d_sim_read_checkpoint(self.h, str(directory).encode(), int(step)))
+
+    def gradient(self, steps, every=5):
+        """On a new simulation: init() and `steps` steps, then the reverse (g3d_sim_gradient).  Returns the mean p_z of
+        the live bunch at the end and its derivatives in the envelope values of the first grid laser and in the
+        bunch's current-profile values (numpy float64; empty where the deck has none)."""
+        c = self._deck.c
+        nenv = int(c.glasers[0].n_env) if c.n_glasers > 0 else 0
+        nprof = int(c.n_prof)
+        loss = C.c_double()
+        genv, gprof = np.zeros(max(nenv, 1)), np.zeros(max(nprof, 1))
+        dp = C.POINTER(C.c_double)
+        check(lib().g3d_sim_gradient(self.h, int(steps), int(every), C.byref(loss), genv.ctypes.data_as(dp), nenv,
+                                     gprof.ctypes.data_as(dp), nprof))
+        return loss.value, genv[:nenv], gprof[:nprof]
 
     FAMILIES = ("stream_halfcell", "pass_rows_fwd", "pass_cols", "pass_rows_inv", "unused", "beam_push", "currents",
                 "sponge_real_fields")

@@ -469,3 +469,26 @@ def _push_step(geo, F, state, fix, qm):
     z, rem, alive = fix
     out, z2, rem2, al = bunch_push_raw(geo, F, state, z, rem, qm, alive)
     return out, z2, rem2, al
+
+
+class WitnessEnergy(torch.autograd.Function):
+    """The mean p_z of the live bunch after the march of a deck, as a function of the envelope values of its first
+    grid laser and the values of its bunch's current profile (float64 tensors on any device).  make_deck(env, prof)
+    builds the deck from numpy arrays; steps and every as g3d_sim_gradient.  The forward runs the whole march and
+    its reverse on the GPU (Sim.gradient: the transposed field updates, the reverse pushes, a state checkpoint
+    every `every` steps with each segment run again); backward scales the stored gradient by the incoming one."""
+    @staticmethod
+    def forward(ctx, env, prof, make_deck, steps, every):
+        deck = make_deck(env.detach().cpu().numpy().astype("float64"), prof.detach().cpu().numpy().astype("float64"))
+        sim = g3d_amd.Sim(deck)
+        try:
+            loss, genv, gprof = sim.gradient(int(steps), int(every))
+        finally:
+            sim.close()
+        ctx.save_for_backward(torch.from_numpy(genv).to(env.device), torch.from_numpy(gprof).to(prof.device))
+        return torch.tensor(loss, dtype=torch.float64, device=env.device)
+
+    @staticmethod
+    def backward(ctx, g):
+        genv, gprof = ctx.saved_tensors
+        return g * genv, g * gprof, None, None, None

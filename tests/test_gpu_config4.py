@@ -3,13 +3,18 @@ amplitude along zeta is a piecewise-linear a0(zeta) (the reference's envelope_fu
 the hat functions) and the witness whose current profile is piecewise linear (distribution_z = function, profile_z,
 sampled by the inverse of its cumulative distribution).  On the config-4 deck at 32 x 32 modes and 64 slices: the
 witness drawn at init equals the reference's particle for particle, and after init and two steps the particles and
-fields agree at the bars of the FP64 form."""
+fields agree at the bars of the FP64 form.
+
+The gradient of the config-4 loss (the mean p_z of the live witness after the steps) in its 32 envelope and 16
+profile values by the reverse of the march (Sim.gradient, g3d_sim_gradient): against central differences of the
+engine's loss and of the reference's, the same with checkpoints every step and every second step, through
+torch.autograd (g3d_autograd.WitnessEnergy), and at the configuration's full size."""
 import numpy as np
 import pytest
 
 import g3d_amd
 import refpy
-from decks import config4_deck, particle_parity
+from decks import config4_deck, config4_params, particle_parity
 from test_gpu_engine import compare_fields
 
 pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not refpy.available(), reason="oracle/_ref not built")]
@@ -69,3 +74,124 @@ def test_config4_fd_derivatives_against_reference():
         print("d loss / d %s[%d]: engine %.9e reference %.9e" % (which, k, out["gpu"], out["ref"]), flush=True)
         assert out["ref"] != 0
         assert abs(out["gpu"] - out["ref"]) <= 1e-5 * abs(out["ref"]) + 1e-7, (which, k, out)
+
+
+def medium(env=None, prof=None):
+    """config 4 on 64 x 64 modes and 256 slices, 16384 witness electrons, two steps"""
+    return config4_deck(env=env, prof=prof, n=64, nz=256, nwit=16384, steps=2)
+
+
+def run_loss(cls, deck, steps):
+    s = cls(deck)
+    s.init()
+    for _ in range(steps):
+        s.step()
+    p = s.array("parts")
+    if hasattr(s, "close"):
+        s.close()
+    return float(p[(p["alive"] == 1) & (p["sp"] == 0)]["pz"].mean())
+
+
+def central(cls, make, which, k, h, steps=2):
+    env0, prof0 = config4_params()
+    val = []
+    for sgn in (1, -1):
+        env, prof = env0.copy(), prof0.copy()
+        (env if which == "env" else prof)[k] += sgn * h
+        val.append(run_loss(cls, make(env, prof), steps))
+    return (val[0] - val[1]) / (2 * h)
+
+
+# the bar of the gradient against central differences: 1e-6 absolute plus 1e-4 of the derivative.  The loss is
+# smooth only piecewise (a witness particle's current moves to the next half-cell when its zeta crosses a boundary,
+# an electron held over to the next step joins the march at a slice, F is float32), so a difference across such an
+# event is no derivative: the best of the steps 1e-3 ... 1e-5 is compared
+FD_STEPS = (1e-3, 3e-4, 1e-4, 3e-5, 1e-5)
+
+
+def fd_bar(g):
+    return 1e-6 + 1e-4 * abs(g)
+
+
+def test_config4_gradient_against_differences():
+    env0, prof0 = config4_params()
+    s = g3d_amd.Sim(medium(env0, prof0))
+    loss, genv, gprof = s.gradient(2, every=1)
+    s.close()
+    s = g3d_amd.Sim(medium(env0, prof0))
+    loss2, genv2, gprof2 = s.gradient(2, every=2)
+    s.close()
+    assert genv.shape == (32,) and gprof.shape == (16,)
+    assert abs(loss - run_loss(g3d_amd.Sim, medium(env0, prof0), 2)) <= 1e-12 * abs(loss)
+    np.testing.assert_allclose(genv2, genv, rtol=1e-10, atol=1e-14 * np.abs(genv).max())
+    np.testing.assert_allclose(gprof2, gprof, rtol=1e-10, atol=1e-14 * np.abs(gprof).max())
+    for which, k in (("env", 8), ("env", 12), ("env", 20), ("prof", 3), ("prof", 7), ("prof", 12)):
+        g = (genv if which == "env" else gprof)[k]
+        fds = [central(g3d_amd.Sim, medium, which, k, h) for h in FD_STEPS]
+        best = min(abs(f - g) for f in fds)
+        print("d loss / d %s[%d]: adjoint %.9e, central differences %s, closest %.2e"
+              % (which, k, g, " ".join("%.6e" % f for f in fds), best), flush=True)
+        assert g != 0 and best <= fd_bar(g), (which, k, g, fds)
+
+
+def test_config4_gradient_against_reference_differences():
+    """the adjoint gradient against central differences of the reference's own build on the same deck, at a step
+    where the loss is smooth (one envelope value at 1e-3, one profile value at 1e-5)"""
+    refpy.set_threads(16)
+    env0, prof0 = config4_params()
+    s = g3d_amd.Sim(medium(env0, prof0))
+    _, genv, gprof = s.gradient(2, every=1)
+    s.close()
+    for which, k, h in (("env", 8, 1e-3), ("prof", 3, 1e-5)):
+        g = (genv if which == "env" else gprof)[k]
+        fr = central(refpy.RefSim, medium, which, k, h)
+        print("d loss / d %s[%d]: adjoint %.9e, the reference's central difference (%g) %.9e" % (which, k, g, h, fr),
+              flush=True)
+        assert abs(fr - g) <= fd_bar(g), (which, k, g, fr)
+
+
+def test_config4_autograd():
+    """the loss and its gradient through torch.autograd: backward gives the engine's gradient times the incoming one"""
+    import torch
+    from g3d_autograd import WitnessEnergy
+    env0, prof0 = config4_params()
+    env = torch.tensor(env0, dtype=torch.float64, requires_grad=True)
+    prof = torch.tensor(prof0, dtype=torch.float64, requires_grad=True)
+    out = WitnessEnergy.apply(env, prof, small_deck, 2, 1)
+    (3.0 * out).backward()
+    s = g3d_amd.Sim(small_deck(env0, prof0))
+    loss, genv, gprof = s.gradient(2, every=1)
+    s.close()
+    assert abs(out.item() - loss) <= 1e-12 * abs(loss)
+    np.testing.assert_allclose(env.grad.numpy(), 3.0 * genv, rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(prof.grad.numpy(), 3.0 * gprof, rtol=1e-12, atol=1e-15)
+
+
+def small_deck(env, prof):
+    return config4_deck(env=env, prof=prof, n=32, nz=64, nwit=4096, steps=2)
+
+
+def test_config4_full_gradient():
+    """config 4 at its size (128 x 128 modes, 512 slices, 2^18 witness electrons, 20 steps): the gradient in all 48
+    values with a checkpoint every 5 steps and with one every 20 agree; central differences (1e-4) in two values agree
+    to the loss's own noise at this size (2 %: the steps 1e-4 ... 1e-6 scatter by that much)"""
+    env0, prof0 = config4_params()
+    out = []
+    for every in (5, 20):
+        s = g3d_amd.Sim(config4_deck(env0, prof0))
+        out.append(s.gradient(20, every=every))
+        s.close()
+    (l5, e5, p5), (l20, e20, p20) = out
+    print("config 4: loss %.12e, |grad env| %.4e, |grad prof| %.4e" % (l5, np.linalg.norm(e5), np.linalg.norm(p5)),
+          flush=True)
+    # the plasma deposit adds with atomics, so two runs of the 20 steps differ in the last bits (1e-12 of the loss)
+    de, dp = np.abs(e20 - e5).max() / np.abs(e5).max(), np.abs(p20 - p5).max() / np.abs(p5).max()
+    print("config 4: checkpoints every 5 and every 20 steps: loss %.2e, gradients %.2e, %.2e apart (relative)"
+          % (abs(l20 - l5) / abs(l5), de, dp), flush=True)
+    assert abs(l20 - l5) <= 1e-10 * abs(l5)
+    assert de <= 1e-6 and dp <= 1e-6
+    for which, k in (("env", 5), ("prof", 3)):
+        g = (e5 if which == "env" else p5)[k]
+        f = central(g3d_amd.Sim, lambda e, p: config4_deck(e, p), which, k, 1e-4, steps=20)
+        print("config 4: d loss / d %s[%d]: adjoint %.6e, central difference (1e-4) %.6e" % (which, k, g, f), flush=True)
+        assert abs(f - g) <= 0.02 * abs(g), (which, k, g, f)
