@@ -11,6 +11,9 @@ TscDeposit / TscGather: the TSC deposit of particle values onto a periodic plane
 other's adjoint, with the derivatives in the particle positions.  BorisFunction: the Boris rotation of the
 streaming push with its hand-written vector-Jacobian product.  All float64 on the GPU.
 
+WitnessEnergy: the mean p_z of the live bunch after a whole run as a function of the grid laser's envelope values
+and the bunch's current-profile values, its gradient from the reverse of the march (g3d_sim_gradient).
+
 torch runs its kernels on its current stream and the library on its own, so each call synchronises the device
 before and after the library call.
 """
