@@ -1324,6 +1324,10 @@ void Sim::march_plasma(double s0, double s1) {
                 pass_mixed(m, fft, jsr_f.data(), slab_f.data(), tg, Xf, Tf, F.data() + long(n - a) * nk, fpl,
                            last ? rho_lines.data() + long(n) * (nx + ny) : nullptr, nullptr, i0, j0, 0, rpw);
                 toc(F_MODES);
+                if (tape_ && !last)
+                    for (int c = 0; c < 6; ++c)
+                        tape_copy(tape_->Fmid.data() + size_t(pass) * F.size() + size_t(c) * fpl + size_t(n - a) * nk,
+                                  F.data() + size_t(c) * fpl + size_t(n - a) * nk, size_t(nk) * sizeof(float));
                 continue;
             }
             const LaunchTimer lt = launch_timer(F_STREAM);

@@ -18,8 +18,9 @@
 // envelope values (the hat functions at each slice), the bunch's zeta in the profile values (the inverse of the
 // cumulative distribution, each particle in its trapezoid).
 //
-// FP64 form, Fourier y, a uniform zeta grid, the whole window on this GPU, electrons only, one bunch, no window
-// motion.
+// Fourier y, a uniform zeta grid, the whole window on this GPU, electrons only, one bunch, no window motion.  The
+// reverse is FP64 in both forms: for the FP32-mixed form it applies the transposes of the same linear field updates
+// and the FP64 push reverses at the mixed trajectory (the FP32 roundings of the forward taken as the identity).
 #include <algorithm>
 #include <cmath>
 #include <deque>
@@ -80,7 +81,6 @@ void Sim::gradient(int steps, int every, double* loss, double* genv, int nenv, d
     bool ions = false;
     for (const auto& is : cfg.ions) ions = ions || cfg.loaded(is);
     if (!yf) throw std::runtime_error("gradient: the Fourier y grid only");
-    if (cfg.mixed) throw std::runtime_error("gradient: the FP64 form only (mixed = 0)");
     if (a != 0 || b != N) throw std::runtime_error("gradient: the whole window on this GPU");
     if (!cfg.plasma_on || !cfg.self_fields) throw std::runtime_error("gradient: a plasma with self fields");
     if (ions) throw std::runtime_error("gradient: electrons only (no mobile or ionizing ions)");

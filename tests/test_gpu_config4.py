@@ -7,8 +7,9 @@ fields agree at the bars of the FP64 form.
 
 The gradient of the config-4 loss (the mean p_z of the live witness after the steps) in its 32 envelope and 16
 profile values by the reverse of the march (Sim.gradient, g3d_sim_gradient): against central differences of the
-engine's loss and of the reference's, the same with checkpoints every step and every second step, through
-torch.autograd (g3d_autograd.WitnessEnergy), and at the configuration's full size."""
+engine's loss and of the reference's, the same with checkpoints every step and every second step, the FP32-mixed
+form's against the FP64 form's, through torch.autograd (g3d_autograd.WitnessEnergy), and at the configuration's full
+size."""
 import numpy as np
 import pytest
 
@@ -132,6 +133,24 @@ def test_config4_gradient_against_differences():
         print("d loss / d %s[%d]: adjoint %.9e, central differences %s, closest %.2e"
               % (which, k, g, " ".join("%.6e" % f for f in fds), best), flush=True)
         assert g != 0 and best <= fd_bar(g), (which, k, g, fds)
+
+
+def test_config4_mixed_gradient():
+    """the gradient of the FP32-mixed form (its forward, the FP64 reverse at its trajectory) against the FP64 form's:
+    the two losses agree to 1e-8 and so do their derivatives, to 1e-5 of the largest (central differences of the
+    mixed loss resolve no better than 1e-5: its FP32 field pass rounds at 6e-8)"""
+    env0, prof0 = config4_params()
+    out = []
+    for mixed in (0, 1):
+        s = g3d_amd.Sim(config4_deck(env=env0, prof=prof0, n=64, nz=256, nwit=16384, steps=2, mixed=mixed))
+        out.append(s.gradient(2, every=1))
+        s.close()
+    (l0, e0, p0), (l1, e1, p1) = out
+    de, dp = np.abs(e1 - e0).max() / np.abs(e0).max(), np.abs(p1 - p0).max() / np.abs(p0).max()
+    print("mixed against FP64: loss %.2e, gradients %.2e (envelope) %.2e (profile) apart, relative"
+          % (abs(l1 - l0) / abs(l0), de, dp), flush=True)
+    assert abs(l1 - l0) <= 1e-8 * abs(l0)
+    assert de <= 1e-5 and dp <= 1e-5
 
 
 def test_config4_gradient_against_reference_differences():
