@@ -258,6 +258,14 @@ int g3d_sim_gradient(g3d_sim* sim, int steps, int every, double* loss, double* g
         S(sim).gradient(steps, every, loss, genv, nenv, gprof, nprof);
     });
 }
+int g3d_sim_tangent(g3d_sim* sim, int steps, const double* denv, int nenv, const double* dprof, int nprof,
+                    double* loss, double* dloss) {
+    return g3d::guard([&] {
+        if (!loss || !dloss || (nenv > 0 && !denv) || (nprof > 0 && !dprof))
+            throw std::invalid_argument("tangent: missing direction or output");
+        S(sim).tangent(steps, denv, nenv, dprof, nprof, loss, dloss);
+    });
+}
 int g3d_sim_timing(g3d_sim* sim, int enable) {
     return g3d::guard([&] {
         g3d::Sim& s = S(sim);

@@ -29,6 +29,9 @@ G3D_HD St6<T> axpy(const St6<T>& a, double h, const St6<T>& b) {
 }
 G3D_HD double twrap(double x, double L) { return wrap_mod(x, L); }
 G3D_HD Db twrap(const Db& x, double L) { return x + (wrap_mod(x.v, L) - x.v); }
+using B1 = Dual<1>;                 // one direction: the tangent-linear push
+G3D_HD B1 bt1(double v, double d) { B1 r; r.v = v; r.d[0] = d; return r; }
+G3D_HD B1 twrap(const B1& x, double L) { return x + (wrap_mod(x.v, L) - x.v); }
 
 // the gather point of (x, y, ζ) (gather_t on a uniform ζ axis with the first slice at index 0): nodes, TSC weights
 // and their x, y derivatives, the slice pair and its ζ weights; ok = false where the point has no slice pair
@@ -258,6 +261,50 @@ G3D_HD void bpush_vjp_one(const g3d_bpush_ctx& c, const g3d_bpush_adj& a, long i
     out();
 }
 
+// particle i, tangent-linear: the primal push decides the sub-steps (held fixed); the sub-steps then run again on
+// dual numbers of one direction from the tangent of the start state, each stage's gather adding the tangent of the
+// fields (dF, the layout of F) at its point; the pushed tangents replace dst ([6][stride]: x y zeta px py pz)
+G3D_HD void bpush_tlm_one(const g3d_bpush_ctx& c, const double* dF, double* dst, long stride, long i) {
+    const int alive = c.alive[i];
+    if (alive != 1 && alive != 3) return;
+    const double z0 = alive == 3 ? c.s1 - c.rem[i] : c.s0;
+    const double qm = c.qm[i];
+    const St6<double> Y0{c.x[i], c.y[i], c.zeta[i], c.px[i], c.py[i], c.pz[i]};
+    PrimalFld pf{&c};
+    St6<double> Y = Y0;
+    const PushEnd e = run_primal(c, Y, z0, qm, -1, pf);
+    if (e.how == -1) return;                        // the failed push keeps the start state
+    const long nk = long(c.nx) * c.ny;
+    St6<B1> Yd{bt1(Y0.x, dst[i]), bt1(Y0.y, dst[stride + i]), bt1(Y0.zeta, dst[2 * stride + i]),
+               bt1(Y0.px, dst[3 * stride + i]), bt1(Y0.py, dst[4 * stride + i]), bt1(Y0.pz, dst[5 * stride + i])};
+    auto fld = [&](int, const B1& x, const B1& y, const B1& zeta, B1 E[3], B1 B[3]) {
+        const GP3 g = gp3(c, x.v, y.v, zeta.v);
+        for (int comp = 0; comp < 6; ++comp) {
+            double gx, gy, gzeta;
+            const double v = gp3_comp(c, g, comp, &gx, &gy, &gzeta);
+            double d = gx * x.d[0] + gy * y.d[0] + gzeta * zeta.d[0];
+            if (g.ok) {
+                const double* q0 = dF + (long(comp) * c.nsl + g.fl) * nk;
+                for (int t = 0; t < 9; ++t) d += g.w[t] * (g.gz * q0[g.k[t]] + g.fz * q0[nk + g.k[t]]);
+            }
+            if (comp < 3) E[comp] = bt1(v, d); else B[comp - 3] = bt1(v, d);
+        }
+    };
+    double z = z0;
+    for (int j = 0; j < e.steps; ++j) {
+        const St6<double> Yp{Yd.x.v, Yd.y.v, Yd.zeta.v, Yd.px.v, Yd.py.v, Yd.pz.v};
+        const double ds = sub_len(c, Yp, z);
+        tsub(c, Yd, z, ds, qm, fld);
+        z += ds;
+    }
+    dst[i] = Yd.x.d[0]; dst[stride + i] = Yd.y.d[0]; dst[2 * stride + i] = Yd.zeta.d[0];
+    dst[3 * stride + i] = Yd.px.d[0]; dst[4 * stride + i] = Yd.py.d[0]; dst[5 * stride + i] = Yd.pz.d[0];
+}
+__global__ void __launch_bounds__(kT) k_bpush_tlm(g3d_bpush_ctx c, const double* dF, double* dst, long stride) {
+    const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < c.np) bpush_tlm_one(c, dF, dst, stride, i);
+}
+
 __global__ void __launch_bounds__(kT) k_bpush(g3d_bpush_ctx c) {
     const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < c.np) bpush_one(c, i);
@@ -267,6 +314,12 @@ __global__ void __launch_bounds__(kT) k_bpush_vjp(g3d_bpush_ctx c, g3d_bpush_adj
     if (i < c.np) bpush_vjp_one(c, a, i, [](double* p, double v) { atomicAdd(p, v); });
 }
 }  // namespace
+
+void bunch_push_tlm(const g3d_bpush_ctx& c, const double* dF, double* dst, long stride) {
+    if (c.np <= 0) return;
+    hipLaunchKernelGGL(k_bpush_tlm, dim3(blocks_for(c.np, kT)), dim3(kT), 0, stream(), c, dF, dst, stride);
+    launch_check("bunch push tangent");
+}
 }  // namespace g3d
 
 using namespace g3d;

@@ -60,6 +60,9 @@ struct Config {
     bool loaded(const g3d_ion& s) const { return s.mobile || s.ionization; }
 };
 
+// a piecewise-linear function as the sum of its hat functions v[k] max(0, 1 - |x - x_k| / h), x_k = x0 + k h
+double hat_sum(const double* v, int n, double x0, double h, double x);
+
 // structure-of-arrays particle storage; `shared` borrows w, sp, id, rem of another store (the second buffer of
 // the streaming list)
 struct PartStore {
@@ -143,6 +146,8 @@ public:
     // g3d_grad.cpp; no counterpart in the reference).  State checkpoints every `every` steps; each segment is
     // run again with its tape and reversed.
     void gradient(int steps, int every, double* loss, double* genv, int nenv, double* gprof, int nprof);
+    // the derivative of the same loss along the direction (denv, dprof) by the tangent-linear run (g3d_grad.cpp)
+    void tangent(int steps, const double* denv, int nenv, const double* dprof, int nprof, double* loss, double* dloss);
     StepTape* tape_ = nullptr;
     void tape_beam(StepTape& t, bool before);     // the beam state into t.beam0 (before the push) or t.beam1
     bool record_dprof_ = false;       // load_bunch_profile records the profile derivatives below

@@ -75,17 +75,17 @@ struct Adjoints {
     DevBuf<double> lP, lPacc, gP, zeros;   // streaming particles: x y zeta z px py pz, [7][nstrm]
 };
 
-}  // namespace
-
-void Sim::gradient(int steps, int every, double* loss, double* genv, int nenv, double* gprof, int nprof) {
+// what the reverse and the tangent-linear run of the march cover
+void check_reversible(const Sim& sm, int steps, int every, int nenv, int nprof) {
+    const Config& cfg = sm.cfg;
     bool ions = false;
     for (const auto& is : cfg.ions) ions = ions || cfg.loaded(is);
-    if (!yf) throw std::runtime_error("gradient: the Fourier y grid only");
-    if (a != 0 || b != N) throw std::runtime_error("gradient: the whole window on this GPU");
+    if (!sm.yf) throw std::runtime_error("gradient: the Fourier y grid only");
+    if (sm.a != 0 || sm.b != sm.N) throw std::runtime_error("gradient: the whole window on this GPU");
     if (!cfg.plasma_on || !cfg.self_fields) throw std::runtime_error("gradient: a plasma with self fields");
     if (ions) throw std::runtime_error("gradient: electrons only (no mobile or ionizing ions)");
     if (cfg.move_x || cfg.move_y) throw std::runtime_error("gradient: a fixed window (move_x = move_y = 0)");
-    if (!zax.uniform) throw std::runtime_error("gradient: a uniform zeta grid");
+    if (!sm.zax.uniform) throw std::runtime_error("gradient: a uniform zeta grid");
     if (cfg.bunches.size() != 1 || cfg.freeze_bunch) throw std::runtime_error("gradient: one bunch, not frozen");
     if (nenv > 0 && (cfg.glasers.empty() || cfg.glasers[0].n_env != nenv))
         throw std::runtime_error("gradient: the first grid laser has no envelope of " + std::to_string(nenv) + " values");
@@ -94,7 +94,14 @@ void Sim::gradient(int steps, int every, double* loss, double* genv, int nenv, d
     if (nprof > 0 && (cfg.bunches[0].emitx > 0 || cfg.bunches[0].emity > 0))
         throw std::runtime_error("gradient: the profile gradient needs a bunch without emittance");
     if (steps < 1 || every < 1) throw std::runtime_error("gradient: steps and every must be positive");
-    if (istep != 0 || s != 0 || np != 0) throw std::runtime_error("gradient: runs from init() on a new simulation");
+    if (sm.istep != 0 || sm.s != 0 || sm.np != 0) throw std::runtime_error("gradient: runs from init() on a new simulation");
+
+}
+
+}  // namespace
+
+void Sim::gradient(int steps, int every, double* loss, double* genv, int nenv, double* gprof, int nprof) {
+    check_reversible(*this, steps, every, nenv, nprof);
 
     record_dprof_ = nprof > 0;
     dprof_.clear();
@@ -438,6 +445,262 @@ void Sim::gradient(int steps, int every, double* loss, double* genv, int nenv, d
         for (long i = 0; i < np0; ++i)
             for (int m = 0; m < nprof; ++m) gprof[m] += lz[size_t(i)] * dprof_[size_t(i) * nprof + m];
     }
+    hip_check(hipStreamSynchronize(stream()), "synchronize");
+}
+
+// The tangent-linear run of the same march (no counterpart in the reference): the derivative of the loss of
+// Sim::gradient along a direction (denv, dprof) of the parameters, by carrying the tangents of every state array
+// forward through the steps beside the primal ones.  The field side is linear for fixed particles, so its tangent
+// runs the forward's own kernels on the tangent arrays (the march of D = 0, the pass's row transforms and per-mode
+// kernel, the sponge, the currents' filter and transforms) with the real fields kept in FP64; the pushes and the
+// beam deposit run on dual numbers of one direction (k_spush_tlm, k_bpush_tlm, k_dep_tlm), reading the primal
+// state from the step's tape.  A check of the reverse: <gradient, direction> equals this derivative.
+void Sim::tangent(int steps, const double* denv, int nenv, const double* dprof, int nprof, double* loss, double* dloss) {
+    check_reversible(*this, steps, 1, nenv, nprof);
+    record_dprof_ = nprof > 0;
+    dprof_.clear();
+    init();
+    record_dprof_ = false;
+    const long pf = long(ns) * nk, pj = long(ns + 1) * nk, fpl = long(ns + 2) * nk, Fsz = 6 * fpl, ts = pf;
+    const double h = cfg.zeta[1] - cfg.zeta[0], opb = 1 + beta_b, D = 0.5 * cfg.dz;
+    const int P = cfg.iterations;
+    const bool sp = cfg.sponge > 0 || cfg.sponge_y > 0;
+    SpecGeom g = geom();
+    g.cm = 0;
+    double ia0 = 0;
+    iA.to_host(&ia0, 0, 1);
+    double hmin = 1e300;
+    for (int m = 0; m + 1 < N; ++m) hmin = std::min(hmin, cfg.zeta[m + 1] - cfg.zeta[m]);
+    std::vector<double> cen(size_t(N - 1));
+    for (int m = 0; m < N - 1; ++m) cen[size_t(m)] = 0.5 * (cfg.zeta[m] + cfg.zeta[m + 1]);
+
+    // the tangent state
+    DevBuf<cx> tR[6], tjk[3], tj0[2], tjs0[3], tpin, tsin, tsi0, tdl, tX, tT;
+    for (auto& r : tR) r.resize(size_t(pf));
+    for (auto& r : tjk) r.resize(size_t(pj));
+    for (auto& r : tj0) r.resize(size_t(pj));
+    for (auto& r : tjs0) r.resize(size_t(pj));
+    for (DevBuf<cx>* q : {&tpin, &tsin, &tsi0, &tdl, &tX, &tT}) q->resize(3 * size_t(nk));
+    DevBuf<double> tF(static_cast<size_t>(Fsz)), tFm(static_cast<size_t>(Fsz)), Fd(static_cast<size_t>(Fsz));
+    DevBuf<double> tFmid(6 * size_t(nk)), tJ(5 * size_t(nk)), F0d(6 * size_t(nk)), F1d(6 * size_t(nk));
+    DevBuf<double> dF0(6 * size_t(nk)), dF1(6 * size_t(nk)), tjw(3 * size_t(ns + 3) * size_t(nk)), tP;
+    long cap = np;
+    DevBuf<double> tW(6 * size_t(cap));
+    if (nprof > 0) {                                  // the bunch's zeta along dprof
+        std::vector<double> w(6 * size_t(cap), 0.0);
+        for (long i = 0; i < np; ++i) {
+            double v = 0;
+            for (int m = 0; m < nprof; ++m) v += dprof[m] * dprof_[size_t(i) * nprof + m];
+            w[2 * size_t(cap) + size_t(i)] = v;
+        }
+        tW.from_host(w.data(), 0, w.size());
+    }
+    auto tmview = [&](double Dm) {
+        MarchView v = mview(Dm);
+        v.Rx = tR[0].data(); v.Ry = tR[1].data(); v.Ez = tR[2].data(); v.Bz = tR[3].data(); v.Lx = tR[4].data();
+        v.Ly = tR[5].data(); v.j0x = tj0[0].data(); v.j0y = tj0[1].data(); v.jkx = tjk[0].data();
+        v.jky = tjk[1].data(); v.jkz = tjk[2].data(); v.cm = 0;
+        return v;
+    };
+    auto sponge_tlm = [&] {
+        if (sp) {
+            sponge_pack(g, tR[0].data(), tR[1].data(), tjk[2].data(), hcg.data(), wlo.data(), whi.data(), gm2_b, opb,
+                        S1.data(), S2.data());
+            fft_run(fft, S1.data(), ns, true);
+            sponge_mask(g, S1.data(), maskx.data(), masky_n.data(), masky_h.data(), ns);
+            fft_run(fft, S1.data(), ns, false);
+        }
+        fields_pack3(g, sp ? S1.data() : nullptr, S2.data(), opb, tR[0].data(), tR[1].data(), tR[4].data(),
+                     tR[5].data(), tR[2].data(), tR[3].data(), T3.data(), ts, ns);
+        fft_run(fft, T3.data(), 3 * ns, true);
+        fields_to_F_double(T3.data(), ts, tF.data(), fpl, nk, ns);
+        copy_planes(tj0[0].data(), tjk[0].data(), pj);
+        copy_planes(tj0[1].data(), tjk[1].data(), pj);
+    };
+    // init: the laser along denv (its spectra are linear in the envelope), the march of D = 0, the sponge
+    if (nenv > 0) {
+        const auto& Lz = cfg.glasers[0];
+        GridLaserArgs la;
+        la.amp = Lz.a0 * Lz.kL; la.waist = Lz.waist; la.zR = 0.5 * Lz.kL * Lz.waist * Lz.waist; la.zfoc = Lz.z_focus;
+        la.zc = Lz.zeta_c; la.tau = Lz.tau; la.kL = Lz.kL; la.cep = Lz.cep;
+        la.rc = std::cos(Lz.pol_angle); la.rs = std::sin(Lz.pol_angle); la.dx = dx; la.dy = dy; la.X0 = X0; la.Y0 = Y0;
+        la.circular = Lz.circular; la.helicity = Lz.helicity; la.nx = cfg.nx; la.ny = cfg.ny; la.nk = nk;
+        la.zeta = zs.data(); la.yn = nullptr;
+        std::vector<double> e(static_cast<size_t>(N));
+        for (int n = 0; n < N; ++n) e[size_t(n)] = hat_sum(denv, nenv, Lz.env_z0, Lz.env_dz, cfg.zeta[n]);
+        DevBuf<double> envd(static_cast<size_t>(N));
+        envd.from_host(e.data(), 0, size_t(N));
+        la.env = envd.data();
+        grid_laser_fill(la, T3.data(), a, ns);
+        fft_run(fft, T3.data(), ns, false);
+        grid_laser_add(g, T3.data(), tR[0].data(), tR[1].data(), ns, 0);
+    }
+    march_all(tmview(0.0), a, b);
+    sponge_tlm();
+
+    CurrentsAdj ca{};
+    ca.g = g; ca.hcg = hcg.data(); ca.zf = cfg.zeta_filter; ca.zu = zax.uniform; ca.keep0 = cfg.box_mean;
+    ca.d.zax = zax; ca.d.yax = yax; ca.d.hcg = hcg.data(); ca.d.ia = iA.data(); ca.d.id = iD.data(); ca.d.J = nullptr;
+    ca.d.dx = dx; ca.d.dy = dy; ca.d.nh = ns + 3; ca.d.a = a; ca.d.N = N; ca.d.nx = cfg.nx; ca.d.ny = cfg.ny;
+    ca.d.yf = 1; ca.d.nk = nk;
+    std::unordered_map<unsigned long long, std::array<double, 6>> carry;   // x y zeta px py pz by id
+    StepTape t;
+    for (int k = 0; k < steps; ++k) {
+        t.s0 = s;
+        tape_beam(t, true);
+        if (t.F0.size() != F.size()) t.F0.resize(F.size());
+        d2d(t.F0.data(), F.data(), F.size() * sizeof(float));
+        tape_ = &t;
+        try { step(); } catch (...) { tape_ = nullptr; throw; }
+        tape_ = nullptr;
+        if (restreamed_total) throw std::runtime_error("gradient: beam particles were re-streamed");
+        const long np_step = t.np;
+        // the beam push along the tangent, in the fields before the step
+        planes_to_double(t.F0.data(), Fsz, Fd.data(), Fsz, Fsz, 1);
+        Parts b0 = beam.view();
+        {
+            double* d0 = t.beam0.data();
+            b0.x = d0; b0.y = d0 + np_step; b0.zeta = d0 + 2 * np_step; b0.px = d0 + 3 * np_step;
+            b0.py = d0 + 4 * np_step; b0.pz = d0 + 5 * np_step; b0.z = d0 + 6 * np_step; b0.rem = d0 + 7 * np_step;
+            b0.alive = t.alive0.data(); b0.qm = t.qm.data();
+        }
+        g3d_bpush_ctx bc{};
+        bc.nx = cfg.nx; bc.ny = cfg.ny; bc.nsl = ns + 2; bc.dx = dx; bc.dy = dy; bc.zeta0 = cfg.zeta[0]; bc.ihz = 1 / h;
+        bc.F = Fd.data();
+        bc.laser_on = laser.on; bc.laser_pol = laser.pol; bc.a0 = laser.a0; bc.kL = laser.kL; bc.tau = laser.tau;
+        bc.cep = laser.cep; bc.eta_c = laser.eta_c; bc.kx = laser.kx; bc.kz = laser.kz;
+        bc.X0 = X0; bc.zc = cfg.zeta_center; bc.s0 = t.s0; bc.s1 = t.s0 + cfg.dz; bc.dz_push = cfg.dz_push;
+        bc.push_phase = cfg.push_phase; bc.hmin = hmin; bc.zhand = -1e300; bc.zmax = cfg.zeta[N - 1];
+        bc.ylo = -1e300; bc.yhi = 1e300; bc.vz_min = cfg.vz_min;
+        bc.x = b0.x; bc.y = b0.y; bc.zeta = b0.zeta; bc.px = b0.px; bc.py = b0.py; bc.pz = b0.pz; bc.z = b0.z;
+        bc.rem = b0.rem; bc.qm = b0.qm; bc.alive = b0.alive; bc.np = np_step;
+        bunch_push_tlm(bc, tF.data(), tW.data(), cap);
+        // the beam currents of the pushed beam
+        Parts b1 = beam.view();
+        {
+            double* d1 = t.beam1.data();
+            b1.x = d1; b1.y = d1 + np_step; b1.zeta = d1 + 2 * np_step; b1.px = d1 + 3 * np_step;
+            b1.py = d1 + 4 * np_step; b1.pz = d1 + 5 * np_step; b1.z = d1 + 6 * np_step; b1.rem = d1 + 7 * np_step;
+            b1.alive = t.alive1.data(); b1.qm = t.qm.data();
+        }
+        tjw.zero();
+        double* W = tW.data();
+        deposit_tlm(ca, b1, np_step, W, W + cap, W + 3 * cap, W + 4 * cap, W + 5 * cap, tjw.data());
+        currents_zfilter(g, tjw.data(), hcg.data(), cfg.zeta_filter, zax.uniform, S1.data(), tjk[2].data());
+        fft_run(fft, S1.data(), ns + 1, false);
+        fft_run(fft, tjk[2].data(), ns + 1, false);
+        currents_unpack(g, S1.data(), tjk[0].data(), tjk[1].data(), tjk[2].data(), cfg.plasma_on && cfg.box_mean);
+        // the march of the plasma
+        const long ns_ = t.nstrm;
+        if (tP.size() != 7 * size_t(ns_)) tP.resize(7 * size_t(ns_));
+        std::vector<unsigned long long> ids(static_cast<size_t>(ns_));
+        std::vector<int> alive0(static_cast<size_t>(ns_));
+        t.ids.to_host(ids.data(), 0, size_t(ns_));
+        t.pa.to_host(alive0.data(), size_t(N - 2) * size_t(ns_), size_t(ns_));
+        {
+            std::vector<double> tp(7 * size_t(ns_), 0.0);
+            const int slot[6] = {0, 1, 2, 4, 5, 6};
+            for (long i = 0; i < ns_; ++i) {
+                if (alive0[size_t(i)] != 6) continue;
+                const auto it = carry.find(ids[size_t(i)]);
+                if (it == carry.end()) continue;
+                for (int c6 = 0; c6 < 6; ++c6) tp[size_t(slot[c6]) * ns_ + i] = it->second[size_t(c6)];
+            }
+            tP.from_host(tp.data(), 0, tp.size());
+        }
+        tpin.zero(); tsin.zero(); tsi0.zero(); tdl.zero();
+        ModeArgs m{};
+        m.mv = tmview(D);
+        m.sin_ = tsin.data(); m.si0 = tsi0.data(); m.pin = tpin.data(); m.dl = tdl.data();
+        m.js0x = tjs0[0].data(); m.js0y = tjs0[1].data(); m.js0z = tjs0[2].data();
+        m.jkx = tjk[0].data(); m.jky = tjk[1].data(); m.jkz = tjk[2].data();
+        m.kxs = kxs.data(); m.kys = kys.data(); m.filt = filt.data(); m.filty = filty.data();
+        m.a2 = cfg.smooth_length * cfg.smooth_length; m.extrap = cfg.predictor_extrapolate; m.keep0 = cfg.box_mean;
+        m.ny = cfg.ny; m.kf = cfg.k_filter;
+        const double sc = 1.0 / (double(cfg.nx) * cfg.ny);
+        auto real_fields_t = [&](double* dst, long stride) {
+            fft_rows(fft, tT.data(), 3L * cfg.ny, true, sc);
+            fields_to_F_double(tT.data(), nk, dst, stride, nk, 1);
+        };
+        // front: slice N-1
+        m.n = N - 1; m.first = 1; m.last = 1; m.top = 1; m.wex = 0; m.qcorr = 0;
+        m.bRx = m.bRy = m.bEz = m.bBz = nullptr;
+        pass_modes(m, fft, tX.data(), tT.data(), nullptr, 1);
+        real_fields_t(tFm.data() + long(N - 1 - a) * nk, fpl);
+        g3d_spush_ctx c{};
+        c.nx = cfg.nx; c.ny = cfg.ny; c.dx = dx; c.dy = dy; c.zeta0 = cfg.zeta[0]; c.ihz = 1 / h;
+        c.Delta = cfg.dz; c.s1 = t.s0 + cfg.dz; c.sTrap = c.s1 + (cfg.trap_advance - 1) * cfg.dz;
+        c.vz_beam = cfg.vz_beam; c.ia = ia0; c.F0 = F0d.data(); c.F1 = F1d.data(); c.J = nullptr;
+        c.q = t.pq.data(); c.qm = t.pq.data() + ns_; c.w = t.pq.data() + 2 * ns_; c.np = ns_;
+        for (int n = N - 2; n >= a; --n) {
+            double* st = t.ps.data() + size_t(n) * 7 * size_t(ns_);
+            c.n = n; c.zn = cfg.zeta[n]; c.zn1 = cfg.zeta[n + 1];
+            c.x = st; c.y = st + ns_; c.zeta = st + 2 * ns_; c.z = st + 3 * ns_; c.px = st + 4 * ns_;
+            c.py = st + 5 * ns_; c.pz = st + 6 * ns_; c.alive = t.pa.data() + size_t(n) * ns_;
+            for (int p = 0; p < P; ++p) {
+                const bool last = p == P - 1;
+                const float* f0 = p == 0 ? t.F0.data() + long(n - a) * nk
+                                         : t.Fmid.data() + size_t(p - 1) * size_t(Fsz) + long(n - a) * nk;
+                planes_to_double(f0, fpl, F0d.data(), nk, nk, 6);
+                planes_to_double(t.Fmarch.data() + long(n + 1 - a) * nk, fpl, F1d.data(), nk, nk, 6);
+                if (p == 0) {
+                    tFmid.zero();
+                    add_planes(tFmid.data(), nk, tF.data() + long(n - a) * nk, fpl, nk, 6);
+                }
+                dF1.zero();
+                add_planes(dF1.data(), nk, tFm.data() + long(n + 1 - a) * nk, fpl, nk, 6);
+                tJ.zero();
+                stream_push_tlm(c, tFmid.data(), dF1.data(), tP.data(), tJ.data(), last);
+                pass_rows_fwd(tJ.data(), TileGeom{}, fft, tX.data(), nullptr, nullptr, 0, 0, 0);
+                m.n = n; m.first = p == 0; m.last = last; m.top = 0;
+                m.wex = (n >= 1 && n + 1 <= N - 2 && n != N - 2) ? (cen[size_t(n)] - cen[size_t(n - 1)]) / (cen[size_t(n + 1)] - cen[size_t(n)]) : 0.0;
+                m.qcorr = D * wlo_h[size_t(n + 1)];
+                // several passes: a pass before the last leaves slice n at s0 (the kernel reads this only as a flag)
+                cx* flag = P > 1 ? tX.data() : nullptr;
+                m.bRx = m.bRy = m.bEz = m.bBz = flag;
+                pass_modes(m, fft, tX.data(), tT.data(), nullptr, 0);
+                if (last) real_fields_t(tFm.data() + long(n - a) * nk, fpl);
+                else real_fields_t(tFmid.data(), nk);
+            }
+        }
+        axpy_planes(tjk[2].data(), tjs0[2].data(), -1, pj);   // jk_z -= js0_z after the slices
+        // the electrons held over to the next step and those trapped into the beam keep their tangents
+        {
+            std::vector<double> tp(7 * size_t(ns_));
+            tP.to_host(tp.data(), 0, tp.size());
+            carry.clear();
+            const int slot[6] = {0, 1, 2, 4, 5, 6};
+            for (long i = 0; i < ns_; ++i) {
+                std::array<double, 6>& v = carry[ids[size_t(i)]];
+                for (int c6 = 0; c6 < 6; ++c6) v[size_t(c6)] = tp[size_t(slot[c6]) * ns_ + i];
+            }
+            if (!t.added.empty()) {
+                const long ncap = np_step + long(t.added.size());
+                std::vector<double> w(6 * size_t(cap)), w2(6 * size_t(ncap), 0.0);
+                tW.to_host(w.data(), 0, w.size());
+                for (int c6 = 0; c6 < 6; ++c6) {
+                    for (long i = 0; i < np_step; ++i) w2[size_t(c6) * ncap + size_t(i)] = w[size_t(c6) * cap + size_t(i)];
+                    for (size_t q = 0; q < t.added.size(); ++q) w2[size_t(c6) * ncap + size_t(np_step) + q] = carry[t.added[q]][size_t(c6)];
+                }
+                cap = ncap;
+                tW.resize(6 * size_t(cap));
+                tW.from_host(w2.data(), 0, w2.size());
+            }
+        }
+        if (np != cap) throw std::runtime_error("tangent: the beam list changed other than by trapped electrons");
+        sponge_tlm();
+    }
+    // the loss and its derivative along the direction
+    const std::vector<g3d_particle> ps = host_particles();
+    std::vector<double> tpz(static_cast<size_t>(cap));
+    tW.to_host(tpz.data(), 5 * size_t(cap), size_t(cap));
+    double sum = 0, dsum = 0;
+    long cnt = 0;
+    for (long i = 0; i < np; ++i)
+        if (ps[size_t(i)].alive == 1 && ps[size_t(i)].sp == 0) { sum += ps[size_t(i)].pz; dsum += tpz[size_t(i)]; ++cnt; }
+    if (!cnt) throw std::runtime_error("tangent: no live bunch particle at the end");
+    *loss = sum / double(cnt);
+    *dloss = dsum / double(cnt);
     hip_check(hipStreamSynchronize(stream()), "synchronize");
 }
 

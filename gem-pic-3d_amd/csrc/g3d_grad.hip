@@ -311,6 +311,51 @@ __global__ void k_dep_adj(CurrentsAdj c, Parts P, long np, const double* ljw, do
     lpz[ip] -= f * (sx * px + sy * py) / (pz * pz);
 }
 
+// the same deposit's tangent (the tangent of zeta moves no current: held fixed)
+__global__ void k_dep_tlm(CurrentsAdj c, Parts P, long np, const double* tx, const double* ty, const double* tpx,
+                          const double* tpy, const double* tpz, double* djw) {
+    const long ip = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (ip >= np || P.alive[ip] != 1) return;
+    const DepositArgs& d = c.d;
+    int m; double fz;
+    d.zax.locate(P.zeta[ip], m, fz);
+    const int hw = m - (d.a - 2);
+    if (hw < 0 || hw >= d.nh || m < 0 || m > d.N - 2) return;
+    const double px = P.px[ip], py = P.py[ip], pz = P.pz[ip];
+    const double f = P.q[ip] * P.w[ip] / (d.hcg[m + 2] * d.dx) * d.ia[0];
+    int ix[3], jn[3]; double wx[3], wn[3];
+    tsc_periodic(P.x[ip], d.dx, d.nx, ix, wx);
+    tsc_periodic(P.y[ip], d.dy, d.ny, jn, wn);
+    const double xi = P.x[ip] / d.dx, fx = xi - floor(xi + 0.5);
+    const double yi = P.y[ip] / d.dy, fy = yi - floor(yi + 0.5);
+    const double dwx[3] = {-(0.5 - fx) / d.dx, -2 * fx / d.dx, (0.5 + fx) / d.dx};
+    const double dwn[3] = {-(0.5 - fy) / d.dy, -2 * fy / d.dy, (0.5 + fy) / d.dy};
+    const double ax = px / pz, ay = py / pz;
+    const double dax = tpx[ip] / pz - px * tpz[ip] / (pz * pz), day = tpy[ip] / pz - py * tpz[ip] / (pz * pz);
+    const long nk = d.nk, pl = long(d.nh) * nk, base = long(hw) * nk;
+    for (int v = 0; v < 3; ++v)
+        for (int u = 0; u < 3; ++u) {
+            const long k = base + long(jn[v]) * d.nx + ix[u];
+            const double W = f * wx[u] * wn[v];
+            const double dW = f * (dwx[u] * wn[v] * tx[ip] + wx[u] * dwn[v] * ty[ip]);
+            atomicAdd(&djw[k], dW * ax + W * dax);
+            atomicAdd(&djw[pl + k], dW * ay + W * day);
+            atomicAdd(&djw[2 * pl + k], dW);
+        }
+}
+__global__ void k_to_F_d(const cx* T, long ts, double* F, long fpl, long nk, long tot) {
+    const long idx = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= tot) return;
+    const cx R = T[idx], L = T[ts + idx], Z = T[2 * ts + idx];
+    F[0 * fpl + idx] = 0.5 * (R.re + L.re);
+    F[1 * fpl + idx] = 0.5 * (R.im + L.im);
+    F[3 * fpl + idx] = 0.5 * (L.im - R.im);
+    F[4 * fpl + idx] = 0.5 * (R.re - L.re);
+    F[2 * fpl + idx] = Z.re;
+    F[5 * fpl + idx] = Z.im;
+    (void)nk;
+}
+
 __global__ void k_seed_pz(Parts P, long np, int sp0, double inv, double* lpz) {
     const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= np) return;
@@ -391,6 +436,13 @@ void currents_adj(const CurrentsAdj& c, const FftPlan& p, Parts P, long np, doub
     G3D_GLAUNCH(k_dep_adj, np, c, P, np, ljw, lx, ly, lpx, lpy, lpz);
 }
 
+void deposit_tlm(const CurrentsAdj& c, Parts P, long np, const double* dx, const double* dy, const double* dpx,
+                 const double* dpy, const double* dpz, double* djw) {
+    G3D_GLAUNCH(k_dep_tlm, np, c, P, np, dx, dy, dpx, dpy, dpz, djw);
+}
+void fields_to_F_double(const cx* T, long ts, double* F, long fpl, long nk, int count) {
+    G3D_GLAUNCH(k_to_F_d, long(count) * nk, T, ts, F, fpl, nk, long(count) * nk);
+}
 void seed_mean_pz(Parts P, long np, int sp0, double inv_count, double* lpz) {
     G3D_GLAUNCH(k_seed_pz, np, P, np, sp0, inv_count, lpz);
 }

@@ -64,4 +64,15 @@ void seed_mean_pz(Parts P, long np, int sp0, double inv_count, double* lpz);
 // per slice l < count: out[l] += Re sum_i conj(lRx) X + conj(lRy) Y, (X, Y) the unpacked pair of T (grid laser)
 void laser_projection(const SpecGeom& g, const cx* T, const cx* lRx, const cx* lRy, int count, double* out);
 
+// ---------------------------------------------------------------- the tangent-linear march (Sim::tangent)
+// the streaming push of one pass on the tangents (g3d_spush.hip): st ([7][np]) in, and out with write; dJ added to
+void stream_push_tlm(const g3d_spush_ctx& c, const double* dF0, const double* dF1, double* st, double* dJ, int write);
+// the beam push on the tangents (g3d_bpush.hip): dst ([6][stride], x y zeta px py pz) in and out; dF in F's layout
+void bunch_push_tlm(const g3d_bpush_ctx& c, const double* dF, double* dst, long stride);
+// the tangent of the beam deposit (jw's layout, added to) from the tangents of x, y, p_x, p_y, p_z (zeta: none)
+void deposit_tlm(const CurrentsAdj& c, Parts P, long np, const double* dx, const double* dy, const double* dpx,
+                 const double* dpy, const double* dpz, double* djw);
+// E, B of count slices from the packed planes T (ts apart) into the double planes F (fpl apart)
+void fields_to_F_double(const cx* T, long ts, double* F, long fpl, long nk, int count);
+
 }  // namespace g3d

@@ -25,6 +25,9 @@ using Dn = Dual<kNd>;
 // a periodic coordinate moved into [0, L): the same shift on the value and no change of its derivatives
 __device__ __forceinline__ double twrap(double x, double L) { return wrap_mod(x, L); }
 __device__ __forceinline__ Dn twrap(const Dn& x, double L) { return x + (wrap_mod(x.v, L) - x.v); }
+using D1 = Dual<1>;                 // one direction: the tangent-linear push
+__device__ __forceinline__ D1 dt1(double v, double d) { D1 r; r.v = v; r.d[0] = d; return r; }
+__device__ __forceinline__ D1 twrap(const D1& x, double L) { return x + (wrap_mod(x.v, L) - x.v); }
 
 // TSC weights of x on a periodic uniform axis (tsc_periodic: the index coordinate x / d) and their derivatives
 // with respect to x
@@ -233,7 +236,59 @@ __global__ void __launch_bounds__(kT) k_spush_vjp(g3d_spush_ctx c, g3d_spush_adj
         }
     }
 }
+
+// The tangent-linear push: the primal state and fields as k_spush (the state arrays are read, not written), the
+// tangents of the state (st: x y zeta z px py pz, [7][np]) and of the two field planes (dF0, dF1) carried through
+// it on dual numbers of one direction; the tangent of the deposit is added to dJ, and with `write` the pushed
+// tangents replace st.
+
+__global__ void __launch_bounds__(kT) k_spush_tlm(g3d_spush_ctx c, const double* dF0, const double* dF1, double* st,
+                                                  double* dJ, int write) {
+    const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= c.np) return;
+    const long nk = long(c.nx) * c.ny, np = c.np;
+    SP<D1> p{dt1(c.x[i], st[i]), dt1(c.y[i], st[np + i]), dt1(c.zeta[i], st[2 * np + i]), dt1(c.z[i], st[3 * np + i]),
+             dt1(c.px[i], st[4 * np + i]), dt1(c.py[i], st[5 * np + i]), dt1(c.pz[i], st[6 * np + i])};
+    int alive = c.alive[i];
+    auto gather = [&](const D1& xg, const D1& yg, const D1& zm, D1 E[3], D1 B[3]) {
+        const GatherPoint g = gather_point(c, xg.v, yg.v, zm.v);
+        for (int comp = 0; comp < 6; ++comp) {
+            double gx, gy, gzeta;
+            const double s = gather_comp(c, g, comp, &gx, &gy, &gzeta);
+            double ds = gx * xg.d[0] + gy * yg.d[0] + gzeta * zm.d[0];
+            for (int t = 0; t < 9; ++t) ds += g.w[t] * (g.gz * dF0[comp * nk + g.k[t]] + g.fz * dF1[comp * nk + g.k[t]]);
+            if (comp < 3) E[comp] = dt1(s, ds); else B[comp - 3] = dt1(s, ds);
+        }
+    };
+    auto deposit = [&](const D1& x, const D1& y, const D1& f, const D1& vx, const D1& vy, const D1& vz) {
+        int ix[3], jn[3]; double wx[3], wn[3], dwx[3], dwn[3];
+        tsc_wd(x.v, c.dx, c.nx, ix, wx, dwx);
+        tsc_wd(y.v, c.dy, c.ny, jn, wn, dwn);
+        const int cq = f.v < 0 ? 3 : 4;
+        for (int v = 0; v < 3; ++v)
+            for (int u = 0; u < 3; ++u) {
+                const long k = long(jn[v]) * c.nx + ix[u];
+                const D1 W = dt1(wx[u] * wn[v], dwx[u] * wn[v] * x.d[0] + wx[u] * dwn[v] * y.d[0]);
+                const D1 ww = f * W * c.ia;
+                atomicAdd(&dJ[k], (ww * vx).d[0]);
+                atomicAdd(&dJ[nk + k], (ww * vy).d[0]);
+                atomicAdd(&dJ[2 * nk + k], (ww * vz).d[0]);
+                atomicAdd(&dJ[cq * nk + k], ww.d[0]);
+            }
+    };
+    tpush<D1>(c, p, c.q[i], c.qm[i], c.w[i], alive, gather, deposit);
+    if (write) {
+        st[i] = p.x.d[0]; st[np + i] = p.y.d[0]; st[2 * np + i] = p.zeta.d[0]; st[3 * np + i] = p.z.d[0];
+        st[4 * np + i] = p.px.d[0]; st[5 * np + i] = p.py.d[0]; st[6 * np + i] = p.pz.d[0];
+    }
+}
 }  // namespace
+
+void stream_push_tlm(const g3d_spush_ctx& c, const double* dF0, const double* dF1, double* st, double* dJ, int write) {
+    if (c.np <= 0) return;
+    hipLaunchKernelGGL(k_spush_tlm, dim3(blocks_for(c.np, kT)), dim3(kT), 0, stream(), c, dF0, dF1, st, dJ, write);
+    launch_check("stream push tangent");
+}
 }  // namespace g3d
 
 using namespace g3d;

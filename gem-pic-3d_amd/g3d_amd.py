@@ -329,6 +329,8 @@ def _load():
         "g3d_sim_read_checkpoint": ([P, C.c_char_p, C.c_long], C.c_int),
         "g3d_sim_gradient": ([P, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int,
                               C.POINTER(C.c_double), C.c_int], C.c_int),
+        "g3d_sim_tangent": ([P, C.c_int, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double), C.c_int,
+                             C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
         "g3d_sim_timers": ([P, C.POINTER(C.c_double), C.POINTER(C.c_long), C.c_int, C.c_char_p, C.c_int], C.c_int),
     }
     for name, (args, res) in sig.items():
@@ -483,6 +485,19 @@ class Sim(SimBase):
         check(lib().g3d_sim_gradient(self.h, int(steps), int(every), C.byref(loss), genv.ctypes.data_as(dp), nenv,
                                      gprof.ctypes.data_as(dp), nprof))
         return loss.value, genv[:nenv], gprof[:nprof]
+
+    def tangent(self, steps, denv, dprof):
+        """On a new simulation: the loss of gradient() and its derivative along (denv, dprof) by the tangent-linear
+        run (g3d_sim_tangent).  Returns (loss, dloss)."""
+        denv = np.ascontiguousarray(denv, dtype=np.float64)
+        dprof = np.ascontiguousarray(dprof, dtype=np.float64)
+        loss, dloss = C.c_double(), C.c_double()
+        dp = C.POINTER(C.c_double)
+        e = denv if denv.size else np.zeros(1)
+        p = dprof if dprof.size else np.zeros(1)
+        check(lib().g3d_sim_tangent(self.h, int(steps), e.ctypes.data_as(dp), int(denv.size), p.ctypes.data_as(dp),
+                                    int(dprof.size), C.byref(loss), C.byref(dloss)))
+        return loss.value, dloss.value
 
     FAMILIES = ("stream_halfcell", "pass_rows_fwd", "pass_cols", "pass_rows_inv", "unused", "beam_push", "currents",
                 "sponge_real_fields")

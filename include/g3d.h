@@ -460,6 +460,13 @@ int g3d_sim_read_checkpoint(g3d_sim* sim, const char* dir, long step);
  * fixed window; a run in which beam particles are re-streamed is refused. */
 int g3d_sim_gradient(g3d_sim* sim, int steps, int every, double* loss, double* genv, int nenv, double* gprof,
                      int nprof);
+/* The tangent-linear run of the same march (no counterpart in the reference): on a new simulation, the loss of
+ * g3d_sim_gradient and its derivative along the direction (denv, dprof) of the envelope and profile values, by
+ * carrying the tangents of the state forward through init() and the steps beside the primal ones (the forward's
+ * field kernels on the tangent arrays, the pushes and the beam deposit on dual numbers).  The same coverage as
+ * g3d_sim_gradient; <gradient, direction> equals *dloss. */
+int g3d_sim_tangent(g3d_sim* sim, int steps, const double* denv, int nenv, const double* dprof, int nprof,
+                    double* loss, double* dloss);
 /* HIP event timing of the kernel families: bit f of mask times family f (-1: all, 0: off); clears the sums.
  * g3d_sim_timers reads the accumulated milliseconds and launch counts with the family names. */
 int g3d_sim_timing(g3d_sim* sim, int mask);

@@ -7,7 +7,8 @@ fields agree at the bars of the FP64 form.
 
 The gradient of the config-4 loss (the mean p_z of the live witness after the steps) in its 32 envelope and 16
 profile values by the reverse of the march (Sim.gradient, g3d_sim_gradient): against central differences of the
-engine's loss and of the reference's, the same with checkpoints every step and every second step, the FP32-mixed
+engine's loss and of the reference's, against the tangent-linear run of the whole march, the same with checkpoints
+every step and every second step, the FP32-mixed
 form's against the FP64 form's, through torch.autograd (g3d_autograd.WitnessEnergy), and at the configuration's full
 size."""
 import numpy as np
@@ -135,6 +136,26 @@ def test_config4_gradient_against_differences():
         assert g != 0 and best <= fd_bar(g), (which, k, g, fds)
 
 
+def test_config4_tangent_linear():
+    """the tangent-linear run of the whole march (Sim.tangent: the tangents carried forward beside the primal state)
+    along one envelope value, one profile value and a random direction equals the adjoint gradient's projection: the
+    reverse is the transpose of the linearised march to rounding"""
+    env0, prof0 = config4_params()
+    s = g3d_amd.Sim(medium(env0, prof0))
+    loss, genv, gprof = s.gradient(2, every=1)
+    s.close()
+    rng = np.random.default_rng(7)
+    for name, de, dp in (("env[12]", np.eye(32)[12], np.zeros(16)), ("prof[7]", np.zeros(32), np.eye(16)[7]),
+                         ("random", rng.standard_normal(32), rng.standard_normal(16))):
+        s = g3d_amd.Sim(medium(env0, prof0))
+        l2, dl = s.tangent(2, de, dp)
+        s.close()
+        ref = float(genv @ de + gprof @ dp)
+        print("tangent-linear along %s: %.15e, adjoint %.15e" % (name, dl, ref), flush=True)
+        assert abs(l2 - loss) <= 1e-12 * abs(loss)
+        assert abs(dl - ref) <= 1e-12 * abs(ref) + 1e-15, (name, dl, ref)
+
+
 def test_config4_mixed_gradient():
     """the gradient of the FP32-mixed form (its forward, the FP64 reverse at its trajectory) against the FP64 form's:
     the two losses agree to 1e-8 and so do their derivatives, to 1e-5 of the largest (central differences of the
@@ -192,8 +213,10 @@ def small_deck(env, prof):
 
 def test_config4_full_gradient():
     """config 4 at its size (128 x 128 modes, 512 slices, 2^18 witness electrons, 20 steps): the gradient in all 48
-    values with a checkpoint every 5 steps and with one every 20 agree; central differences (1e-4) in two values agree
-    to the loss's own noise at this size (2 %: the steps 1e-4 ... 1e-6 scatter by that much)"""
+    values with a checkpoint every 5 steps and with one every 20 agree; the tangent-linear run along a random direction
+    equals the gradient's projection (1e-7: two runs' trajectories differ in the last bits, the plasma deposit adding
+    with atomics); central differences (1e-4) in two values agree to the loss's own noise at this size (2 %: the steps
+    1e-4 ... 1e-6 scatter by that much)"""
     env0, prof0 = config4_params()
     out = []
     for every in (5, 20):
@@ -209,6 +232,14 @@ def test_config4_full_gradient():
           % (abs(l20 - l5) / abs(l5), de, dp), flush=True)
     assert abs(l20 - l5) <= 1e-10 * abs(l5)
     assert de <= 1e-6 and dp <= 1e-6
+    rng = np.random.default_rng(11)
+    de, dp = rng.standard_normal(32), rng.standard_normal(16)
+    s = g3d_amd.Sim(config4_deck(env0, prof0))
+    _, dl = s.tangent(20, de, dp)
+    s.close()
+    ref = float(e5 @ de + p5 @ dp)
+    print("config 4: tangent-linear along a random direction %.12e, adjoint %.12e" % (dl, ref), flush=True)
+    assert abs(dl - ref) <= 1e-7 * abs(ref)
     for which, k in (("env", 5), ("prof", 3)):
         g = (e5 if which == "env" else p5)[k]
         f = central(g3d_amd.Sim, lambda e, p: config4_deck(e, p), which, k, 1e-4, steps=20)
