@@ -914,18 +914,23 @@ void Sim::relax_start_field() {
     hip_check(hipStreamSynchronize(stream()), "synchronize");
 }
 
+GridLaserArgs Sim::grid_laser_args(const g3d_glaser& Lz) {
+    GridLaserArgs la;
+    la.amp = Lz.a0 * Lz.kL; la.waist = Lz.waist; la.zR = 0.5 * Lz.kL * Lz.waist * Lz.waist; la.zfoc = Lz.z_focus;
+    la.zc = Lz.zeta_c; la.tau = Lz.tau; la.kL = Lz.kL; la.cep = Lz.cep;
+    la.rc = std::cos(Lz.pol_angle); la.rs = std::sin(Lz.pol_angle); la.dx = dx; la.dy = dy; la.X0 = X0; la.Y0 = Y0;
+    la.circular = Lz.circular; la.helicity = Lz.helicity; la.nx = cfg.nx; la.ny = cfg.ny; la.nk = nk;
+    la.zeta = zs.data(); la.yn = yf ? nullptr : yn.data();
+    la.env = nullptr;
+    return la;
+}
+
 void Sim::add_grid_lasers() {
     ensure_layout(false, false);
     if (cfg.glasers.empty()) return;
     const SpecGeom g = geom();
     for (const auto& Lz : cfg.glasers) {
-        GridLaserArgs la;
-        la.amp = Lz.a0 * Lz.kL; la.waist = Lz.waist; la.zR = 0.5 * Lz.kL * Lz.waist * Lz.waist; la.zfoc = Lz.z_focus;
-        la.zc = Lz.zeta_c; la.tau = Lz.tau; la.kL = Lz.kL; la.cep = Lz.cep;
-        la.rc = std::cos(Lz.pol_angle); la.rs = std::sin(Lz.pol_angle); la.dx = dx; la.dy = dy; la.X0 = X0; la.Y0 = Y0;
-        la.circular = Lz.circular; la.helicity = Lz.helicity; la.nx = cfg.nx; la.ny = cfg.ny; la.nk = nk;
-        la.zeta = zs.data(); la.yn = yf ? nullptr : yn.data();
-        la.env = nullptr;
+        GridLaserArgs la = grid_laser_args(Lz);
         DevBuf<double> envd;
         if (Lz.n_env > 0) {          // envelope_function: a(zeta) of each slice (src/sim_fields.cpp:335)
             std::vector<double> e(static_cast<size_t>(N));

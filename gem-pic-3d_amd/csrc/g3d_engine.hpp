@@ -109,6 +109,7 @@ public:
     void comoving_field();
     void relax_start_field();
     void add_grid_lasers();
+    GridLaserArgs grid_laser_args(const g3d_glaser& Lz);     // the fill of one grid laser (env: none)
     void load_plasma(double s0, double s1);
     void stream_spectra();
     void march_slice(int n, double D);

@@ -75,6 +75,72 @@ struct Adjoints {
     DevBuf<double> lP, lPacc, gP, zeros;   // streaming particles: x y zeta z px py pz, [7][nstrm]
 };
 
+// the beam of a step from its tape: the state before (after = false) or after the push, with the constants (q, w,
+// species, ids) of the live beam, whose first t.np particles are the step's
+Parts tape_beam_parts(Sim& sm, StepTape& t, bool after) {
+    Parts p = sm.beam.view();
+    double* d = after ? t.beam1.data() : t.beam0.data();
+    p.x = d; p.y = d + t.np; p.zeta = d + 2 * t.np; p.px = d + 3 * t.np; p.py = d + 4 * t.np;
+    p.pz = d + 5 * t.np; p.z = d + 6 * t.np; p.rem = d + 7 * t.np;
+    p.alive = after ? t.alive1.data() : t.alive0.data();
+    p.qm = t.qm.data();
+    return p;
+}
+// the beam push of a step (Sim::push) as the context of its reverse and tangent: the state before the push, the
+// fields before the step in FP64 (Fd, F's layout)
+g3d_bpush_ctx beam_push_ctx(Sim& sm, StepTape& t, double* Fd, double hmin) {
+    const Config& cfg = sm.cfg;
+    const Parts b0 = tape_beam_parts(sm, t, false);
+    g3d_bpush_ctx bc{};
+    bc.nx = cfg.nx; bc.ny = cfg.ny; bc.nsl = sm.ns + 2; bc.dx = sm.dx; bc.dy = sm.dy; bc.zeta0 = cfg.zeta[0];
+    bc.ihz = 1 / (cfg.zeta[1] - cfg.zeta[0]);
+    bc.F = Fd;
+    bc.laser_on = sm.laser.on; bc.laser_pol = sm.laser.pol; bc.a0 = sm.laser.a0; bc.kL = sm.laser.kL;
+    bc.tau = sm.laser.tau; bc.cep = sm.laser.cep; bc.eta_c = sm.laser.eta_c; bc.kx = sm.laser.kx; bc.kz = sm.laser.kz;
+    bc.X0 = sm.X0; bc.zc = cfg.zeta_center; bc.s0 = t.s0; bc.s1 = t.s0 + cfg.dz; bc.dz_push = cfg.dz_push;
+    bc.push_phase = cfg.push_phase; bc.hmin = hmin; bc.zhand = -1e300; bc.zmax = cfg.zeta[sm.N - 1];
+    bc.ylo = -1e300; bc.yhi = 1e300; bc.vz_min = cfg.vz_min;
+    bc.x = b0.x; bc.y = b0.y; bc.zeta = b0.zeta; bc.px = b0.px; bc.py = b0.py; bc.pz = b0.pz; bc.z = b0.z;
+    bc.rem = b0.rem; bc.qm = b0.qm; bc.alive = b0.alive; bc.np = t.np;
+    return bc;
+}
+// the streaming push of a step's half-cells (k_stream_halfcell) as the context of its reverse and tangent: the
+// constants; the half-cell, its particles and the fields are set per pass
+g3d_spush_ctx stream_push_ctx(Sim& sm, StepTape& t, double ia0, double* F0d, double* F1d) {
+    const Config& cfg = sm.cfg;
+    const long ns_ = t.nstrm;
+    g3d_spush_ctx c{};
+    c.nx = cfg.nx; c.ny = cfg.ny; c.dx = sm.dx; c.dy = sm.dy; c.zeta0 = cfg.zeta[0];
+    c.ihz = 1 / (cfg.zeta[1] - cfg.zeta[0]);
+    c.Delta = cfg.dz; c.s1 = t.s0 + cfg.dz; c.sTrap = c.s1 + (cfg.trap_advance - 1) * cfg.dz;
+    c.vz_beam = cfg.vz_beam; c.ia = ia0; c.F0 = F0d; c.F1 = F1d; c.J = nullptr;
+    c.q = t.pq.data(); c.qm = t.pq.data() + ns_; c.w = t.pq.data() + 2 * ns_; c.np = ns_;
+    return c;
+}
+// half-cell n of the tape in c: its slice positions, the particles before it and their codes
+void stream_push_slot(const Sim& sm, StepTape& t, int n, g3d_spush_ctx& c) {
+    const long ns_ = t.nstrm;
+    double* st = t.ps.data() + size_t(n) * 7 * size_t(ns_);
+    c.n = n; c.zn = sm.cfg.zeta[n]; c.zn1 = sm.cfg.zeta[n + 1];
+    c.x = st; c.y = st + ns_; c.zeta = st + 2 * ns_; c.z = st + 3 * ns_; c.px = st + 4 * ns_;
+    c.py = st + 5 * ns_; c.pz = st + 6 * ns_; c.alive = t.pa.data() + size_t(n) * ns_;
+}
+// the beam currents (Sim::currents) as the context of their reverse and tangent
+CurrentsAdj currents_args(Sim& sm, const SpecGeom& g) {
+    const Config& cfg = sm.cfg;
+    CurrentsAdj ca{};
+    ca.g = g; ca.hcg = sm.hcg.data(); ca.zf = cfg.zeta_filter; ca.zu = sm.zax.uniform; ca.keep0 = cfg.box_mean;
+    ca.d.zax = sm.zax; ca.d.yax = sm.yax; ca.d.hcg = sm.hcg.data(); ca.d.ia = sm.iA.data(); ca.d.id = sm.iD.data();
+    ca.d.J = nullptr; ca.d.dx = sm.dx; ca.d.dy = sm.dy; ca.d.nh = sm.ns + 3; ca.d.a = sm.a; ca.d.N = sm.N;
+    ca.d.nx = cfg.nx; ca.d.ny = cfg.ny; ca.d.yf = 1; ca.d.nk = sm.nk;
+    return ca;
+}
+// the predictor's extrapolation weight of half-cell n (march_plasma: none for the first half-cell marched)
+double pass_wex(const std::vector<double>& cen, int n, int N) {
+    return (n >= 1 && n + 1 <= N - 2 && n != N - 2) ? (cen[size_t(n)] - cen[size_t(n - 1)]) / (cen[size_t(n + 1)] - cen[size_t(n)])
+                                                     : 0.0;
+}
+
 // what the reverse and the tangent-linear run of the march cover
 void check_reversible(const Sim& sm, int steps, int every, int nenv, int nprof) {
     const Config& cfg = sm.cfg;
@@ -253,21 +319,14 @@ void Sim::gradient(int steps, int every, double* loss, double* genv, int nenv, d
             A.lP.from_host(lp.data(), 0, lp.size());
         }
         axpy_planes(A.js0[2].data(), A.jk[2].data(), -1, pj);       // jk_z -= js0_z after the slices
-        g3d_spush_ctx c{};
-        c.nx = cfg.nx; c.ny = cfg.ny; c.dx = dx; c.dy = dy; c.zeta0 = cfg.zeta[0]; c.ihz = 1 / h;
-        c.Delta = cfg.dz; c.s1 = t.s0 + cfg.dz; c.sTrap = c.s1 + (cfg.trap_advance - 1) * cfg.dz;
-        c.vz_beam = cfg.vz_beam; c.ia = ia0; c.F0 = A.F0d.data(); c.F1 = A.F1d.data(); c.J = nullptr;
-        c.q = t.pq.data(); c.qm = t.pq.data() + ns_; c.w = t.pq.data() + 2 * ns_; c.np = ns_;
+        g3d_spush_ctx c = stream_push_ctx(*this, t, ia0, A.F0d.data(), A.F1d.data());
         g3d_spush_adj ad{};
         ad.bJ = A.bJ.data(); ad.gF0 = A.gF0.data(); ad.gF1 = A.gF1.data();
         double* gp = A.gP.data();
         ad.gx = gp; ad.gy = gp + ns_; ad.gzeta = gp + 2 * ns_; ad.gz = gp + 3 * ns_; ad.gpx = gp + 4 * ns_;
         ad.gpy = gp + 5 * ns_; ad.gpz = gp + 6 * ns_;
         for (int n = a; n <= N - 2; ++n) {
-            double* st = t.ps.data() + size_t(n) * 7 * size_t(ns_);
-            c.n = n; c.zn = cfg.zeta[n]; c.zn1 = cfg.zeta[n + 1];
-            c.x = st; c.y = st + ns_; c.zeta = st + 2 * ns_; c.z = st + 3 * ns_; c.px = st + 4 * ns_;
-            c.py = st + 5 * ns_; c.pz = st + 6 * ns_; c.alive = t.pa.data() + size_t(n) * ns_;
+            stream_push_slot(*this, t, n, c);
             A.lPacc.zero();
             for (int p = P - 1; p >= 0; --p) {
                 if (p == P - 1) fields_to_F_adj(A.Fm.data() + long(n - a) * nk, fpl, nk, 1, A.lT.data(), nk);
@@ -275,7 +334,7 @@ void Sim::gradient(int steps, int every, double* loss, double* genv, int nenv, d
                 fft_run(fft, A.lT.data(), 3, false);
                 PassAdj pa = pb;
                 pa.n = n; pa.first = p == 0; pa.last = p == P - 1; pa.top = 0; pa.bk = P > 1;
-                pa.wex = (n >= 1 && n + 1 <= N - 2 && n != N - 2) ? (cen[size_t(n)] - cen[size_t(n - 1)]) / (cen[size_t(n + 1)] - cen[size_t(n)]) : 0.0;
+                pa.wex = pass_wex(cen, n, N);
                 pa.qcorr = D * wlo_h[size_t(n + 1)];
                 pass_mode_adj(pa);
                 fft_rows(fft, A.lU.data(), 3L * cfg.ny, true, 1.0);
@@ -317,22 +376,8 @@ void Sim::gradient(int steps, int every, double* loss, double* genv, int nenv, d
         pass_mode_adj(pa);
     };
 
-    CurrentsAdj ca{};
-    ca.g = g; ca.ljkx = A.jk[0].data(); ca.ljky = A.jk[1].data(); ca.ljkz = A.jk[2].data(); ca.hcg = hcg.data();
-    ca.zf = cfg.zeta_filter; ca.zu = zax.uniform; ca.keep0 = cfg.box_mean;
-    ca.d.zax = zax; ca.d.yax = yax; ca.d.hcg = hcg.data(); ca.d.ia = iA.data(); ca.d.id = iD.data(); ca.d.J = nullptr;
-    ca.d.dx = dx; ca.d.dy = dy; ca.d.nh = ns + 3; ca.d.a = a; ca.d.N = N; ca.d.nx = cfg.nx; ca.d.ny = cfg.ny;
-    ca.d.yf = 1; ca.d.nk = nk;
-
-    auto beam_parts = [&](StepTape& t, bool after) {
-        Parts p = beam.view();
-        double* d = after ? t.beam1.data() : t.beam0.data();
-        p.x = d; p.y = d + t.np; p.zeta = d + 2 * t.np; p.px = d + 3 * t.np; p.py = d + 4 * t.np;
-        p.pz = d + 5 * t.np; p.z = d + 6 * t.np; p.rem = d + 7 * t.np;
-        p.alive = after ? t.alive1.data() : t.alive0.data();
-        p.qm = t.qm.data();
-        return p;
-    };
+    CurrentsAdj ca = currents_args(*this, g);
+    ca.ljkx = A.jk[0].data(); ca.ljky = A.jk[1].data(); ca.ljkz = A.jk[2].data();
 
     auto step_rev = [&](StepTape& t) {
         sponge_rev();
@@ -349,21 +394,11 @@ void Sim::gradient(int steps, int every, double* loss, double* genv, int nenv, d
         march_rev(t);
         // the beam currents of the pushed beam
         double* W = A.W.data();
-        currents_adj(ca, fft, beam_parts(t, true), t.np, W, W + npm, W + 3 * npm, W + 4 * npm, W + 5 * npm,
+        currents_adj(ca, fft, tape_beam_parts(*this, t, true), t.np, W, W + npm, W + 3 * npm, W + 4 * npm, W + 5 * npm,
                      A.X.data(), A.Y.data(), A.P.data(), A.S.data(), A.jw.data());
         // the beam push (src/sim_particles.cpp:6-24) in the fields before the step
         planes_to_double(t.F0.data(), F.size(), A.Fd.data(), Fsz, Fsz, 1);
-        const Parts b0 = beam_parts(t, false);
-        g3d_bpush_ctx bc{};
-        bc.nx = cfg.nx; bc.ny = cfg.ny; bc.nsl = ns + 2; bc.dx = dx; bc.dy = dy; bc.zeta0 = cfg.zeta[0]; bc.ihz = 1 / h;
-        bc.F = A.Fd.data();
-        bc.laser_on = laser.on; bc.laser_pol = laser.pol; bc.a0 = laser.a0; bc.kL = laser.kL; bc.tau = laser.tau;
-        bc.cep = laser.cep; bc.eta_c = laser.eta_c; bc.kx = laser.kx; bc.kz = laser.kz;
-        bc.X0 = X0; bc.zc = cfg.zeta_center; bc.s0 = t.s0; bc.s1 = t.s0 + cfg.dz; bc.dz_push = cfg.dz_push;
-        bc.push_phase = cfg.push_phase; bc.hmin = hmin; bc.zhand = -1e300; bc.zmax = cfg.zeta[N - 1];
-        bc.ylo = -1e300; bc.yhi = 1e300; bc.vz_min = cfg.vz_min;
-        bc.x = b0.x; bc.y = b0.y; bc.zeta = b0.zeta; bc.px = b0.px; bc.py = b0.py; bc.pz = b0.pz; bc.z = b0.z;
-        bc.rem = b0.rem; bc.qm = b0.qm; bc.alive = b0.alive; bc.np = t.np;
+        const g3d_bpush_ctx bc = beam_push_ctx(*this, t, A.Fd.data(), hmin);
         double* Wn = A.Wn.data();
         g3d_bpush_adj ba{};
         ba.bx = W; ba.by = W + npm; ba.bzeta = W + 2 * npm; ba.bpx = W + 3 * npm; ba.bpy = W + 4 * npm;
@@ -412,12 +447,7 @@ void Sim::gradient(int steps, int every, double* loss, double* genv, int nenv, d
     // the laser: R += the spectra of the laser field, linear in the envelope at each slice
     if (nenv > 0) {
         const auto& Lz = cfg.glasers[0];
-        GridLaserArgs la;
-        la.amp = Lz.a0 * Lz.kL; la.waist = Lz.waist; la.zR = 0.5 * Lz.kL * Lz.waist * Lz.waist; la.zfoc = Lz.z_focus;
-        la.zc = Lz.zeta_c; la.tau = Lz.tau; la.kL = Lz.kL; la.cep = Lz.cep;
-        la.rc = std::cos(Lz.pol_angle); la.rs = std::sin(Lz.pol_angle); la.dx = dx; la.dy = dy; la.X0 = X0; la.Y0 = Y0;
-        la.circular = Lz.circular; la.helicity = Lz.helicity; la.nx = cfg.nx; la.ny = cfg.ny; la.nk = nk;
-        la.zeta = zs.data(); la.yn = nullptr;
+        GridLaserArgs la = grid_laser_args(Lz);
         std::vector<double> ones(size_t(N), 1.0);
         DevBuf<double> envd(static_cast<size_t>(N));
         envd.from_host(ones.data(), 0, size_t(N));
@@ -520,12 +550,7 @@ void Sim::tangent(int steps, const double* denv, int nenv, const double* dprof, 
     // init: the laser along denv (its spectra are linear in the envelope), the march of D = 0, the sponge
     if (nenv > 0) {
         const auto& Lz = cfg.glasers[0];
-        GridLaserArgs la;
-        la.amp = Lz.a0 * Lz.kL; la.waist = Lz.waist; la.zR = 0.5 * Lz.kL * Lz.waist * Lz.waist; la.zfoc = Lz.z_focus;
-        la.zc = Lz.zeta_c; la.tau = Lz.tau; la.kL = Lz.kL; la.cep = Lz.cep;
-        la.rc = std::cos(Lz.pol_angle); la.rs = std::sin(Lz.pol_angle); la.dx = dx; la.dy = dy; la.X0 = X0; la.Y0 = Y0;
-        la.circular = Lz.circular; la.helicity = Lz.helicity; la.nx = cfg.nx; la.ny = cfg.ny; la.nk = nk;
-        la.zeta = zs.data(); la.yn = nullptr;
+        GridLaserArgs la = grid_laser_args(Lz);
         std::vector<double> e(static_cast<size_t>(N));
         for (int n = 0; n < N; ++n) e[size_t(n)] = hat_sum(denv, nenv, Lz.env_z0, Lz.env_dz, cfg.zeta[n]);
         DevBuf<double> envd(static_cast<size_t>(N));
@@ -538,11 +563,7 @@ void Sim::tangent(int steps, const double* denv, int nenv, const double* dprof, 
     march_all(tmview(0.0), a, b);
     sponge_tlm();
 
-    CurrentsAdj ca{};
-    ca.g = g; ca.hcg = hcg.data(); ca.zf = cfg.zeta_filter; ca.zu = zax.uniform; ca.keep0 = cfg.box_mean;
-    ca.d.zax = zax; ca.d.yax = yax; ca.d.hcg = hcg.data(); ca.d.ia = iA.data(); ca.d.id = iD.data(); ca.d.J = nullptr;
-    ca.d.dx = dx; ca.d.dy = dy; ca.d.nh = ns + 3; ca.d.a = a; ca.d.N = N; ca.d.nx = cfg.nx; ca.d.ny = cfg.ny;
-    ca.d.yf = 1; ca.d.nk = nk;
+    const CurrentsAdj ca = currents_args(*this, g);
     std::unordered_map<unsigned long long, std::array<double, 6>> carry;   // x y zeta px py pz by id
     StepTape t;
     for (int k = 0; k < steps; ++k) {
@@ -557,35 +578,11 @@ void Sim::tangent(int steps, const double* denv, int nenv, const double* dprof, 
         const long np_step = t.np;
         // the beam push along the tangent, in the fields before the step
         planes_to_double(t.F0.data(), Fsz, Fd.data(), Fsz, Fsz, 1);
-        Parts b0 = beam.view();
-        {
-            double* d0 = t.beam0.data();
-            b0.x = d0; b0.y = d0 + np_step; b0.zeta = d0 + 2 * np_step; b0.px = d0 + 3 * np_step;
-            b0.py = d0 + 4 * np_step; b0.pz = d0 + 5 * np_step; b0.z = d0 + 6 * np_step; b0.rem = d0 + 7 * np_step;
-            b0.alive = t.alive0.data(); b0.qm = t.qm.data();
-        }
-        g3d_bpush_ctx bc{};
-        bc.nx = cfg.nx; bc.ny = cfg.ny; bc.nsl = ns + 2; bc.dx = dx; bc.dy = dy; bc.zeta0 = cfg.zeta[0]; bc.ihz = 1 / h;
-        bc.F = Fd.data();
-        bc.laser_on = laser.on; bc.laser_pol = laser.pol; bc.a0 = laser.a0; bc.kL = laser.kL; bc.tau = laser.tau;
-        bc.cep = laser.cep; bc.eta_c = laser.eta_c; bc.kx = laser.kx; bc.kz = laser.kz;
-        bc.X0 = X0; bc.zc = cfg.zeta_center; bc.s0 = t.s0; bc.s1 = t.s0 + cfg.dz; bc.dz_push = cfg.dz_push;
-        bc.push_phase = cfg.push_phase; bc.hmin = hmin; bc.zhand = -1e300; bc.zmax = cfg.zeta[N - 1];
-        bc.ylo = -1e300; bc.yhi = 1e300; bc.vz_min = cfg.vz_min;
-        bc.x = b0.x; bc.y = b0.y; bc.zeta = b0.zeta; bc.px = b0.px; bc.py = b0.py; bc.pz = b0.pz; bc.z = b0.z;
-        bc.rem = b0.rem; bc.qm = b0.qm; bc.alive = b0.alive; bc.np = np_step;
-        bunch_push_tlm(bc, tF.data(), tW.data(), cap);
+        bunch_push_tlm(beam_push_ctx(*this, t, Fd.data(), hmin), tF.data(), tW.data(), cap);
         // the beam currents of the pushed beam
-        Parts b1 = beam.view();
-        {
-            double* d1 = t.beam1.data();
-            b1.x = d1; b1.y = d1 + np_step; b1.zeta = d1 + 2 * np_step; b1.px = d1 + 3 * np_step;
-            b1.py = d1 + 4 * np_step; b1.pz = d1 + 5 * np_step; b1.z = d1 + 6 * np_step; b1.rem = d1 + 7 * np_step;
-            b1.alive = t.alive1.data(); b1.qm = t.qm.data();
-        }
         tjw.zero();
         double* W = tW.data();
-        deposit_tlm(ca, b1, np_step, W, W + cap, W + 3 * cap, W + 4 * cap, W + 5 * cap, tjw.data());
+        deposit_tlm(ca, tape_beam_parts(*this, t, true), np_step, W, W + cap, W + 3 * cap, W + 4 * cap, W + 5 * cap, tjw.data());
         currents_zfilter(g, tjw.data(), hcg.data(), cfg.zeta_filter, zax.uniform, S1.data(), tjk[2].data());
         fft_run(fft, S1.data(), ns + 1, false);
         fft_run(fft, tjk[2].data(), ns + 1, false);
@@ -627,16 +624,9 @@ void Sim::tangent(int steps, const double* denv, int nenv, const double* dprof, 
         m.bRx = m.bRy = m.bEz = m.bBz = nullptr;
         pass_modes(m, fft, tX.data(), tT.data(), nullptr, 1);
         real_fields_t(tFm.data() + long(N - 1 - a) * nk, fpl);
-        g3d_spush_ctx c{};
-        c.nx = cfg.nx; c.ny = cfg.ny; c.dx = dx; c.dy = dy; c.zeta0 = cfg.zeta[0]; c.ihz = 1 / h;
-        c.Delta = cfg.dz; c.s1 = t.s0 + cfg.dz; c.sTrap = c.s1 + (cfg.trap_advance - 1) * cfg.dz;
-        c.vz_beam = cfg.vz_beam; c.ia = ia0; c.F0 = F0d.data(); c.F1 = F1d.data(); c.J = nullptr;
-        c.q = t.pq.data(); c.qm = t.pq.data() + ns_; c.w = t.pq.data() + 2 * ns_; c.np = ns_;
+        g3d_spush_ctx c = stream_push_ctx(*this, t, ia0, F0d.data(), F1d.data());
         for (int n = N - 2; n >= a; --n) {
-            double* st = t.ps.data() + size_t(n) * 7 * size_t(ns_);
-            c.n = n; c.zn = cfg.zeta[n]; c.zn1 = cfg.zeta[n + 1];
-            c.x = st; c.y = st + ns_; c.zeta = st + 2 * ns_; c.z = st + 3 * ns_; c.px = st + 4 * ns_;
-            c.py = st + 5 * ns_; c.pz = st + 6 * ns_; c.alive = t.pa.data() + size_t(n) * ns_;
+            stream_push_slot(*this, t, n, c);
             for (int p = 0; p < P; ++p) {
                 const bool last = p == P - 1;
                 const float* f0 = p == 0 ? t.F0.data() + long(n - a) * nk
@@ -653,7 +643,7 @@ void Sim::tangent(int steps, const double* denv, int nenv, const double* dprof, 
                 stream_push_tlm(c, tFmid.data(), dF1.data(), tP.data(), tJ.data(), last);
                 pass_rows_fwd(tJ.data(), TileGeom{}, fft, tX.data(), nullptr, nullptr, 0, 0, 0);
                 m.n = n; m.first = p == 0; m.last = last; m.top = 0;
-                m.wex = (n >= 1 && n + 1 <= N - 2 && n != N - 2) ? (cen[size_t(n)] - cen[size_t(n - 1)]) / (cen[size_t(n + 1)] - cen[size_t(n)]) : 0.0;
+                m.wex = pass_wex(cen, n, N);
                 m.qcorr = D * wlo_h[size_t(n + 1)];
                 // several passes: a pass before the last leaves slice n at s0 (the kernel reads this only as a flag)
                 cx* flag = P > 1 ? tX.data() : nullptr;
