@@ -19,6 +19,13 @@ included), the plain FP64 form.
 gamma 1000 in the oblique plane wave a0 = 1, dz = 20/k0); a step is the RK4 push of every beam electron, its
 deposit and currents, the march of all slices, sponge and real fields; metric: beam pushes per second.
 
+--workload gradient: config 4 (128 x 128 modes over 120/k0, 512 slices of 0.5/k0, 20 steps of 10/k0, plasma
+2e-3 n_c at 2 x 1 electrons per cell, a Gaussian driver whose amplitude along zeta is a 32-value piecewise-linear
+a0(zeta) = 1.5 + N(0, 0.05), a witness of 2^18 electrons at gamma 100 with a 16-value piecewise-linear current
+profile); the gradient of the witness's mean p_z after the 20 steps in the 48 values by the reverse of the march
+(checkpoints every 5 steps), beside the loss alone and the tangent-linear run along one direction; metric:
+gradients per second, FP64 form.
+
 Prints one JSON line.  --gpus above 1 is refused: this build runs on one GPU.
 """
 import argparse
@@ -180,6 +187,95 @@ def vacuum_main(args, torch):
         "cpu_baseline": cpu,
         "kernel_families_one_extra_step": r["fam"],
         "init_s": r["t_init"],
+    }
+    print(json.dumps(out))
+
+
+def config4_deck(steps=20):
+    """config 4: 128 x 128 modes over 120/k0, 512 slices of 0.5/k0, `steps` steps of 10/k0, plasma 2e-3 n_c at
+    2 x 1 electrons per cell, the Gaussian driver (waist 25/k0) with a0(zeta) through 32 values 2/k0 apart (seeded
+    1.5 + N(0, 0.05)), the witness of 2^18 electrons at gamma 100 (sigma_r 2/k0, 5 pC) with the current profile
+    through 16 values (ones) over zeta 90 ... 120/k0"""
+    import g3d_amd
+    rng = np.random.default_rng(4)
+    env = 1.5 + 0.05 * rng.standard_normal(32)
+    Lw = 511 * 0.5
+    d = g3d_amd.Deck(LAMBDA0)
+    d.c.nx = 128; d.c.Lx = 120.0
+    d.y_uniform(128, 120.0)
+    d.zeta_uniform(512, Lw)
+    d.set(dz=10.0, z_end=10.0 * steps, sponge=8.0, sponge_y=8.0, move_x=0, move_y=0, plasma_on=1, n0=2e-3,
+          ppc_x=2, ppc_y=1, iterations=2, mixed=0, charge=5e-12, gamma=100.0, sigx=2.0, sigy=2.0, sigz=5.0,
+          emitx=0.0, emity=0.0, espread=0.0, nmacro=1 << 18, seed=11)
+    d.add_grid_laser(a0=1.0, waist=25.0, tau=1.0, zeta_c=Lw - 41.0, z_focus=0.0)
+    d.set_envelope(0, list(env), Lw - 72.0, 2.0)
+    d.set_profile([1.0] * 16, 90.0, 120.0)
+    return d
+
+
+def cpu_baseline_config4(threads, steps=2):
+    """the reference's CPU build on config 4 itself for `steps` steps after init; the gradient by central
+    differences costs 96 + 1 such runs of 20 steps"""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import refpy
+    if not refpy.available():
+        return None
+    refpy.set_threads(threads)
+    ref = refpy.RefSim(config4_deck())
+    ref.init()
+    t = time.perf_counter()
+    for _ in range(steps):
+        ref.step()
+    dt = (time.perf_counter() - t) / steps
+    return {"value": 1.0 / (97 * 20 * dt), "unit": "gradients/s", "cores": threads, "kind": "reference",
+            "sample": "reference CPU build (g++ -O3 -fopenmp, %d threads), config 4, %d steps: %.2f s per step; the "
+                      "48-value gradient by central differences = 97 runs of 20 steps" % (threads, steps, dt)}
+
+
+def gradient_main(args, torch):
+    import g3d_amd
+    t = time.perf_counter()
+    s = g3d_amd.Sim(config4_deck())
+    s.init()
+    for _ in range(20):
+        s.step()
+    t_loss = time.perf_counter() - t
+    s.close()
+    times = []
+    for _ in range(max(1, args.steps // 4)):
+        s = g3d_amd.Sim(config4_deck())
+        t = time.perf_counter()
+        loss, genv, gprof = s.gradient(20, every=5)
+        times.append(time.perf_counter() - t)
+        s.close()
+    t_grad = min(times)
+    rng = np.random.default_rng(11)
+    de, dp = rng.standard_normal(32), rng.standard_normal(16)
+    s = g3d_amd.Sim(config4_deck())
+    t = time.perf_counter()
+    _, dl = s.tangent(20, de, dp)
+    t_tlm = time.perf_counter() - t
+    s.close()
+    cpu = None
+    if not args.no_cpu:
+        thr = args.cpu_threads or min(16, os.cpu_count() or 1)
+        try:
+            cpu = cpu_baseline_config4(thr)
+        except Exception as e:
+            print("cpu baseline failed: %s" % e, file=sys.stderr)
+    out = {
+        "metric": "config-4 gradients/s (48 parameters, 20 steps, reverse of the march)",
+        "value": 1.0 / t_grad, "unit": "gradients/s", "n_gpus": 1, "steps": 20, "warmup": 0,
+        "ms_per_step": 1e3 * t_grad / 20, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "config 4: 128x128x512, 2^18 witness, 20 steps, gradient in 32 envelope + 16 profile "
+                               "values, checkpoints every 5 steps", "loss": loss,
+                   "grad_norm": float(np.sqrt(genv @ genv + gprof @ gprof))},
+        "roofline": None,
+        "times_s": {"loss": t_loss, "gradient": t_grad, "tangent_one_direction": t_tlm,
+                    "gradient_over_loss": t_grad / t_loss},
+        "tangent_vs_adjoint_rel": abs(dl - float(genv @ de + gprof @ dp)) / abs(dl),
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 
@@ -444,7 +540,7 @@ def main():
     ap.add_argument("--precision", choices=("both", "mixed", "fp64"), default="both",
                     help="both (default): the FP64 and the FP32-mixed march on the same deck; the line reports the "
                          "mixed one and its field error against FP64")
-    ap.add_argument("--workload", choices=("lwfa", "vacuum", "freehand"), default="lwfa",
+    ap.add_argument("--workload", choices=("lwfa", "vacuum", "freehand", "gradient"), default="lwfa",
                     help="lwfa (default): the plasma march of the headline metric; vacuum: the beam-push deck; "
                          "freehand: config 3 (free-hand y, ions, ADK)")
     ap.add_argument("--copy-calib", type=float, default=0.0,
@@ -465,6 +561,9 @@ def main():
         return
     if args.workload == "freehand":
         freehand_main(args, torch)
+        return
+    if args.workload == "gradient":
+        gradient_main(args, torch)
         return
 
     forms = {"fp64": [False], "mixed": [True], "both": [False, True]}[args.precision]
