@@ -595,7 +595,10 @@ def main():
                                % (args.nz, "" if args.witness else "no ", "FP32-mixed" if mixed else "FP64"),
                    "N": head["N"], "nstrm": head["nstrm"], "pushes_per_step": head["pushes_per_step"],
                    "stream_pushes_per_step": head["stream_pushes_per_step"],
-                   "witness_pushes_per_step": head["beam_pushes_per_step"]},
+                   "witness_pushes_per_step": head["beam_pushes_per_step"],
+                   # a slice solve: one slice of the march for all modes with its transforms (every pass of every
+                   # half-cell, and the front slice)
+                   "slice_solves_per_s": ((head["N"] - 1) * sim_passes(head["sim"]) + 1) * 1000.0 / head["ms_step"]},
         "roofline": {"kernel": head["kernel"], "bound": "hbm", "achieved": head["achieved"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": head["achieved"] / HBM_PEAK_GBS, "traffic": head["traffic"],
                      "bytes_per_launch": head["bpl"], "pushes_per_launch": head["push_per_launch"],
