@@ -901,8 +901,9 @@ __global__ void k_lattice(LatticeArgs L, Parts A, Parts B, long n) {
     if (g >= n) return;
     const int pp = L.px * L.py;
     const long t = g / L.per_tile, r = g - t * L.per_tile;
-    const long c = r / pp;
-    const int sub = int(r - c * pp);
+    const long cpt = L.per_tile / pp;                // cells of a tile: the sub-position outermost
+    const int sub = int(r / cpt);
+    const long c = r - sub * cpt;
     const int jl = int(c / L.TX), il = int(c - long(jl) * L.TX);
     const int b2 = sub / L.px, a2 = sub - b2 * L.px;
     const int tx = int(t % L.ntx), ty = int(t / L.ntx);
