@@ -9,6 +9,7 @@
 // forward update is transposed in reverse order with conj(a) = -a, conj(b) = -b.  The march over slices runs front
 // to rear (n = hi-1 ... lo, slice n reading slice n+1); its adjoint runs rear to front, carrying the adjoint of the
 // slice n+1 values from the update of slice n to that of slice n+1.
+#include "g3d_boris_rev.hpp"
 #include "g3d_kernels.hpp"
 #include "g3d_march_adj.hpp"
 #include "g3d_runtime.hpp"
@@ -100,63 +101,20 @@ __global__ void __launch_bounds__(256) k_boris(g3d_boris_args a) {
     boris(ux, uy, uz, E, B, a.qm[p], a.dt[p]);
     a.ux[p] = ux; a.uy[p] = uy; a.uz[p] = uz;
 }
-__device__ __forceinline__ void cross3(const double a[3], const double b[3], double c[3]) {
-    c[0] = a[1] * b[2] - a[2] * b[1]; c[1] = a[2] * b[0] - a[0] * b[2]; c[2] = a[0] * b[1] - a[1] * b[0];
-}
-// reverse mode of boris() (g3d_common.hpp), statement by statement
+// reverse mode of boris() (g3d_common.hpp): boris_rev (g3d_boris_rev.hpp)
 __global__ void __launch_bounds__(256) k_boris_vjp(g3d_boris_args a, g3d_boris_adj g) {
     const long p = long(blockIdx.x) * blockDim.x + threadIdx.x;
     if (p >= a.np) return;
     const double E[3] = {a.Ex[p], a.Ey[p], a.Ez[p]}, B[3] = {a.Bx[p], a.By[p], a.Bz[p]};
     const double u[3] = {a.ux[p], a.uy[p], a.uz[p]};
-    const double qm = a.qm[p], dt = a.dt[p];
-    // forward values
-    const double qh = 0.5 * qm * dt;
-    double av[3], t[3], pv[3], c[3];
-    for (int k = 0; k < 3; ++k) av[k] = u[k] + qh * E[k];
-    const double gm = sqrt(1.0 + av[0] * av[0] + av[1] * av[1] + av[2] * av[2]);
-    for (int k = 0; k < 3; ++k) t[k] = qh * B[k] / gm;
-    const double t2 = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
-    cross3(av, t, c);
-    for (int k = 0; k < 3; ++k) pv[k] = av[k] + c[k];
-    const double sf = 2.0 / (1.0 + t2);
-    // reverse: u' = b + qh E, b = a + sf (p x t)
     const double ub[3] = {g.ubx[p], g.uby[p], g.ubz[p]};
-    double ab[3], Eb[3], Bb[3], tb[3], pb[3];
-    double qhb = ub[0] * E[0] + ub[1] * E[1] + ub[2] * E[2];
-    for (int k = 0; k < 3; ++k) { Eb[k] = qh * ub[k]; ab[k] = ub[k]; }
-    double pxt[3];
-    cross3(pv, t, pxt);
-    const double sfb = ub[0] * pxt[0] + ub[1] * pxt[1] + ub[2] * pxt[2];
-    double txb[3], bxp[3];
-    cross3(t, ub, txb);                  // d(p x t) . ub / dp = t x ub
-    cross3(ub, pv, bxp);                 // d(p x t) . ub / dt = ub x p
-    for (int k = 0; k < 3; ++k) { pb[k] = sf * txb[k]; tb[k] = sf * bxp[k]; }
-    // sf = 2 / (1 + t2)
-    const double t2b = -sfb * sf * sf * 0.5;
-    // p = a + a x t
-    double txp[3], pxa[3];
-    cross3(t, pb, txp);                  // d(a x t) . pb / da = t x pb
-    cross3(pb, av, pxa);                 // d(a x t) . pb / dt = pb x a
-    for (int k = 0; k < 3; ++k) { ab[k] += pb[k] + txp[k]; tb[k] += pxa[k]; }
-    // t2 = t . t
-    for (int k = 0; k < 3; ++k) tb[k] += 2 * t2b * t[k];
-    // t = qh B / gm
-    double gmb = 0;
-    for (int k = 0; k < 3; ++k) {
-        Bb[k] = qh * tb[k] / gm;
-        qhb += tb[k] * B[k] / gm;
-        gmb -= tb[k] * t[k] / gm;
-    }
-    // gm = sqrt(1 + a . a)
-    for (int k = 0; k < 3; ++k) ab[k] += gmb * av[k] / gm;
-    // a = u + qh E
-    for (int k = 0; k < 3; ++k) { Eb[k] += qh * ab[k]; qhb += ab[k] * E[k]; }
+    double ab[3], Eb[3] = {0, 0, 0}, Bb[3] = {0, 0, 0}, qmb = 0, dtb = 0;
+    boris_rev(u, E, B, a.qm[p], a.dt[p], ub, ab, Eb, Bb, qmb, dtb);
     g.gux[p] = ab[0]; g.guy[p] = ab[1]; g.guz[p] = ab[2];
     g.gEx[p] = Eb[0]; g.gEy[p] = Eb[1]; g.gEz[p] = Eb[2];
     g.gBx[p] = Bb[0]; g.gBy[p] = Bb[1]; g.gBz[p] = Bb[2];
-    g.gqm[p] = 0.5 * dt * qhb;
-    g.gdt[p] = 0.5 * qm * qhb;
+    g.gqm[p] = qmb;
+    g.gdt[p] = dtb;
 }
 
 }  // namespace

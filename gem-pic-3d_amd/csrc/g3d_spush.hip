@@ -3,14 +3,16 @@
 //
 // Forward: the operations of stream_one (g3d_particles.hip; src/sim_stream.hpp:104-171) for an electron with no
 // ionization and no analytic laser, in the same order, with the gather and the deposit of the Fourier-y path.
-// Reverse: the same code run on dual numbers (value + 13 partial derivatives: the start state x, y, ζ, z, p_x,
-// p_y, p_z and the gathered E, B), so that one lane gets the gradient of
+// Reverse: the forward once with its intermediates, then every statement transposed in reverse order (the Boris
+// rotations through boris_rev), so that one lane gets the gradient of
 //     S = <adjoint of the pushed state, pushed state> + <adjoint of J, this electron's deposit>
 // with respect to its start state and its gathered fields; the latter go back to the field planes through the
-// transpose of the gather (the TSC and ζ weights of the midpoint).  The position dependence of the gathered fields
+// transpose of the gather (the TSC and ζ weights of the midpoint).  The tangent-linear push runs the same code on
+// dual numbers of one direction.  The position dependence of the gathered fields
 // and of the deposit weights enters through the derivatives of the TSC weights.  Branches (entry of a not yet
 // streaming electron, trapping, the clamp of v_z, the sign of the charge) follow the values, as in the forward
 // push, and are held fixed.  No counterpart of the reverse in the reference.
+#include "g3d_boris_rev.hpp"
 #include "g3d_dual.hpp"
 #include "g3d_kernels.hpp"
 #include "g3d_runtime.hpp"
@@ -18,13 +20,9 @@
 namespace g3d {
 namespace {
 constexpr int kT = 256;
-constexpr int kNd = 13;             // x y ζ z px py pz | Ex Ey Ez Bx By Bz
-
-using Dn = Dual<kNd>;
 
 // a periodic coordinate moved into [0, L): the same shift on the value and no change of its derivatives
 __device__ __forceinline__ double twrap(double x, double L) { return wrap_mod(x, L); }
-__device__ __forceinline__ Dn twrap(const Dn& x, double L) { return x + (wrap_mod(x.v, L) - x.v); }
 using D1 = Dual<1>;                 // one direction: the tangent-linear push
 __device__ __forceinline__ D1 dt1(double v, double d) { D1 r; r.v = v; r.d[0] = d; return r; }
 __device__ __forceinline__ D1 twrap(const D1& x, double L) { return x + (wrap_mod(x.v, L) - x.v); }
@@ -185,56 +183,245 @@ __global__ void __launch_bounds__(kT) k_spush(g3d_spush_ctx c) {
     c.alive[i] = alive;
 }
 
+// The TSC weights of a point, their x and y derivatives and nodes (tsc_wd on both axes)
+struct Tsc2 {
+    long k[9];
+    double w[9], wdx[9], wdy[9];
+};
+__device__ __forceinline__ Tsc2 tsc2(const g3d_spush_ctx& c, double x, double y) {
+    Tsc2 t;
+    int ix[3], jn[3]; double wx[3], wn[3], dwx[3], dwn[3];
+    tsc_wd(x, c.dx, c.nx, ix, wx, dwx);
+    tsc_wd(y, c.dy, c.ny, jn, wn, dwn);
+    for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i) {
+            t.k[3 * j + i] = long(jn[j]) * c.nx + ix[i];
+            t.w[3 * j + i] = wx[i] * wn[j];
+            t.wdx[3 * j + i] = dwx[i] * wn[j];
+            t.wdy[3 * j + i] = wx[i] * dwn[j];
+        }
+    return t;
+}
+// the reverse of a deposit of F v at (xd, yd) into J (F the whole factor with the inverse area): adds the adjoints
+// of F, v and the point
+__device__ __forceinline__ void deposit_rev(const g3d_spush_ctx& c, const double* bJ, double xd, double yd, double F,
+                                            const double v[3], double& Fb, double vb[3], double& xdb, double& ydb) {
+    const long nk = long(c.nx) * c.ny;
+    const Tsc2 t = tsc2(c, xd, yd);
+    const int cq = F < 0 ? 3 : 4;
+    double sw = 0, sx = 0, sy = 0, sv[3] = {0, 0, 0};
+    for (int q = 0; q < 9; ++q) {
+        const long k = t.k[q];
+        const double b0 = bJ[k], b1 = bJ[nk + k], b2 = bJ[2 * nk + k];
+        const double G = b0 * v[0] + b1 * v[1] + b2 * v[2] + bJ[cq * nk + k];
+        sw += t.w[q] * G;
+        sx += t.wdx[q] * G;
+        sy += t.wdy[q] * G;
+        sv[0] += t.w[q] * b0; sv[1] += t.w[q] * b1; sv[2] += t.w[q] * b2;
+    }
+    Fb += sw;
+    for (int q = 0; q < 3; ++q) vb[q] += F * sv[q];
+    xdb += F * sx;
+    ydb += F * sy;
+}
+
+// The reverse of the push (T = double in tpush), written by hand: the forward once with its intermediates kept, then
+// every statement transposed in reverse order; the Boris rotations through boris_rev.  The branch decisions and
+// the skipped deposit of a zero factor are the forward's.  Gives the same numbers as the forward-mode reverse on
+// 13-direction dual numbers, in registers (that one spilled 828 bytes per lane).
 __global__ void __launch_bounds__(kT) k_spush_vjp(g3d_spush_ctx c, g3d_spush_adj a) {
     const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= c.np) return;
     const long nk = long(c.nx) * c.ny;
-    SP<Dn> p{dvar<kNd>(c.x[i], 0), dvar<kNd>(c.y[i], 1), dvar<kNd>(c.zeta[i], 2), dvar<kNd>(c.z[i], 3),
-             dvar<kNd>(c.px[i], 4), dvar<kNd>(c.py[i], 5), dvar<kNd>(c.pz[i], 6)};
+    const double x = c.x[i], y = c.y[i], zeta = c.zeta[i], z = c.z[i];
+    const double p[3] = {c.px[i], c.py[i], c.pz[i]};
+    const double q = c.q[i], qm = c.qm[i], w = c.w[i];
+    double xb = a.bx[i], yb = a.by[i], zetab = a.bzeta[i], zb = a.bz[i];
+    double pb[3] = {a.bpx[i], a.bpy[i], a.bpz[i]};
+    auto out = [&](double gx, double gy, double gzeta, double gz, const double gp[3]) {
+        a.gx[i] = gx; a.gy[i] = gy; a.gzeta[i] = gzeta; a.gz[i] = gz; a.gpx[i] = gp[0]; a.gpy[i] = gp[1];
+        a.gpz[i] = gp[2];
+    };
     int alive = c.alive[i];
-    Dn S = dconst<kNd>(0);
-    GatherPoint gp{};
-    bool gathered = false;
-    auto gather = [&](const Dn& xg, const Dn& yg, const Dn& zm, Dn E[3], Dn B[3]) {
-        gp = gather_point(c, xg.v, yg.v, zm.v);
-        gathered = true;
+    const double zn = c.zn, zn1 = c.zn1, h = zn1 - zn, Lx = c.nx * c.dx, Ly = c.ny * c.dy;
+    if (alive == 6 && zeta > zn) alive = 1;
+    if (alive != 1) { out(xb, yb, zetab, zb, pb); return; }   // not pushed: the identity
+    // ---- forward, keeping the intermediates
+    const int hmode = zeta >= zn1 - 1e-9 * h ? 0 : (zeta - zn > 0 ? 1 : 2);
+    const double hl = hmode == 0 ? h : (hmode == 1 ? zeta - zn : 0.0);
+    const double g0 = sqrt(1 + p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    const double A0 = 1 + p[0] * p[0] + p[1] * p[1], D0 = g0 * (g0 + p[2]);
+    const double omv0 = A0 / D0;
+    const double dt0 = hl / omv0;
+    const double xm = x + 0.5 * dt0 * p[0] / g0, ym = y + 0.5 * dt0 * p[1] / g0;
+    const double zetam = zeta - 0.5 * hl;
+    double E[3], B[3];
+    {
+        const GatherPoint gp = gather_point(c, twrap(xm, Lx), twrap(ym, Ly), zetam);
         for (int comp = 0; comp < 6; ++comp) {
-            double gx, gy, gzeta;
-            const double s = gather_comp(c, gp, comp, &gx, &gy, &gzeta);
-            Dn r = dvar<kNd>(s, 7 + comp);
-            for (int k = 0; k < 7; ++k) r.d[k] = gx * xg.d[k] + gy * yg.d[k] + gzeta * zm.d[k];
-            if (comp < 3) E[comp] = r; else B[comp - 3] = r;
-        }
-    };
-    auto deposit = [&](const Dn& x, const Dn& y, const Dn& f, const Dn& vx, const Dn& vy, const Dn& vz) {
-        int ix[3], jn[3]; double wx[3], wn[3], dwx[3], dwn[3];
-        tsc_wd(x.v, c.dx, c.nx, ix, wx, dwx);
-        tsc_wd(y.v, c.dy, c.ny, jn, wn, dwn);
-        const int cq = f.v < 0 ? 3 : 4;
-        for (int v = 0; v < 3; ++v)
-            for (int u = 0; u < 3; ++u) {
-                const long k = long(jn[v]) * c.nx + ix[u];
-                Dn W = dconst<kNd>(wx[u] * wn[v]);
-                for (int t = 0; t < kNd; ++t) W.d[t] = dwx[u] * wn[v] * x.d[t] + wx[u] * dwn[v] * y.d[t];
-                const Dn ww = f * W * c.ia;
-                const double* bJ = a.bJ;
-                S = S + ww * (bJ[k] * vx + bJ[nk + k] * vy + bJ[2 * nk + k] * vz + bJ[cq * nk + k]);
-            }
-    };
-    tpush<Dn>(c, p, c.q[i], c.qm[i], c.w[i], alive, gather, deposit);
-    S = S + a.bx[i] * p.x + a.by[i] * p.y + a.bzeta[i] * p.zeta + a.bz[i] * p.z + a.bpx[i] * p.px +
-        a.bpy[i] * p.py + a.bpz[i] * p.pz;
-    a.gx[i] = S.d[0]; a.gy[i] = S.d[1]; a.gzeta[i] = S.d[2]; a.gz[i] = S.d[3];
-    a.gpx[i] = S.d[4]; a.gpy[i] = S.d[5]; a.gpz[i] = S.d[6];
-    if (!gathered) return;
-    for (int comp = 0; comp < 6; ++comp) {          // the transpose of the gather
-        const double e = S.d[7 + comp];
-        if (e == 0) continue;
-        for (int t = 0; t < 9; ++t) {
-            atomicAdd(&a.gF0[comp * nk + gp.k[t]], e * gp.w[t] * gp.gz);
-            atomicAdd(&a.gF1[comp * nk + gp.k[t]], e * gp.w[t] * gp.fz);
+            const double v = gather_comp(c, gp, comp, nullptr, nullptr, nullptr);
+            if (comp < 3) E[comp] = v; else B[comp - 3] = v;
         }
     }
+    double u1[3] = {p[0], p[1], p[2]};
+    boris(u1[0], u1[1], u1[2], E, B, qm, dt0);
+    const double g1 = sqrt(1 + u1[0] * u1[0] + u1[1] * u1[1] + u1[2] * u1[2]);
+    const double A1 = 1 + u1[0] * u1[0] + u1[1] * u1[1], D1 = g1 * (g1 + u1[2]);
+    const double omv1 = A1 / D1;
+    const double ssum = omv0 + omv1;
+    const double dt = 2 * hl / ssum;
+    double u2[3] = {p[0], p[1], p[2]};
+    boris(u2[0], u2[1], u2[2], E, B, qm, dt);
+    const double g2 = sqrt(1 + u2[0] * u2[0] + u2[1] * u2[1] + u2[2] * u2[2]);
+    double v2[3];
+    for (int k = 0; k < 3; ++k) v2[k] = 0.5 * (p[k] / g0 + u2[k] / g2);
+    const bool trapped = z + v2[2] * dt >= c.sTrap;
+    double uF[3], gF, vF[3], DT, dtq = 0, vzc = 0;
+    bool vz_clamped = false, dtp_clamped = false;
+    if (trapped) {
+        vz_clamped = !(v2[2] > 1e-3);
+        vzc = vz_clamped ? 1e-3 : v2[2];
+        dtq = (c.sTrap - z) / vzc;
+        dtp_clamped = !(dtq > 0);
+        DT = dtp_clamped ? 0.0 : dtq;
+        for (int k = 0; k < 3; ++k) uF[k] = p[k];
+        boris(uF[0], uF[1], uF[2], E, B, qm, DT);
+        gF = sqrt(1 + uF[0] * uF[0] + uF[1] * uF[1] + uF[2] * uF[2]);
+        for (int k = 0; k < 3; ++k) vF[k] = 0.5 * (p[k] / g0 + uF[k] / gF);
+    } else {
+        for (int k = 0; k < 3; ++k) { uF[k] = u2[k]; vF[k] = v2[k]; }
+        gF = g2;
+        DT = dt;
+    }
+    const double x1 = x + vF[0] * DT, y1 = y + vF[1] * DT;
+    const double f = q * w * DT / (c.Delta * h * c.dx);
+    // ---- reverse
+    double uFb[3] = {pb[0], pb[1], pb[2]};           // p1 = uF
+    double vFb[3] = {0, 0, 0}, DTb = 0, gFb = 0, x1b = xb, y1b = yb;
+    double pbv[3] = {0, 0, 0};                        // the adjoint of the start momentum
+    double xs = 0, ys = 0, zs_ = 0, zetas = 0;       // the adjoints of the start position
+    if (trapped) {
+        const double vze = uF[2] / gF;
+        if (vze >= c.vz_beam) {                       // the beam current of the trapped electron at (x1, y1)
+            const double C = q * w * c.ia / (h * c.dx);
+            const double F2 = C / vze;
+            const double v3[3] = {uF[0] / gF, uF[1] / gF, vze};
+            double F2b = 0, v3b[3] = {0, 0, 0}, xdb = 0, ydb = 0;
+            deposit_rev(c, a.bJ, twrap(x1, Lx), twrap(y1, Ly), F2, v3, F2b, v3b, xdb, ydb);
+            x1b += xdb; y1b += ydb;
+            double vzeb = v3b[2] - F2b * C / (vze * vze);
+            uFb[0] += v3b[0] / gF; uFb[1] += v3b[1] / gF; uFb[2] += vzeb / gF;
+            gFb -= (v3b[0] * uF[0] + v3b[1] * uF[1] + vzeb * uF[2]) / (gF * gF);
+        }
+    }
+    if (f != 0) {                                     // the current of the half-cell at the midpoint
+        const double F = f * c.ia;
+        double Fb = 0, xdb = 0, ydb = 0;
+        deposit_rev(c, a.bJ, twrap(0.5 * (x + x1), Lx), twrap(0.5 * (y + y1), Ly), F, vF, Fb, vFb, xdb, ydb);
+        xs += 0.5 * xdb; x1b += 0.5 * xdb;
+        ys += 0.5 * ydb; y1b += 0.5 * ydb;
+        DTb += Fb * c.ia * q * w / (c.Delta * h * c.dx);
+    }
+    // zeta1: trapped: zeta - (1 - vF_z) DT, else zn; z1: trapped: s1, else z + vF_z DT
+    if (trapped) {
+        zetas += zetab;
+        vFb[2] += zetab * DT;
+        DTb -= zetab * (1 - vF[2]);
+    } else {
+        zs_ += zb;
+        vFb[2] += zb * DT;
+        DTb += zb * vF[2];
+    }
+    // x1 = x + vF_x DT, y1 likewise
+    vFb[0] += x1b * DT; vFb[1] += y1b * DT; DTb += x1b * vF[0] + y1b * vF[1];
+    xs += x1b; ys += y1b;
+    // vF = (p / g0 + uF / gF) / 2
+    double g0b = 0;
+    for (int k = 0; k < 3; ++k) {
+        pbv[k] += 0.5 * vFb[k] / g0;
+        g0b -= 0.5 * vFb[k] * p[k] / (g0 * g0);
+        uFb[k] += 0.5 * vFb[k] / gF;
+        gFb -= 0.5 * vFb[k] * uF[k] / (gF * gF);
+    }
+    for (int k = 0; k < 3; ++k) uFb[k] += gFb * uF[k] / gF;
+    double Eb[3] = {0, 0, 0}, Bb[3] = {0, 0, 0}, qmb = 0, dtb = 0;
+    double ab[3];
+    if (trapped) {
+        double DTbb = 0;                              // the adjoint of dtp through its own rotation
+        boris_rev(p, E, B, qm, DT, uFb, ab, Eb, Bb, qmb, DTbb);
+        for (int k = 0; k < 3; ++k) pbv[k] += ab[k];
+        DTb += DTbb;
+        // dtp = max(dtq, 0), dtq = (sTrap - z) / vzc, vzc = max(v2_z, 1e-3)
+        const double dtqb = dtp_clamped ? 0.0 : DTb;
+        zs_ -= dtqb / vzc;
+        const double vzcb = -dtqb * dtq / vzc;
+        double v2b[3] = {0, 0, vz_clamped ? 0.0 : vzcb};
+        // v2 = (p / g0 + u2 / g2) / 2
+        double u2b[3], g2b = 0;
+        for (int k = 0; k < 3; ++k) {
+            pbv[k] += 0.5 * v2b[k] / g0;
+            g0b -= 0.5 * v2b[k] * p[k] / (g0 * g0);
+            u2b[k] = 0.5 * v2b[k] / g2;
+            g2b -= 0.5 * v2b[k] * u2[k] / (g2 * g2);
+        }
+        for (int k = 0; k < 3; ++k) u2b[k] += g2b * u2[k] / g2;
+        boris_rev(p, E, B, qm, dt, u2b, ab, Eb, Bb, qmb, dtb);
+        for (int k = 0; k < 3; ++k) pbv[k] += ab[k];
+    } else {
+        dtb += DTb;
+        boris_rev(p, E, B, qm, dt, uFb, ab, Eb, Bb, qmb, dtb);
+        for (int k = 0; k < 3; ++k) pbv[k] += ab[k];
+    }
+    // dt = 2 hl / (omv0 + omv1)
+    double hlb = dtb * 2 / ssum;
+    const double sb = -dtb * dt / ssum;
+    double omv0b = sb;
+    const double omv1b = sb;
+    // omv1 = A1 / D1, A1 = 1 + u1x^2 + u1y^2, D1 = g1 (g1 + u1z)
+    {
+        const double A1b = omv1b / D1, D1b = -omv1b * omv1 / D1;
+        double u1b[3] = {2 * u1[0] * A1b, 2 * u1[1] * A1b, D1b * g1};
+        const double g1b = D1b * (2 * g1 + u1[2]);
+        for (int k = 0; k < 3; ++k) u1b[k] += g1b * u1[k] / g1;
+        double dt0b = 0;
+        boris_rev(p, E, B, qm, dt0, u1b, ab, Eb, Bb, qmb, dt0b);
+        for (int k = 0; k < 3; ++k) pbv[k] += ab[k];
+        // the gather at (xm, ym, zetam), its point formed again: the field values' adjoints go to the planes, their
+        // position derivatives on
+        const GatherPoint gp = gather_point(c, twrap(xm, Lx), twrap(ym, Ly), zetam);
+        double xmb = 0, ymb = 0, zetamb = 0;
+        for (int comp = 0; comp < 6; ++comp) {
+            const double e = comp < 3 ? Eb[comp] : Bb[comp - 3];
+            double gxc, gyc, gzc;
+            gather_comp(c, gp, comp, &gxc, &gyc, &gzc);
+            xmb += e * gxc; ymb += e * gyc; zetamb += e * gzc;
+            if (e == 0) continue;
+            for (int t = 0; t < 9; ++t) {
+                atomicAdd(&a.gF0[comp * nk + gp.k[t]], e * gp.w[t] * gp.gz);
+                atomicAdd(&a.gF1[comp * nk + gp.k[t]], e * gp.w[t] * gp.fz);
+            }
+        }
+        // zetam = zeta - hl / 2; xm = x + dt0 px / (2 g0); ym likewise
+        zetas += zetamb;
+        hlb -= 0.5 * zetamb;
+        xs += xmb; ys += ymb;
+        dt0b += 0.5 * (xmb * p[0] + ymb * p[1]) / g0;
+        pbv[0] += 0.5 * xmb * dt0 / g0;
+        pbv[1] += 0.5 * ymb * dt0 / g0;
+        g0b -= 0.5 * (xmb * p[0] + ymb * p[1]) * dt0 / (g0 * g0);
+        // dt0 = hl / omv0
+        hlb += dt0b / omv0;
+        omv0b -= dt0b * dt0 / omv0;
+    }
+    // omv0 = A0 / D0, A0 = 1 + px^2 + py^2, D0 = g0 (g0 + pz)
+    {
+        const double A0b = omv0b / D0, D0b = -omv0b * omv0 / D0;
+        pbv[0] += 2 * p[0] * A0b; pbv[1] += 2 * p[1] * A0b; pbv[2] += D0b * g0;
+        g0b += D0b * (2 * g0 + p[2]);
+    }
+    for (int k = 0; k < 3; ++k) pbv[k] += g0b * p[k] / g0;
+    if (hmode == 1) zetas += hlb;                      // hl = zeta - zn
+    out(xs, ys, zetas, zs_, pbv);
 }
 
 // The tangent-linear push: the primal state and fields as k_spush (the state arrays are read, not written), the
