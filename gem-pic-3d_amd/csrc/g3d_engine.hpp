@@ -151,6 +151,7 @@ public:
     void tangent(int steps, const double* denv, int nenv, const double* dprof, int nprof, double* loss, double* dloss);
     StepTape* tape_ = nullptr;
     void tape_beam(StepTape& t, bool before);     // the beam state into t.beam0 (before the push) or t.beam1
+    void init_recording(bool dprof);              // init(), recording the profile derivatives when dprof
     bool record_dprof_ = false;       // load_bunch_profile records the profile derivatives below
     std::vector<double> dprof_;       // per loaded particle: d zeta_f / d prof[m] (nprof), then dx, dy, dzeta / d zeta_f
 

@@ -32,6 +32,14 @@
 
 namespace g3d {
 
+// init() with the profile derivatives of the bunch recorded (when asked), the flag cleared however init ends
+void Sim::init_recording(bool dprof) {
+    record_dprof_ = dprof;
+    dprof_.clear();
+    try { init(); } catch (...) { record_dprof_ = false; throw; }
+    record_dprof_ = false;
+}
+
 void Sim::tape_beam(StepTape& t, bool before) {
     t.np = np;
     DevBuf<double>& d = before ? t.beam0 : t.beam1;
@@ -169,10 +177,7 @@ void check_reversible(const Sim& sm, int steps, int every, int nenv, int nprof) 
 void Sim::gradient(int steps, int every, double* loss, double* genv, int nenv, double* gprof, int nprof) {
     check_reversible(*this, steps, every, nenv, nprof);
 
-    record_dprof_ = nprof > 0;
-    dprof_.clear();
-    init();
-    record_dprof_ = false;
+    init_recording(nprof > 0);
     const long np0 = np;
     auto check_lists = [&] {
         if (restreamed_total || np != np0 + trapped_total)
@@ -211,13 +216,15 @@ void Sim::gradient(int steps, int every, double* loss, double* genv, int nenv, d
         jw_zero = false;
     };
 
-    // forward, with the checkpoints
+    // forward, with the checkpoints; the state at the end is kept too, to leave the simulation there
     std::vector<Snapshot> snaps;
     for (int k = 0; k < steps; ++k) {
         if (k % every == 0) { snaps.emplace_back(); take(snaps.back()); }
         step();
         check_lists();
     }
+    Snapshot at_end;
+    take(at_end);
     {
         double sum = 0;
         long cnt = 0;
@@ -428,6 +435,7 @@ void Sim::gradient(int steps, int every, double* loss, double* genv, int nenv, d
         for (int k = k1 - 1; k >= k0; --k) step_rev(tapes[size_t(k - k0)]);
     }
     tapes.clear();
+    restore(at_end);
 
     // init() in reverse as far as the parameters: j0 = jk, the sponge and real fields, the march of D = 0
     sponge_rev();
@@ -487,10 +495,7 @@ void Sim::gradient(int steps, int every, double* loss, double* genv, int nenv, d
 // state from the step's tape.  A check of the reverse: <gradient, direction> equals this derivative.
 void Sim::tangent(int steps, const double* denv, int nenv, const double* dprof, int nprof, double* loss, double* dloss) {
     check_reversible(*this, steps, 1, nenv, nprof);
-    record_dprof_ = nprof > 0;
-    dprof_.clear();
-    init();
-    record_dprof_ = false;
+    init_recording(nprof > 0);
     const long pf = long(ns) * nk, pj = long(ns + 1) * nk, fpl = long(ns + 2) * nk, Fsz = 6 * fpl, ts = pf;
     const double h = cfg.zeta[1] - cfg.zeta[0], opb = 1 + beta_b, D = 0.5 * cfg.dz;
     const int P = cfg.iterations;

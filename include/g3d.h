@@ -456,8 +456,9 @@ int g3d_sim_read_checkpoint(g3d_sim* sim, const char* dir, long step);
  * values of the first grid laser in genv and in the nprof current-profile values of the bunch in gprof (either
  * count may be 0).  The reverse re-runs each segment from its checkpoint with a tape and applies the transposed
  * field updates and the dual-number reverses of the streaming and beam pushes, holding the forward's discrete
- * decisions fixed.  Either form (the reverse in FP64), Fourier y, uniform zeta grid, electrons only, one bunch, a
- * fixed window; a run in which beam particles are re-streamed is refused. */
+ * decisions fixed.  The simulation is left at the end of the steps.  Either form (the reverse in FP64), Fourier y,
+ * uniform zeta grid, electrons only, one bunch, a fixed window; a run in which beam particles are re-streamed is
+ * refused. */
 int g3d_sim_gradient(g3d_sim* sim, int steps, int every, double* loss, double* genv, int nenv, double* gprof,
                      int nprof);
 /* The tangent-linear run of the same march (no counterpart in the reference): on a new simulation, the loss of

@@ -119,6 +119,9 @@ def test_config4_gradient_against_differences():
     env0, prof0 = config4_params()
     s = g3d_amd.Sim(medium(env0, prof0))
     loss, genv, gprof = s.gradient(2, every=1)
+    p = s.array("parts")                             # the simulation is left at the end of the steps
+    assert s.info()["istep"] == 2
+    assert abs(float(p[(p["alive"] == 1) & (p["sp"] == 0)]["pz"].mean()) - loss) <= 1e-12 * abs(loss)
     s.close()
     s = g3d_amd.Sim(medium(env0, prof0))
     loss2, genv2, gprof2 = s.gradient(2, every=2)
