@@ -86,13 +86,14 @@ __device__ __forceinline__ void boris_d(double& ux, double& uy, double& uz, cons
     ux = bx + qh * E[0]; uy = by + qh * E[1]; uz = bz + qh * E[2];
 }
 
-// LDS windows of a tile, node-major so that one node is one address: the field window first (12 doubles per node:
-// E, B at slice fl then fl + 1, the float fields converted once when the window is loaded; six 16-byte reads per
-// stencil node) and the current window after it (5 doubles per node: j_x, j_y, j_z, rho-, rho+)
+// LDS windows of a tile, node-major so that one node is one address: the field window first (12 floats per node:
+// E, B at slice fl then fl + 1, the float fields as stored, widened to double in the gather; three 16-byte reads
+// per stencil node) and the current window after it (5 doubles per node: j_x, j_y, j_z, rho-, rho+).  25 x 17
+// nodes (16 x 8 tile, margin 3) take 37.4 KB.
 constexpr int kFwNode = 12, kDepNode = 5;
 struct WinD {
     double* lds;              // kDepNode doubles per node
-    const double2* fld;       // 6 double2 per node, or null
+    const float4* fld;        // 3 float4 per node, or null
     int x0, y0, WX, WY, fl;
 };
 
@@ -123,13 +124,13 @@ __device__ __forceinline__ void gather_d(const FieldView& c, const InvD& iv, con
             for (int j = 0; j < 3; ++j)
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
-                    const double2* q = w->fld + 6 * ((v0 + j) * w->WX + u0 + i);
-                    const double2 a = q[0], b = q[1], d = q[2], e = q[3], g = q[4], h = q[5];
+                    const float4* q = w->fld + 3 * ((v0 + j) * w->WX + u0 + i);
+                    const float4 a = q[0], b = q[1], d = q[2];
                     const double t = wt[3 * j + i];
-                    s0[0] += t * a.x; s0[1] += t * a.y; s0[2] += t * b.x; s0[3] += t * b.y;
-                    s0[4] += t * d.x; s0[5] += t * d.y;
-                    s1[0] += t * e.x; s1[1] += t * e.y; s1[2] += t * g.x; s1[3] += t * g.y;
-                    s1[4] += t * h.x; s1[5] += t * h.y;
+                    s0[0] += t * double(a.x); s0[1] += t * double(a.y); s0[2] += t * double(a.z);
+                    s0[3] += t * double(a.w); s0[4] += t * double(b.x); s0[5] += t * double(b.y);
+                    s1[0] += t * double(b.z); s1[1] += t * double(b.w); s1[2] += t * double(d.x);
+                    s1[3] += t * double(d.y); s1[4] += t * double(d.z); s1[5] += t * double(d.w);
                 }
             E[0] = f0 * s0[0] + f1 * s1[0]; E[1] = f0 * s0[1] + f1 * s1[1]; E[2] = f0 * s0[2] + f1 * s1[2];
             B[0] = f0 * s0[3] + f1 * s1[3]; B[1] = f0 * s0[4] + f1 * s1[4]; B[2] = f0 * s0[5] + f1 * s1[5];
@@ -336,7 +337,8 @@ k_stream_mixed(StreamArgs c, float* J, float* slab, Parts S, Parts D, long n, Ti
         const int x0 = tx * tg.TX - 1 - tg.M, y0 = ty * tg.TY - 1 - tg.M;
         const FieldView& F = c.fv;
         const int fl = c.n - F.a;
-        double* dep = reinterpret_cast<double*>(wind2 + 6 * pl);
+        float4* fw = reinterpret_cast<float4*>(wind2);
+        double* dep = reinterpret_cast<double*>(fw + 3 * pl);
         for (int r = threadIdx.x; r < wn; r += blockDim.x) dep[r] = 0;
         if (F.self)      // one node per lane: its 12 values from the 6 component planes of slices fl, fl + 1
             for (int r = threadIdx.x; r < pl; r += blockDim.x) {
@@ -344,17 +346,18 @@ k_stream_mixed(StreamArgs c, float* J, float* slab, Parts S, Parts D, long n, Ti
                 const int gx = wrap_index(x0 + u, F.nx), gy = wrap_index(y0 + v, F.ny);
                 const float* src = F.F + long(fl) * F.nk + long(gy) * F.nx + gx;
                 const long cs = long(F.ns + 2) * F.nk;
-                double q[kFwNode];
+                float q[kFwNode];
 #pragma unroll
                 for (int comp = 0; comp < 6; ++comp) {
                     q[comp] = src[comp * cs];
                     q[6 + comp] = src[comp * cs + F.nk];
                 }
 #pragma unroll
-                for (int k = 0; k < 6; ++k) wind2[6 * r + k] = make_double2(q[2 * k], q[2 * k + 1]);
+                for (int k = 0; k < 3; ++k)
+                    fw[3 * r + k] = make_float4(q[4 * k], q[4 * k + 1], q[4 * k + 2], q[4 * k + 3]);
             }
         __syncthreads();
-        WinD w{dep, F.self ? wind2 : nullptr, x0, y0, tg.WX, tg.WY, fl};
+        WinD w{dep, F.self ? fw : nullptr, x0, y0, tg.WX, tg.WY, fl};
         const InvD iv{1.0 / F.dx, 1.0 / F.dy};
         const HalfCellD hc = half_cell(c, rho);
         unsigned cnt = 0;
@@ -594,7 +597,7 @@ void stream_mixed(const StreamArgs& a, float* J, float* slab, Parts src, Parts d
     const int bs = tg.ntile ? tg.block : kT;
     if (bs < 64 || bs > kStreamBlockMax || bs % 64) throw std::runtime_error("stream half-cell (mixed): bad block size");
     const long tail_blocks = nstrm > tg.n_tiled ? long(blocks_for(nstrm - tg.n_tiled, bs)) : 0;
-    const size_t lds = tg.ntile ? size_t(tg.WX) * tg.WY * (kDepNode + kFwNode) * sizeof(double) : 0;
+    const size_t lds = tg.ntile ? size_t(tg.WX) * tg.WY * (kDepNode * sizeof(double) + kFwNode * sizeof(float)) : 0;
     const dim3 grid(unsigned(tg.ntile + tg.max_extra + tail_blocks)), blk(bs);
     if (a.npush && long(grid.x) > a.npush_cap) throw std::runtime_error("stream half-cell (mixed): push counter too small");
     auto* k = a.laser.on ? (tg.wpe == 4 ? k_stream_mixed<4, true> : k_stream_mixed<0, true>)
