@@ -35,12 +35,16 @@ G3D_HD Cur9 load_cur(const MarchView& c, long i, int n) {
 // Slice n of level s1 for mode i.  `old` holds R_x, R_y, E_z, B_z of slice n at level s0 (the explicit part),
 // `up` the six values of slice n+1 at level s1 (unused at the front slice n = N-1), J the currents.  Returns
 // slice n at s1.  The operand order is that of the reference so that the FP64 results round identically.
-G3D_HD Mode6 march_mode(const MarchView& c, long i, int n, const Mode6& old, const Mode6& up, const Cur9& J) {
-    int ixx, j;
-    mode_xy(c, i, ixx, j);
-    const double kxv = c.kx[ixx], kyv = c.ky[j];
+// march_mode_k: the same with the mode's wavenumbers and the slice's weights given (a chain over the slices of one
+// mode loads the wavenumbers once and each slice's weights ahead of it)
+struct SliceW { double wl, wu, h; };
+G3D_HD SliceW slice_w(const MarchView& c, int n) {
+    return SliceW{c.wlo[n], c.whi[n], n < c.N - 1 ? c.hcg[n + 2] : 0.0};
+}
+G3D_HD Mode6 march_mode_k(const MarchView& c, double kxv, double kyv, const SliceW& w, int n, const Mode6& old,
+                          const Mode6& up, const Cur9& J) {
     const cx a = ik(kxv), b = ik(kyv);
-    const double k2 = kxv * kxv + kyv * kyv, D = c.D, wl = c.wlo[n], wu = c.whi[n];
+    const double k2 = kxv * kxv + kyv * kyv, D = c.D, wl = w.wl, wu = w.wu;
     const cx Jx = wl * (J.j0x_d + J.jkx_d) + wu * (J.j0x_u + J.jkx_u);
     const cx Jy = wl * (J.j0y_d + J.jky_d) + wu * (J.j0y_u + J.jky_u);
     const cx Rsx = old.Rx + D * (a * old.Ez + b * old.Bz) - D * Jx;
@@ -50,7 +54,7 @@ G3D_HD Mode6 march_mode(const MarchView& c, long i, int n, const Mode6& old, con
         r.Rx = Rsx; r.Ry = Rsy; r.Ez = r.Bz = r.Lx = r.Ly = mk(0);
         return r;
     }
-    const double h = c.hcg[n + 2];
+    const double h = w.h;
     const double qh = 0.25 * h, hh = 0.5 * h;
     const cx upx = J.jkx_u, upy = J.jky_u, upz = J.jkz_u;
     const cx Lsx = up.Lx - qh * (a * up.Ez - b * up.Bz) - hh * upx;
@@ -68,6 +72,11 @@ G3D_HD Mode6 march_mode(const MarchView& c, long i, int n, const Mode6& old, con
     r.Lx = Lsx - qh * (a * Ezn - b * Bzn);
     r.Ly = Lsy - qh * (a * Bzn + b * Ezn);
     return r;
+}
+G3D_HD Mode6 march_mode(const MarchView& c, long i, int n, const Mode6& old, const Mode6& up, const Cur9& J) {
+    int ixx, j;
+    mode_xy(c, i, ixx, j);
+    return march_mode_k(c, c.kx[ixx], c.ky[j], slice_w(c, n), n, old, up, J);
 }
 
 G3D_HD Mode6 march_mode(const MarchView& c, long i, int n, const Mode6& old, const Mode6& up) {

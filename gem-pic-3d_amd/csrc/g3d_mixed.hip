@@ -31,7 +31,8 @@ constexpr int kT = 256;
 // cross, the march amplifies such perturbations by a few hundred (measured at 256 x 256 x 1024: a per-slice field
 // error of 1.7e-4 with an FP32 push against 6.5e-5 with an FP64 push, both with the FP32 field pass).  Without
 // packed FP32 the FP64 and FP32 vector rates of gfx950 are equal; what FP64 adds is the conversions of the float
-// fields (done once per window node) and its division and square root, which this form takes from the hardware
+// fields (in the gather, from a float window: cheaper than the twice as many LDS reads of a double window) and its
+// division and square root, which this form takes from the hardware
 // estimates (v_rcp_f64, v_rsq_f64) refined by two Newton steps each (the relative error of the estimate is squared
 // by each step) instead of the IEEE sequences of the plain FP64 form.
 __device__ __forceinline__ double rcp_d(double x) {
