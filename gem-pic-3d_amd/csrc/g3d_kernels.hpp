@@ -58,6 +58,10 @@ struct TileGeom {
     // add their window to the current planes with atomics
     int chunk, max_extra;
     int aflush;                       // FP32-mixed: every window added to the current planes with atomics (no slabs)
+    // FP32-mixed: tile workgroups XCD-contiguous (workgroup b runs on XCD b mod 8; with max_extra and ntile
+    // multiples of 8, XCD x takes tiles x ntile/8 ... (x+1) ntile/8 - 1, so the overlapping field windows of
+    // neighbouring tiles meet in one L2)
+    int xcd;
     const int* xtile;
     const long* xstart;
     const int* xcount;

@@ -320,7 +320,11 @@ k_stream_mixed(StreamArgs c, float* J, float* slab, Parts S, Parts D, long n, Ti
         // chunks), added to J with atomics; then tile b - max_extra's first chunk, written to its slab
         long t, lo, hi;
         const bool extra = b < tg.max_extra;
-        if (!extra) { t = b - tg.max_extra; lo = tg.off[t]; hi = tg.off[t + 1]; }
+        if (!extra) {
+            t = b - tg.max_extra;
+            if (tg.xcd) t = row_block(int(t), int(tg.ntile), 1);
+            lo = tg.off[t]; hi = tg.off[t + 1];
+        }
         else {
             if (b >= tg.xcount[0]) return;
             t = tg.xtile[b]; lo = tg.xstart[b]; hi = tg.off[t + 1];
