@@ -89,8 +89,8 @@ __device__ __forceinline__ void boris_d(double& ux, double& uy, double& uz, cons
 
 // LDS windows of a tile, node-major so that one node is one address: the field window first (12 floats per node:
 // E, B at slice fl then fl + 1, the float fields as stored, widened to double in the gather; three 16-byte reads
-// per stencil node) and the current window after it (5 doubles per node: j_x, j_y, j_z, rho-, rho+).  25 x 17
-// nodes (16 x 8 tile, margin 3) take 37.4 KB.
+// per stencil node) and the current window after it (5 doubles per node: j_x, j_y, j_z, rho-, rho+).  23 x 15
+// nodes (16 x 8 tile, margin 2) take 30.4 KB.
 constexpr int kFwNode = 12, kDepNode = 5;
 struct WinD {
     double* lds;              // kDepNode doubles per node
