@@ -1218,7 +1218,10 @@ void Sim::march_plasma(double s0, double s1) {
     // tiles only when this step loaded tiled plasma (otherwise the slabs would hold an earlier step's current)
     TileGeom tg = n_tiled > 0 ? tiles : TileGeom{};
     tg.idx = nullptr;
-    tg.chunk = 0; tg.max_extra = 0; tg.xtile = nullptr; tg.xstart = nullptr; tg.xcount = nullptr; tg.xcd = 0;
+    tg.chunk = 0; tg.max_extra = 0; tg.xtile = nullptr; tg.xstart = nullptr; tg.xcount = nullptr;
+    // FP32-mixed: the tile workgroups XCD-contiguous (G3D_STREAM_XCD=0: in tile order; the FP64 kernel ran 4 %
+    // slower so)
+    { const char* ex = getenv("G3D_STREAM_XCD"); tg.xcd = mx && (ex ? atoi(ex) : 1) && tg.ntile % 8 == 0 && tg.ntile < (1L << 30); }
     // FP32-mixed: the tile windows go to the current planes with float atomics instead of slabs (3 % faster at
     // 256 x 256: the row kernel then reads one plane set); G3D_STREAM_AFLUSH=0 keeps the slabs
     { const char* e = getenv("G3D_STREAM_AFLUSH"); tg.aflush = (mx && !(e && atoi(e) == 0)) || mxi; }
@@ -1229,13 +1232,9 @@ void Sim::march_plasma(double s0, double s1) {
     if (mx && tg.ntile && chunk_mult > 0) {
         tg.chunk = chunk_mult * tg.block;
         tg.max_extra = int((nstrm + tg.chunk - 1) / tg.chunk);
-        // the tile workgroups XCD-contiguous (G3D_STREAM_XCD=0: in tile order): the further-chunk workgroups
-        // padded to a multiple of 8 so that tile workgroup b - max_extra runs on XCD (b - max_extra) mod 8
-        const char* ex = getenv("G3D_STREAM_XCD");
-        if ((ex ? atoi(ex) : 1) && tg.ntile % 8 == 0 && tg.ntile < (1L << 30)) {
-            tg.xcd = 1;
-            tg.max_extra = (tg.max_extra + 7) & ~7;
-        }
+        // XCD-contiguous tiles: the further-chunk workgroups padded to a multiple of 8 so that tile workgroup
+        // b - max_extra runs on XCD (b - max_extra) mod 8
+        if (tg.xcd) tg.max_extra = (tg.max_extra + 7) & ~7;
         if (chunk_tile.size() < size_t(tg.max_extra)) { chunk_tile.resize(size_t(tg.max_extra)); chunk_start.resize(size_t(tg.max_extra)); }
         if (chunk_cnt.size() < 1) chunk_cnt.resize(1);
         tg.xtile = chunk_tile.data(); tg.xstart = chunk_start.data(); tg.xcount = chunk_cnt.data();
