@@ -1,6 +1,7 @@
 // Engine of the per-slice march: setup, init, step and the member functions of the reference's drop-in files
 // (src/sim_setup.cpp, sim_particles.cpp, sim_fields.cpp, sim_plasma.cpp, sim_ions.cpp) over the gfx950 kernels.
 #include "g3d_engine.hpp"
+#include "g3d_grad.hpp"
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -1283,6 +1284,12 @@ void Sim::march_plasma(double s0, double s1) {
     auto tape_copy = [&](void* dst, const void* src, size_t bytes) {
         hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream()), "hipMemcpy D2D");
     };
+    // the six E, B planes of one slice of F (planes fpl apart) in one strided copy
+    auto tape_slice = [&](float* dst, const float* src) {
+        hip_check(hipMemcpy2DAsync(dst, size_t(fpl) * sizeof(float), src, size_t(fpl) * sizeof(float),
+                                   size_t(nk) * sizeof(float), 6, hipMemcpyDeviceToDevice, stream()),
+                  "hipMemcpy2D D2D");
+    };
     if (tape_) {
         StepTape& t = *tape_;
         const size_t np7 = size_t(N - 1) * 7 * size_t(nlist);
@@ -1307,8 +1314,10 @@ void Sim::march_plasma(double s0, double s1) {
             const size_t b = size_t(nlist) * sizeof(double);
             double* dst = t.ps.data() + size_t(n) * 7 * size_t(nlist);
             const DevBuf<double>* src[7] = {&strmA.x, &strmA.y, &strmA.zeta, &strmA.z, &strmA.px, &strmA.py, &strmA.pz};
-            for (int c = 0; c < 7; ++c) tape_copy(dst + size_t(c) * nlist, src[c]->data(), b);
-            tape_copy(t.pa.data() + size_t(n) * nlist, strmA.alive.data(), size_t(nlist) * sizeof(int));
+            CopySpan sp[8];                  // the seven state arrays and alive in one launch
+            for (int c = 0; c < 7; ++c) sp[c] = CopySpan{dst + size_t(c) * nlist, src[c]->data(), long(b / 4)};
+            sp[7] = CopySpan{t.pa.data() + size_t(n) * nlist, strmA.alive.data(), nlist};
+            copy_spans(sp, 8);
         }
         if (tg.ntile && rebin > 0 && n < N - 2 && (N - 2 - n) % rebin == 0) {
             if (bin_idx.size() < strmA.cap) bin_idx.resize(strmA.cap);
@@ -1340,9 +1349,8 @@ void Sim::march_plasma(double s0, double s1) {
                            last ? rho_lines.data() + long(n) * (nx + ny) : nullptr, nullptr, i0, j0, 0, rpw);
                 toc(F_MODES);
                 if (tape_ && !last)
-                    for (int c = 0; c < 6; ++c)
-                        tape_copy(tape_->Fmid.data() + size_t(pass) * F.size() + size_t(c) * fpl + size_t(n - a) * nk,
-                                  F.data() + size_t(c) * fpl + size_t(n - a) * nk, size_t(nk) * sizeof(float));
+                    tape_slice(tape_->Fmid.data() + size_t(pass) * F.size() + size_t(n - a) * nk,
+                               F.data() + size_t(n - a) * nk);
                 continue;
             }
             const LaunchTimer lt = launch_timer(F_STREAM);
@@ -1378,9 +1386,8 @@ void Sim::march_plasma(double s0, double s1) {
             toc(F_MODES);
             real_fields(n);
             if (tape_ && !last)
-                for (int c = 0; c < 6; ++c)
-                    tape_copy(tape_->Fmid.data() + size_t(pass) * F.size() + size_t(c) * fpl + size_t(n - a) * nk,
-                              F.data() + size_t(c) * fpl + size_t(n - a) * nk, size_t(nk) * sizeof(float));
+                tape_slice(tape_->Fmid.data() + size_t(pass) * F.size() + size_t(n - a) * nk,
+                           F.data() + size_t(n - a) * nk);
         }
         // the passes wrote the new state into strmB: it becomes the list
         std::swap(strmA, strmB);

@@ -378,6 +378,17 @@ __global__ void k_laser_dot(SpecGeom g, const cx* T, const cx* lRx, const cx* lR
         if ((threadIdx.x & 63) == 0 && idx < tot) atomicAdd(out + l, v);
     } else if (idx < tot) atomicAdd(out + l, v);
 }
+struct SpanSet {
+    CopySpan s[kMaxSpans];
+};
+// copy y of the set (blockIdx.y), its words over the x blocks
+__global__ void __launch_bounds__(256) k_copy_spans(SpanSet c) {
+    const CopySpan sp = c.s[blockIdx.y];
+    const unsigned* src = static_cast<const unsigned*>(sp.src);
+    unsigned* dst = static_cast<unsigned*>(sp.dst);
+    for (long i = long(blockIdx.x) * blockDim.x + threadIdx.x; i < sp.words; i += long(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
+}
 }  // namespace
 
 #define G3D_GLAUNCH(kern, n, ...)                                                                            \
@@ -407,6 +418,18 @@ void add_planes(double* y, long ds, const double* x, long ss, long n, int count)
     G3D_GLAUNCH(k_addp, long(count) * n, y, ds, x, ss, n, count);
 }
 void add_values(double* y, const double* x, long n) { G3D_GLAUNCH(k_addp, n, y, n, x, n, n, 1); }
+
+void copy_spans(const CopySpan* spans, int count) {
+    if (count <= 0) return;
+    if (count > kMaxSpans) throw std::runtime_error("copy_spans: at most 8 copies per launch");
+    SpanSet c{};
+    long most = 0;
+    for (int k = 0; k < count; ++k) { c.s[k] = spans[k]; most = std::max(most, spans[k].words); }
+    if (most <= 0) return;
+    const unsigned bx = std::min<unsigned>(blocks_for(most, 256), 2048u);
+    hipLaunchKernelGGL(k_copy_spans, dim3(bx, unsigned(count)), dim3(256), 0, stream(), c);
+    launch_check("copy spans");
+}
 
 void sponge_real_fields_adj(const SpongeAdj& c, const FftPlan& p, double* lF, long fpl, cx* T, cx* X, cx* Y, cx* P) {
     const SpecGeom& g = c.g;

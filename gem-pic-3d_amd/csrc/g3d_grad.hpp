@@ -33,6 +33,14 @@ void planes_to_double(const float* src, long sstride, double* dst, long dstride,
 void add_planes(double* y, long dstride, const double* x, long sstride, long n, int count);
 // y += x over n values
 void add_values(double* y, const double* x, long n);
+// up to kMaxSpans device-to-device copies of whole 4-byte words in one launch (the tape's per-slice records)
+struct CopySpan {
+    void* dst;
+    const void* src;
+    long words;
+};
+constexpr int kMaxSpans = 8;
+void copy_spans(const CopySpan* spans, int count);
 
 // The sponge and the real fields (Sim::sponge_and_real_fields) reversed: lF (ns slices, consumed) -> the adjoints
 // of R, L, E_z, B_z (the sponge rewrites R in place) and of jkz (through the bound field); Fourier y.
