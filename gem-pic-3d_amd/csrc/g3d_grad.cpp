@@ -79,7 +79,7 @@ struct Adjoints {
     DevBuf<double> F, Fm;                  // real fields after the step / after the march, [6][ns + 2][nk]
     DevBuf<double> W, Wn;                  // beam: x y zeta px py pz, [6][np]
     DevBuf<cx> T3, X, Y, P, S, lT, lU;
-    DevBuf<double> jw, Fd, lFp, gF0, gF1, F0d, F1d, bJ;
+    DevBuf<double> jw, Fd, gF0, gF1, F0d, F1d, bJ;
     DevBuf<double> lP, lPacc, gP, zeros;   // streaming particles: x y zeta z px py pz, [7][nstrm]
 };
 
@@ -248,7 +248,7 @@ void Sim::gradient(int steps, int every, double* loss, double* genv, int nenv, d
     A.T3.resize(3 * size_t(pf)); A.X.resize(size_t(pj)); A.Y.resize(size_t(pj)); A.P.resize(size_t(pj));
     A.S.resize(size_t(pj)); A.lT.resize(3 * size_t(nk)); A.lU.resize(3 * size_t(nk));
     A.jw.resize(3 * size_t(ns + 3) * size_t(nk));
-    for (DevBuf<double>* p : {&A.lFp, &A.gF0, &A.gF1, &A.F0d, &A.F1d}) p->resize(6 * size_t(nk));
+    for (DevBuf<double>* p : {&A.gF0, &A.gF1, &A.F0d, &A.F1d}) p->resize(6 * size_t(nk));
     A.bJ.resize(5 * size_t(nk));
     {                                            // the loss: the mean p_z of the live bunch at the end
 
@@ -336,8 +336,9 @@ void Sim::gradient(int steps, int every, double* loss, double* genv, int nenv, d
             stream_push_slot(*this, t, n, c);
             A.lPacc.zero();
             for (int p = P - 1; p >= 0; --p) {
+                // the later pass's field adjoint (gF0 of pass p + 1, kept until this pass clears it below)
                 if (p == P - 1) fields_to_F_adj(A.Fm.data() + long(n - a) * nk, fpl, nk, 1, A.lT.data(), nk);
-                else fields_to_F_adj(A.lFp.data(), nk, nk, 1, A.lT.data(), nk);
+                else fields_to_F_adj(A.gF0.data(), nk, nk, 1, A.lT.data(), nk);
                 fft_run(fft, A.lT.data(), 3, false);
                 PassAdj pa = pb;
                 pa.n = n; pa.first = p == 0; pa.last = p == P - 1; pa.top = 0; pa.bk = P > 1;
@@ -349,17 +350,16 @@ void Sim::gradient(int steps, int every, double* loss, double* genv, int nenv, d
                 real_part_planes(A.lU.data(), A.bJ.data(), 1.0, nk);
                 const float* f0 = p == 0 ? t.F0.data() + long(n - a) * nk
                                          : t.Fmid.data() + size_t(p - 1) * size_t(Fsz) + long(n - a) * nk;
-                planes_to_double(f0, fpl, A.F0d.data(), nk, nk, 6);
-                planes_to_double(t.Fmarch.data() + long(n + 1 - a) * nk, fpl, A.F1d.data(), nk, nk, 6);
-                A.gF0.zero(); A.gF1.zero();
+                pass_prep(f0, t.Fmarch.data() + long(n + 1 - a) * nk, fpl, A.F0d.data(), A.F1d.data(), A.gF0.data(),
+                          A.gF1.data(), nk);
                 const double* bs = p == P - 1 ? A.lP.data() : A.zeros.data();
                 ad.bx = bs; ad.by = bs + ns_; ad.bzeta = bs + 2 * ns_; ad.bz = bs + 3 * ns_; ad.bpx = bs + 4 * ns_;
                 ad.bpy = bs + 5 * ns_; ad.bpz = bs + 6 * ns_;
                 if (g3d_stream_push_vjp(&c, &ad)) throw std::runtime_error(last_error());
-                add_values(A.lPacc.data(), A.gP.data(), 7 * ns_);
-                if (p == 0) add_planes(A.F.data() + long(n - a) * nk, fpl, A.gF0.data(), nk, nk, 6);
-                else d2d(A.lFp.data(), A.gF0.data(), 6 * size_t(nk) * sizeof(double));
-                add_planes(A.Fm.data() + long(n + 1 - a) * nk, fpl, A.gF1.data(), nk, nk, 6);
+                // the pass's particle adjoints summed; its field adjoints into F (the first pass) or, from gF0, into
+                // the pass before; into the march's F of the next slice
+                pass_sums(A.lPacc.data(), A.gP.data(), 7 * ns_, p == 0 ? A.F.data() + long(n - a) * nk : nullptr, fpl,
+                          A.gF0.data(), A.Fm.data() + long(n + 1 - a) * nk, fpl, A.gF1.data(), nk);
             }
             A.lP.swap(A.lPacc);
         }

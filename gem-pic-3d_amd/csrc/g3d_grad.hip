@@ -378,6 +378,29 @@ __global__ void k_laser_dot(SpecGeom g, const cx* T, const cx* lRx, const cx* lR
         if ((threadIdx.x & 63) == 0 && idx < tot) atomicAdd(out + l, v);
     } else if (idx < tot) atomicAdd(out + l, v);
 }
+// a reverse pass's inputs: the two float slices (6 planes fpl apart) into double planes (nk apart), the field
+// adjoints the streaming reverse scatters into cleared
+__global__ void k_pass_prep(const float* f0, const float* f1, long fpl, double* F0d, double* F1d, double* gF0,
+                            double* gF1, long nk) {
+    const long e = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= 6 * nk) return;
+    const long c = e / nk, i = e - c * nk;
+    F0d[e] = double(f0[c * fpl + i]);
+    F1d[e] = double(f1[c * fpl + i]);
+    gF0[e] = 0;
+    gF1[e] = 0;
+}
+// a reverse pass's sums: acc += gP (m values); dst0 += gF0 (planes s0 apart; skipped when null); dst1 += gF1
+__global__ void k_pass_sums(double* acc, const double* gP, long m, double* dst0, long s0, const double* gF0,
+                            double* dst1, long s1, const double* gF1, long nk) {
+    const long e = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e < m) acc[e] += gP[e];
+    if (e >= 6 * nk) return;
+    const long c = e / nk, i = e - c * nk;
+    if (dst0) dst0[c * s0 + i] += gF0[e];
+    dst1[c * s1 + i] += gF1[e];
+}
+
 struct SpanSet {
     CopySpan s[kMaxSpans];
 };
@@ -418,6 +441,15 @@ void add_planes(double* y, long ds, const double* x, long ss, long n, int count)
     G3D_GLAUNCH(k_addp, long(count) * n, y, ds, x, ss, n, count);
 }
 void add_values(double* y, const double* x, long n) { G3D_GLAUNCH(k_addp, n, y, n, x, n, n, 1); }
+
+void pass_prep(const float* f0, const float* f1, long fpl, double* F0d, double* F1d, double* gF0, double* gF1,
+               long nk) {
+    G3D_GLAUNCH(k_pass_prep, 6 * nk, f0, f1, fpl, F0d, F1d, gF0, gF1, nk);
+}
+void pass_sums(double* acc, const double* gP, long m, double* dst0, long s0, const double* gF0, double* dst1, long s1,
+               const double* gF1, long nk) {
+    G3D_GLAUNCH(k_pass_sums, std::max(m, 6 * nk), acc, gP, m, dst0, s0, gF0, dst1, s1, gF1, nk);
+}
 
 void copy_spans(const CopySpan* spans, int count) {
     if (count <= 0) return;

@@ -33,6 +33,13 @@ void planes_to_double(const float* src, long sstride, double* dst, long dstride,
 void add_planes(double* y, long dstride, const double* x, long sstride, long n, int count);
 // y += x over n values
 void add_values(double* y, const double* x, long n);
+// a reverse pass's inputs in one launch: F0d, F1d (6 planes nk apart) from the float slices f0, f1 (planes fpl
+// apart), gF0 and gF1 (6 nk) cleared
+void pass_prep(const float* f0, const float* f1, long fpl, double* F0d, double* F1d, double* gF0, double* gF1, long nk);
+// a reverse pass's sums in one launch: acc += gP over m values, dst0 += gF0 (6 planes s0 apart; null: none),
+// dst1 += gF1 (planes s1 apart)
+void pass_sums(double* acc, const double* gP, long m, double* dst0, long s0, const double* gF0, double* dst1, long s1,
+               const double* gF1, long nk);
 // up to kMaxSpans device-to-device copies of whole 4-byte words in one launch (the tape's per-slice records)
 struct CopySpan {
     void* dst;
