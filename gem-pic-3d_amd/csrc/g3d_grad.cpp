@@ -636,15 +636,11 @@ void Sim::tangent(int steps, const double* denv, int nenv, const double* dprof, 
                 const bool last = p == P - 1;
                 const float* f0 = p == 0 ? t.F0.data() + long(n - a) * nk
                                          : t.Fmid.data() + size_t(p - 1) * size_t(Fsz) + long(n - a) * nk;
-                planes_to_double(f0, fpl, F0d.data(), nk, nk, 6);
-                planes_to_double(t.Fmarch.data() + long(n + 1 - a) * nk, fpl, F1d.data(), nk, nk, 6);
-                if (p == 0) {
-                    tFmid.zero();
-                    add_planes(tFmid.data(), nk, tF.data() + long(n - a) * nk, fpl, nk, 6);
-                }
-                dF1.zero();
-                add_planes(dF1.data(), nk, tFm.data() + long(n + 1 - a) * nk, fpl, nk, 6);
-                tJ.zero();
+                // the fields of the pass and their tangents (the slice's own on the first pass), the tangent
+                // currents cleared: one launch
+                tangent_prep(f0, t.Fmarch.data() + long(n + 1 - a) * nk, fpl, F0d.data(), F1d.data(),
+                             p == 0 ? tF.data() + long(n - a) * nk : nullptr, tFmid.data(),
+                             tFm.data() + long(n + 1 - a) * nk, dF1.data(), tJ.data(), 5 * long(nk), nk);
                 stream_push_tlm(c, tFmid.data(), dF1.data(), tP.data(), tJ.data(), last);
                 pass_rows_fwd(tJ.data(), TileGeom{}, fft, tX.data(), nullptr, nullptr, 0, 0, 0);
                 m.n = n; m.first = p == 0; m.last = last; m.top = 0;

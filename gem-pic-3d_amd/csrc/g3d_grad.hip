@@ -401,6 +401,21 @@ __global__ void k_pass_sums(double* acc, const double* gP, long m, double* dst0,
     dst1[c * s1 + i] += gF1[e];
 }
 
+// a tangent-linear pass's inputs: the two float slices into double planes as k_pass_prep; the tangent fields of
+// the slice (t0, when given) and of slice n + 1 (t1), planes fpl apart, into t0d, t1d (0 + x, as the clear and sum
+// they replace); the tangent current planes J (nJ values) cleared
+__global__ void k_tangent_prep(const float* f0, const float* f1, long fpl, double* F0d, double* F1d, const double* t0,
+                               double* t0d, const double* t1, double* t1d, double* J, long nJ, long nk) {
+    const long e = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e < nJ) J[e] = 0;
+    if (e >= 6 * nk) return;
+    const long c = e / nk, i = e - c * nk;
+    F0d[e] = double(f0[c * fpl + i]);
+    F1d[e] = double(f1[c * fpl + i]);
+    if (t0) t0d[e] = 0.0 + t0[c * fpl + i];
+    t1d[e] = 0.0 + t1[c * fpl + i];
+}
+
 struct SpanSet {
     CopySpan s[kMaxSpans];
 };
@@ -449,6 +464,11 @@ void pass_prep(const float* f0, const float* f1, long fpl, double* F0d, double* 
 void pass_sums(double* acc, const double* gP, long m, double* dst0, long s0, const double* gF0, double* dst1, long s1,
                const double* gF1, long nk) {
     G3D_GLAUNCH(k_pass_sums, std::max(m, 6 * nk), acc, gP, m, dst0, s0, gF0, dst1, s1, gF1, nk);
+}
+
+void tangent_prep(const float* f0, const float* f1, long fpl, double* F0d, double* F1d, const double* t0, double* t0d,
+                  const double* t1, double* t1d, double* J, long nJ, long nk) {
+    G3D_GLAUNCH(k_tangent_prep, std::max(nJ, 6 * nk), f0, f1, fpl, F0d, F1d, t0, t0d, t1, t1d, J, nJ, nk);
 }
 
 void copy_spans(const CopySpan* spans, int count) {

@@ -40,6 +40,10 @@ void pass_prep(const float* f0, const float* f1, long fpl, double* F0d, double* 
 // dst1 += gF1 (planes s1 apart)
 void pass_sums(double* acc, const double* gP, long m, double* dst0, long s0, const double* gF0, double* dst1, long s1,
                const double* gF1, long nk);
+// a tangent-linear pass's inputs in one launch: F0d, F1d as pass_prep; t0d = t0 (when t0 is given) and t1d = t1
+// (6 planes fpl apart into nk apart); J (nJ values) cleared
+void tangent_prep(const float* f0, const float* f1, long fpl, double* F0d, double* F1d, const double* t0, double* t0d,
+                  const double* t1, double* t1d, double* J, long nJ, long nk);
 // up to kMaxSpans device-to-device copies of whole 4-byte words in one launch (the tape's per-slice records)
 struct CopySpan {
     void* dst;
