@@ -717,10 +717,16 @@ __global__ void k_centroid_partial(Parts P, long np, double X0, double Y0, doubl
     }
     if (threadIdx.x == 0) for (int k = 0; k < 3; ++k) part[3 * blockIdx.x + k] = s[k][0];
 }
+constexpr int kCentroidBlocks = 256;
+// the partial sums staged in LDS by the whole workgroup, then added in block order by one lane (the loads off its
+// chain of additions; the same sums in the same order)
 __global__ void k_centroid_final(const double* part, int nb, double* out) {
+    __shared__ double s[3 * kCentroidBlocks];
+    for (int k = threadIdx.x; k < 3 * nb; k += blockDim.x) s[k] = part[k];
+    __syncthreads();
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     double a = 0, b = 0, w = 0;
-    for (int k = 0; k < nb; ++k) { a += part[3 * k]; b += part[3 * k + 1]; w += part[3 * k + 2]; }
+    for (int k = 0; k < nb; ++k) { a += s[3 * k]; b += s[3 * k + 1]; w += s[3 * k + 2]; }
     out[0] = a; out[1] = b; out[2] = w;
 }
 
@@ -869,7 +875,7 @@ void append_born(Parts born, const int* bcnt, Parts p0, Parts p1, long base, lon
 }
 void bunch_centroid(Parts p, long np, double X0, double Y0, double* out3) {
     static double* part = nullptr;
-    const int nb = 256;
+    const int nb = kCentroidBlocks;
     if (!part) hip_check(hipMalloc(reinterpret_cast<void**>(&part), 3 * nb * sizeof(double)), "hipMalloc");
     hipLaunchKernelGGL(k_centroid_partial, dim3(nb), dim3(kT), 0, stream(), p, np, X0, Y0, part);
     hipLaunchKernelGGL(k_centroid_final, dim3(1), dim3(64), 0, stream(), part, nb, out3);
